@@ -1,0 +1,158 @@
+/*
+ * shyhip.h -- C ABI of libshyhip.so, the MI355X (gfx950) implementation of
+ * SHYFEM's semi-implicit hydrodynamic time step and 3D scalar transport.
+ *
+ * The library owns device-resident copies of the model state.  Host arrays
+ * crossing this boundary use the reference's own layouts and types, so a
+ * Fortran caller passes its module arrays directly (c_loc):
+ *   real    = float (32-bit); double precision = double
+ *   element 3D  a(nlvdi,nel), node 3D a(nlvdi,nkn), interfaces a(0:nlvdi,nkn)
+ *   vertex      zenv(3,nel),  connectivity nen3v(3,nel) with 1-based node ids
+ * All entry points return 0 on success and a non-zero code otherwise; the
+ * message is available from shyhip_last_error().  Handles are independent:
+ * calls on different handles may run concurrently from different host
+ * threads (each handle has its own HIP stream and scratch); calls on one
+ * handle must not overlap.
+ *
+ * Reference interfaces replaced (SHYFEM 8_1_1, paths relative to its root):
+ *   shyhip_hydro               hydro                src/shyfem/hydrodynamic.f90:284
+ *   shyhip_hydro_transports    hydro_transports     src/shyfem/hydrodynamic.f90:758
+ *   shyhip_hydro_zeta          hydro_zeta(vqv)      src/shyfem/hydrodynamic.f90:488
+ *   shyhip_hydro_transports_final                   src/shyfem/hydrodynamic.f90:1468
+ *   shyhip_hydro_vertical      hydro_vertical(dzeta) src/shyfem/hydrodynamic.f90:1582
+ *   shyhip_set_dry             set_dry              src/shyfem/wetdry.f90:94
+ *   shyhip_copy_uvz            copy_uvz + copy_depth src/shyfem/transforms.f90:523,
+ *                                                    src/shyfem/elems_dealing.f90:843
+ *   shyhip_scal3sh             scal3sh -> conz3d_omp src/shyfem/concentration.f90:565,
+ *                                                    src/shyfem/concentration_omp.f90:65
+ *   shyhip_system_*            system_initialize/init/assemble/add_rhs/solve/get/finalize
+ *                                                    src/shyfem/simsys_spkaux.f90:85-721
+ *   shyhip_volume              vol_mass -> masscont src/shyfem/check.f90:647,
+ *                                                    src/shyfem/elems_dealing.f90:1191
+ */
+#ifndef SHYHIP_H
+#define SHYHIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct shyhip_model shyhip_model;
+
+/* precision of the device path */
+enum { SHYHIP_FP64 = 0,   /* all state and the zeta solve in FP64 */
+       SHYHIP_FP32 = 1 }; /* FP32 state and transport, FP32 Krylov inside FP64 refinement */
+
+/* element-column solver variant chosen by shyhip_set_params */
+typedef struct {
+    double dt;                                  /* idt                          */
+    float azpar, ampar, afpar, atpar, adpar, aapar;
+    float grav, rowass, vismol;
+    int ireib; float uvmin;                     /* bottom friction (chezy.f90)  */
+    float hzmin, hzoff, hzon, volmin;           /* wet/dry thresholds           */
+    int iclin;
+    /* zeta solver */
+    double rtol, atol;                          /* BiCGStab tolerances (1e-8)   */
+    int itermax;                                /* max matvecs (1000)           */
+    double refine_rtol;                         /* FP32 path: FP64 refinement residual target */
+    int block_rows;                             /* block-ILU(0) block size (<= 1024), 0 = auto */
+    /* scalars */
+    float rkpar, difmol;
+    int itvd, itvdv;
+} shyhip_params;
+
+/* field ids for shyhip_push / shyhip_pull */
+enum {
+    SHYHIP_ZNV = 1, SHYHIP_ZOV, SHYHIP_ZENV, SHYHIP_ZEOV,
+    SHYHIP_UTLNV, SHYHIP_VTLNV, SHYHIP_UTLOV, SHYHIP_VTLOV,
+    SHYHIP_UNV, SHYHIP_VNV, SHYHIP_UOV, SHYHIP_VOV,
+    SHYHIP_ULNV, SHYHIP_VLNV,
+    SHYHIP_WLNV, SHYHIP_WLOV,
+    SHYHIP_HDKNV, SHYHIP_HDKOV, SHYHIP_HDENV, SHYHIP_HDEOV,
+    SHYHIP_IWEGV, SHYHIP_IWETV, SHYHIP_INODV,
+    SHYHIP_RFRICV, SHYHIP_DDXV, SHYHIP_DDYV,
+    SHYHIP_VISV, SHYHIP_DIFV, SHYHIP_DIFHV,
+    SHYHIP_TAUXNV, SHYHIP_TAUYNV, SHYHIP_RZV, SHYHIP_RQV, SHYHIP_MFLUXV,
+    SHYHIP_AREAKV, SHYHIP_CSR_VAL, SHYHIP_RHS, SHYHIP_TRACER0 = 100
+};
+/* dtype of host buffers */
+enum { SHYHIP_F32 = 0, SHYHIP_F64 = 1, SHYHIP_I32 = 2 };
+
+const char *shyhip_last_error(const shyhip_model *h);
+int  shyhip_device_count(int *n);
+
+/* life cycle ------------------------------------------------------------ */
+int  shyhip_create(shyhip_model **h, int precision, int nkn, int nel, int nlvdi, int nlv, int ntracer);
+int  shyhip_destroy(shyhip_model *h);
+int  shyhip_set_params(shyhip_model *h, const shyhip_params *p);
+
+/* basin and layers (init time): nen3v(3,nel) 1-based, xgv/ygv(nkn),
+ * hm3v(3,nel), hev(nel), hlv/hldv(nlvdi), ilhv(nel), ilhkv(nkn).
+ * Builds geometry (set_ev), the zeta-matrix pattern (coo_init_new), the
+ * element colouring (domain_clusterization_ggu), nodal areas (set_area) and
+ * the block-ILU(0) level schedule, and uploads everything to the device. */
+int  shyhip_set_basin(shyhip_model *h, const int *nen3v, const float *xgv, const float *ygv,
+                      const float *hm3v, const float *hev, const float *hlv, const float *hldv,
+                      const int *ilhv, const int *ilhkv);
+/* ibnd(nkn): open-boundary number of each node (0 none); ibtyp(nbnd+1):
+ * type of boundary b at index b; iextb(nkn): node on the external boundary */
+int  shyhip_set_boundaries(shyhip_model *h, int nbnd, const int *ibnd, const int *ibtyp, const int *iextb);
+/* fcorv(nel), czv(nel); any pointer may be NULL (zero) */
+int  shyhip_set_coefficients(shyhip_model *h, const float *fcorv, const float *czv);
+/* TVD (itvd = 2) upwind tables, built on the host from the basin */
+int  shyhip_tvd_init(shyhip_model *h);
+
+/* the reference's start-up sequence for a constant initial level:
+ * setweg(-1), setnod, init_z, setweg(0), setznv, make_new_depth,
+ * copy_depth, zero transports, hydro_vertical, copy_uvz, copy_depth */
+int  shyhip_init_state(shyhip_model *h, float zconst);
+
+/* state transfer in the reference's layout; n = number of values */
+int  shyhip_push(shyhip_model *h, int field, const void *host, int dtype, size_t n);
+int  shyhip_pull(shyhip_model *h, int field, void *host, int dtype, size_t n);
+
+/* forcing ---------------------------------------------------------------- */
+/* level of open boundary ibc (sp111 for ibtyp = 1 boundaries) */
+int  shyhip_set_boundary_level(shyhip_model *h, int ibc, float level);
+/* uniform wind stress per unit density (meteo_forcing.f90:973) */
+int  shyhip_set_wind_uniform(shyhip_model *h, float taux, float tauy);
+
+/* time step -------------------------------------------------------------- */
+int  shyhip_set_dry(shyhip_model *h);
+int  shyhip_copy_uvz(shyhip_model *h);                 /* copy_uvz + copy_depth */
+int  shyhip_hydro(shyhip_model *h, double dtime);      /* hydro at time dtime   */
+int  shyhip_hydro_transports(shyhip_model *h);
+int  shyhip_hydro_zeta(shyhip_model *h);                /* assemble (uses rqv)   */
+int  shyhip_system_solve_device(shyhip_model *h, int *iters);
+int  shyhip_hydro_transports_final(shyhip_model *h);
+int  shyhip_hydro_vertical(shyhip_model *h);
+/* one full model step: set_dry, set_timestep, copy_uvz/copy_depth, hydro,
+ * and scal3sh for every tracer with istot substeps */
+int  shyhip_step(shyhip_model *h, double dtime, int istot);
+/* scal3sh for tracer itr (0-based); clip_small = 1 for non-temperature */
+int  shyhip_scal3sh(shyhip_model *h, int itr, int istot, int clip_small);
+int  shyhip_sync(shyhip_model *h);
+
+/* diagnostics ------------------------------------------------------------ */
+int  shyhip_volume(shyhip_model *h, int mode, double *vol);     /* FP64 total volume */
+int  shyhip_tracer_mass(shyhip_model *h, int itr, double *mass);
+int  shyhip_stats(shyhip_model *h, int *nloop, int *solver_iters, int *refinements);
+/* integer structures built by shyhip_set_basin, for checking:
+ * ntg(nkn+1), cols(nnz) 0-based; color(nel) 1-based colours */
+int  shyhip_get_pattern(shyhip_model *h, int *nnz, int *ntg, int *cols, int *ncolor, int *color);
+
+/* the zeta-system plugin (simsys_spkaux.f90): the stock hydro_zeta keeps
+ * calling system_assemble(ie) with kn/hia/hik; the plugin accumulates on the
+ * host in element order and solves on the device. */
+int  shyhip_system_init(shyhip_model *h);
+int  shyhip_system_assemble(shyhip_model *h, int ie, const int *kn, const float *hia, const float *hik);
+int  shyhip_system_add_rhs(shyhip_model *h, float dt, int n, const float *array);
+int  shyhip_system_solve(shyhip_model *h, int n, const float *z);
+int  shyhip_system_get(shyhip_model *h, int n, float *z);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

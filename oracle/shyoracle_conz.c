@@ -1,0 +1,385 @@
+/*
+ * shyoracle_conz.c -- 3D scalar advection-diffusion of the CPU oracle
+ * (TEST INFRASTRUCTURE; see shyoracle.h).
+ *
+ * Restates src/shyfem/concentration_omp.f90 (conz3d_omp, conz3d_element,
+ * conz3d_nodes), tvd_admin.f90 (tvd_fluxes, tvd_get_upwind_c,
+ * vertical_flux_ie), evgeom.f90 (femintp_xi, xy2xi, adjust_xi),
+ * bnd_routines.f90 make_scal_flux and the substep loop of
+ * concentration.f90 scal3sh.  Element contributions are accumulated colour
+ * by colour, in the colouring's element order, exactly as conz3d_omp does.
+ * Nudging (robs = 0), sinking (wsink = 0) and loads (rload = 0) are zero for
+ * the supported configurations and omitted.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "shyoracle.h"
+
+#define EV(i, ie) m->ev[((i) - 1) + 19 * (ie)]
+#define A3E(a, l, ie) (a)[((l) - 1) + nlvdi * (ie)]
+#define A3K(a, l, k) (a)[((l) - 1) + nlvdi * (k)]
+#define AW(a, l, k) (a)[(l) + (nlvdi + 1) * (k)]
+
+static double dmax(double a, double b) { return a > b ? a : b; }
+static double dmin(double a, double b) { return a < b ? a : b; }
+
+/* adjust_xi: evgeom.f90:577-660 */
+static int adjust_xi(double *xi)
+{
+    double xisum = 0., xiso = 0.;
+    int is = 0, it = 0;
+    for (int ii = 1; ii <= 3; ii++) {
+        xiso += xi[ii - 1];
+        if (xi[ii - 1] < 0.) xi[ii - 1] = 0.;
+        if (xi[ii - 1] > 1.) xi[ii - 1] = 1.;
+        if (xi[ii - 1] == 0.) { is += ii; it++; }
+        xisum += xi[ii - 1];
+    }
+    if (it == 3 || fabs(xiso - 1.) > (double)1.e-8f) return 1;
+    double xiadj = (xisum - 1.) / (3 - it);
+    if (it == 2) {
+        is = 6 - is;
+        xi[is - 1] = 1.;
+    } else {
+        for (int ii = 1; ii <= 3; ii++) {
+            if (ii != is) {
+                xi[ii - 1] -= xiadj;
+                if (xi[ii - 1] < 0.) xi[ii - 1] = 0.;
+                if (xi[ii - 1] > 1.) xi[ii - 1] = 1.;
+            }
+        }
+    }
+    xiso = xi[0] + xi[1] + xi[2];
+    if (fabs(xiso - 1.) > (double)1.e-8f) return 3;
+    return 0;
+}
+
+/* tvd_get_upwind_c (tvd_admin.f90:608-669) with femintp_xi
+ * (evgeom.f90:1413-1445): value of cv interpolated at the upwind point */
+static real tvd_upwind(const oc_mesh *m, int ie, int l, int ic, int id, real cu, const real *cv)
+{
+    const int nlvdi = m->nlvdi;
+    int idx = (id - 1) + 3 * (ic - 1) + 9 * ie;
+    int ienew = m->tvd_ie[idx];
+    if (ienew < 0) return cu;
+    if (m->ilhv[ienew] < l) return cu;
+    double x = m->tvd_x[idx], y = m->tvd_y[idx];
+    double xi[3];
+    for (int ii = 1; ii <= 3; ii++)
+        xi[ii - 1] = EV(ii, ienew) + EV(3 + ii, ienew) * x + EV(6 + ii, ienew) * y;
+    adjust_xi(xi);
+    double zh = 0.;
+    for (int ii = 1; ii <= 3; ii++) zh = zh + (double)A3K(cv, l, m->nen3v[3 * ienew + ii - 1]) * xi[ii - 1];
+    return (real)zh;
+}
+
+static int smartdelta(int a, int b)
+{
+    return (int)((float)((a + b) - abs(a - b)) / (float)((a + b) + abs(a - b)));
+}
+
+/* tvd_fluxes: tvd_admin.f90:673-823 (itvd = 2 upwind-point variant) */
+static void tvd_fluxes(const oc_mesh *m, const oc_state *s, int ie, int l, int itot, int isum,
+                       double dt, const double *cl /* cl[ii] at layer l */, const real *cv,
+                       const double *f, double *fl)
+{
+    const int nlvdi = m->nlvdi;
+    const real eps = (real)1.e-8;
+    fl[0] = fl[1] = fl[2] = 0.;
+    if (itot < 1 || itot > 2) return;
+    int itot2 = itot - 1, itot1 = 2 - itot;
+    real u = A3E(s->ulnv, l, ie), v = A3E(s->vlnv, l, ie);
+    real aj = (real)(24 * EV(10, ie));
+    int ip = isum;
+    ip = itot2 * (6 - ip) + itot1 * ip;
+    for (int ii = 1; ii <= 3; ii++) {
+        if (ii == ip) continue;
+        int ic = itot2 * ii + itot1 * ip;
+        int id = itot2 * ip + itot1 * ii;
+        real fact = (real)(-itot2 + itot1);
+        real conc = (real)cl[ic - 1];
+        real cond = (real)cl[id - 1];
+        int iop = 6 - (id + ic);
+        int tet1 = 1 + iop % 3;
+        real dx = (real)(aj * EV(6 + iop, ie));
+        dx = (real)(-2 * smartdelta(tet1, id)) * dx + dx;
+        real dy = (real)(aj * EV(3 + iop, ie));
+        dy = (real)(-2 * smartdelta(tet1, ic)) * dy + dy;
+        real dis = (real)EV(16 + iop, ie);
+        real vel = (real)fabs((double)(u * dx + v * dy)) / dis;
+        real alfa = (real)((dt * vel) / dis);
+        real conu = tvd_upwind(m, ie, l, ic, id, cond, cv);
+        real grad = cond - conu;
+        real rf;
+        if ((real)fabs((double)(conc - cond)) < eps) rf = -1;
+        else rf = grad / (cond - conc) - (real)1;
+        real psi = (real)dmax(0., dmax(dmin(1., 2. * rf), dmin(2., rf)));
+        real conf = conc + (real)0.5 * psi * (cond - conc) * ((real)1 - alfa);
+        real term = (real)(fact * conf * f[ii - 1]);
+        fl[ic - 1] = fl[ic - 1] - term;
+        fl[id - 1] = fl[id - 1] + term;
+    }
+}
+
+/* conz3d_omp + conz3d_element + conz3d_nodes: concentration_omp.f90:65-886.
+ * Returns 0, or 1 when a column has a zero diagonal. */
+int oc_conz3d(const oc_mesh *m, const oc_state *s, const oc_params *p,
+              real *cn1, const real *cbound, double ddt, int istot, int isact)
+{
+    const int nlvdi = m->nlvdi, nkn = m->nkn;
+    const int itvd = p->itvd, itvdv = p->itvdv;
+    double az = p->azpar, ad = p->adpar, aa = p->aapar;
+    double azt = 1. - az, adt = 1. - ad, aat = 1. - aa;
+    double rstot = istot;
+    double rso = (isact - 1) / rstot, rsn = isact / rstot;
+    double rsot = 1. - rso, rsnt = 1. - rsn;
+    double dt = ddt / rstot;
+    int btvd = itvd > 0, btvdv = itvdv > 0;
+    size_t nn = (size_t)nlvdi * nkn;
+    double *cn = (double *)calloc(nn, sizeof(double));
+    double *cdiag = (double *)calloc(nn, sizeof(double));
+    double *clow = (double *)calloc(nn, sizeof(double));
+    double *chigh = (double *)calloc(nn, sizeof(double));
+    int L2 = nlvdi + 2;
+    double *hold = (double *)malloc(sizeof(double) * 3 * L2);
+    double *hnew = (double *)malloc(sizeof(double) * 3 * L2);
+    double *cl = (double *)malloc(sizeof(double) * 3 * L2);
+    double *wl = (double *)malloc(sizeof(double) * 3 * L2);
+    double *vflux = (double *)malloc(sizeof(double) * 3 * L2);
+    double *haver = (double *)malloc(sizeof(double) * L2);
+    double *presentl = (double *)malloc(sizeof(double) * L2);
+    double *clc = (double *)malloc(sizeof(double) * 3 * L2);
+    double *clm = (double *)malloc(sizeof(double) * 3 * L2);
+    double *clp = (double *)malloc(sizeof(double) * 3 * L2);
+#define H2(a, l, ii) (a)[(l) + L2 * ((ii) - 1)]
+    for (int q = 0; q < m->nel; q++) {
+        int ie = m->color_elems[q];
+        memset(hold, 0, sizeof(double) * 3 * L2);
+        memset(hnew, 0, sizeof(double) * 3 * L2);
+        memset(cl, 0, sizeof(double) * 3 * L2);
+        memset(wl, 0, sizeof(double) * 3 * L2);
+        memset(vflux, 0, sizeof(double) * 3 * L2);
+        memset(haver, 0, sizeof(double) * L2);
+        memset(presentl, 0, sizeof(double) * L2);
+        memset(clc, 0, sizeof(double) * 3 * L2);
+        memset(clm, 0, sizeof(double) * 3 * L2);
+        memset(clp, 0, sizeof(double) * 3 * L2);
+        int kn[3];
+        double b[3], c[3];
+        for (int ii = 0; ii < 3; ii++) {
+            kn[ii] = m->nen3v[3 * ie + ii];
+            b[ii] = EV(ii + 4, ie);
+            c[ii] = EV(ii + 7, ie);
+        }
+        double aj = EV(10, ie), aj4 = 4. * aj;
+        int ilevel = m->ilhv[ie], jlevel = 1;
+        for (int l = jlevel; l <= ilevel; l++) {
+            haver[l] = rso * A3E(s->hdenv, l, ie) + rsot * A3E(s->hdeov, l, ie);
+            presentl[l] = 1.;
+            for (int ii = 1; ii <= 3; ii++) {
+                int k = kn[ii - 1];
+                double hn = A3K(s->hdknv, l, k), ho = A3K(s->hdkov, l, k);
+                H2(hold, l, ii) = rso * hn + rsot * ho;
+                H2(hnew, l, ii) = rsn * hn + rsnt * ho;
+                H2(cl, l, ii) = A3K(cn1, l, k);
+                H2(wl, l, ii) = AW(s->wlnv, l, k);
+            }
+        }
+        for (int ii = 1; ii <= 3; ii++) H2(wl, ilevel, ii) = 0.;
+        /* vertical_flux_ie: tvd_admin.f90:930-1016 */
+        for (int ii = 1; ii <= 3; ii++) {
+            int lmin = jlevel, lmax = ilevel;
+            for (int l = lmin; l <= lmax - 1; l++) {
+                double w = H2(wl, l, ii);
+                double conc = (w > 0.) ? H2(cl, l + 1, ii) : H2(cl, l, ii);
+                double conf = conc;
+                if (btvdv) {
+                    double cond, conu;
+                    if (w > 0.) {
+                        cond = H2(cl, l, ii);
+                        conu = cond;
+                        if (l + 2 <= lmax) conu = H2(cl, l + 2, ii);
+                    } else {
+                        cond = H2(cl, l + 1, ii);
+                        conu = cond;
+                        if (l - 1 >= lmin) conu = H2(cl, l - 1, ii);
+                    }
+                    double hdis = 0.5 * (H2(hold, l, ii) + H2(hold, l + 1, ii));
+                    double alfa = dt * fabs(w) / hdis;
+                    double rf;
+                    if (fabs(conc - cond) < (double)1.e-8f) rf = -1.;
+                    else rf = (cond - conu) / (cond - conc) - 1.;
+                    double psi = dmax(0., dmax(dmin(1., 2. * rf), dmin(2., rf)));
+                    conf = conc + 0.5 * psi * (cond - conc) * (1. - alfa);
+                }
+                H2(vflux, l, ii) = w * conf;
+            }
+            H2(vflux, lmin - 1, ii) = 0.;
+            H2(vflux, lmax, ii) = 0.;
+        }
+        int iext = 0;
+        for (int ii = 0; ii < 3; ii++)
+            if (m->iextb && m->iextb[kn[ii]]) iext++;
+        for (int l = jlevel; l <= ilevel; l++) {
+            double us = az * A3E(s->utlnv, l, ie) + azt * A3E(s->utlov, l, ie);
+            double vs = az * A3E(s->vtlnv, l, ie) + azt * A3E(s->vtlov, l, ie);
+            double rk3 = (real)3 * p->rkpar * A3E(s->difhv, l, ie);
+            int itot = 0, isum = 0;
+            double f[3], fw[3], fd[3], fl[3], wdiff[3];
+            for (int ii = 1; ii <= 3; ii++) {
+                int k = kn[ii - 1];
+                f[ii - 1] = us * b[ii - 1] + vs * c[ii - 1];
+                if (f[ii - 1] < 0.) { itot++; isum += ii; }
+                double waux = 0.;
+                for (int iii = 1; iii <= 3; iii++)
+                    waux = waux + m->wdifhv[(iii - 1) + 3 * (ii - 1) + 9 * ie] * H2(cl, l, iii);
+                wdiff[ii - 1] = waux;
+                double rvptop = (real)(AW(s->difv, l - 1, k) + p->difmol);
+                double rvpbot = (real)(AW(s->difv, l, k) + p->difmol);
+                double hmotop = 2. * rvptop * presentl[l - 1] / (H2(hold, l - 1, ii) + H2(hold, l, ii));
+                double hmobot = 2. * rvpbot * presentl[l + 1] / (H2(hold, l, ii) + H2(hold, l + 1, ii));
+                double hmntop = 2. * rvptop * presentl[l - 1] / (H2(hnew, l - 1, ii) + H2(hnew, l, ii));
+                double hmnbot = 2. * rvpbot * presentl[l + 1] / (H2(hnew, l, ii) + H2(hnew, l + 1, ii));
+                fd[ii - 1] = adt * ((H2(cl, l, ii) - H2(cl, l + 1, ii)) * hmobot -
+                                    (H2(cl, l - 1, ii) - H2(cl, l, ii)) * hmotop);
+                H2(clc, l, ii) += ad * (hmntop + hmnbot);
+                H2(clm, l, ii) -= ad * hmntop;
+                H2(clp, l, ii) -= ad * hmnbot;
+                double w = H2(wl, l - 1, ii);
+                if (l == jlevel) w = 0.;
+                if (w >= 0.) H2(clc, l, ii) += aa * w;
+                else H2(clm, l, ii) += aa * w;
+                w = H2(wl, l, ii);
+                if (l == ilevel) w = 0.;
+                if (w > 0.) H2(clp, l, ii) -= aa * w;
+                else H2(clc, l, ii) -= aa * w;
+                fw[ii - 1] = aat * (H2(vflux, l - 1, ii) - H2(vflux, l, ii));
+            }
+            if (itot == 1) {
+                for (int ii = 0; ii < 3; ii++) fl[ii] = f[ii] * H2(cl, l, isum);
+            } else if (itot == 2) {
+                int is2 = 6 - isum;
+                for (int ii = 0; ii < 3; ii++) fl[ii] = f[ii] * H2(cl, l, ii + 1);
+                fl[is2 - 1] = 0.;
+                fl[is2 - 1] = -(fl[0] + fl[1] + fl[2]);
+            } else {
+                fl[0] = fl[1] = fl[2] = 0.;
+            }
+            if (btvd && iext == 0) {
+                double cll[3] = {H2(cl, l, 1), H2(cl, l, 2), H2(cl, l, 3)};
+                tvd_fluxes(m, s, ie, l, itot, isum, dt, cll, cn1, f, fl);
+            }
+            for (int ii = 1; ii <= 3; ii++) {
+                int k = kn[ii - 1];
+                double hmed = haver[l];
+                double cexpl = aj4 * (H2(hold, l, ii) * H2(cl, l, ii) +
+                                      dt * (0. + 3. * fl[ii - 1] - fw[ii - 1] -
+                                            rk3 * hmed * wdiff[ii - 1] - fd[ii - 1]));
+                double alow = aj4 * dt * H2(clm, l, ii);
+                double ahigh = aj4 * dt * H2(clp, l, ii);
+                double adiag = aj4 * dt * H2(clc, l, ii) + aj4 * (1. + dt * 0.) * H2(hnew, l, ii);
+                A3K(cn, l, k) += cexpl;
+                A3K(clow, l, k) += alow;
+                A3K(chigh, l, k) += ahigh;
+                A3K(cdiag, l, k) += adiag;
+            }
+        }
+    }
+    /* conz3d_nodes */
+    int ret = 0;
+    for (int k = 0; k < nkn && !ret; k++) {
+        int ilevel = m->ilhkv[k], jlevel = 1;
+        for (int l = jlevel; l <= ilevel; l++) {
+            double cconz = cbound ? A3K(cbound, l, k) : 0.;
+            double qflux = s->mfluxv ? A3K(s->mfluxv, l, k) : 0.;
+            if (qflux < 0. && m->ibnd && m->ibnd[k] > 0) cconz = A3K(cn1, l, k);
+            double mflux = qflux * cconz;
+            A3K(cn, l, k) += dt * mflux;
+        }
+        if ((aa == 0. && ad == 0.) || m->nlv == 1) {
+            for (int l = jlevel; l <= ilevel; l++)
+                if (A3K(cdiag, l, k) != 0.) A3K(cn, l, k) = A3K(cn, l, k) / A3K(cdiag, l, k);
+        } else {
+            double aux = 1. / A3K(cdiag, jlevel, k);
+            A3K(chigh, jlevel, k) *= aux;
+            A3K(cn, jlevel, k) *= aux;
+            for (int l = jlevel + 1; l <= ilevel; l++) {
+                if (A3K(cdiag, l, k) == 0.) { ret = 1; break; }
+                aux = 1. / (A3K(cdiag, l, k) - A3K(clow, l, k) * A3K(chigh, l - 1, k));
+                A3K(chigh, l, k) *= aux;
+                A3K(cn, l, k) = (A3K(cn, l, k) - A3K(clow, l, k) * A3K(cn, l - 1, k)) * aux;
+            }
+            for (int l = ilevel - 1; l >= jlevel; l--)
+                A3K(cn, l, k) = A3K(cn, l, k) - A3K(cn, l + 1, k) * A3K(chigh, l, k);
+        }
+    }
+    if (!ret)
+        for (size_t i = 0; i < nn; i++) cn1[i] = (real)cn[i];
+#undef H2
+    free(cn); free(cdiag); free(clow); free(chigh);
+    free(hold); free(hnew); free(cl); free(wl); free(vflux); free(haver); free(presentl);
+    free(clc); free(clm); free(clp);
+    return ret;
+}
+
+/* make_scal_flux: bnd_routines.f90:1215-1303 (ssurf given, rqdsv = 0) */
+static void make_scal_flux(const oc_mesh *m, const oc_state *s, const real *r3v, const real *scal,
+                           real ssurf, real *sconz)
+{
+    const int nlvdi = m->nlvdi;
+    for (int k = 0; k < m->nkn; k++) {
+        int lmax = m->ilhkv[k], lmin = 1;
+        real sflux_lmin = 0;
+        for (int l = lmin; l <= lmax; l++) {
+            real sflux = 0;
+            A3K(sconz, l, k) = 0;
+            real flux = s->mfluxv ? A3K(s->mfluxv, l, k) : (real)0;
+            real conz = r3v ? A3K(r3v, l, k) : OC_FLAG;
+            if (flux != 0 || conz != OC_FLAG) {
+                if (conz <= (real)-990.) conz = A3K(scal, l, k);
+                if (flux < 0) conz = A3K(scal, l, k);
+                sflux = flux * conz;
+                A3K(sconz, l, k) = conz;
+            }
+            if (l == lmin) sflux_lmin = sflux;
+        }
+        real conz = ssurf;
+        if (ssurf <= (real)-990) conz = A3K(scal, lmin, k);
+        sflux_lmin = sflux_lmin + (real)0 * conz;
+        real mfl = s->mfluxv ? A3K(s->mfluxv, lmin, k) : (real)0;
+        if (mfl != 0) A3K(sconz, lmin, k) = sflux_lmin / mfl;
+    }
+}
+
+/* scal3sh substep loop: concentration.f90:565-810 (istot given) */
+int oc_scal3sh(const oc_mesh *m, oc_state *s, const oc_params *p, real *cnv,
+               const real *rcv, int istot, int clip_small)
+{
+    size_t nn = (size_t)m->nlvdi * m->nkn;
+    real *sbconz = (real *)calloc(nn, sizeof(real));
+    int ret = 0;
+    real ssurface = clip_small ? (real)0 : (real)-999.;
+    for (int isact = 1; isact <= istot; isact++) {
+        make_scal_flux(m, s, rcv, cnv, ssurface, sbconz);
+        if (clip_small)
+            for (size_t i = 0; i < nn; i++)
+                if (cnv[i] < (real)1.e-15) cnv[i] = 0;
+        ret = oc_conz3d(m, s, p, cnv, sbconz, (double)(real)p->dt, istot, isact);
+        if (ret) break;
+    }
+    free(sbconz);
+    return ret;
+}
+
+/* total tracer content sum_k sum_l c * vol (FP64 accumulation) */
+double oc_massconc(const oc_mesh *m, const oc_state *s, const real *cnv, int mode)
+{
+    const int nlvdi = m->nlvdi;
+    const real *hd = mode > 0 ? s->hdknv : s->hdkov;
+    double total = 0.;
+    for (int k = 0; k < m->nkn; k++)
+        for (int l = 1; l <= m->ilhkv[k]; l++)
+            total += (double)A3K(cnv, l, k) * (double)(A3K(hd, l, k) * A3K(m->areakv, l, k));
+    return total;
+}

@@ -1,0 +1,771 @@
+// shyhip_api.cpp -- C ABI of libshyhip.so (declared in include/shyhip.h).
+//
+// Owns the device mirrors of the model state, converts between the
+// reference's host layout (Fortran (l,ie), layer fastest) and the device's
+// layer-major layout, and sequences the kernels of one time step in the order
+// of hydro (src/shyfem/hydrodynamic.f90:284-484) and scal3sh
+// (src/shyfem/concentration.f90:565-810).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "shyhip_internal.h"
+
+using namespace shy;
+
+namespace shy {
+
+int hip_check(Model &M, hipError_t e, const char *what) {
+    if (e == hipSuccess) return 0;
+    M.err = std::string("HIP error in ") + what + ": " + hipGetErrorString(e);
+    return 1;
+}
+
+}  // namespace shy
+
+namespace {
+
+thread_local std::string g_err;   // errors before a handle exists
+
+int fail(Model *M, const std::string &msg, int code = 1) {
+    if (M) M->err = msg;
+    else g_err = msg;
+    return code;
+}
+
+template <typename T>
+int dalloc(Model &M, T **p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    HIPCHK(M, hipMalloc((void **)p, n * sizeof(T)));
+    HIPCHK(M, hipMemsetAsync(*p, 0, n * sizeof(T), M.stream));
+    return 0;
+}
+
+int dalloc_s(Model &M, void **p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    HIPCHK(M, hipMalloc(p, n * M.esize()));
+    HIPCHK(M, hipMemsetAsync(*p, 0, n * M.esize(), M.stream));
+    return 0;
+}
+
+template <typename T>
+int upload(Model &M, T *d, const T *h, size_t n) {
+    if (n == 0) return 0;
+    HIPCHK(M, hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, M.stream));
+    HIPCHK(M, hipStreamSynchronize(M.stream));
+    return 0;
+}
+
+// upload host floats/doubles into a device array of the model's storage type
+int upload_s(Model &M, void *d, const double *h, size_t n) {
+    if (M.prec == SHYHIP_FP64) return upload(M, (double *)d, h, n);
+    std::vector<float> t(n);
+    for (size_t i = 0; i < n; i++) t[i] = (float)h[i];
+    return upload(M, (float *)d, t.data(), n);
+}
+
+void dfree(void *p) {
+    if (p) (void)hipFree(p);
+}
+
+int nl_template(int nlvdi) {
+    const int opts[] = {1, 2, 4, 8, 12, 16, 20, 24, 32};
+    for (int o : opts)
+        if (nlvdi <= o) return o;
+    return -1;
+}
+
+// field descriptors for push/pull
+enum Kind { K_ELEM3, K_NODE3, K_NODEW, K_VERT, K_ELEM2, K_NODE2, K_DD, K_IELEM, K_INODE, K_CSR, K_RHS };
+struct Field {
+    void **ptr;
+    Kind kind;
+    bool isint = false;
+    bool isdouble_dev = false;  // device array always double
+};
+
+bool field_of(Model &M, int id, Field &f) {
+    auto S = [&](void **p, Kind k) { f = Field{p, k, false, false}; return true; };
+    switch (id) {
+    case SHYHIP_ZNV: return S(&M.s.zn, K_NODE2);
+    case SHYHIP_ZOV: return S(&M.s.zo, K_NODE2);
+    case SHYHIP_ZENV: return S(&M.s.zen, K_VERT);
+    case SHYHIP_ZEOV: return S(&M.s.zeo, K_VERT);
+    case SHYHIP_UTLNV: return S(&M.s.utn, K_ELEM3);
+    case SHYHIP_VTLNV: return S(&M.s.vtn, K_ELEM3);
+    case SHYHIP_UTLOV: return S(&M.s.uto, K_ELEM3);
+    case SHYHIP_VTLOV: return S(&M.s.vto, K_ELEM3);
+    case SHYHIP_UNV: return S(&M.s.un, K_ELEM2);
+    case SHYHIP_VNV: return S(&M.s.vn, K_ELEM2);
+    case SHYHIP_UOV: return S(&M.s.uo, K_ELEM2);
+    case SHYHIP_VOV: return S(&M.s.vo, K_ELEM2);
+    case SHYHIP_ULNV: return S(&M.s.uln, K_ELEM3);
+    case SHYHIP_VLNV: return S(&M.s.vln, K_ELEM3);
+    case SHYHIP_WLNV: return S(&M.s.wln, K_NODEW);
+    case SHYHIP_WLOV: return S(&M.s.wlo, K_NODEW);
+    case SHYHIP_HDKNV: return S(&M.s.hdkn, K_NODE3);
+    case SHYHIP_HDKOV: return S(&M.s.hdko, K_NODE3);
+    case SHYHIP_HDENV: return S(&M.s.hden, K_ELEM3);
+    case SHYHIP_HDEOV: return S(&M.s.hdeo, K_ELEM3);
+    case SHYHIP_IWEGV: f = Field{(void **)&M.s.iweg, K_IELEM, true}; return true;
+    case SHYHIP_IWETV: f = Field{(void **)&M.s.iwet, K_IELEM, true}; return true;
+    case SHYHIP_INODV: f = Field{(void **)&M.s.inod, K_INODE, true}; return true;
+    case SHYHIP_RFRICV: return S(&M.s.rfric, K_ELEM2);
+    case SHYHIP_DDXV: return S(&M.s.ddx, K_DD);
+    case SHYHIP_DDYV: return S(&M.s.ddy, K_DD);
+    case SHYHIP_VISV: return S(&M.s.visv, K_NODEW);
+    case SHYHIP_DIFV: return S(&M.s.difv, K_NODEW);
+    case SHYHIP_DIFHV: return S(&M.s.difhv, K_ELEM3);
+    case SHYHIP_TAUXNV: return S(&M.s.taux, K_NODE2);
+    case SHYHIP_TAUYNV: return S(&M.s.tauy, K_NODE2);
+    case SHYHIP_RZV: return S(&M.s.rzv, K_NODE2);
+    case SHYHIP_RQV: return S(&M.s.rqv, K_NODE2);
+    case SHYHIP_MFLUXV: return S(&M.s.mflux, K_NODE3);
+    case SHYHIP_AREAKV: return S(&M.m.areakv, K_NODE3);
+    case SHYHIP_CSR_VAL: f = Field{(void **)&M.s.aval, K_CSR, false, true}; return true;
+    case SHYHIP_RHS: f = Field{(void **)&M.s.rhs, K_RHS, false, true}; return true;
+    default:
+        if (id >= SHYHIP_TRACER0 && id < SHYHIP_TRACER0 + M.ntr) return S(&M.s.tracer[id - SHYHIP_TRACER0], K_NODE3);
+        return false;
+    }
+}
+
+// shape: host index h -> device index d
+struct Shape {
+    size_t n;
+    int rows, cols;   // host (rows fastest) = device (cols fastest) transposition
+};
+Shape shape_of(Model &M, Kind k) {
+    switch (k) {
+    case K_ELEM3: return {(size_t)M.nlvdi * M.nel, M.nlvdi, M.nel};
+    case K_NODE3: return {(size_t)M.nlvdi * M.nkn, M.nlvdi, M.nkn};
+    case K_NODEW: return {(size_t)(M.nlvdi + 1) * M.nkn, M.nlvdi + 1, M.nkn};
+    case K_VERT: return {(size_t)3 * M.nel, 3, M.nel};
+    case K_DD: return {(size_t)2 * M.nlvdi * M.nel, 2 * M.nlvdi, M.nel};
+    case K_ELEM2: case K_IELEM: return {(size_t)M.nel, 1, M.nel};
+    case K_NODE2: case K_INODE: return {(size_t)M.nkn, 1, M.nkn};
+    case K_CSR: return {(size_t)M.nnz, 1, M.nnz};
+    case K_RHS: return {(size_t)M.nkn, 1, M.nkn};
+    }
+    return {0, 0, 0};
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *shyhip_last_error(const shyhip_model *h) { return h ? h->M.err.c_str() : g_err.c_str(); }
+
+int shyhip_device_count(int *n) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *n = c;
+    return 0;
+}
+
+int shyhip_create(shyhip_model **out, int precision, int nkn, int nel, int nlvdi, int nlv, int ntracer) {
+    *out = nullptr;
+    if (precision != SHYHIP_FP64 && precision != SHYHIP_FP32) return fail(nullptr, "precision must be FP64 or FP32");
+    if (nkn <= 0 || nel <= 0 || nlvdi <= 0 || nlv <= 0 || nlv > nlvdi || ntracer < 0)
+        return fail(nullptr, "invalid sizes");
+    const int tmpl = nl_template(nlvdi);
+    if (tmpl < 0) return fail(nullptr, "more than 32 layers are not supported");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(nullptr, "no HIP device available");
+    auto *h = new shyhip_model();
+    Model &M = h->M;
+    M.prec = precision;
+    M.nkn = nkn; M.nel = nel; M.nlvdi = nlvdi; M.nlv = nlv; M.ntr = ntracer;
+    M.nl_tmpl = tmpl;
+    if (hip_check(M, hipStreamCreate(&M.stream), "hipStreamCreate")) { delete h; return fail(nullptr, "stream"); }
+    if (hip_check(M, hipHostMalloc((void **)&M.h_pinned, 64 * sizeof(int)), "hipHostMalloc")) {
+        delete h;
+        return fail(nullptr, "pinned");
+    }
+    // default parameters (def_para.f90)
+    shyhip_params &p = M.p;
+    p.dt = 300.; p.azpar = 0.5f; p.ampar = 0.5f; p.afpar = 0.5f; p.atpar = 1.f; p.adpar = 1.f; p.aapar = 1.f;
+    p.grav = 9.81f; p.rowass = 1025.f; p.vismol = 1.0e-6f; p.ireib = 0; p.uvmin = 0.2f;
+    p.hzmin = 0.01f; p.hzoff = 0.05f; p.hzon = 0.10f; p.volmin = 1.f; p.iclin = 0;
+    p.rtol = (double)1.0e-8f; p.atol = (double)1.0e-8f; p.itermax = 1000; p.refine_rtol = 1e-10;
+    p.block_rows = 0; p.rkpar = 0.f; p.difmol = 1.0e-6f; p.itvd = 0; p.itvdv = 0;
+    const size_t ne = nel, nk = nkn, L = nlvdi;
+    State &s = M.s;
+    int rc = 0;
+    rc |= dalloc_s(M, &s.zn, nk); rc |= dalloc_s(M, &s.zo, nk);
+    rc |= dalloc_s(M, &s.zen, 3 * ne); rc |= dalloc_s(M, &s.zeo, 3 * ne);
+    for (void **q : {&s.utn, &s.vtn, &s.uto, &s.vto, &s.uln, &s.vln, &s.hden, &s.hdeo, &s.difhv}) rc |= dalloc_s(M, q, L * ne);
+    for (void **q : {&s.un, &s.vn, &s.uo, &s.vo, &s.rfric}) rc |= dalloc_s(M, q, ne);
+    for (void **q : {&s.wln, &s.wlo, &s.visv, &s.difv}) rc |= dalloc_s(M, q, (L + 1) * nk);
+    for (void **q : {&s.hdkn, &s.hdko, &s.mflux}) rc |= dalloc_s(M, q, L * nk);
+    for (void **q : {&s.taux, &s.tauy, &s.rzv, &s.rqv}) rc |= dalloc_s(M, q, nk);
+    rc |= dalloc_s(M, &s.ddx, 2 * L * ne); rc |= dalloc_s(M, &s.ddy, 2 * L * ne);
+    rc |= dalloc(M, &s.iweg, ne); rc |= dalloc(M, &s.iwet, ne); rc |= dalloc(M, &s.inod, nk);
+    rc |= dalloc(M, &s.zel, 6 * ne);
+    rc |= dalloc(M, &s.rhs, nk); rc |= dalloc(M, &s.x, nk);
+    rc |= dalloc_s(M, &s.hbuf, 3 * L * ne);
+    s.tracer.assign(ntracer, nullptr);
+    for (int t = 0; t < ntracer; t++) rc |= dalloc_s(M, &s.tracer[t], L * nk);
+    rc |= dalloc(M, &M.d_err, 8);
+    if (rc) {
+        std::string e = M.err;
+        shyhip_destroy(h);
+        return fail(nullptr, "device allocation failed: " + e);
+    }
+    if (launch_fill(M, s.rzv, (double)kFlag, nk)) return fail(&M, M.err);
+    *out = h;
+    return 0;
+}
+
+int shyhip_destroy(shyhip_model *h) {
+    if (!h) return 0;
+    Model &M = h->M;
+    if (M.stream) (void)hipStreamSynchronize(M.stream);
+    State &s = M.s;
+    for (void *q : {s.zn, s.zo, s.zen, s.zeo, s.utn, s.vtn, s.uto, s.vto, s.uln, s.vln, s.un, s.vn, s.uo, s.vo,
+                    s.wln, s.wlo, s.hdkn, s.hdko, s.hden, s.hdeo, s.rfric, s.ddx, s.ddy, s.visv, s.difv, s.difhv,
+                    s.taux, s.tauy, s.rzv, s.rqv, s.mflux, s.hbuf})
+        dfree(q);
+    dfree(s.iweg); dfree(s.iwet); dfree(s.inod); dfree(s.zel); dfree(s.aval); dfree(s.rhs); dfree(s.x);
+    for (void *t : s.tracer) dfree(t);
+    Mesh &m = M.m;
+    for (void *q : {(void *)m.nen3v, (void *)m.geo, (void *)m.ang, (void *)m.dis, (void *)m.ilhv, (void *)m.ilhkv,
+                    m.hm3v, m.hev, m.hlv, m.hldv, m.areakv, (void *)m.ibnd, (void *)m.zbound, (void *)m.iextb,
+                    (void *)m.ibtyp, m.fcorv, m.czv, (void *)m.nk_ptr, (void *)m.nk_el, (void *)m.nkc_el,
+                    (void *)m.rowptr, (void *)m.cols, (void *)m.ijp, (void *)m.bnd_nodes, (void *)m.bnd_id,
+                    (void *)m.blk_rowptr, (void *)m.blk_cols, (void *)m.blk_src, (void *)m.blk_diag,
+                    (void *)m.fwd_rows, (void *)m.fwd_lptr, (void *)m.fwd_nlev, (void *)m.bwd_rows,
+                    (void *)m.bwd_lptr, (void *)m.bwd_nlev, (void *)m.tvd_el, (void *)m.tvd_kn, m.tvd_w})
+        dfree(q);
+    Solver &v = M.sv;
+    for (void *q : {v.lu, v.r0, v.r, v.p0, v.p1, v.v0, v.v1, v.s, v.t, v.dx, v.b, v.x0, (void *)v.part,
+                    (void *)v.scal, (void *)v.ctrl, v.a32, (void *)v.r64})
+        dfree(q);
+    dfree(M.d_err);
+    if (M.h_pinned) (void)hipHostFree(M.h_pinned);
+    if (M.stream) (void)hipStreamDestroy(M.stream);
+    delete h;
+    return 0;
+}
+
+int shyhip_set_params(shyhip_model *h, const shyhip_params *p) {
+    if (!h || !p) return fail(nullptr, "null argument");
+    Model &M = h->M;
+    if (p->dt <= 0.) return fail(&M, "dt must be positive");
+    if (p->itvdv > 0 && p->aapar != 0.f)
+        return fail(&M, "implicit vertical advection (aapar != 0) cannot be combined with itvdv > 0");
+    if (p->rkpar != 0.f) return fail(&M, "horizontal diffusion (rkpar != 0) is not supported by the device path");
+    if (p->itvd == 1) return fail(&M, "itvd = 1 (gradient TVD) is not supported by the device path; use 0 or 2");
+    if (p->block_rows < 0 || p->block_rows > kMaxBlockRows) return fail(&M, "block_rows must be in 0..1024");
+    M.p = *p;
+    return 0;
+}
+
+int shyhip_set_basin(shyhip_model *h, const int *nen3v, const float *xgv, const float *ygv, const float *hm3v,
+                     const float *hev, const float *hlv, const float *hldv, const int *ilhv, const int *ilhkv) {
+    if (!h) return fail(nullptr, "null handle");
+    Model &M = h->M;
+    const int nkn = M.nkn, nel = M.nel, L = M.nlvdi;
+    M.h_nen3v.resize(3 * (size_t)nel);
+    for (size_t i = 0; i < 3 * (size_t)nel; i++) {
+        const int k = nen3v[i] - 1;
+        if (k < 0 || k >= nkn) return fail(&M, "nen3v holds a node id outside 1..nkn");
+        M.h_nen3v[i] = k;
+    }
+    for (int ie = 0; ie < nel; ie++)
+        if (ilhv[ie] < 1 || ilhv[ie] > M.nlv) return fail(&M, "ilhv outside 1..nlv");
+    for (int k = 0; k < nkn; k++)
+        if (ilhkv[k] < 1 || ilhkv[k] > M.nlv) return fail(&M, "ilhkv outside 1..nlv");
+    M.h_x.assign(xgv, xgv + nkn);
+    M.h_y.assign(ygv, ygv + nkn);
+    M.h_ilhv.assign(ilhv, ilhv + nel);
+    M.h_ilhkv.assign(ilhkv, ilhkv + nkn);
+    M.h_ev.assign(19 * (size_t)nel, 0.);
+    host_set_ev(nkn, nel, M.h_nen3v.data(), xgv, ygv, M.h_ev.data());
+    for (int ie = 0; ie < nel; ie++)
+        if (!(M.h_ev[19 * (size_t)ie + 9] > 0.)) return fail(&M, "element " + std::to_string(ie + 1) + " is not counter-clockwise");
+    M.nnz = host_node_list(nkn, nel, M.h_nen3v.data(), M.h_rowptr, M.h_cols, M.h_ijp);
+    if (M.nnz <= 0) return fail(&M, "building the zeta-matrix pattern failed");
+    M.ncolor = host_color(nkn, nel, M.h_nen3v.data(), M.h_color);
+    if (M.ncolor <= 0) return fail(&M, "element colouring failed (more than 31 colours)");
+    Mesh &m = M.m;
+    int rc = 0;
+    // geometry, SoA
+    std::vector<double> geo(7 * (size_t)nel), ang(3 * (size_t)nel), dis(3 * (size_t)nel);
+    for (int ie = 0; ie < nel; ie++) {
+        const double *e = &M.h_ev[19 * (size_t)ie];
+        for (int q = 0; q < 6; q++) geo[(size_t)q * nel + ie] = e[3 + q];
+        geo[(size_t)6 * nel + ie] = e[9];
+        for (int q = 0; q < 3; q++) {
+            ang[(size_t)q * nel + ie] = e[10 + q];
+            dis[(size_t)q * nel + ie] = e[16 + q];
+        }
+    }
+    rc |= dalloc(M, &m.nen3v, 3 * (size_t)nel) || upload(M, m.nen3v, M.h_nen3v.data(), 3 * (size_t)nel);
+    rc |= dalloc(M, &m.geo, geo.size()) || upload(M, m.geo, geo.data(), geo.size());
+    rc |= dalloc(M, &m.ang, ang.size()) || upload(M, m.ang, ang.data(), ang.size());
+    rc |= dalloc(M, &m.dis, dis.size()) || upload(M, m.dis, dis.data(), dis.size());
+    rc |= dalloc(M, &m.ilhv, nel) || upload(M, m.ilhv, ilhv, nel);
+    rc |= dalloc(M, &m.ilhkv, nkn) || upload(M, m.ilhkv, ilhkv, nkn);
+    // depths (vertex-major on device)
+    std::vector<double> t(3 * (size_t)nel);
+    for (int ie = 0; ie < nel; ie++)
+        for (int ii = 0; ii < 3; ii++) t[(size_t)ii * nel + ie] = hm3v[3 * (size_t)ie + ii];
+    rc |= dalloc_s(M, &m.hm3v, t.size()) || upload_s(M, m.hm3v, t.data(), t.size());
+    std::vector<double> he(hev, hev + nel), hl(hlv, hlv + L), hd(hldv, hldv + L);
+    rc |= dalloc_s(M, &m.hev, nel) || upload_s(M, m.hev, he.data(), nel);
+    rc |= dalloc_s(M, &m.hlv, L) || upload_s(M, m.hlv, hl.data(), L);
+    rc |= dalloc_s(M, &m.hldv, L) || upload_s(M, m.hldv, hd.data(), L);
+    // set_area (elems_dealing.f90:658-700) in the storage precision, element order
+    {
+        std::vector<double> area((size_t)L * nkn, 0.);
+        if (M.prec == SHYHIP_FP64) {
+            for (int ie = 0; ie < nel; ie++) {
+                const double areafv = (12. * M.h_ev[19 * (size_t)ie + 9]) / 3.;
+                for (int ii = 0; ii < 3; ii++)
+                    for (int l = 0; l < ilhv[ie]; l++) area[(size_t)l * nkn + M.h_nen3v[3 * ie + ii]] += areafv;
+            }
+        } else {
+            std::vector<float> af((size_t)L * nkn, 0.f);
+            for (int ie = 0; ie < nel; ie++) {
+                const float areafv = (float)(12. * M.h_ev[19 * (size_t)ie + 9]) / 3.f;
+                for (int ii = 0; ii < 3; ii++)
+                    for (int l = 0; l < ilhv[ie]; l++) af[(size_t)l * nkn + M.h_nen3v[3 * ie + ii]] += areafv;
+            }
+            for (size_t i = 0; i < af.size(); i++) area[i] = af[i];
+        }
+        rc |= dalloc_s(M, &m.areakv, area.size()) || upload_s(M, m.areakv, area.data(), area.size());
+    }
+    // node -> element incidence: element order and colour order
+    {
+        std::vector<int> ptr(nkn + 1, 0), el(3 * (size_t)nel), elc(3 * (size_t)nel);
+        for (int ie = 0; ie < nel; ie++)
+            for (int ii = 0; ii < 3; ii++) ptr[M.h_nen3v[3 * ie + ii] + 1]++;
+        for (int k = 0; k < nkn; k++) ptr[k + 1] += ptr[k];
+        std::vector<int> pos(ptr.begin(), ptr.end() - 1), pos2 = pos;
+        for (int ie = 0; ie < nel; ie++)
+            for (int ii = 0; ii < 3; ii++) {
+                const int k = M.h_nen3v[3 * ie + ii];
+                el[pos[k]++] = (ie << 2) | ii;
+            }
+        // colour order: colours ascending, elements ascending within a colour
+        for (int c = 1; c <= M.ncolor; c++)
+            for (int ie = 0; ie < nel; ie++) {
+                if (M.h_color[ie] != c) continue;
+                for (int ii = 0; ii < 3; ii++) {
+                    const int k = M.h_nen3v[3 * ie + ii];
+                    elc[pos2[k]++] = (ie << 2) | ii;
+                }
+            }
+        rc |= dalloc(M, &m.nk_ptr, nkn + 1) || upload(M, m.nk_ptr, ptr.data(), ptr.size());
+        rc |= dalloc(M, &m.nk_el, el.size()) || upload(M, m.nk_el, el.data(), el.size());
+        rc |= dalloc(M, &m.nkc_el, elc.size()) || upload(M, m.nkc_el, elc.data(), elc.size());
+    }
+    // CSR pattern
+    rc |= dalloc(M, &m.rowptr, nkn + 1) || upload(M, m.rowptr, M.h_rowptr.data(), nkn + 1);
+    rc |= dalloc(M, &m.cols, M.nnz) || upload(M, m.cols, M.h_cols.data(), M.nnz);
+    rc |= dalloc(M, &m.ijp, M.h_ijp.size()) || upload(M, m.ijp, M.h_ijp.data(), M.h_ijp.size());
+    rc |= dalloc(M, &M.s.aval, M.nnz);
+    // block-ILU(0) schedule
+    {
+        int brows = M.p.block_rows > 0 ? M.p.block_rows : std::min(nkn, 256);
+        if (nkn <= kMaxBlockRows && M.p.block_rows == 0) brows = nkn;   // one block: plain ILU(0)
+        BlockSchedule bs;
+        host_block_schedule(nkn, M.h_rowptr, M.h_cols, brows, bs);
+        M.sv.nblk = bs.nblk;
+        M.sv.brows = bs.brows;
+        int mx = 0;
+        for (int b = 0; b < bs.nblk; b++) {
+            const int b0 = b * brows, b1 = std::min(nkn, b0 + brows);
+            mx = std::max(mx, bs.rowptr[b1] - bs.rowptr[b0]);
+        }
+        M.sv.max_blk_nnz = mx;
+        if ((size_t)mx * M.esize() > 64 * 1024) return fail(&M, "block too large for LDS; lower block_rows");
+        rc |= dalloc(M, &m.blk_rowptr, bs.rowptr.size()) || upload(M, m.blk_rowptr, bs.rowptr.data(), bs.rowptr.size());
+        rc |= dalloc(M, &m.blk_cols, bs.cols.size()) || upload(M, m.blk_cols, bs.cols.data(), bs.cols.size());
+        rc |= dalloc(M, &m.blk_src, bs.src.size()) || upload(M, m.blk_src, bs.src.data(), bs.src.size());
+        rc |= dalloc(M, &m.blk_diag, bs.diag.size()) || upload(M, m.blk_diag, bs.diag.data(), bs.diag.size());
+        rc |= dalloc(M, &m.fwd_rows, bs.fwd_rows.size()) || upload(M, m.fwd_rows, bs.fwd_rows.data(), bs.fwd_rows.size());
+        rc |= dalloc(M, &m.fwd_lptr, bs.fwd_lptr.size()) || upload(M, m.fwd_lptr, bs.fwd_lptr.data(), bs.fwd_lptr.size());
+        rc |= dalloc(M, &m.fwd_nlev, bs.fwd_nlev.size()) || upload(M, m.fwd_nlev, bs.fwd_nlev.data(), bs.fwd_nlev.size());
+        rc |= dalloc(M, &m.bwd_rows, bs.bwd_rows.size()) || upload(M, m.bwd_rows, bs.bwd_rows.data(), bs.bwd_rows.size());
+        rc |= dalloc(M, &m.bwd_lptr, bs.bwd_lptr.size()) || upload(M, m.bwd_lptr, bs.bwd_lptr.data(), bs.bwd_lptr.size());
+        rc |= dalloc(M, &m.bwd_nlev, bs.bwd_nlev.size()) || upload(M, m.bwd_nlev, bs.bwd_nlev.data(), bs.bwd_nlev.size());
+        for (int b = 0; b < bs.nblk; b++)
+            for (int i = b * brows; i < std::min(nkn, (b + 1) * brows); i++)
+                if (bs.diag[i] < 0) return fail(&M, "zeta pattern without diagonal");
+        Solver &v = M.sv;
+        rc |= dalloc_s(M, &v.lu, bs.cols.size());
+        for (void **q : {&v.r0, &v.r, &v.p0, &v.p1, &v.v0, &v.v1, &v.s, &v.t, &v.dx, &v.b, &v.x0})
+            rc |= dalloc_s(M, q, (size_t)nkn);
+        const size_t nparts = std::max((size_t)bs.nblk, (size_t)(nkn + kThreads - 1) / kThreads) + 1;
+        rc |= dalloc(M, &v.part, 4 * nparts);
+        rc |= dalloc(M, &v.scal, 16);
+        rc |= dalloc(M, &v.ctrl, 8);
+        rc |= dalloc(M, &v.r64, (size_t)nkn);
+        if (M.prec == SHYHIP_FP32) rc |= dalloc(M, (float **)&v.a32, (size_t)M.nnz);
+    }
+    if (rc) return fail(&M, "basin upload failed: " + M.err);
+    // defaults: no open boundary, no external flags
+    std::vector<int> zeros(nkn, 0), ty(2, 0);
+    rc |= dalloc(M, &m.ibnd, nkn) || dalloc(M, &m.zbound, nkn) || dalloc(M, &m.iextb, nkn);
+    rc |= dalloc(M, &m.ibtyp, 2);
+    rc |= dalloc_s(M, &m.fcorv, nel) || dalloc_s(M, &m.czv, nel);
+    HIPCHK(M, hipStreamSynchronize(M.stream));
+    return rc;
+}
+
+int shyhip_set_boundaries(shyhip_model *h, int nbnd, const int *ibnd, const int *ibtyp, const int *iextb) {
+    if (!h) return fail(nullptr, "null handle");
+    Model &M = h->M;
+    if (!M.m.ibnd) return fail(&M, "call shyhip_set_basin first");
+    const int nkn = M.nkn;
+    M.nbnd = nbnd;
+    M.h_ibnd.assign(ibnd, ibnd + nkn);
+    M.h_ibtyp.assign(ibtyp, ibtyp + nbnd + 1);
+    M.h_iextb.assign(iextb ? iextb : ibnd, (iextb ? iextb : ibnd) + nkn);
+    if (!iextb) std::fill(M.h_iextb.begin(), M.h_iextb.end(), 0);
+    std::vector<int> zb(nkn, 0), nodes, ids;
+    for (int k = 0; k < nkn; k++) {
+        const int b = ibnd[k];
+        if (b < 0 || b > nbnd) return fail(&M, "ibnd outside 0..nbnd");
+        if (b > 0) {
+            if (ibtyp[b] == 1) zb[k] = 1;
+            nodes.push_back(k);
+            ids.push_back(b);
+        }
+    }
+    dfree(M.m.ibtyp);
+    dfree(M.m.bnd_nodes);
+    dfree(M.m.bnd_id);
+    M.m.nbnd_nodes = (int)nodes.size();
+    int rc = 0;
+    rc |= upload(M, M.m.ibnd, ibnd, nkn) || upload(M, M.m.zbound, zb.data(), nkn) ||
+          upload(M, M.m.iextb, M.h_iextb.data(), nkn);
+    rc |= dalloc(M, &M.m.ibtyp, nbnd + 1) || upload(M, M.m.ibtyp, ibtyp, nbnd + 1);
+    rc |= dalloc(M, &M.m.bnd_nodes, nodes.size()) || upload(M, M.m.bnd_nodes, nodes.data(), nodes.size());
+    rc |= dalloc(M, &M.m.bnd_id, ids.size()) || upload(M, M.m.bnd_id, ids.data(), ids.size());
+    return rc;
+}
+
+int shyhip_set_coefficients(shyhip_model *h, const float *fcorv, const float *czv) {
+    if (!h) return fail(nullptr, "null handle");
+    Model &M = h->M;
+    if (!M.m.fcorv) return fail(&M, "call shyhip_set_basin first");
+    std::vector<double> t(M.nel, 0.);
+    M.has_coriolis = false;
+    if (fcorv) {
+        for (int i = 0; i < M.nel; i++) {
+            t[i] = fcorv[i];
+            if (fcorv[i] != 0.f) M.has_coriolis = true;
+        }
+    }
+    if (upload_s(M, M.m.fcorv, t.data(), M.nel)) return 1;
+    std::fill(t.begin(), t.end(), 0.);
+    if (czv)
+        for (int i = 0; i < M.nel; i++) t[i] = czv[i];
+    return upload_s(M, M.m.czv, t.data(), M.nel);
+}
+
+int shyhip_tvd_init(shyhip_model *h) {
+    if (!h) return fail(nullptr, "null handle");
+    Model &M = h->M;
+    std::vector<int> tel, tkn;
+    std::vector<double> tw;
+    host_tvd_tables(M.nkn, M.nel, M.h_nen3v.data(), M.h_x.data(), M.h_y.data(), M.h_ev.data(), tel, tkn, tw);
+    dfree(M.m.tvd_el); dfree(M.m.tvd_kn); dfree(M.m.tvd_w);
+    int rc = 0;
+    rc |= dalloc(M, &M.m.tvd_el, tel.size()) || upload(M, M.m.tvd_el, tel.data(), tel.size());
+    rc |= dalloc(M, &M.m.tvd_kn, tkn.size()) || upload(M, M.m.tvd_kn, tkn.data(), tkn.size());
+    double *w = nullptr;
+    rc |= dalloc(M, &w, tw.size()) || upload(M, w, tw.data(), tw.size());
+    M.m.tvd_w = w;
+    M.tvd_ready = rc == 0;
+    return rc;
+}
+
+int shyhip_init_state(shyhip_model *h, float zconst) {
+    if (!h) return fail(nullptr, "null handle");
+    Model &M = h->M;
+    HIPCHK(M, hipMemsetAsync(M.d_err, 0, 8 * sizeof(int), M.stream));
+    for (void *q : {M.s.utn, M.s.vtn, M.s.uto, M.s.vto})
+        if (launch_fill(M, q, 0., (size_t)M.nlvdi * M.nel)) return 1;
+    if (launch_init_levels(M, zconst)) return 1;
+    // copy_depth; hydro_vertical (zero transports, equal depths -> w = 0);
+    // compute_velocities; copy_uvz; copy_depth
+    if (launch_copy_old(M)) return 1;            // exchange: old <- new
+    if (launch_setdepth(M, false)) return 1;     // new level again (old == new)
+    if (launch_hydro_vertical(M)) return 1;
+    if (launch_ttov_uvint(M)) return 1;
+    if (launch_copy_old(M)) return 1;
+    if (launch_setdepth(M, false)) return 1;
+    if (launch_hydro_vertical(M)) return 1;
+    if (launch_ttov_uvint(M)) return 1;
+    M.dtime = 0.;
+    M.dtime_old = 0.;
+    HIPCHK(M, hipMemcpyAsync(M.h_pinned, M.d_err, sizeof(int), hipMemcpyDeviceToHost, M.stream));
+    HIPCHK(M, hipStreamSynchronize(M.stream));
+    if (M.h_pinned[0]) return fail(&M, "initialisation failed (device error " + std::to_string(M.h_pinned[0]) + ")");
+    return 0;
+}
+
+int shyhip_push(shyhip_model *h, int field, const void *host, int dtype, size_t n) {
+    if (!h) return fail(nullptr, "null handle");
+    Model &M = h->M;
+    Field f;
+    if (!field_of(M, field, f)) return fail(&M, "unknown field id " + std::to_string(field));
+    const Shape sh = shape_of(M, f.kind);
+    if (n != sh.n) return fail(&M, "push: size mismatch for field " + std::to_string(field));
+    if (f.kind == K_NODE3 && field == SHYHIP_MFLUXV) M.has_mflux = true;
+    if (f.isint) {
+        if (dtype != SHYHIP_I32) return fail(&M, "integer field needs SHYHIP_I32");
+        return upload(M, (int *)*f.ptr, (const int *)host, n);
+    }
+    std::vector<double> t(n);
+    for (int r = 0; r < sh.rows; r++)
+        for (int c = 0; c < sh.cols; c++) {
+            const size_t hi = (size_t)c * sh.rows + r, di = (size_t)r * sh.cols + c;
+            t[di] = dtype == SHYHIP_F64 ? ((const double *)host)[hi] : (double)((const float *)host)[hi];
+        }
+    if (f.isdouble_dev) return upload(M, (double *)*f.ptr, t.data(), n);
+    return upload_s(M, *f.ptr, t.data(), n);
+}
+
+int shyhip_pull(shyhip_model *h, int field, void *host, int dtype, size_t n) {
+    if (!h) return fail(nullptr, "null handle");
+    Model &M = h->M;
+    Field f;
+    if (!field_of(M, field, f)) return fail(&M, "unknown field id " + std::to_string(field));
+    const Shape sh = shape_of(M, f.kind);
+    if (n != sh.n) return fail(&M, "pull: size mismatch for field " + std::to_string(field));
+    HIPCHK(M, hipStreamSynchronize(M.stream));
+    if (f.isint) {
+        if (dtype != SHYHIP_I32) return fail(&M, "integer field needs SHYHIP_I32");
+        HIPCHK(M, hipMemcpy(host, *f.ptr, n * sizeof(int), hipMemcpyDeviceToHost));
+        return 0;
+    }
+    std::vector<double> t(n);
+    if (f.isdouble_dev || M.prec == SHYHIP_FP64) {
+        HIPCHK(M, hipMemcpy(t.data(), *f.ptr, n * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> tf(n);
+        HIPCHK(M, hipMemcpy(tf.data(), *f.ptr, n * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; i++) t[i] = tf[i];
+    }
+    for (int r = 0; r < sh.rows; r++)
+        for (int c = 0; c < sh.cols; c++) {
+            const size_t hi = (size_t)c * sh.rows + r, di = (size_t)r * sh.cols + c;
+            if (dtype == SHYHIP_F64) ((double *)host)[hi] = t[di];
+            else ((float *)host)[hi] = (float)t[di];
+        }
+    return 0;
+}
+
+int shyhip_set_boundary_level(shyhip_model *h, int ibc, float level) {
+    if (!h) return fail(nullptr, "null handle");
+    return launch_set_rzv(h->M, ibc, level);
+}
+
+int shyhip_set_wind_uniform(shyhip_model *h, float taux, float tauy) {
+    if (!h) return fail(nullptr, "null handle");
+    Model &M = h->M;
+    if (launch_fill(M, M.s.taux, taux, M.nkn)) return 1;
+    return launch_fill(M, M.s.tauy, tauy, M.nkn);
+}
+
+int shyhip_set_dry(shyhip_model *h) {
+    if (!h) return fail(nullptr, "null handle");
+    Model &M = h->M;
+    const int bnew = M.dtime != M.dtime_old;
+    if (launch_set_dry(M, 2, bnew, nullptr)) return 1;
+    M.dtime_old = M.dtime;
+    if (launch_set_dry(M, 3, 0, nullptr)) return 1;
+    return 0;
+}
+
+int shyhip_copy_uvz(shyhip_model *h) {
+    if (!h) return fail(nullptr, "null handle");
+    return launch_copy_old(h->M);
+}
+
+int shyhip_hydro_transports(shyhip_model *h) { return h ? launch_hydro_intern(h->M) : fail(nullptr, "null handle"); }
+int shyhip_hydro_zeta(shyhip_model *h) { return h ? launch_zeta_assemble(h->M) : fail(nullptr, "null handle"); }
+int shyhip_hydro_transports_final(shyhip_model *h) { return h ? launch_transports_final(h->M) : fail(nullptr, "null handle"); }
+int shyhip_hydro_vertical(shyhip_model *h) { return h ? launch_hydro_vertical(h->M) : fail(nullptr, "null handle"); }
+
+int shyhip_system_solve_device(shyhip_model *h, int *iters) {
+    if (!h) return fail(nullptr, "null handle");
+    Model &M = h->M;
+    HIPCHK(M, hipMemsetAsync(M.sv.ctrl, 0, 8 * sizeof(int), M.stream));
+    int ref = 0;
+    const int rc = solver_solve(M, iters, &ref);
+    M.last_refine = ref;
+    return rc;
+}
+
+static int check_dev_err(Model &M, const char *where) {
+    HIPCHK(M, hipMemcpyAsync(M.h_pinned, M.d_err, sizeof(int), hipMemcpyDeviceToHost, M.stream));
+    HIPCHK(M, hipStreamSynchronize(M.stream));
+    const int e = M.h_pinned[0];
+    if (!e) return 0;
+    static const char *names[] = {"", "setnod: open boundary node out of the system", "", "setuvd: z or u/v computation",
+                                  "setdepth: non-positive layer thickness", "check_volume: negative volume", "",
+                                  "conz3d: zero diagonal in a column", "penta: singular vertical column"};
+    return fail(&M, std::string(where) + ": " + (e >= 0 && e <= 8 ? names[e] : "device error"), 3);
+}
+
+int shyhip_hydro(shyhip_model *h, double dtime) {
+    if (!h) return fail(nullptr, "null handle");
+    Model &M = h->M;
+    M.dtime = dtime;
+    HIPCHK(M, hipMemsetAsync(M.d_err, 0, sizeof(int), M.stream));
+    int iloop = 0, changed = 0, its_total = 0;
+    if (launch_hydro_intern(M)) return 1;          // hydro_transports (invariant over the wet/dry loop)
+    for (;;) {
+        iloop++;
+        if (iloop > 100) return fail(&M, "hydro: wet/dry loop did not settle", 3);
+        if (launch_setnod(M)) return 1;             // setnod + adjust_mass_flux
+        if (launch_zeta_assemble(M)) return 1;      // system_init + hydro_zeta + system_add_rhs
+        int its = 0;
+        const int rc = shyhip_system_solve_device(h, &its);
+        its_total += its;
+        if (rc) return rc;
+        const int bnew = M.dtime != M.dtime_old;
+        if (launch_system_get_setweg1(M, bnew, &changed)) return 1;
+        M.dtime_old = M.dtime;
+        if (changed == 0) break;
+    }
+    M.last_nloop = iloop;
+    M.last_iters = its_total;
+    if (check_dev_err(M, "hydro loop")) return 3;
+    if (launch_transports_final(M)) return 1;
+    if (launch_setzev_setuvd(M)) return 1;          // setzev + setuvd + baro2l
+    if (launch_setdepth(M, false)) return 1;        // make_new_depth (+ check_volume)
+    if (launch_hydro_vertical(M)) return 1;
+    if (launch_ttov_uvint(M)) return 1;             // compute_velocities: ulnv/vlnv, unv/vnv
+    return check_dev_err(M, "hydro");
+}
+
+int shyhip_scal3sh(shyhip_model *h, int itr, int istot, int clip_small) {
+    if (!h) return fail(nullptr, "null handle");
+    Model &M = h->M;
+    if (itr < 0 || itr >= M.ntr) return fail(&M, "tracer index out of range");
+    if (istot < 1 || istot > 1000) return fail(&M, "istot must be in 1..1000");
+    if (M.p.itvd == 2 && !M.tvd_ready) return fail(&M, "itvd = 2 needs shyhip_tvd_init");
+    for (int isact = 1; isact <= istot; isact++)
+        if (launch_conz3d(M, itr, istot, isact, clip_small)) return 1;
+    return 0;
+}
+
+int shyhip_step(shyhip_model *h, double dtime, int istot) {
+    if (!h) return fail(nullptr, "null handle");
+    Model &M = h->M;
+    if (shyhip_set_dry(h)) return 1;
+    if (launch_copy_old(M)) return 1;
+    if (shyhip_hydro(h, dtime)) return M.err.empty() ? 1 : 3;
+    for (int t = 0; t < M.ntr; t++)
+        if (shyhip_scal3sh(h, t, istot, t == 0 ? 0 : 1)) return 1;
+    return 0;
+}
+
+int shyhip_sync(shyhip_model *h) {
+    if (!h) return fail(nullptr, "null handle");
+    HIPCHK(h->M, hipStreamSynchronize(h->M.stream));
+    return check_dev_err(h->M, "sync");
+}
+
+int shyhip_volume(shyhip_model *h, int mode, double *vol) {
+    if (!h) return fail(nullptr, "null handle");
+    return launch_volume(h->M, mode, vol);
+}
+
+int shyhip_tracer_mass(shyhip_model *h, int itr, double *mass) {
+    if (!h) return fail(nullptr, "null handle");
+    if (itr < 0 || itr >= h->M.ntr) return fail(&h->M, "tracer index out of range");
+    return launch_tracer_mass(h->M, itr, mass);
+}
+
+int shyhip_stats(shyhip_model *h, int *nloop, int *solver_iters, int *refinements) {
+    if (!h) return fail(nullptr, "null handle");
+    if (nloop) *nloop = h->M.last_nloop;
+    if (solver_iters) *solver_iters = h->M.last_iters;
+    if (refinements) *refinements = h->M.last_refine;
+    return 0;
+}
+
+int shyhip_get_pattern(shyhip_model *h, int *nnz, int *ntg, int *cols, int *ncolor, int *color) {
+    if (!h) return fail(nullptr, "null handle");
+    Model &M = h->M;
+    if (nnz) *nnz = M.nnz;
+    if (ncolor) *ncolor = M.ncolor;
+    if (ntg) std::copy(M.h_rowptr.begin(), M.h_rowptr.end(), ntg);
+    if (cols) std::copy(M.h_cols.begin(), M.h_cols.end(), cols);
+    if (color) std::copy(M.h_color.begin(), M.h_color.end(), color);
+    return 0;
+}
+
+// ---- zeta-system plugin (simsys_spkaux.f90 interface) ----------------------
+int shyhip_system_init(shyhip_model *h) {
+    if (!h) return fail(nullptr, "null handle");
+    Model &M = h->M;
+    M.h_coo.assign(M.nnz, 0.);
+    M.h_rvec.assign(M.nkn, 0.);
+    return 0;
+}
+
+int shyhip_system_assemble(shyhip_model *h, int ie, const int *kn, const float *hia, const float *hik) {
+    if (!h) return fail(nullptr, "null handle");
+    Model &M = h->M;
+    if (ie < 1 || ie > M.nel) return fail(&M, "system_assemble: element out of range");
+    if (M.h_coo.size() != (size_t)M.nnz) return fail(&M, "system_assemble before system_init");
+    const int e = ie - 1;
+    for (int i = 0; i < 3; i++) {
+        const int k = kn[i] - 1;
+        if (k != M.h_nen3v[3 * e + i]) return fail(&M, "system_assemble: kn does not match nen3v");
+        for (int j = 0; j < 3; j++) {
+            const int slot = M.h_ijp[9 * (size_t)e + 3 * i + j];
+            M.h_coo[slot] = M.h_coo[slot] + (double)hia[i + 3 * j];   // hia(i,j), column-major
+        }
+        M.h_rvec[k] = M.h_rvec[k] + (double)hik[i];
+    }
+    return 0;
+}
+
+int shyhip_system_add_rhs(shyhip_model *h, float dt, int n, const float *array) {
+    if (!h) return fail(nullptr, "null handle");
+    Model &M = h->M;
+    if (n != M.nkn || M.h_rvec.size() != (size_t)n) return fail(&M, "system_add_rhs: size mismatch");
+    for (int k = 0; k < n; k++) M.h_rvec[k] = M.h_rvec[k] + (double)(dt * array[k]);
+    return 0;
+}
+
+int shyhip_system_solve(shyhip_model *h, int n, const float *z) {
+    if (!h) return fail(nullptr, "null handle");
+    Model &M = h->M;
+    if (n != M.nkn || M.h_coo.size() != (size_t)M.nnz) return fail(&M, "system_solve: size mismatch or no assembly");
+    if (upload(M, M.s.aval, M.h_coo.data(), M.nnz) || upload(M, M.s.rhs, M.h_rvec.data(), n)) return 1;
+    std::vector<double> zz(z, z + n);
+    if (upload_s(M, M.s.zn, zz.data(), n)) return 1;     // initial guess
+    int its = 0;
+    return shyhip_system_solve_device(h, &its);
+}
+
+int shyhip_system_get(shyhip_model *h, int n, float *z) {
+    if (!h) return fail(nullptr, "null handle");
+    Model &M = h->M;
+    if (n != M.nkn) return fail(&M, "system_get: size mismatch");
+    std::vector<double> x(n);
+    HIPCHK(M, hipStreamSynchronize(M.stream));
+    HIPCHK(M, hipMemcpy(x.data(), M.s.x, n * sizeof(double), hipMemcpyDeviceToHost));
+    for (int k = 0; k < n; k++) z[k] = (float)x[k];
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,1161 @@
+// shyhip_hydro.hip -- gfx950 kernels of the semi-implicit hydro step.
+//
+// Every kernel is one work-item per element column or per node column, with
+// layer-major arrays so that a wavefront reads 64 consecutive elements of one
+// layer.  Element -> node accumulations are gathers over the node's incident
+// elements in ascending element order: atomic-free, deterministic, and the
+// same summation order as the reference's sequential element loops.
+// Arithmetic order and operand types follow the reference (`real` locals are
+// the storage type S, double precision locals are double), and the build
+// disables FP contraction, so per-kernel results reproduce the CPU path.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "shyhip_internal.h"
+
+namespace shy {
+
+template <typename S>
+struct HydroArgs {
+    int nkn, nel, nlvdi, nlv;
+    double dt;
+    float az, am, af, at, grav, rowass, vismol, hzmin, hzoff, hzon, volmin, uvmin;
+    int ireib, iclin;
+    const int *nen3v;
+    const double *geo, *ang;
+    const int *ilhv, *ilhkv;
+    const S *hm3v, *hev, *hlv, *hldv, *areakv, *fcorv, *czv;
+    const int *ibnd, *ibtyp, *zbound;
+    const int *nk_ptr, *nk_el;
+    const int *rowptr, *ijp;
+    S *zn, *zo, *zen, *zeo;
+    S *utn, *vtn, *uto, *vto, *uln, *vln;
+    S *un, *vn, *uo, *vo;
+    S *wln;
+    S *hdkn, *hdko, *hden, *hdeo;
+    int *iweg, *iwet, *inod;
+    S *rfric, *ddx, *ddy, *visv;
+    S *taux, *tauy, *rzv, *rqv, *mflux;
+    double *zel, *aval, *rhs, *x;
+    int *err;
+};
+
+template <typename S>
+static HydroArgs<S> make_args(Model &M) {
+    HydroArgs<S> a{};
+    a.nkn = M.nkn; a.nel = M.nel; a.nlvdi = M.nlvdi; a.nlv = M.nlv;
+    a.dt = M.p.dt;
+    a.az = M.p.azpar; a.am = M.p.ampar; a.af = M.p.afpar; a.at = M.p.atpar;
+    a.grav = M.p.grav; a.rowass = M.p.rowass; a.vismol = M.p.vismol;
+    a.hzmin = M.p.hzmin; a.hzoff = M.p.hzoff; a.hzon = M.p.hzon; a.volmin = M.p.volmin; a.uvmin = M.p.uvmin;
+    a.ireib = M.p.ireib; a.iclin = M.p.iclin;
+    a.nen3v = M.m.nen3v; a.geo = M.m.geo; a.ang = M.m.ang;
+    a.ilhv = M.m.ilhv; a.ilhkv = M.m.ilhkv;
+    a.hm3v = (const S *)M.m.hm3v; a.hev = (const S *)M.m.hev; a.hlv = (const S *)M.m.hlv;
+    a.hldv = (const S *)M.m.hldv; a.areakv = (const S *)M.m.areakv;
+    a.fcorv = (const S *)M.m.fcorv; a.czv = (const S *)M.m.czv;
+    a.ibnd = M.m.ibnd; a.ibtyp = M.m.ibtyp; a.zbound = M.m.zbound;
+    a.nk_ptr = M.m.nk_ptr; a.nk_el = M.m.nk_el; a.rowptr = M.m.rowptr; a.ijp = M.m.ijp;
+    a.zn = (S *)M.s.zn; a.zo = (S *)M.s.zo; a.zen = (S *)M.s.zen; a.zeo = (S *)M.s.zeo;
+    a.utn = (S *)M.s.utn; a.vtn = (S *)M.s.vtn; a.uto = (S *)M.s.uto; a.vto = (S *)M.s.vto;
+    a.uln = (S *)M.s.uln; a.vln = (S *)M.s.vln;
+    a.un = (S *)M.s.un; a.vn = (S *)M.s.vn; a.uo = (S *)M.s.uo; a.vo = (S *)M.s.vo;
+    a.wln = (S *)M.s.wln;
+    a.hdkn = (S *)M.s.hdkn; a.hdko = (S *)M.s.hdko; a.hden = (S *)M.s.hden; a.hdeo = (S *)M.s.hdeo;
+    a.iweg = M.s.iweg; a.iwet = M.s.iwet; a.inod = M.s.inod;
+    a.rfric = (S *)M.s.rfric; a.ddx = (S *)M.s.ddx; a.ddy = (S *)M.s.ddy; a.visv = (S *)M.s.visv;
+    a.taux = (S *)M.s.taux; a.tauy = (S *)M.s.tauy; a.rzv = (S *)M.s.rzv; a.rqv = (S *)M.s.rqv;
+    a.mflux = M.has_mflux ? (S *)M.s.mflux : nullptr;
+    a.zel = M.s.zel; a.aval = M.s.aval; a.rhs = M.s.rhs; a.x = M.s.x;
+    a.err = M.d_err;
+    return a;
+}
+
+static inline int nblocks(int n) { return (n + kThreads - 1) / kThreads; }
+
+// ------------------------------------------------------------------ friction
+// bottom_friction, chezy.f90:239-398 (formulations without roughness length)
+template <typename S>
+__device__ inline S friction(const HydroArgs<S> &a, int ie, int lmax) {
+    using R = S;
+    const int nel = a.nel;
+    const R uso = a.uto[(size_t)(lmax - 1) * nel + ie], vso = a.vto[(size_t)(lmax - 1) * nel + ie];
+    const R uv = sqrt(uso * uso + vso * vso);
+    R hzg = a.hdeo[(size_t)(lmax - 1) * nel + ie];
+    if (hzg < (R)a.hzoff) hzg = (R)a.hzoff;
+    const R rfric = a.czv[ie];
+    const R grav = (R)a.grav;
+    R rr = 0, rcd = 0;
+    switch (a.ireib) {
+    case 1: rr = rfric; break;
+    case 2: rcd = grav / ((rfric * rfric) * (R)pow((double)hzg, (double)((R)1 / (R)3))); rr = rcd * uv / (hzg * hzg); break;
+    case 3: rcd = grav / (rfric * rfric); rr = rcd * uv / (hzg * hzg); break;
+    case 4: rr = rfric / hzg; break;
+    case 5: rcd = rfric; rr = rcd * uv / (hzg * hzg); break;
+    case 7:
+        rcd = (rfric >= (R)1) ? grav / ((rfric * rfric) * (R)pow((double)hzg, (double)((R)1 / (R)3))) : rfric;
+        rr = rcd * uv / (hzg * hzg);
+        break;
+    case 10:
+        if (uv > (R)a.uvmin * hzg) { rcd = rfric; rr = rcd * uv / (hzg * hzg); }
+        else rr = (rfric * (R)a.uvmin) / hzg;
+        break;
+    default: rr = 0; break;
+    }
+    return rr;
+}
+
+// ------------------------------------------------------------------ hydro_intern
+// hydro_transports / hydro_intern, hydrodynamic.f90:758-1416.  One work-item
+// per element column.  Without Coriolis (gamma = 0) the 2L x 2L pentadiagonal
+// system decouples into two identical tridiagonal systems in u and v; the
+// tridiagonal elimination below performs exactly the operations that
+// penta_fact/penta_solve (penta.f90:104-321) perform on the decoupled bands.
+// The coupled case runs the pentadiagonal elimination itself.
+template <typename S, int NL, bool CORI>
+__global__ __launch_bounds__(kThreads) void k_hydro_intern(HydroArgs<S> a) {
+    using R = S;
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ie >= a.nel) return;
+    const int nel = a.nel, nkn = a.nkn;
+    const R dt = (R)a.dt, af = (R)a.af, at = (R)a.at, grav = (R)a.grav, rowass = (R)a.rowass;
+    const int ilevel = a.ilhv[ie];
+    const int k1 = a.nen3v[3 * ie], k2 = a.nen3v[3 * ie + 1], k3 = a.nen3v[3 * ie + 2];
+    const int kn[3] = {k1, k2, k3};
+    double b[3], c[3];
+    double bz = 0., cz = 0., zmm = 0., taux = 0., tauy = 0.;
+    for (int ii = 0; ii < 3; ii++) {
+        b[ii] = a.geo[(size_t)ii * nel + ie];
+        c[ii] = a.geo[(size_t)(3 + ii) * nel + ie];
+        const R ze = a.zeo[(size_t)ii * nel + ie];
+        const double zz = (double)ze;
+        zmm += (double)ze;
+        bz += zz * b[ii];
+        cz += zz * c[ii];
+        taux += (double)a.taux[kn[ii]];
+        tauy += (double)a.tauy[kn[ii]];
+    }
+    zmm = zmm * kDrittl;
+    taux = taux * kDrittl;
+    tauy = tauy * kDrittl;
+    const R gammat = a.fcorv[ie];
+    const R gamma = af * dt * gammat;
+    const R rfric = friction(a, ie, ilevel);
+    a.rfric[ie] = rfric;
+
+    // layer thickness, inverse thickness, element viscosity
+    R hact[NL + 2], rhact[NL + 2], alev[NL + 1];
+    hact[0] = 0;
+#pragma unroll
+    for (int l = 1; l <= NL + 1; l++) hact[l] = (l <= ilevel) ? a.hdeo[(size_t)(l - 1) * nel + ie] : (R)0;
+    if (a.iclin) hact[1] = (R)(hact[1] - zmm);
+#pragma unroll
+    for (int l = 0; l <= NL + 1; l++) rhact[l] = (hact[l] <= (R)0) ? (R)0 : (R)1 / hact[l];
+#pragma unroll
+    for (int l = 0; l <= NL; l++) {
+        if (l <= ilevel) {
+            double vis = (double)a.vismol;
+            const R sum = a.visv[(size_t)l * nkn + k1] + a.visv[(size_t)l * nkn + k2] + a.visv[(size_t)l * nkn + k3];
+            vis = vis + (double)(sum / (R)3);
+            alev[l] = (R)vis;
+        } else {
+            alev[l] = 0;
+        }
+    }
+
+    double sxu[NL], sxv[NL], sh[NL];      // solutions: -F^x, -F^y and H rows
+    if constexpr (!CORI) {
+        double xd[NL], up[NL], gx[NL], gy[NL], gh[NL];
+#pragma unroll
+        for (int l = 1; l <= NL; l++) {
+            if (l > ilevel) break;
+            const bool bfirst = (l == 1), blast = (l == ilevel);
+            const int lp = (l + 1 < ilevel) ? l + 1 : ilevel;
+            const int lm = (l - 1 > 1) ? l - 1 : 1;
+            const R uui = a.uto[(size_t)(l - 1) * nel + ie], uuip = a.uto[(size_t)(lp - 1) * nel + ie];
+            const R uuim = a.uto[(size_t)(lm - 1) * nel + ie];
+            const R vvi = a.vto[(size_t)(l - 1) * nel + ie], vvip = a.vto[(size_t)(lp - 1) * nel + ie];
+            const R vvim = a.vto[(size_t)(lm - 1) * nel + ie];
+            const R hhi = hact[l], hhip = hact[l + 1], hhim = hact[l - 1];
+            R rhp = 0, rhm = 0;
+            if (hhi > (R)0) {
+                if (hhip > (R)0) rhp = (R)2.0 * alev[l] / (hhi + hhip);
+                if (hhim > (R)0) rhm = (R)2.0 * alev[l - 1] / (hhi + hhim);
+            }
+            const R aux = dt * at;
+            R aa = aux * rhact[l] * (rhm + rhp);
+            R aat = (R)1 * rhact[l] * (rhm + rhp);
+            const R bb = aux * rhact[l + 1] * rhp;
+            const R bbt = (R)1 * rhact[l + 1] * rhp;
+            const R cc = aux * rhact[l - 1] * rhm;
+            const R cct = (R)1 * rhact[l - 1] * rhm;
+            double ppx = 0., ppy = 0.;
+            if (bfirst) { ppx = ppx - taux; ppy = ppy - tauy; }
+            if (blast) { aa = aa + dt * rfric; aat = aat + rfric; }
+            const R gravx = (R)(1.0 * grav * hhi * bz);
+            const R gravy = (R)(1.0 * grav * hhi * cz);
+            const R presx = 0, presy = 0, xexpl = 0, yexpl = 0;
+            const R ppx_aux = aat * uui - bbt * uuip - cct * uuim - gammat * vvi + gravx + (hhi / rowass) * presx + xexpl;
+            const R ppy_aux = aat * vvi - bbt * vvip - cct * vvim + gammat * uui + gravy + (hhi / rowass) * presy + yexpl;
+            ppx = ppx + (double)ppx_aux;
+            ppy = ppy + (double)ppy_aux;
+            const double d = (double)((R)1 + aa);
+            up[l - 1] = blast ? 0.0 : (double)(-bb);
+            const double lo = bfirst ? 0.0 : (double)(-cc);
+            if (l == 1) {
+                xd[0] = d;
+                gx[0] = ppx;
+                gy[0] = ppy;
+                gh[0] = (double)hhi;
+            } else {
+                const double w = lo / xd[l - 2];
+                xd[l - 1] = d - w * up[l - 2];
+                gx[l - 1] = ppx - w * gx[l - 2];
+                gy[l - 1] = ppy - w * gy[l - 2];
+                gh[l - 1] = (double)hhi - w * gh[l - 2];
+            }
+        }
+        double su = 0., sv = 0., shh = 0.;
+#pragma unroll
+        for (int l = NL; l >= 1; l--) {
+            if (l > ilevel) continue;
+            if (l == ilevel) {
+                su = gx[l - 1] / xd[l - 1];
+                sv = gy[l - 1] / xd[l - 1];
+                shh = gh[l - 1] / xd[l - 1];
+            } else {
+                su = (gx[l - 1] - up[l - 1] * su) / xd[l - 1];
+                sv = (gy[l - 1] - up[l - 1] * sv) / xd[l - 1];
+                shh = (gh[l - 1] - up[l - 1] * shh) / xd[l - 1];
+            }
+            sxu[l - 1] = su;
+            sxv[l - 1] = sv;
+            sh[l - 1] = shh;
+        }
+    } else {
+        // coupled u/v columns: pentadiagonal elimination of penta.f90
+        constexpr int N2 = 2 * NL;
+        double vm2[N2], vm1[N2], v0[N2], vp1[N2], vp2[N2], r0[N2], r1[N2], r2[N2];
+#pragma unroll
+        for (int i = 0; i < N2; i++) { vm2[i] = vm1[i] = v0[i] = vp1[i] = vp2[i] = 0.; r0[i] = r1[i] = r2[i] = 0.; }
+        const int n = 2 * ilevel;
+        for (int l = 1; l <= ilevel; l++) {
+            const bool bfirst = (l == 1), blast = (l == ilevel);
+            const int lp = (l + 1 < ilevel) ? l + 1 : ilevel;
+            const int lm = (l - 1 > 1) ? l - 1 : 1;
+            const R uui = a.uto[(size_t)(l - 1) * nel + ie], uuip = a.uto[(size_t)(lp - 1) * nel + ie];
+            const R uuim = a.uto[(size_t)(lm - 1) * nel + ie];
+            const R vvi = a.vto[(size_t)(l - 1) * nel + ie], vvip = a.vto[(size_t)(lp - 1) * nel + ie];
+            const R vvim = a.vto[(size_t)(lm - 1) * nel + ie];
+            const R hhi = hact[l], hhip = hact[l + 1], hhim = hact[l - 1];
+            R rhp = 0, rhm = 0;
+            if (hhi > (R)0) {
+                if (hhip > (R)0) rhp = (R)2.0 * alev[l] / (hhi + hhip);
+                if (hhim > (R)0) rhm = (R)2.0 * alev[l - 1] / (hhi + hhim);
+            }
+            const R aux = dt * at;
+            R aa = aux * rhact[l] * (rhm + rhp);
+            R aat = (R)1 * rhact[l] * (rhm + rhp);
+            const R bb = aux * rhact[l + 1] * rhp;
+            const R bbt = (R)1 * rhact[l + 1] * rhp;
+            const R cc = aux * rhact[l - 1] * rhm;
+            const R cct = (R)1 * rhact[l - 1] * rhm;
+            double ppx = 0., ppy = 0.;
+            if (bfirst) { ppx = ppx - taux; ppy = ppy - tauy; }
+            if (blast) { aa = aa + dt * rfric; aat = aat + rfric; }
+            const R gravx = (R)(1.0 * grav * hhi * bz);
+            const R gravy = (R)(1.0 * grav * hhi * cz);
+            const R ppx_aux = aat * uui - bbt * uuip - cct * uuim - gammat * vvi + gravx;
+            const R ppy_aux = aat * vvi - bbt * vvip - cct * vvim + gammat * uui + gravy;
+            ppx = ppx + (double)ppx_aux;
+            ppy = ppy + (double)ppy_aux;
+            const int ju = 2 * l - 2, jv = 2 * l - 1;
+            v0[ju] = (double)((R)1 + aa);
+            v0[jv] = (double)((R)1 + aa);
+            vm1[jv] = (double)gamma;
+            vp1[ju] = (double)(-gamma);
+            if (!blast) { vp2[ju] = -bb; vp2[jv] = -bb; }
+            if (!bfirst) { vm2[ju] = -cc; vm2[jv] = -cc; }
+            r0[ju] = ppx; r0[jv] = ppy;
+            r1[ju] = (double)hhi; r2[jv] = (double)hhi;
+        }
+        double xx[N2], yy[N2], zz[N2 + 1];
+        int bad = 0;
+        zz[0] = 0.;
+        for (int i = 0; i < n; i++) {
+            double t = v0[i];
+            if (i >= 1) t = t - yy[i - 1] * zz[i];
+            if (i >= 2) t = t - vm2[i] * vp2[i - 2];
+            if (t == 0.) bad = 1;
+            xx[i] = t;
+            yy[i] = (i < n - 1) ? vp1[i] - ((i >= 1) ? zz[i] * vp2[i - 1] : 0.) : 0.;
+            if (i == 0) yy[0] = vp1[0];
+            if (i + 1 < n) zz[i + 1] = (vm1[i + 1] - ((i >= 1) ? vm2[i + 1] * yy[i - 1] : 0.)) / t;
+            if (i + 2 < n) vm2[i + 2] = vm2[i + 2] / t;
+        }
+        if (bad) atomicCAS(a.err, 0, ERR_PENTA);
+        double g0[N2], g1[N2], g2[N2];
+        for (int i = 0; i < n; i++) {
+            g0[i] = r0[i]; g1[i] = r1[i]; g2[i] = r2[i];
+            if (i >= 1) { g0[i] -= zz[i] * g0[i - 1]; g1[i] -= zz[i] * g1[i - 1]; g2[i] -= zz[i] * g2[i - 1]; }
+            if (i >= 2) { g0[i] -= vm2[i] * g0[i - 2]; g1[i] -= vm2[i] * g1[i - 2]; g2[i] -= vm2[i] * g2[i - 2]; }
+        }
+        double s0[N2], s1[N2], s2[N2];
+        for (int i = n - 1; i >= 0; i--) {
+            double t0 = g0[i], t1 = g1[i], t2 = g2[i];
+            if (i + 1 < n) { t0 -= yy[i] * s0[i + 1]; t1 -= yy[i] * s1[i + 1]; t2 -= yy[i] * s2[i + 1]; }
+            if (i + 2 < n) { t0 -= vp2[i] * s0[i + 2]; t1 -= vp2[i] * s1[i + 2]; t2 -= vp2[i] * s2[i + 2]; }
+            s0[i] = t0 / xx[i]; s1[i] = t1 / xx[i]; s2[i] = t2 / xx[i];
+        }
+        // store the coupled result into the same per-layer slots
+        double dbb = 0., dbc = 0., dcb = 0., dcc = 0.;
+        for (int l = 1; l <= ilevel; l++) {
+            const int ju = 2 * l - 2, jv = 2 * l - 1;
+            sxu[l - 1] = s0[ju];
+            sxv[l - 1] = s0[jv];
+            a.ddx[(size_t)ju * nel + ie] = (S)s1[ju];
+            a.ddx[(size_t)jv * nel + ie] = (S)s1[jv];
+            a.ddy[(size_t)ju * nel + ie] = (S)s2[ju];
+            a.ddy[(size_t)jv * nel + ie] = (S)s2[jv];
+            dbb += s1[ju]; dbc += s2[ju]; dcb += s1[jv]; dcc += s2[jv];
+        }
+        a.zel[(size_t)0 * nel + ie] = dbb;
+        a.zel[(size_t)1 * nel + ie] = dbc;
+        a.zel[(size_t)2 * nel + ie] = dcb;
+        a.zel[(size_t)3 * nel + ie] = dcc;
+    }
+    // u^hat = u_old - dt * A^-1 F ; store A^-1 H
+    const R dtafix = dt * (R)1;
+    double uold = 0., vold = 0., uhat = 0., vhat = 0.;
+    double dsum = 0.;
+#pragma unroll
+    for (int l = 1; l <= NL; l++) {
+        if (l > ilevel) break;
+        const R uo = a.uto[(size_t)(l - 1) * nel + ie], vo = a.vto[(size_t)(l - 1) * nel + ie];
+        const S unew = (S)((double)uo - (double)dtafix * sxu[l - 1]);
+        const S vnew = (S)((double)vo - (double)dtafix * sxv[l - 1]);
+        a.utn[(size_t)(l - 1) * nel + ie] = unew;
+        a.vtn[(size_t)(l - 1) * nel + ie] = vnew;
+        uold += (double)uo; vold += (double)vo;
+        uhat += (double)unew; vhat += (double)vnew;
+        if constexpr (!CORI) {
+            const int ju = 2 * l - 2, jv = 2 * l - 1;
+            a.ddx[(size_t)ju * nel + ie] = (S)sh[l - 1];
+            a.ddx[(size_t)jv * nel + ie] = (S)0;
+            a.ddy[(size_t)ju * nel + ie] = (S)0;
+            a.ddy[(size_t)jv * nel + ie] = (S)sh[l - 1];
+            dsum += sh[l - 1];
+        }
+    }
+    if constexpr (!CORI) {
+        a.zel[(size_t)0 * nel + ie] = dsum;
+        a.zel[(size_t)1 * nel + ie] = 0.;
+        a.zel[(size_t)2 * nel + ie] = 0.;
+        a.zel[(size_t)3 * nel + ie] = dsum;
+    }
+    const double az = (double)a.az;
+    a.zel[(size_t)4 * nel + ie] = az * uhat + (1. - az) * uold;
+    a.zel[(size_t)5 * nel + ie] = az * vhat + (1. - az) * vold;
+}
+
+// ------------------------------------------------------------------ hydro_zeta
+// element matrix of hydro_zeta (hydrodynamic.f90:600-728) for element ie,
+// returned row i (after level boundary conditions and the excluded-area rule)
+template <typename S>
+__device__ inline void zeta_row(const HydroArgs<S> &a, int ie, int i, S hia_row[3], S &hik_i) {
+    using R = S;
+    const int nel = a.nel;
+    const double dt = (double)(R)a.dt;
+    const double az = a.az, am = a.am;
+    const double delta = dt * dt * az * am * (double)a.grav * 1;
+    const double aj = a.geo[(size_t)6 * nel + ie];
+    double b[3], c[3], z[3];
+    int kn[3];
+    for (int q = 0; q < 3; q++) {
+        kn[q] = a.nen3v[3 * ie + q];
+        b[q] = a.geo[(size_t)q * nel + ie];
+        c[q] = a.geo[(size_t)(3 + q) * nel + ie];
+        z[q] = (double)a.zeo[(size_t)q * nel + ie];
+    }
+    const double dbb = a.zel[ie], dbc = a.zel[(size_t)nel + ie], dcb = a.zel[(size_t)2 * nel + ie];
+    const double dcc = a.zel[(size_t)3 * nel + ie], ut = a.zel[(size_t)4 * nel + ie], vt = a.zel[(size_t)5 * nel + ie];
+    R hia[3][3], hik[3];
+    for (int n = 0; n < 3; n++) {
+        for (int m = 0; m < 3; m++) {
+            const double abn = b[n] * (b[m] * dbb + c[m] * dbc);
+            const double acn = c[n] * (b[m] * dcb + c[m] * dcc);
+            const double h11 = delta * (abn + acn);
+            hia[n][m] = (R)(aj * ((n == m ? 4. : 0.) + 12. * h11));
+        }
+        const double acu = (double)hia[n][0] * z[0] + (double)hia[n][1] * z[1] + (double)hia[n][2] * z[2];
+        const double andg = 4. * aj * dt * 0.;
+        hik[n] = (R)(acu + andg + 12. * aj * dt * (ut * b[n] + vt * c[n]));
+    }
+    for (int q = 0; q < 3; q++) {
+        const R rz = a.rzv[kn[q]];
+        if (rz != (R)kFlag) {
+            const double rw = (double)rz;
+            const int j1 = (q + 1) % 3, j2 = (q + 2) % 3;
+            hik[j1] = (R)((double)hik[j1] - rw * (double)hia[j1][q]);
+            hik[j2] = (R)((double)hik[j2] - rw * (double)hia[j2][q]);
+            hia[q][j1] = 0; hia[q][j2] = 0;
+            hia[j1][q] = 0; hia[j2][q] = 0;
+            hik[q] = (R)(rw * (double)hia[q][q]);
+        }
+    }
+    if (a.iweg[ie] != 0) {
+        const double hh999 = aj * 12.;
+        for (int n = 0; n < 3; n++) {
+            for (int m = 0; m < 3; m++) hia[n][m] = (R)(hh999 * (b[n] * b[m] + c[n] * c[m]));
+            hik[n] = 0;
+        }
+        for (int n = 0; n < 3; n++) {
+            const int iv = a.inod[kn[n]];
+            if (iv != 0 && iv != -2) {
+                for (int m = 0; m < 3; m++) hia[n][m] = 0;
+                hik[n] = 0;
+            }
+        }
+    }
+    for (int m = 0; m < 3; m++) hia_row[m] = hia[i][m];
+    hik_i = hik[i];
+}
+
+// system_init + hydro_zeta + system_assemble + system_add_rhs as a row
+// gather: row k sums its incident elements' rows in element order, the
+// order of the reference's scatter into c2coo / rvec2d.
+template <typename S>
+__global__ __launch_bounds__(kThreads) void k_zeta_rows(HydroArgs<S> a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nkn) return;
+    const int r0 = a.rowptr[k], r1 = a.rowptr[k + 1];
+    for (int q = r0; q < r1; q++) a.aval[q] = 0.;
+    double rhs = 0.;
+    for (int q = a.nk_ptr[k]; q < a.nk_ptr[k + 1]; q++) {
+        const int pk = a.nk_el[q];
+        const int ie = pk >> 2, i = pk & 3;
+        S row[3], hik;
+        zeta_row(a, ie, i, row, hik);
+        for (int j = 0; j < 3; j++) {
+            const int slot = a.ijp[9 * ie + 3 * i + j];
+            a.aval[slot] = a.aval[slot] + (double)row[j];
+        }
+        rhs = rhs + (double)hik;
+    }
+    const S dt = (S)a.dt;
+    rhs = rhs + (double)(dt * a.rqv[k]);
+    a.rhs[k] = rhs;
+}
+
+// ------------------------------------------------------------------ wet/dry
+// setweg(iweich), wetdry.f90:128-347, one work-item per element; the number
+// of elements that changed state is accumulated in *cnt.
+template <typename S>
+__global__ __launch_bounds__(kThreads) void k_setweg(HydroArgs<S> a, int iweich, int bnewtime, int *cnt) {
+    using R = S;
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ie >= a.nel) return;
+    const int nel = a.nel;
+    int iwh = 0;
+    int weg = a.iweg[ie], wet = a.iwet[ie];
+    const int kn[3] = {a.nen3v[3 * ie], a.nen3v[3 * ie + 1], a.nen3v[3 * ie + 2]};
+    if (iweich == -1) {
+        weg = 0;
+        wet = 0;
+    } else if (iweich == 0) {
+        const R aomega = (R)a.geo[(size_t)6 * nel + ie];
+        R zmin = (R)a.hzmin + (R)a.volmin / ((R)4 * aomega);
+        if (zmin > (R)a.hzoff) zmin = (R)a.hzmin + (R)0.5 * ((R)a.hzmin + (R)a.hzoff);
+        int iweg = 0;
+        for (int ii = 0; ii < 3; ii++) {
+            const R hm = a.hm3v[(size_t)ii * nel + ie];
+            const R hzg = a.zen[(size_t)ii * nel + ie] + hm;
+            if (hzg < (R)a.hzoff) iweg++;
+            if (hzg < zmin) a.zen[(size_t)ii * nel + ie] = zmin - hm;
+        }
+        weg = iweg;
+        if (iweg == 0) wet = 1;
+        if (iweg > 0) iwh++;
+    } else if (iweich == 1 || iweich == 2) {
+        const R hzlim = (iweich == 1) ? (R)a.hzmin : (R)a.hzoff;
+        if (weg == 0) {
+            int iweg = 0;
+            for (int ii = 0; ii < 3; ii++) {
+                const R hzg = a.zn[kn[ii]] + a.hm3v[(size_t)ii * nel + ie];
+                if (hzg < hzlim) iweg++;
+            }
+            if (iweg > 0) {
+                if (weg == 0) { iwh++; wet = 0; }
+                weg = iweg;
+            }
+        }
+    } else if (iweich == 3) {
+        if (weg > 0) {
+            int iweg = 0;
+            R hztot = 0;
+            for (int ii = 0; ii < 3; ii++) {
+                const R hzg = a.zn[kn[ii]] + a.hm3v[(size_t)ii * nel + ie];
+                hztot = hztot + hzg;
+                if (hzg < (R)a.hzon) iweg++;
+            }
+            hztot = hztot / (R)3;
+            if (hztot >= (R)a.hzon && -wet > 5) iweg = 0;
+            if (iweg == 0) {
+                if (weg > 0) { iwh++; wet = 0; }
+                weg = 0;
+            }
+        }
+    }
+    if (bnewtime || wet == 0) {
+        if (weg == 0) wet++;
+        else wet--;
+    }
+    a.iweg[ie] = weg;
+    a.iwet[ie] = wet;
+    if (iwh && cnt) atomicAdd(cnt, iwh);
+}
+
+// setnod (topo_update.f90:166-270) + adjust_mass_flux (bnd_routines.f90:1185)
+template <typename S>
+__global__ __launch_bounds__(kThreads) void k_setnod(HydroArgs<S> a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nkn) return;
+    double win = 0.;
+    for (int q = a.nk_ptr[k]; q < a.nk_ptr[k + 1]; q++) {
+        const int pk = a.nk_el[q];
+        const int ie = pk >> 2, ii = pk & 3;
+        if (a.iweg[ie] == 0) win = win + a.ang[(size_t)ii * a.nel + ie];
+    }
+    int inod;
+    if (win > 359.99) inod = 0;
+    else if (win == 0.) inod = -2;
+    else inod = -1;
+    const int ibc = a.ibnd[k];
+    if (ibc > 0) {
+        const int typ = a.ibtyp[ibc];
+        if (typ >= 3) {
+            if (inod == -1) inod = ibc;
+        } else if (typ > 0) {
+            if (inod != -1) atomicCAS(a.err, 0, ERR_SETNOD);
+            inod = ibc;
+        }
+    }
+    a.inod[k] = inod;
+    if (inod == -2) {
+        a.rqv[k] = 0;
+        if (a.mflux)
+            for (int l = 0; l < a.nlv; l++) a.mflux[(size_t)l * a.nkn + k] = 0;
+    }
+}
+
+// system_get: z = real(x)
+template <typename S>
+__global__ __launch_bounds__(kThreads) void k_system_get(HydroArgs<S> a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nkn) return;
+    a.zn[k] = (S)a.x[k];
+}
+
+// ------------------------------------------------------------------ final transports
+// hydro_transports_final, hydrodynamic.f90:1468-1578
+template <typename S>
+__global__ __launch_bounds__(kThreads) void k_transports_final(HydroArgs<S> a) {
+    using R = S;
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ie >= a.nel) return;
+    const int nel = a.nel;
+    const R dt = (R)a.dt;
+    const double beta = (double)(dt * (R)a.grav) * (double)a.am;
+    double bz = 0., cz = 0.;
+    for (int ii = 0; ii < 3; ii++) {
+        const int kk = a.nen3v[3 * ie + ii];
+        const double dz = (double)(a.zn[kk] - a.zeo[(size_t)ii * nel + ie]);
+        bz = bz + dz * a.geo[(size_t)ii * nel + ie];
+        cz = cz + dz * a.geo[(size_t)(3 + ii) * nel + ie];
+    }
+    const int ilevel = a.ilhv[ie];
+    double um = 0., vm = 0.;
+    for (int l = 1; l <= ilevel; l++) {
+        const int ju = 2 * l - 2, jv = 2 * l - 1;
+        const double du = beta * ((double)a.ddx[(size_t)ju * nel + ie] * bz + (double)a.ddy[(size_t)ju * nel + ie] * cz);
+        const double dv = beta * ((double)a.ddx[(size_t)jv * nel + ie] * bz + (double)a.ddy[(size_t)jv * nel + ie] * cz);
+        const size_t o = (size_t)(l - 1) * nel + ie;
+        const S u = (S)((double)a.utn[o] - du * 1.0);
+        const S v = (S)((double)a.vtn[o] - dv * 1.0);
+        a.utn[o] = u;
+        a.vtn[o] = v;
+        um = um + (double)u;
+        vm = vm + (double)v;
+    }
+    a.un[ie] = (S)um;
+    a.vn[ie] = (S)vm;
+}
+
+// ------------------------------------------------------------------ setzev/setuvd/baro2l
+// setzev (wetdry.f90:594), setuvd (wetdry.f90:350-590), baro2l
+// (transforms.f90:437-481).  The layer thickness baro2l reads is the one
+// before make_new_depth, i.e. the old level's.
+template <typename S>
+__global__ __launch_bounds__(kThreads) void k_setzev_setuvd(HydroArgs<S> a) {
+    using R = S;
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ie >= a.nel) return;
+    const int nel = a.nel;
+    if (a.iweg[ie] == 0) {
+        for (int ii = 0; ii < 3; ii++) a.zen[(size_t)ii * nel + ie] = a.zn[a.nen3v[3 * ie + ii]];
+        return;
+    }
+    if (!(a.iweg[ie] > 0)) return;
+    const R dt = (R)a.dt, hzmin = (R)a.hzmin, az = (R)a.az, azt = (R)1 - az;
+    const R eps = (R)1.e-5;
+    const R uo = a.uo[ie], vo = a.vo[ie];
+    const R axdt = (R)3 * dt, adt = (R)1 / axdt;
+    double zm = 0., zmed, d1, d2, det;
+    R z[3], b[3], c[3], zn[3], hm[3], u, v, zz;
+    for (int ii = 0; ii < 3; ii++) {
+        zn[ii] = a.zen[(size_t)ii * nel + ie];
+        zm = zm + (double)zn[ii];
+        b[ii] = (R)a.geo[(size_t)ii * nel + ie];
+        c[ii] = (R)a.geo[(size_t)(3 + ii) * nel + ie];
+        hm[ii] = a.hm3v[(size_t)ii * nel + ie];
+    }
+    zmed = zm / 3.;
+    int itot = 0, isum = 0;
+    for (int ii = 0; ii < 3; ii++) {
+        if (zmed + (double)hm[ii] < (double)hzmin) {
+            z[ii] = hzmin - hm[ii];
+            itot++;
+            isum += ii + 1;
+        } else {
+            z[ii] = (R)zmed;
+        }
+        zm = zm - (double)z[ii];
+    }
+    bool fail = false;
+    if (itot == 1) {
+        const int i1 = (isum % 3) + 1, i2 = (i1 % 3) + 1;
+        if ((double)z[i1 - 1] + zm * 0.5 + (double)hm[i1 - 1] < (double)hzmin) {
+            z[i1 - 1] = hzmin - hm[i1 - 1];
+            z[i2 - 1] = (R)(3. * zmed - (double)z[i1 - 1] - (double)z[isum - 1]);
+        } else if ((double)z[i2 - 1] + zm * 0.5 + (double)hm[i2 - 1] < (double)hzmin) {
+            z[i2 - 1] = hzmin - hm[i2 - 1];
+            z[i1 - 1] = (R)(3. * zmed - (double)z[i2 - 1] - (double)z[isum - 1]);
+        } else {
+            z[i1 - 1] = (R)((double)z[i1 - 1] + 0.5 * zm);
+            z[i2 - 1] = (R)((double)z[i2 - 1] + 0.5 * zm);
+        }
+    } else if (itot == 2) {
+        const int i1 = 6 - isum;
+        z[i1 - 1] = (R)((double)z[i1 - 1] + zm);
+    } else if (itot == 3) {
+        fail = true;
+    }
+    zm = 0.;
+    for (int ii = 0; ii < 3; ii++) {
+        zm = zm + (double)z[ii];
+        if (z[ii] + hm[ii] < hzmin - eps) fail = true;
+    }
+    if (fabs(zm - 3. * zmed) > (double)eps) fail = true;
+    if (fail) { atomicCAS(a.err, 0, ERR_SETUVD); return; }
+    d1 = (double)((z[0] - zn[0]) * adt - azt * (b[0] * uo + c[0] * vo));
+    d2 = (double)((z[1] - zn[1]) * adt - azt * (b[1] * uo + c[1] * vo));
+    det = (double)((R)1 / (b[0] * c[1] - b[1] * c[0]));
+    u = (R)(det * ((double)c[1] * d1 - (double)c[0] * d2));
+    v = (R)(det * (-(double)b[1] * d1 + (double)b[0] * d2));
+    itot = 0;
+    for (int ii = 0; ii < 3; ii++) {
+        zz = axdt * (b[ii] * (azt * uo + az * u) + c[ii] * (azt * vo + az * v)) + zn[ii];
+        if (zz + hm[ii] < hzmin - eps) itot++;
+    }
+    if (itot > 0) {
+        if (az == (R)0) { atomicCAS(a.err, 0, ERR_SETUVD); return; }
+        det = (double)((R)1 / (az * (b[0] * c[1] - b[1] * c[0])));
+        u = (R)(det * ((double)c[1] * d1 - (double)c[0] * d2));
+        v = (R)(det * (-(double)b[1] * d1 + (double)b[0] * d2));
+    }
+    itot = 0;
+    zm = 0.;
+    for (int ii = 0; ii < 3; ii++) {
+        zz = axdt * (b[ii] * (azt * uo + az * u) + c[ii] * (azt * vo + az * v)) + zn[ii];
+        if (zz + hm[ii] < hzmin - eps) itot++;
+        a.zen[(size_t)ii * nel + ie] = zz;
+        zm = zm + (double)zz;
+    }
+    if (fabs(zm - 3. * zmed) > (double)eps || itot > 0) { atomicCAS(a.err, 0, ERR_SETUVD); return; }
+    a.un[ie] = u;
+    a.vn[ie] = v;
+    // baro2l
+    const int ilevel = a.ilhv[ie];
+    R htot = 0;
+    for (int l = 1; l <= ilevel; l++) htot = htot + a.hdeo[(size_t)(l - 1) * nel + ie];
+    for (int l = 1; l <= ilevel; l++) {
+        const R weight = a.hdeo[(size_t)(l - 1) * nel + ie] / htot;
+        a.utn[(size_t)(l - 1) * nel + ie] = u * weight;
+        a.vtn[(size_t)(l - 1) * nel + ie] = v * weight;
+    }
+}
+
+// ------------------------------------------------------------------ setdepth
+// setdepth for z-layers (elems_dealing.f90:930-1156): element thickness
+template <typename S>
+__global__ __launch_bounds__(kThreads) void k_setdepth_elem(HydroArgs<S> a, const S *zen, S *hden) {
+    using R = S;
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ie >= a.nel) return;
+    const int nel = a.nel, lmax = a.ilhv[ie];
+    R zmed = 0;
+    for (int ii = 0; ii < 3; ii++) zmed = zmed + zen[(size_t)ii * nel + ie];
+    zmed = zmed / (R)3;
+    const R htot = a.hev[ie];
+    bool bad = false;
+    for (int l = 1; l <= a.nlvdi; l++) {
+        R h = 0;
+        if (l <= lmax) {
+            if (lmax == 1) {
+                h = htot + zmed;
+            } else if (l < lmax) {
+                h = a.hldv[l - 1];
+                if (l == 1) h = h + zmed;
+            } else {
+                h = htot - a.hlv[lmax - 2];
+                if (h < (R)0) bad = true;
+            }
+            if (h <= (R)0) bad = true;
+        }
+        hden[(size_t)(l - 1) * nel + ie] = h;
+    }
+    if (bad) atomicCAS(a.err, 0, ERR_SETDEPTH);
+}
+
+// node thickness: volume contributions of the incident elements (element
+// order, per layer) divided by the nodal area
+template <typename S, int NL>
+__global__ __launch_bounds__(kThreads) void k_setdepth_node(HydroArgs<S> a, const S *zen, S *hdkn) {
+    using R = S;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nkn) return;
+    const int nel = a.nel, nkn = a.nkn;
+    R acc[NL];
+#pragma unroll
+    for (int l = 0; l < NL; l++) acc[l] = 0;
+    bool bad = false;
+    for (int q = a.nk_ptr[k]; q < a.nk_ptr[k + 1]; q++) {
+        const int pk = a.nk_el[q];
+        const int ie = pk >> 2, ii = pk & 3;
+        const R areael = (R)(12. * a.geo[(size_t)6 * nel + ie]);
+        const R areafv = areael / (R)3;
+        const int lmax = a.ilhv[ie];
+        const R htot = a.hm3v[(size_t)ii * nel + ie];
+        const R z = zen[(size_t)ii * nel + ie];
+        if (lmax == 1) {
+            acc[0] = acc[0] + areafv * (htot + z);
+        } else {
+            const R hlast = htot - a.hlv[lmax - 2];
+            if (hlast < (R)0) bad = true;
+#pragma unroll
+            for (int l = 1; l <= NL; l++) {
+                if (l < lmax) acc[l - 1] = acc[l - 1] + areafv * a.hldv[l - 1];
+                if (l == 1) acc[0] = acc[0] + areafv * z;
+                if (l == lmax) acc[l - 1] = acc[l - 1] + areafv * hlast;
+            }
+        }
+    }
+    const int lmaxk = a.ilhkv[k];
+    bool stop = false;
+#pragma unroll
+    for (int l = 1; l <= NL; l++) {
+        if (l <= lmaxk && !stop) {
+            const R ar = a.areakv[(size_t)(l - 1) * nkn + k];
+            if (ar > (R)0) acc[l - 1] = acc[l - 1] / ar;
+            else stop = true;
+        }
+        if (l <= lmaxk && acc[l - 1] <= (R)0) bad = true;
+        if (l <= a.nlvdi) hdkn[(size_t)(l - 1) * nkn + k] = (l <= lmaxk) ? acc[l - 1] : (R)0;
+    }
+    if (bad) atomicCAS(a.err, 0, ERR_SETDEPTH);
+}
+
+// ------------------------------------------------------------------ hydro_vertical
+// hydrodynamic.f90:1582-1781: divergence gathered per node and layer, then
+// the column is integrated from the bottom; w = 0 on zeta boundaries.
+template <typename S, int NL>
+__global__ __launch_bounds__(kThreads) void k_hydro_vertical(HydroArgs<S> a) {
+    using R = S;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nkn) return;
+    const int nel = a.nel, nkn = a.nkn, nlvdi = a.nlvdi;
+    const R az = (R)a.az, azt = (R)1 - az, dt = (R)a.dt;
+    R vf[NL], va[NL];
+#pragma unroll
+    for (int l = 0; l < NL; l++) { vf[l] = 0; va[l] = 0; }
+    for (int q = a.nk_ptr[k]; q < a.nk_ptr[k + 1]; q++) {
+        const int pk = a.nk_el[q];
+        const int ie = pk >> 2, ii = pk & 3;
+        const R aj = (R)(4. * a.geo[(size_t)6 * nel + ie]);
+        const R b = (R)a.geo[(size_t)ii * nel + ie], c = (R)a.geo[(size_t)(3 + ii) * nel + ie];
+        const int ilevel = a.ilhv[ie];
+#pragma unroll
+        for (int l = 1; l <= NL; l++) {
+            if (l > ilevel) break;
+            const size_t o = (size_t)(l - 1) * nel + ie;
+            const R ffn = a.utn[o] * b + a.vtn[o] * c;
+            const R ffo = a.uto[o] * b + a.vto[o] * c;
+            const R ff = ffn * az + ffo * azt;
+            vf[l - 1] = vf[l - 1] + (R)3 * aj * ff;
+            va[l - 1] = va[l - 1] + aj;
+        }
+    }
+    const int lmax = a.ilhkv[k];
+    R w[NL + 1];
+#pragma unroll
+    for (int l = 0; l <= NL; l++) w[l] = 0;
+#pragma unroll
+    for (int l = NL; l >= 1; l--) {
+        if (l > lmax) continue;
+        const size_t o = (size_t)(l - 1) * nkn + k;
+        const R ar = a.areakv[o];
+        const R voln = a.hdkn[o] * ar, volo = a.hdko[o] * ar;
+        const R dvdt = (voln - volo) / dt;
+        const R qq = a.mflux ? a.mflux[o] : (R)0;
+        const R wdiv = vf[l - 1] + qq;
+        w[l - 1] = w[l] + wdiv - dvdt;
+    }
+    w[0] = 0;
+#pragma unroll
+    for (int l = 2; l <= NL; l++) {
+        if (l > lmax) break;
+        const R atop = va[l - 1];
+        if (atop > (R)0) w[l - 1] = w[l - 1] / atop;
+    }
+    const bool zb = a.zbound[k] != 0;
+    for (int l = 0; l <= nlvdi; l++) a.wln[(size_t)l * nkn + k] = (zb || l > NL) ? (S)0 : w[l < NL ? l : NL];
+}
+
+// ttov (transforms.f90:98-122) + uvint (transforms.f90:336-362)
+template <typename S>
+__global__ __launch_bounds__(kThreads) void k_ttov_uvint(HydroArgs<S> a) {
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ie >= a.nel) return;
+    const int nel = a.nel;
+    double u = 0., v = 0.;
+    for (int l = 1; l <= a.nlvdi; l++) {
+        const size_t o = (size_t)(l - 1) * nel + ie;
+        const S h = a.hden[o];
+        const S ut = a.utn[o], vt = a.vtn[o];
+        a.uln[o] = (h > (S)0) ? ut / h : (S)0;
+        a.vln[o] = (h > (S)0) ? vt / h : (S)0;
+        if (l <= a.ilhv[ie]) { u = u + (double)ut; v = v + (double)vt; }
+    }
+    a.un[ie] = (S)u;
+    a.vn[ie] = (S)v;
+}
+
+// setznv (wetdry.f90:619-700) at start-up
+template <typename S>
+__global__ __launch_bounds__(kThreads) void k_setznv(HydroArgs<S> a) {
+    using R = S;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nkn) return;
+    const int nel = a.nel;
+    R z = (R)kFlag, v1 = 0, v2 = 0;
+    for (int q = a.nk_ptr[k]; q < a.nk_ptr[k + 1]; q++) {
+        const int pk = a.nk_el[q];
+        const int ie = pk >> 2, ii = pk & 3;
+        const R ze = a.zen[(size_t)ii * nel + ie];
+        if (a.iweg[ie] == 0) {
+            R zz = ze;
+            if (z != (R)kFlag && z != zz) zz = (zz > z) ? zz : z;
+            z = zz;
+        } else {
+            const R area = (R)(4. * a.geo[(size_t)6 * nel + ie]);
+            v1 = v1 + ze * area;
+            v2 = v2 + area;
+        }
+    }
+    if (z == (R)kFlag) z = v1 / v2;
+    a.zn[k] = z;
+}
+
+template <typename S>
+__global__ void k_fill(S *p, S v, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <typename S>
+__global__ void k_copy(S *dst, const S *src, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+template <typename S>
+__global__ void k_set_rzv(S *rzv, const int *nodes, const int *bid, int n, int ibc, S level) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && bid[i] == ibc) rzv[nodes[i]] = level;
+}
+
+// FP64 total volume (masscont, elems_dealing.f90:1191-1220): per-block sums
+// in a fixed tree order, then a single-block pass over the partials.
+template <typename S>
+__global__ __launch_bounds__(kThreads) void k_volume_part(HydroArgs<S> a, const S *hdk, const S *conc, double *part) {
+    __shared__ double red[kThreads];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double t = 0.;
+    if (k < a.nkn) {
+        for (int l = 1; l <= a.ilhkv[k]; l++) {
+            const size_t o = (size_t)(l - 1) * a.nkn + k;
+            const S vol = hdk[o] * a.areakv[o];
+            t += conc ? (double)conc[o] * (double)vol : (double)vol;
+        }
+    }
+    red[threadIdx.x] = t;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(kThreads) void k_sum_parts(const double *part, int n, double *out) {
+    __shared__ double red[kThreads];
+    double t = 0.;
+    for (int i = threadIdx.x; i < n; i += kThreads) t += part[i];
+    red[threadIdx.x] = t;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+// ------------------------------------------------------------------ launchers
+#define LAUNCH(M, kern, grid, ...)                                                         \
+    do {                                                                                   \
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), 0, (M).stream, __VA_ARGS__);  \
+        HIPCHK(M, hipGetLastError());                                                      \
+    } while (0)
+
+template <typename S>
+static int set_dry_t(Model &M, int iweich, int bnewtime, int *changed) {
+    auto a = make_args<S>(M);
+    int *cnt = M.d_err + 1;
+    HIPCHK(M, hipMemsetAsync(cnt, 0, sizeof(int), M.stream));
+    LAUNCH(M, k_setweg<S>, nblocks(M.nel), a, iweich, bnewtime, cnt);
+    if (changed) {
+        HIPCHK(M, hipMemcpyAsync(M.h_pinned, cnt, sizeof(int), hipMemcpyDeviceToHost, M.stream));
+        HIPCHK(M, hipStreamSynchronize(M.stream));
+        *changed = M.h_pinned[0];
+    }
+    return 0;
+}
+int launch_set_dry(Model &M, int iweich, int bnewtime, int *changed) {
+    return M.prec == SHYHIP_FP64 ? set_dry_t<double>(M, iweich, bnewtime, changed)
+                                 : set_dry_t<float>(M, iweich, bnewtime, changed);
+}
+
+template <typename S>
+static int setnod_t(Model &M) {
+    auto a = make_args<S>(M);
+    LAUNCH(M, k_setnod<S>, nblocks(M.nkn), a);
+    return 0;
+}
+int launch_setnod(Model &M) { return M.prec == SHYHIP_FP64 ? setnod_t<double>(M) : setnod_t<float>(M); }
+
+template <typename S, int NL>
+static int hydro_intern_nl(Model &M) {
+    auto a = make_args<S>(M);
+    if (M.has_coriolis) LAUNCH(M, (k_hydro_intern<S, NL, true>), nblocks(M.nel), a);
+    else LAUNCH(M, (k_hydro_intern<S, NL, false>), nblocks(M.nel), a);
+    return 0;
+}
+template <typename S>
+static int hydro_intern_t(Model &M) {
+    switch (M.nl_tmpl) {
+    case 1: return hydro_intern_nl<S, 1>(M);
+    case 2: return hydro_intern_nl<S, 2>(M);
+    case 4: return hydro_intern_nl<S, 4>(M);
+    case 8: return hydro_intern_nl<S, 8>(M);
+    case 12: return hydro_intern_nl<S, 12>(M);
+    case 16: return hydro_intern_nl<S, 16>(M);
+    case 20: return hydro_intern_nl<S, 20>(M);
+    case 24: return hydro_intern_nl<S, 24>(M);
+    default: return hydro_intern_nl<S, 32>(M);
+    }
+}
+int launch_hydro_intern(Model &M) { return M.prec == SHYHIP_FP64 ? hydro_intern_t<double>(M) : hydro_intern_t<float>(M); }
+
+template <typename S>
+static int zeta_t(Model &M) {
+    auto a = make_args<S>(M);
+    LAUNCH(M, k_zeta_rows<S>, nblocks(M.nkn), a);
+    return 0;
+}
+int launch_zeta_assemble(Model &M) { return M.prec == SHYHIP_FP64 ? zeta_t<double>(M) : zeta_t<float>(M); }
+
+template <typename S>
+static int sysget_t(Model &M, int bnewtime, int *changed) {
+    auto a = make_args<S>(M);
+    LAUNCH(M, k_system_get<S>, nblocks(M.nkn), a);
+    return set_dry_t<S>(M, 1, bnewtime, changed);
+}
+int launch_system_get_setweg1(Model &M, int bnewtime, int *changed) {
+    return M.prec == SHYHIP_FP64 ? sysget_t<double>(M, bnewtime, changed) : sysget_t<float>(M, bnewtime, changed);
+}
+
+template <typename S>
+static int final_t(Model &M) {
+    auto a = make_args<S>(M);
+    LAUNCH(M, k_transports_final<S>, nblocks(M.nel), a);
+    return 0;
+}
+int launch_transports_final(Model &M) { return M.prec == SHYHIP_FP64 ? final_t<double>(M) : final_t<float>(M); }
+
+template <typename S>
+static int setuvd_t(Model &M) {
+    auto a = make_args<S>(M);
+    LAUNCH(M, k_setzev_setuvd<S>, nblocks(M.nel), a);
+    return 0;
+}
+int launch_setzev_setuvd(Model &M) { return M.prec == SHYHIP_FP64 ? setuvd_t<double>(M) : setuvd_t<float>(M); }
+
+template <typename S, int NL>
+static int setdepth_nl(Model &M, bool old_level) {
+    auto a = make_args<S>(M);
+    const S *zen = old_level ? a.zeo : a.zen;
+    S *hden = old_level ? a.hdeo : a.hden;
+    S *hdkn = old_level ? a.hdko : a.hdkn;
+    LAUNCH(M, k_setdepth_elem<S>, nblocks(M.nel), a, zen, hden);
+    LAUNCH(M, (k_setdepth_node<S, NL>), nblocks(M.nkn), a, zen, hdkn);
+    return 0;
+}
+template <typename S>
+static int setdepth_t(Model &M, bool o) {
+    switch (M.nl_tmpl) {
+    case 1: return setdepth_nl<S, 1>(M, o);
+    case 2: return setdepth_nl<S, 2>(M, o);
+    case 4: return setdepth_nl<S, 4>(M, o);
+    case 8: return setdepth_nl<S, 8>(M, o);
+    case 12: return setdepth_nl<S, 12>(M, o);
+    case 16: return setdepth_nl<S, 16>(M, o);
+    case 20: return setdepth_nl<S, 20>(M, o);
+    case 24: return setdepth_nl<S, 24>(M, o);
+    default: return setdepth_nl<S, 32>(M, o);
+    }
+}
+int launch_setdepth(Model &M, bool old_level) {
+    return M.prec == SHYHIP_FP64 ? setdepth_t<double>(M, old_level) : setdepth_t<float>(M, old_level);
+}
+
+template <typename S, int NL>
+static int vertical_nl(Model &M) {
+    auto a = make_args<S>(M);
+    LAUNCH(M, (k_hydro_vertical<S, NL>), nblocks(M.nkn), a);
+    return 0;
+}
+template <typename S>
+static int vertical_t(Model &M) {
+    if (M.nlv == 1) {
+        auto a = make_args<S>(M);
+        const size_t n = (size_t)(M.nlvdi + 1) * M.nkn;
+        LAUNCH(M, k_fill<S>, (int)((n + kThreads - 1) / kThreads), a.wln, (S)0, n);
+        return 0;
+    }
+    switch (M.nl_tmpl) {
+    case 2: return vertical_nl<S, 2>(M);
+    case 4: return vertical_nl<S, 4>(M);
+    case 8: return vertical_nl<S, 8>(M);
+    case 12: return vertical_nl<S, 12>(M);
+    case 16: return vertical_nl<S, 16>(M);
+    case 20: return vertical_nl<S, 20>(M);
+    case 24: return vertical_nl<S, 24>(M);
+    default: return vertical_nl<S, 32>(M);
+    }
+}
+int launch_hydro_vertical(Model &M) { return M.prec == SHYHIP_FP64 ? vertical_t<double>(M) : vertical_t<float>(M); }
+
+template <typename S>
+static int ttov_t(Model &M) {
+    auto a = make_args<S>(M);
+    LAUNCH(M, k_ttov_uvint<S>, nblocks(M.nel), a);
+    return 0;
+}
+int launch_ttov_uvint(Model &M) { return M.prec == SHYHIP_FP64 ? ttov_t<double>(M) : ttov_t<float>(M); }
+
+template <typename S>
+static int copy_old_t(Model &M) {
+    // copy_uvz + copy_depth: arrays that are rewritten before being read are
+    // exchanged; z and the vertex levels are copied (they are read first).
+    std::swap(M.s.utn, M.s.uto);
+    std::swap(M.s.vtn, M.s.vto);
+    std::swap(M.s.un, M.s.uo);
+    std::swap(M.s.vn, M.s.vo);
+    std::swap(M.s.wln, M.s.wlo);
+    std::swap(M.s.hdkn, M.s.hdko);
+    std::swap(M.s.hden, M.s.hdeo);
+    auto a = make_args<S>(M);
+    LAUNCH(M, k_copy<S>, nblocks(M.nkn), a.zo, (const S *)a.zn, (size_t)M.nkn);
+    LAUNCH(M, k_copy<S>, nblocks(3 * M.nel), a.zeo, (const S *)a.zen, (size_t)3 * M.nel);
+    return 0;
+}
+int launch_copy_old(Model &M) { return M.prec == SHYHIP_FP64 ? copy_old_t<double>(M) : copy_old_t<float>(M); }
+
+template <typename S>
+static int set_rzv_t(Model &M, int ibc, float level) {
+    if (M.m.nbnd_nodes == 0) return 0;
+    LAUNCH(M, k_set_rzv<S>, nblocks(M.m.nbnd_nodes), (S *)M.s.rzv, M.m.bnd_nodes, M.m.bnd_id, M.m.nbnd_nodes, ibc,
+           (S)level);
+    return 0;
+}
+int launch_set_rzv(Model &M, int ibc, float level) {
+    return M.prec == SHYHIP_FP64 ? set_rzv_t<double>(M, ibc, level) : set_rzv_t<float>(M, ibc, level);
+}
+
+int launch_fill(Model &M, void *ptr, double value, size_t n) {
+    const int g = (int)((n + kThreads - 1) / kThreads);
+    if (M.prec == SHYHIP_FP64) LAUNCH(M, k_fill<double>, g, (double *)ptr, value, n);
+    else LAUNCH(M, k_fill<float>, g, (float *)ptr, (float)value, n);
+    return 0;
+}
+
+template <typename S>
+static int init_levels_t(Model &M, float zconst) {
+    // setweg(-1), setnod, init_z, setweg(0), setznv, make_new_depth, copy_depth
+    if (set_dry_t<S>(M, -1, 1, nullptr)) return 1;
+    if (setnod_t<S>(M)) return 1;
+    auto a = make_args<S>(M);
+    LAUNCH(M, k_fill<S>, nblocks(M.nkn), a.zn, (S)zconst, (size_t)M.nkn);
+    LAUNCH(M, k_fill<S>, nblocks(3 * M.nel), a.zen, (S)zconst, (size_t)3 * M.nel);
+    if (set_dry_t<S>(M, 0, 0, nullptr)) return 1;
+    LAUNCH(M, k_setznv<S>, nblocks(M.nkn), a);
+    if (setdepth_t<S>(M, false)) return 1;
+    return 0;
+}
+int launch_init_levels(Model &M, float zconst) {
+    return M.prec == SHYHIP_FP64 ? init_levels_t<double>(M, zconst) : init_levels_t<float>(M, zconst);
+}
+
+template <typename S>
+static int volume_t(Model &M, const void *hdk, const void *conc, double *out) {
+    auto a = make_args<S>(M);
+    const int nb = nblocks(M.nkn);
+    double *part = M.sv.part;   // large enough: >= nblocks(nkn)
+    LAUNCH(M, k_volume_part<S>, nb, a, (const S *)hdk, (const S *)conc, part);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kThreads), 0, M.stream, part, nb, M.sv.scal + 15);
+    HIPCHK(M, hipGetLastError());
+    HIPCHK(M, hipMemcpyAsync(out, M.sv.scal + 15, sizeof(double), hipMemcpyDeviceToHost, M.stream));
+    HIPCHK(M, hipStreamSynchronize(M.stream));
+    return 0;
+}
+int launch_volume(Model &M, int mode, double *vol) {
+    const void *hdk = mode > 0 ? M.s.hdkn : M.s.hdko;
+    return M.prec == SHYHIP_FP64 ? volume_t<double>(M, hdk, nullptr, vol) : volume_t<float>(M, hdk, nullptr, vol);
+}
+int launch_tracer_mass(Model &M, int itr, double *mass) {
+    return M.prec == SHYHIP_FP64 ? volume_t<double>(M, M.s.hdkn, M.s.tracer[itr], mass)
+                                 : volume_t<float>(M, M.s.hdkn, M.s.tracer[itr], mass);
+}
+
+}  // namespace shy

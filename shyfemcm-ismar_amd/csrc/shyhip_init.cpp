@@ -1,0 +1,354 @@
+// shyhip_init.cpp -- host-side, init-time preprocessing of the basin.
+//
+// These run once per basin on the CPU and produce the integer structures and
+// FP64 geometry the device kernels consume:
+//   host_set_ev        element geometry, src/utils/shyutil/evgeom.f90:167-349
+//   host_node_list     zeta-matrix pattern + element slots,
+//                      src/shyfem/coo_matrix.f90:51-172,330-404
+//   host_color         greedy element colouring + balancing,
+//                      src/shyfem/subset.f90:371-711
+//   host_tvd_tables    upwind points of the itvd=2 scheme with their
+//                      interpolation weights, src/shyfem/tvd_admin.f90:361-474,
+//                      src/utils/shyutil/evgeom.f90:531-660
+//   host_block_schedule  block-ILU(0) pattern and level sets (no reference
+//                      counterpart: replaces ILUT's sequential factorisation)
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "shyhip_internal.h"
+
+namespace shy {
+
+// b and c coefficients are corrected so that each triple sums to zero
+// (evgeom.f90:1007-1042)
+static void zero_sum(double &v1, double &v2, double &v3) {
+    const double vv = v1 + v2 + v3;
+    if (vv == 0.) return;
+    if ((v1 == 0. && v2 == 0.) || (v1 == 0. && v3 == 0.) || (v2 == 0. && v3 == 0.)) return;
+    if (v1 == 0.) {
+        v2 = v2 - vv / 2.;
+        v3 = -v2;
+    } else if (v2 == 0.) {
+        v3 = v3 - vv / 2.;
+        v1 = -v3;
+    } else if (v3 == 0.) {
+        v1 = v1 - vv / 2.;
+        v2 = -v1;
+    } else {
+        v1 = v1 - vv / 3.;
+        v2 = v2 - vv / 3.;
+        v3 = v3 - vv / 3.;
+        v3 = -v1 - v2;
+    }
+}
+
+void host_set_ev(int nkn, int nel, const int *nen3v0, const float *xg, const float *yg, double *ev) {
+    (void)nkn;
+    const double pi = 4. * std::atan(1.);
+    const double rad = 180. / pi;
+    for (int ie = 0; ie < nel; ie++) {
+        const int *kn = nen3v0 + 3 * ie;
+        const double x1 = xg[kn[0]], y1 = yg[kn[0]];
+        const double x2 = xg[kn[1]], y2 = yg[kn[1]];
+        const double x3 = xg[kn[2]], y3 = yg[kn[2]];
+        const double aj = (x2 - x1) * (y3 - y1) - (x3 - x1) * (y2 - y1);
+        const double aji = 1. / aj;
+        double a[3] = {(x2 * y3 - x3 * y2) * aji, (x3 * y1 - x1 * y3) * aji, (x1 * y2 - x2 * y1) * aji};
+        double b[3] = {(y2 - y3) * aji, (y3 - y1) * aji, (y1 - y2) * aji};
+        double c[3] = {(x3 - x2) * aji, (x1 - x3) * aji, (x2 - x1) * aji};
+        zero_sum(b[0], b[1], b[2]);
+        zero_sum(c[0], c[1], c[2]);
+        const double ss1 = (x3 - x2) * (x3 - x2) + (y3 - y2) * (y3 - y2);
+        const double ss2 = (x3 - x1) * (x3 - x1) + (y3 - y1) * (y3 - y1);
+        const double ss3 = (x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1);
+        const double s1 = std::sqrt(ss1), s2 = std::sqrt(ss2), s3 = std::sqrt(ss3);
+        double *e = ev + 19 * (size_t)ie;
+        for (int i = 0; i < 3; i++) {
+            e[i] = a[i];
+            e[3 + i] = b[i];
+            e[6 + i] = c[i];
+            e[13 + i] = aj * std::sqrt(b[i] * b[i] + c[i] * c[i]);
+        }
+        e[9] = aj / 24.;
+        e[10] = std::acos((ss2 + ss3 - ss1) / (2. * s2 * s3)) * rad;
+        e[11] = std::acos((ss1 + ss3 - ss2) / (2. * s1 * s3)) * rad;
+        e[12] = std::acos((ss1 + ss2 - ss3) / (2. * s1 * s2)) * rad;
+        e[16] = s1;
+        e[17] = s2;
+        e[18] = s3;
+    }
+}
+
+int host_node_list(int nkn, int nel, const int *nen3v0, std::vector<int> &ntg, std::vector<int> &cols,
+                   std::vector<int> &ijp) {
+    // adjacency through element edges, plus the node itself, sorted and unique
+    std::vector<std::vector<int>> adj(nkn);
+    for (int ie = 0; ie < nel; ie++)
+        for (int ii = 0; ii < 3; ii++) {
+            int k1 = nen3v0[3 * ie + ii], k2 = nen3v0[3 * ie + (ii + 1) % 3];
+            adj[k1].push_back(k2);
+            adj[k2].push_back(k1);
+        }
+    ntg.assign(nkn + 1, 0);
+    cols.clear();
+    for (int k = 0; k < nkn; k++) {
+        auto &a = adj[k];
+        a.push_back(k);
+        std::sort(a.begin(), a.end());
+        a.erase(std::unique(a.begin(), a.end()), a.end());
+        cols.insert(cols.end(), a.begin(), a.end());
+        ntg[k + 1] = (int)cols.size();
+    }
+    ijp.assign(9 * (size_t)nel, -1);
+    for (int ie = 0; ie < nel; ie++)
+        for (int i = 0; i < 3; i++) {
+            const int ki = nen3v0[3 * ie + i];
+            for (int j = 0; j < 3; j++) {
+                const int kj = nen3v0[3 * ie + j];
+                auto first = cols.begin() + ntg[ki], last = cols.begin() + ntg[ki + 1];
+                auto it = std::lower_bound(first, last, kj);
+                if (it == last || *it != kj) return -1;
+                ijp[9 * (size_t)ie + 3 * i + j] = (int)(it - cols.begin());
+            }
+        }
+    return (int)cols.size();
+}
+
+// Greedy colouring in a rotating sweep; a node's colour bitmask forbids two
+// elements of one colour sharing a node; then elements move from the largest
+// to the smallest colour until no move is possible.
+int host_color(int nkn, int nel, const int *nen3v0, std::vector<int> &color) {
+    std::vector<int> col(nel + 1, 0);
+    std::vector<uint32_t> nodemask(nkn, 0u);
+    auto allowed = [&](int ie, int from, int to) {
+        if (col[ie] != from) return false;
+        const int *kn = nen3v0 + 3 * (ie - 1);
+        for (int ii = 0; ii < 3; ii++)
+            if ((nodemask[kn[ii]] >> to) & 1u) return false;
+        return true;
+    };
+    auto recolor = [&](int ie, int to) {
+        const int from = col[ie];
+        col[ie] = to;
+        const int *kn = nen3v0 + 3 * (ie - 1);
+        for (int ii = 0; ii < 3; ii++) nodemask[kn[ii]] = (nodemask[kn[ii]] & ~(1u << from)) | (1u << to);
+    };
+    auto sweep = [&](int from, int to, int &start, int limit) {
+        int moved = 0, ie = start;
+        for (int n = 1; n <= nel && moved < limit; n++) {
+            ie = ie + 1;
+            if (ie > nel) ie = ie % nel;
+            if (allowed(ie, from, to)) {
+                moved++;
+                recolor(ie, to);
+            }
+        }
+        start = ie;
+        return moved;
+    };
+    int ncolor = 0, start = 0;
+    for (int c = 1;; c++) {
+        if (c > 31) return -1;
+        if (sweep(0, c, start, nel) == 0) {
+            ncolor = c - 1;
+            break;
+        }
+    }
+    std::vector<int> count(ncolor + 1, 0);
+    auto fill = [&]() {
+        std::fill(count.begin(), count.end(), 0);
+        for (int ie = 1; ie <= nel; ie++) count[col[ie]]++;
+    };
+    fill();
+    const int naver = (int)std::lround((float)nel / (float)ncolor);
+    int start2 = 0;
+    for (;;) {
+        int nmin = nel, nmax = 0, cmin = 0, cmax = 0;
+        for (int c = 1; c <= ncolor; c++) {
+            if (count[c] > nmax) { nmax = count[c]; cmax = c; }
+            if (count[c] < nmin) { nmin = count[c]; cmin = c; }
+        }
+        const int limit = std::min(nmax - naver, naver - nmin);
+        const int moved = sweep(cmax, cmin, start2, limit);
+        fill();
+        if (moved == 0) break;
+    }
+    color.assign(nel, 0);
+    for (int ie = 1; ie <= nel; ie++) color[ie - 1] = col[ie];
+    // every element coloured, no node shared within a colour
+    std::vector<uint32_t> seen(nkn, 0u);
+    for (int ie = 0; ie < nel; ie++) {
+        const int c = color[ie];
+        if (c < 1 || c > ncolor) return -2;
+        for (int ii = 0; ii < 3; ii++) {
+            uint32_t &m = seen[nen3v0[3 * ie + ii]];
+            if ((m >> c) & 1u) return -2;
+            m |= 1u << c;
+        }
+    }
+    return ncolor;
+}
+
+// natural coordinates of point (x,y) in element ie, clipped and renormalised
+// as adjust_xi does (evgeom.f90:577-660)
+static bool natural_coords(const double *ev, int ie, double x, double y, double xi[3]) {
+    const double *e = ev + 19 * (size_t)ie;
+    double sum = 0., clipped = 0.;
+    int nzero = 0, zsum = 0;
+    for (int i = 0; i < 3; i++) {
+        xi[i] = e[i] + e[3 + i] * x + e[6 + i] * y;
+        sum += xi[i];
+        xi[i] = std::min(1., std::max(0., xi[i]));
+        if (xi[i] == 0.) {
+            zsum += i + 1;
+            nzero++;
+        }
+        clipped += xi[i];
+    }
+    if (nzero == 3 || std::fabs(sum - 1.) > (double)1.e-8f) return false;
+    const double adj = (clipped - 1.) / (3 - nzero);
+    if (nzero == 2) {
+        xi[6 - zsum - 1] = 1.;
+    } else {
+        for (int i = 0; i < 3; i++)
+            if (i + 1 != zsum) xi[i] = std::min(1., std::max(0., xi[i] - adj));
+    }
+    return std::fabs(xi[0] + xi[1] + xi[2] - 1.) <= (double)1.e-8f;
+}
+
+// For each ordered vertex pair (ic,id) of an element the upwind point is
+// 2*x(ic) - x(id) (stored as a default real); the element containing it is
+// found by a uniform-grid search (lowest element index on ties) and the
+// interpolation weights are stored.  Pair p = 2*(ic-1) + t enumerates id =
+// mod(ic,3)+1 (t=0) and mod(ic+1,3)+1 (t=1).
+void host_tvd_tables(int nkn, int nel, const int *nen3v0, const float *xg, const float *yg, const double *ev,
+                     std::vector<int> &tel, std::vector<int> &tkn, std::vector<double> &tw) {
+    double xmin = 1e300, xmax = -1e300, ymin = 1e300, ymax = -1e300;
+    for (int k = 0; k < nkn; k++) {
+        xmin = std::min(xmin, (double)xg[k]);
+        xmax = std::max(xmax, (double)xg[k]);
+        ymin = std::min(ymin, (double)yg[k]);
+        ymax = std::max(ymax, (double)yg[k]);
+    }
+    const int nc = (int)std::sqrt((double)nel) + 1;
+    const double dxc = (xmax - xmin) / nc * (1 + 1e-9) + 1e-9, dyc = (ymax - ymin) / nc * (1 + 1e-9) + 1e-9;
+    std::vector<std::vector<int>> cell((size_t)nc * nc);
+    for (int ie = 0; ie < nel; ie++) {
+        const int *kn = nen3v0 + 3 * ie;
+        double bx0 = 1e300, bx1 = -1e300, by0 = 1e300, by1 = -1e300;
+        for (int ii = 0; ii < 3; ii++) {
+            bx0 = std::min(bx0, (double)xg[kn[ii]]);
+            bx1 = std::max(bx1, (double)xg[kn[ii]]);
+            by0 = std::min(by0, (double)yg[kn[ii]]);
+            by1 = std::max(by1, (double)yg[kn[ii]]);
+        }
+        const int cx0 = (int)std::floor((bx0 - xmin) / dxc), cx1 = (int)std::floor((bx1 - xmin) / dxc);
+        const int cy0 = (int)std::floor((by0 - ymin) / dyc), cy1 = (int)std::floor((by1 - ymin) / dyc);
+        for (int cy = cy0; cy <= cy1; cy++)
+            for (int cx = cx0; cx <= cx1; cx++) cell[(size_t)cx + (size_t)nc * cy].push_back(ie);
+    }
+    auto locate = [&](double px, double py) {
+        const int cx = (int)std::floor((px - xmin) / dxc), cy = (int)std::floor((py - ymin) / dyc);
+        if (cx < 0 || cy < 0 || cx >= nc || cy >= nc) return -1;
+        int best = -1;
+        for (int ie : cell[(size_t)cx + (size_t)nc * cy]) {
+            if (best >= 0 && ie > best) continue;
+            const int *kn = nen3v0 + 3 * ie;
+            const double x1 = xg[kn[0]], y1 = yg[kn[0]], x2 = xg[kn[1]], y2 = yg[kn[1]];
+            const double x3 = xg[kn[2]], y3 = yg[kn[2]];
+            const double d = (x2 - x1) * (y3 - y1) - (x3 - x1) * (y2 - y1);
+            const double l1 = ((x2 - px) * (y3 - py) - (x3 - px) * (y2 - py)) / d;
+            const double l2 = ((x3 - px) * (y1 - py) - (x1 - px) * (y3 - py)) / d;
+            const double l3 = 1. - l1 - l2;
+            if (l1 >= -1e-12 && l2 >= -1e-12 && l3 >= -1e-12) best = ie;
+        }
+        return best;
+    };
+    tel.assign(6 * (size_t)nel, -1);
+    tkn.assign(18 * (size_t)nel, 0);
+    tw.assign(18 * (size_t)nel, 0.);
+    for (int ie = 0; ie < nel; ie++) {
+        for (int ic = 1; ic <= 3; ic++) {
+            const int kc = nen3v0[3 * ie + ic - 1];
+            for (int t = 0; t < 2; t++) {
+                const int id = (t == 0) ? (ic % 3) + 1 : ((ic + 1) % 3) + 1;
+                const int kd = nen3v0[3 * ie + id - 1];
+                const float xu = (float)(2. * (double)xg[kc] - (double)xg[kd]);
+                const float yu = (float)(2. * (double)yg[kc] - (double)yg[kd]);
+                const int p = 2 * (ic - 1) + t;
+                const int up = locate(xu, yu);
+                if (up < 0) continue;
+                double xi[3];
+                if (!natural_coords(ev, up, (double)xu, (double)yu, xi)) continue;
+                tel[(size_t)p * nel + ie] = up;
+                for (int ii = 0; ii < 3; ii++) {
+                    tkn[((size_t)ii * 6 + p) * nel + ie] = nen3v0[3 * up + ii];
+                    tw[((size_t)ii * 6 + p) * nel + ie] = xi[ii];
+                }
+            }
+        }
+    }
+}
+
+// Block-Jacobi ILU(0): rows are split into contiguous blocks of `brows`;
+// couplings outside the diagonal block are dropped.  For each block the
+// lower and upper triangular level sets are built for a one-workgroup
+// level-scheduled factorisation and solve.
+void host_block_schedule(int n, const std::vector<int> &rowptr, const std::vector<int> &cols, int brows,
+                         BlockSchedule &bs) {
+    bs.brows = brows;
+    bs.nblk = (n + brows - 1) / brows;
+    bs.rowptr.assign(n + 1, 0);
+    bs.cols.clear();
+    bs.src.clear();
+    bs.diag.assign(n, -1);
+    for (int i = 0; i < n; i++) {
+        const int b0 = (i / brows) * brows, b1 = std::min(n, b0 + brows);
+        for (int q = rowptr[i]; q < rowptr[i + 1]; q++) {
+            const int j = cols[q];
+            if (j < b0 || j >= b1) continue;
+            if (j == i) bs.diag[i] = (int)bs.cols.size();
+            bs.cols.push_back(j - b0);
+            bs.src.push_back(q);
+        }
+        bs.rowptr[i + 1] = (int)bs.cols.size();
+    }
+    const int stride = brows + 1;
+    bs.fwd_rows.assign(n, 0);
+    bs.bwd_rows.assign(n, 0);
+    bs.fwd_lptr.assign((size_t)bs.nblk * stride, 0);
+    bs.bwd_lptr.assign((size_t)bs.nblk * stride, 0);
+    bs.fwd_nlev.assign(bs.nblk, 0);
+    bs.bwd_nlev.assign(bs.nblk, 0);
+    std::vector<int> lev(brows);
+    for (int b = 0; b < bs.nblk; b++) {
+        const int b0 = b * brows, nb = std::min(n, b0 + brows) - b0;
+        for (int dir = 0; dir < 2; dir++) {
+            int nlev = 0;
+            for (int t = 0; t < nb; t++) {
+                const int li = dir == 0 ? t : nb - 1 - t;
+                const int i = b0 + li;
+                int lv = 0;
+                for (int q = bs.rowptr[i]; q < bs.rowptr[i + 1]; q++) {
+                    const int lj = bs.cols[q];
+                    if ((dir == 0 && lj < li) || (dir == 1 && lj > li)) lv = std::max(lv, lev[lj] + 1);
+                }
+                lev[li] = lv;
+                nlev = std::max(nlev, lv + 1);
+            }
+            std::vector<int> cnt(nlev + 1, 0);
+            for (int li = 0; li < nb; li++) cnt[lev[li] + 1]++;
+            for (int l = 0; l < nlev; l++) cnt[l + 1] += cnt[l];
+            int *lptr = (dir == 0 ? bs.fwd_lptr.data() : bs.bwd_lptr.data()) + (size_t)b * stride;
+            int *rows = (dir == 0 ? bs.fwd_rows.data() : bs.bwd_rows.data()) + b0;
+            for (int l = 0; l <= nlev; l++) lptr[l] = cnt[l];
+            std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+            for (int li = 0; li < nb; li++) rows[pos[lev[li]]++] = li;
+            (dir == 0 ? bs.fwd_nlev : bs.bwd_nlev)[b] = nlev;
+        }
+    }
+}
+
+}  // namespace shy

@@ -1,0 +1,152 @@
+// shyhip_internal.h -- device-side data model of libshyhip.
+//
+// Device layout (chosen for coalescing on CDNA, not the reference's):
+//   element 3D   a[l*nel + ie]        (layer-major, element fastest)
+//   node 3D      a[l*nkn + k]
+//   interfaces   w[l*nkn + k], l = 0..nlvdi
+//   vertex       z[ii*nel + ie]
+//   geometry     geo[q*nel + ie], q = b1 b2 b3 c1 c2 c3 aj(=area/12)
+// Host push/pull transposes from/to the reference's (l,ie) layout.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/shyhip.h"
+
+namespace shy {
+
+constexpr float kFlag = -9988765.0f;                 // mkonst flag (const.f90:119)
+constexpr double kDrittl = (double)(1.0f / 3.0f);    // default-real 1./3. in double
+constexpr int kThreads = 256;
+constexpr int kMaxBlockRows = 1024;                  // block-ILU(0) rows per workgroup (LDS)
+
+// error codes written by kernels into Model::d_err[0]
+enum DevErr { ERR_NONE = 0, ERR_SETNOD = 1, ERR_SETUVD = 3, ERR_SETDEPTH = 4, ERR_VOLUME = 5,
+              ERR_CONZ = 7, ERR_PENTA = 8 };
+
+struct Mesh {                 // device copies of init-time data
+    int *nen3v = nullptr;     // [3*nel] element-major (ie*3+ii), 0-based
+    double *geo = nullptr;    // [7*nel]
+    double *ang = nullptr;    // [3*nel] vertex angles (ev 11-13)
+    double *dis = nullptr;    // [3*nel] edge lengths (ev 17-19)
+    int *ilhv = nullptr, *ilhkv = nullptr;
+    void *hm3v = nullptr;     // S [3*nel] vertex-major
+    void *hev = nullptr, *hlv = nullptr, *hldv = nullptr, *areakv = nullptr;
+    int *ibnd = nullptr, *zbound = nullptr, *iextb = nullptr, *ibtyp = nullptr;
+    void *fcorv = nullptr, *czv = nullptr;
+    int *nk_ptr = nullptr;    // node -> incident (element<<2 | vertex), element order
+    int *nk_el = nullptr;
+    int *nkc_el = nullptr;    // same lists in colour order
+    int *rowptr = nullptr, *cols = nullptr, *ijp = nullptr;   // zeta CSR; ijp[ie*9+i*3+j]
+    int *bnd_nodes = nullptr; int *bnd_id = nullptr; int nbnd_nodes = 0;
+    // block-ILU(0)
+    int *blk_rowptr = nullptr, *blk_cols = nullptr, *blk_src = nullptr, *blk_diag = nullptr;
+    int *fwd_rows = nullptr, *fwd_lptr = nullptr, *fwd_nlev = nullptr;
+    int *bwd_rows = nullptr, *bwd_lptr = nullptr, *bwd_nlev = nullptr;
+    // TVD upwind tables: per element 6 ordered pairs (ic,id); [p*nel + ie]
+    int *tvd_el = nullptr;    // upwind element or -1
+    int *tvd_kn = nullptr;    // [3][6][nel] its nodes
+    void *tvd_w = nullptr;    // S [3][6][nel] interpolation weights (adjusted xi)
+};
+
+struct State {
+    void *zn = nullptr, *zo = nullptr, *zen = nullptr, *zeo = nullptr;
+    void *utn = nullptr, *vtn = nullptr, *uto = nullptr, *vto = nullptr;
+    void *uln = nullptr, *vln = nullptr;
+    void *un = nullptr, *vn = nullptr, *uo = nullptr, *vo = nullptr;
+    void *wln = nullptr, *wlo = nullptr;
+    void *hdkn = nullptr, *hdko = nullptr, *hden = nullptr, *hdeo = nullptr;
+    int *iweg = nullptr, *iwet = nullptr, *inod = nullptr;
+    void *rfric = nullptr, *ddx = nullptr, *ddy = nullptr;
+    void *visv = nullptr, *difv = nullptr, *difhv = nullptr;
+    void *taux = nullptr, *tauy = nullptr, *rzv = nullptr, *rqv = nullptr, *mflux = nullptr;
+    std::vector<void *> tracer;
+    double *zel = nullptr;    // [6*nel] dbb dbc dcb dcc ut vt (hydro_zeta inputs)
+    double *aval = nullptr;   // [nnz] assembled zeta matrix (FP64, as c2coo)
+    double *rhs = nullptr;    // [nkn] (as rvec2d)
+    double *x = nullptr;      // [nkn] solution (FP64)
+    void *hbuf = nullptr;     // conz3d horizontal terms [l][ii][ie]
+};
+
+struct Solver {               // Krylov work space (value type V = S of the model)
+    int nblk = 0, brows = 0, max_blk_nnz = 0;
+    void *lu = nullptr;       // block factors (values of the block pattern)
+    void *r0 = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr, *v0 = nullptr, *v1 = nullptr,
+         *s = nullptr, *t = nullptr, *dx = nullptr, *b = nullptr, *x0 = nullptr;
+    double *part = nullptr;   // [4*max(nblk, nblocks(nkn))] per-workgroup partial sums
+    double *scal = nullptr;   // [16] device scalars
+    int *ctrl = nullptr;      // [8] ticket counter, flags, matvec counter
+    void *a32 = nullptr;      // FP32 copy of the matrix values (FP32 path)
+    double *r64 = nullptr;    // FP64 residual (refinement)
+};
+
+struct Model {
+    int prec = SHYHIP_FP64;
+    int nkn = 0, nel = 0, nlvdi = 0, nlv = 0, ntr = 0;
+    int nl_tmpl = 1;          // kernel template depth (>= nlvdi)
+    shyhip_params p{};
+    hipStream_t stream = nullptr;
+    Mesh m;
+    State s;
+    Solver sv;
+    int nnz = 0, ncolor = 0, nbnd = 0;
+    int *d_err = nullptr;     // [8] device error words + counters
+    int *h_pinned = nullptr;  // pinned host mirror for small readbacks
+    double dtime = 0., dtime_old = -1.;
+    bool has_coriolis = false, has_mflux = false, tvd_ready = false;
+    int last_nloop = 0, last_iters = 0, last_refine = 0;
+    std::string err;
+    // host-side copies needed for init / plugin / checks
+    std::vector<int> h_nen3v, h_rowptr, h_cols, h_ijp, h_color, h_ilhv, h_ilhkv, h_ibnd, h_ibtyp, h_iextb;
+    std::vector<float> h_x, h_y;
+    std::vector<double> h_ev;             // [19*nel] reference ev layout
+    std::vector<double> h_coo, h_rvec;    // plugin accumulation (c2coo, rvec2d)
+    size_t esize() const { return prec == SHYHIP_FP64 ? 8 : 4; }
+};
+
+// host-side mesh preprocessing (shyhip_init.cpp)
+void host_set_ev(int nkn, int nel, const int *nen3v0, const float *x, const float *y, double *ev);
+int host_node_list(int nkn, int nel, const int *nen3v0, std::vector<int> &ntg, std::vector<int> &cols,
+                   std::vector<int> &ijp);
+int host_color(int nkn, int nel, const int *nen3v0, std::vector<int> &color);
+void host_tvd_tables(int nkn, int nel, const int *nen3v0, const float *x, const float *y, const double *ev,
+                     std::vector<int> &tel, std::vector<int> &tkn, std::vector<double> &tw);
+struct BlockSchedule {
+    int nblk = 0, brows = 0;
+    std::vector<int> rowptr, cols, src, diag, fwd_rows, fwd_lptr, fwd_nlev, bwd_rows, bwd_lptr, bwd_nlev;
+};
+void host_block_schedule(int n, const std::vector<int> &rowptr, const std::vector<int> &cols, int brows,
+                         BlockSchedule &bs);
+
+// kernel launchers (shyhip_hydro.hip, shyhip_solver.hip, shyhip_conz.hip)
+int launch_set_dry(Model &M, int iweich, int bnewtime, int *changed);
+int launch_setnod(Model &M);
+int launch_hydro_intern(Model &M);
+int launch_zeta_assemble(Model &M);
+int launch_system_get_setweg1(Model &M, int bnewtime, int *changed);
+int launch_transports_final(Model &M);
+int launch_setzev_setuvd(Model &M);
+int launch_setdepth(Model &M, bool old_level);
+int launch_hydro_vertical(Model &M);
+int launch_ttov_uvint(Model &M);
+int launch_init_levels(Model &M, float zconst);
+int launch_copy_old(Model &M);
+int launch_set_rzv(Model &M, int ibc, float level);
+int launch_fill(Model &M, void *ptr, double value, size_t n);
+int launch_volume(Model &M, int mode, double *vol);
+int launch_tracer_mass(Model &M, int itr, double *mass);
+int solver_setup(Model &M);
+int solver_solve(Model &M, int *iters, int *refinements);
+int launch_conz3d(Model &M, int itr, int istot, int isact, int clip_small);
+
+// helpers
+int hip_check(Model &M, hipError_t e, const char *what);
+#define HIPCHK(M, call) do { if (hip_check((M), (call), #call)) return 1; } while (0)
+
+}  // namespace shy
+
+struct shyhip_model {
+    shy::Model M;
+};

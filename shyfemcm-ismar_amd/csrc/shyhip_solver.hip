@@ -1,0 +1,473 @@
+// shyhip_solver.hip -- zeta system solve on gfx950.
+//
+// Preconditioner: block-Jacobi ILU(0).  Rows are split into contiguous blocks
+// (<= 1024 rows, one workgroup each); each diagonal block is factorised and
+// solved in LDS with level scheduling (rows of one level in parallel,
+// workgroup barriers between levels).  A basin with at most one block of
+// nodes gets exactly ILU(0) (spk_ilut.f90:1832-1950, lusol :2309-2345).
+//
+// Krylov method: BiCGStab in the arrangement the reference uses -- runrc with
+// ipar(2)=1, which bisinit maps to LEFT preconditioning (spk_iters.f90:
+// 1061-1334, 3433-3500) -- with stopping test ||M^-1 r|| <= rtol*||M^-1 r0||
+// + atol and at most itermax matvecs.  One iteration is three kernels: every
+// vector update is fused into the SpMV/preconditioner kernel that consumes
+// it, and every dot product is a per-workgroup partial reduced by the last
+// workgroup to finish (deterministic fixed-order reduction, no spinning).
+// The FP32 path runs this in FP32 inside an FP64 iterative refinement.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "shyhip_internal.h"
+
+namespace shy {
+
+enum { SC_RHO = 0, SC_ALPHA = 1, SC_OMEGA = 2, SC_TARGET = 3, SC_RNORM = 4, SC_BETA = 5, SC_NORM0 = 6,
+       SC_RES64 = 7, SC_BNORM = 8 };
+enum { CT_TICKET = 0, CT_CONV = 1, CT_FAIL = 2, CT_NMV = 3, CT_REFCONV = 5 };
+
+template <typename V>
+struct SolvArgs {
+    int n, nblk, brows, itermax;
+    double rtol, atol;
+    const int *rowptr, *cols;
+    const V *A;
+    const int *brp, *bcol, *bsrc, *bdiag, *frows, *flptr, *fnlev, *brows_l, *blptr, *bnlev;
+    V *lu;
+    const double *aval;
+    V *b, *x0, *r0, *r, *p0, *p1, *v0, *v1, *s, *t, *dx;
+    double *part, *scal;
+    int *ctrl;
+};
+
+template <typename V>
+static SolvArgs<V> solv_args(Model &M, const V *A, V *b, V *x0) {
+    SolvArgs<V> a{};
+    a.n = M.nkn; a.nblk = M.sv.nblk; a.brows = M.sv.brows; a.itermax = M.p.itermax;
+    a.rtol = M.p.rtol; a.atol = M.p.atol;
+    a.rowptr = M.m.rowptr; a.cols = M.m.cols; a.A = A;
+    a.brp = M.m.blk_rowptr; a.bcol = M.m.blk_cols; a.bsrc = M.m.blk_src; a.bdiag = M.m.blk_diag;
+    a.frows = M.m.fwd_rows; a.flptr = M.m.fwd_lptr; a.fnlev = M.m.fwd_nlev;
+    a.brows_l = M.m.bwd_rows; a.blptr = M.m.bwd_lptr; a.bnlev = M.m.bwd_nlev;
+    a.lu = (V *)M.sv.lu; a.aval = M.s.aval;
+    a.b = b; a.x0 = x0;
+    a.r0 = (V *)M.sv.r0; a.r = (V *)M.sv.r; a.p0 = (V *)M.sv.p0; a.p1 = (V *)M.sv.p1;
+    a.v0 = (V *)M.sv.v0; a.v1 = (V *)M.sv.v1; a.s = (V *)M.sv.s; a.t = (V *)M.sv.t; a.dx = (V *)M.sv.dx;
+    a.part = M.sv.part; a.scal = M.sv.scal; a.ctrl = M.sv.ctrl;
+    return a;
+}
+
+// ---------------------------------------------------------------- reductions
+template <int NQ>
+__device__ inline void block_reduce_store(double (&vals)[NQ], double *part, int blk) {
+    __shared__ double red[NQ][kThreads];
+    for (int q = 0; q < NQ; q++) red[q][threadIdx.x] = vals[q];
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int q = 0; q < NQ; q++) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < NQ; q++) part[(size_t)blk * 4 + q] = red[q][0];
+}
+
+// returns true in the last workgroup to arrive; sums[] then holds the totals
+template <int NQ>
+__device__ inline bool last_block_sum(const double *part, int *ticket, double (&sums)[NQ]) {
+    __shared__ int is_last;
+    __shared__ double red[NQ][kThreads];
+    __threadfence();
+    if (threadIdx.x == 0) is_last = (atomicAdd(ticket, 1) == (int)gridDim.x - 1);
+    __syncthreads();
+    if (!is_last) return false;
+    const volatile double *vp = part;
+    double loc[NQ];
+    for (int q = 0; q < NQ; q++) loc[q] = 0.;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += kThreads)
+        for (int q = 0; q < NQ; q++) loc[q] += vp[(size_t)i * 4 + q];
+    for (int q = 0; q < NQ; q++) red[q][threadIdx.x] = loc[q];
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int q = 0; q < NQ; q++) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+        __syncthreads();
+    }
+    for (int q = 0; q < NQ; q++) sums[q] = red[q][0];
+    if (threadIdx.x == 0) *ticket = 0;
+    return true;
+}
+
+__device__ inline bool breakdown(double x) {   // brkdn: zero or not finite
+    if (x > 0.) return !(1. / x > 0.);
+    if (x < 0.) return !(1. / x < 0.);
+    return true;
+}
+
+// ---------------------------------------------------------------- block solve
+// y (LDS, local rows of block blk) <- (LU)^-1 y, in lusol's order
+template <typename V>
+__device__ inline void block_lusol(const SolvArgs<V> &a, int blk, int b0, V *y) {
+    const int stride = a.brows + 1;
+    const int *lp = a.flptr + (size_t)blk * stride;
+    const int nf = a.fnlev[blk];
+    for (int lv = 0; lv < nf; lv++) {
+        for (int t = lp[lv] + threadIdx.x; t < lp[lv + 1]; t += blockDim.x) {
+            const int li = a.frows[b0 + t], i = b0 + li;
+            V acc = y[li];
+            for (int q = a.brp[i]; q < a.bdiag[i]; q++) acc = acc - a.lu[q] * y[a.bcol[q]];
+            y[li] = acc;
+        }
+        __syncthreads();
+    }
+    const int *bp = a.blptr + (size_t)blk * stride;
+    const int nb = a.bnlev[blk];
+    for (int lv = 0; lv < nb; lv++) {
+        for (int t = bp[lv] + threadIdx.x; t < bp[lv + 1]; t += blockDim.x) {
+            const int li = a.brows_l[b0 + t], i = b0 + li;
+            V acc = y[li];
+            for (int q = a.bdiag[i] + 1; q < a.brp[i + 1]; q++) acc = acc - a.lu[q] * y[a.bcol[q]];
+            y[li] = a.lu[a.bdiag[i]] * acc;
+        }
+        __syncthreads();
+    }
+}
+
+// ILU(0) of each diagonal block (ilu0's IKJ arrangement; the pivot is stored
+// inverted), level by level in LDS.
+template <typename V>
+__global__ __launch_bounds__(kThreads) void k_bilu_factor(SolvArgs<V> a) {
+    extern __shared__ unsigned char smem[];
+    V *val = reinterpret_cast<V *>(smem);
+    const int blk = blockIdx.x, b0 = blk * a.brows;
+    const int nb = min(a.n, b0 + a.brows) - b0;
+    const int q0 = a.brp[b0], q1 = a.brp[b0 + nb];
+    for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) val[q - q0] = (V)a.aval[a.bsrc[q]];
+    __syncthreads();
+    const int stride = a.brows + 1;
+    const int *lp = a.flptr + (size_t)blk * stride;
+    const int nf = a.fnlev[blk];
+    for (int lv = 0; lv < nf; lv++) {
+        for (int t = lp[lv] + threadIdx.x; t < lp[lv + 1]; t += blockDim.x) {
+            const int li = a.frows[b0 + t], i = b0 + li;
+            const int ie0 = a.brp[i], ied = a.bdiag[i], ie1 = a.brp[i + 1];
+            for (int q = ie0; q < ied; q++) {
+                const int jrow = b0 + a.bcol[q];
+                const V tl = val[q - q0] * val[a.bdiag[jrow] - q0];
+                val[q - q0] = tl;
+                // subtract tl * (U part of row jrow) where row i has an entry
+                int qi = q + 1;
+                for (int jj = a.bdiag[jrow] + 1; jj < a.brp[jrow + 1]; jj++) {
+                    const int cj = a.bcol[jj];
+                    while (qi < ie1 && a.bcol[qi] < cj) qi++;
+                    if (qi < ie1 && a.bcol[qi] == cj) val[qi - q0] = val[qi - q0] - tl * val[jj - q0];
+                }
+            }
+            const V piv = val[ied - q0];
+            if (piv == (V)0) atomicCAS(a.ctrl + CT_FAIL, 0, 2000 + i);
+            val[ied - q0] = (V)1 / piv;
+        }
+        __syncthreads();
+    }
+    for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) a.lu[q] = val[q - q0];
+}
+
+// r0 = M^-1 (b - A x0); r = p = r0; rho = (r0,r0); target = rtol*|r0| + atol
+template <typename V>
+__global__ __launch_bounds__(kThreads) void k_bicg_init(SolvArgs<V> a) {
+    __shared__ V y[kMaxBlockRows];
+    const int blk = blockIdx.x, b0 = blk * a.brows;
+    const int nb = min(a.n, b0 + a.brows) - b0;
+    for (int li = threadIdx.x; li < nb; li += blockDim.x) {
+        const int i = b0 + li;
+        V t = 0;
+        for (int q = a.rowptr[i]; q < a.rowptr[i + 1]; q++) t = t + a.A[q] * a.x0[a.cols[q]];
+        y[li] = a.b[i] - t;
+    }
+    __syncthreads();
+    block_lusol(a, blk, b0, y);
+    double loc[1] = {0.};
+    for (int li = threadIdx.x; li < nb; li += blockDim.x) {
+        const int i = b0 + li;
+        const V r = y[li];
+        a.r0[i] = r; a.r[i] = r; a.p0[i] = r;
+        a.v0[i] = 0; a.dx[i] = 0;
+        loc[0] += (double)r * (double)r;
+    }
+    block_reduce_store<1>(loc, a.part, blk);
+    double sums[1];
+    if (last_block_sum<1>(a.part, a.ctrl + CT_TICKET, sums) && threadIdx.x == 0) {
+        const double rho = sums[0];
+        const double nrm = sqrt(rho);
+        a.scal[SC_RHO] = rho;
+        a.scal[SC_NORM0] = nrm;
+        a.scal[SC_RNORM] = nrm;
+        a.scal[SC_TARGET] = a.rtol * nrm + a.atol;
+        a.scal[SC_BETA] = 0.;
+        a.scal[SC_OMEGA] = 0.;
+        a.ctrl[CT_NMV] = 1;
+        a.ctrl[CT_CONV] = (nrm <= a.rtol * nrm + a.atol) ? 1 : 0;
+        a.ctrl[CT_FAIL] = 0;
+    }
+}
+
+// p = r + beta (p - omega v) (fused); v = M^-1 A p; alpha = rho / (r0,v)
+template <typename V>
+__global__ __launch_bounds__(kThreads) void k_bicg_pv(SolvArgs<V> a, int par) {
+    if (a.ctrl[CT_CONV] || a.ctrl[CT_FAIL]) return;
+    __shared__ V y[kMaxBlockRows];
+    const V *pold = par ? a.p1 : a.p0;
+    V *pnew = par ? a.p0 : a.p1;
+    const V *vold = par ? a.v1 : a.v0;
+    V *vnew = par ? a.v0 : a.v1;
+    const V beta = (V)a.scal[SC_BETA], omega = (V)a.scal[SC_OMEGA];
+    const int blk = blockIdx.x, b0 = blk * a.brows;
+    const int nb = min(a.n, b0 + a.brows) - b0;
+    for (int li = threadIdx.x; li < nb; li += blockDim.x) {
+        const int i = b0 + li;
+        V t = 0;
+        for (int q = a.rowptr[i]; q < a.rowptr[i + 1]; q++) {
+            const int j = a.cols[q];
+            const V pj = a.r[j] + beta * (pold[j] - omega * vold[j]);
+            t = t + a.A[q] * pj;
+        }
+        y[li] = t;
+        pnew[i] = a.r[i] + beta * (pold[i] - omega * vold[i]);
+    }
+    __syncthreads();
+    block_lusol(a, blk, b0, y);
+    double loc[1] = {0.};
+    for (int li = threadIdx.x; li < nb; li += blockDim.x) {
+        const int i = b0 + li;
+        vnew[i] = y[li];
+        loc[0] += (double)a.r0[i] * (double)y[li];
+    }
+    block_reduce_store<1>(loc, a.part, blk);
+    double sums[1];
+    if (last_block_sum<1>(a.part, a.ctrl + CT_TICKET, sums) && threadIdx.x == 0) {
+        a.ctrl[CT_NMV] += 1;
+        const double d = sums[0];
+        if (breakdown(d)) { a.ctrl[CT_FAIL] = -3; return; }
+        a.scal[SC_ALPHA] = a.scal[SC_RHO] / d;
+    }
+}
+
+// s = r - alpha v (fused); t = M^-1 A s; omega = (t,s)/(t,t)
+template <typename V>
+__global__ __launch_bounds__(kThreads) void k_bicg_st(SolvArgs<V> a, int par) {
+    if (a.ctrl[CT_CONV] || a.ctrl[CT_FAIL]) return;
+    __shared__ V y[kMaxBlockRows];
+    const V *vnew = par ? a.v0 : a.v1;
+    const V alpha = (V)a.scal[SC_ALPHA];
+    const int blk = blockIdx.x, b0 = blk * a.brows;
+    const int nb = min(a.n, b0 + a.brows) - b0;
+    for (int li = threadIdx.x; li < nb; li += blockDim.x) {
+        const int i = b0 + li;
+        V t = 0;
+        for (int q = a.rowptr[i]; q < a.rowptr[i + 1]; q++) {
+            const int j = a.cols[q];
+            const V sj = a.r[j] - alpha * vnew[j];
+            t = t + a.A[q] * sj;
+        }
+        y[li] = t;
+        a.s[i] = a.r[i] - alpha * vnew[i];
+    }
+    __syncthreads();
+    block_lusol(a, blk, b0, y);
+    double loc[2] = {0., 0.};
+    for (int li = threadIdx.x; li < nb; li += blockDim.x) {
+        const int i = b0 + li;
+        const V tv = y[li];
+        a.t[i] = tv;
+        loc[0] += (double)tv * (double)tv;
+        loc[1] += (double)tv * (double)a.s[i];
+    }
+    block_reduce_store<2>(loc, a.part, blk);
+    double sums[2];
+    if (last_block_sum<2>(a.part, a.ctrl + CT_TICKET, sums) && threadIdx.x == 0) {
+        a.ctrl[CT_NMV] += 1;
+        if (breakdown(sums[0])) { a.ctrl[CT_FAIL] = -3; return; }
+        const double om = sums[1] / sums[0];
+        if (breakdown(om)) { a.ctrl[CT_FAIL] = -3; return; }
+        a.scal[SC_OMEGA] = om;
+    }
+}
+
+// dx += alpha p + omega s; r = s - omega t; convergence test; beta
+template <typename V>
+__global__ __launch_bounds__(kThreads) void k_bicg_update(SolvArgs<V> a, int par) {
+    if (a.ctrl[CT_CONV] || a.ctrl[CT_FAIL]) return;
+    const V *pnew = par ? a.p0 : a.p1;
+    const V alpha = (V)a.scal[SC_ALPHA], omega = (V)a.scal[SC_OMEGA];
+    const int blk = blockIdx.x, b0 = blk * a.brows;
+    const int nb = min(a.n, b0 + a.brows) - b0;
+    double loc[2] = {0., 0.};
+    for (int li = threadIdx.x; li < nb; li += blockDim.x) {
+        const int i = b0 + li;
+        const V tmp = alpha * pnew[i] + omega * a.s[i];
+        a.dx[i] = a.dx[i] + tmp;
+        const V r = a.s[i] - omega * a.t[i];
+        a.r[i] = r;
+        loc[0] += (double)r * (double)r;
+        loc[1] += (double)r * (double)a.r0[i];
+    }
+    block_reduce_store<2>(loc, a.part, blk);
+    double sums[2];
+    if (last_block_sum<2>(a.part, a.ctrl + CT_TICKET, sums) && threadIdx.x == 0) {
+        if (a.itermax > 0 && a.ctrl[CT_NMV] >= a.itermax) { a.ctrl[CT_FAIL] = -1; return; }
+        const double rn = sqrt(sums[0]);
+        a.scal[SC_RNORM] = rn;
+        if (!(rn > a.scal[SC_TARGET])) { a.ctrl[CT_CONV] = 1; return; }
+        const double rho_old = a.scal[SC_RHO];
+        const double rho = sums[1];
+        a.scal[SC_BETA] = rho * a.scal[SC_ALPHA] / (a.scal[SC_OMEGA] * rho_old);
+        a.scal[SC_RHO] = rho;
+        if (breakdown(rho)) a.ctrl[CT_FAIL] = -3;
+    }
+}
+
+// x = x0 + dx (tidycg) into the FP64 solution
+template <typename V>
+__global__ void k_bicg_finish(const V *x0, const V *dx, double *x, int n, int add) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (add) x[i] = x[i] + (double)dx[i];
+    else x[i] = (double)(x0[i] + dx[i]);
+}
+
+__global__ void k_to_f32(const double *s, float *d, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = (float)s[i];
+}
+
+template <typename S>
+__global__ void k_guess(const S *zn, double *x, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = (double)zn[i];
+}
+
+// FP64 residual r64 = b - A x with |r64|^2 and |b|^2 (refinement driver)
+__global__ __launch_bounds__(kThreads) void k_resid64(const int *rowptr, const int *cols, const double *A,
+                                                       const double *b, const double *x, double *r64, float *r32,
+                                                       int n, double *part, double *scal, int *ctrl, double tol) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double loc[2] = {0., 0.};
+    if (i < n) {
+        double t = 0.;
+        for (int q = rowptr[i]; q < rowptr[i + 1]; q++) t = t + A[q] * x[cols[q]];
+        const double r = b[i] - t;
+        r64[i] = r;
+        r32[i] = (float)r;
+        loc[0] = r * r;
+        loc[1] = b[i] * b[i];
+    }
+    block_reduce_store<2>(loc, part, blockIdx.x);
+    double sums[2];
+    if (last_block_sum<2>(part, ctrl + CT_TICKET, sums) && threadIdx.x == 0) {
+        scal[SC_RES64] = sqrt(sums[0]);
+        scal[SC_BNORM] = sqrt(sums[1]);
+        ctrl[CT_REFCONV] = (sqrt(sums[0]) <= tol * sqrt(sums[1])) ? 1 : 0;
+    }
+}
+
+// ---------------------------------------------------------------- host driver
+static int read_ctrl(Model &M, int *out, int n) {
+    HIPCHK(M, hipMemcpyAsync(M.h_pinned, M.sv.ctrl, sizeof(int) * n, hipMemcpyDeviceToHost, M.stream));
+    HIPCHK(M, hipStreamSynchronize(M.stream));
+    for (int i = 0; i < n; i++) out[i] = M.h_pinned[i];
+    return 0;
+}
+
+template <typename V>
+static int bicgstab(Model &M, SolvArgs<V> &a, int *iters, int chunk) {
+    const int g = a.nblk;
+    hipLaunchKernelGGL(k_bicg_init<V>, dim3(g), dim3(kThreads), 0, M.stream, a);
+    HIPCHK(M, hipGetLastError());
+    int it = 0, c[4] = {0, 0, 0, 0};
+    const int maxit = M.p.itermax > 0 ? (M.p.itermax + 1) / 2 + 1 : 100000;
+    for (;;) {
+        for (int q = 0; q < chunk; q++, it++) {
+            const int par = it & 1;
+            hipLaunchKernelGGL(k_bicg_pv<V>, dim3(g), dim3(kThreads), 0, M.stream, a, par);
+            hipLaunchKernelGGL(k_bicg_st<V>, dim3(g), dim3(kThreads), 0, M.stream, a, par);
+            hipLaunchKernelGGL(k_bicg_update<V>, dim3(g), dim3(kThreads), 0, M.stream, a, par);
+        }
+        HIPCHK(M, hipGetLastError());
+        if (read_ctrl(M, c, 4)) return 1;
+        if (c[CT_CONV] || c[CT_FAIL] || it >= maxit) break;
+    }
+    if (iters) *iters = c[CT_NMV] / 2;
+    if (c[CT_FAIL] != 0) {
+        M.err = "zeta solver: BiCGStab failed (code " + std::to_string(c[CT_FAIL]) + ")";
+        return 2;
+    }
+    if (!c[CT_CONV]) {
+        M.err = "zeta solver: no convergence";
+        return 2;
+    }
+    return 0;
+}
+
+int solver_setup(Model &M) { (void)M; return 0; }
+
+int solver_solve(Model &M, int *iters, int *refinements) {
+    const int n = M.nkn;
+    const int g1 = (n + kThreads - 1) / kThreads;
+    int it_total = 0, nref = 0;
+    const size_t lds = (size_t)M.sv.max_blk_nnz * M.esize();
+    if (M.prec == SHYHIP_FP64) {
+        // initial guess = current level (system_solve passes znv); the Krylov
+        // x0 vector is the FP64 solution array itself
+        hipLaunchKernelGGL(k_guess<double>, dim3(g1), dim3(kThreads), 0, M.stream, (const double *)M.s.zn, M.s.x, n);
+        auto a = solv_args<double>(M, M.s.aval, M.s.rhs, M.s.x);
+        hipLaunchKernelGGL(k_bilu_factor<double>, dim3(M.sv.nblk), dim3(kThreads), lds, M.stream, a);
+        HIPCHK(M, hipGetLastError());
+        int its = 0;
+        const int rc = bicgstab<double>(M, a, &its, 4);
+        it_total = its;
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_bicg_finish<double>, dim3(g1), dim3(kThreads), 0, M.stream, (const double *)M.s.x,
+                           (const double *)a.dx, M.s.x, n, 0);
+        HIPCHK(M, hipGetLastError());
+    } else {
+        // FP32 BiCGStab on the FP32-rounded matrix inside FP64 refinement:
+        // x <- x + A32^-1 (b - A64 x) until |b - A64 x| <= refine_rtol |b|
+        hipLaunchKernelGGL(k_guess<float>, dim3(g1), dim3(kThreads), 0, M.stream, (const float *)M.s.zn, M.s.x, n);
+        float *a32 = (float *)M.sv.a32;
+        hipLaunchKernelGGL(k_to_f32, dim3((M.nnz + kThreads - 1) / kThreads), dim3(kThreads), 0, M.stream,
+                           (const double *)M.s.aval, a32, M.nnz);
+        float *b32 = (float *)M.sv.b, *x0 = (float *)M.sv.x0;
+        auto a = solv_args<float>(M, a32, b32, x0);
+        hipLaunchKernelGGL(k_bilu_factor<float>, dim3(M.sv.nblk), dim3(kThreads), lds, M.stream, a);
+        HIPCHK(M, hipGetLastError());
+        a.rtol = 1.0e-4;
+        a.atol = 0.0;
+        const int maxref = 12;
+        for (nref = 0; nref < maxref; nref++) {
+            hipLaunchKernelGGL(k_resid64, dim3(g1), dim3(kThreads), 0, M.stream, M.m.rowptr, M.m.cols,
+                               (const double *)M.s.aval, (const double *)M.s.rhs, (const double *)M.s.x, M.sv.r64,
+                               b32, n, M.sv.part, M.sv.scal, M.sv.ctrl, M.p.refine_rtol);
+            HIPCHK(M, hipGetLastError());
+            int c[6];
+            if (read_ctrl(M, c, 6)) return 1;
+            if (c[CT_REFCONV]) break;
+            HIPCHK(M, hipMemsetAsync(x0, 0, sizeof(float) * n, M.stream));
+            int its = 0;
+            const int rc = bicgstab<float>(M, a, &its, 4);
+            it_total += its;
+            if (rc == 1) return rc;     // runtime error; a stalled inner solve still refines
+            hipLaunchKernelGGL(k_bicg_finish<float>, dim3(g1), dim3(kThreads), 0, M.stream, (const float *)x0,
+                               (const float *)a.dx, M.s.x, n, 1);
+            HIPCHK(M, hipGetLastError());
+        }
+        if (nref >= maxref) {
+            M.err = "zeta solver: FP64 refinement did not reach the residual target";
+            return 2;
+        }
+    }
+    if (iters) *iters = it_total;
+    if (refinements) *refinements = nref;
+    return 0;
+}
+
+}  // namespace shy

@@ -1,0 +1,236 @@
+"""Python host-side driver of libshyhip.so (the gfx950 SHYFEM hydro step).
+
+Mirrors the reference's call surface for the time step -- ``hydro``
+(src/shyfem/hydrodynamic.f90:284), ``scal3sh``/``conz3d``
+(src/shyfem/concentration.f90:565, concentration_omp.f90:65) and the
+``system_*`` solver plugin (src/shyfem/simsys_spkaux.f90:85-721) -- through the
+C ABI declared in include/shyhip.h.  Arrays cross the boundary in the
+reference's own layout: ``a(l, ie)`` with the layer index fastest, which in
+numpy is a C-ordered array of shape ``(nel, nlvdi)``.
+
+The native library is required: importing works without it, but creating a
+:class:`Model` raises if ``libshyhip.so`` is missing or no GPU is visible.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIBPATH = os.path.join(PKG, "libshyhip.so")
+
+FP64, FP32 = 0, 1
+F32, F64, I32 = 0, 1, 2
+
+FIELDS = dict(znv=1, zov=2, zenv=3, zeov=4, utlnv=5, vtlnv=6, utlov=7, vtlov=8, unv=9, vnv=10, uov=11, vov=12,
+              ulnv=13, vlnv=14, wlnv=15, wlov=16, hdknv=17, hdkov=18, hdenv=19, hdeov=20, iwegv=21, iwetv=22,
+              inodv=23, rfricv=24, ddxv=25, ddyv=26, visv=27, difv=28, difhv=29, tauxnv=30, tauynv=31, rzv=32,
+              rqv=33, mfluxv=34, areakv=35, csr_val=36, rhs=37)
+TRACER0 = 100
+INT_FIELDS = {"iwegv", "iwetv", "inodv"}
+
+
+class ShyParams(C.Structure):
+    _fields_ = [("dt", C.c_double), ("azpar", C.c_float), ("ampar", C.c_float), ("afpar", C.c_float),
+                ("atpar", C.c_float), ("adpar", C.c_float), ("aapar", C.c_float), ("grav", C.c_float),
+                ("rowass", C.c_float), ("vismol", C.c_float), ("ireib", C.c_int), ("uvmin", C.c_float),
+                ("hzmin", C.c_float), ("hzoff", C.c_float), ("hzon", C.c_float), ("volmin", C.c_float),
+                ("iclin", C.c_int), ("rtol", C.c_double), ("atol", C.c_double), ("itermax", C.c_int),
+                ("refine_rtol", C.c_double), ("block_rows", C.c_int), ("rkpar", C.c_float),
+                ("difmol", C.c_float), ("itvd", C.c_int), ("itvdv", C.c_int)]
+
+
+DEFAULTS = dict(dt=300.0, azpar=0.5, ampar=0.5, afpar=0.5, atpar=1.0, adpar=1.0, aapar=0.0, grav=9.81,
+                rowass=1025.0, vismol=1.0e-6, ireib=5, uvmin=0.2, hzmin=0.01, hzoff=0.05, hzon=0.10, volmin=1.0,
+                iclin=0, rtol=float(np.float32(1.0e-8)), atol=float(np.float32(1.0e-8)), itermax=1000,
+                refine_rtol=1.0e-10, block_rows=0, rkpar=0.0, difmol=1.0e-6, itvd=0, itvdv=0)
+
+_lib = None
+
+
+class ShyhipError(RuntimeError):
+    pass
+
+
+def load_library(path: str = LIBPATH):
+    """Load libshyhip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ShyhipError(f"{path} is missing: build it with `make -C {PKG}` (or __graft_entry__.build())")
+    lib = C.CDLL(path)
+    lib.shyhip_last_error.restype = C.c_char_p
+    lib.shyhip_last_error.argtypes = [C.c_void_p]
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Model:
+    """One basin resident on the GPU, stepped through libshyhip."""
+
+    def __init__(self, basin, precision: str = "fp64", params: dict | None = None, czv: float = 2.5e-3,
+                 fcor: float = 0.0, visv: float = 0.0, difv: float = 0.0, zconst: float = 0.0,
+                 tracers: dict | None = None):
+        self.lib = lib = load_library()
+        self.basin = b = basin
+        self.prec = FP64 if precision == "fp64" else FP32
+        self.np_real = np.float64 if self.prec == FP64 else np.float32
+        p = dict(DEFAULTS)
+        if params:
+            p.update(params)
+        self.params_dict = p
+        self.tracer_names = list(tracers.keys()) if tracers else []
+        h = C.c_void_p()
+        self.nlvdi = b.nlv
+        rc = lib.shyhip_create(C.byref(h), self.prec, b.nkn, b.nel, b.nlv, b.nlv, len(self.tracer_names))
+        if rc:
+            raise ShyhipError(f"shyhip_create: {lib.shyhip_last_error(None).decode()}")
+        self.h = h
+        self._chk(lib.shyhip_set_params(h, C.byref(ShyParams(**p))), "set_params")
+        nen = np.ascontiguousarray(b.nen3v + 1, dtype=np.int32)
+        f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)  # noqa: E731
+        self._keep = [nen]
+        self._chk(lib.shyhip_set_basin(h, _p(nen), _p(f32(b.xgv)), _p(f32(b.ygv)), _p(f32(b.hm3v)), _p(f32(b.hev)),
+                                       _p(f32(b.hlv)), _p(f32(b.hldv)), _p(np.ascontiguousarray(b.ilhv, np.int32)),
+                                       _p(np.ascontiguousarray(b.ilhkv, np.int32))), "set_basin")
+        ibtyp = np.ascontiguousarray(b.ibtyp, dtype=np.int32)
+        self._chk(lib.shyhip_set_boundaries(h, len(ibtyp) - 1, _p(np.ascontiguousarray(b.ibnd, np.int32)),
+                                            _p(ibtyp), _p(np.ascontiguousarray(b.iextb, np.int32))), "set_boundaries")
+        self._chk(lib.shyhip_set_coefficients(h, _p(np.full(b.nel, fcor, np.float32)),
+                                              _p(np.full(b.nel, czv, np.float32))), "set_coefficients")
+        self.push("visv", np.full((b.nkn, b.nlv + 1), visv, np.float32))
+        self.push("difv", np.full((b.nkn, b.nlv + 1), difv, np.float32))
+        if p["itvd"] == 2:
+            self._chk(lib.shyhip_tvd_init(h), "tvd_init")
+        self._chk(lib.shyhip_init_state(h, C.c_float(zconst)), "init_state")
+        if tracers:
+            for i, (name, arr) in enumerate(tracers.items()):
+                self.push(TRACER0 + i, np.asarray(arr, dtype=np.float64))
+        self.t = 0.0
+
+    # ------------------------------------------------------------------
+    def _chk(self, rc, what):
+        if rc:
+            msg = self.lib.shyhip_last_error(self.h).decode()
+            raise ShyhipError(f"shyhip_{what} failed ({rc}): {msg}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.shyhip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _shape(self, name):
+        b = self.basin
+        L = self.nlvdi
+        shapes = dict(znv=(b.nkn,), zov=(b.nkn,), zenv=(b.nel, 3), zeov=(b.nel, 3), unv=(b.nel,), vnv=(b.nel,),
+                      uov=(b.nel,), vov=(b.nel,), wlnv=(b.nkn, L + 1), wlov=(b.nkn, L + 1), visv=(b.nkn, L + 1),
+                      difv=(b.nkn, L + 1), iwegv=(b.nel,), iwetv=(b.nel,), inodv=(b.nkn,), rfricv=(b.nel,),
+                      ddxv=(b.nel, 2 * L), ddyv=(b.nel, 2 * L), tauxnv=(b.nkn,), tauynv=(b.nkn,), rzv=(b.nkn,),
+                      rqv=(b.nkn,), hdknv=(b.nkn, L), hdkov=(b.nkn, L), mfluxv=(b.nkn, L), areakv=(b.nkn, L),
+                      rhs=(b.nkn,))
+        if name in shapes:
+            return shapes[name]
+        if name in ("utlnv", "vtlnv", "utlov", "vtlov", "ulnv", "vlnv", "hdenv", "hdeov", "difhv"):
+            return (b.nel, L)
+        if isinstance(name, int) and name >= TRACER0:
+            return (b.nkn, L)
+        raise KeyError(name)
+
+    def push(self, name, arr):
+        fid = name if isinstance(name, int) else FIELDS[name]
+        if name in INT_FIELDS:
+            a = np.ascontiguousarray(arr, dtype=np.int32)
+            dt = I32
+        else:
+            a = np.ascontiguousarray(arr, dtype=np.float64)
+            dt = F64
+        self._chk(self.lib.shyhip_push(self.h, fid, _p(a), dt, C.c_size_t(a.size)), f"push({name})")
+
+    def pull(self, name, dtype=np.float64):
+        if isinstance(name, str) and name in self.tracer_names:
+            fid = TRACER0 + self.tracer_names.index(name)
+            shape = (self.basin.nkn, self.nlvdi)
+        elif name == "csr_val":
+            fid, shape = FIELDS[name], (self.nnz(),)
+        else:
+            fid = name if isinstance(name, int) else FIELDS[name]
+            shape = self._shape(name)
+        if name in INT_FIELDS:
+            out = np.zeros(shape, dtype=np.int32)
+            self._chk(self.lib.shyhip_pull(self.h, fid, _p(out), I32, C.c_size_t(out.size)), f"pull({name})")
+            return out
+        out = np.zeros(shape, dtype=np.float64 if dtype == np.float64 else np.float32)
+        self._chk(self.lib.shyhip_pull(self.h, fid, _p(out), F64 if out.dtype == np.float64 else F32,
+                                       C.c_size_t(out.size)), f"pull({name})")
+        return out
+
+    def nnz(self):
+        n = C.c_int()
+        nc = C.c_int()
+        self._chk(self.lib.shyhip_get_pattern(self.h, C.byref(n), None, None, C.byref(nc), None), "get_pattern")
+        return n.value
+
+    def pattern(self):
+        n = self.nnz()
+        ntg = np.zeros(self.basin.nkn + 1, np.int32)
+        cols = np.zeros(n, np.int32)
+        color = np.zeros(self.basin.nel, np.int32)
+        nc = C.c_int()
+        self._chk(self.lib.shyhip_get_pattern(self.h, None, _p(ntg), _p(cols), C.byref(nc), _p(color)), "get_pattern")
+        return ntg, cols, nc.value, color
+
+    # ------------------------------------------------------------------ forcing / step
+    def set_boundary(self, level, ibc=1):
+        self._chk(self.lib.shyhip_set_boundary_level(self.h, ibc, C.c_float(level)), "set_boundary_level")
+
+    def set_wind(self, taux, tauy):
+        self._chk(self.lib.shyhip_set_wind_uniform(self.h, C.c_float(taux), C.c_float(tauy)), "set_wind_uniform")
+
+    def step(self, bnd_level=None, istot=1):
+        """set_dry, set_timestep, copy_uvz/copy_depth, sp111 level, hydro, scal3sh."""
+        if bnd_level is not None:
+            self.set_boundary(bnd_level)
+        self.t += self.params_dict["dt"]
+        self._chk(self.lib.shyhip_step(self.h, C.c_double(self.t), istot), "step")
+
+    def sync(self):
+        self._chk(self.lib.shyhip_sync(self.h), "sync")
+
+    def stats(self):
+        a, b_, c = C.c_int(), C.c_int(), C.c_int()
+        self._chk(self.lib.shyhip_stats(self.h, C.byref(a), C.byref(b_), C.byref(c)), "stats")
+        return dict(nloop=a.value, solver_iters=b_.value, refinements=c.value)
+
+    def volume(self, mode=1):
+        v = C.c_double()
+        self._chk(self.lib.shyhip_volume(self.h, mode, C.byref(v)), "volume")
+        return v.value
+
+    def tracer_mass(self, name):
+        v = C.c_double()
+        self._chk(self.lib.shyhip_tracer_mass(self.h, self.tracer_names.index(name), C.byref(v)), "tracer_mass")
+        return v.value
+
+
+def gpu_available() -> bool:
+    try:
+        lib = load_library()
+    except (OSError, ShyhipError):
+        return False
+    n = C.c_int(0)
+    lib.shyhip_device_count(C.byref(n))
+    return n.value > 0
