@@ -75,7 +75,9 @@ enum {
     SHYHIP_RFRICV, SHYHIP_DDXV, SHYHIP_DDYV,
     SHYHIP_VISV, SHYHIP_DIFV, SHYHIP_DIFHV,
     SHYHIP_TAUXNV, SHYHIP_TAUYNV, SHYHIP_RZV, SHYHIP_RQV, SHYHIP_MFLUXV,
-    SHYHIP_AREAKV, SHYHIP_CSR_VAL, SHYHIP_RHS, SHYHIP_TRACER0 = 100
+    SHYHIP_AREAKV, SHYHIP_CSR_VAL, SHYHIP_RHS,
+    SHYHIP_TRACER0 = 100,      /* tracer itr: SHYHIP_TRACER0 + itr, node 3D            */
+    SHYHIP_TRACER_BC0 = 200    /* its open-boundary value (-999 = none), node 3D        */
 };
 /* dtype of host buffers */
 enum { SHYHIP_F32 = 0, SHYHIP_F64 = 1, SHYHIP_I32 = 2 };

@@ -35,7 +35,8 @@ class OcMesh(C.Structure):
                 ("hlv", _P), ("hldv", _P), ("areakv", _P), ("ibnd", _P), ("ibtyp", _P), ("iextb", _P),
                 ("fcorv", _P), ("czv", _P), ("wdifhv", _P), ("ntg", _P), ("cols", _P), ("ijp", _P),
                 ("nnz", C.c_int), ("ncolor", C.c_int), ("color_ptr", _P), ("color_elems", _P),
-                ("tvd_ie", _P), ("tvd_x", _P), ("tvd_y", _P)]
+                ("tvd_ie", _P), ("tvd_x", _P), ("tvd_y", _P), ("nbndo", C.c_int), ("bndo_k", _P),
+                ("bndo_nx", _P), ("bndo_ny", _P), ("bndo_ptr", _P), ("bndo_nb", _P), ("bndo_w", _P)]
 
 
 _STATE_FIELDS = ["znv", "zov", "zenv", "zeov", "utlnv", "vtlnv", "utlov", "vtlov", "unv", "vnv", "uov",
@@ -157,6 +158,24 @@ class OracleModel:
                            ncolor=nc, color_ptr=ptr(self.color_ptr), color_elems=ptr(self.color_elems),
                            tvd_ie=ptr(self.tvd_ie), tvd_x=ptr(self.tvd_x), tvd_y=ptr(self.tvd_y))
         lib.oc_set_area(C.byref(self.mesh), ptr(self.areakv))
+        # scalar open-boundary tables (bndo_init)
+        nb = len(b.bnd_nodes)
+        maxnb = 64
+        self.bndo_k = np.zeros(max(nb, 1), np.int32)
+        self.bndo_nx = np.zeros(max(nb, 1), R)
+        self.bndo_ny = np.zeros(max(nb, 1), R)
+        self.bndo_ptr = np.zeros(nb + 1, np.int32)
+        self.bndo_nb = np.zeros(max(nb, 1) * maxnb, np.int32)
+        self.bndo_w = np.zeros(max(nb, 1) * maxnb, R)
+        bnodes = np.ascontiguousarray(b.bnd_nodes, np.int32)
+        if nb:
+            rc = lib.oc_bndo_init(C.byref(self.mesh), ptr(self.xgv), ptr(self.ygv), nb, ptr(bnodes),
+                                  ptr(self.bndo_k), ptr(self.bndo_nx), ptr(self.bndo_ny), ptr(self.bndo_ptr),
+                                  ptr(self.bndo_nb), ptr(self.bndo_w), maxnb)
+            assert rc == nb, "bndo_init failed"
+        self.mesh.nbndo = nb
+        for n in ("bndo_k", "bndo_nx", "bndo_ny", "bndo_ptr", "bndo_nb", "bndo_w"):
+            setattr(self.mesh, n, ptr(getattr(self, n)))
         if p["itvd"] == 2:
             lib.oc_tvd_init(C.byref(self.mesh), ptr(self.xgv), ptr(self.ygv), ptr(self.tvd_ie), ptr(self.tvd_x),
                             ptr(self.tvd_y))
@@ -187,9 +206,15 @@ class OracleModel:
         self.tracers = {}
         self.t = 0.0
         self._init(zconst)
+        self.tracer_bc = {}
         if tracers:
             for name, arr in tracers.items():
                 self.tracers[name] = np.ascontiguousarray(arr, dtype=R).copy()
+                # boundary values: the initial values on the open-boundary nodes,
+                # -999 (no value) elsewhere
+                bc = np.full((b.nkn, nlvdi), -999.0, dtype=R)
+                bc[b.bnd_nodes] = self.tracers[name][b.bnd_nodes]
+                self.tracer_bc[name] = bc
 
     # ---------------------------------------------------------------- init
     def _init(self, zconst):
@@ -261,7 +286,7 @@ class OracleModel:
         if rc != 0:
             raise RuntimeError(f"oracle hydro failed with code {rc}")
         for name, arr in self.tracers.items():
-            rc = lib.oc_scal3sh(m, s, p, ptr(arr), None, istot, 0 if name == "temp" else 1)
+            rc = lib.oc_scal3sh(m, s, p, ptr(arr), ptr(self.tracer_bc[name]), istot, 0 if name == "temp" else 1)
             if rc != 0:
                 raise RuntimeError(f"oracle scal3sh({name}) failed with code {rc}")
         return self.stats

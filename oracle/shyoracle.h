@@ -71,6 +71,13 @@ typedef struct {
     /* TVD upwind tables, index (id,ic,ie) -> (id-1) + 3*(ic-1) + 9*ie */
     const int *tvd_ie;     /* upwind element (0-based) or -1 */
     const real *tvd_x, *tvd_y;
+    /* scalar open-boundary tables (bndo_init) */
+    int nbndo;
+    const int *bndo_k;     /* [nbndo] open-boundary nodes */
+    const real *bndo_nx, *bndo_ny;   /* [nbndo] inward normal */
+    const int *bndo_ptr;   /* [nbndo+1] */
+    const int *bndo_nb;    /* inner neighbour nodes */
+    const real *bndo_w;    /* their weights */
 } oc_mesh;
 
 typedef struct {
@@ -169,8 +176,12 @@ int  oc_hydro(const oc_mesh *m, oc_state *s, const oc_params *p, oc_stats *st);
 int  oc_conz3d(const oc_mesh *m, const oc_state *s, const oc_params *p,
                real *cn1, const real *cbound, double ddt, int istot, int isact);
 int  oc_scal3sh(const oc_mesh *m, oc_state *s, const oc_params *p, real *cnv,
-                const real *rcv, int istot, int clip_small);
+                const real *rcv, int istot, int clip_small);   /* rcv: boundary values or NULL */
 double oc_massconc(const oc_mesh *m, const oc_state *s, const real *cnv, int mode);
+int  oc_bndo_init(const oc_mesh *m, const real *xgv, const real *ygv, int nb, const int *bnodes,
+                  int *bndo_k, real *nx, real *ny, int *ptr, int *nbr, real *w, int maxnb);
+void oc_uvtopr(const oc_mesh *m, const oc_state *s, real *uprv, real *vprv);
+int  oc_bndo_setbc(const oc_mesh *m, real *cv, const real *rbc, const real *uprv, const real *vprv);
 
 #ifdef __cplusplus
 }

@@ -352,6 +352,177 @@ static void make_scal_flux(const oc_mesh *m, const oc_state *s, const real *r3v,
     }
 }
 
+static int cmp_ll(const void *a, const void *b)
+{
+    long long x = *(const long long *)a, y = *(const long long *)b;
+    return (x > y) - (x < y);
+}
+
+/* bndo_init: bndo.f90:210-470.  Open-boundary nodes in boundary order
+ * (type 1 and 2 boundaries are "external"); the inward normal comes from the
+ * neighbouring boundary nodes along the basin boundary (kantv, built from the
+ * counter-clockwise node lists of connections.f90:594-612); the inner
+ * neighbours are the non-boundary vertices of elements touching the node,
+ * weighted by element area and normalised.  Returns nbndo or -1. */
+int oc_bndo_init(const oc_mesh *m, const real *xgv, const real *ygv, int nb, const int *bnodes,
+                 int *bndo_k, real *nx, real *ny, int *ptr, int *nbr, real *w, int maxnb)
+{
+    int nkn = m->nkn, nel = m->nel;
+    int *iop = (int *)calloc((size_t)nkn, sizeof(int));
+    int *knext = (int *)malloc(sizeof(int) * (size_t)nkn);
+    int *klast = (int *)malloc(sizeof(int) * (size_t)nkn);
+    for (int k = 0; k < nkn; k++) { knext[k] = -1; klast[k] = -1; }
+    /* boundary edges (one element only) in element orientation */
+    long long *key = (long long *)malloc(sizeof(long long) * 3 * (size_t)nel);
+    for (int ie = 0; ie < nel; ie++)
+        for (int ii = 0; ii < 3; ii++) {
+            long long a = m->nen3v[3 * ie + ii], c = m->nen3v[3 * ie + (ii + 1) % 3];
+            long long lo = a < c ? a : c, hi = a < c ? c : a;
+            key[3 * ie + ii] = lo * nkn + hi;
+        }
+    long long *sorted = (long long *)malloc(sizeof(long long) * 3 * (size_t)nel);
+    memcpy(sorted, key, sizeof(long long) * 3 * (size_t)nel);
+    qsort(sorted, 3 * (size_t)nel, sizeof(long long), cmp_ll);
+    for (int ie = 0; ie < nel; ie++)
+        for (int ii = 0; ii < 3; ii++) {
+            long long kk = key[3 * ie + ii];
+            size_t lo = 0, hi = 3 * (size_t)nel;
+            while (lo < hi) { size_t mid = (lo + hi) / 2; if (sorted[mid] < kk) lo = mid + 1; else hi = mid; }
+            int cnt = 0;
+            while (lo < 3 * (size_t)nel && sorted[lo] == kk) { cnt++; lo++; }
+            if (cnt == 1) {
+                int k1 = m->nen3v[3 * ie + ii], k2 = m->nen3v[3 * ie + (ii + 1) % 3];
+                knext[k1] = k2;
+                klast[k2] = k1;
+            }
+        }
+    free(key); free(sorted);
+    for (int i = 0; i < nb; i++) { bndo_k[i] = bnodes[i]; iop[bnodes[i]] = i + 1; }
+    for (int i = 0; i < nb; i++) {
+        int k = bnodes[i];
+        int kn = knext[k], kl = klast[k];
+        int inext = (kn >= 0) ? iop[kn] : 0, ilast = (kl >= 0) ? iop[kl] : 0;
+        real dx, dy;
+        if (inext > 0 && ilast > 0) { dx = xgv[kn] - xgv[kl]; dy = ygv[kn] - ygv[kl]; }
+        else if (inext > 0) { dx = xgv[kn] - xgv[k]; dy = ygv[kn] - ygv[k]; }
+        else if (ilast > 0) { dx = xgv[k] - xgv[kl]; dy = ygv[k] - ygv[kl]; }
+        else { free(iop); free(knext); free(klast); return -1; }
+        nx[i] = -dy;
+        ny[i] = dx;
+    }
+    /* neighbour lists in element order (bndinsert) */
+    int *cnt = (int *)calloc((size_t)nb, sizeof(int));
+    for (int i = 0; i < nb * maxnb; i++) { nbr[i] = -1; w[i] = 0; }
+    for (int ie = 0; ie < nel; ie++) {
+        real area = (real)(12. * m->ev[9 + 19 * ie]);
+        for (int ii = 0; ii < 3; ii++) {
+            int ib = iop[m->nen3v[3 * ie + ii]];
+            if (ib < 1) continue;
+            for (int iii = 0; iii < 3; iii++) {
+                int kn = m->nen3v[3 * ie + iii];
+                if (iop[kn] > 0) continue;
+                int *lst = nbr + (ib - 1) * maxnb;
+                real *wl = w + (ib - 1) * maxnb;
+                int j, found = 0;
+                for (j = 0; j < cnt[ib - 1]; j++)
+                    if (lst[j] == kn) { wl[j] = wl[j] + area; found = 1; break; }
+                if (!found) {
+                    if (cnt[ib - 1] >= maxnb) { free(iop); free(knext); free(klast); free(cnt); return -1; }
+                    lst[cnt[ib - 1]] = kn;
+                    wl[cnt[ib - 1]] = area;
+                    cnt[ib - 1]++;
+                }
+            }
+        }
+    }
+    /* compact into CSR and normalise */
+    ptr[0] = 0;
+    for (int i = 0; i < nb; i++) {
+        real tot = 0;
+        for (int j = 0; j < cnt[i]; j++) tot = tot + w[i * maxnb + j];
+        for (int j = 0; j < cnt[i]; j++) {
+            nbr[ptr[i] + j] = nbr[i * maxnb + j];
+            w[ptr[i] + j] = (tot > 0) ? w[i * maxnb + j] / tot : w[i * maxnb + j];
+        }
+        ptr[i + 1] = ptr[i] + cnt[i];
+    }
+    free(iop); free(knext); free(klast); free(cnt);
+    return nb;
+}
+
+/* uvtopr: transforms.f90:215-270 (nodal velocities of wet elements) */
+void oc_uvtopr(const oc_mesh *m, const oc_state *s, real *uprv, real *vprv)
+{
+    const int nlvdi = m->nlvdi;
+    size_t nn = (size_t)nlvdi * m->nkn;
+    real *vv = (real *)calloc(nn, sizeof(real));
+    memset(uprv, 0, sizeof(real) * nn);
+    memset(vprv, 0, sizeof(real) * nn);
+    for (int ie = 0; ie < m->nel; ie++) {
+        if (s->iwegv[ie] != 0) continue;
+        real aj = (real)EV(10, ie);
+        for (int l = 1; l <= m->ilhv[ie]; l++)
+            for (int ii = 0; ii < 3; ii++) {
+                int k = m->nen3v[3 * ie + ii];
+                A3K(vv, l, k) = A3K(vv, l, k) + aj;
+                A3K(uprv, l, k) = A3K(uprv, l, k) + aj * A3E(s->ulnv, l, ie);
+                A3K(vprv, l, k) = A3K(vprv, l, k) + aj * A3E(s->vlnv, l, ie);
+            }
+    }
+    for (size_t i = 0; i < nn; i++)
+        if (vv[i] > 0) { uprv[i] = uprv[i] / vv[i]; vprv[i] = vprv[i] / vv[i]; }
+    free(vv);
+}
+
+/* bndo_setbc = bndo_impbc + bndo_adjbc (bndo.f90:620-890) for level (type 1)
+ * boundaries: impose rbc where it is not the -999 flag, then on outflow
+ * (n . u <= 0) or ambient flag take the weighted mean of the inner
+ * neighbours.  Returns 0, or 1 on a node without neighbours. */
+int oc_bndo_setbc(const oc_mesh *m, real *cv, const real *rbc, const real *uprv, const real *vprv)
+{
+    const int nlvdi = m->nlvdi;
+    const real flag = (real)-999.;
+    for (int i = 0; i < m->nbndo; i++) {
+        int k = m->bndo_k[i];
+        int ibc = m->ibnd[k];
+        if (m->ibtyp[ibc] != 1) continue;
+        int lmax = m->ilhkv[k];
+        if (rbc)
+            for (int l = 1; l <= lmax; l++) {
+                real rb = A3K(rbc, l, k);
+                if (rb != flag) A3K(cv, l, k) = rb;
+            }
+        real dx = m->bndo_nx[i], dy = m->bndo_ny[i];
+        for (int l = 1; l <= lmax; l++) {
+            real scal = dx * A3K(uprv, l, k) + dy * A3K(vprv, l, k);
+            int bout = scal <= 0;
+            int bamb = A3K(cv, l, k) == flag;
+            real bc = 0, tweight = 0;
+            int ntbc;
+            if (bout || bamb) {
+                ntbc = 0;
+                for (int j = m->bndo_ptr[i]; j < m->bndo_ptr[i + 1]; j++) {
+                    int ko = m->bndo_nb[j];
+                    if (l <= m->ilhkv[ko]) {
+                        ntbc++;
+                        real weight = m->bndo_w[j];
+                        tweight = tweight + weight;
+                        bc = bc + weight * A3K(cv, l, ko);
+                    }
+                }
+            } else {
+                ntbc = m->bndo_ptr[i + 1] - m->bndo_ptr[i];
+                tweight = 1;
+                bc = A3K(cv, l, k);
+            }
+            if (ntbc > 0) A3K(cv, l, k) = bc / tweight;
+            else if (l > 1) A3K(cv, l, k) = A3K(cv, l - 1, k);
+            else return 1;
+        }
+    }
+    return 0;
+}
+
 /* scal3sh substep loop: concentration.f90:565-810 (istot given) */
 int oc_scal3sh(const oc_mesh *m, oc_state *s, const oc_params *p, real *cnv,
                const real *rcv, int istot, int clip_small)
@@ -360,6 +531,8 @@ int oc_scal3sh(const oc_mesh *m, oc_state *s, const oc_params *p, real *cnv,
     real *sbconz = (real *)calloc(nn, sizeof(real));
     int ret = 0;
     real ssurface = clip_small ? (real)0 : (real)-999.;
+    real *uprv = (real *)malloc(sizeof(real) * nn), *vprv = (real *)malloc(sizeof(real) * nn);
+    oc_uvtopr(m, s, uprv, vprv);                 /* make_prvel at the end of hydro */
     for (int isact = 1; isact <= istot; isact++) {
         make_scal_flux(m, s, rcv, cnv, ssurface, sbconz);
         if (clip_small)
@@ -367,7 +540,9 @@ int oc_scal3sh(const oc_mesh *m, oc_state *s, const oc_params *p, real *cnv,
                 if (cnv[i] < (real)1.e-15) cnv[i] = 0;
         ret = oc_conz3d(m, s, p, cnv, sbconz, (double)(real)p->dt, istot, isact);
         if (ret) break;
+        if (m->nbndo > 0 && oc_bndo_setbc(m, cnv, rcv, uprv, vprv)) { ret = 2; break; }
     }
+    free(uprv); free(vprv);
     free(sbconz);
     return ret;
 }

@@ -133,6 +133,8 @@ bool field_of(Model &M, int id, Field &f) {
     case SHYHIP_RHS: f = Field{(void **)&M.s.rhs, K_RHS, false, true}; return true;
     default:
         if (id >= SHYHIP_TRACER0 && id < SHYHIP_TRACER0 + M.ntr) return S(&M.s.tracer[id - SHYHIP_TRACER0], K_NODE3);
+        if (id >= SHYHIP_TRACER_BC0 && id < SHYHIP_TRACER_BC0 + M.ntr)
+            return S(&M.s.tracer_bc[id - SHYHIP_TRACER_BC0], K_NODE3);
         return false;
     }
 }
@@ -212,7 +214,11 @@ int shyhip_create(shyhip_model **out, int precision, int nkn, int nel, int nlvdi
     rc |= dalloc(M, &s.rhs, nk); rc |= dalloc(M, &s.x, nk);
     rc |= dalloc_s(M, &s.hbuf, 3 * L * ne);
     s.tracer.assign(ntracer, nullptr);
-    for (int t = 0; t < ntracer; t++) rc |= dalloc_s(M, &s.tracer[t], L * nk);
+    s.tracer_bc.assign(ntracer, nullptr);
+    for (int t = 0; t < ntracer; t++) {
+        rc |= dalloc_s(M, &s.tracer[t], L * nk);
+        rc |= dalloc_s(M, &s.tracer_bc[t], L * nk);
+    }
     rc |= dalloc(M, &M.d_err, 8);
     if (rc) {
         std::string e = M.err;
@@ -220,6 +226,8 @@ int shyhip_create(shyhip_model **out, int precision, int nkn, int nel, int nlvdi
         return fail(nullptr, "device allocation failed: " + e);
     }
     if (launch_fill(M, s.rzv, (double)kFlag, nk)) return fail(&M, M.err);
+    for (int t = 0; t < ntracer; t++)
+        if (launch_fill(M, s.tracer_bc[t], -999., L * nk)) return fail(&M, M.err);
     *out = h;
     return 0;
 }
@@ -235,6 +243,9 @@ int shyhip_destroy(shyhip_model *h) {
         dfree(q);
     dfree(s.iweg); dfree(s.iwet); dfree(s.inod); dfree(s.zel); dfree(s.aval); dfree(s.rhs); dfree(s.x);
     for (void *t : s.tracer) dfree(t);
+    for (void *t : s.tracer_bc) dfree(t);
+    dfree(M.m.bndo_k); dfree(M.m.bndo_ptr); dfree(M.m.bndo_nb); dfree(M.m.bndo_nx); dfree(M.m.bndo_ny);
+    dfree(M.m.bndo_w);
     Mesh &m = M.m;
     for (void *q : {(void *)m.nen3v, (void *)m.geo, (void *)m.ang, (void *)m.dis, (void *)m.ilhv, (void *)m.ilhkv,
                     m.hm3v, m.hev, m.hlv, m.hldv, m.areakv, (void *)m.ibnd, (void *)m.zbound, (void *)m.iextb,
@@ -452,6 +463,27 @@ int shyhip_set_boundaries(shyhip_model *h, int nbnd, const int *ibnd, const int 
     rc |= dalloc(M, &M.m.ibtyp, nbnd + 1) || upload(M, M.m.ibtyp, ibtyp, nbnd + 1);
     rc |= dalloc(M, &M.m.bnd_nodes, nodes.size()) || upload(M, M.m.bnd_nodes, nodes.data(), nodes.size());
     rc |= dalloc(M, &M.m.bnd_id, ids.size()) || upload(M, M.m.bnd_id, ids.data(), ids.size());
+    if (rc) return rc;
+    // scalar open-boundary tables for the external (type 1 and 2) boundaries
+    std::vector<int> bnodes;
+    for (int k = 0; k < nkn; k++) {
+        const int b = ibnd[k];
+        if (b > 0 && (ibtyp[b] == 1 || ibtyp[b] == 2)) bnodes.push_back(k);
+    }
+    std::vector<double> bnx, bny, bw;
+    std::vector<int> bptr, bnb;
+    if (host_bndo_tables(nkn, M.nel, M.h_nen3v.data(), M.h_x.data(), M.h_y.data(), M.h_ev.data(), bnodes, bnx, bny,
+                         bptr, bnb, bw, M.prec == SHYHIP_FP32) < 0)
+        return fail(&M, "open boundary with a single node is not permitted");
+    dfree(M.m.bndo_k); dfree(M.m.bndo_ptr); dfree(M.m.bndo_nb); dfree(M.m.bndo_nx); dfree(M.m.bndo_ny);
+    dfree(M.m.bndo_w);
+    M.m.nbndo = (int)bnodes.size();
+    rc |= dalloc(M, &M.m.bndo_k, bnodes.size()) || upload(M, M.m.bndo_k, bnodes.data(), bnodes.size());
+    rc |= dalloc(M, &M.m.bndo_ptr, bptr.size()) || upload(M, M.m.bndo_ptr, bptr.data(), bptr.size());
+    rc |= dalloc(M, &M.m.bndo_nb, bnb.size()) || upload(M, M.m.bndo_nb, bnb.data(), bnb.size());
+    rc |= dalloc_s(M, &M.m.bndo_nx, bnx.size()) || upload_s(M, M.m.bndo_nx, bnx.data(), bnx.size());
+    rc |= dalloc_s(M, &M.m.bndo_ny, bny.size()) || upload_s(M, M.m.bndo_ny, bny.data(), bny.size());
+    rc |= dalloc_s(M, &M.m.bndo_w, bw.size()) || upload_s(M, M.m.bndo_w, bw.data(), bw.size());
     return rc;
 }
 
@@ -659,8 +691,10 @@ int shyhip_scal3sh(shyhip_model *h, int itr, int istot, int clip_small) {
     if (itr < 0 || itr >= M.ntr) return fail(&M, "tracer index out of range");
     if (istot < 1 || istot > 1000) return fail(&M, "istot must be in 1..1000");
     if (M.p.itvd == 2 && !M.tvd_ready) return fail(&M, "itvd = 2 needs shyhip_tvd_init");
-    for (int isact = 1; isact <= istot; isact++)
+    for (int isact = 1; isact <= istot; isact++) {
         if (launch_conz3d(M, itr, istot, isact, clip_small)) return 1;
+        if (launch_bndo(M, itr)) return 1;           // bndo_setbc
+    }
     return 0;
 }
 

@@ -342,6 +342,107 @@ static int conz_t(Model &M, int itr, int istot, int isact, int clip) {
     }
 }
 
+// uvtopr (transforms.f90:215-270) at the open-boundary nodes followed by
+// bndo_setbc = bndo_impbc + bndo_adjbc (bndo.f90:620-890): impose the
+// boundary value, and on outflow take the weighted mean of the inner
+// neighbours.  One work-item per open-boundary node; it writes only its own
+// column and reads only inner nodes.
+template <typename S, int NL>
+__global__ __launch_bounds__(kThreads) void k_bndo(int nbndo, const int *bk, const S *bnx, const S *bny, const int *bptr,
+                                                   const int *bnb, const S *bw, const int *ibnd, const int *ibtyp,
+                                                   const int *ilhkv, const int *ilhv, const int *iweg,
+                                                   const int *nk_ptr, const int *nk_el, const double *geo, const S *uln,
+                                                   const S *vln, S *cv, const S *rbc, int nkn, int nel, int *err) {
+    using R = S;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nbndo) return;
+    const int k = bk[i];
+    if (ibtyp[ibnd[k]] != 1) return;
+    R vv[NL], up[NL], vp[NL];
+#pragma unroll
+    for (int l = 0; l < NL; l++) { vv[l] = 0; up[l] = 0; vp[l] = 0; }
+    for (int q = nk_ptr[k]; q < nk_ptr[k + 1]; q++) {
+        const int ie = nk_el[q] >> 2;
+        if (iweg[ie] != 0) continue;
+        const R aj = (R)geo[(size_t)6 * nel + ie];
+        const int lmax = ilhv[ie];
+#pragma unroll
+        for (int l = 1; l <= NL; l++) {
+            if (l > lmax) break;
+            vv[l - 1] = vv[l - 1] + aj;
+            up[l - 1] = up[l - 1] + aj * uln[(size_t)(l - 1) * nel + ie];
+            vp[l - 1] = vp[l - 1] + aj * vln[(size_t)(l - 1) * nel + ie];
+        }
+    }
+    const int lmax = ilhkv[k];
+    const R flag = (R)-999.;
+    if (rbc) {
+        for (int l = 1; l <= lmax; l++) {
+            const R rb = rbc[(size_t)(l - 1) * nkn + k];
+            if (rb != flag) cv[(size_t)(l - 1) * nkn + k] = rb;
+        }
+    }
+    const R dx = bnx[i], dy = bny[i];
+#pragma unroll
+    for (int l = 1; l <= NL; l++) {
+        if (l > lmax) break;
+        R u = up[l - 1], v = vp[l - 1];
+        if (vv[l - 1] > (R)0) { u = u / vv[l - 1]; v = v / vv[l - 1]; }
+        const R scal = dx * u + dy * v;
+        const size_t o = (size_t)(l - 1) * nkn + k;
+        const bool binside = scal <= (R)0 || cv[o] == flag;
+        R bc = 0, tw = 0;
+        int ntbc = 0;
+        if (binside) {
+            for (int j = bptr[i]; j < bptr[i + 1]; j++) {
+                const int ko = bnb[j];
+                if (l <= ilhkv[ko]) {
+                    ntbc++;
+                    tw = tw + bw[j];
+                    bc = bc + bw[j] * cv[(size_t)(l - 1) * nkn + ko];
+                }
+            }
+        } else {
+            ntbc = bptr[i + 1] - bptr[i];
+            tw = 1;
+            bc = cv[o];
+        }
+        if (ntbc > 0) cv[o] = bc / tw;
+        else if (l > 1) cv[o] = cv[o - nkn];
+        else atomicCAS(err, 0, ERR_CONZ);
+    }
+}
+
+template <typename S, int NL>
+static int bndo_nl(Model &M, int itr) {
+    if (M.m.nbndo == 0) return 0;
+    const int g = (M.m.nbndo + kThreads - 1) / kThreads;
+    hipLaunchKernelGGL((k_bndo<S, NL>), dim3(g), dim3(kThreads), 0, M.stream, M.m.nbndo, M.m.bndo_k,
+                       (const S *)M.m.bndo_nx, (const S *)M.m.bndo_ny, M.m.bndo_ptr, M.m.bndo_nb,
+                       (const S *)M.m.bndo_w, M.m.ibnd, M.m.ibtyp, M.m.ilhkv, M.m.ilhv, M.s.iweg, M.m.nk_ptr,
+                       M.m.nk_el, M.m.geo, (const S *)M.s.uln, (const S *)M.s.vln, (S *)M.s.tracer[itr],
+                       (const S *)M.s.tracer_bc[itr], M.nkn, M.nel, M.d_err);
+    HIPCHK(M, hipGetLastError());
+    return 0;
+}
+
+template <typename S>
+static int bndo_t(Model &M, int itr) {
+    switch (M.nl_tmpl) {
+    case 1: return bndo_nl<S, 1>(M, itr);
+    case 2: return bndo_nl<S, 2>(M, itr);
+    case 4: return bndo_nl<S, 4>(M, itr);
+    case 8: return bndo_nl<S, 8>(M, itr);
+    case 12: return bndo_nl<S, 12>(M, itr);
+    case 16: return bndo_nl<S, 16>(M, itr);
+    case 20: return bndo_nl<S, 20>(M, itr);
+    case 24: return bndo_nl<S, 24>(M, itr);
+    default: return bndo_nl<S, 32>(M, itr);
+    }
+}
+
+int launch_bndo(Model &M, int itr) { return M.prec == SHYHIP_FP64 ? bndo_t<double>(M, itr) : bndo_t<float>(M, itr); }
+
 int launch_conz3d(Model &M, int itr, int istot, int isact, int clip_small) {
     return M.prec == SHYHIP_FP64 ? conz_t<double>(M, itr, istot, isact, clip_small)
                                  : conz_t<float>(M, itr, istot, isact, clip_small);

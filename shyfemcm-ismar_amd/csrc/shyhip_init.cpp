@@ -292,6 +292,86 @@ void host_tvd_tables(int nkn, int nel, const int *nen3v0, const float *xg, const
     }
 }
 
+// Scalar open-boundary tables (bndo.f90:210-470): inward normal from the
+// neighbouring open-boundary nodes along the basin boundary (the boundary
+// neighbours of a node are the ends of its counter-clockwise node list,
+// connections.f90:594-612), and the inner vertices of the elements touching
+// the node with area weights, accumulated in element order and normalised.
+// Weights and normals are rounded like the reference's default reals when
+// the model runs in FP32.
+int host_bndo_tables(int nkn, int nel, const int *nen3v0, const float *x, const float *y, const double *ev,
+                     const std::vector<int> &bnodes, std::vector<double> &nx, std::vector<double> &ny,
+                     std::vector<int> &ptr, std::vector<int> &nbr, std::vector<double> &w, bool f32) {
+    auto rnd = [f32](double v) { return f32 ? (double)(float)v : v; };
+    std::vector<long long> key(3 * (size_t)nel);
+    for (int ie = 0; ie < nel; ie++)
+        for (int ii = 0; ii < 3; ii++) {
+            const long long a = nen3v0[3 * ie + ii], c = nen3v0[3 * ie + (ii + 1) % 3];
+            key[3 * (size_t)ie + ii] = std::min(a, c) * nkn + std::max(a, c);
+        }
+    std::vector<long long> sorted(key);
+    std::sort(sorted.begin(), sorted.end());
+    std::vector<int> knext(nkn, -1), klast(nkn, -1);
+    for (int ie = 0; ie < nel; ie++)
+        for (int ii = 0; ii < 3; ii++) {
+            const long long kk = key[3 * (size_t)ie + ii];
+            auto r = std::equal_range(sorted.begin(), sorted.end(), kk);
+            if (r.second - r.first == 1) {
+                knext[nen3v0[3 * ie + ii]] = nen3v0[3 * ie + (ii + 1) % 3];
+                klast[nen3v0[3 * ie + (ii + 1) % 3]] = nen3v0[3 * ie + ii];
+            }
+        }
+    const int nb = (int)bnodes.size();
+    std::vector<int> iop(nkn, 0);
+    for (int i = 0; i < nb; i++) iop[bnodes[i]] = i + 1;
+    nx.assign(nb, 0.);
+    ny.assign(nb, 0.);
+    for (int i = 0; i < nb; i++) {
+        const int k = bnodes[i], kn = knext[k], kl = klast[k];
+        const bool hn = kn >= 0 && iop[kn] > 0, hl = kl >= 0 && iop[kl] > 0;
+        double dx, dy;
+        if (hn && hl) { dx = rnd((double)x[kn] - x[kl]); dy = rnd((double)y[kn] - y[kl]); }
+        else if (hn) { dx = rnd((double)x[kn] - x[k]); dy = rnd((double)y[kn] - y[k]); }
+        else if (hl) { dx = rnd((double)x[k] - x[kl]); dy = rnd((double)y[k] - y[kl]); }
+        else return -1;                       // one-node open boundary
+        nx[i] = -dy;
+        ny[i] = dx;
+    }
+    std::vector<std::vector<int>> ln(nb);
+    std::vector<std::vector<double>> lw(nb);
+    for (int ie = 0; ie < nel; ie++) {
+        const double area = rnd(12. * ev[19 * (size_t)ie + 9]);
+        for (int ii = 0; ii < 3; ii++) {
+            const int ib = iop[nen3v0[3 * ie + ii]];
+            if (ib < 1) continue;
+            for (int iii = 0; iii < 3; iii++) {
+                const int kn = nen3v0[3 * ie + iii];
+                if (iop[kn] > 0) continue;
+                auto &L = ln[ib - 1];
+                auto &W = lw[ib - 1];
+                size_t j = 0;
+                for (; j < L.size(); j++)
+                    if (L[j] == kn) break;
+                if (j < L.size()) W[j] = rnd(W[j] + area);
+                else { L.push_back(kn); W.push_back(area); }
+            }
+        }
+    }
+    ptr.assign(nb + 1, 0);
+    nbr.clear();
+    w.clear();
+    for (int i = 0; i < nb; i++) {
+        double tot = 0.;
+        for (double v : lw[i]) tot = rnd(tot + v);
+        for (size_t j = 0; j < ln[i].size(); j++) {
+            nbr.push_back(ln[i][j]);
+            w.push_back(tot > 0. ? rnd(lw[i][j] / tot) : lw[i][j]);
+        }
+        ptr[i + 1] = (int)nbr.size();
+    }
+    return nb;
+}
+
 // Block-Jacobi ILU(0): rows are split into contiguous blocks of `brows`;
 // couplings outside the diagonal block are dropped.  For each block the
 // lower and upper triangular level sets are built for a one-workgroup

@@ -48,7 +48,11 @@ struct Mesh {                 // device copies of init-time data
     // TVD upwind tables: per element 6 ordered pairs (ic,id); [p*nel + ie]
     int *tvd_el = nullptr;    // upwind element or -1
     int *tvd_kn = nullptr;    // [3][6][nel] its nodes
-    void *tvd_w = nullptr;    // S [3][6][nel] interpolation weights (adjusted xi)
+    void *tvd_w = nullptr;    // double [3][6][nel] interpolation weights (adjusted xi)
+    // scalar open boundary (bndo_init): nodes, inward normals, inner neighbours
+    int nbndo = 0;
+    int *bndo_k = nullptr, *bndo_ptr = nullptr, *bndo_nb = nullptr;
+    void *bndo_nx = nullptr, *bndo_ny = nullptr, *bndo_w = nullptr;   // S
 };
 
 struct State {
@@ -63,6 +67,7 @@ struct State {
     void *visv = nullptr, *difv = nullptr, *difhv = nullptr;
     void *taux = nullptr, *tauy = nullptr, *rzv = nullptr, *rqv = nullptr, *mflux = nullptr;
     std::vector<void *> tracer;
+    std::vector<void *> tracer_bc;   // boundary values per tracer (-999 = none)
     double *zel = nullptr;    // [6*nel] dbb dbc dcb dcc ut vt (hydro_zeta inputs)
     double *aval = nullptr;   // [nnz] assembled zeta matrix (FP64, as c2coo)
     double *rhs = nullptr;    // [nkn] (as rvec2d)
@@ -111,6 +116,9 @@ void host_set_ev(int nkn, int nel, const int *nen3v0, const float *x, const floa
 int host_node_list(int nkn, int nel, const int *nen3v0, std::vector<int> &ntg, std::vector<int> &cols,
                    std::vector<int> &ijp);
 int host_color(int nkn, int nel, const int *nen3v0, std::vector<int> &color);
+int host_bndo_tables(int nkn, int nel, const int *nen3v0, const float *x, const float *y, const double *ev,
+                     const std::vector<int> &bnodes, std::vector<double> &nx, std::vector<double> &ny,
+                     std::vector<int> &ptr, std::vector<int> &nbr, std::vector<double> &w, bool f32);
 void host_tvd_tables(int nkn, int nel, const int *nen3v0, const float *x, const float *y, const double *ev,
                      std::vector<int> &tel, std::vector<int> &tkn, std::vector<double> &tw);
 struct BlockSchedule {
@@ -140,6 +148,7 @@ int launch_tracer_mass(Model &M, int itr, double *mass);
 int solver_setup(Model &M);
 int solver_solve(Model &M, int *iters, int *refinements);
 int launch_conz3d(Model &M, int itr, int istot, int isact, int clip_small);
+int launch_bndo(Model &M, int itr);
 
 // helpers
 int hip_check(Model &M, hipError_t e, const char *what);

@@ -30,6 +30,7 @@ FIELDS = dict(znv=1, zov=2, zenv=3, zeov=4, utlnv=5, vtlnv=6, utlov=7, vtlov=8, 
               inodv=23, rfricv=24, ddxv=25, ddyv=26, visv=27, difv=28, difhv=29, tauxnv=30, tauynv=31, rzv=32,
               rqv=33, mfluxv=34, areakv=35, csr_val=36, rhs=37)
 TRACER0 = 100
+TRACER_BC0 = 200
 INT_FIELDS = {"iwegv", "iwetv", "inodv"}
 
 
@@ -113,7 +114,12 @@ class Model:
         self._chk(lib.shyhip_init_state(h, C.c_float(zconst)), "init_state")
         if tracers:
             for i, (name, arr) in enumerate(tracers.items()):
-                self.push(TRACER0 + i, np.asarray(arr, dtype=np.float64))
+                a = np.asarray(arr, dtype=np.float64)
+                self.push(TRACER0 + i, a)
+                # open-boundary values: the initial values on the boundary nodes
+                bc = np.full((b.nkn, b.nlv), -999.0)
+                bc[b.bnd_nodes] = a[b.bnd_nodes]
+                self.push(TRACER_BC0 + i, bc)
         self.t = 0.0
 
     # ------------------------------------------------------------------
