@@ -58,6 +58,9 @@ typedef struct {
     int itermax;                                /* max matvecs (1000)           */
     double refine_rtol;                         /* FP32 path: FP64 refinement residual target */
     int block_rows;                             /* block-ILU(0) block size (<= 1024), 0 = auto */
+    int tri_sweeps;                             /* 0: exact level-scheduled triangular solves;
+                                                   k > 0: k Jacobi sweeps per triangular factor */
+    double inner_rtol;                          /* FP32 path: relative tolerance of the inner solve */
     /* scalars */
     float rkpar, difmol;
     int itvd, itvdv;
@@ -136,6 +139,13 @@ int  shyhip_step(shyhip_model *h, double dtime, int istot);
 /* scal3sh for tracer itr (0-based); clip_small = 1 for non-temperature */
 int  shyhip_scal3sh(shyhip_model *h, int itr, int istot, int clip_small);
 int  shyhip_sync(shyhip_model *h);
+/* kernel-group timing with HIP events on the handle's stream (off by
+ * default; enabling it adds a synchronisation at the end of every step).
+ * id = 0 .. 14: set_dry, copy_uvz, hydro_intern, setnod, zeta_assemble,
+ * zeta_solve, system_get_setweg, transports_final, setuvd_baro2l, setdepth,
+ * hydro_vertical, ttov_uvint, conz3d_elem, conz3d_node, bndo_setbc */
+int  shyhip_set_timing(shyhip_model *h, int on);
+int  shyhip_get_timing(shyhip_model *h, int id, const char **name, double *total_ms, long *count);
 
 /* diagnostics ------------------------------------------------------------ */
 int  shyhip_volume(shyhip_model *h, int mode, double *vol);     /* FP64 total volume */

@@ -26,6 +26,36 @@ int hip_check(Model &M, hipError_t e, const char *what) {
     return 1;
 }
 
+void timer_begin(Model &M, int id) {
+    if (!M.timing) return;
+    Model::TEv t{id, nullptr, nullptr};
+    (void)hipEventCreate(&t.a);
+    (void)hipEventCreate(&t.b);
+    (void)hipEventRecord(t.a, M.stream);
+    M.tev.push_back(t);
+}
+
+void timer_end(Model &M) {
+    if (!M.timing || M.tev.empty()) return;
+    (void)hipEventRecord(M.tev.back().b, M.stream);
+}
+
+int timer_flush(Model &M) {
+    if (M.tev.empty()) return 0;
+    HIPCHK(M, hipStreamSynchronize(M.stream));
+    for (auto &t : M.tev) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+            M.t_ms[t.id] += ms;
+            M.t_cnt[t.id] += 1;
+        }
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    M.tev.clear();
+    return 0;
+}
+
 }  // namespace shy
 
 namespace {
@@ -197,7 +227,7 @@ int shyhip_create(shyhip_model **out, int precision, int nkn, int nel, int nlvdi
     p.grav = 9.81f; p.rowass = 1025.f; p.vismol = 1.0e-6f; p.ireib = 0; p.uvmin = 0.2f;
     p.hzmin = 0.01f; p.hzoff = 0.05f; p.hzon = 0.10f; p.volmin = 1.f; p.iclin = 0;
     p.rtol = (double)1.0e-8f; p.atol = (double)1.0e-8f; p.itermax = 1000; p.refine_rtol = 1e-10;
-    p.block_rows = 0; p.rkpar = 0.f; p.difmol = 1.0e-6f; p.itvd = 0; p.itvdv = 0;
+    p.block_rows = 0; p.tri_sweeps = 0; p.inner_rtol = 1e-6; p.rkpar = 0.f; p.difmol = 1.0e-6f; p.itvd = 0; p.itvdv = 0;
     const size_t ne = nel, nk = nkn, L = nlvdi;
     State &s = M.s;
     int rc = 0;
@@ -275,6 +305,8 @@ int shyhip_set_params(shyhip_model *h, const shyhip_params *p) {
     if (p->rkpar != 0.f) return fail(&M, "horizontal diffusion (rkpar != 0) is not supported by the device path");
     if (p->itvd == 1) return fail(&M, "itvd = 1 (gradient TVD) is not supported by the device path; use 0 or 2");
     if (p->block_rows < 0 || p->block_rows > kMaxBlockRows) return fail(&M, "block_rows must be in 0..1024");
+    if (p->tri_sweeps < 0 || p->tri_sweeps > 64) return fail(&M, "tri_sweeps must be in 0..64");
+    if (!(p->inner_rtol > 0. && p->inner_rtol < 1.)) return fail(&M, "inner_rtol must be in (0,1)");
     M.p = *p;
     return 0;
 }
@@ -417,7 +449,8 @@ int shyhip_set_basin(shyhip_model *h, const int *nen3v, const float *xgv, const 
         for (void **q : {&v.r0, &v.r, &v.p0, &v.p1, &v.v0, &v.v1, &v.s, &v.t, &v.dx, &v.b, &v.x0})
             rc |= dalloc_s(M, q, (size_t)nkn);
         const size_t nparts = std::max((size_t)bs.nblk, (size_t)(nkn + kThreads - 1) / kThreads) + 1;
-        rc |= dalloc(M, &v.part, 4 * nparts);
+        v.nparts = (int)nparts;
+        rc |= dalloc(M, &v.part, 8 * nparts);
         rc |= dalloc(M, &v.scal, 16);
         rc |= dalloc(M, &v.ctrl, 8);
         rc |= dalloc(M, &v.r64, (size_t)nkn);
@@ -659,29 +692,49 @@ int shyhip_hydro(shyhip_model *h, double dtime) {
     M.dtime = dtime;
     HIPCHK(M, hipMemsetAsync(M.d_err, 0, sizeof(int), M.stream));
     int iloop = 0, changed = 0, its_total = 0;
+    timer_begin(M, T_INTERN);
     if (launch_hydro_intern(M)) return 1;          // hydro_transports (invariant over the wet/dry loop)
+    timer_end(M);
     for (;;) {
         iloop++;
         if (iloop > 100) return fail(&M, "hydro: wet/dry loop did not settle", 3);
+        timer_begin(M, T_SETNOD);
         if (launch_setnod(M)) return 1;             // setnod + adjust_mass_flux
+        timer_end(M);
+        timer_begin(M, T_ZETA);
         if (launch_zeta_assemble(M)) return 1;      // system_init + hydro_zeta + system_add_rhs
+        timer_end(M);
         int its = 0;
+        timer_begin(M, T_SOLVE);
         const int rc = shyhip_system_solve_device(h, &its);
+        timer_end(M);
         its_total += its;
         if (rc) return rc;
         const int bnew = M.dtime != M.dtime_old;
+        timer_begin(M, T_WETDRY);
         if (launch_system_get_setweg1(M, bnew, &changed)) return 1;
+        timer_end(M);
         M.dtime_old = M.dtime;
         if (changed == 0) break;
     }
     M.last_nloop = iloop;
     M.last_iters = its_total;
     if (check_dev_err(M, "hydro loop")) return 3;
+    timer_begin(M, T_FINAL);
     if (launch_transports_final(M)) return 1;
+    timer_end(M);
+    timer_begin(M, T_SETUVD);
     if (launch_setzev_setuvd(M)) return 1;          // setzev + setuvd + baro2l
+    timer_end(M);
+    timer_begin(M, T_DEPTH);
     if (launch_setdepth(M, false)) return 1;        // make_new_depth (+ check_volume)
+    timer_end(M);
+    timer_begin(M, T_VERT);
     if (launch_hydro_vertical(M)) return 1;
+    timer_end(M);
+    timer_begin(M, T_TTOV);
     if (launch_ttov_uvint(M)) return 1;             // compute_velocities: ulnv/vlnv, unv/vnv
+    timer_end(M);
     return check_dev_err(M, "hydro");
 }
 
@@ -693,7 +746,9 @@ int shyhip_scal3sh(shyhip_model *h, int itr, int istot, int clip_small) {
     if (M.p.itvd == 2 && !M.tvd_ready) return fail(&M, "itvd = 2 needs shyhip_tvd_init");
     for (int isact = 1; isact <= istot; isact++) {
         if (launch_conz3d(M, itr, istot, isact, clip_small)) return 1;
+        timer_begin(M, T_BNDO);
         if (launch_bndo(M, itr)) return 1;           // bndo_setbc
+        timer_end(M);
     }
     return 0;
 }
@@ -701,11 +756,38 @@ int shyhip_scal3sh(shyhip_model *h, int itr, int istot, int clip_small) {
 int shyhip_step(shyhip_model *h, double dtime, int istot) {
     if (!h) return fail(nullptr, "null handle");
     Model &M = h->M;
+    timer_begin(M, T_SETDRY);
     if (shyhip_set_dry(h)) return 1;
+    timer_end(M);
+    timer_begin(M, T_COPY);
     if (launch_copy_old(M)) return 1;
+    timer_end(M);
     if (shyhip_hydro(h, dtime)) return M.err.empty() ? 1 : 3;
     for (int t = 0; t < M.ntr; t++)
         if (shyhip_scal3sh(h, t, istot, t == 0 ? 0 : 1)) return 1;
+    return timer_flush(M);
+}
+
+int shyhip_set_timing(shyhip_model *h, int on) {
+    if (!h) return fail(nullptr, "null handle");
+    Model &M = h->M;
+    if (timer_flush(M)) return 1;
+    M.timing = on != 0;
+    for (int i = 0; i < T_NTIMERS; i++) { M.t_ms[i] = 0.; M.t_cnt[i] = 0; }
+    return 0;
+}
+
+int shyhip_get_timing(shyhip_model *h, int id, const char **name, double *total_ms, long *count) {
+    static const char *names[T_NTIMERS] = {"set_dry", "copy_uvz", "hydro_intern", "setnod", "zeta_assemble",
+                                           "zeta_solve", "system_get_setweg", "transports_final", "setuvd_baro2l",
+                                           "setdepth", "hydro_vertical", "ttov_uvint", "conz3d_elem", "conz3d_node",
+                                           "bndo_setbc"};
+    if (!h) return fail(nullptr, "null handle");
+    if (id < 0 || id >= T_NTIMERS) return fail(&h->M, "timer id out of range");
+    if (timer_flush(h->M)) return 1;
+    if (name) *name = names[id];
+    if (total_ms) *total_ms = h->M.t_ms[id];
+    if (count) *count = h->M.t_cnt[id];
     return 0;
 }
 

@@ -320,9 +320,13 @@ static int conz_nl(Model &M, int itr, int istot, int isact, int clip) {
     a.hbuf = (S *)M.s.hbuf;
     a.err = M.d_err;
     const int ge = (M.nel + kThreads - 1) / kThreads, gn = (M.nkn + kThreads - 1) / kThreads;
+    timer_begin(M, T_CONZ_ELEM);
     hipLaunchKernelGGL(k_conz_elem<S>, dim3(ge), dim3(kThreads), 0, M.stream, a);
+    timer_end(M);
     HIPCHK(M, hipGetLastError());
+    timer_begin(M, T_CONZ_NODE);
     hipLaunchKernelGGL((k_conz_node<S, NL>), dim3(gn), dim3(kThreads), 0, M.stream, a);
+    timer_end(M);
     HIPCHK(M, hipGetLastError());
     return 0;
 }

@@ -76,16 +76,21 @@ struct State {
 };
 
 struct Solver {               // Krylov work space (value type V = S of the model)
-    int nblk = 0, brows = 0, max_blk_nnz = 0;
+    int nblk = 0, brows = 0, max_blk_nnz = 0, nparts = 0;
+    int last_its[8] = {};     // iteration count of the previous solve, per solve kind
     void *lu = nullptr;       // block factors (values of the block pattern)
     void *r0 = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr, *v0 = nullptr, *v1 = nullptr,
          *s = nullptr, *t = nullptr, *dx = nullptr, *b = nullptr, *x0 = nullptr;
-    double *part = nullptr;   // [4*max(nblk, nblocks(nkn))] per-workgroup partial sums
+    double *part = nullptr;   // [8*nparts] per-workgroup partial sums (4 regions x 2)
     double *scal = nullptr;   // [16] device scalars
     int *ctrl = nullptr;      // [8] ticket counter, flags, matvec counter
     void *a32 = nullptr;      // FP32 copy of the matrix values (FP32 path)
     double *r64 = nullptr;    // FP64 residual (refinement)
 };
+
+// kernel groups timed with HIP events when timing is enabled
+enum TimerId { T_SETDRY = 0, T_COPY, T_INTERN, T_SETNOD, T_ZETA, T_SOLVE, T_WETDRY, T_FINAL, T_SETUVD, T_DEPTH,
+               T_VERT, T_TTOV, T_CONZ_ELEM, T_CONZ_NODE, T_BNDO, T_NTIMERS };
 
 struct Model {
     int prec = SHYHIP_FP64;
@@ -109,7 +114,18 @@ struct Model {
     std::vector<double> h_ev;             // [19*nel] reference ev layout
     std::vector<double> h_coo, h_rvec;    // plugin accumulation (c2coo, rvec2d)
     size_t esize() const { return prec == SHYHIP_FP64 ? 8 : 4; }
+    // timing
+    bool timing = false;
+    struct TEv { int id; hipEvent_t a, b; };
+    std::vector<TEv> tev;
+    double t_ms[T_NTIMERS] = {};
+    long t_cnt[T_NTIMERS] = {};
 };
+
+// event marks around a launch group (no-op unless timing is enabled)
+void timer_begin(Model &M, int id);
+void timer_end(Model &M);
+int timer_flush(Model &M);
 
 // host-side mesh preprocessing (shyhip_init.cpp)
 void host_set_ev(int nkn, int nel, const int *nen3v0, const float *x, const float *y, double *ev);

@@ -11,24 +11,31 @@
 // 1061-1334, 3433-3500) -- with stopping test ||M^-1 r|| <= rtol*||M^-1 r0||
 // + atol and at most itermax matvecs.  One iteration is three kernels: every
 // vector update is fused into the SpMV/preconditioner kernel that consumes
-// it, and every dot product is a per-workgroup partial reduced by the last
-// workgroup to finish (deterministic fixed-order reduction, no spinning).
+// it, and every dot product is written as per-workgroup partials that every
+// workgroup of the next kernel reduces itself in the same fixed order -- no
+// atomics or device-scope fences, visibility comes from the kernel boundary,
+// and all workgroups (and all runs) see bitwise identical scalars.
 // The FP32 path runs this in FP32 inside an FP64 iterative refinement.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "shyhip_internal.h"
 
 namespace shy {
 
-enum { SC_RHO = 0, SC_ALPHA = 1, SC_OMEGA = 2, SC_TARGET = 3, SC_RNORM = 4, SC_BETA = 5, SC_NORM0 = 6,
-       SC_RES64 = 7, SC_BNORM = 8 };
-enum { CT_TICKET = 0, CT_CONV = 1, CT_FAIL = 2, CT_NMV = 3, CT_REFCONV = 5 };
+// scalar slots; rho/alpha/omega are double-buffered by iteration parity so a
+// kernel never overwrites a value that another workgroup of it still reads
+enum { SC_TARGET = 0, SC_NORM0 = 1, SC_RHO = 2, SC_ALPHA = 4, SC_OMEGA = 6, SC_RNORM = 8, SC_RES64 = 9,
+       SC_BNORM = 10 };
+enum { CT_CONV = 1, CT_FAIL = 2, CT_NMV = 3, CT_REFCONV = 5 };
+// partial-sum regions: [region][q][workgroup]
+enum { PR_A = 0, PR_B = 1, PR_C = 2, PR_R = 3 };
 
 template <typename V>
 struct SolvArgs {
-    int n, nblk, brows, itermax;
+    int n, nblk, brows, itermax, sweeps;
     double rtol, atol;
     const int *rowptr, *cols;
     const V *A;
@@ -38,12 +45,13 @@ struct SolvArgs {
     V *b, *x0, *r0, *r, *p0, *p1, *v0, *v1, *s, *t, *dx;
     double *part, *scal;
     int *ctrl;
+    int nparts;      // stride of one partial-sum array
 };
 
 template <typename V>
 static SolvArgs<V> solv_args(Model &M, const V *A, V *b, V *x0) {
     SolvArgs<V> a{};
-    a.n = M.nkn; a.nblk = M.sv.nblk; a.brows = M.sv.brows; a.itermax = M.p.itermax;
+    a.n = M.nkn; a.nblk = M.sv.nblk; a.brows = M.sv.brows; a.itermax = M.p.itermax; a.sweeps = M.p.tri_sweeps;
     a.rtol = M.p.rtol; a.atol = M.p.atol;
     a.rowptr = M.m.rowptr; a.cols = M.m.cols; a.A = A;
     a.brp = M.m.blk_rowptr; a.bcol = M.m.blk_cols; a.bsrc = M.m.blk_src; a.bdiag = M.m.blk_diag;
@@ -54,12 +62,14 @@ static SolvArgs<V> solv_args(Model &M, const V *A, V *b, V *x0) {
     a.r0 = (V *)M.sv.r0; a.r = (V *)M.sv.r; a.p0 = (V *)M.sv.p0; a.p1 = (V *)M.sv.p1;
     a.v0 = (V *)M.sv.v0; a.v1 = (V *)M.sv.v1; a.s = (V *)M.sv.s; a.t = (V *)M.sv.t; a.dx = (V *)M.sv.dx;
     a.part = M.sv.part; a.scal = M.sv.scal; a.ctrl = M.sv.ctrl;
+    a.nparts = M.sv.nparts;
     return a;
 }
 
 // ---------------------------------------------------------------- reductions
+// per-workgroup partials: part[(region*2 + q)*nparts + workgroup]
 template <int NQ>
-__device__ inline void block_reduce_store(double (&vals)[NQ], double *part, int blk) {
+__device__ inline void store_partials(double (&vals)[NQ], double *part, int region, int nparts) {
     __shared__ double red[NQ][kThreads];
     for (int q = 0; q < NQ; q++) red[q][threadIdx.x] = vals[q];
     __syncthreads();
@@ -69,23 +79,18 @@ __device__ inline void block_reduce_store(double (&vals)[NQ], double *part, int 
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int q = 0; q < NQ; q++) part[(size_t)blk * 4 + q] = red[q][0];
+        for (int q = 0; q < NQ; q++) part[((size_t)region * 2 + q) * nparts + blockIdx.x] = red[q][0];
 }
 
-// returns true in the last workgroup to arrive; sums[] then holds the totals
+// total of the partials of the previous kernel (n workgroups), computed by
+// every workgroup in the same order; the result is returned to all threads
 template <int NQ>
-__device__ inline bool last_block_sum(const double *part, int *ticket, double (&sums)[NQ]) {
-    __shared__ int is_last;
+__device__ inline void sum_partials(const double *part, int region, int nparts, int n, double (&out)[NQ]) {
     __shared__ double red[NQ][kThreads];
-    __threadfence();
-    if (threadIdx.x == 0) is_last = (atomicAdd(ticket, 1) == (int)gridDim.x - 1);
-    __syncthreads();
-    if (!is_last) return false;
-    const volatile double *vp = part;
     double loc[NQ];
     for (int q = 0; q < NQ; q++) loc[q] = 0.;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += kThreads)
-        for (int q = 0; q < NQ; q++) loc[q] += vp[(size_t)i * 4 + q];
+    for (int i = threadIdx.x; i < n; i += kThreads)
+        for (int q = 0; q < NQ; q++) loc[q] += part[((size_t)region * 2 + q) * nparts + i];
     for (int q = 0; q < NQ; q++) red[q][threadIdx.x] = loc[q];
     __syncthreads();
     for (int s = kThreads / 2; s > 0; s >>= 1) {
@@ -93,9 +98,8 @@ __device__ inline bool last_block_sum(const double *part, int *ticket, double (&
             for (int q = 0; q < NQ; q++) red[q][threadIdx.x] += red[q][threadIdx.x + s];
         __syncthreads();
     }
-    for (int q = 0; q < NQ; q++) sums[q] = red[q][0];
-    if (threadIdx.x == 0) *ticket = 0;
-    return true;
+    for (int q = 0; q < NQ; q++) out[q] = red[q][0];
+    __syncthreads();
 }
 
 __device__ inline bool breakdown(double x) {   // brkdn: zero or not finite
@@ -105,9 +109,60 @@ __device__ inline bool breakdown(double x) {   // brkdn: zero or not finite
 }
 
 // ---------------------------------------------------------------- block solve
-// y (LDS, local rows of block blk) <- (LU)^-1 y, in lusol's order
+// y (LDS, local rows of block blk) <- (LU)^-1 y.  Exact: level-scheduled
+// forward and backward substitution in lusol's order.  With a.sweeps = k > 0
+// each triangular factor is applied by k Jacobi sweeps instead
+// (x <- b - L x, then x <- D^-1 (y - U x)), double-buffered in LDS so the
+// result does not depend on the order in which wavefronts run.
 template <typename V>
-__device__ inline void block_lusol(const SolvArgs<V> &a, int blk, int b0, V *y) {
+__device__ inline void block_lusol(const SolvArgs<V> &a, int blk, int b0, V *y, V *w) {
+    const int nb = min(a.n, b0 + a.brows) - b0;
+    if (a.sweeps > 0) {
+        constexpr int RPT = kMaxBlockRows / kThreads;   // rows per work-item
+        V bl[RPT], nv[RPT];
+        // forward: (I + L) x = y, starting from x = y
+        int nr = 0;
+        for (int li = threadIdx.x; li < nb; li += blockDim.x, nr++) {
+            bl[nr] = y[li];
+            w[li] = y[li];
+        }
+        __syncthreads();
+        for (int sw = 1; sw < a.sweeps; sw++) {
+            nr = 0;
+            for (int li = threadIdx.x; li < nb; li += blockDim.x, nr++) {
+                const int i = b0 + li;
+                V acc = bl[nr];
+                for (int q = a.brp[i]; q < a.bdiag[i]; q++) acc = acc - a.lu[q] * w[a.bcol[q]];
+                nv[nr] = acc;
+            }
+            __syncthreads();
+            nr = 0;
+            for (int li = threadIdx.x; li < nb; li += blockDim.x, nr++) w[li] = nv[nr];
+            __syncthreads();
+        }
+        // backward: (D + U) x = w, starting from x = D^-1 w
+        nr = 0;
+        for (int li = threadIdx.x; li < nb; li += blockDim.x, nr++) {
+            const int i = b0 + li;
+            bl[nr] = w[li];
+            y[li] = a.lu[a.bdiag[i]] * w[li];
+        }
+        __syncthreads();
+        for (int sw = 1; sw < a.sweeps; sw++) {
+            nr = 0;
+            for (int li = threadIdx.x; li < nb; li += blockDim.x, nr++) {
+                const int i = b0 + li;
+                V acc = bl[nr];
+                for (int q = a.bdiag[i] + 1; q < a.brp[i + 1]; q++) acc = acc - a.lu[q] * y[a.bcol[q]];
+                nv[nr] = a.lu[a.bdiag[i]] * acc;
+            }
+            __syncthreads();
+            nr = 0;
+            for (int li = threadIdx.x; li < nb; li += blockDim.x, nr++) y[li] = nv[nr];
+            __syncthreads();
+        }
+        return;
+    }
     const int stride = a.brows + 1;
     const int *lp = a.flptr + (size_t)blk * stride;
     const int nf = a.fnlev[blk];
@@ -121,8 +176,8 @@ __device__ inline void block_lusol(const SolvArgs<V> &a, int blk, int b0, V *y) 
         __syncthreads();
     }
     const int *bp = a.blptr + (size_t)blk * stride;
-    const int nb = a.bnlev[blk];
-    for (int lv = 0; lv < nb; lv++) {
+    const int nbl = a.bnlev[blk];
+    for (int lv = 0; lv < nbl; lv++) {
         for (int t = bp[lv] + threadIdx.x; t < bp[lv + 1]; t += blockDim.x) {
             const int li = a.brows_l[b0 + t], i = b0 + li;
             V acc = y[li];
@@ -172,10 +227,11 @@ __global__ __launch_bounds__(kThreads) void k_bilu_factor(SolvArgs<V> a) {
     for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) a.lu[q] = val[q - q0];
 }
 
-// r0 = M^-1 (b - A x0); r = p = r0; rho = (r0,r0); target = rtol*|r0| + atol
+// r0 = M^-1 (b - A x0); r = p = r0; (r0,r0) into both slots of region C
 template <typename V>
 __global__ __launch_bounds__(kThreads) void k_bicg_init(SolvArgs<V> a) {
     __shared__ V y[kMaxBlockRows];
+    __shared__ V w[kMaxBlockRows];
     const int blk = blockIdx.x, b0 = blk * a.brows;
     const int nb = min(a.n, b0 + a.brows) - b0;
     for (int li = threadIdx.x; li < nb; li += blockDim.x) {
@@ -185,8 +241,8 @@ __global__ __launch_bounds__(kThreads) void k_bicg_init(SolvArgs<V> a) {
         y[li] = a.b[i] - t;
     }
     __syncthreads();
-    block_lusol(a, blk, b0, y);
-    double loc[1] = {0.};
+    block_lusol(a, blk, b0, y, w);
+    double loc[2] = {0., 0.};
     for (int li = threadIdx.x; li < nb; li += blockDim.x) {
         const int i = b0 + li;
         const V r = y[li];
@@ -194,33 +250,55 @@ __global__ __launch_bounds__(kThreads) void k_bicg_init(SolvArgs<V> a) {
         a.v0[i] = 0; a.dx[i] = 0;
         loc[0] += (double)r * (double)r;
     }
-    block_reduce_store<1>(loc, a.part, blk);
-    double sums[1];
-    if (last_block_sum<1>(a.part, a.ctrl + CT_TICKET, sums) && threadIdx.x == 0) {
-        const double rho = sums[0];
-        const double nrm = sqrt(rho);
-        a.scal[SC_RHO] = rho;
-        a.scal[SC_NORM0] = nrm;
-        a.scal[SC_RNORM] = nrm;
-        a.scal[SC_TARGET] = a.rtol * nrm + a.atol;
-        a.scal[SC_BETA] = 0.;
-        a.scal[SC_OMEGA] = 0.;
-        a.ctrl[CT_NMV] = 1;
-        a.ctrl[CT_CONV] = (nrm <= a.rtol * nrm + a.atol) ? 1 : 0;
-        a.ctrl[CT_FAIL] = 0;
-    }
+    loc[1] = loc[0];
+    store_partials<2>(loc, a.part, PR_C, a.nparts);
+    if (blk == 0 && threadIdx.x == 0) { a.ctrl[CT_CONV] = 0; a.ctrl[CT_FAIL] = 0; a.ctrl[CT_NMV] = 1; }
 }
 
-// p = r + beta (p - omega v) (fused); v = M^-1 A p; alpha = rho / (r0,v)
+// iteration it, first kernel: convergence test of the previous iteration
+// (stopbis), beta, p = r + beta (p - omega v) (fused), v = M^-1 A p, (r0,v)
 template <typename V>
-__global__ __launch_bounds__(kThreads) void k_bicg_pv(SolvArgs<V> a, int par) {
+__global__ __launch_bounds__(kThreads) void k_bicg_pv(SolvArgs<V> a, int it) {
     if (a.ctrl[CT_CONV] || a.ctrl[CT_FAIL]) return;
     __shared__ V y[kMaxBlockRows];
+    __shared__ V w[kMaxBlockRows];
+    const int par = it & 1;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    double rs[2];
+    sum_partials<2>(a.part, PR_C, a.nparts, a.nblk, rs);
+    const double rn = sqrt(rs[0]);
+    double target, beta = 0., omega = 0.;
+    if (it == 0) {
+        target = a.rtol * rn + a.atol;
+        if (lead) { a.scal[SC_TARGET] = target; a.scal[SC_NORM0] = rn; }
+    } else {
+        if (a.itermax > 0 && 1 + 2 * it >= a.itermax) {
+            if (lead) a.ctrl[CT_FAIL] = -1;
+            return;
+        }
+        target = a.scal[SC_TARGET];
+    }
+    if (lead) { a.scal[SC_RNORM] = rn; a.ctrl[CT_NMV] = 1 + 2 * it; }
+    if (!(rn > target)) {
+        if (lead) a.ctrl[CT_CONV] = 1;
+        return;
+    }
+    const double rho = rs[1];
+    if (it > 0) {
+        const int pp = (it - 1) & 1;
+        omega = a.scal[SC_OMEGA + pp];
+        beta = rho * a.scal[SC_ALPHA + pp] / (omega * a.scal[SC_RHO + pp]);
+        if (breakdown(rho)) {
+            if (lead) a.ctrl[CT_FAIL] = -3;
+            return;
+        }
+    }
+    if (lead) a.scal[SC_RHO + par] = rho;
     const V *pold = par ? a.p1 : a.p0;
     V *pnew = par ? a.p0 : a.p1;
     const V *vold = par ? a.v1 : a.v0;
     V *vnew = par ? a.v0 : a.v1;
-    const V beta = (V)a.scal[SC_BETA], omega = (V)a.scal[SC_OMEGA];
+    const V vb = (V)beta, vo = (V)omega;
     const int blk = blockIdx.x, b0 = blk * a.brows;
     const int nb = min(a.n, b0 + a.brows) - b0;
     for (int li = threadIdx.x; li < nb; li += blockDim.x) {
@@ -228,37 +306,41 @@ __global__ __launch_bounds__(kThreads) void k_bicg_pv(SolvArgs<V> a, int par) {
         V t = 0;
         for (int q = a.rowptr[i]; q < a.rowptr[i + 1]; q++) {
             const int j = a.cols[q];
-            const V pj = a.r[j] + beta * (pold[j] - omega * vold[j]);
+            const V pj = a.r[j] + vb * (pold[j] - vo * vold[j]);
             t = t + a.A[q] * pj;
         }
         y[li] = t;
-        pnew[i] = a.r[i] + beta * (pold[i] - omega * vold[i]);
+        pnew[i] = a.r[i] + vb * (pold[i] - vo * vold[i]);
     }
     __syncthreads();
-    block_lusol(a, blk, b0, y);
+    block_lusol(a, blk, b0, y, w);
     double loc[1] = {0.};
     for (int li = threadIdx.x; li < nb; li += blockDim.x) {
         const int i = b0 + li;
         vnew[i] = y[li];
         loc[0] += (double)a.r0[i] * (double)y[li];
     }
-    block_reduce_store<1>(loc, a.part, blk);
-    double sums[1];
-    if (last_block_sum<1>(a.part, a.ctrl + CT_TICKET, sums) && threadIdx.x == 0) {
-        a.ctrl[CT_NMV] += 1;
-        const double d = sums[0];
-        if (breakdown(d)) { a.ctrl[CT_FAIL] = -3; return; }
-        a.scal[SC_ALPHA] = a.scal[SC_RHO] / d;
-    }
+    store_partials<1>(loc, a.part, PR_A, a.nparts);
 }
 
-// s = r - alpha v (fused); t = M^-1 A s; omega = (t,s)/(t,t)
+// alpha = rho / (r0,v); s = r - alpha v (fused); t = M^-1 A s; (t,t), (t,s)
 template <typename V>
-__global__ __launch_bounds__(kThreads) void k_bicg_st(SolvArgs<V> a, int par) {
+__global__ __launch_bounds__(kThreads) void k_bicg_st(SolvArgs<V> a, int it) {
     if (a.ctrl[CT_CONV] || a.ctrl[CT_FAIL]) return;
     __shared__ V y[kMaxBlockRows];
+    __shared__ V w[kMaxBlockRows];
+    const int par = it & 1;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    double d[1];
+    sum_partials<1>(a.part, PR_A, a.nparts, a.nblk, d);
+    if (breakdown(d[0])) {
+        if (lead) a.ctrl[CT_FAIL] = -3;
+        return;
+    }
+    const double alpha = a.scal[SC_RHO + par] / d[0];
+    if (lead) a.scal[SC_ALPHA + par] = alpha;
     const V *vnew = par ? a.v0 : a.v1;
-    const V alpha = (V)a.scal[SC_ALPHA];
+    const V va = (V)alpha;
     const int blk = blockIdx.x, b0 = blk * a.brows;
     const int nb = min(a.n, b0 + a.brows) - b0;
     for (int li = threadIdx.x; li < nb; li += blockDim.x) {
@@ -266,14 +348,14 @@ __global__ __launch_bounds__(kThreads) void k_bicg_st(SolvArgs<V> a, int par) {
         V t = 0;
         for (int q = a.rowptr[i]; q < a.rowptr[i + 1]; q++) {
             const int j = a.cols[q];
-            const V sj = a.r[j] - alpha * vnew[j];
+            const V sj = a.r[j] - va * vnew[j];
             t = t + a.A[q] * sj;
         }
         y[li] = t;
-        a.s[i] = a.r[i] - alpha * vnew[i];
+        a.s[i] = a.r[i] - va * vnew[i];
     }
     __syncthreads();
-    block_lusol(a, blk, b0, y);
+    block_lusol(a, blk, b0, y, w);
     double loc[2] = {0., 0.};
     for (int li = threadIdx.x; li < nb; li += blockDim.x) {
         const int i = b0 + li;
@@ -282,23 +364,29 @@ __global__ __launch_bounds__(kThreads) void k_bicg_st(SolvArgs<V> a, int par) {
         loc[0] += (double)tv * (double)tv;
         loc[1] += (double)tv * (double)a.s[i];
     }
-    block_reduce_store<2>(loc, a.part, blk);
-    double sums[2];
-    if (last_block_sum<2>(a.part, a.ctrl + CT_TICKET, sums) && threadIdx.x == 0) {
-        a.ctrl[CT_NMV] += 1;
-        if (breakdown(sums[0])) { a.ctrl[CT_FAIL] = -3; return; }
-        const double om = sums[1] / sums[0];
-        if (breakdown(om)) { a.ctrl[CT_FAIL] = -3; return; }
-        a.scal[SC_OMEGA] = om;
-    }
+    store_partials<2>(loc, a.part, PR_B, a.nparts);
 }
 
-// dx += alpha p + omega s; r = s - omega t; convergence test; beta
+// omega = (t,s)/(t,t); dx += alpha p + omega s; r = s - omega t; (r,r), (r,r0)
 template <typename V>
-__global__ __launch_bounds__(kThreads) void k_bicg_update(SolvArgs<V> a, int par) {
+__global__ __launch_bounds__(kThreads) void k_bicg_update(SolvArgs<V> a, int it) {
     if (a.ctrl[CT_CONV] || a.ctrl[CT_FAIL]) return;
+    const int par = it & 1;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    double ts[2];
+    sum_partials<2>(a.part, PR_B, a.nparts, a.nblk, ts);
+    if (breakdown(ts[0])) {
+        if (lead) a.ctrl[CT_FAIL] = -3;
+        return;
+    }
+    const double om = ts[1] / ts[0];
+    if (breakdown(om)) {
+        if (lead) a.ctrl[CT_FAIL] = -3;
+        return;
+    }
+    if (lead) a.scal[SC_OMEGA + par] = om;
     const V *pnew = par ? a.p0 : a.p1;
-    const V alpha = (V)a.scal[SC_ALPHA], omega = (V)a.scal[SC_OMEGA];
+    const V alpha = (V)a.scal[SC_ALPHA + par], omega = (V)om;
     const int blk = blockIdx.x, b0 = blk * a.brows;
     const int nb = min(a.n, b0 + a.brows) - b0;
     double loc[2] = {0., 0.};
@@ -311,19 +399,7 @@ __global__ __launch_bounds__(kThreads) void k_bicg_update(SolvArgs<V> a, int par
         loc[0] += (double)r * (double)r;
         loc[1] += (double)r * (double)a.r0[i];
     }
-    block_reduce_store<2>(loc, a.part, blk);
-    double sums[2];
-    if (last_block_sum<2>(a.part, a.ctrl + CT_TICKET, sums) && threadIdx.x == 0) {
-        if (a.itermax > 0 && a.ctrl[CT_NMV] >= a.itermax) { a.ctrl[CT_FAIL] = -1; return; }
-        const double rn = sqrt(sums[0]);
-        a.scal[SC_RNORM] = rn;
-        if (!(rn > a.scal[SC_TARGET])) { a.ctrl[CT_CONV] = 1; return; }
-        const double rho_old = a.scal[SC_RHO];
-        const double rho = sums[1];
-        a.scal[SC_BETA] = rho * a.scal[SC_ALPHA] / (a.scal[SC_OMEGA] * rho_old);
-        a.scal[SC_RHO] = rho;
-        if (breakdown(rho)) a.ctrl[CT_FAIL] = -3;
-    }
+    store_partials<2>(loc, a.part, PR_C, a.nparts);
 }
 
 // x = x0 + dx (tidycg) into the FP64 solution
@@ -346,10 +422,10 @@ __global__ void k_guess(const S *zn, double *x, int n) {
     if (i < n) x[i] = (double)zn[i];
 }
 
-// FP64 residual r64 = b - A x with |r64|^2 and |b|^2 (refinement driver)
+// FP64 residual r64 = b - A x with partial |r64|^2 and |b|^2 (refinement)
 __global__ __launch_bounds__(kThreads) void k_resid64(const int *rowptr, const int *cols, const double *A,
                                                        const double *b, const double *x, double *r64, float *r32,
-                                                       int n, double *part, double *scal, int *ctrl, double tol) {
+                                                       int n, double *part, int nparts) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double loc[2] = {0., 0.};
     if (i < n) {
@@ -361,12 +437,17 @@ __global__ __launch_bounds__(kThreads) void k_resid64(const int *rowptr, const i
         loc[0] = r * r;
         loc[1] = b[i] * b[i];
     }
-    block_reduce_store<2>(loc, part, blockIdx.x);
-    double sums[2];
-    if (last_block_sum<2>(part, ctrl + CT_TICKET, sums) && threadIdx.x == 0) {
-        scal[SC_RES64] = sqrt(sums[0]);
-        scal[SC_BNORM] = sqrt(sums[1]);
-        ctrl[CT_REFCONV] = (sqrt(sums[0]) <= tol * sqrt(sums[1])) ? 1 : 0;
+    store_partials<2>(loc, part, PR_R, nparts);
+}
+
+__global__ __launch_bounds__(kThreads) void k_resid_check(const double *part, int nparts, int n, double *scal,
+                                                           int *ctrl, double tol) {
+    double rs[2];
+    sum_partials<2>(part, PR_R, nparts, n, rs);
+    if (threadIdx.x == 0) {
+        scal[SC_RES64] = sqrt(rs[0]);
+        scal[SC_BNORM] = sqrt(rs[1]);
+        ctrl[CT_REFCONV] = (sqrt(rs[0]) <= tol * sqrt(rs[1])) ? 1 : 0;
     }
 }
 
@@ -378,25 +459,31 @@ static int read_ctrl(Model &M, int *out, int n) {
     return 0;
 }
 
+// Iterations are enqueued in chunks between host checks of the device flags;
+// the first chunk is the iteration count of the previous solve of the same
+// kind (`slot`), so a converging solve usually needs a single round trip.
+// Kernels enqueued after convergence return immediately.
 template <typename V>
-static int bicgstab(Model &M, SolvArgs<V> &a, int *iters, int chunk) {
+static int bicgstab(Model &M, SolvArgs<V> &a, int *iters, int slot) {
     const int g = a.nblk;
     hipLaunchKernelGGL(k_bicg_init<V>, dim3(g), dim3(kThreads), 0, M.stream, a);
     HIPCHK(M, hipGetLastError());
     int it = 0, c[4] = {0, 0, 0, 0};
     const int maxit = M.p.itermax > 0 ? (M.p.itermax + 1) / 2 + 1 : 100000;
+    int chunk = std::max(2, M.sv.last_its[slot] + 1);
     for (;;) {
         for (int q = 0; q < chunk; q++, it++) {
-            const int par = it & 1;
-            hipLaunchKernelGGL(k_bicg_pv<V>, dim3(g), dim3(kThreads), 0, M.stream, a, par);
-            hipLaunchKernelGGL(k_bicg_st<V>, dim3(g), dim3(kThreads), 0, M.stream, a, par);
-            hipLaunchKernelGGL(k_bicg_update<V>, dim3(g), dim3(kThreads), 0, M.stream, a, par);
+            hipLaunchKernelGGL(k_bicg_pv<V>, dim3(g), dim3(kThreads), 0, M.stream, a, it);
+            hipLaunchKernelGGL(k_bicg_st<V>, dim3(g), dim3(kThreads), 0, M.stream, a, it);
+            hipLaunchKernelGGL(k_bicg_update<V>, dim3(g), dim3(kThreads), 0, M.stream, a, it);
         }
         HIPCHK(M, hipGetLastError());
         if (read_ctrl(M, c, 4)) return 1;
         if (c[CT_CONV] || c[CT_FAIL] || it >= maxit) break;
+        chunk = 2;
     }
-    if (iters) *iters = c[CT_NMV] / 2;
+    if (iters) *iters = (c[CT_NMV] - 1) / 2;
+    M.sv.last_its[slot] = (c[CT_NMV] - 1) / 2;
     if (c[CT_FAIL] != 0) {
         M.err = "zeta solver: BiCGStab failed (code " + std::to_string(c[CT_FAIL]) + ")";
         return 2;
@@ -423,7 +510,7 @@ int solver_solve(Model &M, int *iters, int *refinements) {
         hipLaunchKernelGGL(k_bilu_factor<double>, dim3(M.sv.nblk), dim3(kThreads), lds, M.stream, a);
         HIPCHK(M, hipGetLastError());
         int its = 0;
-        const int rc = bicgstab<double>(M, a, &its, 4);
+        const int rc = bicgstab<double>(M, a, &its, 0);
         it_total = its;
         if (rc) return rc;
         hipLaunchKernelGGL(k_bicg_finish<double>, dim3(g1), dim3(kThreads), 0, M.stream, (const double *)M.s.x,
@@ -440,20 +527,22 @@ int solver_solve(Model &M, int *iters, int *refinements) {
         auto a = solv_args<float>(M, a32, b32, x0);
         hipLaunchKernelGGL(k_bilu_factor<float>, dim3(M.sv.nblk), dim3(kThreads), lds, M.stream, a);
         HIPCHK(M, hipGetLastError());
-        a.rtol = 1.0e-4;
+        a.rtol = M.p.inner_rtol;
         a.atol = 0.0;
         const int maxref = 12;
         for (nref = 0; nref < maxref; nref++) {
             hipLaunchKernelGGL(k_resid64, dim3(g1), dim3(kThreads), 0, M.stream, M.m.rowptr, M.m.cols,
                                (const double *)M.s.aval, (const double *)M.s.rhs, (const double *)M.s.x, M.sv.r64,
-                               b32, n, M.sv.part, M.sv.scal, M.sv.ctrl, M.p.refine_rtol);
+                               b32, n, M.sv.part, M.sv.nparts);
+            hipLaunchKernelGGL(k_resid_check, dim3(1), dim3(kThreads), 0, M.stream, (const double *)M.sv.part,
+                               M.sv.nparts, g1, M.sv.scal, M.sv.ctrl, M.p.refine_rtol);
             HIPCHK(M, hipGetLastError());
             int c[6];
             if (read_ctrl(M, c, 6)) return 1;
             if (c[CT_REFCONV]) break;
             HIPCHK(M, hipMemsetAsync(x0, 0, sizeof(float) * n, M.stream));
             int its = 0;
-            const int rc = bicgstab<float>(M, a, &its, 4);
+            const int rc = bicgstab<float>(M, a, &its, 1 + std::min(nref, 6));
             it_total += its;
             if (rc == 1) return rc;     // runtime error; a stalled inner solve still refines
             hipLaunchKernelGGL(k_bicg_finish<float>, dim3(g1), dim3(kThreads), 0, M.stream, (const float *)x0,
