@@ -1,0 +1,114 @@
+"""CPU checks of the C ABI and the host-side package (no GPU compute).
+
+* libshyhip.so loads and exports every function include/shyhip.h declares;
+* the package's parameter block matches the header's struct layout;
+* the mesh generators produce the basins the workloads are defined on
+  (orientation, levels, boundary lists), deterministically per seed.
+"""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import shyhip
+from shyhip import mesh
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "shyhip.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    return sorted(set(re.findall(r"\b(shyhip_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(shyhip.LIBPATH):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "shyfemcm-ismar_amd")], check=True)
+    return shyhip.load_library()
+
+
+def test_header_declares_entry_points():
+    names = _declared()
+    for must in ("shyhip_create", "shyhip_hydro", "shyhip_hydro_zeta", "shyhip_hydro_transports",
+                 "shyhip_hydro_transports_final", "shyhip_hydro_vertical", "shyhip_scal3sh",
+                 "shyhip_system_init", "shyhip_system_assemble", "shyhip_system_add_rhs",
+                 "shyhip_system_solve", "shyhip_system_get"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol(lib):
+    missing = [n for n in _declared() if not hasattr(lib, n)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", shyhip.LIBPATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r"\bT (shyhip_\w+)", out))
+    assert set(_declared()) <= exported
+
+
+def test_params_struct_matches_header():
+    """Field order and types of shyhip_params in the header and in ctypes."""
+    text = open(HEADER).read()
+    body = re.search(r"typedef struct\s*\{(.*?)\}\s*shyhip_params;", text, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        typ, names = decl.split(None, 1)
+        for nm in names.split(","):
+            fields.append((nm.strip(), typ))
+    ct = [(n, t) for n, t in shyhip.ShyParams._fields_]
+    assert [n for n, _ in fields] == [n for n, _ in ct]
+    cmap = {"double": C.c_double, "int": C.c_int, "float": C.c_float}
+    for (n, t), (_, ctt) in zip(fields, ct):
+        assert cmap[t] is ctt, n
+
+
+def test_error_text_without_gpu(lib):
+    """Entry points report errors through return codes and the last-error
+    text, never by aborting; a null handle is rejected."""
+    lib.shyhip_step.argtypes = [C.c_void_p, C.c_double, C.c_int]
+    rc = lib.shyhip_step(None, 0.0, 1)
+    assert rc != 0
+    assert b"null handle" in lib.shyhip_last_error(None)
+
+
+def test_tidal_basin_shape():
+    b = mesh.tidal_basin()
+    assert (b.nkn, b.nel, b.nlv) == (861, 1600, 1)
+    x, y = b.xgv[b.nen3v].astype(float), b.ygv[b.nen3v].astype(float)
+    area2 = (x[:, 1] - x[:, 0]) * (y[:, 2] - y[:, 0]) - (x[:, 2] - x[:, 0]) * (y[:, 1] - y[:, 0])
+    assert np.allclose(area2, 250.0 ** 2)                    # counter-clockwise right triangles
+    assert np.all(b.xgv[b.bnd_nodes] == 0) and len(b.bnd_nodes) == 21
+    assert np.all((b.hm3v > 9.0) & (b.hm3v < 11.0))
+    b2 = mesh.tidal_basin()
+    assert np.array_equal(b.hm3v, b2.hm3v)
+
+
+def test_lagoon_levels_and_orientation():
+    b = mesh.lagoon(4000, seed=20261202, lx=12000.0, ly=6000.0, nlv=12, tile=16)
+    x, y = b.xgv[b.nen3v].astype(float), b.ygv[b.nen3v].astype(float)
+    area2 = (x[:, 1] - x[:, 0]) * (y[:, 2] - y[:, 0]) - (x[:, 2] - x[:, 0]) * (y[:, 1] - y[:, 0])
+    assert np.all(area2 > 0)
+    assert np.all((b.ilhv >= 1) & (b.ilhv <= 12))
+    # node levels are the maximum over the incident elements
+    lk = np.zeros(b.nkn, np.int64)
+    np.maximum.at(lk, b.nen3v.reshape(-1), np.repeat(b.ilhv, 3))
+    assert np.array_equal(lk, b.ilhkv)
+    # every vertex of a multi-layer element lies below the top of its last
+    # layer, so drying cannot empty a lower layer
+    multi = b.ilhv > 1
+    assert np.all(b.hm3v[multi].min(1) > b.hlv[b.ilhv[multi] - 2])
+    hmin = b.hm3v.min()
+    assert hmin >= 0.05 - 1e-6 and b.hm3v.max() <= 20.0 + 1e-6
+    assert len(b.bnd_nodes) > 0 and np.all(b.ibnd[b.bnd_nodes] == 1)
+    b2 = mesh.lagoon(4000, seed=20261202, lx=12000.0, ly=6000.0, nlv=12, tile=16)
+    assert np.array_equal(b.nen3v, b2.nen3v) and np.array_equal(b.hm3v, b2.hm3v)

@@ -1,0 +1,405 @@
+"""CPU checks that pin the oracle (oracle/) before it is trusted as the
+checker of the GPU path.
+
+Each test compares the oracle against something it does not restate itself:
+the reference's own known-answer band systems, numpy/scipy library routines,
+closed forms of the continuous problem, or exact discrete invariants
+(conservation, symmetry, maximum principle).  A plausible slip in the
+restatement -- a dropped term, a wrong sign or index, a transposed operand --
+fails at least one of them.
+"""
+import ctypes as C
+import dataclasses
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import pyoracle
+from shyhip import mesh
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+ptr = pyoracle.ptr
+
+
+@pytest.fixture(scope="module")
+def lib64():
+    return pyoracle.load("f64")
+
+
+def _closed(b):
+    """The same basin with every boundary closed."""
+    return dataclasses.replace(b, ibnd=np.zeros(b.nkn, np.int32), ibtyp=np.array([0], np.int32),
+                               bnd_nodes=np.zeros(0, np.int32))
+
+
+# ---------------------------------------------------------------- band solvers
+
+def test_penta_known_answer(lib64):
+    """penta_example_2 (penta.f90:376-420): x = 1..10."""
+    kat = json.load(open(os.path.join(GOLDEN, "penta_known_answers.json")))["penta_example_2"]
+    n = kat["n"]
+    val = np.array([kat["l2"], kat["l1"], kat["d"], kat["u1"], kat["u2"]], dtype=np.float64).T.copy()
+    assert lib64.oc_penta_fact(n, ptr(val)) == 0
+    r = np.array(kat["r"], dtype=np.float64)
+    s = np.zeros(n)
+    lib64.oc_penta_solve(n, ptr(val), ptr(r), ptr(s))
+    rms = np.sqrt(np.mean((s - np.array(kat["x"])) ** 2))                # band_check, penta.f90:482-500
+    assert rms < 1e-12
+
+
+def test_tria_known_answer(lib64):
+    """tri_example_1 (penta.f90:454-478): x = (1, 2), through the multi-RHS
+    tridiagonal solver used for one-layer columns."""
+    kat = json.load(open(os.path.join(GOLDEN, "penta_known_answers.json")))["tri_example_1"]
+    n = kat["n"]
+    v = np.array([kat["l1"], kat["d"], kat["u1"]], dtype=np.float64).T.copy()
+    r = np.array(kat["r"] * 3, dtype=np.float64)        # the same RHS three times
+    s = np.zeros(3 * n)
+    lib64.oc_tria_multi(n, 3, ptr(v), ptr(r), ptr(s))
+    for j in range(3):
+        assert np.allclose(s[j * n:(j + 1) * n], kat["x"], rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_penta_random_vs_solve_banded(lib64, seed):
+    """test_random (penta.f90:504-560): diagonally dominant random bands of
+    size 4..40 against LAPACK's banded solver."""
+    rng = np.random.default_rng(seed)
+    for _ in range(20):
+        n = int(rng.integers(4, 41))
+        val = rng.uniform(-1, 1, (n, 5))
+        val[:, 2] = np.abs(val).sum(1) * rng.choice([-1.0, 1.0], n)
+        val[0, :2] = 0.0
+        val[1, 0] = 0.0
+        val[-1, 3:] = 0.0
+        val[-2, 4] = 0.0
+        x = rng.uniform(-1, 1, n)
+        ab = np.zeros((5, n))                       # LAPACK banded storage, (2, 2) bands
+        for j in range(-2, 3):
+            for i in range(n):
+                if 0 <= i + j < n:
+                    ab[2 - j, i + j] = val[i, j + 2]
+        dense = np.zeros((n, n))
+        for j in range(-2, 3):
+            for i in range(n):
+                if 0 <= i + j < n:
+                    dense[i, i + j] = val[i, j + 2]
+        r = dense @ x
+        f = val.copy()
+        assert lib64.oc_penta_fact(n, ptr(f)) == 0
+        s = np.zeros(n)
+        lib64.oc_penta_solve(n, ptr(f), ptr(r), ptr(s))
+        ref = scipy.linalg.solve_banded((2, 2), ab, r)
+        assert np.allclose(s, ref, rtol=0, atol=1e-12)
+        assert np.allclose(s, x, rtol=0, atol=1e-11)
+
+
+# ---------------------------------------------------------------- geometry / topology
+
+def test_set_ev_closed_form(lib64):
+    """set_ev (evgeom.f90:167-349) on the lagoon mesh: b, c are the P1 shape
+    function gradients (b_i = dN_i/dx), sum to zero, reproduce a linear field's
+    gradient; ev(10)*12 is the element area; the angles sum to 180 deg."""
+    b = mesh.lagoon(600, seed=3, lx=3000.0, ly=1500.0, nlv=1, tile=8)
+    x = b.xgv.astype(np.float64)
+    y = b.ygv.astype(np.float64)
+    ev = np.zeros((b.nel, 19))
+    lib64.oc_set_ev(b.nkn, b.nel, ptr(np.ascontiguousarray(b.nen3v, np.int32)), ptr(x), ptr(y), ptr(ev))
+    xe, ye = x[b.nen3v], y[b.nen3v]
+    area = 0.5 * ((xe[:, 1] - xe[:, 0]) * (ye[:, 2] - ye[:, 0]) - (xe[:, 2] - xe[:, 0]) * (ye[:, 1] - ye[:, 0]))
+    assert np.all(area > 0)
+    assert np.allclose(ev[:, 9] * 12.0, area, rtol=1e-12)
+    bb, cc = ev[:, 3:6], ev[:, 6:9]
+    assert np.allclose(bb.sum(1), 0.0, atol=1e-15 * np.abs(bb).max())
+    assert np.allclose(cc.sum(1), 0.0, atol=1e-15 * np.abs(cc).max())
+    # gradient of f = 3x - 2y + 5 from nodal values
+    f = 3 * xe - 2 * ye + 5
+    assert np.allclose((bb * f).sum(1), 3.0, rtol=1e-9)
+    assert np.allclose((cc * f).sum(1), -2.0, rtol=1e-9)
+    # b_i, c_i against the inverse of the P1 interpolation matrix
+    for ie in range(0, b.nel, 97):
+        M = np.column_stack([np.ones(3), xe[ie], ye[ie]])
+        inv = np.linalg.inv(M)
+        assert np.allclose(bb[ie], inv[1], rtol=1e-8, atol=1e-14)
+        assert np.allclose(cc[ie], inv[2], rtol=1e-8, atol=1e-14)
+    assert np.allclose(ev[:, 10:13].sum(1), 180.0, atol=1e-9)
+
+
+def _edges(nen3v):
+    e = np.concatenate([nen3v[:, [0, 1]], nen3v[:, [1, 2]], nen3v[:, [2, 0]]])
+    e = np.sort(e, axis=1)
+    return np.unique(e, axis=0)
+
+
+def test_node_list_closed_form(lib64):
+    """construct_node_list / coo_init_new (coo_matrix.f90:51-404): row k is
+    the sorted set {k} + edge neighbours; nnz = sum(deg + 1); every element
+    matrix entry (i, j) maps to the slot holding column kn(j) in row kn(i)."""
+    b = mesh.lagoon(800, seed=5, lx=3000.0, ly=1500.0, nlv=1, tile=8)
+    nen = np.ascontiguousarray(b.nen3v, np.int32)
+    ntg = np.zeros(b.nkn + 1, np.int32)
+    maxnnz = 8 * b.nkn + 64
+    cols = np.zeros(maxnnz, np.int32)
+    ijp = np.zeros((b.nel, 3, 3), np.int32)
+    nnz = lib64.oc_node_list(b.nkn, b.nel, ptr(nen), ptr(ntg), ptr(cols), maxnnz, ptr(ijp))
+    e = _edges(nen)
+    adj = sp.coo_matrix((np.ones(2 * len(e)), (np.r_[e[:, 0], e[:, 1]], np.r_[e[:, 1], e[:, 0]])),
+                        shape=(b.nkn, b.nkn)).tocsr() + sp.identity(b.nkn, format="csr")
+    adj.sort_indices()
+    assert nnz == adj.nnz == 2 * len(e) + b.nkn
+    assert np.array_equal(ntg, adj.indptr)
+    assert np.array_equal(cols[:nnz], adj.indices)
+    for ie in range(b.nel):
+        for i in range(3):
+            for j in range(3):
+                slot = ijp[ie, j, i]
+                assert ntg[nen[ie, i]] <= slot < ntg[nen[ie, i] + 1]
+                assert cols[slot] == nen[ie, j]
+
+
+def test_color_invariants(lib64):
+    """domain_clusterization_ggu (subset.f90:371-711) and check_color
+    (subset.f90:671-707): every element has exactly one colour, no two
+    elements of one colour share a node, colour sizes are balanced to within
+    the greedy adjustment, and the partition lists every element once."""
+    b = mesh.lagoon(2000, seed=9, lx=6000.0, ly=3000.0, nlv=1, tile=16)
+    nen = np.ascontiguousarray(b.nen3v, np.int32)
+    color = np.zeros(b.nel, np.int32)
+    cptr = np.zeros(33, np.int32)
+    celems = np.zeros(b.nel, np.int32)
+    nc = lib64.oc_color(b.nkn, b.nel, ptr(nen), ptr(color), ptr(cptr), ptr(celems))
+    assert 3 <= nc <= 32
+    assert cptr[0] == 0 and cptr[nc] == b.nel
+    assert np.array_equal(np.sort(celems), np.arange(b.nel))
+    for c in range(nc):
+        els = celems[cptr[c]:cptr[c + 1]]
+        assert np.all(color[els] == c + 1)
+        nodes = nen[els].reshape(-1)
+        assert len(np.unique(nodes)) == len(nodes), f"colour {c + 1} shares a node"
+    # maximum node degree bounds the number of colours a greedy pass needs
+    deg = np.bincount(nen.reshape(-1), minlength=b.nkn)
+    assert nc <= 3 * deg.max() - 2
+    # a second run is identical (deterministic)
+    color2 = np.zeros(b.nel, np.int32)
+    lib64.oc_color(b.nkn, b.nel, ptr(nen), ptr(color2), ptr(np.zeros(33, np.int32)), ptr(np.zeros(b.nel, np.int32)))
+    assert np.array_equal(color, color2)
+
+
+# ---------------------------------------------------------------- sparse solver
+
+def _fem_matrix(seed=1, n_target=1200, dirichlet=0):
+    """A zeta-like system on a lagoon mesh: lumped mass + dt^2 g h stiffness,
+    assembled through the oracle's own CSR pattern (0-based CSR)."""
+    b = mesh.lagoon(n_target, seed=seed, lx=4000.0, ly=2000.0, nlv=1, tile=8)
+    x = b.xgv.astype(np.float64)
+    y = b.ygv.astype(np.float64)
+    nen = b.nen3v
+    xe, ye = x[nen], y[nen]
+    area = 0.5 * ((xe[:, 1] - xe[:, 0]) * (ye[:, 2] - ye[:, 0]) - (xe[:, 2] - xe[:, 0]) * (ye[:, 1] - ye[:, 0]))
+    bb = np.stack([ye[:, 1] - ye[:, 2], ye[:, 2] - ye[:, 0], ye[:, 0] - ye[:, 1]], 1) / (2 * area[:, None])
+    cc = np.stack([xe[:, 2] - xe[:, 1], xe[:, 0] - xe[:, 2], xe[:, 1] - xe[:, 0]], 1) / (2 * area[:, None])
+    k = 300.0 ** 2 * 9.81 * b.hev.astype(np.float64)
+    rows, cols, vals = [], [], []
+    for i in range(3):
+        for j in range(3):
+            rows.append(nen[:, i])
+            cols.append(nen[:, j])
+            vals.append(area / 3 * (i == j) + area * k * (bb[:, i] * bb[:, j] + cc[:, i] * cc[:, j]))
+    A = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(b.nkn, b.nkn)).tocsr()
+    A.sum_duplicates()
+    A.sort_indices()
+    return A
+
+
+def _msr_to_lu(n, alu, jlu, ju):
+    """L (unit lower) and U from SPARSKIT's modified sparse row LU storage
+    (1-based; alu(i) holds 1/U(i,i))."""
+    L = np.eye(n)
+    U = np.zeros((n, n))
+    for i in range(1, n + 1):
+        U[i - 1, i - 1] = 1.0 / alu[i]
+        for k in range(jlu[i], ju[i]):
+            L[i - 1, jlu[k] - 1] = alu[k]
+        for k in range(ju[i], jlu[i + 1]):
+            U[i - 1, jlu[k] - 1] = alu[k]
+    return L, U
+
+
+def test_ilu0_matches_pattern(lib64):
+    """ilu0 (spk_ilut.f90:1788-1885): (L U)_ij = A_ij on every position of
+    A's pattern, and L, U live on A's pattern only."""
+    A = _fem_matrix(n_target=250)
+    n = A.shape[0]
+    a = A.data.copy()
+    ja = A.indices.astype(np.int32)
+    ia = A.indptr.astype(np.int32)
+    size = A.nnz + n + 2
+    alu = np.zeros(size)
+    jlu = np.zeros(size, np.int32)
+    ju = np.zeros(n + 2, np.int32)
+    assert lib64.oc_ilu0(n, ptr(a), ptr(ja), ptr(ia), ptr(alu), ptr(jlu), ptr(ju)) == 0
+    L, U = _msr_to_lu(n, alu, jlu, ju)
+    LU = L @ U
+    Ad = A.toarray()
+    mask = Ad != 0
+    assert np.allclose(LU[mask], Ad[mask], rtol=1e-12, atol=1e-12 * np.abs(Ad).max())
+    assert np.all((L - np.eye(n))[~mask] == 0) and np.all(U[~mask] == 0)
+    # lusol applies (LU)^-1
+    rhs = np.random.default_rng(0).uniform(-1, 1, n)
+    z = np.zeros(n)
+    lib64.oc_lusol(n, ptr(rhs), ptr(z), ptr(alu), ptr(jlu), ptr(ju))
+    assert np.allclose(LU @ z, rhs, rtol=1e-10, atol=1e-12)
+
+
+def test_ilut_without_dropping_is_exact_lu(lib64):
+    """ilut (spk_ilut.f90:1-420) with lfil = n and droptol = 0 keeps every
+    fill-in: LU = A and lusol solves the system exactly; with the reference's
+    (lfil=5, droptol=1e-2) each row keeps at most lfil entries per factor."""
+    A = _fem_matrix(n_target=200)
+    n = A.shape[0]
+    a, ja, ia = A.data.copy(), A.indices.astype(np.int32), A.indptr.astype(np.int32)
+    iwk = 2 * n * n
+    alu = np.zeros(iwk + 1)
+    jlu = np.zeros(iwk + 1, np.int32)
+    ju = np.zeros(n + 2, np.int32)
+    assert lib64.oc_ilut(n, ptr(a), ptr(ja), ptr(ia), n, C.c_double(0.0), ptr(alu), ptr(jlu), ptr(ju), iwk) == 0
+    L, U = _msr_to_lu(n, alu, jlu, ju)
+    assert np.allclose(L @ U, A.toarray(), rtol=0, atol=1e-13 * np.abs(A.data).max())
+    rhs = np.random.default_rng(1).uniform(-1, 1, n)
+    z = np.zeros(n)
+    lib64.oc_lusol(n, ptr(rhs), ptr(z), ptr(alu), ptr(jlu), ptr(ju))
+    assert np.allclose(z, spla.spsolve(A.tocsc(), rhs), rtol=1e-9, atol=1e-12)
+    iwk = 2 * A.nnz
+    alu = np.zeros(iwk + 1)
+    jlu = np.zeros(iwk + 1, np.int32)
+    assert lib64.oc_ilut(n, ptr(a), ptr(ja), ptr(ia), 5, C.c_double(0.01), ptr(alu), ptr(jlu), ptr(ju), iwk) == 0
+    for i in range(1, n + 1):
+        assert ju[i] - jlu[i] <= 5 and jlu[i + 1] - ju[i] <= 5
+
+
+def test_bcgstab_vs_spsolve(lib64):
+    """bcgstab with ILUT(5, 1e-2) (spk_iters.f90:1061-1334, runrc
+    spk_itaux.f90) against scipy's direct solve; the left-preconditioned
+    stopping test gives a true residual below 1e-7 relative here."""
+    A = _fem_matrix(n_target=3000, seed=4)
+    n = A.shape[0]
+    a, ja, ia = A.data.copy(), A.indices.astype(np.int32), A.indptr.astype(np.int32)
+    iwk = 2 * A.nnz
+    alu = np.zeros(iwk + 1)
+    jlu = np.zeros(iwk + 1, np.int32)
+    ju = np.zeros(n + 2, np.int32)
+    assert lib64.oc_ilut(n, ptr(a), ptr(ja), ptr(ia), 5, C.c_double(0.01), ptr(alu), ptr(jlu), ptr(ju), iwk) == 0
+    xt = np.random.default_rng(2).uniform(-1, 1, n)
+    rhs = A @ xt
+    x = np.zeros(n)
+    its = C.c_int(0)
+    rtol = float(np.float32(1e-8))
+    ierr = lib64.oc_bcgstab(n, ptr(a), ptr(ja), ptr(ia), ptr(alu), ptr(jlu), ptr(ju), ptr(rhs), ptr(x),
+                            C.c_double(rtol), C.c_double(rtol), 1000, C.byref(its))
+    assert ierr == 0 and 0 < its.value < 200
+    ref = spla.spsolve(A.tocsc(), rhs)
+    assert np.linalg.norm(A @ x - rhs) / np.linalg.norm(rhs) < 1e-7
+    assert np.allclose(x, ref, rtol=0, atol=1e-6 * np.abs(ref).max())
+
+
+# ---------------------------------------------------------------- whole step invariants
+
+def test_lake_at_rest(lib64):
+    """No forcing, flat surface: the step leaves the state exactly at rest."""
+    b = mesh.tidal_basin(noise=1.0)
+    om = pyoracle.OracleModel(_closed(b), "f64", params=dict(dt=300.0, ireib=5, itpre=1))
+    for _ in range(3):
+        om.step(None)
+    assert np.all(om.a["znv"] == 0.0)
+    assert np.all(om.a["utlnv"] == 0.0) and np.all(om.a["vtlnv"] == 0.0)
+
+
+def test_wind_setup_closed_form():
+    """Closed rectangular basin, uniform depth h, uniform wind stress tau_x,
+    linear bottom friction: the steady state has no transport and the surface
+    slope that balances the wind, g (h + zeta) dzeta/dx = tau_x, dzeta/dy = 0
+    (hydro_intern's wind, pressure and friction terms, hydrodynamic.f90:
+    1132-1321, and the zeta system, :488-754).  Volume is conserved to
+    round-off with a tight solve."""
+    b = _closed(mesh.tidal_basin(noise=0.0))
+    om = pyoracle.OracleModel(b, "f64", params=dict(dt=300.0, ireib=1, itpre=1, rtol=1e-15, atol=1e-15), czv=1e-3)
+    tx, ty = mesh.wind_stress(8.0, 0.0)
+    om.set_wind(tx, ty)
+    v0 = om.volume()
+    for _ in range(1000):
+        om.step(None)
+    assert abs(om.volume() - v0) / v0 < 1e-13
+    z = om.a["znv"]
+    ev = om.ev
+    gx = (ev[:, 3:6] * z[b.nen3v]).sum(1)
+    gy = (ev[:, 6:9] * z[b.nen3v]).sum(1)
+    H = om.a["hdenv"].reshape(b.nel, -1)[:, 0]
+    ratio = 9.81 * H * gx / tx
+    print("g H dz/dx / tau: mean", ratio.mean(), "range", ratio.min(), ratio.max())
+    assert abs(ratio.mean() - 1.0) < 2e-3
+    assert abs(gy.mean()) < 1e-3 * gx.mean()
+    assert np.abs(om.a["utlnv"]).max() < 2e-3 * (tx / 1e-3)          # residual circulation << tau/r
+
+
+def _closed_lagoon(n=1500, nlv=4, seed=11):
+    b = _closed(mesh.lagoon(n, seed=seed, lx=6000.0, ly=3000.0, nlv=nlv, tile=16))
+    salt = mesh.smooth_tracer(b, seed + 1, 25.0, 38.0)
+    params = dict(dt=60.0, ireib=5, itpre=1, itvd=2, itvdv=1, aapar=0.0, rtol=1e-15, atol=1e-15)
+    return b, salt, params
+
+
+def test_volume_and_tracer_mass_conserved_closed_lagoon():
+    """3D closed lagoon with wetting and drying under strong wind: total
+    volume (masscont, elems_dealing.f90:1191-1220) and tracer mass are
+    conserved to FP64 round-off, and the TVD scheme keeps the tracer within
+    its initial bounds (min/max check, concentration.f90:2199)."""
+    b, salt, params = _closed_lagoon()
+    om = pyoracle.OracleModel(b, "f64", params=params, visv=1e-3, difv=1e-4, tracers=dict(salt=salt))
+    om.set_wind(*mesh.wind_stress(15.0, 0.4))
+    v0, m0 = om.volume(), om.tracer_mass("salt")
+    valid = np.arange(b.nlv)[None, :] < b.ilhkv[:, None]
+    lo, hi = salt[valid].min(), salt[valid].max()
+    for _ in range(40):
+        om.step(None)
+    c = om.tracers["salt"]
+    assert (om.a["iwegv"] != 0).sum() > 0                     # some elements dry
+    assert abs(om.volume() - v0) / v0 < 1e-13
+    assert abs(om.tracer_mass("salt") - m0) / m0 < 1e-13
+    assert c[valid].min() >= lo - 1e-12 and c[valid].max() <= hi + 1e-12
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_constant_tracer_preserved(prec):
+    """A constant field stays constant through advection, TVD and vertical
+    diffusion with the open boundary active (conz3d's consistency with the
+    continuity step, concentration_omp.f90:705-734)."""
+    b = mesh.lagoon(1500, seed=7, lx=6000.0, ly=3000.0, nlv=4, tile=16)
+    c0 = np.full((b.nkn, b.nlv), 20.0)
+    params = dict(dt=60.0, ireib=5, itpre=1, itvd=2, itvdv=1, aapar=0.0, rtol=1e-15, atol=1e-15)
+    om = pyoracle.OracleModel(b, prec, params=params, visv=1e-3, difv=1e-4, tracers=dict(salt=c0))
+    for n in range(5):
+        om.step(mesh.tide_level((n + 1) * 60.0) - 0.1)
+    valid = np.arange(b.nlv)[None, :] < b.ilhkv[:, None]
+    dev = np.abs(om.tracers["salt"][valid] - 20.0).max()
+    assert dev < (1e-9 if prec == "f64" else 2e-4) * 20.0
+
+
+def test_reference_precision_tracks_fp64():
+    """The reference-precision build (f32 state, f64 locals and solve) stays
+    within FP32 storage error of the all-FP64 oracle over a tidal cycle
+    fraction on the 2D basin."""
+    b = mesh.tidal_basin()
+    p = dict(dt=300.0, ireib=5, itpre=2)
+    o32 = pyoracle.OracleModel(b, "f32", params=p)
+    o64 = pyoracle.OracleModel(b, "f64", params=p)
+    for n in range(48):
+        lev = mesh.tide_level((n + 1) * 300.0)
+        o32.step(lev)
+        o64.step(lev)
+    assert np.abs(o32.a["znv"] - o64.a["znv"]).max() < 2e-5
