@@ -124,7 +124,7 @@ def cpu_baseline(b, tr, args):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--steps", type=int, default=1440)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--nodes", type=int, default=50000)
     ap.add_argument("--layers", type=int, default=12)

@@ -91,6 +91,9 @@ int  shyhip_device_count(int *n);
 /* life cycle ------------------------------------------------------------ */
 int  shyhip_create(shyhip_model **h, int precision, int nkn, int nel, int nlvdi, int nlv, int ntracer);
 int  shyhip_destroy(shyhip_model *h);
+/* the handle's parameters (defaults after create: the reference's STR
+ * defaults, default_parms in para.f90 / def_para.f90) */
+int  shyhip_get_params(const shyhip_model *h, shyhip_params *p);
 int  shyhip_set_params(shyhip_model *h, const shyhip_params *p);
 
 /* basin and layers (init time): nen3v(3,nel) 1-based, xgv/ygv(nkn),

@@ -296,6 +296,12 @@ int shyhip_destroy(shyhip_model *h) {
     return 0;
 }
 
+int shyhip_get_params(const shyhip_model *h, shyhip_params *p) {
+    if (!h || !p) return fail(nullptr, "null argument");
+    *p = h->M.p;
+    return 0;
+}
+
 int shyhip_set_params(shyhip_model *h, const shyhip_params *p) {
     if (!h || !p) return fail(nullptr, "null argument");
     Model &M = h->M;

@@ -149,12 +149,16 @@ def _smooth_field(rng, nx, ny, sigma):
 def lagoon(nkn_target: int = 50000, seed: int = 20261202, lx: float = 40000.0, ly: float = 20000.0,
            nlv: int = 12, median: float = 1.5, sigma_ln: float = 1.06,
            hmin: float = 0.05, hmax: float = 20.0, jitter: float = 0.3,
-           tile: int = 32) -> Basin:
+           tile: int = 32, inlet_depth: float = 2.0) -> Basin:
     """Jittered-point Delaunay lagoon with a smooth log-normal depth.
 
     Nodes are numbered tile by tile (tile x tile grid points), so contiguous
     node ranges are compact patches of the basin.  The western side (x = 0)
-    is an open boundary of type 1."""
+    is an open boundary of type 1 in water at least inlet_depth deep: the
+    depth floor is inlet_depth up to 3 grid steps from the boundary and
+    falls linearly to zero at 8 steps, so boundary nodes stay wet through
+    the tide (a dry open-boundary node stops the model, topo_update.f90
+    setnod)."""
     from scipy.spatial import Delaunay
     rng = np.random.default_rng(np.uint64(seed))
     ny = max(3, int(round(math.sqrt(nkn_target * ly / lx))))
@@ -175,7 +179,9 @@ def lagoon(nkn_target: int = 50000, seed: int = 20261202, lx: float = 40000.0, l
     xs = x.reshape(-1)[perm].astype(np.float32)
     ys = y.reshape(-1)[perm].astype(np.float32)
     g = _smooth_field(rng, nx, ny, sigma=6.0).reshape(-1)[perm]
-    hnode = np.clip(median * np.exp(sigma_ln * g), hmin, hmax).astype(np.float32)
+    hnode = np.clip(median * np.exp(sigma_ln * g), hmin, hmax)
+    floor = inlet_depth * np.clip((8.0 * dx - xs.astype(np.float64)) / (5.0 * dx), 0.0, 1.0)
+    hnode = np.maximum(hnode, floor).astype(np.float32)
     tri = Delaunay(np.stack([xs.astype(np.float64), ys.astype(np.float64)], axis=1))
     nen3v = tri.simplices.astype(np.int32)
     x1, y1 = xs[nen3v[:, 0]].astype(np.float64), ys[nen3v[:, 0]].astype(np.float64)

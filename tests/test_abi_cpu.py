@@ -112,3 +112,59 @@ def test_lagoon_levels_and_orientation():
     assert len(b.bnd_nodes) > 0 and np.all(b.ibnd[b.bnd_nodes] == 1)
     b2 = mesh.lagoon(4000, seed=20261202, lx=12000.0, ly=6000.0, nlv=12, tile=16)
     assert np.array_equal(b.nen3v, b2.nen3v) and np.array_equal(b.hm3v, b2.hm3v)
+
+
+def _header_enums():
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    vals = {}
+    for body in re.findall(r"enum\s*\{(.*?)\}", text, re.S):
+        nxt = 0
+        for item in body.split(","):
+            item = item.strip()
+            if not item:
+                continue
+            if "=" in item:
+                name, v = (s.strip() for s in item.split("="))
+                nxt = int(v)
+            else:
+                name = item
+            vals[name] = nxt
+            nxt += 1
+    return vals
+
+
+def test_fortran_binding_matches_header():
+    """The ISO_C_BINDING module declares the header's constants with the same
+    values, the parameter block with the same fields in the same order, and
+    an interface for each entry point the Fortran host uses."""
+    src = open(os.path.join(ROOT, "shyfemcm-ismar_amd", "fortran", "shyhip_capi.f90")).read()
+    enums = _header_enums()
+    fparams = dict((n, int(v)) for n, v in re.findall(r"\b(SHYHIP_\w+)\s*=\s*(\d+)", src))
+    assert fparams, "no constants found"
+    for name, v in fparams.items():
+        assert enums.get(name) == v, name
+    for name in enums:
+        if name.startswith("SHYHIP_"):
+            assert name in fparams, f"{name} missing from the Fortran module"
+    body = re.search(r"type, bind\(C\) :: shyhip_params(.*?)end type", src, re.S).group(1)
+    ffields = []
+    for line in body.strip().splitlines():
+        typ, names = line.split("::")
+        ftyp = {"real(c_double)": C.c_double, "real(c_float)": C.c_float, "integer(c_int)": C.c_int}[typ.strip()]
+        ffields += [(n.strip(), ftyp) for n in names.split(",")]
+    assert ffields == list(shyhip.ShyParams._fields_)
+    declared = set(_declared())
+    for name in re.findall(r'bind\(C, name="(shyhip_\w+)"\)', src):
+        assert name in declared, name
+
+
+def test_fortran_host_builds():
+    """fortran/shyhip_run links against libshyhip.so (built by the package
+    Makefile with AMD flang)."""
+    exe = os.path.join(ROOT, "shyfemcm-ismar_amd", "fortran", "shyhip_run")
+    if not os.path.exists("/opt/rocm/llvm/bin/flang"):
+        pytest.skip("flang not present")
+    if not os.path.exists(exe):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "shyfemcm-ismar_amd")], check=True)
+    out = subprocess.run(["ldd", exe], capture_output=True, text=True, check=True).stdout
+    assert "libshyhip.so" in out and "not found" not in out

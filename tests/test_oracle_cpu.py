@@ -348,7 +348,7 @@ def test_wind_setup_closed_form():
 
 
 def _closed_lagoon(n=1500, nlv=4, seed=11):
-    b = _closed(mesh.lagoon(n, seed=seed, lx=6000.0, ly=3000.0, nlv=nlv, tile=16))
+    b = _closed(mesh.lagoon(n, seed=seed, lx=6000.0, ly=3000.0, nlv=nlv, tile=16, inlet_depth=0.0))
     salt = mesh.smooth_tracer(b, seed + 1, 25.0, 38.0)
     params = dict(dt=60.0, ireib=5, itpre=1, itvd=2, itvdv=1, aapar=0.0, rtol=1e-15, atol=1e-15)
     return b, salt, params
