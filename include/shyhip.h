@@ -60,6 +60,9 @@ typedef struct {
     int block_rows;                             /* block-ILU(0) block size (<= 1024), 0 = auto */
     int tri_sweeps;                             /* 0: exact level-scheduled triangular solves;
                                                    k > 0: k Jacobi sweeps per triangular factor */
+    int ilu_sweeps;                             /* 0: exact level-scheduled ILU(0) every solve;
+                                                   k > 0: k fine-grained fixed-point sweeps per
+                                                   solve, started from the previous factors */
     double inner_rtol;                          /* FP32 path: relative tolerance of the inner solve */
     /* scalars */
     float rkpar, difmol;

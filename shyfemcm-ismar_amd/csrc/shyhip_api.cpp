@@ -227,7 +227,7 @@ int shyhip_create(shyhip_model **out, int precision, int nkn, int nel, int nlvdi
     p.grav = 9.81f; p.rowass = 1025.f; p.vismol = 1.0e-6f; p.ireib = 0; p.uvmin = 0.2f;
     p.hzmin = 0.01f; p.hzoff = 0.05f; p.hzon = 0.10f; p.volmin = 1.f; p.iclin = 0;
     p.rtol = (double)1.0e-8f; p.atol = (double)1.0e-8f; p.itermax = 1000; p.refine_rtol = 1e-10;
-    p.block_rows = 0; p.tri_sweeps = 0; p.inner_rtol = 1e-6; p.rkpar = 0.f; p.difmol = 1.0e-6f; p.itvd = 0; p.itvdv = 0;
+    p.block_rows = 0; p.tri_sweeps = 0; p.ilu_sweeps = 0; p.inner_rtol = 1e-6; p.rkpar = 0.f; p.difmol = 1.0e-6f; p.itvd = 0; p.itvdv = 0;
     const size_t ne = nel, nk = nkn, L = nlvdi;
     State &s = M.s;
     int rc = 0;
@@ -279,9 +279,10 @@ int shyhip_destroy(shyhip_model *h) {
     Mesh &m = M.m;
     for (void *q : {(void *)m.nen3v, (void *)m.geo, (void *)m.ang, (void *)m.dis, (void *)m.ilhv, (void *)m.ilhkv,
                     m.hm3v, m.hev, m.hlv, m.hldv, m.areakv, (void *)m.ibnd, (void *)m.zbound, (void *)m.iextb,
-                    (void *)m.ibtyp, m.fcorv, m.czv, (void *)m.nk_ptr, (void *)m.nk_el, (void *)m.nkc_el,
+                    (void *)m.ibtyp, m.fcorv, m.czv, (void *)m.nk_ptr, (void *)m.nk_el, (void *)m.nkc_el, (void *)m.lay_el,
                     (void *)m.rowptr, (void *)m.cols, (void *)m.ijp, (void *)m.bnd_nodes, (void *)m.bnd_id,
-                    (void *)m.blk_rowptr, (void *)m.blk_cols, (void *)m.blk_src, (void *)m.blk_diag,
+                    (void *)m.blk_rowptr, (void *)m.blk_cols, (void *)m.blk_src, (void *)m.blk_diag, (void *)m.blk_cp_ptr,
+                    (void *)m.blk_cp_a, (void *)m.blk_cp_b, (void *)m.blk_jd,
                     (void *)m.fwd_rows, (void *)m.fwd_lptr, (void *)m.fwd_nlev, (void *)m.bwd_rows,
                     (void *)m.bwd_lptr, (void *)m.bwd_nlev, (void *)m.tvd_el, (void *)m.tvd_kn, m.tvd_w})
         dfree(q);
@@ -312,6 +313,7 @@ int shyhip_set_params(shyhip_model *h, const shyhip_params *p) {
     if (p->itvd == 1) return fail(&M, "itvd = 1 (gradient TVD) is not supported by the device path; use 0 or 2");
     if (p->block_rows < 0 || p->block_rows > kMaxBlockRows) return fail(&M, "block_rows must be in 0..1024");
     if (p->tri_sweeps < 0 || p->tri_sweeps > 64) return fail(&M, "tri_sweeps must be in 0..64");
+    if (p->ilu_sweeps < 0 || p->ilu_sweeps > 64) return fail(&M, "ilu_sweeps must be in 0..64");
     if (!(p->inner_rtol > 0. && p->inner_rtol < 1.)) return fail(&M, "inner_rtol must be in (0,1)");
     M.p = *p;
     return 0;
@@ -363,6 +365,15 @@ int shyhip_set_basin(shyhip_model *h, const int *nen3v, const float *xgv, const 
     rc |= dalloc(M, &m.dis, dis.size()) || upload(M, m.dis, dis.data(), dis.size());
     rc |= dalloc(M, &m.ilhv, nel) || upload(M, m.ilhv, ilhv, nel);
     rc |= dalloc(M, &m.ilhkv, nkn) || upload(M, m.ilhkv, ilhkv, nkn);
+    {
+        if (nel >= (1 << kLayShift) || M.nlv > 32) return fail(&M, "basin too large for the element-layer list");
+        std::vector<int> lay;
+        for (int l = 1; l <= M.nlv; l++)
+            for (int ie = 0; ie < nel; ie++)
+                if (ilhv[ie] >= l) lay.push_back(ie | ((l - 1) << kLayShift));
+        m.nel_layers = (int)lay.size();
+        rc |= dalloc(M, &m.lay_el, lay.size()) || upload(M, m.lay_el, lay.data(), lay.size());
+    }
     // depths (vertex-major on device)
     std::vector<double> t(3 * (size_t)nel);
     for (int ie = 0; ie < nel; ie++)
@@ -441,6 +452,12 @@ int shyhip_set_basin(shyhip_model *h, const int *nen3v, const float *xgv, const 
         rc |= dalloc(M, &m.blk_cols, bs.cols.size()) || upload(M, m.blk_cols, bs.cols.data(), bs.cols.size());
         rc |= dalloc(M, &m.blk_src, bs.src.size()) || upload(M, m.blk_src, bs.src.data(), bs.src.size());
         rc |= dalloc(M, &m.blk_diag, bs.diag.size()) || upload(M, m.blk_diag, bs.diag.data(), bs.diag.size());
+        rc |= dalloc(M, &m.blk_cp_ptr, bs.cp_ptr.size()) || upload(M, m.blk_cp_ptr, bs.cp_ptr.data(), bs.cp_ptr.size());
+        rc |= dalloc(M, &m.blk_cp_a, std::max<size_t>(1, bs.cp_a.size())) ||
+              upload(M, m.blk_cp_a, bs.cp_a.data(), bs.cp_a.size());
+        rc |= dalloc(M, &m.blk_cp_b, std::max<size_t>(1, bs.cp_b.size())) ||
+              upload(M, m.blk_cp_b, bs.cp_b.data(), bs.cp_b.size());
+        rc |= dalloc(M, &m.blk_jd, bs.ent_jd.size()) || upload(M, m.blk_jd, bs.ent_jd.data(), bs.ent_jd.size());
         rc |= dalloc(M, &m.fwd_rows, bs.fwd_rows.size()) || upload(M, m.fwd_rows, bs.fwd_rows.data(), bs.fwd_rows.size());
         rc |= dalloc(M, &m.fwd_lptr, bs.fwd_lptr.size()) || upload(M, m.fwd_lptr, bs.fwd_lptr.data(), bs.fwd_lptr.size());
         rc |= dalloc(M, &m.fwd_nlev, bs.fwd_nlev.size()) || upload(M, m.fwd_nlev, bs.fwd_nlev.data(), bs.fwd_nlev.size());
@@ -674,29 +691,34 @@ int shyhip_hydro_vertical(shyhip_model *h) { return h ? launch_hydro_vertical(h-
 int shyhip_system_solve_device(shyhip_model *h, int *iters) {
     if (!h) return fail(nullptr, "null handle");
     Model &M = h->M;
-    HIPCHK(M, hipMemsetAsync(M.sv.ctrl, 0, 8 * sizeof(int), M.stream));
     int ref = 0;
     const int rc = solver_solve(M, iters, &ref);
     M.last_refine = ref;
     return rc;
 }
 
-static int check_dev_err(Model &M, const char *where) {
-    HIPCHK(M, hipMemcpyAsync(M.h_pinned, M.d_err, sizeof(int), hipMemcpyDeviceToHost, M.stream));
-    HIPCHK(M, hipStreamSynchronize(M.stream));
-    const int e = M.h_pinned[0];
+// device error word d_err[0]: set by kernels (first error wins), cleared
+// when it is reported
+static int report_dev_err(Model &M, int e, const char *where) {
     if (!e) return 0;
+    HIPCHK(M, hipMemsetAsync(M.d_err, 0, sizeof(int), M.stream));
+    HIPCHK(M, hipStreamSynchronize(M.stream));
     static const char *names[] = {"", "setnod: open boundary node out of the system", "", "setuvd: z or u/v computation",
                                   "setdepth: non-positive layer thickness", "check_volume: negative volume", "",
                                   "conz3d: zero diagonal in a column", "penta: singular vertical column"};
     return fail(&M, std::string(where) + ": " + (e >= 0 && e <= 8 ? names[e] : "device error"), 3);
 }
 
-int shyhip_hydro(shyhip_model *h, double dtime) {
-    if (!h) return fail(nullptr, "null handle");
-    Model &M = h->M;
+static int check_dev_err(Model &M, const char *where) {
+    HIPCHK(M, hipMemcpyAsync(M.h_pinned, M.d_err, sizeof(int), hipMemcpyDeviceToHost, M.stream));
+    HIPCHK(M, hipStreamSynchronize(M.stream));
+    return report_dev_err(M, M.h_pinned[0], where);
+}
+
+// hydro; with final_check the device error word is read at the end (a
+// caller that synchronises later anyway, shyhip_step, checks it there)
+static int hydro_impl(Model &M, double dtime, bool final_check) {
     M.dtime = dtime;
-    HIPCHK(M, hipMemsetAsync(M.d_err, 0, sizeof(int), M.stream));
     int iloop = 0, changed = 0, its_total = 0;
     timer_begin(M, T_INTERN);
     if (launch_hydro_intern(M)) return 1;          // hydro_transports (invariant over the wet/dry loop)
@@ -712,7 +734,9 @@ int shyhip_hydro(shyhip_model *h, double dtime) {
         timer_end(M);
         int its = 0;
         timer_begin(M, T_SOLVE);
-        const int rc = shyhip_system_solve_device(h, &its);
+        int ref = 0;
+        const int rc = solver_solve(M, &its, &ref);
+        M.last_refine = ref;
         timer_end(M);
         its_total += its;
         if (rc) return rc;
@@ -725,7 +749,7 @@ int shyhip_hydro(shyhip_model *h, double dtime) {
     }
     M.last_nloop = iloop;
     M.last_iters = its_total;
-    if (check_dev_err(M, "hydro loop")) return 3;
+    if (report_dev_err(M, M.h_pinned[0], "hydro loop")) return 3;   // read with the setweg counter
     timer_begin(M, T_FINAL);
     if (launch_transports_final(M)) return 1;
     timer_end(M);
@@ -741,7 +765,12 @@ int shyhip_hydro(shyhip_model *h, double dtime) {
     timer_begin(M, T_TTOV);
     if (launch_ttov_uvint(M)) return 1;             // compute_velocities: ulnv/vlnv, unv/vnv
     timer_end(M);
-    return check_dev_err(M, "hydro");
+    return final_check ? check_dev_err(M, "hydro") : 0;
+}
+
+int shyhip_hydro(shyhip_model *h, double dtime) {
+    if (!h) return fail(nullptr, "null handle");
+    return hydro_impl(h->M, dtime, true);
 }
 
 int shyhip_scal3sh(shyhip_model *h, int itr, int istot, int clip_small) {
@@ -768,10 +797,11 @@ int shyhip_step(shyhip_model *h, double dtime, int istot) {
     timer_begin(M, T_COPY);
     if (launch_copy_old(M)) return 1;
     timer_end(M);
-    if (shyhip_hydro(h, dtime)) return M.err.empty() ? 1 : 3;
+    if (hydro_impl(M, dtime, false)) return M.err.empty() ? 1 : 3;
     for (int t = 0; t < M.ntr; t++)
         if (shyhip_scal3sh(h, t, istot, t == 0 ? 0 : 1)) return 1;
-    return timer_flush(M);
+    if (timer_flush(M)) return 1;
+    return check_dev_err(M, "step");
 }
 
 int shyhip_set_timing(shyhip_model *h, int on) {

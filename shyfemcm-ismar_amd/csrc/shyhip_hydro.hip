@@ -551,8 +551,9 @@ __global__ __launch_bounds__(kThreads) void k_setnod(HydroArgs<S> a) {
 
 // system_get: z = real(x)
 template <typename S>
-__global__ __launch_bounds__(kThreads) void k_system_get(HydroArgs<S> a) {
+__global__ __launch_bounds__(kThreads) void k_system_get(HydroArgs<S> a, int *cnt) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) *cnt = 0;     // counter of the setweg that follows
     if (k >= a.nkn) return;
     a.zn[k] = (S)a.x[k];
 }
@@ -778,58 +779,66 @@ __global__ __launch_bounds__(kThreads) void k_setdepth_node(HydroArgs<S> a, cons
 
 // ------------------------------------------------------------------ hydro_vertical
 // hydrodynamic.f90:1582-1781: divergence gathered per node and layer, then
-// the column is integrated from the bottom; w = 0 on zeta boundaries.
+// the column is integrated from the bottom; w = 0 on zeta boundaries.  One
+// workgroup per tile of consecutive nodes: one work-item per node-layer
+// gathers the incident elements (element order, the reference's scatter
+// order) into LDS, then one work-item per node integrates its column.
+template <int NL>
+struct VertTile {
+    static constexpr int TN = NL <= 16 ? 64 : 32;
+};
+
 template <typename S, int NL>
 __global__ __launch_bounds__(kThreads) void k_hydro_vertical(HydroArgs<S> a) {
     using R = S;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= a.nkn) return;
+    constexpr int TN = VertTile<NL>::TN;
+    __shared__ R vfL[NL][TN], vaL[NL][TN];
     const int nel = a.nel, nkn = a.nkn, nlvdi = a.nlvdi;
+    const int k0 = blockIdx.x * TN;
     const R az = (R)a.az, azt = (R)1 - az, dt = (R)a.dt;
-    R vf[NL], va[NL];
-#pragma unroll
-    for (int l = 0; l < NL; l++) { vf[l] = 0; va[l] = 0; }
-    for (int q = a.nk_ptr[k]; q < a.nk_ptr[k + 1]; q++) {
-        const int pk = a.nk_el[q];
-        const int ie = pk >> 2, ii = pk & 3;
-        const R aj = (R)(4. * a.geo[(size_t)6 * nel + ie]);
-        const R b = (R)a.geo[(size_t)ii * nel + ie], c = (R)a.geo[(size_t)(3 + ii) * nel + ie];
-        const int ilevel = a.ilhv[ie];
-#pragma unroll
-        for (int l = 1; l <= NL; l++) {
-            if (l > ilevel) break;
+    const int nlv = min(a.nlv, NL);
+    for (int idx = threadIdx.x; idx < TN * nlv; idx += blockDim.x) {
+        const int kk = idx % TN, l = idx / TN + 1, k = k0 + kk;
+        if (k >= nkn || l > a.ilhkv[k]) continue;
+        R vf = 0, va = 0;
+        for (int q = a.nk_ptr[k]; q < a.nk_ptr[k + 1]; q++) {
+            const int pk = a.nk_el[q];
+            const int ie = pk >> 2, ii = pk & 3;
+            if (a.ilhv[ie] < l) continue;
+            const R aj = (R)(4. * a.geo[(size_t)6 * nel + ie]);
+            const R b = (R)a.geo[(size_t)ii * nel + ie], c = (R)a.geo[(size_t)(3 + ii) * nel + ie];
             const size_t o = (size_t)(l - 1) * nel + ie;
             const R ffn = a.utn[o] * b + a.vtn[o] * c;
             const R ffo = a.uto[o] * b + a.vto[o] * c;
             const R ff = ffn * az + ffo * azt;
-            vf[l - 1] = vf[l - 1] + (R)3 * aj * ff;
-            va[l - 1] = va[l - 1] + aj;
+            vf = vf + (R)3 * aj * ff;
+            va = va + aj;
         }
+        vfL[l - 1][kk] = vf;
+        vaL[l - 1][kk] = va;
     }
+    __syncthreads();
+    const int kk = threadIdx.x, k = k0 + kk;
+    if (kk >= TN || k >= nkn) return;
     const int lmax = a.ilhkv[k];
-    R w[NL + 1];
-#pragma unroll
-    for (int l = 0; l <= NL; l++) w[l] = 0;
-#pragma unroll
-    for (int l = NL; l >= 1; l--) {
-        if (l > lmax) continue;
+    const bool zb = a.zbound[k] != 0;
+    // bottom-up: w(l-1) = w(l) + div - dV/dt, then divide by the top area
+    R wb = 0;
+    for (int l = lmax; l >= 1; l--) {
         const size_t o = (size_t)(l - 1) * nkn + k;
         const R ar = a.areakv[o];
         const R voln = a.hdkn[o] * ar, volo = a.hdko[o] * ar;
         const R dvdt = (voln - volo) / dt;
         const R qq = a.mflux ? a.mflux[o] : (R)0;
-        const R wdiv = vf[l - 1] + qq;
-        w[l - 1] = w[l] + wdiv - dvdt;
+        const R wdiv = vfL[l - 1][kk] + qq;
+        const R wl = wb + wdiv - dvdt;
+        R wout = wl;
+        if (l == 1) wout = 0;
+        else if (vaL[l - 1][kk] > (R)0) wout = wl / vaL[l - 1][kk];
+        a.wln[(size_t)(l - 1) * nkn + k] = zb ? (S)0 : wout;
+        wb = wl;
     }
-    w[0] = 0;
-#pragma unroll
-    for (int l = 2; l <= NL; l++) {
-        if (l > lmax) break;
-        const R atop = va[l - 1];
-        if (atop > (R)0) w[l - 1] = w[l - 1] / atop;
-    }
-    const bool zb = a.zbound[k] != 0;
-    for (int l = 0; l <= nlvdi; l++) a.wln[(size_t)l * nkn + k] = (zb || l > NL) ? (S)0 : w[l < NL ? l : NL];
+    for (int l = lmax; l <= nlvdi; l++) a.wln[(size_t)l * nkn + k] = (S)0;
 }
 
 // ttov (transforms.f90:98-122) + uvint (transforms.f90:336-362)
@@ -938,16 +947,20 @@ __global__ __launch_bounds__(kThreads) void k_sum_parts(const double *part, int 
         HIPCHK(M, hipGetLastError());                                                      \
     } while (0)
 
+// setweg(iweich); with `changed`, the number of elements that changed state
+// is read back together with the device error word (d_err[0], kept in
+// M.h_pinned[0] for the caller).  cnt_zeroed: the previous kernel in the
+// stream already cleared the counter.
 template <typename S>
-static int set_dry_t(Model &M, int iweich, int bnewtime, int *changed) {
+static int set_dry_t(Model &M, int iweich, int bnewtime, int *changed, bool cnt_zeroed = false) {
     auto a = make_args<S>(M);
     int *cnt = M.d_err + 1;
-    HIPCHK(M, hipMemsetAsync(cnt, 0, sizeof(int), M.stream));
+    if (changed && !cnt_zeroed) HIPCHK(M, hipMemsetAsync(cnt, 0, sizeof(int), M.stream));
     LAUNCH(M, k_setweg<S>, nblocks(M.nel), a, iweich, bnewtime, cnt);
     if (changed) {
-        HIPCHK(M, hipMemcpyAsync(M.h_pinned, cnt, sizeof(int), hipMemcpyDeviceToHost, M.stream));
+        HIPCHK(M, hipMemcpyAsync(M.h_pinned, M.d_err, 2 * sizeof(int), hipMemcpyDeviceToHost, M.stream));
         HIPCHK(M, hipStreamSynchronize(M.stream));
-        *changed = M.h_pinned[0];
+        *changed = M.h_pinned[1];
     }
     return 0;
 }
@@ -998,8 +1011,8 @@ int launch_zeta_assemble(Model &M) { return M.prec == SHYHIP_FP64 ? zeta_t<doubl
 template <typename S>
 static int sysget_t(Model &M, int bnewtime, int *changed) {
     auto a = make_args<S>(M);
-    LAUNCH(M, k_system_get<S>, nblocks(M.nkn), a);
-    return set_dry_t<S>(M, 1, bnewtime, changed);
+    LAUNCH(M, k_system_get<S>, nblocks(M.nkn), a, M.d_err + 1);
+    return set_dry_t<S>(M, 1, bnewtime, changed, true);
 }
 int launch_system_get_setweg1(Model &M, int bnewtime, int *changed) {
     return M.prec == SHYHIP_FP64 ? sysget_t<double>(M, bnewtime, changed) : sysget_t<float>(M, bnewtime, changed);
@@ -1052,7 +1065,7 @@ int launch_setdepth(Model &M, bool old_level) {
 template <typename S, int NL>
 static int vertical_nl(Model &M) {
     auto a = make_args<S>(M);
-    LAUNCH(M, (k_hydro_vertical<S, NL>), nblocks(M.nkn), a);
+    LAUNCH(M, (k_hydro_vertical<S, NL>), (M.nkn + VertTile<NL>::TN - 1) / VertTile<NL>::TN, a);
     return 0;
 }
 template <typename S>

@@ -429,6 +429,31 @@ void host_block_schedule(int n, const std::vector<int> &rowptr, const std::vecto
             (dir == 0 ? bs.fwd_nlev : bs.bwd_nlev)[b] = nlev;
         }
     }
+    // update pairs of the fine-grained factorisation, in ascending k
+    const size_t ne = bs.cols.size();
+    bs.cp_ptr.assign(ne + 1, 0);
+    bs.ent_jd.assign(ne, -1);
+    bs.cp_a.clear();
+    bs.cp_b.clear();
+    for (int i = 0; i < n; i++) {
+        const int b0 = (i / brows) * brows, li = i - b0;
+        for (int q = bs.rowptr[i]; q < bs.rowptr[i + 1]; q++) {
+            const int lj = bs.cols[q];
+            bs.ent_jd[q] = lj < li ? bs.diag[b0 + lj] : (lj == li ? -2 : -1);
+            const int m = std::min(li, lj);
+            for (int qk = bs.rowptr[i]; qk < bs.rowptr[i + 1]; qk++) {
+                const int lk = bs.cols[qk];
+                if (lk >= m) break;
+                const int r0 = bs.rowptr[b0 + lk], r1 = bs.rowptr[b0 + lk + 1];
+                const int *hit = std::lower_bound(bs.cols.data() + r0, bs.cols.data() + r1, lj);
+                if (hit != bs.cols.data() + r1 && *hit == lj) {
+                    bs.cp_a.push_back(qk);
+                    bs.cp_b.push_back((int)(hit - bs.cols.data()));
+                }
+            }
+            bs.cp_ptr[q + 1] = (int)bs.cp_a.size();
+        }
+    }
 }
 
 }  // namespace shy

@@ -21,6 +21,7 @@ constexpr float kFlag = -9988765.0f;                 // mkonst flag (const.f90:1
 constexpr double kDrittl = (double)(1.0f / 3.0f);    // default-real 1./3. in double
 constexpr int kThreads = 256;
 constexpr int kMaxBlockRows = 1024;                  // block-ILU(0) rows per workgroup (LDS)
+constexpr int kLayShift = 26;                        // element ids below 2^26, layers below 32
 
 // error codes written by kernels into Model::d_err[0]
 enum DevErr { ERR_NONE = 0, ERR_SETNOD = 1, ERR_SETUVD = 3, ERR_SETDEPTH = 4, ERR_VOLUME = 5,
@@ -39,10 +40,13 @@ struct Mesh {                 // device copies of init-time data
     int *nk_ptr = nullptr;    // node -> incident (element<<2 | vertex), element order
     int *nk_el = nullptr;
     int *nkc_el = nullptr;    // same lists in colour order
+    int *lay_el = nullptr;    // active element-layers, layer by layer: ie | (l-1) << kLayShift
+    int nel_layers = 0;
     int *rowptr = nullptr, *cols = nullptr, *ijp = nullptr;   // zeta CSR; ijp[ie*9+i*3+j]
     int *bnd_nodes = nullptr; int *bnd_id = nullptr; int nbnd_nodes = 0;
     // block-ILU(0)
     int *blk_rowptr = nullptr, *blk_cols = nullptr, *blk_src = nullptr, *blk_diag = nullptr;
+    int *blk_cp_ptr = nullptr, *blk_cp_a = nullptr, *blk_cp_b = nullptr, *blk_jd = nullptr;
     int *fwd_rows = nullptr, *fwd_lptr = nullptr, *fwd_nlev = nullptr;
     int *bwd_rows = nullptr, *bwd_lptr = nullptr, *bwd_nlev = nullptr;
     // TVD upwind tables: per element 6 ordered pairs (ic,id); [p*nel + ie]
@@ -78,6 +82,8 @@ struct State {
 struct Solver {               // Krylov work space (value type V = S of the model)
     int nblk = 0, brows = 0, max_blk_nnz = 0, nparts = 0;
     int last_its[8] = {};     // iteration count of the previous solve, per solve kind
+    bool lu_valid = false;    // lu holds factors of an earlier matrix (warm start of the sweeps)
+    int last_nref = 0;        // FP64 refinement rounds of the previous solve
     void *lu = nullptr;       // block factors (values of the block pattern)
     void *r0 = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr, *v0 = nullptr, *v1 = nullptr,
          *s = nullptr, *t = nullptr, *dx = nullptr, *b = nullptr, *x0 = nullptr;
@@ -140,6 +146,10 @@ void host_tvd_tables(int nkn, int nel, const int *nen3v0, const float *x, const 
 struct BlockSchedule {
     int nblk = 0, brows = 0;
     std::vector<int> rowptr, cols, src, diag, fwd_rows, fwd_lptr, fwd_nlev, bwd_rows, bwd_lptr, bwd_nlev;
+    // fine-grained ILU(0) sweeps: for entry q of the block pattern, the pairs
+    // (q_ik, q_kj), k < min(i, j), of the ILU(0) update, and ent_jd[q] = the
+    // diagonal entry of column j for an L entry, -2 for a diagonal, -1 for U
+    std::vector<int> cp_ptr, cp_a, cp_b, ent_jd;
 };
 void host_block_schedule(int n, const std::vector<int> &rowptr, const std::vector<int> &cols, int brows,
                          BlockSchedule &bs);

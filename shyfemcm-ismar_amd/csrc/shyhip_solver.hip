@@ -1,10 +1,16 @@
 // shyhip_solver.hip -- zeta system solve on gfx950.
 //
 // Preconditioner: block-Jacobi ILU(0).  Rows are split into contiguous blocks
-// (<= 1024 rows, one workgroup each); each diagonal block is factorised and
-// solved in LDS with level scheduling (rows of one level in parallel,
-// workgroup barriers between levels).  A basin with at most one block of
-// nodes gets exactly ILU(0) (spk_ilut.f90:1832-1950, lusol :2309-2345).
+// (<= 1024 rows, one workgroup each).  Each diagonal block is factorised
+// either exactly, with level scheduling in LDS (rows of one level in parallel,
+// workgroup barriers between levels; a basin of at most one block of nodes
+// then gets exactly ILU(0), spk_ilut.f90:1832-1950, lusol :2309-2345), or by
+// fine-grained fixed-point sweeps over all factor entries at once (the
+// asynchronous ILU of Chow and Patel, SIAM J. Sci. Comput. 37 (2015) C169,
+// run here as Jacobi sweeps so the result is deterministic), started from the
+// factors of the previous solve: the zeta matrix changes little from one
+// step to the next, so a few sweeps track it.  The block factors are staged
+// in LDS by every kernel that applies them.
 //
 // Krylov method: BiCGStab in the arrangement the reference uses -- runrc with
 // ipar(2)=1, which bisinit maps to LEFT preconditioning (spk_iters.f90:
@@ -15,7 +21,10 @@
 // workgroup of the next kernel reduces itself in the same fixed order -- no
 // atomics or device-scope fences, visibility comes from the kernel boundary,
 // and all workgroups (and all runs) see bitwise identical scalars.
-// The FP32 path runs this in FP32 inside an FP64 iterative refinement.
+// The FP32 path runs this in FP32 inside an FP64 iterative refinement whose
+// control also lives on the device: every kernel of a refinement round
+// returns at once when the FP64 residual test has passed, so the host
+// enqueues the expected number of rounds and reads the flags once.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,17 +38,21 @@ namespace shy {
 // kernel never overwrites a value that another workgroup of it still reads
 enum { SC_TARGET = 0, SC_NORM0 = 1, SC_RHO = 2, SC_ALPHA = 4, SC_OMEGA = 6, SC_RNORM = 8, SC_RES64 = 9,
        SC_BNORM = 10 };
-enum { CT_CONV = 1, CT_FAIL = 2, CT_NMV = 3, CT_REFCONV = 5 };
+// control words: convergence / failure of the current Krylov solve, its
+// matvec count, iterations of all inner solves of this zeta solve, FP64
+// refinement converged, factorisation failure, refinements done
+enum { CT_CONV = 1, CT_FAIL = 2, CT_NMV = 3, CT_ITS = 4, CT_REFCONV = 5, CT_FACTOR = 6, CT_NREF = 7 };
 // partial-sum regions: [region][q][workgroup]
 enum { PR_A = 0, PR_B = 1, PR_C = 2, PR_R = 3 };
 
 template <typename V>
 struct SolvArgs {
-    int n, nblk, brows, itermax, sweeps;
+    int n, nblk, brows, mx, itermax, sweeps;
     double rtol, atol;
     const int *rowptr, *cols;
     const V *A;
     const int *brp, *bcol, *bsrc, *bdiag, *frows, *flptr, *fnlev, *brows_l, *blptr, *bnlev;
+    const int *cp_ptr, *cp_a, *cp_b, *bjd;
     V *lu;
     const double *aval;
     V *b, *x0, *r0, *r, *p0, *p1, *v0, *v1, *s, *t, *dx;
@@ -51,12 +64,14 @@ struct SolvArgs {
 template <typename V>
 static SolvArgs<V> solv_args(Model &M, const V *A, V *b, V *x0) {
     SolvArgs<V> a{};
-    a.n = M.nkn; a.nblk = M.sv.nblk; a.brows = M.sv.brows; a.itermax = M.p.itermax; a.sweeps = M.p.tri_sweeps;
+    a.n = M.nkn; a.nblk = M.sv.nblk; a.brows = M.sv.brows; a.mx = M.sv.max_blk_nnz; a.itermax = M.p.itermax;
+    a.sweeps = M.p.tri_sweeps;
     a.rtol = M.p.rtol; a.atol = M.p.atol;
     a.rowptr = M.m.rowptr; a.cols = M.m.cols; a.A = A;
     a.brp = M.m.blk_rowptr; a.bcol = M.m.blk_cols; a.bsrc = M.m.blk_src; a.bdiag = M.m.blk_diag;
     a.frows = M.m.fwd_rows; a.flptr = M.m.fwd_lptr; a.fnlev = M.m.fwd_nlev;
     a.brows_l = M.m.bwd_rows; a.blptr = M.m.bwd_lptr; a.bnlev = M.m.bwd_nlev;
+    a.cp_ptr = M.m.blk_cp_ptr; a.cp_a = M.m.blk_cp_a; a.cp_b = M.m.blk_cp_b; a.bjd = M.m.blk_jd;
     a.lu = (V *)M.sv.lu; a.aval = M.s.aval;
     a.b = b; a.x0 = x0;
     a.r0 = (V *)M.sv.r0; a.r = (V *)M.sv.r; a.p0 = (V *)M.sv.p0; a.p1 = (V *)M.sv.p1;
@@ -109,14 +124,50 @@ __device__ inline bool breakdown(double x) {   // brkdn: zero or not finite
 }
 
 // ---------------------------------------------------------------- block solve
+// One diagonal block of the preconditioner: row starts, diagonal positions,
+// local columns and factor values (diagonal stored inverted), either in
+// global memory or staged in LDS.  Entry indices are relative to the arrays
+// the view points at.
+template <typename V>
+struct BlockView {
+    const int *rp, *dg, *col;
+    const V *lu;
+};
+
+// LDS bytes of a kernel that stages its block: y and w (brows values each),
+// the block's factor values and columns, row starts and diagonals
+template <typename V>
+static size_t staged_lds(int brows, int mx) {
+    return (size_t)(2 * brows + mx) * sizeof(V) + (size_t)(mx + 2 * brows + 1) * sizeof(int);
+}
+
+template <typename V, bool STAGE>
+__device__ inline BlockView<V> load_block(const SolvArgs<V> &a, int b0, int nb, unsigned char *lds) {
+    if (!STAGE) return BlockView<V>{a.brp + b0, a.bdiag + b0, a.bcol, a.lu};
+    V *llu = reinterpret_cast<V *>(lds);
+    int *lrp = reinterpret_cast<int *>(llu + a.mx);
+    int *ldg = lrp + a.brows + 1;
+    int *lcol = ldg + a.brows;
+    const int q0 = a.brp[b0], ne = a.brp[b0 + nb] - q0;
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+        llu[e] = a.lu[q0 + e];
+        lcol[e] = a.bcol[q0 + e];
+    }
+    for (int li = threadIdx.x; li <= nb; li += blockDim.x) {
+        lrp[li] = a.brp[b0 + li] - q0;
+        if (li < nb) ldg[li] = a.bdiag[b0 + li] - q0;
+    }
+    return BlockView<V>{lrp, ldg, lcol, llu};   // visible after the caller's barrier
+}
+
 // y (LDS, local rows of block blk) <- (LU)^-1 y.  Exact: level-scheduled
 // forward and backward substitution in lusol's order.  With a.sweeps = k > 0
 // each triangular factor is applied by k Jacobi sweeps instead
 // (x <- b - L x, then x <- D^-1 (y - U x)), double-buffered in LDS so the
 // result does not depend on the order in which wavefronts run.
 template <typename V>
-__device__ inline void block_lusol(const SolvArgs<V> &a, int blk, int b0, V *y, V *w) {
-    const int nb = min(a.n, b0 + a.brows) - b0;
+__device__ inline void block_lusol(const SolvArgs<V> &a, const BlockView<V> &B, int blk, int b0, int nb, V *y,
+                                   V *w) {
     if (a.sweeps > 0) {
         constexpr int RPT = kMaxBlockRows / kThreads;   // rows per work-item
         V bl[RPT], nv[RPT];
@@ -130,9 +181,8 @@ __device__ inline void block_lusol(const SolvArgs<V> &a, int blk, int b0, V *y, 
         for (int sw = 1; sw < a.sweeps; sw++) {
             nr = 0;
             for (int li = threadIdx.x; li < nb; li += blockDim.x, nr++) {
-                const int i = b0 + li;
                 V acc = bl[nr];
-                for (int q = a.brp[i]; q < a.bdiag[i]; q++) acc = acc - a.lu[q] * w[a.bcol[q]];
+                for (int q = B.rp[li]; q < B.dg[li]; q++) acc = acc - B.lu[q] * w[B.col[q]];
                 nv[nr] = acc;
             }
             __syncthreads();
@@ -143,18 +193,16 @@ __device__ inline void block_lusol(const SolvArgs<V> &a, int blk, int b0, V *y, 
         // backward: (D + U) x = w, starting from x = D^-1 w
         nr = 0;
         for (int li = threadIdx.x; li < nb; li += blockDim.x, nr++) {
-            const int i = b0 + li;
             bl[nr] = w[li];
-            y[li] = a.lu[a.bdiag[i]] * w[li];
+            y[li] = B.lu[B.dg[li]] * w[li];
         }
         __syncthreads();
         for (int sw = 1; sw < a.sweeps; sw++) {
             nr = 0;
             for (int li = threadIdx.x; li < nb; li += blockDim.x, nr++) {
-                const int i = b0 + li;
                 V acc = bl[nr];
-                for (int q = a.bdiag[i] + 1; q < a.brp[i + 1]; q++) acc = acc - a.lu[q] * y[a.bcol[q]];
-                nv[nr] = a.lu[a.bdiag[i]] * acc;
+                for (int q = B.dg[li] + 1; q < B.rp[li + 1]; q++) acc = acc - B.lu[q] * y[B.col[q]];
+                nv[nr] = B.lu[B.dg[li]] * acc;
             }
             __syncthreads();
             nr = 0;
@@ -168,9 +216,9 @@ __device__ inline void block_lusol(const SolvArgs<V> &a, int blk, int b0, V *y, 
     const int nf = a.fnlev[blk];
     for (int lv = 0; lv < nf; lv++) {
         for (int t = lp[lv] + threadIdx.x; t < lp[lv + 1]; t += blockDim.x) {
-            const int li = a.frows[b0 + t], i = b0 + li;
+            const int li = a.frows[b0 + t];
             V acc = y[li];
-            for (int q = a.brp[i]; q < a.bdiag[i]; q++) acc = acc - a.lu[q] * y[a.bcol[q]];
+            for (int q = B.rp[li]; q < B.dg[li]; q++) acc = acc - B.lu[q] * y[B.col[q]];
             y[li] = acc;
         }
         __syncthreads();
@@ -179,10 +227,10 @@ __device__ inline void block_lusol(const SolvArgs<V> &a, int blk, int b0, V *y, 
     const int nbl = a.bnlev[blk];
     for (int lv = 0; lv < nbl; lv++) {
         for (int t = bp[lv] + threadIdx.x; t < bp[lv + 1]; t += blockDim.x) {
-            const int li = a.brows_l[b0 + t], i = b0 + li;
+            const int li = a.brows_l[b0 + t];
             V acc = y[li];
-            for (int q = a.bdiag[i] + 1; q < a.brp[i + 1]; q++) acc = acc - a.lu[q] * y[a.bcol[q]];
-            y[li] = a.lu[a.bdiag[i]] * acc;
+            for (int q = B.dg[li] + 1; q < B.rp[li + 1]; q++) acc = acc - B.lu[q] * y[B.col[q]];
+            y[li] = B.lu[B.dg[li]] * acc;
         }
         __syncthreads();
     }
@@ -192,7 +240,7 @@ __device__ inline void block_lusol(const SolvArgs<V> &a, int blk, int b0, V *y, 
 // inverted), level by level in LDS.
 template <typename V>
 __global__ __launch_bounds__(kThreads) void k_bilu_factor(SolvArgs<V> a) {
-    extern __shared__ unsigned char smem[];
+    extern __shared__ double smem[];
     V *val = reinterpret_cast<V *>(smem);
     const int blk = blockIdx.x, b0 = blk * a.brows;
     const int nb = min(a.n, b0 + a.brows) - b0;
@@ -219,7 +267,7 @@ __global__ __launch_bounds__(kThreads) void k_bilu_factor(SolvArgs<V> a) {
                 }
             }
             const V piv = val[ied - q0];
-            if (piv == (V)0) atomicCAS(a.ctrl + CT_FAIL, 0, 2000 + i);
+            if (piv == (V)0) atomicCAS(a.ctrl + CT_FACTOR, 0, 1 + i);
             val[ied - q0] = (V)1 / piv;
         }
         __syncthreads();
@@ -227,21 +275,71 @@ __global__ __launch_bounds__(kThreads) void k_bilu_factor(SolvArgs<V> a) {
     for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) a.lu[q] = val[q - q0];
 }
 
-// r0 = M^-1 (b - A x0); r = p = r0; (r0,r0) into both slots of region C
+// `sweeps` Jacobi sweeps of the ILU(0) fixed point, per block, from the
+// factors in lu: for every entry (i,j) of the block pattern at once,
+//   l_ij = (a_ij - sum_{k<j} l_ik u_kj) / u_jj   (i > j)
+//   u_ij =  a_ij - sum_{k<i} l_ik u_kj           (i <= j)
+// reading the previous sweep's values (two LDS buffers).  Its fixed point is
+// the ILU(0) factorisation.
 template <typename V>
-__global__ __launch_bounds__(kThreads) void k_bicg_init(SolvArgs<V> a) {
-    __shared__ V y[kMaxBlockRows];
-    __shared__ V w[kMaxBlockRows];
+__global__ __launch_bounds__(kThreads) void k_bilu_sweep(SolvArgs<V> a, int sweeps) {
+    extern __shared__ double smem[];
+    V *av = reinterpret_cast<V *>(smem);
+    V *x0 = av + a.mx, *x1 = x0 + a.mx;
     const int blk = blockIdx.x, b0 = blk * a.brows;
     const int nb = min(a.n, b0 + a.brows) - b0;
+    const int q0 = a.brp[b0], ne = a.brp[b0 + nb] - q0;
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+        const int q = q0 + e;
+        av[e] = (V)a.aval[a.bsrc[q]];
+        const V f = a.lu[q];
+        x0[e] = a.bjd[q] == -2 ? (V)1 / f : f;
+    }
+    __syncthreads();
+    for (int sw = 0; sw < sweeps; sw++) {
+        for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+            const int q = q0 + e;
+            V acc = av[e];
+            for (int p = a.cp_ptr[q]; p < a.cp_ptr[q + 1]; p++) acc = acc - x0[a.cp_a[p] - q0] * x0[a.cp_b[p] - q0];
+            const int jd = a.bjd[q];
+            x1[e] = jd >= 0 ? acc / x0[jd - q0] : acc;
+        }
+        __syncthreads();
+        V *t = x0;
+        x0 = x1;
+        x1 = t;
+    }
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+        const int q = q0 + e;
+        const V f = x0[e];
+        if (a.bjd[q] == -2) {
+            if (f == (V)0) atomicCAS(a.ctrl + CT_FACTOR, 0, 1 + b0);
+            a.lu[q] = (V)1 / f;
+        } else {
+            a.lu[q] = f;
+        }
+    }
+}
+
+// r0 = M^-1 (b - A x0) (x0 = 0 when zero_x0); r = p = r0; (r0,r0) into both
+// slots of region C
+template <typename V, bool STAGE>
+__global__ __launch_bounds__(kThreads) void k_bicg_init(SolvArgs<V> a, int zero_x0) {
+    if (a.ctrl[CT_REFCONV]) return;
+    extern __shared__ double smem[];
+    V *y = reinterpret_cast<V *>(smem), *w = y + a.brows;
+    const int blk = blockIdx.x, b0 = blk * a.brows;
+    const int nb = min(a.n, b0 + a.brows) - b0;
+    const BlockView<V> B = load_block<V, STAGE>(a, b0, nb, reinterpret_cast<unsigned char *>(w + a.brows));
     for (int li = threadIdx.x; li < nb; li += blockDim.x) {
         const int i = b0 + li;
         V t = 0;
-        for (int q = a.rowptr[i]; q < a.rowptr[i + 1]; q++) t = t + a.A[q] * a.x0[a.cols[q]];
+        if (!zero_x0)
+            for (int q = a.rowptr[i]; q < a.rowptr[i + 1]; q++) t = t + a.A[q] * a.x0[a.cols[q]];
         y[li] = a.b[i] - t;
     }
     __syncthreads();
-    block_lusol(a, blk, b0, y, w);
+    block_lusol(a, B, blk, b0, nb, y, w);
     double loc[2] = {0., 0.};
     for (int li = threadIdx.x; li < nb; li += blockDim.x) {
         const int i = b0 + li;
@@ -257,11 +355,11 @@ __global__ __launch_bounds__(kThreads) void k_bicg_init(SolvArgs<V> a) {
 
 // iteration it, first kernel: convergence test of the previous iteration
 // (stopbis), beta, p = r + beta (p - omega v) (fused), v = M^-1 A p, (r0,v)
-template <typename V>
+template <typename V, bool STAGE>
 __global__ __launch_bounds__(kThreads) void k_bicg_pv(SolvArgs<V> a, int it) {
-    if (a.ctrl[CT_CONV] || a.ctrl[CT_FAIL]) return;
-    __shared__ V y[kMaxBlockRows];
-    __shared__ V w[kMaxBlockRows];
+    if (a.ctrl[CT_CONV] || a.ctrl[CT_FAIL] || a.ctrl[CT_REFCONV]) return;
+    extern __shared__ double smem[];
+    V *y = reinterpret_cast<V *>(smem), *w = y + a.brows;
     const int par = it & 1;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     double rs[2];
@@ -301,6 +399,7 @@ __global__ __launch_bounds__(kThreads) void k_bicg_pv(SolvArgs<V> a, int it) {
     const V vb = (V)beta, vo = (V)omega;
     const int blk = blockIdx.x, b0 = blk * a.brows;
     const int nb = min(a.n, b0 + a.brows) - b0;
+    const BlockView<V> B = load_block<V, STAGE>(a, b0, nb, reinterpret_cast<unsigned char *>(w + a.brows));
     for (int li = threadIdx.x; li < nb; li += blockDim.x) {
         const int i = b0 + li;
         V t = 0;
@@ -313,7 +412,7 @@ __global__ __launch_bounds__(kThreads) void k_bicg_pv(SolvArgs<V> a, int it) {
         pnew[i] = a.r[i] + vb * (pold[i] - vo * vold[i]);
     }
     __syncthreads();
-    block_lusol(a, blk, b0, y, w);
+    block_lusol(a, B, blk, b0, nb, y, w);
     double loc[1] = {0.};
     for (int li = threadIdx.x; li < nb; li += blockDim.x) {
         const int i = b0 + li;
@@ -324,11 +423,11 @@ __global__ __launch_bounds__(kThreads) void k_bicg_pv(SolvArgs<V> a, int it) {
 }
 
 // alpha = rho / (r0,v); s = r - alpha v (fused); t = M^-1 A s; (t,t), (t,s)
-template <typename V>
+template <typename V, bool STAGE>
 __global__ __launch_bounds__(kThreads) void k_bicg_st(SolvArgs<V> a, int it) {
-    if (a.ctrl[CT_CONV] || a.ctrl[CT_FAIL]) return;
-    __shared__ V y[kMaxBlockRows];
-    __shared__ V w[kMaxBlockRows];
+    if (a.ctrl[CT_CONV] || a.ctrl[CT_FAIL] || a.ctrl[CT_REFCONV]) return;
+    extern __shared__ double smem[];
+    V *y = reinterpret_cast<V *>(smem), *w = y + a.brows;
     const int par = it & 1;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     double d[1];
@@ -343,6 +442,7 @@ __global__ __launch_bounds__(kThreads) void k_bicg_st(SolvArgs<V> a, int it) {
     const V va = (V)alpha;
     const int blk = blockIdx.x, b0 = blk * a.brows;
     const int nb = min(a.n, b0 + a.brows) - b0;
+    const BlockView<V> B = load_block<V, STAGE>(a, b0, nb, reinterpret_cast<unsigned char *>(w + a.brows));
     for (int li = threadIdx.x; li < nb; li += blockDim.x) {
         const int i = b0 + li;
         V t = 0;
@@ -355,7 +455,7 @@ __global__ __launch_bounds__(kThreads) void k_bicg_st(SolvArgs<V> a, int it) {
         a.s[i] = a.r[i] - va * vnew[i];
     }
     __syncthreads();
-    block_lusol(a, blk, b0, y, w);
+    block_lusol(a, B, blk, b0, nb, y, w);
     double loc[2] = {0., 0.};
     for (int li = threadIdx.x; li < nb; li += blockDim.x) {
         const int i = b0 + li;
@@ -370,7 +470,7 @@ __global__ __launch_bounds__(kThreads) void k_bicg_st(SolvArgs<V> a, int it) {
 // omega = (t,s)/(t,t); dx += alpha p + omega s; r = s - omega t; (r,r), (r,r0)
 template <typename V>
 __global__ __launch_bounds__(kThreads) void k_bicg_update(SolvArgs<V> a, int it) {
-    if (a.ctrl[CT_CONV] || a.ctrl[CT_FAIL]) return;
+    if (a.ctrl[CT_CONV] || a.ctrl[CT_FAIL] || a.ctrl[CT_REFCONV]) return;
     const int par = it & 1;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     double ts[2];
@@ -384,7 +484,7 @@ __global__ __launch_bounds__(kThreads) void k_bicg_update(SolvArgs<V> a, int it)
         if (lead) a.ctrl[CT_FAIL] = -3;
         return;
     }
-    if (lead) a.scal[SC_OMEGA + par] = om;
+    if (lead) { a.scal[SC_OMEGA + par] = om; a.ctrl[CT_ITS] += 1; }
     const V *pnew = par ? a.p0 : a.p1;
     const V alpha = (V)a.scal[SC_ALPHA + par], omega = (V)om;
     const int blk = blockIdx.x, b0 = blk * a.brows;
@@ -402,11 +502,12 @@ __global__ __launch_bounds__(kThreads) void k_bicg_update(SolvArgs<V> a, int it)
     store_partials<2>(loc, a.part, PR_C, a.nparts);
 }
 
-// x = x0 + dx (tidycg) into the FP64 solution
+// x = x0 + dx (tidycg) into the FP64 solution; with add, x += dx unless the
+// refinement has converged
 template <typename V>
-__global__ void k_bicg_finish(const V *x0, const V *dx, double *x, int n, int add) {
+__global__ void k_bicg_finish(const V *x0, const V *dx, double *x, int n, int add, const int *ctrl) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || (add && ctrl[CT_REFCONV])) return;
     if (add) x[i] = x[i] + (double)dx[i];
     else x[i] = (double)(x0[i] + dx[i]);
 }
@@ -416,16 +517,19 @@ __global__ void k_to_f32(const double *s, float *d, int n) {
     if (i < n) d[i] = (float)s[i];
 }
 
+// initial guess x = zn; clears the factorisation flag of this solve
 template <typename S>
-__global__ void k_guess(const S *zn, double *x, int n) {
+__global__ void k_guess(const S *zn, double *x, int n, int *ctrl) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) { ctrl[CT_FACTOR] = 0; ctrl[CT_ITS] = 0; ctrl[CT_NREF] = 0; ctrl[CT_REFCONV] = 0; }
     if (i < n) x[i] = (double)zn[i];
 }
 
 // FP64 residual r64 = b - A x with partial |r64|^2 and |b|^2 (refinement)
 __global__ __launch_bounds__(kThreads) void k_resid64(const int *rowptr, const int *cols, const double *A,
                                                        const double *b, const double *x, double *r64, float *r32,
-                                                       int n, double *part, int nparts) {
+                                                       int n, double *part, int nparts, const int *ctrl) {
+    if (ctrl[CT_REFCONV]) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double loc[2] = {0., 0.};
     if (i < n) {
@@ -442,12 +546,15 @@ __global__ __launch_bounds__(kThreads) void k_resid64(const int *rowptr, const i
 
 __global__ __launch_bounds__(kThreads) void k_resid_check(const double *part, int nparts, int n, double *scal,
                                                            int *ctrl, double tol) {
+    if (ctrl[CT_REFCONV]) return;
     double rs[2];
     sum_partials<2>(part, PR_R, nparts, n, rs);
     if (threadIdx.x == 0) {
         scal[SC_RES64] = sqrt(rs[0]);
         scal[SC_BNORM] = sqrt(rs[1]);
-        ctrl[CT_REFCONV] = (sqrt(rs[0]) <= tol * sqrt(rs[1])) ? 1 : 0;
+        const int conv = (sqrt(rs[0]) <= tol * sqrt(rs[1])) ? 1 : 0;
+        ctrl[CT_REFCONV] = conv;
+        if (!conv) ctrl[CT_NREF] += 1;
     }
 }
 
@@ -459,31 +566,71 @@ static int read_ctrl(Model &M, int *out, int n) {
     return 0;
 }
 
+static constexpr size_t kLdsLimit = 64 * 1024;
+static constexpr size_t kLdsStatic = 3 * kThreads * sizeof(double);   // reduction buffers
+
+// dynamic LDS of the kernels that apply the preconditioner: with the block
+// staged, or y and w only (blocks too large to stage read factors from
+// global memory)
+template <typename V>
+static bool stage_fits(const Model &M) {
+    return staged_lds<V>(M.sv.brows, M.sv.max_blk_nnz) + kLdsStatic <= kLdsLimit;
+}
+
+template <typename V>
+static size_t apply_lds(const Model &M) {
+    return stage_fits<V>(M) ? staged_lds<V>(M.sv.brows, M.sv.max_blk_nnz) : 2 * (size_t)M.sv.brows * sizeof(V);
+}
+
+// block factors of the current matrix: exact level-scheduled ILU(0) on the
+// first solve (or always with ilu_sweeps = 0), else ilu_sweeps fixed-point
+// sweeps from the previous factors
+template <typename V>
+static int factor(Model &M, SolvArgs<V> &a) {
+    const size_t lds_sweep = 3 * (size_t)M.sv.max_blk_nnz * sizeof(V);
+    if (M.p.ilu_sweeps > 0 && M.sv.lu_valid && lds_sweep <= kLdsLimit) {
+        hipLaunchKernelGGL(k_bilu_sweep<V>, dim3(M.sv.nblk), dim3(kThreads), lds_sweep, M.stream, a,
+                           M.p.ilu_sweeps);
+    } else {
+        const size_t lds = (size_t)M.sv.max_blk_nnz * sizeof(V);
+        hipLaunchKernelGGL(k_bilu_factor<V>, dim3(M.sv.nblk), dim3(kThreads), lds, M.stream, a);
+    }
+    HIPCHK(M, hipGetLastError());
+    M.sv.lu_valid = true;
+    return 0;
+}
+
 // Iterations are enqueued in chunks between host checks of the device flags;
 // the first chunk is the iteration count of the previous solve of the same
 // kind (`slot`), so a converging solve usually needs a single round trip.
 // Kernels enqueued after convergence return immediately.
-template <typename V>
-static int bicgstab(Model &M, SolvArgs<V> &a, int *iters, int slot) {
+template <typename V, bool STAGE>
+static int bicgstab_t(Model &M, SolvArgs<V> &a, int *iters, int slot, int zero_x0) {
     const int g = a.nblk;
-    hipLaunchKernelGGL(k_bicg_init<V>, dim3(g), dim3(kThreads), 0, M.stream, a);
+    const size_t lds = apply_lds<V>(M);
+    hipLaunchKernelGGL((k_bicg_init<V, STAGE>), dim3(g), dim3(kThreads), lds, M.stream, a, zero_x0);
     HIPCHK(M, hipGetLastError());
-    int it = 0, c[4] = {0, 0, 0, 0};
+    int it = 0, c[8] = {};
     const int maxit = M.p.itermax > 0 ? (M.p.itermax + 1) / 2 + 1 : 100000;
     int chunk = std::max(2, M.sv.last_its[slot] + 1);
     for (;;) {
         for (int q = 0; q < chunk; q++, it++) {
-            hipLaunchKernelGGL(k_bicg_pv<V>, dim3(g), dim3(kThreads), 0, M.stream, a, it);
-            hipLaunchKernelGGL(k_bicg_st<V>, dim3(g), dim3(kThreads), 0, M.stream, a, it);
+            hipLaunchKernelGGL((k_bicg_pv<V, STAGE>), dim3(g), dim3(kThreads), lds, M.stream, a, it);
+            hipLaunchKernelGGL((k_bicg_st<V, STAGE>), dim3(g), dim3(kThreads), lds, M.stream, a, it);
             hipLaunchKernelGGL(k_bicg_update<V>, dim3(g), dim3(kThreads), 0, M.stream, a, it);
         }
         HIPCHK(M, hipGetLastError());
-        if (read_ctrl(M, c, 4)) return 1;
+        if (read_ctrl(M, c, 8)) return 1;
         if (c[CT_CONV] || c[CT_FAIL] || it >= maxit) break;
         chunk = 2;
     }
     if (iters) *iters = (c[CT_NMV] - 1) / 2;
     M.sv.last_its[slot] = (c[CT_NMV] - 1) / 2;
+    if (c[CT_FACTOR] != 0) {
+        M.sv.lu_valid = false;
+        M.err = "zeta solver: zero pivot in the block factorisation (row " + std::to_string(c[CT_FACTOR]) + ")";
+        return 2;
+    }
     if (c[CT_FAIL] != 0) {
         M.err = "zeta solver: BiCGStab failed (code " + std::to_string(c[CT_FAIL]) + ")";
         return 2;
@@ -495,64 +642,108 @@ static int bicgstab(Model &M, SolvArgs<V> &a, int *iters, int slot) {
     return 0;
 }
 
+template <typename V>
+static int bicgstab(Model &M, SolvArgs<V> &a, int *iters, int slot, int zero_x0) {
+    return stage_fits<V>(M) ? bicgstab_t<V, true>(M, a, iters, slot, zero_x0)
+                            : bicgstab_t<V, false>(M, a, iters, slot, zero_x0);
+}
+
+// one inner FP32 solve of the refinement, enqueued without host checks:
+// init from a zero correction, then `chunk` iterations (no-ops once
+// converged or once the refinement has converged)
+template <typename V, bool STAGE>
+static int enqueue_inner_t(Model &M, SolvArgs<V> &a, int chunk) {
+    const int g = a.nblk;
+    const size_t lds = apply_lds<V>(M);
+    hipLaunchKernelGGL((k_bicg_init<V, STAGE>), dim3(g), dim3(kThreads), lds, M.stream, a, 1);
+    for (int it = 0; it < chunk; it++) {
+        hipLaunchKernelGGL((k_bicg_pv<V, STAGE>), dim3(g), dim3(kThreads), lds, M.stream, a, it);
+        hipLaunchKernelGGL((k_bicg_st<V, STAGE>), dim3(g), dim3(kThreads), lds, M.stream, a, it);
+        hipLaunchKernelGGL(k_bicg_update<V>, dim3(g), dim3(kThreads), 0, M.stream, a, it);
+    }
+    HIPCHK(M, hipGetLastError());
+    return 0;
+}
+
+template <typename V>
+static int enqueue_inner(Model &M, SolvArgs<V> &a, int chunk) {
+    return stage_fits<V>(M) ? enqueue_inner_t<V, true>(M, a, chunk) : enqueue_inner_t<V, false>(M, a, chunk);
+}
+
 int solver_setup(Model &M) { (void)M; return 0; }
 
 int solver_solve(Model &M, int *iters, int *refinements) {
     const int n = M.nkn;
     const int g1 = (n + kThreads - 1) / kThreads;
     int it_total = 0, nref = 0;
-    const size_t lds = (size_t)M.sv.max_blk_nnz * M.esize();
     if (M.prec == SHYHIP_FP64) {
         // initial guess = current level (system_solve passes znv); the Krylov
         // x0 vector is the FP64 solution array itself
-        hipLaunchKernelGGL(k_guess<double>, dim3(g1), dim3(kThreads), 0, M.stream, (const double *)M.s.zn, M.s.x, n);
+        hipLaunchKernelGGL(k_guess<double>, dim3(g1), dim3(kThreads), 0, M.stream, (const double *)M.s.zn, M.s.x, n,
+                           M.sv.ctrl);
         auto a = solv_args<double>(M, M.s.aval, M.s.rhs, M.s.x);
-        hipLaunchKernelGGL(k_bilu_factor<double>, dim3(M.sv.nblk), dim3(kThreads), lds, M.stream, a);
-        HIPCHK(M, hipGetLastError());
+        if (factor<double>(M, a)) return 1;
         int its = 0;
-        const int rc = bicgstab<double>(M, a, &its, 0);
+        const int rc = bicgstab<double>(M, a, &its, 0, 0);
         it_total = its;
         if (rc) return rc;
         hipLaunchKernelGGL(k_bicg_finish<double>, dim3(g1), dim3(kThreads), 0, M.stream, (const double *)M.s.x,
-                           (const double *)a.dx, M.s.x, n, 0);
+                           (const double *)a.dx, M.s.x, n, 0, (const int *)M.sv.ctrl);
         HIPCHK(M, hipGetLastError());
     } else {
         // FP32 BiCGStab on the FP32-rounded matrix inside FP64 refinement:
-        // x <- x + A32^-1 (b - A64 x) until |b - A64 x| <= refine_rtol |b|
-        hipLaunchKernelGGL(k_guess<float>, dim3(g1), dim3(kThreads), 0, M.stream, (const float *)M.s.zn, M.s.x, n);
+        // x <- x + A32^-1 (b - A64 x) until |b - A64 x| <= refine_rtol |b|;
+        // each inner solve starts from a zero correction and runs at most
+        // `chunk` iterations (a partial correction still refines).  The
+        // rounds of the previous solve are enqueued at once; the flags are
+        // read after them.
+        hipLaunchKernelGGL(k_guess<float>, dim3(g1), dim3(kThreads), 0, M.stream, (const float *)M.s.zn, M.s.x, n,
+                           M.sv.ctrl);
         float *a32 = (float *)M.sv.a32;
         hipLaunchKernelGGL(k_to_f32, dim3((M.nnz + kThreads - 1) / kThreads), dim3(kThreads), 0, M.stream,
                            (const double *)M.s.aval, a32, M.nnz);
         float *b32 = (float *)M.sv.b, *x0 = (float *)M.sv.x0;
         auto a = solv_args<float>(M, a32, b32, x0);
-        hipLaunchKernelGGL(k_bilu_factor<float>, dim3(M.sv.nblk), dim3(kThreads), lds, M.stream, a);
-        HIPCHK(M, hipGetLastError());
+        if (factor<float>(M, a)) return 1;
         a.rtol = M.p.inner_rtol;
         a.atol = 0.0;
         const int maxref = 12;
-        for (nref = 0; nref < maxref; nref++) {
+        const int itmax_inner = M.p.itermax > 0 ? (M.p.itermax + 1) / 2 : 500;
+        const int chunk = std::min(itmax_inner, std::max(4, M.sv.last_its[1] + 3));
+        auto resid = [&]() -> int {
             hipLaunchKernelGGL(k_resid64, dim3(g1), dim3(kThreads), 0, M.stream, M.m.rowptr, M.m.cols,
                                (const double *)M.s.aval, (const double *)M.s.rhs, (const double *)M.s.x, M.sv.r64,
-                               b32, n, M.sv.part, M.sv.nparts);
+                               b32, n, M.sv.part, M.sv.nparts, (const int *)M.sv.ctrl);
             hipLaunchKernelGGL(k_resid_check, dim3(1), dim3(kThreads), 0, M.stream, (const double *)M.sv.part,
                                M.sv.nparts, g1, M.sv.scal, M.sv.ctrl, M.p.refine_rtol);
             HIPCHK(M, hipGetLastError());
-            int c[6];
-            if (read_ctrl(M, c, 6)) return 1;
+            return 0;
+        };
+        int plan = std::max(1, M.sv.last_nref), enq = 0, c[8] = {};
+        for (;;) {
+            for (int r = 0; r < plan && enq < maxref; r++, enq++) {
+                if (resid() || enqueue_inner<float>(M, a, chunk)) return 1;
+                hipLaunchKernelGGL(k_bicg_finish<float>, dim3(g1), dim3(kThreads), 0, M.stream, (const float *)x0,
+                                   (const float *)a.dx, M.s.x, n, 1, (const int *)M.sv.ctrl);
+            }
+            if (resid()) return 1;
+            if (read_ctrl(M, c, 8)) return 1;
+            if (c[CT_FACTOR] != 0) {
+                M.sv.lu_valid = false;
+                M.err = "zeta solver: zero pivot in the block factorisation (row " + std::to_string(c[CT_FACTOR]) + ")";
+                return 2;
+            }
             if (c[CT_REFCONV]) break;
-            HIPCHK(M, hipMemsetAsync(x0, 0, sizeof(float) * n, M.stream));
-            int its = 0;
-            const int rc = bicgstab<float>(M, a, &its, 1 + std::min(nref, 6));
-            it_total += its;
-            if (rc == 1) return rc;     // runtime error; a stalled inner solve still refines
-            hipLaunchKernelGGL(k_bicg_finish<float>, dim3(g1), dim3(kThreads), 0, M.stream, (const float *)x0,
-                               (const float *)a.dx, M.s.x, n, 1);
-            HIPCHK(M, hipGetLastError());
+            if (enq >= maxref) {
+                M.err = "zeta solver: FP64 refinement did not reach the residual target";
+                return 2;
+            }
+            plan = 1;
         }
-        if (nref >= maxref) {
-            M.err = "zeta solver: FP64 refinement did not reach the residual target";
-            return 2;
-        }
+        nref = c[CT_NREF];
+        it_total = c[CT_ITS];
+        M.sv.last_nref = nref;
+        M.sv.last_its[1] = nref > 0 ? (it_total + nref - 1) / nref : 0;
     }
     if (iters) *iters = it_total;
     if (refinements) *refinements = nref;

@@ -118,8 +118,20 @@ def test_lagoon_structures_and_masks(gpu):
     assert np.array_equal(gm.pull("hdknv"), om.a["hdknv"])
 
 
-def test_lagoon_steps_fp64(gpu):
+# device solver variants: exact block ILU(0) with exact triangular solves;
+# Jacobi-sweep triangular solves; fine-grained ILU(0) sweeps warm-started
+# from the previous step's factors
+SOLVER_VARIANTS = {
+    "exact": dict(),
+    "tri_sweeps": dict(block_rows=256, tri_sweeps=3),
+    "ilu_sweeps": dict(block_rows=256, tri_sweeps=3, ilu_sweeps=2),
+}
+
+
+@pytest.mark.parametrize("variant", list(SOLVER_VARIANTS))
+def test_lagoon_steps_fp64(gpu, variant):
     b, tr, params = _lagoon_case()
+    params = dict(params, **SOLVER_VARIANTS[variant])
     om, gm = _pair(gpu, b, "fp64", "f64", params, visv=1e-3, difv=1e-4, tracers=tr)
     tx, ty = mesh.wind_stress(8.0, 0.7)
     om.set_wind(tx, ty)
