@@ -1,18 +1,22 @@
 #!/usr/bin/env python3
 """Throughput of the SHYFEM semi-implicit hydro step + T/S transport on one MI355X.
 
-Workload: the 3D lagoon -- 50k nodes of a seeded jittered-point Delaunay
-triangulation (~98k elements) in a 40 x 20 km basin, log-normal depth
-(median 1.5 m) with intertidal flats, 12 z-layers of 1 m, M2+K1 tide on the
-western boundary, uniform 8 m/s wind, T and S tracers (TVD horizontal and
-vertical), dt = 60 s.  Synthetic data; inputs resident on the device before
-the timed region.
+Headline workload: the 3D lagoon -- 50k nodes of a seeded jittered-point
+Delaunay triangulation (~98k elements) in a 40 x 20 km basin, log-normal
+depth (median 1.5 m) with intertidal flats, 12 z-layers of 1 m, M2+K1 tide
+on the western boundary, uniform 8 m/s wind, T and S tracers (TVD
+horizontal and vertical), dt = 60 s, 1440 steps.  Synthetic data; inputs
+resident on the device before the timed region.
 
 A "step" is one model time step: set_dry, copy_uvz/copy_depth, hydro (column
 solves, zeta assembly, sparse solve, wet/dry loop, final transports, depths,
 vertical velocity) and scal3sh for T and S.  Two device paths are timed:
 plain FP64, and FP32 state/transport with the FP32 Krylov solve inside FP64
-iterative refinement (the metric's headline value).  Prints one JSON line.
+iterative refinement (the headline value).  Beside them, unless switched
+off: a batched ensemble of uneven lagoons (all members in one model) against
+the same members run one after the other on the plain FP64 path, a serving
+stream with p50/p99 latency, and the CPU oracle on a bounded sample.
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -40,33 +44,51 @@ def _lagoon(args):
     return b, tr
 
 
-def _level(t):
+def _level(t, phase=0.0):
     from shyhip import mesh
-    return mesh.tide_level(t, amps=(0.4, 0.1), periods=(mesh.M2_PERIOD, mesh.K1_PERIOD), phases=(0.0, 0.7))
+    return mesh.tide_level(t, amps=(0.4, 0.1), periods=(mesh.M2_PERIOD, mesh.K1_PERIOD), phases=(phase, 0.7))
 
 
 PARAMS = dict(dt=60.0, ireib=5, itvd=2, itvdv=1, aapar=0.0, itpre=2, block_rows=256, tri_sweeps=3)
 WIND = (8.0, 0.9)
 
 
-def _kernel_bytes(b, p, ntr):
-    """Algorithmic HBM bytes per launch of each kernel group (each listed
-    array read or written once at its stored width p)."""
+def _block_nnz(b, brows):
+    """Entries of the zeta pattern inside the diagonal blocks of brows rows,
+    and all entries."""
+    e = np.concatenate([b.nen3v[:, [0, 1]], b.nen3v[:, [1, 2]], b.nen3v[:, [2, 0]]])
+    e = np.unique(np.sort(e, axis=1), axis=0)
+    inside = (e[:, 0] // brows) == (e[:, 1] // brows)
+    return int(2 * inside.sum() + b.nkn), int(2 * len(e) + b.nkn)
+
+
+def _kernel_bytes(b, p, brows):
+    """Algorithmic HBM bytes per launch of each timed kernel: every listed
+    array read or written once at its stored width p (gathers that hit
+    cache are not counted)."""
     EL = int(b.ilhv.sum())
     NL = int(b.ilhkv.sum())
     E, N = b.nel, b.nkn
+    nnzb, nnz = _block_nnz(b, brows)
+    lu = nnzb * (p + 4) + 2 * N * 4             # staged block factors + row starts/diagonals
+    spmv = nnz * (p + 4) + N * 4                # values, columns, row pointers
     return {
-        # utlov vtlov hdeov read, utlnv vtlnv written, ddxv ddyv (2 per layer) written,
+        # utlov vtlov hdeov read, utlnv vtlnv written, ddxv ddyv (2 planes) written,
         # visv at the interfaces, per element geometry/levels/summaries
         "hydro_intern": 9 * EL * p + (NL + N) * p + E * (12 + 56 + 6 * p + 4 + 48),
         # utlnv vtlnv utlov vtlov ulnv vlnv read, 3 horizontal terms written, c gathered
         "conz3d_elem": 9 * EL * p + NL * p + E * (12 + 56 + 4),
         # horizontal terms read; c hdknv hdkov wlnv difv read, c written
         "conz3d_node": 3 * EL * p + 6 * NL * p + 2 * N * p,
-        # ddxv ddyv read, utlnv vtlnv read+written
         "transports_final": 8 * EL * p + E * (12 + 56 + 3 * p + 2 * p),
         "hydro_vertical": 4 * EL * p + 4 * NL * p + N * p,
         "setdepth": EL * p + NL * p + E * 3 * p,
+        # BiCGStab, first kernel: s t p v r0 read, dx read+written, r p v written
+        "bicg_a": 10 * N * p + spmv + lu,
+        # second kernel: r v r0 read, s t written
+        "bicg_b": 5 * N * p + spmv + lu,
+        # block ILU(0): values gathered from A, factors written, update lists read
+        "bilu_factor": nnzb * (8 + p + 4 + 4 + 4) + 3 * nnzb * 4,
     }
 
 
@@ -86,6 +108,8 @@ def run_gpu(b, tr, prec, args):
         m.step(_level(t))
     m.sync()
     elapsed = time.perf_counter() - t0
+    zn = m.pull("znv")
+    stats = m.stats()
     # kernel timing pass (events on the library's stream), outside the timed region
     m.set_timing(True)
     nt = max(5, min(20, args.steps))
@@ -94,10 +118,8 @@ def run_gpu(b, tr, prec, args):
         m.step(_level(t))
     timing = m.timing()
     m.set_timing(False)
-    zn = m.pull("znv")
-    stats = m.stats()
-    vol = m.volume()
-    return dict(model=m, elapsed=elapsed, timing=timing, nt=nt, znv=zn, stats=stats, volume=vol, t=t)
+    m.close()
+    return dict(elapsed=elapsed, timing=timing, nt=nt, znv=zn, stats=stats)
 
 
 def cpu_baseline(b, tr, args):
@@ -117,8 +139,83 @@ def cpu_baseline(b, tr, args):
         n += 1
         if time.perf_counter() - t0 > 4 * args.cpu_seconds:
             break
+    return n, time.perf_counter() - t0
+
+
+def ensemble_members(args):
+    """Uneven lagoons (log-uniform sizes), each with its own wind (0-10 m/s:
+    stronger wind sets the surface down by more than the 1 m top z-layer on
+    the upwind side of these shallow lagoons, which stops the model as it
+    stops the reference, check_volume) and tidal phase, T and S; one
+    vertical grid of args.ens_layers layers."""
+    from shyhip import mesh
+    rng = np.random.default_rng(np.uint64(args.seed + 404))
+    out = []
+    for i in range(args.ens_members):
+        n = int(math.exp(rng.uniform(math.log(args.ens_min), math.log(args.ens_max))))
+        lx = 40000.0 * math.sqrt(n / 50000.0)
+        b = mesh.lagoon(n, seed=args.seed + 1000 + i, lx=lx, ly=lx / 2, nlv=args.ens_layers, tile=32)
+        ph = float(rng.uniform(0, 2 * math.pi))
+        wind = mesh.wind_stress(float(rng.uniform(0, 10)), float(rng.uniform(0, 2 * math.pi)))
+        out.append(dict(basin=b, wind=wind, level=(lambda t, ph=ph: _level(t, ph)),
+                        tracers=dict(temp=mesh.smooth_tracer(b, args.seed + 2000 + i, 15.0, 25.0),
+                                     salt=mesh.smooth_tracer(b, args.seed + 3000 + i, 25.0, 38.0))))
+    return out
+
+
+def run_ensemble(mems, prec, steps, warmup):
+    from shyhip import ensemble
+    ens = ensemble.Ensemble(mems, prec, params=PARAMS, visv=1e-3, difv=1e-4)
+    for _ in range(warmup):
+        ens.step()
+    ens.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ens.step()
+    ens.sync()
     el = time.perf_counter() - t0
-    return n, el
+    z = [ens.pull("znv", i) for i in range(len(mems))]
+    ens.close()
+    EL = sum(int(m["basin"].ilhv.sum()) for m in mems)
+    return EL * steps / el, el, z
+
+
+def run_members_alone(mems, prec, steps, warmup):
+    """The same members one after the other, each its own model (no batching)."""
+    import shyhip
+    tot_el, tot_t = 0, 0.0
+    for mem in mems:
+        m = shyhip.Model(mem["basin"], prec, params=PARAMS, visv=1e-3, difv=1e-4, tracers=mem["tracers"])
+        m.set_wind(*mem["wind"])
+        t = 0.0
+        for _ in range(warmup):
+            t += PARAMS["dt"]
+            m.step(mem["level"](t))
+        m.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            t += PARAMS["dt"]
+            m.step(mem["level"](t))
+        m.sync()
+        tot_t += time.perf_counter() - t0
+        tot_el += int(mem["basin"].ilhv.sum()) * steps
+        m.close()
+    return tot_el / tot_t, tot_t
+
+
+def run_serving(args):
+    from shyhip import serve
+    reqs = serve.poisson_stream(args.srv_requests, rate=4.0, seed=args.seed + 505,
+                                sizes=(5000, args.srv_max_nodes), steps=(120, 360), nlv=8)
+    params = dict(dt=60.0, ireib=5, block_rows=256, tri_sweeps=3)
+    srv = serve.Server("fp32", params=params, slots=args.srv_slots, hbm_budget=64 << 30)
+    done, wall = srv.serve(reqs)
+    s = serve.summary(done, wall)
+    s.update(slots=args.srv_slots, rate_per_s=4.0,
+             workload=f"{len(reqs)} requests, Poisson 4/s, lagoons of 5k-{args.srv_max_nodes // 1000}k nodes "
+                      f"(log-uniform), 8 layers, 120-360 steps of 60 s, wind ramp to 30 m/s, surge 0.3-1.2 m, "
+                      f"FP32 + FP64 refinement, hydro only")
+    return s
 
 
 def main():
@@ -132,7 +229,16 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--ens-members", type=int, default=16)
+    ap.add_argument("--ens-min", type=int, default=5000)
+    ap.add_argument("--ens-max", type=int, default=40000)
+    ap.add_argument("--ens-layers", type=int, default=10)
+    ap.add_argument("--ens-steps", type=int, default=120)
+    ap.add_argument("--srv-requests", type=int, default=12)
+    ap.add_argument("--srv-max-nodes", type=int, default=100000)
+    ap.add_argument("--srv-slots", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design note, "
@@ -145,19 +251,20 @@ def main():
     for prec in ("fp64", "fp32"):
         res[prec] = run_gpu(b, tr, prec, args)
     thr = {p: EL * args.steps / r["elapsed"] for p, r in res.items()}
-    zerr = float(np.max(np.abs(res["fp32"]["znv"] - res["fp64"]["znv"])))
+    dz = np.abs(res["fp32"]["znv"] - res["fp64"]["znv"])
 
-    # roofline of the dominant non-solver kernel of the headline (FP32) path
+    # roofline of the headline path's dominant kernel (largest time per step)
     head = res["fp32"]
-    kb = _kernel_bytes(b, 4, len(tr))
-    avg = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in head["timing"].items()}
-    dom = max((k for k in kb if avg.get(k, 0.0) > 0), key=lambda k: avg[k] * (head["timing"][k][1]))
-    achieved = kb[dom] / (avg[dom] * 1e-3) / 1e9
+    kb = _kernel_bytes(b, 4, PARAMS["block_rows"])
     per_step = {k: v[0] / head["nt"] for k, v in head["timing"].items()}
-    step_bytes = sum(kb[k] * head["timing"][k][1] / head["nt"] for k in kb if k in head["timing"])
+    avg = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in head["timing"].items()}
+    dom = max((k for k in kb if avg.get(k, 0.0) > 0), key=lambda k: per_step[k])
+    achieved = kb[dom] / (avg[dom] * 1e-3) / 1e9
+    kernel_gbs = {k: round(kb[k] / (avg[k] * 1e-3) / 1e9, 1) for k in kb if avg.get(k, 0.0) > 0}
 
     out = {
-        "metric": "elem*layer*steps/s (FP32 + FP64-refined path; plain FP64 path reported beside it)",
+        "metric": "elem*layer*steps/s (FP32 + FP64-refined path; plain FP64 path, batched ensemble and "
+                  "serving p50/p99 reported beside it)",
         "value": round(thr["fp32"], 1),
         "unit": "elem*layer*steps/s",
         "n_gpus": 1,
@@ -175,16 +282,36 @@ def main():
         "fp64_value": round(thr["fp64"], 1),
         "fp64_ms_per_step": round(1e3 * res["fp64"]["elapsed"] / args.steps, 4),
         "fp32_over_fp64": round(thr["fp32"] / thr["fp64"], 3),
-        "zeta_err_fp32_vs_fp64_m": zerr,
+        "zeta_err_fp32_vs_fp64_m": {"max": float(dz.max()), "p99": float(np.percentile(dz, 99)),
+                                    "rms": float(np.sqrt(np.mean(dz ** 2)))},
         "solver": {p: res[p]["stats"] for p in res},
         "kernel_ms_per_step_fp32": {k: round(v, 4) for k, v in per_step.items() if v > 0},
         "kernel_ms_per_step_fp64": {k: round(v[0] / res["fp64"]["nt"], 4) for k, v in res["fp64"]["timing"].items()
                                     if v[0] > 0},
+        "kernel_algorithmic_GBps_fp32": kernel_gbs,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_launch": kb[dom], "avg_launch_ms": round(avg[dom], 5)},
-        "step_algorithmic_GBps_non_solver": round(step_bytes / (sum(per_step.values()) * 1e-3) / 1e9, 1),
     }
+    if not args.no_ensemble:
+        mems = ensemble_members(args)
+        e32, t32, z32 = run_ensemble(mems, "fp32", args.ens_steps, 5)
+        e64, t64, z64 = run_ensemble(mems, "fp64", args.ens_steps, 5)
+        a64, ta64 = run_members_alone(mems, "fp64", max(10, args.ens_steps // 4), 3)
+        out["ensemble"] = {
+            "workload": f"{len(mems)} lagoons of {args.ens_min // 1000}k-{args.ens_max // 1000}k nodes (log-uniform), "
+                        f"{args.ens_layers} layers, own wind 0-10 m/s and tidal phase, T+S, dt=60 s",
+            "nodes": int(sum(m["basin"].nkn for m in mems)),
+            "elem_layers": int(sum(int(m["basin"].ilhv.sum()) for m in mems)),
+            "batched_fp32_value": round(e32, 1), "batched_fp64_value": round(e64, 1),
+            "sequential_fp64_value": round(a64, 1),
+            "batched_fp32_over_sequential_fp64": round(e32 / a64, 3),
+            "batched_fp32_over_batched_fp64": round(e32 / e64, 3),
+            "steps": args.ens_steps,
+            "zeta_err_batched_fp32_vs_fp64_m": float(max(np.abs(a - c).max() for a, c in zip(z32, z64))),
+        }
+    if not args.no_serving:
+        out["serving"] = run_serving(args)
     if not args.no_cpu:
         n, el = cpu_baseline(b, tr, args)
         out["cpu_baseline"] = {"value": round(EL * n / el, 1), "unit": "elem*layer*steps/s", "cores": 1,

@@ -60,9 +60,6 @@ typedef struct {
     int block_rows;                             /* block-ILU(0) block size (<= 1024), 0 = auto */
     int tri_sweeps;                             /* 0: exact level-scheduled triangular solves;
                                                    k > 0: k Jacobi sweeps per triangular factor */
-    int ilu_sweeps;                             /* 0: exact level-scheduled ILU(0) every solve;
-                                                   k > 0: k fine-grained fixed-point sweeps per
-                                                   solve, started from the previous factors */
     double inner_rtol;                          /* FP32 path: relative tolerance of the inner solve */
     /* scalars */
     float rkpar, difmol;
@@ -99,6 +96,16 @@ int  shyhip_destroy(shyhip_model *h);
 int  shyhip_get_params(const shyhip_model *h, shyhip_params *p);
 int  shyhip_set_params(shyhip_model *h, const shyhip_params *p);
 
+/* several independent basins in one model (an ensemble): member m owns the
+ * nodes node_start[m] .. node_start[m+1]-1 and the elements
+ * elem_start[m] .. elem_start[m+1]-1 (0-based, node_start[nmem] = nkn,
+ * elem_start[nmem] = nel); its elements use only its nodes.  Colouring and
+ * upwind tables are built per member, so each member's element order and
+ * upwind points are those it has alone.  Call before shyhip_set_basin;
+ * without it the model is one member.  No counterpart in the reference,
+ * which runs one basin per process. */
+int  shyhip_set_members(shyhip_model *h, int nmem, const int *node_start, const int *elem_start);
+
 /* basin and layers (init time): nen3v(3,nel) 1-based, xgv/ygv(nkn),
  * hm3v(3,nel), hev(nel), hlv/hldv(nlvdi), ilhv(nel), ilhkv(nkn).
  * Builds geometry (set_ev), the zeta-matrix pattern (coo_init_new), the
@@ -127,6 +134,9 @@ int  shyhip_pull(shyhip_model *h, int field, void *host, int dtype, size_t n);
 /* forcing ---------------------------------------------------------------- */
 /* level of open boundary ibc (sp111 for ibtyp = 1 boundaries) */
 int  shyhip_set_boundary_level(shyhip_model *h, int ibc, float level);
+/* levels of all open boundaries at once: levels[b-1] for boundary b = 1..n,
+ * n = nbnd (applied to the type 1 boundaries) */
+int  shyhip_set_boundary_levels(shyhip_model *h, int n, const float *levels);
 /* uniform wind stress per unit density (meteo_forcing.f90:973) */
 int  shyhip_set_wind_uniform(shyhip_model *h, float taux, float tauy);
 
@@ -147,9 +157,10 @@ int  shyhip_scal3sh(shyhip_model *h, int itr, int istot, int clip_small);
 int  shyhip_sync(shyhip_model *h);
 /* kernel-group timing with HIP events on the handle's stream (off by
  * default; enabling it adds a synchronisation at the end of every step).
- * id = 0 .. 14: set_dry, copy_uvz, hydro_intern, setnod, zeta_assemble,
+ * id = 0 .. 17: set_dry, copy_uvz, hydro_intern, setnod, zeta_assemble,
  * zeta_solve, system_get_setweg, transports_final, setuvd_baro2l, setdepth,
- * hydro_vertical, ttov_uvint, conz3d_elem, conz3d_node, bndo_setbc */
+ * hydro_vertical, ttov_uvint, conz3d_elem, conz3d_node, bndo_setbc, and
+ * inside zeta_solve the single kernels bilu_factor, bicg_a, bicg_b */
 int  shyhip_set_timing(shyhip_model *h, int on);
 int  shyhip_get_timing(shyhip_model *h, int id, const char **name, double *total_ms, long *count);
 

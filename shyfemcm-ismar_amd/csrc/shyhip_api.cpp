@@ -32,12 +32,14 @@ void timer_begin(Model &M, int id) {
     (void)hipEventCreate(&t.a);
     (void)hipEventCreate(&t.b);
     (void)hipEventRecord(t.a, M.stream);
+    M.tev_open.push_back((int)M.tev.size());
     M.tev.push_back(t);
 }
 
 void timer_end(Model &M) {
-    if (!M.timing || M.tev.empty()) return;
-    (void)hipEventRecord(M.tev.back().b, M.stream);
+    if (!M.timing || M.tev_open.empty()) return;
+    (void)hipEventRecord(M.tev[M.tev_open.back()].b, M.stream);
+    M.tev_open.pop_back();
 }
 
 int timer_flush(Model &M) {
@@ -227,7 +229,7 @@ int shyhip_create(shyhip_model **out, int precision, int nkn, int nel, int nlvdi
     p.grav = 9.81f; p.rowass = 1025.f; p.vismol = 1.0e-6f; p.ireib = 0; p.uvmin = 0.2f;
     p.hzmin = 0.01f; p.hzoff = 0.05f; p.hzon = 0.10f; p.volmin = 1.f; p.iclin = 0;
     p.rtol = (double)1.0e-8f; p.atol = (double)1.0e-8f; p.itermax = 1000; p.refine_rtol = 1e-10;
-    p.block_rows = 0; p.tri_sweeps = 0; p.ilu_sweeps = 0; p.inner_rtol = 1e-6; p.rkpar = 0.f; p.difmol = 1.0e-6f; p.itvd = 0; p.itvdv = 0;
+    p.block_rows = 0; p.tri_sweeps = 0; p.inner_rtol = 1e-6; p.rkpar = 0.f; p.difmol = 1.0e-6f; p.itvd = 0; p.itvdv = 0;
     const size_t ne = nel, nk = nkn, L = nlvdi;
     State &s = M.s;
     int rc = 0;
@@ -291,6 +293,8 @@ int shyhip_destroy(shyhip_model *h) {
                     (void *)v.scal, (void *)v.ctrl, v.a32, (void *)v.r64})
         dfree(q);
     dfree(M.d_err);
+    dfree(M.d_levels);
+    if (M.h_levels) (void)hipHostFree(M.h_levels);
     if (M.h_pinned) (void)hipHostFree(M.h_pinned);
     if (M.stream) (void)hipStreamDestroy(M.stream);
     delete h;
@@ -313,7 +317,6 @@ int shyhip_set_params(shyhip_model *h, const shyhip_params *p) {
     if (p->itvd == 1) return fail(&M, "itvd = 1 (gradient TVD) is not supported by the device path; use 0 or 2");
     if (p->block_rows < 0 || p->block_rows > kMaxBlockRows) return fail(&M, "block_rows must be in 0..1024");
     if (p->tri_sweeps < 0 || p->tri_sweeps > 64) return fail(&M, "tri_sweeps must be in 0..64");
-    if (p->ilu_sweeps < 0 || p->ilu_sweeps > 64) return fail(&M, "ilu_sweeps must be in 0..64");
     if (!(p->inner_rtol > 0. && p->inner_rtol < 1.)) return fail(&M, "inner_rtol must be in (0,1)");
     M.p = *p;
     return 0;
@@ -344,7 +347,32 @@ int shyhip_set_basin(shyhip_model *h, const int *nen3v, const float *xgv, const 
         if (!(M.h_ev[19 * (size_t)ie + 9] > 0.)) return fail(&M, "element " + std::to_string(ie + 1) + " is not counter-clockwise");
     M.nnz = host_node_list(nkn, nel, M.h_nen3v.data(), M.h_rowptr, M.h_cols, M.h_ijp);
     if (M.nnz <= 0) return fail(&M, "building the zeta-matrix pattern failed");
-    M.ncolor = host_color(nkn, nel, M.h_nen3v.data(), M.h_color);
+    if (M.mem_node.empty()) {
+        M.mem_node = {0, nkn};
+        M.mem_elem = {0, nel};
+    }
+    for (size_t m = 0; m + 1 < M.mem_node.size(); m++)
+        for (int ie = M.mem_elem[m]; ie < M.mem_elem[m + 1]; ie++)
+            for (int ii = 0; ii < 3; ii++) {
+                const int k = M.h_nen3v[3 * (size_t)ie + ii];
+                if (k < M.mem_node[m] || k >= M.mem_node[m + 1])
+                    return fail(&M, "element " + std::to_string(ie + 1) + " uses a node of another member");
+            }
+    // element colouring of every member on its own (domain_clusterization_ggu)
+    M.h_color.assign(nel, 0);
+    M.ncolor = 0;
+    for (size_t m = 0; m + 1 < M.mem_node.size(); m++) {
+        const int k0 = M.mem_node[m], e0 = M.mem_elem[m], ne = M.mem_elem[m + 1] - e0;
+        std::vector<int> loc(3 * (size_t)ne), col;
+        for (size_t i = 0; i < loc.size(); i++) loc[i] = M.h_nen3v[3 * (size_t)e0 + i] - k0;
+        const int nc = host_color(M.mem_node[m + 1] - k0, ne, loc.data(), col);
+        if (nc <= 0) {
+            M.ncolor = nc;
+            break;
+        }
+        std::copy(col.begin(), col.end(), M.h_color.begin() + e0);
+        M.ncolor = std::max(M.ncolor, nc);
+    }
     if (M.ncolor <= 0) return fail(&M, "element colouring failed (more than 31 colours)");
     Mesh &m = M.m;
     int rc = 0;
@@ -447,6 +475,12 @@ int shyhip_set_basin(shyhip_model *h, const int *nen3v, const float *xgv, const 
             mx = std::max(mx, bs.rowptr[b1] - bs.rowptr[b0]);
         }
         M.sv.max_blk_nnz = mx;
+        int mxp = 0;
+        for (int b = 0; b < bs.nblk; b++) {
+            const int b0 = b * brows, b1 = std::min(nkn, b0 + brows);
+            mxp = std::max(mxp, bs.cp_ptr[bs.rowptr[b1]] - bs.cp_ptr[bs.rowptr[b0]]);
+        }
+        M.sv.max_blk_pairs = mxp;
         if ((size_t)mx * M.esize() > 64 * 1024) return fail(&M, "block too large for LDS; lower block_rows");
         rc |= dalloc(M, &m.blk_rowptr, bs.rowptr.size()) || upload(M, m.blk_rowptr, bs.rowptr.data(), bs.rowptr.size());
         rc |= dalloc(M, &m.blk_cols, bs.cols.size()) || upload(M, m.blk_cols, bs.cols.data(), bs.cols.size());
@@ -473,9 +507,9 @@ int shyhip_set_basin(shyhip_model *h, const int *nen3v, const float *xgv, const 
             rc |= dalloc_s(M, q, (size_t)nkn);
         const size_t nparts = std::max((size_t)bs.nblk, (size_t)(nkn + kThreads - 1) / kThreads) + 1;
         v.nparts = (int)nparts;
-        rc |= dalloc(M, &v.part, 8 * nparts);
+        rc |= dalloc(M, &v.part, 32 * nparts);        // 4 regions x 8 slots
         rc |= dalloc(M, &v.scal, 16);
-        rc |= dalloc(M, &v.ctrl, 8);
+        rc |= dalloc(M, &v.ctrl, 16);
         rc |= dalloc(M, &v.r64, (size_t)nkn);
         if (M.prec == SHYHIP_FP32) rc |= dalloc(M, (float **)&v.a32, (size_t)M.nnz);
     }
@@ -512,6 +546,14 @@ int shyhip_set_boundaries(shyhip_model *h, int nbnd, const int *ibnd, const int 
     dfree(M.m.ibtyp);
     dfree(M.m.bnd_nodes);
     dfree(M.m.bnd_id);
+    dfree(M.d_levels);
+    if (M.h_levels) (void)hipHostFree(M.h_levels);
+    M.d_levels = nullptr;
+    M.h_levels = nullptr;
+    if (nbnd > 0) {
+        if (dalloc(M, &M.d_levels, nbnd)) return 1;
+        HIPCHK(M, hipHostMalloc((void **)&M.h_levels, nbnd * sizeof(float)));
+    }
     M.m.nbnd_nodes = (int)nodes.size();
     int rc = 0;
     rc |= upload(M, M.m.ibnd, ibnd, nkn) || upload(M, M.m.zbound, zb.data(), nkn) ||
@@ -565,9 +607,28 @@ int shyhip_set_coefficients(shyhip_model *h, const float *fcorv, const float *cz
 int shyhip_tvd_init(shyhip_model *h) {
     if (!h) return fail(nullptr, "null handle");
     Model &M = h->M;
-    std::vector<int> tel, tkn;
-    std::vector<double> tw;
-    host_tvd_tables(M.nkn, M.nel, M.h_nen3v.data(), M.h_x.data(), M.h_y.data(), M.h_ev.data(), tel, tkn, tw);
+    // upwind tables per member (a member's upwind points lie in its own elements)
+    const size_t nel = M.nel;
+    std::vector<int> tel(6 * nel, -1), tkn(18 * nel, 0);
+    std::vector<double> tw(18 * nel, 0.);
+    for (size_t m = 0; m + 1 < M.mem_node.size(); m++) {
+        const int k0 = M.mem_node[m], e0 = M.mem_elem[m];
+        const int nk = M.mem_node[m + 1] - k0, ne = M.mem_elem[m + 1] - e0;
+        std::vector<int> loc(3 * (size_t)ne), ltel, ltkn;
+        std::vector<double> ltw;
+        for (size_t i = 0; i < loc.size(); i++) loc[i] = M.h_nen3v[3 * (size_t)e0 + i] - k0;
+        host_tvd_tables(nk, ne, loc.data(), M.h_x.data() + k0, M.h_y.data() + k0, M.h_ev.data() + 19 * (size_t)e0,
+                        ltel, ltkn, ltw);
+        for (int p = 0; p < 6; p++)
+            for (int ie = 0; ie < ne; ie++) {
+                const int up = ltel[(size_t)p * ne + ie];
+                tel[(size_t)p * nel + e0 + ie] = up < 0 ? -1 : up + e0;
+                for (int q = 0; q < 3; q++) {
+                    tkn[((size_t)q * 6 + p) * nel + e0 + ie] = ltkn[((size_t)q * 6 + p) * ne + ie] + k0;
+                    tw[((size_t)q * 6 + p) * nel + e0 + ie] = ltw[((size_t)q * 6 + p) * ne + ie];
+                }
+            }
+    }
     dfree(M.m.tvd_el); dfree(M.m.tvd_kn); dfree(M.m.tvd_w);
     int rc = 0;
     rc |= dalloc(M, &M.m.tvd_el, tel.size()) || upload(M, M.m.tvd_el, tel.data(), tel.size());
@@ -659,6 +720,30 @@ int shyhip_pull(shyhip_model *h, int field, void *host, int dtype, size_t n) {
 int shyhip_set_boundary_level(shyhip_model *h, int ibc, float level) {
     if (!h) return fail(nullptr, "null handle");
     return launch_set_rzv(h->M, ibc, level);
+}
+
+int shyhip_set_boundary_levels(shyhip_model *h, int n, const float *levels) {
+    if (!h) return fail(nullptr, "null handle");
+    Model &M = h->M;
+    if (n != M.nbnd || !M.d_levels) return fail(&M, "set_boundary_levels: n must equal nbnd");
+    std::copy(levels, levels + n, M.h_levels);
+    HIPCHK(M, hipMemcpyAsync(M.d_levels, M.h_levels, n * sizeof(float), hipMemcpyHostToDevice, M.stream));
+    return launch_set_rzv_all(M);
+}
+
+int shyhip_set_members(shyhip_model *h, int nmem, const int *node_start, const int *elem_start) {
+    if (!h) return fail(nullptr, "null handle");
+    Model &M = h->M;
+    if (M.m.nen3v) return fail(&M, "set_members must precede set_basin");
+    if (nmem < 1 || node_start[0] != 0 || elem_start[0] != 0 || node_start[nmem] != M.nkn ||
+        elem_start[nmem] != M.nel)
+        return fail(&M, "set_members: ranges must start at 0 and end at nkn / nel");
+    for (int m = 0; m < nmem; m++)
+        if (node_start[m + 1] <= node_start[m] || elem_start[m + 1] <= elem_start[m])
+            return fail(&M, "set_members: empty or decreasing member range");
+    M.mem_node.assign(node_start, node_start + nmem + 1);
+    M.mem_elem.assign(elem_start, elem_start + nmem + 1);
+    return 0;
 }
 
 int shyhip_set_wind_uniform(shyhip_model *h, float taux, float tauy) {
@@ -817,7 +902,7 @@ int shyhip_get_timing(shyhip_model *h, int id, const char **name, double *total_
     static const char *names[T_NTIMERS] = {"set_dry", "copy_uvz", "hydro_intern", "setnod", "zeta_assemble",
                                            "zeta_solve", "system_get_setweg", "transports_final", "setuvd_baro2l",
                                            "setdepth", "hydro_vertical", "ttov_uvint", "conz3d_elem", "conz3d_node",
-                                           "bndo_setbc"};
+                                           "bndo_setbc", "bilu_factor", "bicg_a", "bicg_b"};
     if (!h) return fail(nullptr, "null handle");
     if (id < 0 || id >= T_NTIMERS) return fail(&h->M, "timer id out of range");
     if (timer_flush(h->M)) return 1;

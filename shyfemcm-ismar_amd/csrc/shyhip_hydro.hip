@@ -169,7 +169,7 @@ __global__ __launch_bounds__(kThreads) void k_hydro_intern(HydroArgs<S> a) {
         double xd[NL], up[NL], gx[NL], gy[NL], gh[NL];
 #pragma unroll
         for (int l = 1; l <= NL; l++) {
-            if (l > ilevel) break;
+            if (l > ilevel) continue;
             const bool bfirst = (l == 1), blast = (l == ilevel);
             const int lp = (l + 1 < ilevel) ? l + 1 : ilevel;
             const int lm = (l - 1 > 1) ? l - 1 : 1;
@@ -331,7 +331,7 @@ __global__ __launch_bounds__(kThreads) void k_hydro_intern(HydroArgs<S> a) {
     double dsum = 0.;
 #pragma unroll
     for (int l = 1; l <= NL; l++) {
-        if (l > ilevel) break;
+        if (l > ilevel) continue;
         const R uo = a.uto[(size_t)(l - 1) * nel + ie], vo = a.vto[(size_t)(l - 1) * nel + ie];
         const S unew = (S)((double)uo - (double)dtafix * sxu[l - 1]);
         const S vnew = (S)((double)vo - (double)dtafix * sxv[l - 1]);
@@ -904,6 +904,13 @@ __global__ void k_set_rzv(S *rzv, const int *nodes, const int *bid, int n, int i
     if (i < n && bid[i] == ibc) rzv[nodes[i]] = level;
 }
 
+// all level boundaries (type 1) at once: level[b-1] on the nodes of boundary b
+template <typename S>
+__global__ void k_set_rzv_all(S *rzv, const int *nodes, const int *bid, const int *ibtyp, int n, const float *level) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && ibtyp[bid[i]] == 1) rzv[nodes[i]] = (S)level[bid[i] - 1];
+}
+
 // FP64 total volume (masscont, elems_dealing.f90:1191-1220): per-block sums
 // in a fixed tree order, then a single-block pass over the partials.
 template <typename S>
@@ -1124,6 +1131,18 @@ static int set_rzv_t(Model &M, int ibc, float level) {
 }
 int launch_set_rzv(Model &M, int ibc, float level) {
     return M.prec == SHYHIP_FP64 ? set_rzv_t<double>(M, ibc, level) : set_rzv_t<float>(M, ibc, level);
+}
+
+int launch_set_rzv_all(Model &M) {
+    const int n = M.m.nbnd_nodes;
+    if (n == 0) return 0;
+    if (M.prec == SHYHIP_FP64)
+        LAUNCH(M, k_set_rzv_all<double>, nblocks(n), (double *)M.s.rzv, M.m.bnd_nodes, M.m.bnd_id, M.m.ibtyp, n,
+               (const float *)M.d_levels);
+    else
+        LAUNCH(M, k_set_rzv_all<float>, nblocks(n), (float *)M.s.rzv, M.m.bnd_nodes, M.m.bnd_id, M.m.ibtyp, n,
+               (const float *)M.d_levels);
+    return 0;
 }
 
 int launch_fill(Model &M, void *ptr, double value, size_t n) {

@@ -80,23 +80,22 @@ struct State {
 };
 
 struct Solver {               // Krylov work space (value type V = S of the model)
-    int nblk = 0, brows = 0, max_blk_nnz = 0, nparts = 0;
+    int nblk = 0, brows = 0, max_blk_nnz = 0, max_blk_pairs = 0, nparts = 0;
     int last_its[8] = {};     // iteration count of the previous solve, per solve kind
-    bool lu_valid = false;    // lu holds factors of an earlier matrix (warm start of the sweeps)
     int last_nref = 0;        // FP64 refinement rounds of the previous solve
     void *lu = nullptr;       // block factors (values of the block pattern)
     void *r0 = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr, *v0 = nullptr, *v1 = nullptr,
          *s = nullptr, *t = nullptr, *dx = nullptr, *b = nullptr, *x0 = nullptr;
-    double *part = nullptr;   // [8*nparts] per-workgroup partial sums (4 regions x 2)
+    double *part = nullptr;   // [32*nparts] per-workgroup partial sums (4 regions x 8)
     double *scal = nullptr;   // [16] device scalars
-    int *ctrl = nullptr;      // [8] ticket counter, flags, matvec counter
+    int *ctrl = nullptr;      // [16] control words of the solve (flags, counters)
     void *a32 = nullptr;      // FP32 copy of the matrix values (FP32 path)
     double *r64 = nullptr;    // FP64 residual (refinement)
 };
 
 // kernel groups timed with HIP events when timing is enabled
 enum TimerId { T_SETDRY = 0, T_COPY, T_INTERN, T_SETNOD, T_ZETA, T_SOLVE, T_WETDRY, T_FINAL, T_SETUVD, T_DEPTH,
-               T_VERT, T_TTOV, T_CONZ_ELEM, T_CONZ_NODE, T_BNDO, T_NTIMERS };
+               T_VERT, T_TTOV, T_CONZ_ELEM, T_CONZ_NODE, T_BNDO, T_FACTOR, T_BICG_A, T_BICG_B, T_NTIMERS };
 
 struct Model {
     int prec = SHYHIP_FP64;
@@ -113,6 +112,10 @@ struct Model {
     double dtime = 0., dtime_old = -1.;
     bool has_coriolis = false, has_mflux = false, tvd_ready = false;
     int last_nloop = 0, last_iters = 0, last_refine = 0;
+    // independent basins held in one model: member m owns nodes
+    // [mem_node[m], mem_node[m+1]) and elements [mem_elem[m], mem_elem[m+1])
+    std::vector<int> mem_node, mem_elem;
+    float *d_levels = nullptr, *h_levels = nullptr;   // boundary levels staging (nbnd)
     std::string err;
     // host-side copies needed for init / plugin / checks
     std::vector<int> h_nen3v, h_rowptr, h_cols, h_ijp, h_color, h_ilhv, h_ilhkv, h_ibnd, h_ibtyp, h_iextb;
@@ -124,6 +127,7 @@ struct Model {
     bool timing = false;
     struct TEv { int id; hipEvent_t a, b; };
     std::vector<TEv> tev;
+    std::vector<int> tev_open;          // groups begun and not yet ended (nesting)
     double t_ms[T_NTIMERS] = {};
     long t_cnt[T_NTIMERS] = {};
 };
@@ -146,9 +150,9 @@ void host_tvd_tables(int nkn, int nel, const int *nen3v0, const float *x, const 
 struct BlockSchedule {
     int nblk = 0, brows = 0;
     std::vector<int> rowptr, cols, src, diag, fwd_rows, fwd_lptr, fwd_nlev, bwd_rows, bwd_lptr, bwd_nlev;
-    // fine-grained ILU(0) sweeps: for entry q of the block pattern, the pairs
-    // (q_ik, q_kj), k < min(i, j), of the ILU(0) update, and ent_jd[q] = the
-    // diagonal entry of column j for an L entry, -2 for a diagonal, -1 for U
+    // ILU(0) updates: for entry q of the block pattern, the pairs (q_ik, q_kj),
+    // k < min(i, j) ascending, and ent_jd[q] = the diagonal entry of column j
+    // for an L entry, -2 for a diagonal, -1 for U
     std::vector<int> cp_ptr, cp_a, cp_b, ent_jd;
 };
 void host_block_schedule(int n, const std::vector<int> &rowptr, const std::vector<int> &cols, int brows,
@@ -168,6 +172,7 @@ int launch_ttov_uvint(Model &M);
 int launch_init_levels(Model &M, float zconst);
 int launch_copy_old(Model &M);
 int launch_set_rzv(Model &M, int ibc, float level);
+int launch_set_rzv_all(Model &M);
 int launch_fill(Model &M, void *ptr, double value, size_t n);
 int launch_volume(Model &M, int mode, double *vol);
 int launch_tracer_mass(Model &M, int itr, double *mass);

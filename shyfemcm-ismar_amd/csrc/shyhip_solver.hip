@@ -1,16 +1,14 @@
 // shyhip_solver.hip -- zeta system solve on gfx950.
 //
 // Preconditioner: block-Jacobi ILU(0).  Rows are split into contiguous blocks
-// (<= 1024 rows, one workgroup each).  Each diagonal block is factorised
-// either exactly, with level scheduling in LDS (rows of one level in parallel,
-// workgroup barriers between levels; a basin of at most one block of nodes
-// then gets exactly ILU(0), spk_ilut.f90:1832-1950, lusol :2309-2345), or by
-// fine-grained fixed-point sweeps over all factor entries at once (the
-// asynchronous ILU of Chow and Patel, SIAM J. Sci. Comput. 37 (2015) C169,
-// run here as Jacobi sweeps so the result is deterministic), started from the
-// factors of the previous solve: the zeta matrix changes little from one
-// step to the next, so a few sweeps track it.  The block factors are staged
-// in LDS by every kernel that applies them.
+// (<= 1024 rows, one workgroup each); each diagonal block is factorised in
+// LDS with level scheduling (rows of one level in parallel, workgroup
+// barriers between levels).  Every factor entry (i,j) is computed as
+// a_ij - sum_k l_ik u_kj over the pattern, k ascending -- the updates of
+// ilu0's IKJ loop (spk_ilut.f90:1832-1950) in their order, listed once on
+// the host -- so a basin of at most one block of nodes gets exactly ILU(0);
+// lusol :2309-2345 applies it.  The block factors are staged in LDS by every
+// kernel that applies them.
 //
 // Krylov method: BiCGStab in the arrangement the reference uses -- runrc with
 // ipar(2)=1, which bisinit maps to LEFT preconditioning (spk_iters.f90:
@@ -38,16 +36,18 @@ namespace shy {
 // kernel never overwrites a value that another workgroup of it still reads
 enum { SC_TARGET = 0, SC_NORM0 = 1, SC_RHO = 2, SC_ALPHA = 4, SC_OMEGA = 6, SC_RNORM = 8, SC_RES64 = 9,
        SC_BNORM = 10 };
-// control words: convergence / failure of the current Krylov solve, its
+// control words: iteration (+1) at which the current Krylov solve stopped,
+// its convergence (same stamp) / breakdown / iteration-limit flags, its
 // matvec count, iterations of all inner solves of this zeta solve, FP64
 // refinement converged, factorisation failure, refinements done
-enum { CT_CONV = 1, CT_FAIL = 2, CT_NMV = 3, CT_ITS = 4, CT_REFCONV = 5, CT_FACTOR = 6, CT_NREF = 7 };
+enum { CT_STOPIT = 0, CT_CONV = 1, CT_FAIL = 2, CT_NMV = 3, CT_ITS = 4, CT_REFCONV = 5, CT_FACTOR = 6,
+       CT_NREF = 7, CT_MAXIT = 8, CT_WORDS = 16 };
 // partial-sum regions: [region][q][workgroup]
 enum { PR_A = 0, PR_B = 1, PR_C = 2, PR_R = 3 };
 
 template <typename V>
 struct SolvArgs {
-    int n, nblk, brows, mx, itermax, sweeps;
+    int n, nblk, brows, mx, mxp, itermax, sweeps;
     double rtol, atol;
     const int *rowptr, *cols;
     const V *A;
@@ -64,7 +64,8 @@ struct SolvArgs {
 template <typename V>
 static SolvArgs<V> solv_args(Model &M, const V *A, V *b, V *x0) {
     SolvArgs<V> a{};
-    a.n = M.nkn; a.nblk = M.sv.nblk; a.brows = M.sv.brows; a.mx = M.sv.max_blk_nnz; a.itermax = M.p.itermax;
+    a.n = M.nkn; a.nblk = M.sv.nblk; a.brows = M.sv.brows; a.mx = M.sv.max_blk_nnz; a.mxp = M.sv.max_blk_pairs;
+    a.itermax = M.p.itermax;
     a.sweeps = M.p.tri_sweeps;
     a.rtol = M.p.rtol; a.atol = M.p.atol;
     a.rowptr = M.m.rowptr; a.cols = M.m.cols; a.A = A;
@@ -82,39 +83,48 @@ static SolvArgs<V> solv_args(Model &M, const V *A, V *b, V *x0) {
 }
 
 // ---------------------------------------------------------------- reductions
-// per-workgroup partials: part[(region*2 + q)*nparts + workgroup]
+// Workgroup sums: butterfly within each 64-lane wavefront (every lane ends
+// with the same total), then the wavefront totals added in a fixed order.
+// The same data always give the same bits.
+constexpr int kWaves = kThreads / 64;
+constexpr int kRegion = 8;            // partial-sum slots per region
+
+__device__ inline double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <int NQ>
+__device__ inline void block_sum(double (&v)[NQ]) {
+    __shared__ double red[NQ][kWaves];
+    for (int q = 0; q < NQ; q++) v[q] = wave_sum(v[q]);
+    if ((threadIdx.x & 63) == 0)
+        for (int q = 0; q < NQ; q++) red[q][threadIdx.x >> 6] = v[q];
+    __syncthreads();
+    for (int q = 0; q < NQ; q++) {
+        double t = red[q][0];
+        for (int w = 1; w < kWaves; w++) t += red[q][w];
+        v[q] = t;
+    }
+    __syncthreads();
+}
+
+// per-workgroup partials: part[(region*kRegion + q)*nparts + workgroup]
 template <int NQ>
 __device__ inline void store_partials(double (&vals)[NQ], double *part, int region, int nparts) {
-    __shared__ double red[NQ][kThreads];
-    for (int q = 0; q < NQ; q++) red[q][threadIdx.x] = vals[q];
-    __syncthreads();
-    for (int s = kThreads / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s)
-            for (int q = 0; q < NQ; q++) red[q][threadIdx.x] += red[q][threadIdx.x + s];
-        __syncthreads();
-    }
+    block_sum<NQ>(vals);
     if (threadIdx.x == 0)
-        for (int q = 0; q < NQ; q++) part[((size_t)region * 2 + q) * nparts + blockIdx.x] = red[q][0];
+        for (int q = 0; q < NQ; q++) part[((size_t)region * kRegion + q) * nparts + blockIdx.x] = vals[q];
 }
 
 // total of the partials of the previous kernel (n workgroups), computed by
 // every workgroup in the same order; the result is returned to all threads
 template <int NQ>
 __device__ inline void sum_partials(const double *part, int region, int nparts, int n, double (&out)[NQ]) {
-    __shared__ double red[NQ][kThreads];
-    double loc[NQ];
-    for (int q = 0; q < NQ; q++) loc[q] = 0.;
+    for (int q = 0; q < NQ; q++) out[q] = 0.;
     for (int i = threadIdx.x; i < n; i += kThreads)
-        for (int q = 0; q < NQ; q++) loc[q] += part[((size_t)region * 2 + q) * nparts + i];
-    for (int q = 0; q < NQ; q++) red[q][threadIdx.x] = loc[q];
-    __syncthreads();
-    for (int s = kThreads / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s)
-            for (int q = 0; q < NQ; q++) red[q][threadIdx.x] += red[q][threadIdx.x + s];
-        __syncthreads();
-    }
-    for (int q = 0; q < NQ; q++) out[q] = red[q][0];
-    __syncthreads();
+        for (int q = 0; q < NQ; q++) out[q] += part[((size_t)region * kRegion + q) * nparts + i];
+    block_sum<NQ>(out);
 }
 
 __device__ inline bool breakdown(double x) {   // brkdn: zero or not finite
@@ -236,92 +246,61 @@ __device__ inline void block_lusol(const SolvArgs<V> &a, const BlockView<V> &B, 
     }
 }
 
-// ILU(0) of each diagonal block (ilu0's IKJ arrangement; the pivot is stored
-// inverted), level by level in LDS.
-template <typename V>
+// ILU(0) of each diagonal block, level by level (the pivot is stored
+// inverted).  Row i, entries in column order:
+//   l_ij = (a_ij - sum_{k<j} l_ik u_kj) * (1/u_jj)   (j < i)
+//   u_ij =  a_ij - sum_{k<i} l_ik u_kj               (j >= i)
+// with the (k ascending) update pairs of entry q in cp_a/cp_b[cp_ptr[q]..].
+// STAGE: the block's update lists are copied to LDS as well.
+template <typename V, bool STAGE>
 __global__ __launch_bounds__(kThreads) void k_bilu_factor(SolvArgs<V> a) {
     extern __shared__ double smem[];
     V *val = reinterpret_cast<V *>(smem);
     const int blk = blockIdx.x, b0 = blk * a.brows;
     const int nb = min(a.n, b0 + a.brows) - b0;
-    const int q0 = a.brp[b0], q1 = a.brp[b0 + nb];
+    const int q0 = a.brp[b0], q1 = a.brp[b0 + nb], ne = q1 - q0;
+    const int p0 = a.cp_ptr[q0], np = a.cp_ptr[q1] - p0;
+    const int *cpp = a.cp_ptr + q0, *cpa = a.cp_a + p0, *cpb = a.cp_b + p0, *jdv = a.bjd + q0;
+    int qoff = q0, poff = p0;       // entry / pair index bases of the lists in use
+    if (STAGE) {
+        int *lp = reinterpret_cast<int *>(val + a.mx);
+        int *la = lp + a.mx + 1, *lb = la + a.mxp, *lj = lb + a.mxp;
+        for (int e = threadIdx.x; e <= ne; e += blockDim.x) {
+            lp[e] = a.cp_ptr[q0 + e];
+            if (e < ne) lj[e] = a.bjd[q0 + e];
+        }
+        for (int p = threadIdx.x; p < np; p += blockDim.x) {
+            la[p] = a.cp_a[p0 + p];
+            lb[p] = a.cp_b[p0 + p];
+        }
+        cpp = lp; cpa = la; cpb = lb; jdv = lj;
+    }
     for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) val[q - q0] = (V)a.aval[a.bsrc[q]];
     __syncthreads();
     const int stride = a.brows + 1;
-    const int *lp = a.flptr + (size_t)blk * stride;
+    const int *lvp = a.flptr + (size_t)blk * stride;
     const int nf = a.fnlev[blk];
     for (int lv = 0; lv < nf; lv++) {
-        for (int t = lp[lv] + threadIdx.x; t < lp[lv + 1]; t += blockDim.x) {
+        for (int t = lvp[lv] + threadIdx.x; t < lvp[lv + 1]; t += blockDim.x) {
             const int li = a.frows[b0 + t], i = b0 + li;
-            const int ie0 = a.brp[i], ied = a.bdiag[i], ie1 = a.brp[i + 1];
-            for (int q = ie0; q < ied; q++) {
-                const int jrow = b0 + a.bcol[q];
-                const V tl = val[q - q0] * val[a.bdiag[jrow] - q0];
-                val[q - q0] = tl;
-                // subtract tl * (U part of row jrow) where row i has an entry
-                int qi = q + 1;
-                for (int jj = a.bdiag[jrow] + 1; jj < a.brp[jrow + 1]; jj++) {
-                    const int cj = a.bcol[jj];
-                    while (qi < ie1 && a.bcol[qi] < cj) qi++;
-                    if (qi < ie1 && a.bcol[qi] == cj) val[qi - q0] = val[qi - q0] - tl * val[jj - q0];
-                }
+            const int r0 = a.brp[i] - q0, r1 = a.brp[i + 1] - q0, dg = a.bdiag[i] - q0;
+            for (int e = r0; e < r1; e++) {
+                V acc = val[e];
+                for (int p = cpp[e] - poff; p < cpp[e + 1] - poff; p++)
+                    acc = acc - val[cpa[p] - qoff] * val[cpb[p] - qoff];
+                const int jd = jdv[e];
+                val[e] = jd >= 0 ? acc * val[jd - qoff] : acc;
             }
-            const V piv = val[ied - q0];
+            const V piv = val[dg];
             if (piv == (V)0) atomicCAS(a.ctrl + CT_FACTOR, 0, 1 + i);
-            val[ied - q0] = (V)1 / piv;
+            val[dg] = (V)1 / piv;
         }
         __syncthreads();
     }
     for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) a.lu[q] = val[q - q0];
 }
 
-// `sweeps` Jacobi sweeps of the ILU(0) fixed point, per block, from the
-// factors in lu: for every entry (i,j) of the block pattern at once,
-//   l_ij = (a_ij - sum_{k<j} l_ik u_kj) / u_jj   (i > j)
-//   u_ij =  a_ij - sum_{k<i} l_ik u_kj           (i <= j)
-// reading the previous sweep's values (two LDS buffers).  Its fixed point is
-// the ILU(0) factorisation.
-template <typename V>
-__global__ __launch_bounds__(kThreads) void k_bilu_sweep(SolvArgs<V> a, int sweeps) {
-    extern __shared__ double smem[];
-    V *av = reinterpret_cast<V *>(smem);
-    V *x0 = av + a.mx, *x1 = x0 + a.mx;
-    const int blk = blockIdx.x, b0 = blk * a.brows;
-    const int nb = min(a.n, b0 + a.brows) - b0;
-    const int q0 = a.brp[b0], ne = a.brp[b0 + nb] - q0;
-    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
-        const int q = q0 + e;
-        av[e] = (V)a.aval[a.bsrc[q]];
-        const V f = a.lu[q];
-        x0[e] = a.bjd[q] == -2 ? (V)1 / f : f;
-    }
-    __syncthreads();
-    for (int sw = 0; sw < sweeps; sw++) {
-        for (int e = threadIdx.x; e < ne; e += blockDim.x) {
-            const int q = q0 + e;
-            V acc = av[e];
-            for (int p = a.cp_ptr[q]; p < a.cp_ptr[q + 1]; p++) acc = acc - x0[a.cp_a[p] - q0] * x0[a.cp_b[p] - q0];
-            const int jd = a.bjd[q];
-            x1[e] = jd >= 0 ? acc / x0[jd - q0] : acc;
-        }
-        __syncthreads();
-        V *t = x0;
-        x0 = x1;
-        x1 = t;
-    }
-    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
-        const int q = q0 + e;
-        const V f = x0[e];
-        if (a.bjd[q] == -2) {
-            if (f == (V)0) atomicCAS(a.ctrl + CT_FACTOR, 0, 1 + b0);
-            a.lu[q] = (V)1 / f;
-        } else {
-            a.lu[q] = f;
-        }
-    }
-}
-
-// r0 = M^-1 (b - A x0) (x0 = 0 when zero_x0); r = p = r0; (r0,r0) into both
+// r0 = M^-1 (b - A x0) (x0 = 0 when zero_x0), dx = 0; (r0,r0) into both
 // slots of region C
 template <typename V, bool STAGE>
 __global__ __launch_bounds__(kThreads) void k_bicg_init(SolvArgs<V> a, int zero_x0) {
@@ -344,72 +323,120 @@ __global__ __launch_bounds__(kThreads) void k_bicg_init(SolvArgs<V> a, int zero_
     for (int li = threadIdx.x; li < nb; li += blockDim.x) {
         const int i = b0 + li;
         const V r = y[li];
-        a.r0[i] = r; a.r[i] = r; a.p0[i] = r;
-        a.v0[i] = 0; a.dx[i] = 0;
+        a.r0[i] = r;
+        a.dx[i] = 0;
         loc[0] += (double)r * (double)r;
     }
     loc[1] = loc[0];
     store_partials<2>(loc, a.part, PR_C, a.nparts);
-    if (blk == 0 && threadIdx.x == 0) { a.ctrl[CT_CONV] = 0; a.ctrl[CT_FAIL] = 0; a.ctrl[CT_NMV] = 1; }
+    if (blk == 0 && threadIdx.x == 0) {
+        a.ctrl[CT_STOPIT] = 0; a.ctrl[CT_CONV] = 0; a.ctrl[CT_FAIL] = 0; a.ctrl[CT_MAXIT] = 0; a.ctrl[CT_NMV] = 1;
+    }
 }
 
-// iteration it, first kernel: convergence test of the previous iteration
-// (stopbis), beta, p = r + beta (p - omega v) (fused), v = M^-1 A p, (r0,v)
+// Iteration it, first kernel.  For it > 0 it completes iteration it-1 from
+// the partial sums of k_bicg_b: omega = (t,s)/(t,t), x += alpha p + omega s,
+// r = s - omega t, with |r|^2 = (s,s) - 2 omega (t,s) + omega^2 (t,t) and
+// rho = (r0,s) - omega (r0,t); then the stopping test (stopbis), beta, the
+// new direction p = r + beta (p - omega v) (neighbour rows formed on the
+// fly), v = M^-1 A p and the partial (r0,v).  A stop decided by this kernel
+// (convergence, iteration limit) is stamped with it+1 in CT_STOPIT, so that
+// its own workgroups still finish the update of x; breakdowns are found by
+// every workgroup before any update.
 template <typename V, bool STAGE>
-__global__ __launch_bounds__(kThreads) void k_bicg_pv(SolvArgs<V> a, int it) {
-    if (a.ctrl[CT_CONV] || a.ctrl[CT_FAIL] || a.ctrl[CT_REFCONV]) return;
+__global__ __launch_bounds__(kThreads) void k_bicg_a(SolvArgs<V> a, int it) {
+    const int stop = a.ctrl[CT_STOPIT];
+    if (a.ctrl[CT_FAIL] || a.ctrl[CT_REFCONV] || (stop != 0 && stop <= it)) return;
     extern __shared__ double smem[];
     V *y = reinterpret_cast<V *>(smem), *w = y + a.brows;
-    const int par = it & 1;
+    const int par = it & 1, pp = par ^ 1;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    double rs[2];
-    sum_partials<2>(a.part, PR_C, a.nparts, a.nblk, rs);
-    const double rn = sqrt(rs[0]);
-    double target, beta = 0., omega = 0.;
+    const int blk = blockIdx.x, b0 = blk * a.brows;
+    const int nb = min(a.n, b0 + a.brows) - b0;
+    double rn, rho, omega = 0., alpha = 0.;
+    if (it == 0) {
+        double rs[2];
+        sum_partials<2>(a.part, PR_C, a.nparts, a.nblk, rs);
+        rn = sqrt(rs[0]);
+        rho = rs[1];
+    } else {
+        double q[5];   // (t,t) (t,s) (s,s) (r0,s) (r0,t)
+        sum_partials<5>(a.part, PR_B, a.nparts, a.nblk, q);
+        if (breakdown(q[0])) {
+            if (lead) a.ctrl[CT_FAIL] = -3;
+            return;
+        }
+        omega = q[1] / q[0];
+        if (breakdown(omega)) {
+            if (lead) a.ctrl[CT_FAIL] = -3;
+            return;
+        }
+        alpha = a.scal[SC_ALPHA + pp];
+        rn = sqrt(fmax(0., q[2] - 2. * omega * q[1] + omega * omega * q[0]));
+        rho = q[3] - omega * q[4];
+        const V va = (V)alpha, vo = (V)omega;
+        const V *pold = pp ? a.p1 : a.p0;
+        for (int li = threadIdx.x; li < nb; li += blockDim.x) {
+            const int i = b0 + li;
+            a.dx[i] = a.dx[i] + (va * pold[i] + vo * a.s[i]);
+            a.r[i] = a.s[i] - vo * a.t[i];
+        }
+        if (lead) { a.scal[SC_OMEGA + pp] = omega; a.ctrl[CT_ITS] += 1; }
+    }
+    double target;
     if (it == 0) {
         target = a.rtol * rn + a.atol;
         if (lead) { a.scal[SC_TARGET] = target; a.scal[SC_NORM0] = rn; }
     } else {
-        if (a.itermax > 0 && 1 + 2 * it >= a.itermax) {
-            if (lead) a.ctrl[CT_FAIL] = -1;
-            return;
-        }
         target = a.scal[SC_TARGET];
     }
     if (lead) { a.scal[SC_RNORM] = rn; a.ctrl[CT_NMV] = 1 + 2 * it; }
     if (!(rn > target)) {
-        if (lead) a.ctrl[CT_CONV] = 1;
+        if (lead) { a.ctrl[CT_STOPIT] = it + 1; a.ctrl[CT_CONV] = 1; }
         return;
     }
-    const double rho = rs[1];
+    if (it > 0 && a.itermax > 0 && 1 + 2 * it >= a.itermax) {
+        if (lead) { a.ctrl[CT_STOPIT] = it + 1; a.ctrl[CT_MAXIT] = 1; }
+        return;
+    }
+    double beta = 0.;
     if (it > 0) {
-        const int pp = (it - 1) & 1;
-        omega = a.scal[SC_OMEGA + pp];
-        beta = rho * a.scal[SC_ALPHA + pp] / (omega * a.scal[SC_RHO + pp]);
         if (breakdown(rho)) {
             if (lead) a.ctrl[CT_FAIL] = -3;
             return;
         }
+        beta = rho * alpha / (omega * a.scal[SC_RHO + pp]);
     }
     if (lead) a.scal[SC_RHO + par] = rho;
-    const V *pold = par ? a.p1 : a.p0;
-    V *pnew = par ? a.p0 : a.p1;
-    const V *vold = par ? a.v1 : a.v0;
-    V *vnew = par ? a.v0 : a.v1;
+    const V *pold = pp ? a.p1 : a.p0;     // direction of iteration it-1 (it = 0: r0)
+    V *pnew = par ? a.p1 : a.p0;
+    const V *vold = pp ? a.v1 : a.v0;
+    V *vnew = par ? a.v1 : a.v0;
     const V vb = (V)beta, vo = (V)omega;
-    const int blk = blockIdx.x, b0 = blk * a.brows;
-    const int nb = min(a.n, b0 + a.brows) - b0;
     const BlockView<V> B = load_block<V, STAGE>(a, b0, nb, reinterpret_cast<unsigned char *>(w + a.brows));
-    for (int li = threadIdx.x; li < nb; li += blockDim.x) {
-        const int i = b0 + li;
-        V t = 0;
-        for (int q = a.rowptr[i]; q < a.rowptr[i + 1]; q++) {
-            const int j = a.cols[q];
-            const V pj = a.r[j] + vb * (pold[j] - vo * vold[j]);
-            t = t + a.A[q] * pj;
+    if (it == 0) {
+        // p = r0 (set by init in p0 = p of parity 0's predecessor slot)
+        for (int li = threadIdx.x; li < nb; li += blockDim.x) {
+            const int i = b0 + li;
+            V t = 0;
+            for (int q = a.rowptr[i]; q < a.rowptr[i + 1]; q++) t = t + a.A[q] * a.r0[a.cols[q]];
+            y[li] = t;
+            pnew[i] = a.r0[i];
         }
-        y[li] = t;
-        pnew[i] = a.r[i] + vb * (pold[i] - vo * vold[i]);
+    } else {
+        for (int li = threadIdx.x; li < nb; li += blockDim.x) {
+            const int i = b0 + li;
+            V t = 0;
+            for (int q = a.rowptr[i]; q < a.rowptr[i + 1]; q++) {
+                const int j = a.cols[q];
+                const V rj = a.s[j] - vo * a.t[j];
+                const V pj = rj + vb * (pold[j] - vo * vold[j]);
+                t = t + a.A[q] * pj;
+            }
+            y[li] = t;
+            const V ri = a.s[i] - vo * a.t[i];
+            pnew[i] = ri + vb * (pold[i] - vo * vold[i]);
+        }
     }
     __syncthreads();
     block_lusol(a, B, blk, b0, nb, y, w);
@@ -422,10 +449,11 @@ __global__ __launch_bounds__(kThreads) void k_bicg_pv(SolvArgs<V> a, int it) {
     store_partials<1>(loc, a.part, PR_A, a.nparts);
 }
 
-// alpha = rho / (r0,v); s = r - alpha v (fused); t = M^-1 A s; (t,t), (t,s)
+// Iteration it, second kernel: alpha = rho / (r0,v); s = r - alpha v
+// (fused); t = M^-1 A s; partials (t,t) (t,s) (s,s) (r0,s) (r0,t).
 template <typename V, bool STAGE>
-__global__ __launch_bounds__(kThreads) void k_bicg_st(SolvArgs<V> a, int it) {
-    if (a.ctrl[CT_CONV] || a.ctrl[CT_FAIL] || a.ctrl[CT_REFCONV]) return;
+__global__ __launch_bounds__(kThreads) void k_bicg_b(SolvArgs<V> a, int it) {
+    if (a.ctrl[CT_STOPIT] || a.ctrl[CT_FAIL] || a.ctrl[CT_REFCONV]) return;
     extern __shared__ double smem[];
     V *y = reinterpret_cast<V *>(smem), *w = y + a.brows;
     const int par = it & 1;
@@ -438,7 +466,8 @@ __global__ __launch_bounds__(kThreads) void k_bicg_st(SolvArgs<V> a, int it) {
     }
     const double alpha = a.scal[SC_RHO + par] / d[0];
     if (lead) a.scal[SC_ALPHA + par] = alpha;
-    const V *vnew = par ? a.v0 : a.v1;
+    const V *vnew = par ? a.v1 : a.v0;
+    const V *rv = it == 0 ? a.r0 : a.r;
     const V va = (V)alpha;
     const int blk = blockIdx.x, b0 = blk * a.brows;
     const int nb = min(a.n, b0 + a.brows) - b0;
@@ -448,58 +477,26 @@ __global__ __launch_bounds__(kThreads) void k_bicg_st(SolvArgs<V> a, int it) {
         V t = 0;
         for (int q = a.rowptr[i]; q < a.rowptr[i + 1]; q++) {
             const int j = a.cols[q];
-            const V sj = a.r[j] - va * vnew[j];
+            const V sj = rv[j] - va * vnew[j];
             t = t + a.A[q] * sj;
         }
         y[li] = t;
-        a.s[i] = a.r[i] - va * vnew[i];
+        a.s[i] = rv[i] - va * vnew[i];
     }
     __syncthreads();
     block_lusol(a, B, blk, b0, nb, y, w);
-    double loc[2] = {0., 0.};
+    double loc[5] = {0., 0., 0., 0., 0.};
     for (int li = threadIdx.x; li < nb; li += blockDim.x) {
         const int i = b0 + li;
-        const V tv = y[li];
-        a.t[i] = tv;
-        loc[0] += (double)tv * (double)tv;
-        loc[1] += (double)tv * (double)a.s[i];
+        const double tv = (double)y[li], sv = (double)a.s[i], r0 = (double)a.r0[i];
+        a.t[i] = y[li];
+        loc[0] += tv * tv;
+        loc[1] += tv * sv;
+        loc[2] += sv * sv;
+        loc[3] += r0 * sv;
+        loc[4] += r0 * tv;
     }
-    store_partials<2>(loc, a.part, PR_B, a.nparts);
-}
-
-// omega = (t,s)/(t,t); dx += alpha p + omega s; r = s - omega t; (r,r), (r,r0)
-template <typename V>
-__global__ __launch_bounds__(kThreads) void k_bicg_update(SolvArgs<V> a, int it) {
-    if (a.ctrl[CT_CONV] || a.ctrl[CT_FAIL] || a.ctrl[CT_REFCONV]) return;
-    const int par = it & 1;
-    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    double ts[2];
-    sum_partials<2>(a.part, PR_B, a.nparts, a.nblk, ts);
-    if (breakdown(ts[0])) {
-        if (lead) a.ctrl[CT_FAIL] = -3;
-        return;
-    }
-    const double om = ts[1] / ts[0];
-    if (breakdown(om)) {
-        if (lead) a.ctrl[CT_FAIL] = -3;
-        return;
-    }
-    if (lead) { a.scal[SC_OMEGA + par] = om; a.ctrl[CT_ITS] += 1; }
-    const V *pnew = par ? a.p0 : a.p1;
-    const V alpha = (V)a.scal[SC_ALPHA + par], omega = (V)om;
-    const int blk = blockIdx.x, b0 = blk * a.brows;
-    const int nb = min(a.n, b0 + a.brows) - b0;
-    double loc[2] = {0., 0.};
-    for (int li = threadIdx.x; li < nb; li += blockDim.x) {
-        const int i = b0 + li;
-        const V tmp = alpha * pnew[i] + omega * a.s[i];
-        a.dx[i] = a.dx[i] + tmp;
-        const V r = a.s[i] - omega * a.t[i];
-        a.r[i] = r;
-        loc[0] += (double)r * (double)r;
-        loc[1] += (double)r * (double)a.r0[i];
-    }
-    store_partials<2>(loc, a.part, PR_C, a.nparts);
+    store_partials<5>(loc, a.part, PR_B, a.nparts);
 }
 
 // x = x0 + dx (tidycg) into the FP64 solution; with add, x += dx unless the
@@ -582,25 +579,28 @@ static size_t apply_lds(const Model &M) {
     return stage_fits<V>(M) ? staged_lds<V>(M.sv.brows, M.sv.max_blk_nnz) : 2 * (size_t)M.sv.brows * sizeof(V);
 }
 
-// block factors of the current matrix: exact level-scheduled ILU(0) on the
-// first solve (or always with ilu_sweeps = 0), else ilu_sweeps fixed-point
-// sweeps from the previous factors
+// block factors of the current matrix (exact block ILU(0))
+static size_t factor_lds_staged(const Model &M, size_t vsize) {
+    return (size_t)M.sv.max_blk_nnz * vsize + (size_t)(2 * M.sv.max_blk_nnz + 1 + 2 * M.sv.max_blk_pairs) * sizeof(int);
+}
+
 template <typename V>
 static int factor(Model &M, SolvArgs<V> &a) {
-    const size_t lds_sweep = 3 * (size_t)M.sv.max_blk_nnz * sizeof(V);
-    if (M.p.ilu_sweeps > 0 && M.sv.lu_valid && lds_sweep <= kLdsLimit) {
-        hipLaunchKernelGGL(k_bilu_sweep<V>, dim3(M.sv.nblk), dim3(kThreads), lds_sweep, M.stream, a,
-                           M.p.ilu_sweeps);
+    const size_t staged = factor_lds_staged(M, sizeof(V));
+    timer_begin(M, T_FACTOR);
+    if (staged <= kLdsLimit) {
+        hipLaunchKernelGGL((k_bilu_factor<V, true>), dim3(M.sv.nblk), dim3(kThreads), staged, M.stream, a);
     } else {
         const size_t lds = (size_t)M.sv.max_blk_nnz * sizeof(V);
-        hipLaunchKernelGGL(k_bilu_factor<V>, dim3(M.sv.nblk), dim3(kThreads), lds, M.stream, a);
+        hipLaunchKernelGGL((k_bilu_factor<V, false>), dim3(M.sv.nblk), dim3(kThreads), lds, M.stream, a);
     }
+    timer_end(M);
     HIPCHK(M, hipGetLastError());
-    M.sv.lu_valid = true;
     return 0;
 }
 
-// Iterations are enqueued in chunks between host checks of the device flags;
+// One iteration is two kernels (k_bicg_a, k_bicg_b).  Iterations are
+// enqueued in chunks between host checks of the device flags;
 // the first chunk is the iteration count of the previous solve of the same
 // kind (`slot`), so a converging solve usually needs a single round trip.
 // Kernels enqueued after convergence return immediately.
@@ -610,29 +610,31 @@ static int bicgstab_t(Model &M, SolvArgs<V> &a, int *iters, int slot, int zero_x
     const size_t lds = apply_lds<V>(M);
     hipLaunchKernelGGL((k_bicg_init<V, STAGE>), dim3(g), dim3(kThreads), lds, M.stream, a, zero_x0);
     HIPCHK(M, hipGetLastError());
-    int it = 0, c[8] = {};
+    int it = 0, c[CT_WORDS] = {};
     const int maxit = M.p.itermax > 0 ? (M.p.itermax + 1) / 2 + 1 : 100000;
     int chunk = std::max(2, M.sv.last_its[slot] + 1);
     for (;;) {
         for (int q = 0; q < chunk; q++, it++) {
-            hipLaunchKernelGGL((k_bicg_pv<V, STAGE>), dim3(g), dim3(kThreads), lds, M.stream, a, it);
-            hipLaunchKernelGGL((k_bicg_st<V, STAGE>), dim3(g), dim3(kThreads), lds, M.stream, a, it);
-            hipLaunchKernelGGL(k_bicg_update<V>, dim3(g), dim3(kThreads), 0, M.stream, a, it);
+            timer_begin(M, T_BICG_A);
+            hipLaunchKernelGGL((k_bicg_a<V, STAGE>), dim3(g), dim3(kThreads), lds, M.stream, a, it);
+            timer_end(M);
+            timer_begin(M, T_BICG_B);
+            hipLaunchKernelGGL((k_bicg_b<V, STAGE>), dim3(g), dim3(kThreads), lds, M.stream, a, it);
+            timer_end(M);
         }
         HIPCHK(M, hipGetLastError());
-        if (read_ctrl(M, c, 8)) return 1;
-        if (c[CT_CONV] || c[CT_FAIL] || it >= maxit) break;
+        if (read_ctrl(M, c, CT_WORDS)) return 1;
+        if (c[CT_CONV] || c[CT_FAIL] || c[CT_MAXIT] || it >= maxit) break;
         chunk = 2;
     }
     if (iters) *iters = (c[CT_NMV] - 1) / 2;
     M.sv.last_its[slot] = (c[CT_NMV] - 1) / 2;
     if (c[CT_FACTOR] != 0) {
-        M.sv.lu_valid = false;
         M.err = "zeta solver: zero pivot in the block factorisation (row " + std::to_string(c[CT_FACTOR]) + ")";
         return 2;
     }
-    if (c[CT_FAIL] != 0) {
-        M.err = "zeta solver: BiCGStab failed (code " + std::to_string(c[CT_FAIL]) + ")";
+    if (c[CT_FAIL] != 0 || c[CT_MAXIT] != 0) {
+        M.err = "zeta solver: BiCGStab failed (code " + std::to_string(c[CT_FAIL] ? c[CT_FAIL] : -1) + ")";
         return 2;
     }
     if (!c[CT_CONV]) {
@@ -657,9 +659,12 @@ static int enqueue_inner_t(Model &M, SolvArgs<V> &a, int chunk) {
     const size_t lds = apply_lds<V>(M);
     hipLaunchKernelGGL((k_bicg_init<V, STAGE>), dim3(g), dim3(kThreads), lds, M.stream, a, 1);
     for (int it = 0; it < chunk; it++) {
-        hipLaunchKernelGGL((k_bicg_pv<V, STAGE>), dim3(g), dim3(kThreads), lds, M.stream, a, it);
-        hipLaunchKernelGGL((k_bicg_st<V, STAGE>), dim3(g), dim3(kThreads), lds, M.stream, a, it);
-        hipLaunchKernelGGL(k_bicg_update<V>, dim3(g), dim3(kThreads), 0, M.stream, a, it);
+        timer_begin(M, T_BICG_A);
+        hipLaunchKernelGGL((k_bicg_a<V, STAGE>), dim3(g), dim3(kThreads), lds, M.stream, a, it);
+        timer_end(M);
+        timer_begin(M, T_BICG_B);
+        hipLaunchKernelGGL((k_bicg_b<V, STAGE>), dim3(g), dim3(kThreads), lds, M.stream, a, it);
+        timer_end(M);
     }
     HIPCHK(M, hipGetLastError());
     return 0;
@@ -719,7 +724,7 @@ int solver_solve(Model &M, int *iters, int *refinements) {
             HIPCHK(M, hipGetLastError());
             return 0;
         };
-        int plan = std::max(1, M.sv.last_nref), enq = 0, c[8] = {};
+        int plan = std::max(1, M.sv.last_nref), enq = 0, c[CT_WORDS] = {};
         for (;;) {
             for (int r = 0; r < plan && enq < maxref; r++, enq++) {
                 if (resid() || enqueue_inner<float>(M, a, chunk)) return 1;
@@ -727,9 +732,8 @@ int solver_solve(Model &M, int *iters, int *refinements) {
                                    (const float *)a.dx, M.s.x, n, 1, (const int *)M.sv.ctrl);
             }
             if (resid()) return 1;
-            if (read_ctrl(M, c, 8)) return 1;
+            if (read_ctrl(M, c, CT_WORDS)) return 1;
             if (c[CT_FACTOR] != 0) {
-                M.sv.lu_valid = false;
                 M.err = "zeta solver: zero pivot in the block factorisation (row " + std::to_string(c[CT_FACTOR]) + ")";
                 return 2;
             }

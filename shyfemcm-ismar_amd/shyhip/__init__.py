@@ -40,7 +40,7 @@ class ShyParams(C.Structure):
                 ("rowass", C.c_float), ("vismol", C.c_float), ("ireib", C.c_int), ("uvmin", C.c_float),
                 ("hzmin", C.c_float), ("hzoff", C.c_float), ("hzon", C.c_float), ("volmin", C.c_float),
                 ("iclin", C.c_int), ("rtol", C.c_double), ("atol", C.c_double), ("itermax", C.c_int),
-                ("refine_rtol", C.c_double), ("block_rows", C.c_int), ("tri_sweeps", C.c_int), ("ilu_sweeps", C.c_int),
+                ("refine_rtol", C.c_double), ("block_rows", C.c_int), ("tri_sweeps", C.c_int),
                 ("inner_rtol", C.c_double), ("rkpar", C.c_float),
                 ("difmol", C.c_float), ("itvd", C.c_int), ("itvdv", C.c_int)]
 
@@ -48,7 +48,7 @@ class ShyParams(C.Structure):
 DEFAULTS = dict(dt=300.0, azpar=0.5, ampar=0.5, afpar=0.5, atpar=1.0, adpar=1.0, aapar=0.0, grav=9.81,
                 rowass=1025.0, vismol=1.0e-6, ireib=5, uvmin=0.2, hzmin=0.01, hzoff=0.05, hzon=0.10, volmin=1.0,
                 iclin=0, rtol=float(np.float32(1.0e-8)), atol=float(np.float32(1.0e-8)), itermax=1000,
-                refine_rtol=1.0e-10, block_rows=0, tri_sweeps=0, ilu_sweeps=0, inner_rtol=1.0e-6, rkpar=0.0, difmol=1.0e-6, itvd=0, itvdv=0)
+                refine_rtol=1.0e-10, block_rows=0, tri_sweeps=0, inner_rtol=1.0e-6, rkpar=0.0, difmol=1.0e-6, itvd=0, itvdv=0)
 
 _lib = None
 
@@ -80,7 +80,9 @@ class Model:
 
     def __init__(self, basin, precision: str = "fp64", params: dict | None = None, czv: float = 2.5e-3,
                  fcor: float = 0.0, visv: float = 0.0, difv: float = 0.0, zconst: float = 0.0,
-                 tracers: dict | None = None):
+                 tracers: dict | None = None, members=None):
+        """members = (node_start, elem_start): the basin is a union of
+        independent member basins (shyhip.ensemble)."""
         self.lib = lib = load_library()
         self.basin = b = basin
         self.prec = FP64 if precision == "fp64" else FP32
@@ -97,6 +99,10 @@ class Model:
             raise ShyhipError(f"shyhip_create: {lib.shyhip_last_error(None).decode()}")
         self.h = h
         self._chk(lib.shyhip_set_params(h, C.byref(ShyParams(**p))), "set_params")
+        if members is not None:
+            ns = np.ascontiguousarray(members[0], dtype=np.int32)
+            es = np.ascontiguousarray(members[1], dtype=np.int32)
+            self._chk(lib.shyhip_set_members(h, len(ns) - 1, _p(ns), _p(es)), "set_members")
         nen = np.ascontiguousarray(b.nen3v + 1, dtype=np.int32)
         f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)  # noqa: E731
         self._keep = [nen]
@@ -204,6 +210,11 @@ class Model:
     def set_boundary(self, level, ibc=1):
         self._chk(self.lib.shyhip_set_boundary_level(self.h, ibc, C.c_float(level)), "set_boundary_level")
 
+    def set_boundary_levels(self, levels):
+        """Levels of all open boundaries (boundary b = 1..nbnd gets levels[b-1])."""
+        lv = np.ascontiguousarray(levels, dtype=np.float32)
+        self._chk(self.lib.shyhip_set_boundary_levels(self.h, len(lv), _p(lv)), "set_boundary_levels")
+
     def set_wind(self, taux, tauy):
         self._chk(self.lib.shyhip_set_wind_uniform(self.h, C.c_float(taux), C.c_float(tauy)), "set_wind_uniform")
 
@@ -231,7 +242,7 @@ class Model:
         name = C.c_char_p()
         ms = C.c_double()
         cnt = C.c_long()
-        for i in range(15):
+        for i in range(18):
             self._chk(self.lib.shyhip_get_timing(self.h, i, C.byref(name), C.byref(ms), C.byref(cnt)), "get_timing")
             out[name.value.decode()] = (ms.value, cnt.value)
         return out

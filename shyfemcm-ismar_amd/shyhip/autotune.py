@@ -1,12 +1,10 @@
 """Autotuner for the zeta-solver layout of one basin.
 
-The device solver has three tuning knobs with no counterpart in the
+The device solver has two tuning knobs with no counterpart in the
 reference (its ILUT + BiCGStab is sequential):
 
   block_rows   rows per block-Jacobi ILU(0) block (one workgroup each)
   tri_sweeps   Jacobi sweeps per triangular factor (0 = exact substitution)
-  ilu_sweeps   fixed-point sweeps refreshing the factors each solve
-               (0 = exact level-scheduled factorisation every solve)
 
 All settings solve the same system to the same tolerance; they trade
 iterations against the cost of one iteration.  `tune` times a short run of
@@ -25,7 +23,7 @@ import numpy as np
 
 CACHE_DIR = os.environ.get("SHYHIP_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "shyhip"))
 
-DEFAULT_GRID = dict(block_rows=(128, 256, 512), tri_sweeps=(2, 3, 4), ilu_sweeps=(1, 2))
+DEFAULT_GRID = dict(block_rows=(128, 256, 512), tri_sweeps=(3, 4))
 
 
 def basin_signature(basin) -> str:
@@ -43,7 +41,7 @@ def _candidates(grid):
 
 def tune(basin, precision: str, params: dict, forcing, steps: int = 30, warmup: int = 5, grid=None,
          cache: bool = True, verbose: bool = False, **model_kw) -> dict:
-    """Pick block_rows / tri_sweeps / ilu_sweeps for `basin`.
+    """Pick block_rows / tri_sweeps for `basin`.
 
     forcing(model, n) applies the forcing of step n (boundary level, wind)
     and advances the model one step.  Returns the chosen settings and the

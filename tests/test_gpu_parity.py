@@ -118,13 +118,13 @@ def test_lagoon_structures_and_masks(gpu):
     assert np.array_equal(gm.pull("hdknv"), om.a["hdknv"])
 
 
-# device solver variants: exact block ILU(0) with exact triangular solves;
-# Jacobi-sweep triangular solves; fine-grained ILU(0) sweeps warm-started
-# from the previous step's factors
+# device solver variants: one block (exact ILU(0), exact triangular
+# solves); 256-row blocks with Jacobi-sweep triangular solves; 128-row
+# blocks with exact triangular solves
 SOLVER_VARIANTS = {
     "exact": dict(),
     "tri_sweeps": dict(block_rows=256, tri_sweeps=3),
-    "ilu_sweeps": dict(block_rows=256, tri_sweeps=3, ilu_sweeps=2),
+    "blocks128": dict(block_rows=128),
 }
 
 
