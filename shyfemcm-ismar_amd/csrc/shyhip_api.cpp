@@ -229,7 +229,7 @@ int shyhip_create(shyhip_model **out, int precision, int nkn, int nel, int nlvdi
     p.grav = 9.81f; p.rowass = 1025.f; p.vismol = 1.0e-6f; p.ireib = 0; p.uvmin = 0.2f;
     p.hzmin = 0.01f; p.hzoff = 0.05f; p.hzon = 0.10f; p.volmin = 1.f; p.iclin = 0;
     p.rtol = (double)1.0e-8f; p.atol = (double)1.0e-8f; p.itermax = 1000; p.refine_rtol = 1e-10;
-    p.block_rows = 0; p.tri_sweeps = 0; p.inner_rtol = 1e-6; p.rkpar = 0.f; p.difmol = 1.0e-6f; p.itvd = 0; p.itvdv = 0;
+    p.block_rows = 0; p.tri_sweeps = 0; p.refactor_every = 1; p.inner_rtol = 1e-6; p.rkpar = 0.f; p.difmol = 1.0e-6f; p.itvd = 0; p.itvdv = 0;
     const size_t ne = nel, nk = nkn, L = nlvdi;
     State &s = M.s;
     int rc = 0;
@@ -281,7 +281,7 @@ int shyhip_destroy(shyhip_model *h) {
     Mesh &m = M.m;
     for (void *q : {(void *)m.nen3v, (void *)m.geo, (void *)m.ang, (void *)m.dis, (void *)m.ilhv, (void *)m.ilhkv,
                     m.hm3v, m.hev, m.hlv, m.hldv, m.areakv, (void *)m.ibnd, (void *)m.zbound, (void *)m.iextb,
-                    (void *)m.ibtyp, m.fcorv, m.czv, (void *)m.nk_ptr, (void *)m.nk_el, (void *)m.nkc_el, (void *)m.lay_el,
+                    (void *)m.ibtyp, m.fcorv, m.czv, (void *)m.nk_ptr, (void *)m.nk_el, (void *)m.nkc_el,
                     (void *)m.rowptr, (void *)m.cols, (void *)m.ijp, (void *)m.bnd_nodes, (void *)m.bnd_id,
                     (void *)m.blk_rowptr, (void *)m.blk_cols, (void *)m.blk_src, (void *)m.blk_diag, (void *)m.blk_cp_ptr,
                     (void *)m.blk_cp_a, (void *)m.blk_cp_b, (void *)m.blk_jd,
@@ -317,6 +317,7 @@ int shyhip_set_params(shyhip_model *h, const shyhip_params *p) {
     if (p->itvd == 1) return fail(&M, "itvd = 1 (gradient TVD) is not supported by the device path; use 0 or 2");
     if (p->block_rows < 0 || p->block_rows > kMaxBlockRows) return fail(&M, "block_rows must be in 0..1024");
     if (p->tri_sweeps < 0 || p->tri_sweeps > 64) return fail(&M, "tri_sweeps must be in 0..64");
+    if (p->refactor_every < 1) return fail(&M, "refactor_every must be >= 1");
     if (!(p->inner_rtol > 0. && p->inner_rtol < 1.)) return fail(&M, "inner_rtol must be in (0,1)");
     M.p = *p;
     return 0;
@@ -393,15 +394,7 @@ int shyhip_set_basin(shyhip_model *h, const int *nen3v, const float *xgv, const 
     rc |= dalloc(M, &m.dis, dis.size()) || upload(M, m.dis, dis.data(), dis.size());
     rc |= dalloc(M, &m.ilhv, nel) || upload(M, m.ilhv, ilhv, nel);
     rc |= dalloc(M, &m.ilhkv, nkn) || upload(M, m.ilhkv, ilhkv, nkn);
-    {
-        if (nel >= (1 << kLayShift) || M.nlv > 32) return fail(&M, "basin too large for the element-layer list");
-        std::vector<int> lay;
-        for (int l = 1; l <= M.nlv; l++)
-            for (int ie = 0; ie < nel; ie++)
-                if (ilhv[ie] >= l) lay.push_back(ie | ((l - 1) << kLayShift));
-        m.nel_layers = (int)lay.size();
-        rc |= dalloc(M, &m.lay_el, lay.size()) || upload(M, m.lay_el, lay.data(), lay.size());
-    }
+    if (M.nlv > 32) return fail(&M, "at most 32 layers");
     // depths (vertex-major on device)
     std::vector<double> t(3 * (size_t)nel);
     for (int ie = 0; ie < nel; ie++)
@@ -590,6 +583,7 @@ int shyhip_set_coefficients(shyhip_model *h, const float *fcorv, const float *cz
     Model &M = h->M;
     if (!M.m.fcorv) return fail(&M, "call shyhip_set_basin first");
     std::vector<double> t(M.nel, 0.);
+    const bool had = M.has_coriolis;
     M.has_coriolis = false;
     if (fcorv) {
         for (int i = 0; i < M.nel; i++) {
@@ -597,6 +591,11 @@ int shyhip_set_coefficients(shyhip_model *h, const float *fcorv, const float *cz
             if (fcorv[i] != 0.f) M.has_coriolis = true;
         }
     }
+    // without Coriolis the cross terms of ddxv/ddyv are not written: clear
+    // what a coupled step may have left there
+    if (had && !M.has_coriolis)
+        for (void *q : {M.s.ddx, M.s.ddy})
+            if (launch_fill(M, q, 0., 2 * (size_t)M.nlvdi * M.nel)) return 1;
     if (upload_s(M, M.m.fcorv, t.data(), M.nel)) return 1;
     std::fill(t.begin(), t.end(), 0.);
     if (czv)
@@ -829,6 +828,7 @@ static int hydro_impl(Model &M, double dtime, bool final_check) {
         timer_begin(M, T_WETDRY);
         if (launch_system_get_setweg1(M, bnew, &changed)) return 1;
         timer_end(M);
+        if (changed) M.sv.force_refactor = true;    // rows of the zeta matrix changed
         M.dtime_old = M.dtime;
         if (changed == 0) break;
     }

@@ -1,11 +1,11 @@
 // shyhip_conz.hip -- 3D scalar advection-diffusion (conz3d) on gfx950.
 //
 // Two kernels per substep:
-//   k_conz_elem  one work-item per element-layer (the active element-layers
-//                are listed layer by layer): horizontal advective fluxes
-//                (upwind rule or the itvd=2 TVD scheme with precomputed
-//                upwind interpolation weights) -> 3*fl per vertex, written
-//                layer-major to a scratch array.
+//   k_conz_elem  one workgroup per tile of elements, one work-item per
+//                element-layer: horizontal advective fluxes (upwind rule or
+//                the itvd=2 TVD scheme with precomputed upwind interpolation
+//                weights) -> 3*fl per vertex, written layer-major to a
+//                scratch array.
 //   k_conz_node  one workgroup per tile of consecutive nodes.  Phase 1, one
 //                work-item per node-layer: the vertical diffusion/advection
 //                terms of the node column (they depend on an element only
@@ -29,12 +29,12 @@ namespace shy {
 
 template <typename S>
 struct ConzArgs {
-    int nkn, nel, nlvdi, nlv, itvd, itvdv, clip, nel_layers;
+    int nkn, nel, nlvdi, nlv, itvd, itvdv, clip;
     double dt, az, ad, aa, rso, rsn;
     float difmol;
     const int *nen3v, *ilhv, *ilhkv, *iextb, *ibnd;
     const double *geo, *dis;
-    const int *nkc_ptr, *nkc_el, *lay_el;
+    const int *nkc_ptr, *nkc_el;
     const int *tvd_el, *tvd_kn;
     const double *tvd_w;
     const S *utn, *vtn, *uto, *vto, *uln, *vln;
@@ -60,94 +60,135 @@ __device__ inline S cval_s(const ConzArgs<S> &a, int l, int k) {
 }
 
 // ------------------------------------------------------------------ element phase
+// One workgroup per tile of kElemTile consecutive elements.  The tile's
+// geometry, levels and TVD upwind tables are staged in LDS once; then the
+// work-items sweep the tile's element-layers (kElemTile elements x 4 layers
+// at a time), each reading only its layer's transports and concentrations.
+constexpr int kElemTile = 64;
+
+struct ElemTileLds {
+    double b[3][kElemTile], c[3][kElemTile], dis[3][kElemTile];
+    double tw[3][6][kElemTile];
+    int kn[3][kElemTile], tkn[3][6][kElemTile], tel[6][kElemTile], tlev[6][kElemTile];
+    int ilev[kElemTile], iext[kElemTile];
+    double ajr[kElemTile];
+};
+
 template <typename S>
 __global__ __launch_bounds__(kThreads) void k_conz_elem(ConzArgs<S> a) {
     using R = S;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.nel_layers) return;
-    const int code = a.lay_el[t];
-    const int ie = code & ((1 << kLayShift) - 1), l = (code >> kLayShift) + 1;
-    const int nel = a.nel;
-    int kn[3];
-    double b[3], c[3];
-    int iext = 0;
-    for (int ii = 0; ii < 3; ii++) {
-        kn[ii] = a.nen3v[3 * ie + ii];
-        b[ii] = a.geo[(size_t)ii * nel + ie];
-        c[ii] = a.geo[(size_t)(3 + ii) * nel + ie];
-        if (a.iextb[kn[ii]]) iext++;
-    }
-    const double azt = 1. - a.az;
-    const bool tvd = a.itvd > 0 && iext == 0;
-    const R ajr = (R)(24. * a.geo[(size_t)6 * nel + ie]);
-    const size_t o = (size_t)(l - 1) * nel + ie;
-    const double us = a.az * (double)a.utn[o] + azt * (double)a.uto[o];
-    const double vs = a.az * (double)a.vtn[o] + azt * (double)a.vto[o];
-    double f[3], cl[3], fl[3];
-    int itot = 0, isum = 0;
-    for (int ii = 0; ii < 3; ii++) {
-        f[ii] = us * b[ii] + vs * c[ii];
-        if (f[ii] < 0.) { itot++; isum += ii + 1; }
-        cl[ii] = cval(a, l, kn[ii]);
-    }
-    if (itot == 1) {
-        for (int ii = 0; ii < 3; ii++) fl[ii] = f[ii] * cl[isum - 1];
-    } else if (itot == 2) {
-        const int is2 = 6 - isum;
-        for (int ii = 0; ii < 3; ii++) fl[ii] = f[ii] * cl[ii];
-        fl[is2 - 1] = 0.;
-        fl[is2 - 1] = -(fl[0] + fl[1] + fl[2]);
-    } else {
-        fl[0] = fl[1] = fl[2] = 0.;
-    }
-    if (tvd) {
-        // tvd_fluxes, tvd_admin.f90:673-823
-        fl[0] = fl[1] = fl[2] = 0.;
-        if (itot >= 1 && itot <= 2) {
-            const int itot2 = itot - 1, itot1 = 2 - itot;
-            const R u = a.uln[o], v = a.vln[o];
-            const int ip = itot2 * (6 - isum) + itot1 * isum;
-            for (int ii = 1; ii <= 3; ii++) {
-                if (ii == ip) continue;
-                const int ic = itot2 * ii + itot1 * ip;
-                const int id = itot2 * ip + itot1 * ii;
-                const R fact = (R)(-itot2 + itot1);
-                const R conc = (R)cl[ic - 1], cond = (R)cl[id - 1];
-                const int iop = 6 - (id + ic);
-                const int tet1 = 1 + iop % 3;
-                R dx = (R)((double)ajr * c[iop - 1]);
-                dx = (R)(-2 * (tet1 == id ? 1 : 0)) * dx + dx;
-                R dy = (R)((double)ajr * b[iop - 1]);
-                dy = (R)(-2 * (tet1 == ic ? 1 : 0)) * dy + dy;
-                const R dis = (R)a.dis[(size_t)(iop - 1) * nel + ie];
-                const R vel = (R)fabs((double)(u * dx + v * dy)) / dis;
-                const R alfa = (R)((a.dt * (double)vel) / (double)dis);
-                // upwind value: pair index of (ic,id)
-                const int pidx = 2 * (ic - 1) + ((id == (ic % 3) + 1) ? 0 : 1);
-                R conu = cond;
-                const int up = a.tvd_el[(size_t)pidx * nel + ie];
-                if (up >= 0 && a.ilhv[up] >= l) {
-                    double zh = 0.;
-                    for (int q = 0; q < 3; q++) {
-                        const int kq = a.tvd_kn[((size_t)q * 6 + pidx) * nel + ie];
-                        zh = zh + (double)cval_s(a, l, kq) * a.tvd_w[((size_t)q * 6 + pidx) * nel + ie];
-                    }
-                    conu = (R)zh;
+    __shared__ ElemTileLds T;
+    const int nel = a.nel, e0 = blockIdx.x * kElemTile;
+    const int ne = min(kElemTile, nel - e0);
+    const bool use_tvd = a.itvd > 0;
+    __shared__ int lmax_tile;
+    if (threadIdx.x == 0) lmax_tile = 0;
+    __syncthreads();
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+        const int ie = e0 + e;
+        int iext = 0;
+        for (int ii = 0; ii < 3; ii++) {
+            const int k = a.nen3v[3 * ie + ii];
+            T.kn[ii][e] = k;
+            T.b[ii][e] = a.geo[(size_t)ii * nel + ie];
+            T.c[ii][e] = a.geo[(size_t)(3 + ii) * nel + ie];
+            T.dis[ii][e] = a.dis[(size_t)ii * nel + ie];
+            if (a.iextb[k]) iext++;
+        }
+        T.iext[e] = iext;
+        T.ajr[e] = a.geo[(size_t)6 * nel + ie];
+        const int lv = a.ilhv[ie];
+        T.ilev[e] = lv;
+        atomicMax(&lmax_tile, lv);
+        if (use_tvd && iext == 0) {
+            for (int p = 0; p < 6; p++) {
+                const int up = a.tvd_el[(size_t)p * nel + ie];
+                T.tel[p][e] = up;
+                T.tlev[p][e] = up >= 0 ? a.ilhv[up] : 0;
+                for (int q = 0; q < 3; q++) {
+                    T.tkn[q][p][e] = a.tvd_kn[((size_t)q * 6 + p) * nel + ie];
+                    T.tw[q][p][e] = a.tvd_w[((size_t)q * 6 + p) * nel + ie];
                 }
-                const R grad = cond - conu;
-                R rf;
-                if ((R)fabs((double)(conc - cond)) < (R)1.e-8) rf = -1;
-                else rf = grad / (cond - conc) - (R)1;
-                const double rfd = (double)rf;
-                const R psi = (R)fmax(0., fmax(fmin(1., 2. * rfd), fmin(2., rfd)));
-                const R conf = conc + (R)0.5 * psi * (cond - conc) * ((R)1 - alfa);
-                const R term = (R)((double)(fact * conf) * f[ii - 1]);
-                fl[ic - 1] = fl[ic - 1] - (double)term;
-                fl[id - 1] = fl[id - 1] + (double)term;
             }
         }
     }
-    for (int ii = 0; ii < 3; ii++) a.hbuf[((size_t)(l - 1) * 3 + ii) * nel + ie] = (S)(3. * fl[ii]);
+    __syncthreads();
+    const int lmt = lmax_tile;
+    const double azt = 1. - a.az;
+    for (int idx = threadIdx.x; idx < kElemTile * lmt; idx += blockDim.x) {
+        const int e = idx % kElemTile, l = idx / kElemTile + 1;
+        if (e >= ne || l > T.ilev[e]) continue;
+        const int ie = e0 + e;
+        int kn[3];
+        double b[3], c[3];
+        for (int ii = 0; ii < 3; ii++) { kn[ii] = T.kn[ii][e]; b[ii] = T.b[ii][e]; c[ii] = T.c[ii][e]; }
+        const bool tvd = use_tvd && T.iext[e] == 0;
+        const R ajr = (R)(24. * T.ajr[e]);
+        const size_t o = (size_t)(l - 1) * nel + ie;
+        const double us = a.az * (double)a.utn[o] + azt * (double)a.uto[o];
+        const double vs = a.az * (double)a.vtn[o] + azt * (double)a.vto[o];
+        double f[3], cl[3], fl[3];
+        int itot = 0, isum = 0;
+        for (int ii = 0; ii < 3; ii++) {
+            f[ii] = us * b[ii] + vs * c[ii];
+            if (f[ii] < 0.) { itot++; isum += ii + 1; }
+            cl[ii] = cval(a, l, kn[ii]);
+        }
+        if (itot == 1) {
+            for (int ii = 0; ii < 3; ii++) fl[ii] = f[ii] * cl[isum - 1];
+        } else if (itot == 2) {
+            const int is2 = 6 - isum;
+            for (int ii = 0; ii < 3; ii++) fl[ii] = f[ii] * cl[ii];
+            fl[is2 - 1] = 0.;
+            fl[is2 - 1] = -(fl[0] + fl[1] + fl[2]);
+        } else {
+            fl[0] = fl[1] = fl[2] = 0.;
+        }
+        if (tvd) {
+            // tvd_fluxes, tvd_admin.f90:673-823
+            fl[0] = fl[1] = fl[2] = 0.;
+            if (itot >= 1 && itot <= 2) {
+                const int itot2 = itot - 1, itot1 = 2 - itot;
+                const R u = a.uln[o], v = a.vln[o];
+                const int ip = itot2 * (6 - isum) + itot1 * isum;
+                for (int ii = 1; ii <= 3; ii++) {
+                    if (ii == ip) continue;
+                    const int ic = itot2 * ii + itot1 * ip;
+                    const int id = itot2 * ip + itot1 * ii;
+                    const R fact = (R)(-itot2 + itot1);
+                    const R conc = (R)cl[ic - 1], cond = (R)cl[id - 1];
+                    const int iop = 6 - (id + ic);
+                    const int tet1 = 1 + iop % 3;
+                    R dx = (R)((double)ajr * c[iop - 1]);
+                    dx = (R)(-2 * (tet1 == id ? 1 : 0)) * dx + dx;
+                    R dy = (R)((double)ajr * b[iop - 1]);
+                    dy = (R)(-2 * (tet1 == ic ? 1 : 0)) * dy + dy;
+                    const R dis = (R)T.dis[iop - 1][e];
+                    const R vel = (R)fabs((double)(u * dx + v * dy)) / dis;
+                    const R alfa = (R)((a.dt * (double)vel) / (double)dis);
+                    // upwind value: pair index of (ic,id)
+                    const int pidx = 2 * (ic - 1) + ((id == (ic % 3) + 1) ? 0 : 1);
+                    R conu = cond;
+                    if (T.tel[pidx][e] >= 0 && T.tlev[pidx][e] >= l) {
+                        double zh = 0.;
+                        for (int q = 0; q < 3; q++) zh = zh + (double)cval_s(a, l, T.tkn[q][pidx][e]) * T.tw[q][pidx][e];
+                        conu = (R)zh;
+                    }
+                    const R grad = cond - conu;
+                    R rf;
+                    if ((R)fabs((double)(conc - cond)) < (R)1.e-8) rf = -1;
+                    else rf = grad / (cond - conc) - (R)1;
+                    const double rfd = (double)rf;
+                    const R psi = (R)fmax(0., fmax(fmin(1., 2. * rfd), fmin(2., rfd)));
+                    const R conf = conc + (R)0.5 * psi * (cond - conc) * ((R)1 - alfa);
+                    const R term = (R)((double)(fact * conf) * f[ii - 1]);
+                    fl[ic - 1] = fl[ic - 1] - (double)term;
+                    fl[id - 1] = fl[id - 1] + (double)term;
+                }
+            }
+        }
+        for (int ii = 0; ii < 3; ii++) a.hbuf[((size_t)(l - 1) * 3 + ii) * nel + ie] = (S)(3. * fl[ii]);
+    }
 }
 
 // ------------------------------------------------------------------ node phase
@@ -325,7 +366,7 @@ static int conz_nl(Model &M, int itr, int istot, int isact, int clip) {
     a.az = M.p.azpar; a.ad = M.p.adpar; a.aa = M.p.aapar; a.difmol = M.p.difmol;
     a.nen3v = M.m.nen3v; a.ilhv = M.m.ilhv; a.ilhkv = M.m.ilhkv; a.iextb = M.m.iextb; a.ibnd = M.m.ibnd;
     a.geo = M.m.geo; a.dis = M.m.dis;
-    a.nkc_ptr = M.m.nk_ptr; a.nkc_el = M.m.nkc_el; a.lay_el = M.m.lay_el; a.nel_layers = M.m.nel_layers;
+    a.nkc_ptr = M.m.nk_ptr; a.nkc_el = M.m.nkc_el;
     a.tvd_el = M.m.tvd_el; a.tvd_kn = M.m.tvd_kn; a.tvd_w = (const double *)M.m.tvd_w;
     a.utn = (const S *)M.s.utn; a.vtn = (const S *)M.s.vtn; a.uto = (const S *)M.s.uto; a.vto = (const S *)M.s.vto;
     a.uln = (const S *)M.s.uln; a.vln = (const S *)M.s.vln;
@@ -336,7 +377,7 @@ static int conz_nl(Model &M, int itr, int istot, int isact, int clip) {
     a.cbound = nullptr;
     a.hbuf = (S *)M.s.hbuf;
     a.err = M.d_err;
-    const int ge = (M.m.nel_layers + kThreads - 1) / kThreads;
+    const int ge = (M.nel + kElemTile - 1) / kElemTile;
     timer_begin(M, T_CONZ_ELEM);
     hipLaunchKernelGGL(k_conz_elem<S>, dim3(ge), dim3(kThreads), 0, M.stream, a);
     timer_end(M);

@@ -340,10 +340,10 @@ __global__ __launch_bounds__(kThreads) void k_hydro_intern(HydroArgs<S> a) {
         uold += (double)uo; vold += (double)vo;
         uhat += (double)unew; vhat += (double)vnew;
         if constexpr (!CORI) {
+            // decoupled columns: the cross terms ddx(jv), ddy(ju) are zero and
+            // stay zero in the arrays (never written on this path)
             const int ju = 2 * l - 2, jv = 2 * l - 1;
             a.ddx[(size_t)ju * nel + ie] = (S)sh[l - 1];
-            a.ddx[(size_t)jv * nel + ie] = (S)0;
-            a.ddy[(size_t)ju * nel + ie] = (S)0;
             a.ddy[(size_t)jv * nel + ie] = (S)sh[l - 1];
             dsum += sh[l - 1];
         }
@@ -559,8 +559,9 @@ __global__ __launch_bounds__(kThreads) void k_system_get(HydroArgs<S> a, int *cn
 }
 
 // ------------------------------------------------------------------ final transports
-// hydro_transports_final, hydrodynamic.f90:1468-1578
-template <typename S>
+// hydro_transports_final, hydrodynamic.f90:1468-1578; without Coriolis the
+// cross terms ddx(jv), ddy(ju) are zero and are not read
+template <typename S, bool CORI>
 __global__ __launch_bounds__(kThreads) void k_transports_final(HydroArgs<S> a) {
     using R = S;
     const int ie = blockIdx.x * blockDim.x + threadIdx.x;
@@ -579,8 +580,14 @@ __global__ __launch_bounds__(kThreads) void k_transports_final(HydroArgs<S> a) {
     double um = 0., vm = 0.;
     for (int l = 1; l <= ilevel; l++) {
         const int ju = 2 * l - 2, jv = 2 * l - 1;
-        const double du = beta * ((double)a.ddx[(size_t)ju * nel + ie] * bz + (double)a.ddy[(size_t)ju * nel + ie] * cz);
-        const double dv = beta * ((double)a.ddx[(size_t)jv * nel + ie] * bz + (double)a.ddy[(size_t)jv * nel + ie] * cz);
+        double du, dv;
+        if constexpr (CORI) {
+            du = beta * ((double)a.ddx[(size_t)ju * nel + ie] * bz + (double)a.ddy[(size_t)ju * nel + ie] * cz);
+            dv = beta * ((double)a.ddx[(size_t)jv * nel + ie] * bz + (double)a.ddy[(size_t)jv * nel + ie] * cz);
+        } else {
+            du = beta * ((double)a.ddx[(size_t)ju * nel + ie] * bz);
+            dv = beta * ((double)a.ddy[(size_t)jv * nel + ie] * cz);
+        }
         const size_t o = (size_t)(l - 1) * nel + ie;
         const S u = (S)((double)a.utn[o] - du * 1.0);
         const S v = (S)((double)a.vtn[o] - dv * 1.0);
@@ -710,20 +717,19 @@ __global__ __launch_bounds__(kThreads) void k_setdepth_elem(HydroArgs<S> a, cons
     zmed = zmed / (R)3;
     const R htot = a.hev[ie];
     bool bad = false;
-    for (int l = 1; l <= a.nlvdi; l++) {
+    // layers below the element's bottom keep their zero thickness
+    for (int l = 1; l <= lmax; l++) {
         R h = 0;
-        if (l <= lmax) {
-            if (lmax == 1) {
-                h = htot + zmed;
-            } else if (l < lmax) {
-                h = a.hldv[l - 1];
-                if (l == 1) h = h + zmed;
-            } else {
-                h = htot - a.hlv[lmax - 2];
-                if (h < (R)0) bad = true;
-            }
-            if (h <= (R)0) bad = true;
+        if (lmax == 1) {
+            h = htot + zmed;
+        } else if (l < lmax) {
+            h = a.hldv[l - 1];
+            if (l == 1) h = h + zmed;
+        } else {
+            h = htot - a.hlv[lmax - 2];
+            if (h < (R)0) bad = true;
         }
+        if (h <= (R)0) bad = true;
         hden[(size_t)(l - 1) * nel + ie] = h;
     }
     if (bad) atomicCAS(a.err, 0, ERR_SETDEPTH);
@@ -772,7 +778,7 @@ __global__ __launch_bounds__(kThreads) void k_setdepth_node(HydroArgs<S> a, cons
             else stop = true;
         }
         if (l <= lmaxk && acc[l - 1] <= (R)0) bad = true;
-        if (l <= a.nlvdi) hdkn[(size_t)(l - 1) * nkn + k] = (l <= lmaxk) ? acc[l - 1] : (R)0;
+        if (l <= lmaxk) hdkn[(size_t)(l - 1) * nkn + k] = acc[l - 1];
     }
     if (bad) atomicCAS(a.err, 0, ERR_SETDEPTH);
 }
@@ -848,13 +854,16 @@ __global__ __launch_bounds__(kThreads) void k_ttov_uvint(HydroArgs<S> a) {
     if (ie >= a.nel) return;
     const int nel = a.nel;
     double u = 0., v = 0.;
-    for (int l = 1; l <= a.nlvdi; l++) {
+    // layers below the element's bottom keep their zero velocities
+    const int lmax = a.ilhv[ie];
+    for (int l = 1; l <= lmax; l++) {
         const size_t o = (size_t)(l - 1) * nel + ie;
         const S h = a.hden[o];
         const S ut = a.utn[o], vt = a.vtn[o];
         a.uln[o] = (h > (S)0) ? ut / h : (S)0;
         a.vln[o] = (h > (S)0) ? vt / h : (S)0;
-        if (l <= a.ilhv[ie]) { u = u + (double)ut; v = v + (double)vt; }
+        u = u + (double)ut;
+        v = v + (double)vt;
     }
     a.un[ie] = (S)u;
     a.vn[ie] = (S)v;
@@ -1028,7 +1037,8 @@ int launch_system_get_setweg1(Model &M, int bnewtime, int *changed) {
 template <typename S>
 static int final_t(Model &M) {
     auto a = make_args<S>(M);
-    LAUNCH(M, k_transports_final<S>, nblocks(M.nel), a);
+    if (M.has_coriolis) LAUNCH(M, (k_transports_final<S, true>), nblocks(M.nel), a);
+    else LAUNCH(M, (k_transports_final<S, false>), nblocks(M.nel), a);
     return 0;
 }
 int launch_transports_final(Model &M) { return M.prec == SHYHIP_FP64 ? final_t<double>(M) : final_t<float>(M); }

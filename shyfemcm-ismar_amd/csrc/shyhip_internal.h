@@ -21,7 +21,6 @@ constexpr float kFlag = -9988765.0f;                 // mkonst flag (const.f90:1
 constexpr double kDrittl = (double)(1.0f / 3.0f);    // default-real 1./3. in double
 constexpr int kThreads = 256;
 constexpr int kMaxBlockRows = 1024;                  // block-ILU(0) rows per workgroup (LDS)
-constexpr int kLayShift = 26;                        // element ids below 2^26, layers below 32
 
 // error codes written by kernels into Model::d_err[0]
 enum DevErr { ERR_NONE = 0, ERR_SETNOD = 1, ERR_SETUVD = 3, ERR_SETDEPTH = 4, ERR_VOLUME = 5,
@@ -40,8 +39,6 @@ struct Mesh {                 // device copies of init-time data
     int *nk_ptr = nullptr;    // node -> incident (element<<2 | vertex), element order
     int *nk_el = nullptr;
     int *nkc_el = nullptr;    // same lists in colour order
-    int *lay_el = nullptr;    // active element-layers, layer by layer: ie | (l-1) << kLayShift
-    int nel_layers = 0;
     int *rowptr = nullptr, *cols = nullptr, *ijp = nullptr;   // zeta CSR; ijp[ie*9+i*3+j]
     int *bnd_nodes = nullptr; int *bnd_id = nullptr; int nbnd_nodes = 0;
     // block-ILU(0)
@@ -83,6 +80,10 @@ struct Solver {               // Krylov work space (value type V = S of the mode
     int nblk = 0, brows = 0, max_blk_nnz = 0, max_blk_pairs = 0, nparts = 0;
     int last_its[8] = {};     // iteration count of the previous solve, per solve kind
     int last_nref = 0;        // FP64 refinement rounds of the previous solve
+    bool factored = false;    // lu holds block factors of an earlier matrix
+    bool force_refactor = false;
+    int since_factor = 0;     // solves since the last factorisation
+    int its_after_factor = 0; // iterations of the first solve with the current factors
     void *lu = nullptr;       // block factors (values of the block pattern)
     void *r0 = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr, *v0 = nullptr, *v1 = nullptr,
          *s = nullptr, *t = nullptr, *dx = nullptr, *b = nullptr, *x0 = nullptr;

@@ -300,6 +300,39 @@ __global__ __launch_bounds__(kThreads) void k_bilu_factor(SolvArgs<V> a) {
     for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) a.lu[q] = val[q - q0];
 }
 
+// The operands of one matrix row -- values and NV gathered vectors at the
+// row's columns, and the vectors at the row itself -- held in registers so
+// their loads are issued before a kernel's scalar reduction.  Rows longer
+// than kRowRegs read the rest from memory.
+constexpr int kRowRegs = 10;
+
+template <typename V, int NV>
+struct RowOps {
+    V a[kRowRegs];
+    V x[NV][kRowRegs];
+    V own[NV];
+    int q0 = 0, q1 = 0, m = 0;
+    template <typename... P>
+    __device__ inline void load(const SolvArgs<V> &s, int i, bool valid, P... vec) {
+        const V *vs[NV] = {vec...};
+        if (!valid) return;
+        q0 = s.rowptr[i];
+        q1 = s.rowptr[i + 1];
+        m = min(q1 - q0, kRowRegs);
+#pragma unroll
+        for (int k = 0; k < kRowRegs; k++) {
+            if (k < m) {
+                const int j = s.cols[q0 + k];
+                a[k] = s.A[q0 + k];
+#pragma unroll
+                for (int v = 0; v < NV; v++) x[v][k] = vs[v][j];
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; v++) own[v] = vs[v][i];
+    }
+};
+
 // r0 = M^-1 (b - A x0) (x0 = 0 when zero_x0), dx = 0; (r0,r0) into both
 // slots of region C
 template <typename V, bool STAGE>
@@ -353,6 +386,16 @@ __global__ __launch_bounds__(kThreads) void k_bicg_a(SolvArgs<V> a, int it) {
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     const int blk = blockIdx.x, b0 = blk * a.brows;
     const int nb = min(a.n, b0 + a.brows) - b0;
+    const V *pold_ = pp ? a.p1 : a.p0;     // direction of iteration it-1 (it = 0: unused)
+    const V *vold_ = pp ? a.v1 : a.v0;
+    const BlockView<V> B = load_block<V, STAGE>(a, b0, nb, reinterpret_cast<unsigned char *>(w + a.brows));
+    // operands of this work-item's row, loaded before the scalar reductions
+    RowOps<V, 4> ro;
+    const bool pre = a.brows <= kThreads;
+    if (pre) {
+        if (it == 0) ro.load(a, b0 + threadIdx.x, threadIdx.x < nb, a.r0, a.r0, a.r0, a.r0);
+        else ro.load(a, b0 + threadIdx.x, threadIdx.x < nb, a.s, a.t, pold_, vold_);
+    }
     double rn, rho, omega = 0., alpha = 0.;
     if (it == 0) {
         double rs[2];
@@ -408,13 +451,42 @@ __global__ __launch_bounds__(kThreads) void k_bicg_a(SolvArgs<V> a, int it) {
         beta = rho * alpha / (omega * a.scal[SC_RHO + pp]);
     }
     if (lead) a.scal[SC_RHO + par] = rho;
-    const V *pold = pp ? a.p1 : a.p0;     // direction of iteration it-1 (it = 0: r0)
+    const V *pold = pold_;
     V *pnew = par ? a.p1 : a.p0;
-    const V *vold = pp ? a.v1 : a.v0;
+    const V *vold = vold_;
     V *vnew = par ? a.v1 : a.v0;
     const V vb = (V)beta, vo = (V)omega;
-    const BlockView<V> B = load_block<V, STAGE>(a, b0, nb, reinterpret_cast<unsigned char *>(w + a.brows));
-    if (it == 0) {
+    if (pre) {
+        if (threadIdx.x < nb) {
+            const int i = b0 + threadIdx.x;
+            V t = 0;
+            if (it == 0) {
+#pragma unroll
+                for (int k = 0; k < kRowRegs; k++)
+                    if (k < ro.m) t = t + ro.a[k] * ro.x[0][k];
+                for (int q = ro.q0 + ro.m; q < ro.q1; q++) t = t + a.A[q] * a.r0[a.cols[q]];
+                pnew[i] = ro.own[0];
+            } else {
+#pragma unroll
+                for (int k = 0; k < kRowRegs; k++) {
+                    if (k < ro.m) {
+                        const V rj = ro.x[0][k] - vo * ro.x[1][k];
+                        const V pj = rj + vb * (ro.x[2][k] - vo * ro.x[3][k]);
+                        t = t + ro.a[k] * pj;
+                    }
+                }
+                for (int q = ro.q0 + ro.m; q < ro.q1; q++) {
+                    const int j = a.cols[q];
+                    const V rj = a.s[j] - vo * a.t[j];
+                    const V pj = rj + vb * (pold[j] - vo * vold[j]);
+                    t = t + a.A[q] * pj;
+                }
+                const V ri = ro.own[0] - vo * ro.own[1];
+                pnew[i] = ri + vb * (ro.own[2] - vo * ro.own[3]);
+            }
+            y[threadIdx.x] = t;
+        }
+    } else if (it == 0) {
         // p = r0 (set by init in p0 = p of parity 0's predecessor slot)
         for (int li = threadIdx.x; li < nb; li += blockDim.x) {
             const int i = b0 + li;
@@ -458,6 +530,15 @@ __global__ __launch_bounds__(kThreads) void k_bicg_b(SolvArgs<V> a, int it) {
     V *y = reinterpret_cast<V *>(smem), *w = y + a.brows;
     const int par = it & 1;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const V *vnew = par ? a.v1 : a.v0;
+    const V *rv = it == 0 ? a.r0 : a.r;
+    const int blk = blockIdx.x, b0 = blk * a.brows;
+    const int nb = min(a.n, b0 + a.brows) - b0;
+    const BlockView<V> B = load_block<V, STAGE>(a, b0, nb, reinterpret_cast<unsigned char *>(w + a.brows));
+    // operands of this work-item's row, loaded before the scalar reduction
+    RowOps<V, 2> ro;
+    const bool pre = a.brows <= kThreads;
+    if (pre) ro.load(a, b0 + threadIdx.x, threadIdx.x < nb, rv, vnew);
     double d[1];
     sum_partials<1>(a.part, PR_A, a.nparts, a.nblk, d);
     if (breakdown(d[0])) {
@@ -466,22 +547,33 @@ __global__ __launch_bounds__(kThreads) void k_bicg_b(SolvArgs<V> a, int it) {
     }
     const double alpha = a.scal[SC_RHO + par] / d[0];
     if (lead) a.scal[SC_ALPHA + par] = alpha;
-    const V *vnew = par ? a.v1 : a.v0;
-    const V *rv = it == 0 ? a.r0 : a.r;
     const V va = (V)alpha;
-    const int blk = blockIdx.x, b0 = blk * a.brows;
-    const int nb = min(a.n, b0 + a.brows) - b0;
-    const BlockView<V> B = load_block<V, STAGE>(a, b0, nb, reinterpret_cast<unsigned char *>(w + a.brows));
-    for (int li = threadIdx.x; li < nb; li += blockDim.x) {
-        const int i = b0 + li;
-        V t = 0;
-        for (int q = a.rowptr[i]; q < a.rowptr[i + 1]; q++) {
-            const int j = a.cols[q];
-            const V sj = rv[j] - va * vnew[j];
-            t = t + a.A[q] * sj;
+    if (pre) {
+        if (threadIdx.x < nb) {
+            const int i = b0 + threadIdx.x;
+            V t = 0;
+#pragma unroll
+            for (int k = 0; k < kRowRegs; k++)
+                if (k < ro.m) t = t + ro.a[k] * (ro.x[0][k] - va * ro.x[1][k]);
+            for (int q = ro.q0 + ro.m; q < ro.q1; q++) {
+                const int j = a.cols[q];
+                t = t + a.A[q] * (rv[j] - va * vnew[j]);
+            }
+            y[threadIdx.x] = t;
+            a.s[i] = ro.own[0] - va * ro.own[1];
         }
-        y[li] = t;
-        a.s[i] = rv[i] - va * vnew[i];
+    } else {
+        for (int li = threadIdx.x; li < nb; li += blockDim.x) {
+            const int i = b0 + li;
+            V t = 0;
+            for (int q = a.rowptr[i]; q < a.rowptr[i + 1]; q++) {
+                const int j = a.cols[q];
+                const V sj = rv[j] - va * vnew[j];
+                t = t + a.A[q] * sj;
+            }
+            y[li] = t;
+            a.s[i] = rv[i] - va * vnew[i];
+        }
     }
     __syncthreads();
     block_lusol(a, B, blk, b0, nb, y, w);
@@ -579,13 +671,27 @@ static size_t apply_lds(const Model &M) {
     return stage_fits<V>(M) ? staged_lds<V>(M.sv.brows, M.sv.max_blk_nnz) : 2 * (size_t)M.sv.brows * sizeof(V);
 }
 
-// block factors of the current matrix (exact block ILU(0))
+// block factors of the current matrix (exact block ILU(0)).  With
+// refactor_every = k > 1 the factors of an earlier matrix of the same
+// pattern are kept for up to k solves: the zeta matrix changes little from
+// one step to the next and the solve reaches the same tolerance with the
+// older preconditioner.  A change of the wet/dry state or iterations grown
+// by half since the factorisation force a new one.
 static size_t factor_lds_staged(const Model &M, size_t vsize) {
     return (size_t)M.sv.max_blk_nnz * vsize + (size_t)(2 * M.sv.max_blk_nnz + 1 + 2 * M.sv.max_blk_pairs) * sizeof(int);
 }
 
 template <typename V>
 static int factor(Model &M, SolvArgs<V> &a) {
+    Solver &v = M.sv;
+    if (v.factored && !v.force_refactor && v.since_factor + 1 < M.p.refactor_every) {
+        v.since_factor++;
+        return 0;
+    }
+    v.factored = true;
+    v.force_refactor = false;
+    v.since_factor = 0;
+    v.its_after_factor = -1;
     const size_t staged = factor_lds_staged(M, sizeof(V));
     timer_begin(M, T_FACTOR);
     if (staged <= kLdsLimit) {
@@ -630,6 +736,7 @@ static int bicgstab_t(Model &M, SolvArgs<V> &a, int *iters, int slot, int zero_x
     if (iters) *iters = (c[CT_NMV] - 1) / 2;
     M.sv.last_its[slot] = (c[CT_NMV] - 1) / 2;
     if (c[CT_FACTOR] != 0) {
+        M.sv.factored = false;
         M.err = "zeta solver: zero pivot in the block factorisation (row " + std::to_string(c[CT_FACTOR]) + ")";
         return 2;
     }
@@ -734,6 +841,7 @@ int solver_solve(Model &M, int *iters, int *refinements) {
             if (resid()) return 1;
             if (read_ctrl(M, c, CT_WORDS)) return 1;
             if (c[CT_FACTOR] != 0) {
+                M.sv.factored = false;
                 M.err = "zeta solver: zero pivot in the block factorisation (row " + std::to_string(c[CT_FACTOR]) + ")";
                 return 2;
             }
@@ -749,6 +857,8 @@ int solver_solve(Model &M, int *iters, int *refinements) {
         M.sv.last_nref = nref;
         M.sv.last_its[1] = nref > 0 ? (it_total + nref - 1) / nref : 0;
     }
+    if (M.sv.its_after_factor < 0) M.sv.its_after_factor = it_total;
+    else if (2 * it_total > 3 * M.sv.its_after_factor + 4) M.sv.force_refactor = true;
     if (iters) *iters = it_total;
     if (refinements) *refinements = nref;
     return 0;

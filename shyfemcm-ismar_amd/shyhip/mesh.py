@@ -55,11 +55,11 @@ class Basin:
     meta: dict = field(default_factory=dict)
 
 
-def _layers(nlv: int, total: float | None = None):
+def _layers(nlv: int, dz: float = 1.0):
     if nlv == 1:
         hlv = np.array([10000.0], dtype=np.float32)
     else:
-        hlv = np.arange(1, nlv + 1, dtype=np.float32)   # 1 m z-layers
+        hlv = (dz * np.arange(1, nlv + 1)).astype(np.float32)   # z-layers of dz metres
     hldv = np.diff(np.concatenate([[0.0], hlv])).astype(np.float32)
     return hlv, hldv
 
@@ -149,7 +149,7 @@ def _smooth_field(rng, nx, ny, sigma):
 def lagoon(nkn_target: int = 50000, seed: int = 20261202, lx: float = 40000.0, ly: float = 20000.0,
            nlv: int = 12, median: float = 1.5, sigma_ln: float = 1.06,
            hmin: float = 0.05, hmax: float = 20.0, jitter: float = 0.3,
-           tile: int = 32, inlet_depth: float = 2.0) -> Basin:
+           tile: int = 32, inlet_depth: float = 2.0, dz: float = 1.0) -> Basin:
     """Jittered-point Delaunay lagoon with a smooth log-normal depth.
 
     Nodes are numbered tile by tile (tile x tile grid points), so contiguous
@@ -197,7 +197,7 @@ def lagoon(nkn_target: int = 50000, seed: int = 20261202, lx: float = 40000.0, l
     eorder = np.lexsort((nen3v.max(axis=1), nen3v.min(axis=1)))
     nen3v = np.ascontiguousarray(nen3v[eorder])
     hm3v = hnode[nen3v]
-    hlv, hldv = _layers(nlv)
+    hlv, hldv = _layers(nlv, dz)
     hm3v, hev, ilhv = _set_levels(hm3v, hlv, nlv)
     nkn = len(xs)
     ibnd = np.zeros(nkn, dtype=np.int32)

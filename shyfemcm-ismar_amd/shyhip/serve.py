@@ -158,10 +158,14 @@ def summary(requests, wall: float) -> dict:
                 throughput=work / wall if wall > 0 else None, wall_s=wall)
 
 
-def poisson_stream(n, rate, seed, sizes=(5000, 50000), steps=(120, 360), nlv=8):
+def poisson_stream(n, rate, seed, sizes=(5000, 50000), steps=(120, 360), nlv=8, median_depth=10.0, dz=3.0):
     """Synthetic request stream: Poisson arrivals (rate per second), mesh
     sizes log-uniform in `sizes`, step counts uniform in `steps`, wind
-    ramping to 30 m/s at a random direction, surge 0.3-1.2 m."""
+    ramping to 30 m/s at a random direction, surge 0.3-1.2 m.  The basins are
+    coastal bays (log-normal depth of median `median_depth`, z-layers of dz
+    metres): a 30 m/s wind over the 1.5 m lagoon of the throughput workload
+    sets the surface down by more than a 1 m top layer, which stops the
+    model as it stops the reference (check_volume)."""
     from . import mesh
     rng = np.random.default_rng(np.uint64(seed))
     t = 0.0
@@ -170,7 +174,8 @@ def poisson_stream(n, rate, seed, sizes=(5000, 50000), steps=(120, 360), nlv=8):
         t += rng.exponential(1.0 / rate)
         nkn = int(math.exp(rng.uniform(math.log(sizes[0]), math.log(sizes[1]))))
         lx = 40000.0 * math.sqrt(nkn / 50000.0)
-        b = mesh.lagoon(nkn, seed=int(seed) + 17 * (i + 1), lx=lx, ly=lx / 2, nlv=nlv, tile=32)
+        b = mesh.lagoon(nkn, seed=int(seed) + 17 * (i + 1), lx=lx, ly=lx / 2, nlv=nlv, tile=32,
+                        median=median_depth, dz=dz)
         out.append(Request(rid=i, basin=b, nsteps=int(rng.integers(steps[0], steps[1] + 1)), arrival=t,
                            wind_max=30.0, wind_dir=float(rng.uniform(0, 2 * math.pi)),
                            surge=float(rng.uniform(0.3, 1.2)), tide_phase=float(rng.uniform(0, 2 * math.pi))))
