@@ -60,9 +60,6 @@ typedef struct {
     int block_rows;                             /* block-ILU(0) block size (<= 1024), 0 = auto */
     int tri_sweeps;                             /* 0: exact level-scheduled triangular solves;
                                                    k > 0: k Jacobi sweeps per triangular factor */
-    int refactor_every;                         /* block ILU(0) refactorised every k-th solve
-                                                   (1 = every solve), and whenever the wet/dry
-                                                   state changed or the iterations grew */
     double inner_rtol;                          /* FP32 path: relative tolerance of the inner solve */
     /* scalars */
     float rkpar, difmol;

@@ -229,7 +229,7 @@ int shyhip_create(shyhip_model **out, int precision, int nkn, int nel, int nlvdi
     p.grav = 9.81f; p.rowass = 1025.f; p.vismol = 1.0e-6f; p.ireib = 0; p.uvmin = 0.2f;
     p.hzmin = 0.01f; p.hzoff = 0.05f; p.hzon = 0.10f; p.volmin = 1.f; p.iclin = 0;
     p.rtol = (double)1.0e-8f; p.atol = (double)1.0e-8f; p.itermax = 1000; p.refine_rtol = 1e-10;
-    p.block_rows = 0; p.tri_sweeps = 0; p.refactor_every = 1; p.inner_rtol = 1e-6; p.rkpar = 0.f; p.difmol = 1.0e-6f; p.itvd = 0; p.itvdv = 0;
+    p.block_rows = 0; p.tri_sweeps = 0; p.inner_rtol = 1e-6; p.rkpar = 0.f; p.difmol = 1.0e-6f; p.itvd = 0; p.itvdv = 0;
     const size_t ne = nel, nk = nkn, L = nlvdi;
     State &s = M.s;
     int rc = 0;
@@ -292,6 +292,11 @@ int shyhip_destroy(shyhip_model *h) {
     for (void *q : {v.lu, v.r0, v.r, v.p0, v.p1, v.v0, v.v1, v.s, v.t, v.dx, v.b, v.x0, (void *)v.part,
                     (void *)v.scal, (void *)v.ctrl, v.a32, (void *)v.r64})
         dfree(q);
+    for (hipStream_t st : M.tr_stream) (void)hipStreamSynchronize(st);
+    for (hipStream_t st : M.tr_stream) (void)hipStreamDestroy(st);
+    for (hipEvent_t ev : M.tr_done) (void)hipEventDestroy(ev);
+    if (M.ev_hydro) (void)hipEventDestroy(M.ev_hydro);
+    for (void *q : M.tr_hbuf) dfree(q);
     dfree(M.d_err);
     dfree(M.d_levels);
     if (M.h_levels) (void)hipHostFree(M.h_levels);
@@ -317,7 +322,6 @@ int shyhip_set_params(shyhip_model *h, const shyhip_params *p) {
     if (p->itvd == 1) return fail(&M, "itvd = 1 (gradient TVD) is not supported by the device path; use 0 or 2");
     if (p->block_rows < 0 || p->block_rows > kMaxBlockRows) return fail(&M, "block_rows must be in 0..1024");
     if (p->tri_sweeps < 0 || p->tri_sweeps > 64) return fail(&M, "tri_sweeps must be in 0..64");
-    if (p->refactor_every < 1) return fail(&M, "refactor_every must be >= 1");
     if (!(p->inner_rtol > 0. && p->inner_rtol < 1.)) return fail(&M, "inner_rtol must be in (0,1)");
     M.p = *p;
     return 0;
@@ -828,7 +832,6 @@ static int hydro_impl(Model &M, double dtime, bool final_check) {
         timer_begin(M, T_WETDRY);
         if (launch_system_get_setweg1(M, bnew, &changed)) return 1;
         timer_end(M);
-        if (changed) M.sv.force_refactor = true;    // rows of the zeta matrix changed
         M.dtime_old = M.dtime;
         if (changed == 0) break;
     }
@@ -858,17 +861,39 @@ int shyhip_hydro(shyhip_model *h, double dtime) {
     return hydro_impl(h->M, dtime, true);
 }
 
-int shyhip_scal3sh(shyhip_model *h, int itr, int istot, int clip_small) {
-    if (!h) return fail(nullptr, "null handle");
-    Model &M = h->M;
+// scal3sh of tracer itr on stream st with conz3d scratch hbuf
+static int scal3sh_on(Model &M, int itr, int istot, int clip_small, hipStream_t st, void *hbuf) {
     if (itr < 0 || itr >= M.ntr) return fail(&M, "tracer index out of range");
     if (istot < 1 || istot > 1000) return fail(&M, "istot must be in 1..1000");
     if (M.p.itvd == 2 && !M.tvd_ready) return fail(&M, "itvd = 2 needs shyhip_tvd_init");
     for (int isact = 1; isact <= istot; isact++) {
-        if (launch_conz3d(M, itr, istot, isact, clip_small)) return 1;
+        if (launch_conz3d(M, itr, istot, isact, clip_small, st, hbuf)) return 1;
         timer_begin(M, T_BNDO);
-        if (launch_bndo(M, itr)) return 1;           // bndo_setbc
+        if (launch_bndo(M, itr, st)) return 1;       // bndo_setbc
         timer_end(M);
+    }
+    return 0;
+}
+
+int shyhip_scal3sh(shyhip_model *h, int itr, int istot, int clip_small) {
+    if (!h) return fail(nullptr, "null handle");
+    return scal3sh_on(h->M, itr, istot, clip_small, h->M.stream, h->M.s.hbuf);
+}
+
+// streams, events and scratch of the tracers after the first (created once)
+static int tracer_streams(Model &M) {
+    if (!M.tr_stream.empty() || M.ntr < 2) return 0;
+    HIPCHK(M, hipEventCreateWithFlags(&M.ev_hydro, hipEventDisableTiming));
+    for (int t = 1; t < M.ntr; t++) {
+        hipStream_t st = nullptr;
+        hipEvent_t ev = nullptr;
+        HIPCHK(M, hipStreamCreate(&st));
+        M.tr_stream.push_back(st);
+        HIPCHK(M, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        M.tr_done.push_back(ev);
+        void *hb = nullptr;
+        if (dalloc_s(M, &hb, 3 * (size_t)M.nlvdi * M.nel)) return 1;
+        M.tr_hbuf.push_back(hb);
     }
     return 0;
 }
@@ -883,8 +908,22 @@ int shyhip_step(shyhip_model *h, double dtime, int istot) {
     if (launch_copy_old(M)) return 1;
     timer_end(M);
     if (hydro_impl(M, dtime, false)) return M.err.empty() ? 1 : 3;
-    for (int t = 0; t < M.ntr; t++)
-        if (shyhip_scal3sh(h, t, istot, t == 0 ? 0 : 1)) return 1;
+    if (M.ntr > 1 && !M.timing) {
+        // the tracers are transported concurrently: each after hydro, each
+        // on its own stream, joined back into the model's stream
+        if (tracer_streams(M)) return 1;
+        HIPCHK(M, hipEventRecord(M.ev_hydro, M.stream));
+        for (int t = 1; t < M.ntr; t++) HIPCHK(M, hipStreamWaitEvent(M.tr_stream[t - 1], M.ev_hydro, 0));
+        if (scal3sh_on(M, 0, istot, 0, M.stream, M.s.hbuf)) return 1;
+        for (int t = 1; t < M.ntr; t++) {
+            if (scal3sh_on(M, t, istot, 1, M.tr_stream[t - 1], M.tr_hbuf[t - 1])) return 1;
+            HIPCHK(M, hipEventRecord(M.tr_done[t - 1], M.tr_stream[t - 1]));
+            HIPCHK(M, hipStreamWaitEvent(M.stream, M.tr_done[t - 1], 0));
+        }
+    } else {
+        for (int t = 0; t < M.ntr; t++)
+            if (shyhip_scal3sh(h, t, istot, t == 0 ? 0 : 1)) return 1;
+    }
     if (timer_flush(M)) return 1;
     return check_dev_err(M, "step");
 }

@@ -355,7 +355,7 @@ __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
 }
 
 template <typename S, int NL>
-static int conz_nl(Model &M, int itr, int istot, int isact, int clip) {
+static int conz_nl(Model &M, int itr, int istot, int isact, int clip, hipStream_t st, void *hbuf) {
     ConzArgs<S> a{};
     a.nkn = M.nkn; a.nel = M.nel; a.nlvdi = M.nlvdi; a.nlv = M.nlv;
     a.itvd = M.p.itvd; a.itvdv = M.p.itvdv; a.clip = clip;
@@ -375,33 +375,33 @@ static int conz_nl(Model &M, int itr, int istot, int isact, int clip) {
     a.mflux = M.has_mflux ? (const S *)M.s.mflux : nullptr;
     a.c = (S *)M.s.tracer[itr];
     a.cbound = nullptr;
-    a.hbuf = (S *)M.s.hbuf;
+    a.hbuf = (S *)hbuf;
     a.err = M.d_err;
     const int ge = (M.nel + kElemTile - 1) / kElemTile;
     timer_begin(M, T_CONZ_ELEM);
-    hipLaunchKernelGGL(k_conz_elem<S>, dim3(ge), dim3(kThreads), 0, M.stream, a);
+    hipLaunchKernelGGL(k_conz_elem<S>, dim3(ge), dim3(kThreads), 0, st, a);
     timer_end(M);
     HIPCHK(M, hipGetLastError());
     timer_begin(M, T_CONZ_NODE);
     hipLaunchKernelGGL((k_conz_node<S, NL>), dim3((M.nkn + NodeTile<NL>::TN - 1) / NodeTile<NL>::TN), dim3(kThreads),
-                       0, M.stream, a);
+                       0, st, a);
     timer_end(M);
     HIPCHK(M, hipGetLastError());
     return 0;
 }
 
 template <typename S>
-static int conz_t(Model &M, int itr, int istot, int isact, int clip) {
+static int conz_t(Model &M, int itr, int istot, int isact, int clip, hipStream_t st, void *hbuf) {
     switch (M.nl_tmpl) {
-    case 1: return conz_nl<S, 1>(M, itr, istot, isact, clip);
-    case 2: return conz_nl<S, 2>(M, itr, istot, isact, clip);
-    case 4: return conz_nl<S, 4>(M, itr, istot, isact, clip);
-    case 8: return conz_nl<S, 8>(M, itr, istot, isact, clip);
-    case 12: return conz_nl<S, 12>(M, itr, istot, isact, clip);
-    case 16: return conz_nl<S, 16>(M, itr, istot, isact, clip);
-    case 20: return conz_nl<S, 20>(M, itr, istot, isact, clip);
-    case 24: return conz_nl<S, 24>(M, itr, istot, isact, clip);
-    default: return conz_nl<S, 32>(M, itr, istot, isact, clip);
+    case 1: return conz_nl<S, 1>(M, itr, istot, isact, clip, st, hbuf);
+    case 2: return conz_nl<S, 2>(M, itr, istot, isact, clip, st, hbuf);
+    case 4: return conz_nl<S, 4>(M, itr, istot, isact, clip, st, hbuf);
+    case 8: return conz_nl<S, 8>(M, itr, istot, isact, clip, st, hbuf);
+    case 12: return conz_nl<S, 12>(M, itr, istot, isact, clip, st, hbuf);
+    case 16: return conz_nl<S, 16>(M, itr, istot, isact, clip, st, hbuf);
+    case 20: return conz_nl<S, 20>(M, itr, istot, isact, clip, st, hbuf);
+    case 24: return conz_nl<S, 24>(M, itr, istot, isact, clip, st, hbuf);
+    default: return conz_nl<S, 32>(M, itr, istot, isact, clip, st, hbuf);
     }
 }
 
@@ -482,10 +482,10 @@ __global__ __launch_bounds__(kBndoLanes) void k_bndo(int nbndo, const int *bk, c
 }
 
 template <typename S>
-static int bndo_t(Model &M, int itr) {
+static int bndo_t(Model &M, int itr, hipStream_t st) {
     if (M.m.nbndo == 0) return 0;
     if (M.nlv > kBndoLanes) return hip_check(M, hipErrorInvalidValue, "bndo: more layers than lanes");
-    hipLaunchKernelGGL(k_bndo<S>, dim3(M.m.nbndo), dim3(kBndoLanes), 0, M.stream, M.m.nbndo, M.m.bndo_k,
+    hipLaunchKernelGGL(k_bndo<S>, dim3(M.m.nbndo), dim3(kBndoLanes), 0, st, M.m.nbndo, M.m.bndo_k,
                        (const S *)M.m.bndo_nx, (const S *)M.m.bndo_ny, M.m.bndo_ptr, M.m.bndo_nb,
                        (const S *)M.m.bndo_w, M.m.ibnd, M.m.ibtyp, M.m.ilhkv, M.m.ilhv, M.s.iweg, M.m.nk_ptr,
                        M.m.nk_el, M.m.geo, (const S *)M.s.uln, (const S *)M.s.vln, (S *)M.s.tracer[itr],
@@ -494,11 +494,13 @@ static int bndo_t(Model &M, int itr) {
     return 0;
 }
 
-int launch_bndo(Model &M, int itr) { return M.prec == SHYHIP_FP64 ? bndo_t<double>(M, itr) : bndo_t<float>(M, itr); }
+int launch_bndo(Model &M, int itr, hipStream_t st) {
+    return M.prec == SHYHIP_FP64 ? bndo_t<double>(M, itr, st) : bndo_t<float>(M, itr, st);
+}
 
-int launch_conz3d(Model &M, int itr, int istot, int isact, int clip_small) {
-    return M.prec == SHYHIP_FP64 ? conz_t<double>(M, itr, istot, isact, clip_small)
-                                 : conz_t<float>(M, itr, istot, isact, clip_small);
+int launch_conz3d(Model &M, int itr, int istot, int isact, int clip_small, hipStream_t st, void *hbuf) {
+    return M.prec == SHYHIP_FP64 ? conz_t<double>(M, itr, istot, isact, clip_small, st, hbuf)
+                                 : conz_t<float>(M, itr, istot, isact, clip_small, st, hbuf);
 }
 
 }  // namespace shy

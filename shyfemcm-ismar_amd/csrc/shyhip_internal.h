@@ -80,10 +80,6 @@ struct Solver {               // Krylov work space (value type V = S of the mode
     int nblk = 0, brows = 0, max_blk_nnz = 0, max_blk_pairs = 0, nparts = 0;
     int last_its[8] = {};     // iteration count of the previous solve, per solve kind
     int last_nref = 0;        // FP64 refinement rounds of the previous solve
-    bool factored = false;    // lu holds block factors of an earlier matrix
-    bool force_refactor = false;
-    int since_factor = 0;     // solves since the last factorisation
-    int its_after_factor = 0; // iterations of the first solve with the current factors
     void *lu = nullptr;       // block factors (values of the block pattern)
     void *r0 = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr, *v0 = nullptr, *v1 = nullptr,
          *s = nullptr, *t = nullptr, *dx = nullptr, *b = nullptr, *x0 = nullptr;
@@ -117,6 +113,13 @@ struct Model {
     // [mem_node[m], mem_node[m+1]) and elements [mem_elem[m], mem_elem[m+1])
     std::vector<int> mem_node, mem_elem;
     float *d_levels = nullptr, *h_levels = nullptr;   // boundary levels staging (nbnd)
+    // tracers after the first run on streams of their own inside a step
+    // (independent transports, shared read-only state), each with its own
+    // scratch; created on first use
+    std::vector<hipStream_t> tr_stream;
+    std::vector<hipEvent_t> tr_done;
+    hipEvent_t ev_hydro = nullptr;
+    std::vector<void *> tr_hbuf;
     std::string err;
     // host-side copies needed for init / plugin / checks
     std::vector<int> h_nen3v, h_rowptr, h_cols, h_ijp, h_color, h_ilhv, h_ilhkv, h_ibnd, h_ibtyp, h_iextb;
@@ -179,8 +182,8 @@ int launch_volume(Model &M, int mode, double *vol);
 int launch_tracer_mass(Model &M, int itr, double *mass);
 int solver_setup(Model &M);
 int solver_solve(Model &M, int *iters, int *refinements);
-int launch_conz3d(Model &M, int itr, int istot, int isact, int clip_small);
-int launch_bndo(Model &M, int itr);
+int launch_conz3d(Model &M, int itr, int istot, int isact, int clip_small, hipStream_t st, void *hbuf);
+int launch_bndo(Model &M, int itr, hipStream_t st);
 
 // helpers
 int hip_check(Model &M, hipError_t e, const char *what);

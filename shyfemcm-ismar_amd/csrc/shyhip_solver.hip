@@ -13,12 +13,14 @@
 // Krylov method: BiCGStab in the arrangement the reference uses -- runrc with
 // ipar(2)=1, which bisinit maps to LEFT preconditioning (spk_iters.f90:
 // 1061-1334, 3433-3500) -- with stopping test ||M^-1 r|| <= rtol*||M^-1 r0||
-// + atol and at most itermax matvecs.  One iteration is three kernels: every
+// + atol and at most itermax matvecs.  One iteration is two kernels: every
 // vector update is fused into the SpMV/preconditioner kernel that consumes
-// it, and every dot product is written as per-workgroup partials that every
-// workgroup of the next kernel reduces itself in the same fixed order -- no
-// atomics or device-scope fences, visibility comes from the kernel boundary,
-// and all workgroups (and all runs) see bitwise identical scalars.
+// it (the update of x and r closing an iteration is done by the first kernel
+// of the next, from five dot products of the second), and every dot product
+// is written as per-workgroup partials that every workgroup of the next
+// kernel reduces itself in the same fixed order -- no atomics or
+// device-scope fences, visibility comes from the kernel boundary, and all
+// workgroups (and all runs) see bitwise identical scalars.
 // The FP32 path runs this in FP32 inside an FP64 iterative refinement whose
 // control also lives on the device: every kernel of a refinement round
 // returns at once when the FP64 residual test has passed, so the host
@@ -671,27 +673,16 @@ static size_t apply_lds(const Model &M) {
     return stage_fits<V>(M) ? staged_lds<V>(M.sv.brows, M.sv.max_blk_nnz) : 2 * (size_t)M.sv.brows * sizeof(V);
 }
 
-// block factors of the current matrix (exact block ILU(0)).  With
-// refactor_every = k > 1 the factors of an earlier matrix of the same
-// pattern are kept for up to k solves: the zeta matrix changes little from
-// one step to the next and the solve reaches the same tolerance with the
-// older preconditioner.  A change of the wet/dry state or iterations grown
-// by half since the factorisation force a new one.
+// block factors of the current matrix (exact block ILU(0)), every solve: the
+// wet/dry state changes rows of the zeta matrix from one step to the next
+// (dry-element rows, level boundaries), and factors of an earlier matrix
+// then no longer precondition it.
 static size_t factor_lds_staged(const Model &M, size_t vsize) {
     return (size_t)M.sv.max_blk_nnz * vsize + (size_t)(2 * M.sv.max_blk_nnz + 1 + 2 * M.sv.max_blk_pairs) * sizeof(int);
 }
 
 template <typename V>
 static int factor(Model &M, SolvArgs<V> &a) {
-    Solver &v = M.sv;
-    if (v.factored && !v.force_refactor && v.since_factor + 1 < M.p.refactor_every) {
-        v.since_factor++;
-        return 0;
-    }
-    v.factored = true;
-    v.force_refactor = false;
-    v.since_factor = 0;
-    v.its_after_factor = -1;
     const size_t staged = factor_lds_staged(M, sizeof(V));
     timer_begin(M, T_FACTOR);
     if (staged <= kLdsLimit) {
@@ -736,7 +727,6 @@ static int bicgstab_t(Model &M, SolvArgs<V> &a, int *iters, int slot, int zero_x
     if (iters) *iters = (c[CT_NMV] - 1) / 2;
     M.sv.last_its[slot] = (c[CT_NMV] - 1) / 2;
     if (c[CT_FACTOR] != 0) {
-        M.sv.factored = false;
         M.err = "zeta solver: zero pivot in the block factorisation (row " + std::to_string(c[CT_FACTOR]) + ")";
         return 2;
     }
@@ -841,8 +831,7 @@ int solver_solve(Model &M, int *iters, int *refinements) {
             if (resid()) return 1;
             if (read_ctrl(M, c, CT_WORDS)) return 1;
             if (c[CT_FACTOR] != 0) {
-                M.sv.factored = false;
-                M.err = "zeta solver: zero pivot in the block factorisation (row " + std::to_string(c[CT_FACTOR]) + ")";
+                        M.err = "zeta solver: zero pivot in the block factorisation (row " + std::to_string(c[CT_FACTOR]) + ")";
                 return 2;
             }
             if (c[CT_REFCONV]) break;
@@ -857,8 +846,6 @@ int solver_solve(Model &M, int *iters, int *refinements) {
         M.sv.last_nref = nref;
         M.sv.last_its[1] = nref > 0 ? (it_total + nref - 1) / nref : 0;
     }
-    if (M.sv.its_after_factor < 0) M.sv.its_after_factor = it_total;
-    else if (2 * it_total > 3 * M.sv.its_after_factor + 4) M.sv.force_refactor = true;
     if (iters) *iters = it_total;
     if (refinements) *refinements = nref;
     return 0;

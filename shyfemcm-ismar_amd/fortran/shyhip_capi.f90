@@ -42,7 +42,6 @@ module shyhip_capi
     real(c_double) :: refine_rtol
     integer(c_int) :: block_rows
     integer(c_int) :: tri_sweeps
-    integer(c_int) :: refactor_every
     real(c_double) :: inner_rtol
     real(c_float)  :: rkpar, difmol
     integer(c_int) :: itvd, itvdv
