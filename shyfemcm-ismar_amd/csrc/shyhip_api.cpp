@@ -284,12 +284,13 @@ int shyhip_destroy(shyhip_model *h) {
                     (void *)m.ibtyp, m.fcorv, m.czv, (void *)m.nk_ptr, (void *)m.nk_el, (void *)m.nkc_el,
                     (void *)m.rowptr, (void *)m.cols, (void *)m.ijp, (void *)m.bnd_nodes, (void *)m.bnd_id,
                     (void *)m.blk_rowptr, (void *)m.blk_cols, (void *)m.blk_src, (void *)m.blk_diag, (void *)m.blk_cp_ptr,
-                    (void *)m.blk_cp_a, (void *)m.blk_cp_b, (void *)m.blk_jd,
+                    (void *)m.blk_cp_a, (void *)m.blk_cp_b, (void *)m.blk_jd, (void *)m.blk_pcol, (void *)m.blk_prp,
+                    (void *)m.blk_pdg, (void *)m.blk_elp, (void *)m.blk_elr, (void *)m.blk_enlev, (void *)m.ell_col, (void *)m.ell_len, (void *)m.csr_ell,
                     (void *)m.fwd_rows, (void *)m.fwd_lptr, (void *)m.fwd_nlev, (void *)m.bwd_rows,
                     (void *)m.bwd_lptr, (void *)m.bwd_nlev, (void *)m.tvd_el, (void *)m.tvd_kn, m.tvd_w})
         dfree(q);
     Solver &v = M.sv;
-    for (void *q : {v.lu, v.r0, v.r, v.p0, v.p1, v.v0, v.v1, v.s, v.t, v.dx, v.b, v.x0, (void *)v.part,
+    for (void *q : {v.lu, v.ell_a, v.r0, v.r, v.p0, v.p1, v.v0, v.v1, v.s, v.t, v.dx, v.b, v.x0, (void *)v.part,
                     (void *)v.scal, (void *)v.ctrl, v.a32, (void *)v.r64})
         dfree(q);
     for (hipStream_t st : M.tr_stream) (void)hipStreamSynchronize(st);
@@ -498,8 +499,57 @@ int shyhip_set_basin(shyhip_model *h, const int *nen3v, const float *xgv, const 
         for (int b = 0; b < bs.nblk; b++)
             for (int i = b * brows; i < std::min(nkn, (b + 1) * brows); i++)
                 if (bs.diag[i] < 0) return fail(&M, "zeta pattern without diagonal");
+        {
+            // fixed-stride copies of the block pattern (relative indices)
+            const int nbk = bs.nblk;
+            std::vector<int> pcol((size_t)nbk * mx, 0), prp((size_t)nbk * (brows + 1), 0), pdg((size_t)nbk * brows, 0);
+            for (int b = 0; b < nbk; b++) {
+                const int b0 = b * brows, nb = std::min(nkn, b0 + brows) - b0;
+                const int q0 = bs.rowptr[b0], ne = bs.rowptr[b0 + nb] - q0;
+                for (int e = 0; e < ne; e++) pcol[(size_t)b * mx + e] = bs.cols[q0 + e];
+                for (int li = 0; li <= brows; li++)
+                    prp[(size_t)b * (brows + 1) + li] = li <= nb ? bs.rowptr[b0 + li] - q0 : ne;
+                for (int li = 0; li < nb; li++) pdg[(size_t)b * brows + li] = bs.diag[b0 + li] - q0;
+            }
+            // entries by level, per block
+            int dmax = 0;
+            for (int b = 0; b < nbk; b++) dmax = std::max(dmax, bs.ent_nlev[b]);
+            M.sv.max_elev = dmax;
+            std::vector<int> elp((size_t)nbk * (dmax + 1), 0), elr((size_t)nbk * mx, 0);
+            for (int b = 0; b < nbk; b++) {
+                const int b0 = b * brows, nb = std::min(nkn, b0 + brows) - b0;
+                const int q0 = bs.rowptr[b0], ne = bs.rowptr[b0 + nb] - q0, nl = bs.ent_nlev[b];
+                int *lp = elp.data() + (size_t)b * (dmax + 1);
+                for (int e = 0; e < ne; e++) lp[bs.ent_lev[q0 + e] + 1]++;
+                for (int l = 0; l < dmax; l++) lp[l + 1] += lp[l];
+                std::vector<int> pos(lp, lp + nl);
+                for (int e = 0; e < ne; e++) elr[(size_t)b * mx + pos[bs.ent_lev[q0 + e]]++] = e;
+            }
+            rc |= dalloc(M, &m.blk_elp, elp.size()) || upload(M, m.blk_elp, elp.data(), elp.size());
+            rc |= dalloc(M, &m.blk_elr, elr.size()) || upload(M, m.blk_elr, elr.data(), elr.size());
+            rc |= dalloc(M, &m.blk_enlev, bs.ent_nlev.size()) ||
+                  upload(M, m.blk_enlev, bs.ent_nlev.data(), bs.ent_nlev.size());
+            rc |= dalloc(M, &m.blk_pcol, pcol.size()) || upload(M, m.blk_pcol, pcol.data(), pcol.size());
+            rc |= dalloc(M, &m.blk_prp, prp.size()) || upload(M, m.blk_prp, prp.data(), prp.size());
+            rc |= dalloc(M, &m.blk_pdg, pdg.size()) || upload(M, m.blk_pdg, pdg.data(), pdg.size());
+            // ELL slices of the matrix rows
+            std::vector<int> ecol((size_t)kEllWidth * nkn), elen(nkn), cse(M.nnz, -1);
+            for (int i = 0; i < nkn; i++) {
+                const int q0 = M.h_rowptr[i], len = M.h_rowptr[i + 1] - q0;
+                elen[i] = len;
+                for (int k = 0; k < kEllWidth; k++) {
+                    const size_t slot = (size_t)k * nkn + i;
+                    ecol[slot] = k < len ? M.h_cols[q0 + k] : i;
+                    if (k < len) cse[q0 + k] = (int)slot;
+                }
+            }
+            rc |= dalloc(M, &m.ell_col, ecol.size()) || upload(M, m.ell_col, ecol.data(), ecol.size());
+            rc |= dalloc(M, &m.ell_len, elen.size()) || upload(M, m.ell_len, elen.data(), elen.size());
+            rc |= dalloc(M, &m.csr_ell, cse.size()) || upload(M, m.csr_ell, cse.data(), cse.size());
+        }
         Solver &v = M.sv;
-        rc |= dalloc_s(M, &v.lu, bs.cols.size());
+        rc |= dalloc_s(M, &v.lu, (size_t)bs.nblk * mx);
+        rc |= dalloc_s(M, &v.ell_a, (size_t)kEllWidth * nkn);
         for (void **q : {&v.r0, &v.r, &v.p0, &v.p1, &v.v0, &v.v1, &v.s, &v.t, &v.dx, &v.b, &v.x0})
             rc |= dalloc_s(M, q, (size_t)nkn);
         const size_t nparts = std::max((size_t)bs.nblk, (size_t)(nkn + kThreads - 1) / kThreads) + 1;

@@ -378,14 +378,10 @@ static int conz_nl(Model &M, int itr, int istot, int isact, int clip, hipStream_
     a.hbuf = (S *)hbuf;
     a.err = M.d_err;
     const int ge = (M.nel + kElemTile - 1) / kElemTile;
-    timer_begin(M, T_CONZ_ELEM);
-    hipLaunchKernelGGL(k_conz_elem<S>, dim3(ge), dim3(kThreads), 0, st, a);
-    timer_end(M);
+    launch_timed(M, T_CONZ_ELEM, k_conz_elem<S>, dim3(ge), dim3(kThreads), 0, st, a);
     HIPCHK(M, hipGetLastError());
-    timer_begin(M, T_CONZ_NODE);
-    hipLaunchKernelGGL((k_conz_node<S, NL>), dim3((M.nkn + NodeTile<NL>::TN - 1) / NodeTile<NL>::TN), dim3(kThreads),
-                       0, st, a);
-    timer_end(M);
+    launch_timed(M, T_CONZ_NODE, k_conz_node<S, NL>, dim3((M.nkn + NodeTile<NL>::TN - 1) / NodeTile<NL>::TN),
+                 dim3(kThreads), 0, st, a);
     HIPCHK(M, hipGetLastError());
     return 0;
 }

@@ -439,7 +439,7 @@ void host_block_schedule(int n, const std::vector<int> &rowptr, const std::vecto
         const int b0 = (i / brows) * brows, li = i - b0;
         for (int q = bs.rowptr[i]; q < bs.rowptr[i + 1]; q++) {
             const int lj = bs.cols[q];
-            bs.ent_jd[q] = lj < li ? bs.diag[b0 + lj] : (lj == li ? -2 : -1);
+            bs.ent_jd[q] = lj < li ? bs.diag[b0 + lj] : (lj == li ? -2 - li : -1);
             const int m = std::min(li, lj);
             for (int qk = bs.rowptr[i]; qk < bs.rowptr[i + 1]; qk++) {
                 const int lk = bs.cols[qk];
@@ -452,6 +452,22 @@ void host_block_schedule(int n, const std::vector<int> &rowptr, const std::vecto
                 }
             }
             bs.cp_ptr[q + 1] = (int)bs.cp_a.size();
+        }
+    }
+    // entry levels: every entry an update pair or a pivot of q refers to
+    // lies in an earlier row or earlier in the same row, so one pass in
+    // entry order sees them computed
+    bs.ent_lev.assign(ne, 0);
+    bs.ent_nlev.assign(bs.nblk, 0);
+    for (int i = 0; i < n; i++) {
+        const int blk = i / brows;
+        for (int q = bs.rowptr[i]; q < bs.rowptr[i + 1]; q++) {
+            int lv = 0;
+            for (int p = bs.cp_ptr[q]; p < bs.cp_ptr[q + 1]; p++)
+                lv = std::max(lv, std::max(bs.ent_lev[bs.cp_a[p]], bs.ent_lev[bs.cp_b[p]]) + 1);
+            if (bs.ent_jd[q] >= 0) lv = std::max(lv, bs.ent_lev[bs.ent_jd[q]] + 1);
+            bs.ent_lev[q] = lv;
+            bs.ent_nlev[blk] = std::max(bs.ent_nlev[blk], lv + 1);
         }
     }
 }

@@ -9,6 +9,7 @@
 // Host push/pull transposes from/to the reference's (l,ie) layout.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -21,6 +22,7 @@ constexpr float kFlag = -9988765.0f;                 // mkonst flag (const.f90:1
 constexpr double kDrittl = (double)(1.0f / 3.0f);    // default-real 1./3. in double
 constexpr int kThreads = 256;
 constexpr int kMaxBlockRows = 1024;                  // block-ILU(0) rows per workgroup (LDS)
+constexpr int kEllWidth = 10;                        // matrix entries per row held in ELL slots
 
 // error codes written by kernels into Model::d_err[0]
 enum DevErr { ERR_NONE = 0, ERR_SETNOD = 1, ERR_SETUVD = 3, ERR_SETDEPTH = 4, ERR_VOLUME = 5,
@@ -44,6 +46,17 @@ struct Mesh {                 // device copies of init-time data
     // block-ILU(0)
     int *blk_rowptr = nullptr, *blk_cols = nullptr, *blk_src = nullptr, *blk_diag = nullptr;
     int *blk_cp_ptr = nullptr, *blk_cp_a = nullptr, *blk_cp_b = nullptr, *blk_jd = nullptr;
+    // the same blocks at a fixed stride per block (entries blk*mx.., rows
+    // blk*brows.., row starts blk*(brows+1)..; indices relative to the block),
+    // so a kernel can fetch its block without first reading where it starts
+    int *blk_pcol = nullptr, *blk_prp = nullptr, *blk_pdg = nullptr;
+    // entry-level factorisation schedule: level starts [blk*(max_elev+1) + lv]
+    // and the block's entries by level [blk*max_blk_nnz + t]; levels per block
+    int *blk_elp = nullptr, *blk_elr = nullptr, *blk_enlev = nullptr;
+    // zeta matrix rows sliced ELL-style: the first kEllWidth entries of row i
+    // at [k*nkn + i] (columns padded with i), the row length, and for each
+    // CSR entry its slot (or -1 beyond the width)
+    int *ell_col = nullptr, *ell_len = nullptr, *csr_ell = nullptr;
     int *fwd_rows = nullptr, *fwd_lptr = nullptr, *fwd_nlev = nullptr;
     int *bwd_rows = nullptr, *bwd_lptr = nullptr, *bwd_nlev = nullptr;
     // TVD upwind tables: per element 6 ordered pairs (ic,id); [p*nel + ie]
@@ -77,10 +90,11 @@ struct State {
 };
 
 struct Solver {               // Krylov work space (value type V = S of the model)
-    int nblk = 0, brows = 0, max_blk_nnz = 0, max_blk_pairs = 0, nparts = 0;
+    int nblk = 0, brows = 0, max_blk_nnz = 0, max_blk_pairs = 0, nparts = 0, max_elev = 0;
     int last_its[8] = {};     // iteration count of the previous solve, per solve kind
     int last_nref = 0;        // FP64 refinement rounds of the previous solve
-    void *lu = nullptr;       // block factors (values of the block pattern)
+    void *lu = nullptr;       // block factors, fixed stride max_blk_nnz per block
+    void *ell_a = nullptr;    // matrix values in the ELL slots (zero padding)
     void *r0 = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr, *v0 = nullptr, *v1 = nullptr,
          *s = nullptr, *t = nullptr, *dx = nullptr, *b = nullptr, *x0 = nullptr;
     double *part = nullptr;   // [32*nparts] per-workgroup partial sums (4 regions x 8)
@@ -141,6 +155,24 @@ void timer_begin(Model &M, int id);
 void timer_end(Model &M);
 int timer_flush(Model &M);
 
+// Launch of a single kernel timed by events bound to its own dispatch when
+// timing is enabled: the events carry the kernel's start and end
+// timestamps, so the figure is the kernel's execution time, without the
+// marker packets that timer_begin/timer_end put in the stream.
+template <typename... KArgs, typename... Args>
+void launch_timed(Model &M, int id, void (*kernel)(KArgs...), dim3 grid, dim3 block, size_t lds, hipStream_t st,
+                  Args... args) {
+    if (!M.timing) {
+        hipLaunchKernelGGL(kernel, grid, block, lds, st, args...);
+        return;
+    }
+    Model::TEv t{id, nullptr, nullptr};
+    (void)hipEventCreate(&t.a);
+    (void)hipEventCreate(&t.b);
+    hipExtLaunchKernelGGL(kernel, grid, block, (std::uint32_t)lds, st, t.a, t.b, 0u, args...);
+    M.tev.push_back(t);
+}
+
 // host-side mesh preprocessing (shyhip_init.cpp)
 void host_set_ev(int nkn, int nel, const int *nen3v0, const float *x, const float *y, double *ev);
 int host_node_list(int nkn, int nel, const int *nen3v0, std::vector<int> &ntg, std::vector<int> &cols,
@@ -156,8 +188,12 @@ struct BlockSchedule {
     std::vector<int> rowptr, cols, src, diag, fwd_rows, fwd_lptr, fwd_nlev, bwd_rows, bwd_lptr, bwd_nlev;
     // ILU(0) updates: for entry q of the block pattern, the pairs (q_ik, q_kj),
     // k < min(i, j) ascending, and ent_jd[q] = the diagonal entry of column j
-    // for an L entry, -2 for a diagonal, -1 for U
+    // for an L entry, -2 - (local row) for a diagonal, -1 for U
     std::vector<int> cp_ptr, cp_a, cp_b, ent_jd;
+    // entry-level schedule of the factorisation: ent_lev[q] = 1 + the
+    // highest level among the entries that q reads (0 if none); ent_nlev[b]
+    // = levels of block b
+    std::vector<int> ent_lev, ent_nlev;
 };
 void host_block_schedule(int n, const std::vector<int> &rowptr, const std::vector<int> &cols, int brows,
                          BlockSchedule &bs);

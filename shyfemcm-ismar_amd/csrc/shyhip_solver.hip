@@ -55,7 +55,12 @@ struct SolvArgs {
     const V *A;
     const int *brp, *bcol, *bsrc, *bdiag, *frows, *flptr, *fnlev, *brows_l, *blptr, *bnlev;
     const int *cp_ptr, *cp_a, *cp_b, *bjd;
-    V *lu;
+    const int *pcol, *prp, *pdg;     // block pattern at a fixed stride per block
+    const int *elp, *elr, *enlev;    // entry-level factorisation schedule
+    int edmax;
+    const int *ell_col, *ell_len;    // ELL slices of the matrix rows
+    const V *ell_a;
+    V *lu;                           // block factors at a fixed stride per block
     const double *aval;
     V *b, *x0, *r0, *r, *p0, *p1, *v0, *v1, *s, *t, *dx;
     double *part, *scal;
@@ -75,6 +80,9 @@ static SolvArgs<V> solv_args(Model &M, const V *A, V *b, V *x0) {
     a.frows = M.m.fwd_rows; a.flptr = M.m.fwd_lptr; a.fnlev = M.m.fwd_nlev;
     a.brows_l = M.m.bwd_rows; a.blptr = M.m.bwd_lptr; a.bnlev = M.m.bwd_nlev;
     a.cp_ptr = M.m.blk_cp_ptr; a.cp_a = M.m.blk_cp_a; a.cp_b = M.m.blk_cp_b; a.bjd = M.m.blk_jd;
+    a.pcol = M.m.blk_pcol; a.prp = M.m.blk_prp; a.pdg = M.m.blk_pdg;
+    a.elp = M.m.blk_elp; a.elr = M.m.blk_elr; a.enlev = M.m.blk_enlev; a.edmax = M.sv.max_elev;
+    a.ell_col = M.m.ell_col; a.ell_len = M.m.ell_len; a.ell_a = (const V *)M.sv.ell_a;
     a.lu = (V *)M.sv.lu; a.aval = M.s.aval;
     a.b = b; a.x0 = x0;
     a.r0 = (V *)M.sv.r0; a.r = (V *)M.sv.r; a.p0 = (V *)M.sv.p0; a.p1 = (V *)M.sv.p1;
@@ -129,6 +137,26 @@ __device__ inline void sum_partials(const double *part, int region, int nparts, 
     block_sum<NQ>(out);
 }
 
+// The same total with this work-item's first partials loaded early (the
+// loads are issued at kernel start, the reduction done when needed); only
+// the first nq of the NQ slots are read, the others are zero.
+template <int NQ>
+struct Partials {
+    double v[NQ];
+    __device__ inline void load(const double *part, int region, int nparts, int n, int nq) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++)
+            v[q] = (q < nq && (int)threadIdx.x < n) ? part[((size_t)region * kRegion + q) * nparts + threadIdx.x] : 0.;
+    }
+    __device__ inline void total(const double *part, int region, int nparts, int n, int nq, double (&out)[NQ]) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++) out[q] = 0. + v[q];
+        for (int i = threadIdx.x + kThreads; i < n; i += kThreads)
+            for (int q = 0; q < nq; q++) out[q] += part[((size_t)region * kRegion + q) * nparts + i];
+        block_sum<NQ>(out);
+    }
+};
+
 __device__ inline bool breakdown(double x) {   // brkdn: zero or not finite
     if (x > 0.) return !(1. / x > 0.);
     if (x < 0.) return !(1. / x < 0.);
@@ -153,75 +181,118 @@ static size_t staged_lds(int brows, int mx) {
     return (size_t)(2 * brows + mx) * sizeof(V) + (size_t)(mx + 2 * brows + 1) * sizeof(int);
 }
 
-template <typename V, bool STAGE>
-__device__ inline BlockView<V> load_block(const SolvArgs<V> &a, int b0, int nb, unsigned char *lds) {
-    if (!STAGE) return BlockView<V>{a.brp + b0, a.bdiag + b0, a.bcol, a.lu};
-    V *llu = reinterpret_cast<V *>(lds);
-    int *lrp = reinterpret_cast<int *>(llu + a.mx);
-    int *ldg = lrp + a.brows + 1;
-    int *lcol = ldg + a.brows;
-    const int q0 = a.brp[b0], ne = a.brp[b0 + nb] - q0;
-    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
-        llu[e] = a.lu[q0 + e];
-        lcol[e] = a.bcol[q0 + e];
-    }
-    for (int li = threadIdx.x; li <= nb; li += blockDim.x) {
-        lrp[li] = a.brp[b0 + li] - q0;
-        if (li < nb) ldg[li] = a.bdiag[b0 + li] - q0;
-    }
-    return BlockView<V>{lrp, ldg, lcol, llu};   // visible after the caller's barrier
-}
+// The block's factors, columns, row starts and diagonals at the block's
+// fixed stride: load() issues the global loads into registers (up to
+// kStageRegs entries per work-item; the rest are copied by store()), store()
+// writes them to LDS and returns the view (visible after the caller's
+// barrier).  Without STAGE the view points at global memory.
+constexpr int kStageRegs = 8;
 
-// y (LDS, local rows of block blk) <- (LU)^-1 y.  Exact: level-scheduled
-// forward and backward substitution in lusol's order.  With a.sweeps = k > 0
-// each triangular factor is applied by k Jacobi sweeps instead
-// (x <- b - L x, then x <- D^-1 (y - U x)), double-buffered in LDS so the
-// result does not depend on the order in which wavefronts run.
+template <typename V, bool STAGE>
+struct BlockStage {
+    V lu[kStageRegs];
+    int col[kStageRegs];
+    int rp = 0, dg = 0, rpe = 0;
+    __device__ inline void load(const SolvArgs<V> &a, int blk) {
+        if (!STAGE) return;
+        const size_t e0 = (size_t)blk * a.mx;
+#pragma unroll
+        for (int k = 0; k < kStageRegs; k++) {
+            const int e = threadIdx.x + k * kThreads;
+            if (e < a.mx) {
+                lu[k] = a.lu[e0 + e];
+                col[k] = a.pcol[e0 + e];
+            }
+        }
+        const size_t r0 = (size_t)blk * (a.brows + 1);
+        if ((int)threadIdx.x < a.brows) {
+            rp = a.prp[r0 + threadIdx.x];
+            dg = a.pdg[(size_t)blk * a.brows + threadIdx.x];
+        }
+        if (threadIdx.x == 0) rpe = a.prp[r0 + a.brows];
+    }
+    __device__ inline BlockView<V> store(const SolvArgs<V> &a, int blk, unsigned char *lds) const {
+        const size_t e0 = (size_t)blk * a.mx, r0 = (size_t)blk * (a.brows + 1), d0 = (size_t)blk * a.brows;
+        if (!STAGE) return BlockView<V>{a.prp + r0, a.pdg + d0, a.pcol + e0, a.lu + e0};
+        V *llu = reinterpret_cast<V *>(lds);
+        int *lrp = reinterpret_cast<int *>(llu + a.mx);
+        int *ldg = lrp + a.brows + 1;
+        int *lcol = ldg + a.brows;
+#pragma unroll
+        for (int k = 0; k < kStageRegs; k++) {
+            const int e = threadIdx.x + k * kThreads;
+            if (e < a.mx) {
+                llu[e] = lu[k];
+                lcol[e] = col[k];
+            }
+        }
+        for (int e = threadIdx.x + kStageRegs * kThreads; e < a.mx; e += kThreads) {
+            llu[e] = a.lu[e0 + e];
+            lcol[e] = a.pcol[e0 + e];
+        }
+        if ((int)threadIdx.x < a.brows) {
+            lrp[threadIdx.x] = rp;
+            ldg[threadIdx.x] = dg;
+        }
+        for (int li = threadIdx.x + kThreads; li < a.brows; li += kThreads) {
+            lrp[li] = a.prp[r0 + li];
+            ldg[li] = a.pdg[d0 + li];
+        }
+        if (threadIdx.x == 0) lrp[a.brows] = rpe;
+        return BlockView<V>{lrp, ldg, lcol, llu};
+    }
+};
+
+// (LU)^-1 y for the local rows of block blk; y and w are LDS buffers of
+// the block's rows, y holds the right-hand side (visible: the caller's
+// barrier).  Returns the buffer that holds the result.
+//   Exact: level-scheduled forward and backward substitution in lusol's
+//   order, in place in y (rows of a level written by any work-item, a
+//   barrier after each level).
+//   a.sweeps = k > 0: each triangular factor applied by k Jacobi sweeps,
+//   x <- y - L x from x = y, then x <- D^-1 (f - U x) from x = D^-1 f,
+//   alternating between y and w (one barrier per sweep) with the
+//   right-hand sides in registers; the result does not depend on the
+//   order in which wavefronts run, and each work-item reads back only the
+//   rows it wrote itself.
 template <typename V>
-__device__ inline void block_lusol(const SolvArgs<V> &a, const BlockView<V> &B, int blk, int b0, int nb, V *y,
-                                   V *w) {
+__device__ inline V *block_lusol(const SolvArgs<V> &a, const BlockView<V> &B, int blk, int b0, int nb, V *y,
+                                 V *w) {
     if (a.sweeps > 0) {
         constexpr int RPT = kMaxBlockRows / kThreads;   // rows per work-item
-        V bl[RPT], nv[RPT];
-        // forward: (I + L) x = y, starting from x = y
+        V bl[RPT];
+        V *src = y, *dst = w;
         int nr = 0;
-        for (int li = threadIdx.x; li < nb; li += blockDim.x, nr++) {
-            bl[nr] = y[li];
-            w[li] = y[li];
-        }
-        __syncthreads();
+        for (int li = threadIdx.x; li < nb; li += blockDim.x, nr++) bl[nr] = y[li];
         for (int sw = 1; sw < a.sweeps; sw++) {
             nr = 0;
             for (int li = threadIdx.x; li < nb; li += blockDim.x, nr++) {
                 V acc = bl[nr];
-                for (int q = B.rp[li]; q < B.dg[li]; q++) acc = acc - B.lu[q] * w[B.col[q]];
-                nv[nr] = acc;
+                for (int q = B.rp[li]; q < B.dg[li]; q++) acc = acc - B.lu[q] * src[B.col[q]];
+                dst[li] = acc;
             }
             __syncthreads();
-            nr = 0;
-            for (int li = threadIdx.x; li < nb; li += blockDim.x, nr++) w[li] = nv[nr];
-            __syncthreads();
+            V *t = src; src = dst; dst = t;
         }
-        // backward: (D + U) x = w, starting from x = D^-1 w
+        // backward, right-hand side f = the forward result (own rows)
         nr = 0;
         for (int li = threadIdx.x; li < nb; li += blockDim.x, nr++) {
-            bl[nr] = w[li];
-            y[li] = B.lu[B.dg[li]] * w[li];
+            bl[nr] = src[li];
+            dst[li] = B.lu[B.dg[li]] * bl[nr];
         }
         __syncthreads();
+        { V *t = src; src = dst; dst = t; }
         for (int sw = 1; sw < a.sweeps; sw++) {
             nr = 0;
             for (int li = threadIdx.x; li < nb; li += blockDim.x, nr++) {
                 V acc = bl[nr];
-                for (int q = B.dg[li] + 1; q < B.rp[li + 1]; q++) acc = acc - B.lu[q] * y[B.col[q]];
-                nv[nr] = B.lu[B.dg[li]] * acc;
+                for (int q = B.dg[li] + 1; q < B.rp[li + 1]; q++) acc = acc - B.lu[q] * src[B.col[q]];
+                dst[li] = B.lu[B.dg[li]] * acc;
             }
             __syncthreads();
-            nr = 0;
-            for (int li = threadIdx.x; li < nb; li += blockDim.x, nr++) y[li] = nv[nr];
-            __syncthreads();
+            V *t = src; src = dst; dst = t;
         }
-        return;
+        return src;
     }
     const int stride = a.brows + 1;
     const int *lp = a.flptr + (size_t)blk * stride;
@@ -246,26 +317,42 @@ __device__ inline void block_lusol(const SolvArgs<V> &a, const BlockView<V> &B, 
         }
         __syncthreads();
     }
+    return y;
 }
 
-// ILU(0) of each diagonal block, level by level (the pivot is stored
-// inverted).  Row i, entries in column order:
+// ILU(0) of each diagonal block (the pivot is stored inverted).  Row i,
+// entries in column order:
 //   l_ij = (a_ij - sum_{k<j} l_ik u_kj) * (1/u_jj)   (j < i)
 //   u_ij =  a_ij - sum_{k<i} l_ik u_kj               (j >= i)
 // with the (k ascending) update pairs of entry q in cp_a/cp_b[cp_ptr[q]..].
-// STAGE: the block's update lists are copied to LDS as well.
+// Entries are computed level by level of the entry dependence graph (an
+// entry's level is one more than that of every entry it reads), one
+// work-item per entry, a barrier per level; each entry is the same
+// expression in the same order as in a row-by-row factorisation.  With
+// sched the level lists are copied to LDS; STAGE copies the update lists.
 template <typename V, bool STAGE>
-__global__ __launch_bounds__(kThreads) void k_bilu_factor(SolvArgs<V> a) {
+__global__ __launch_bounds__(kThreads) void k_bilu_factor(SolvArgs<V> a, int sched) {
     extern __shared__ double smem[];
     V *val = reinterpret_cast<V *>(smem);
     const int blk = blockIdx.x, b0 = blk * a.brows;
     const int nb = min(a.n, b0 + a.brows) - b0;
     const int q0 = a.brp[b0], q1 = a.brp[b0 + nb], ne = q1 - q0;
     const int p0 = a.cp_ptr[q0], np = a.cp_ptr[q1] - p0;
+    const int nl = a.enlev[blk];
+    const int *elp = a.elp + (size_t)blk * (a.edmax + 1);   // level starts
+    const int *elr = a.elr + (size_t)blk * a.mx;             // entries by level (relative)
+    int *lfree = reinterpret_cast<int *>(val + a.mx);
+    if (sched) {
+        int *l_p = lfree, *l_r = l_p + a.edmax + 1;
+        for (int t = threadIdx.x; t <= nl; t += blockDim.x) l_p[t] = elp[t];
+        for (int t = threadIdx.x; t < ne; t += blockDim.x) l_r[t] = elr[t];
+        elp = l_p; elr = l_r;
+        lfree = l_r + a.mx;
+    }
     const int *cpp = a.cp_ptr + q0, *cpa = a.cp_a + p0, *cpb = a.cp_b + p0, *jdv = a.bjd + q0;
-    int qoff = q0, poff = p0;       // entry / pair index bases of the lists in use
+    int poff = p0;                  // pair index base of the lists in use
     if (STAGE) {
-        int *lp = reinterpret_cast<int *>(val + a.mx);
+        int *lp = lfree;
         int *la = lp + a.mx + 1, *lb = la + a.mxp, *lj = lb + a.mxp;
         for (int e = threadIdx.x; e <= ne; e += blockDim.x) {
             lp[e] = a.cp_ptr[q0 + e];
@@ -279,59 +366,64 @@ __global__ __launch_bounds__(kThreads) void k_bilu_factor(SolvArgs<V> a) {
     }
     for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) val[q - q0] = (V)a.aval[a.bsrc[q]];
     __syncthreads();
-    const int stride = a.brows + 1;
-    const int *lvp = a.flptr + (size_t)blk * stride;
-    const int nf = a.fnlev[blk];
-    for (int lv = 0; lv < nf; lv++) {
-        for (int t = lvp[lv] + threadIdx.x; t < lvp[lv + 1]; t += blockDim.x) {
-            const int li = a.frows[b0 + t], i = b0 + li;
-            const int r0 = a.brp[i] - q0, r1 = a.brp[i + 1] - q0, dg = a.bdiag[i] - q0;
-            for (int e = r0; e < r1; e++) {
-                V acc = val[e];
-                for (int p = cpp[e] - poff; p < cpp[e + 1] - poff; p++)
-                    acc = acc - val[cpa[p] - qoff] * val[cpb[p] - qoff];
-                const int jd = jdv[e];
-                val[e] = jd >= 0 ? acc * val[jd - qoff] : acc;
+    for (int lv = 0; lv < nl; lv++) {
+        for (int t = elp[lv] + threadIdx.x; t < elp[lv + 1]; t += blockDim.x) {
+            const int e = elr[t];
+            V acc = val[e];
+            for (int p = cpp[e] - poff; p < cpp[e + 1] - poff; p++)
+                acc = acc - val[cpa[p] - q0] * val[cpb[p] - q0];
+            const int jd = jdv[e];
+            if (jd >= 0) {
+                val[e] = acc * val[jd - q0];
+            } else if (jd == -1) {
+                val[e] = acc;
+            } else {
+                if (acc == (V)0) atomicCAS(a.ctrl + CT_FACTOR, 0, 1 + b0 + (-2 - jd));
+                val[e] = (V)1 / acc;
             }
-            const V piv = val[dg];
-            if (piv == (V)0) atomicCAS(a.ctrl + CT_FACTOR, 0, 1 + i);
-            val[dg] = (V)1 / piv;
         }
         __syncthreads();
     }
-    for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) a.lu[q] = val[q - q0];
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) a.lu[(size_t)blk * a.mx + e] = val[e];
 }
 
 // The operands of one matrix row -- values and NV gathered vectors at the
-// row's columns, and the vectors at the row itself -- held in registers so
-// their loads are issued before a kernel's scalar reduction.  Rows longer
-// than kRowRegs read the rest from memory.
-constexpr int kRowRegs = 10;
-
+// row's columns, and the vectors at the row itself -- held in registers.
+// The row is read from its ELL slots (padding: column i, value 0), so every
+// load is issued at kernel start without waiting for the row's start; the
+// entries beyond kEllWidth are read from the CSR arrays where they are used.
 template <typename V, int NV>
 struct RowOps {
-    V a[kRowRegs];
-    V x[NV][kRowRegs];
+    V a[kEllWidth];
+    V x[NV][kEllWidth];
     V own[NV];
-    int q0 = 0, q1 = 0, m = 0;
+    int len = 0;
     template <typename... P>
     __device__ inline void load(const SolvArgs<V> &s, int i, bool valid, P... vec) {
         const V *vs[NV] = {vec...};
         if (!valid) return;
-        q0 = s.rowptr[i];
-        q1 = s.rowptr[i + 1];
-        m = min(q1 - q0, kRowRegs);
+        len = s.ell_len[i];
+        int j[kEllWidth];
 #pragma unroll
-        for (int k = 0; k < kRowRegs; k++) {
-            if (k < m) {
-                const int j = s.cols[q0 + k];
-                a[k] = s.A[q0 + k];
-#pragma unroll
-                for (int v = 0; v < NV; v++) x[v][k] = vs[v][j];
-            }
+        for (int k = 0; k < kEllWidth; k++) {
+            const size_t slot = (size_t)k * s.n + i;
+            j[k] = s.ell_col[slot];
+            a[k] = s.ell_a[slot];
         }
 #pragma unroll
         for (int v = 0; v < NV; v++) own[v] = vs[v][i];
+#pragma unroll
+        for (int k = 0; k < kEllWidth; k++)
+#pragma unroll
+            for (int v = 0; v < NV; v++) x[v][k] = vs[v][j[k]];
+    }
+    // CSR range of the entries beyond the ELL slots (empty for most rows)
+    __device__ inline void tail(const SolvArgs<V> &s, int i, int &q0, int &q1) const {
+        q0 = q1 = 0;
+        if (len > kEllWidth) {
+            q1 = s.rowptr[i + 1];
+            q0 = q1 - len + kEllWidth;
+        }
     }
 };
 
@@ -344,7 +436,9 @@ __global__ __launch_bounds__(kThreads) void k_bicg_init(SolvArgs<V> a, int zero_
     V *y = reinterpret_cast<V *>(smem), *w = y + a.brows;
     const int blk = blockIdx.x, b0 = blk * a.brows;
     const int nb = min(a.n, b0 + a.brows) - b0;
-    const BlockView<V> B = load_block<V, STAGE>(a, b0, nb, reinterpret_cast<unsigned char *>(w + a.brows));
+    BlockStage<V, STAGE> st;
+    st.load(a, blk);
+    const BlockView<V> B = st.store(a, blk, reinterpret_cast<unsigned char *>(w + a.brows));
     for (int li = threadIdx.x; li < nb; li += blockDim.x) {
         const int i = b0 + li;
         V t = 0;
@@ -353,11 +447,11 @@ __global__ __launch_bounds__(kThreads) void k_bicg_init(SolvArgs<V> a, int zero_
         y[li] = a.b[i] - t;
     }
     __syncthreads();
-    block_lusol(a, B, blk, b0, nb, y, w);
+    const V *z = block_lusol(a, B, blk, b0, nb, y, w);
     double loc[2] = {0., 0.};
     for (int li = threadIdx.x; li < nb; li += blockDim.x) {
         const int i = b0 + li;
-        const V r = y[li];
+        const V r = z[li];
         a.r0[i] = r;
         a.dx[i] = 0;
         loc[0] += (double)r * (double)r;
@@ -377,36 +471,52 @@ __global__ __launch_bounds__(kThreads) void k_bicg_init(SolvArgs<V> a, int zero_
 // fly), v = M^-1 A p and the partial (r0,v).  A stop decided by this kernel
 // (convergence, iteration limit) is stamped with it+1 in CT_STOPIT, so that
 // its own workgroups still finish the update of x; breakdowns are found by
-// every workgroup before any update.
+// every workgroup before any update.  With blocks of at most one row per
+// work-item, every load that depends on no other load (control words,
+// scalars, the block's factors, the row's operands, the partial sums) is
+// issued before the first test.
 template <typename V, bool STAGE>
 __global__ __launch_bounds__(kThreads) void k_bicg_a(SolvArgs<V> a, int it) {
-    const int stop = a.ctrl[CT_STOPIT];
-    if (a.ctrl[CT_FAIL] || a.ctrl[CT_REFCONV] || (stop != 0 && stop <= it)) return;
     extern __shared__ double smem[];
     V *y = reinterpret_cast<V *>(smem), *w = y + a.brows;
     const int par = it & 1, pp = par ^ 1;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     const int blk = blockIdx.x, b0 = blk * a.brows;
     const int nb = min(a.n, b0 + a.brows) - b0;
-    const V *pold_ = pp ? a.p1 : a.p0;     // direction of iteration it-1 (it = 0: unused)
-    const V *vold_ = pp ? a.v1 : a.v0;
-    const BlockView<V> B = load_block<V, STAGE>(a, b0, nb, reinterpret_cast<unsigned char *>(w + a.brows));
-    // operands of this work-item's row, loaded before the scalar reductions
-    RowOps<V, 4> ro;
+    const V *pold = pp ? a.p1 : a.p0;     // direction of iteration it-1 (it = 0: unused)
+    const V *vold = pp ? a.v1 : a.v0;
+    V *pnew = par ? a.p1 : a.p0;
+    V *vnew = par ? a.v1 : a.v0;
     const bool pre = a.brows <= kThreads;
+    const int i = b0 + threadIdx.x;
+    const bool mine = pre && (int)threadIdx.x < nb;
+    const int nq = it == 0 ? 2 : 5, region = it == 0 ? PR_C : PR_B;
+    // independent loads first
+    const int stop = a.ctrl[CT_STOPIT], fail = a.ctrl[CT_FAIL], refconv = a.ctrl[CT_REFCONV];
+    const double alpha_prev = a.scal[SC_ALPHA + pp], target_prev = a.scal[SC_TARGET], rho_prev = a.scal[SC_RHO + pp];
+    BlockStage<V, STAGE> st;
+    st.load(a, blk);
+    Partials<5> ps;
+    ps.load(a.part, region, a.nparts, a.nblk, nq);
+    RowOps<V, 4> ro;
+    V r0_i = 0, dx_i = 0;
     if (pre) {
-        if (it == 0) ro.load(a, b0 + threadIdx.x, threadIdx.x < nb, a.r0, a.r0, a.r0, a.r0);
-        else ro.load(a, b0 + threadIdx.x, threadIdx.x < nb, a.s, a.t, pold_, vold_);
+        if (it == 0) ro.load(a, i, mine, a.r0, a.r0, a.r0, a.r0);
+        else ro.load(a, i, mine, a.s, a.t, pold, vold);
+        if (mine) {
+            r0_i = a.r0[i];
+            if (it > 0) dx_i = a.dx[i];
+        }
     }
+    if (fail || refconv || (stop != 0 && stop <= it)) return;
+    const BlockView<V> B = st.store(a, blk, reinterpret_cast<unsigned char *>(w + a.brows));
+    double q[5];   // it = 0: (r0,r0) (r0,r0); else (t,t) (t,s) (s,s) (r0,s) (r0,t)
+    ps.total(a.part, region, a.nparts, a.nblk, nq, q);
     double rn, rho, omega = 0., alpha = 0.;
     if (it == 0) {
-        double rs[2];
-        sum_partials<2>(a.part, PR_C, a.nparts, a.nblk, rs);
-        rn = sqrt(rs[0]);
-        rho = rs[1];
+        rn = sqrt(q[0]);
+        rho = q[1];
     } else {
-        double q[5];   // (t,t) (t,s) (s,s) (r0,s) (r0,t)
-        sum_partials<5>(a.part, PR_B, a.nparts, a.nblk, q);
         if (breakdown(q[0])) {
             if (lead) a.ctrl[CT_FAIL] = -3;
             return;
@@ -416,15 +526,21 @@ __global__ __launch_bounds__(kThreads) void k_bicg_a(SolvArgs<V> a, int it) {
             if (lead) a.ctrl[CT_FAIL] = -3;
             return;
         }
-        alpha = a.scal[SC_ALPHA + pp];
+        alpha = alpha_prev;
         rn = sqrt(fmax(0., q[2] - 2. * omega * q[1] + omega * omega * q[0]));
         rho = q[3] - omega * q[4];
         const V va = (V)alpha, vo = (V)omega;
-        const V *pold = pp ? a.p1 : a.p0;
-        for (int li = threadIdx.x; li < nb; li += blockDim.x) {
-            const int i = b0 + li;
-            a.dx[i] = a.dx[i] + (va * pold[i] + vo * a.s[i]);
-            a.r[i] = a.s[i] - vo * a.t[i];
+        if (pre) {
+            if (mine) {
+                a.dx[i] = dx_i + (va * ro.own[2] + vo * ro.own[0]);
+                a.r[i] = ro.own[0] - vo * ro.own[1];
+            }
+        } else {
+            for (int li = threadIdx.x; li < nb; li += blockDim.x) {
+                const int k = b0 + li;
+                a.dx[k] = a.dx[k] + (va * pold[k] + vo * a.s[k]);
+                a.r[k] = a.s[k] - vo * a.t[k];
+            }
         }
         if (lead) { a.scal[SC_OMEGA + pp] = omega; a.ctrl[CT_ITS] += 1; }
     }
@@ -433,7 +549,7 @@ __global__ __launch_bounds__(kThreads) void k_bicg_a(SolvArgs<V> a, int it) {
         target = a.rtol * rn + a.atol;
         if (lead) { a.scal[SC_TARGET] = target; a.scal[SC_NORM0] = rn; }
     } else {
-        target = a.scal[SC_TARGET];
+        target = target_prev;
     }
     if (lead) { a.scal[SC_RNORM] = rn; a.ctrl[CT_NMV] = 1 + 2 * it; }
     if (!(rn > target)) {
@@ -450,34 +566,31 @@ __global__ __launch_bounds__(kThreads) void k_bicg_a(SolvArgs<V> a, int it) {
             if (lead) a.ctrl[CT_FAIL] = -3;
             return;
         }
-        beta = rho * alpha / (omega * a.scal[SC_RHO + pp]);
+        beta = rho * alpha / (omega * rho_prev);
     }
     if (lead) a.scal[SC_RHO + par] = rho;
-    const V *pold = pold_;
-    V *pnew = par ? a.p1 : a.p0;
-    const V *vold = vold_;
-    V *vnew = par ? a.v1 : a.v0;
     const V vb = (V)beta, vo = (V)omega;
     if (pre) {
-        if (threadIdx.x < nb) {
-            const int i = b0 + threadIdx.x;
+        if (mine) {
             V t = 0;
+            int q0, q1;
+            ro.tail(a, i, q0, q1);
             if (it == 0) {
 #pragma unroll
-                for (int k = 0; k < kRowRegs; k++)
-                    if (k < ro.m) t = t + ro.a[k] * ro.x[0][k];
-                for (int q = ro.q0 + ro.m; q < ro.q1; q++) t = t + a.A[q] * a.r0[a.cols[q]];
+                for (int k = 0; k < kEllWidth; k++)
+                    if (k < ro.len) t = t + ro.a[k] * ro.x[0][k];
+                for (int q = q0; q < q1; q++) t = t + a.A[q] * a.r0[a.cols[q]];
                 pnew[i] = ro.own[0];
             } else {
 #pragma unroll
-                for (int k = 0; k < kRowRegs; k++) {
-                    if (k < ro.m) {
+                for (int k = 0; k < kEllWidth; k++) {
+                    if (k < ro.len) {
                         const V rj = ro.x[0][k] - vo * ro.x[1][k];
                         const V pj = rj + vb * (ro.x[2][k] - vo * ro.x[3][k]);
                         t = t + ro.a[k] * pj;
                     }
                 }
-                for (int q = ro.q0 + ro.m; q < ro.q1; q++) {
+                for (int q = q0; q < q1; q++) {
                     const int j = a.cols[q];
                     const V rj = a.s[j] - vo * a.t[j];
                     const V pj = rj + vb * (pold[j] - vo * vold[j]);
@@ -489,45 +602,51 @@ __global__ __launch_bounds__(kThreads) void k_bicg_a(SolvArgs<V> a, int it) {
             y[threadIdx.x] = t;
         }
     } else if (it == 0) {
-        // p = r0 (set by init in p0 = p of parity 0's predecessor slot)
         for (int li = threadIdx.x; li < nb; li += blockDim.x) {
-            const int i = b0 + li;
+            const int k = b0 + li;
             V t = 0;
-            for (int q = a.rowptr[i]; q < a.rowptr[i + 1]; q++) t = t + a.A[q] * a.r0[a.cols[q]];
+            for (int q = a.rowptr[k]; q < a.rowptr[k + 1]; q++) t = t + a.A[q] * a.r0[a.cols[q]];
             y[li] = t;
-            pnew[i] = a.r0[i];
+            pnew[k] = a.r0[k];
         }
     } else {
         for (int li = threadIdx.x; li < nb; li += blockDim.x) {
-            const int i = b0 + li;
+            const int k = b0 + li;
             V t = 0;
-            for (int q = a.rowptr[i]; q < a.rowptr[i + 1]; q++) {
+            for (int q = a.rowptr[k]; q < a.rowptr[k + 1]; q++) {
                 const int j = a.cols[q];
                 const V rj = a.s[j] - vo * a.t[j];
                 const V pj = rj + vb * (pold[j] - vo * vold[j]);
                 t = t + a.A[q] * pj;
             }
             y[li] = t;
-            const V ri = a.s[i] - vo * a.t[i];
-            pnew[i] = ri + vb * (pold[i] - vo * vold[i]);
+            const V ri = a.s[k] - vo * a.t[k];
+            pnew[k] = ri + vb * (pold[k] - vo * vold[k]);
         }
     }
     __syncthreads();
-    block_lusol(a, B, blk, b0, nb, y, w);
+    const V *z = block_lusol(a, B, blk, b0, nb, y, w);
     double loc[1] = {0.};
-    for (int li = threadIdx.x; li < nb; li += blockDim.x) {
-        const int i = b0 + li;
-        vnew[i] = y[li];
-        loc[0] += (double)a.r0[i] * (double)y[li];
+    if (pre) {
+        if (mine) {
+            vnew[i] = z[threadIdx.x];
+            loc[0] = (double)r0_i * (double)z[threadIdx.x];
+        }
+    } else {
+        for (int li = threadIdx.x; li < nb; li += blockDim.x) {
+            const int k = b0 + li;
+            vnew[k] = z[li];
+            loc[0] += (double)a.r0[k] * (double)z[li];
+        }
     }
     store_partials<1>(loc, a.part, PR_A, a.nparts);
 }
 
 // Iteration it, second kernel: alpha = rho / (r0,v); s = r - alpha v
 // (fused); t = M^-1 A s; partials (t,t) (t,s) (s,s) (r0,s) (r0,t).
+// Independent loads are issued first, as in k_bicg_a.
 template <typename V, bool STAGE>
 __global__ __launch_bounds__(kThreads) void k_bicg_b(SolvArgs<V> a, int it) {
-    if (a.ctrl[CT_STOPIT] || a.ctrl[CT_FAIL] || a.ctrl[CT_REFCONV]) return;
     extern __shared__ double smem[];
     V *y = reinterpret_cast<V *>(smem), *w = y + a.brows;
     const int par = it & 1;
@@ -536,59 +655,86 @@ __global__ __launch_bounds__(kThreads) void k_bicg_b(SolvArgs<V> a, int it) {
     const V *rv = it == 0 ? a.r0 : a.r;
     const int blk = blockIdx.x, b0 = blk * a.brows;
     const int nb = min(a.n, b0 + a.brows) - b0;
-    const BlockView<V> B = load_block<V, STAGE>(a, b0, nb, reinterpret_cast<unsigned char *>(w + a.brows));
-    // operands of this work-item's row, loaded before the scalar reduction
-    RowOps<V, 2> ro;
     const bool pre = a.brows <= kThreads;
-    if (pre) ro.load(a, b0 + threadIdx.x, threadIdx.x < nb, rv, vnew);
+    const int i = b0 + threadIdx.x;
+    const bool mine = pre && (int)threadIdx.x < nb;
+    const int stop = a.ctrl[CT_STOPIT], fail = a.ctrl[CT_FAIL], refconv = a.ctrl[CT_REFCONV];
+    const double rho = a.scal[SC_RHO + par];
+    BlockStage<V, STAGE> st;
+    st.load(a, blk);
+    Partials<1> ps;
+    ps.load(a.part, PR_A, a.nparts, a.nblk, 1);
+    RowOps<V, 2> ro;
+    V r0_i = 0;
+    if (pre) {
+        ro.load(a, i, mine, rv, vnew);
+        if (mine) r0_i = a.r0[i];
+    }
+    if (stop || fail || refconv) return;
+    const BlockView<V> B = st.store(a, blk, reinterpret_cast<unsigned char *>(w + a.brows));
     double d[1];
-    sum_partials<1>(a.part, PR_A, a.nparts, a.nblk, d);
+    ps.total(a.part, PR_A, a.nparts, a.nblk, 1, d);
     if (breakdown(d[0])) {
         if (lead) a.ctrl[CT_FAIL] = -3;
         return;
     }
-    const double alpha = a.scal[SC_RHO + par] / d[0];
+    const double alpha = rho / d[0];
     if (lead) a.scal[SC_ALPHA + par] = alpha;
     const V va = (V)alpha;
+    V s_i = 0;
     if (pre) {
-        if (threadIdx.x < nb) {
-            const int i = b0 + threadIdx.x;
+        if (mine) {
             V t = 0;
+            int q0, q1;
+            ro.tail(a, i, q0, q1);
 #pragma unroll
-            for (int k = 0; k < kRowRegs; k++)
-                if (k < ro.m) t = t + ro.a[k] * (ro.x[0][k] - va * ro.x[1][k]);
-            for (int q = ro.q0 + ro.m; q < ro.q1; q++) {
+            for (int k = 0; k < kEllWidth; k++)
+                if (k < ro.len) t = t + ro.a[k] * (ro.x[0][k] - va * ro.x[1][k]);
+            for (int q = q0; q < q1; q++) {
                 const int j = a.cols[q];
                 t = t + a.A[q] * (rv[j] - va * vnew[j]);
             }
             y[threadIdx.x] = t;
-            a.s[i] = ro.own[0] - va * ro.own[1];
+            s_i = ro.own[0] - va * ro.own[1];
+            a.s[i] = s_i;
         }
     } else {
         for (int li = threadIdx.x; li < nb; li += blockDim.x) {
-            const int i = b0 + li;
+            const int k = b0 + li;
             V t = 0;
-            for (int q = a.rowptr[i]; q < a.rowptr[i + 1]; q++) {
+            for (int q = a.rowptr[k]; q < a.rowptr[k + 1]; q++) {
                 const int j = a.cols[q];
                 const V sj = rv[j] - va * vnew[j];
                 t = t + a.A[q] * sj;
             }
             y[li] = t;
-            a.s[i] = rv[i] - va * vnew[i];
+            a.s[k] = rv[k] - va * vnew[k];
         }
     }
     __syncthreads();
-    block_lusol(a, B, blk, b0, nb, y, w);
+    const V *z = block_lusol(a, B, blk, b0, nb, y, w);
     double loc[5] = {0., 0., 0., 0., 0.};
-    for (int li = threadIdx.x; li < nb; li += blockDim.x) {
-        const int i = b0 + li;
-        const double tv = (double)y[li], sv = (double)a.s[i], r0 = (double)a.r0[i];
-        a.t[i] = y[li];
-        loc[0] += tv * tv;
-        loc[1] += tv * sv;
-        loc[2] += sv * sv;
-        loc[3] += r0 * sv;
-        loc[4] += r0 * tv;
+    if (pre) {
+        if (mine) {
+            const double tv = (double)z[threadIdx.x], sv = (double)s_i, r0 = (double)r0_i;
+            a.t[i] = z[threadIdx.x];
+            loc[0] = tv * tv;
+            loc[1] = tv * sv;
+            loc[2] = sv * sv;
+            loc[3] = r0 * sv;
+            loc[4] = r0 * tv;
+        }
+    } else {
+        for (int li = threadIdx.x; li < nb; li += blockDim.x) {
+            const int k = b0 + li;
+            const double tv = (double)z[li], sv = (double)a.s[k], r0 = (double)a.r0[k];
+            a.t[k] = z[li];
+            loc[0] += tv * tv;
+            loc[1] += tv * sv;
+            loc[2] += sv * sv;
+            loc[3] += r0 * sv;
+            loc[4] += r0 * tv;
+        }
     }
     store_partials<5>(loc, a.part, PR_B, a.nparts);
 }
@@ -603,9 +749,16 @@ __global__ void k_bicg_finish(const V *x0, const V *dx, double *x, int n, int ad
     else x[i] = (double)(x0[i] + dx[i]);
 }
 
-__global__ void k_to_f32(const double *s, float *d, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) d[i] = (float)s[i];
+// the solve's copies of the assembled matrix: values in the ELL slots of
+// the iteration kernels (precision V), and for the FP32 path the FP32 CSR
+template <typename V>
+__global__ void k_prep_matrix(const double *aval, const int *csr_ell, V *ell_a, float *a32, int nnz) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nnz) return;
+    const double v = aval[q];
+    if (a32) a32[q] = (float)v;
+    const int slot = csr_ell[q];
+    if (slot >= 0) ell_a[slot] = (V)v;
 }
 
 // initial guess x = zn; clears the factorisation flag of this solve
@@ -677,21 +830,28 @@ static size_t apply_lds(const Model &M) {
 // wet/dry state changes rows of the zeta matrix from one step to the next
 // (dry-element rows, level boundaries), and factors of an earlier matrix
 // then no longer precondition it.
-static size_t factor_lds_staged(const Model &M, size_t vsize) {
-    return (size_t)M.sv.max_blk_nnz * vsize + (size_t)(2 * M.sv.max_blk_nnz + 1 + 2 * M.sv.max_blk_pairs) * sizeof(int);
+static size_t factor_lds_sched(const Model &M) {
+    return (size_t)(M.sv.max_elev + 1 + M.sv.max_blk_nnz) * sizeof(int);
 }
 
+static size_t factor_lds_lists(const Model &M) {
+    return (size_t)(2 * M.sv.max_blk_nnz + 1 + 2 * M.sv.max_blk_pairs) * sizeof(int);
+}
+
+// LDS of the factorisation: the values always; the row schedule and the
+// update lists when they fit (schedule first)
 template <typename V>
 static int factor(Model &M, SolvArgs<V> &a) {
-    const size_t staged = factor_lds_staged(M, sizeof(V));
-    timer_begin(M, T_FACTOR);
-    if (staged <= kLdsLimit) {
-        hipLaunchKernelGGL((k_bilu_factor<V, true>), dim3(M.sv.nblk), dim3(kThreads), staged, M.stream, a);
-    } else {
-        const size_t lds = (size_t)M.sv.max_blk_nnz * sizeof(V);
-        hipLaunchKernelGGL((k_bilu_factor<V, false>), dim3(M.sv.nblk), dim3(kThreads), lds, M.stream, a);
-    }
-    timer_end(M);
+    const size_t vals = (size_t)M.sv.max_blk_nnz * sizeof(V), sch = factor_lds_sched(M), lst = factor_lds_lists(M);
+    const dim3 g(M.sv.nblk), b(kThreads);
+    if (vals + sch + lst <= kLdsLimit)
+        launch_timed(M, T_FACTOR, k_bilu_factor<V, true>, g, b, vals + sch + lst, M.stream, a, 1);
+    else if (vals + lst <= kLdsLimit)
+        launch_timed(M, T_FACTOR, k_bilu_factor<V, true>, g, b, vals + lst, M.stream, a, 0);
+    else if (vals + sch <= kLdsLimit)
+        launch_timed(M, T_FACTOR, k_bilu_factor<V, false>, g, b, vals + sch, M.stream, a, 1);
+    else
+        launch_timed(M, T_FACTOR, k_bilu_factor<V, false>, g, b, vals, M.stream, a, 0);
     HIPCHK(M, hipGetLastError());
     return 0;
 }
@@ -712,12 +872,8 @@ static int bicgstab_t(Model &M, SolvArgs<V> &a, int *iters, int slot, int zero_x
     int chunk = std::max(2, M.sv.last_its[slot] + 1);
     for (;;) {
         for (int q = 0; q < chunk; q++, it++) {
-            timer_begin(M, T_BICG_A);
-            hipLaunchKernelGGL((k_bicg_a<V, STAGE>), dim3(g), dim3(kThreads), lds, M.stream, a, it);
-            timer_end(M);
-            timer_begin(M, T_BICG_B);
-            hipLaunchKernelGGL((k_bicg_b<V, STAGE>), dim3(g), dim3(kThreads), lds, M.stream, a, it);
-            timer_end(M);
+            launch_timed(M, T_BICG_A, k_bicg_a<V, STAGE>, dim3(g), dim3(kThreads), lds, M.stream, a, it);
+            launch_timed(M, T_BICG_B, k_bicg_b<V, STAGE>, dim3(g), dim3(kThreads), lds, M.stream, a, it);
         }
         HIPCHK(M, hipGetLastError());
         if (read_ctrl(M, c, CT_WORDS)) return 1;
@@ -756,12 +912,8 @@ static int enqueue_inner_t(Model &M, SolvArgs<V> &a, int chunk) {
     const size_t lds = apply_lds<V>(M);
     hipLaunchKernelGGL((k_bicg_init<V, STAGE>), dim3(g), dim3(kThreads), lds, M.stream, a, 1);
     for (int it = 0; it < chunk; it++) {
-        timer_begin(M, T_BICG_A);
-        hipLaunchKernelGGL((k_bicg_a<V, STAGE>), dim3(g), dim3(kThreads), lds, M.stream, a, it);
-        timer_end(M);
-        timer_begin(M, T_BICG_B);
-        hipLaunchKernelGGL((k_bicg_b<V, STAGE>), dim3(g), dim3(kThreads), lds, M.stream, a, it);
-        timer_end(M);
+        launch_timed(M, T_BICG_A, k_bicg_a<V, STAGE>, dim3(g), dim3(kThreads), lds, M.stream, a, it);
+        launch_timed(M, T_BICG_B, k_bicg_b<V, STAGE>, dim3(g), dim3(kThreads), lds, M.stream, a, it);
     }
     HIPCHK(M, hipGetLastError());
     return 0;
@@ -783,6 +935,9 @@ int solver_solve(Model &M, int *iters, int *refinements) {
         // x0 vector is the FP64 solution array itself
         hipLaunchKernelGGL(k_guess<double>, dim3(g1), dim3(kThreads), 0, M.stream, (const double *)M.s.zn, M.s.x, n,
                            M.sv.ctrl);
+        hipLaunchKernelGGL(k_prep_matrix<double>, dim3((M.nnz + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                           M.stream, (const double *)M.s.aval, (const int *)M.m.csr_ell, (double *)M.sv.ell_a,
+                           (float *)nullptr, M.nnz);
         auto a = solv_args<double>(M, M.s.aval, M.s.rhs, M.s.x);
         if (factor<double>(M, a)) return 1;
         int its = 0;
@@ -802,8 +957,9 @@ int solver_solve(Model &M, int *iters, int *refinements) {
         hipLaunchKernelGGL(k_guess<float>, dim3(g1), dim3(kThreads), 0, M.stream, (const float *)M.s.zn, M.s.x, n,
                            M.sv.ctrl);
         float *a32 = (float *)M.sv.a32;
-        hipLaunchKernelGGL(k_to_f32, dim3((M.nnz + kThreads - 1) / kThreads), dim3(kThreads), 0, M.stream,
-                           (const double *)M.s.aval, a32, M.nnz);
+        hipLaunchKernelGGL(k_prep_matrix<float>, dim3((M.nnz + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                           M.stream, (const double *)M.s.aval, (const int *)M.m.csr_ell, (float *)M.sv.ell_a, a32,
+                           M.nnz);
         float *b32 = (float *)M.sv.b, *x0 = (float *)M.sv.x0;
         auto a = solv_args<float>(M, a32, b32, x0);
         if (factor<float>(M, a)) return 1;

@@ -50,13 +50,14 @@ class Request:
 def footprint_bytes(basin, precision: str, ntracer: int = 0) -> int:
     """Estimated device memory of one resident run: about 30 element-layer
     and 20 node-layer state arrays, the zeta matrix with its block factors
-    and 12 Krylov vectors, plus the connectivity and geometry tables."""
+    (CSR and fixed-stride copies), its ELL slices and 12 Krylov vectors, plus
+    the connectivity and geometry tables."""
     p = 8 if precision == "fp64" else 4
     EL = basin.nel * basin.nlv
     NL = basin.nkn * (basin.nlv + 1)
     nnz = 7 * basin.nkn
-    return int(30 * EL * p + (20 + 2 * ntracer) * NL * p + nnz * (8 + 4 + 2 * p + 16) + 12 * basin.nkn * 8
-               + basin.nel * (12 + 56 + 6 * 12 + 18 * 12))
+    return int(30 * EL * p + (20 + 2 * ntracer) * NL * p + nnz * (8 + 4 + 3 * p + 24) + 12 * basin.nkn * 8
+               + 10 * basin.nkn * (8 + p) + basin.nel * (12 + 56 + 6 * 12 + 18 * 12))
 
 
 def forcing(req: Request, n: int, dt: float):
