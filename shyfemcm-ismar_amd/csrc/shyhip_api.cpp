@@ -229,7 +229,7 @@ int shyhip_create(shyhip_model **out, int precision, int nkn, int nel, int nlvdi
     p.grav = 9.81f; p.rowass = 1025.f; p.vismol = 1.0e-6f; p.ireib = 0; p.uvmin = 0.2f;
     p.hzmin = 0.01f; p.hzoff = 0.05f; p.hzon = 0.10f; p.volmin = 1.f; p.iclin = 0;
     p.rtol = (double)1.0e-8f; p.atol = (double)1.0e-8f; p.itermax = 1000; p.refine_rtol = 1e-10;
-    p.block_rows = 0; p.tri_sweeps = 0; p.inner_rtol = 1e-6; p.rkpar = 0.f; p.difmol = 1.0e-6f; p.itvd = 0; p.itvdv = 0;
+    p.block_rows = 0; p.tri_sweeps = 0; p.inner_rtol = 1e-5; p.rkpar = 0.f; p.difmol = 1.0e-6f; p.itvd = 0; p.itvdv = 0;
     const size_t ne = nel, nk = nkn, L = nlvdi;
     State &s = M.s;
     int rc = 0;
@@ -273,7 +273,7 @@ int shyhip_destroy(shyhip_model *h) {
                     s.wln, s.wlo, s.hdkn, s.hdko, s.hden, s.hdeo, s.rfric, s.ddx, s.ddy, s.visv, s.difv, s.difhv,
                     s.taux, s.tauy, s.rzv, s.rqv, s.mflux, s.hbuf})
         dfree(q);
-    dfree(s.iweg); dfree(s.iwet); dfree(s.inod); dfree(s.zel); dfree(s.aval); dfree(s.rhs); dfree(s.x);
+    dfree(s.iweg); dfree(s.iwet); dfree(s.inod); dfree(s.zel); dfree(s.aval); dfree(s.zrow); dfree(s.rhs); dfree(s.x);
     for (void *t : s.tracer) dfree(t);
     for (void *t : s.tracer_bc) dfree(t);
     dfree(M.m.bndo_k); dfree(M.m.bndo_ptr); dfree(M.m.bndo_nb); dfree(M.m.bndo_nx); dfree(M.m.bndo_ny);
@@ -282,6 +282,7 @@ int shyhip_destroy(shyhip_model *h) {
     for (void *q : {(void *)m.nen3v, (void *)m.geo, (void *)m.ang, (void *)m.dis, (void *)m.ilhv, (void *)m.ilhkv,
                     m.hm3v, m.hev, m.hlv, m.hldv, m.areakv, (void *)m.ibnd, (void *)m.zbound, (void *)m.iextb,
                     (void *)m.ibtyp, m.fcorv, m.czv, (void *)m.nk_ptr, (void *)m.nk_el, (void *)m.nkc_el,
+                    (void *)m.inc_cnt, (void *)m.inc_el, (void *)m.inc_elc,
                     (void *)m.rowptr, (void *)m.cols, (void *)m.ijp, (void *)m.bnd_nodes, (void *)m.bnd_id,
                     (void *)m.blk_rowptr, (void *)m.blk_cols, (void *)m.blk_src, (void *)m.blk_diag, (void *)m.blk_cp_ptr,
                     (void *)m.blk_cp_a, (void *)m.blk_cp_b, (void *)m.blk_jd, (void *)m.blk_pcol, (void *)m.blk_prp,
@@ -453,12 +454,24 @@ int shyhip_set_basin(shyhip_model *h, const int *nen3v, const float *xgv, const 
         rc |= dalloc(M, &m.nk_ptr, nkn + 1) || upload(M, m.nk_ptr, ptr.data(), ptr.size());
         rc |= dalloc(M, &m.nk_el, el.size()) || upload(M, m.nk_el, el.data(), el.size());
         rc |= dalloc(M, &m.nkc_el, elc.size()) || upload(M, m.nkc_el, elc.data(), elc.size());
+        std::vector<int> icnt(nkn), iel((size_t)kIncWidth * nkn, 0), ielc((size_t)kIncWidth * nkn, 0);
+        for (int k = 0; k < nkn; k++) {
+            icnt[k] = ptr[k + 1] - ptr[k];
+            for (int s = 0; s < kIncWidth && s < icnt[k]; s++) {
+                iel[(size_t)s * nkn + k] = el[ptr[k] + s];
+                ielc[(size_t)s * nkn + k] = elc[ptr[k] + s];
+            }
+        }
+        rc |= dalloc(M, &m.inc_cnt, icnt.size()) || upload(M, m.inc_cnt, icnt.data(), icnt.size());
+        rc |= dalloc(M, &m.inc_el, iel.size()) || upload(M, m.inc_el, iel.data(), iel.size());
+        rc |= dalloc(M, &m.inc_elc, ielc.size()) || upload(M, m.inc_elc, ielc.data(), ielc.size());
     }
     // CSR pattern
     rc |= dalloc(M, &m.rowptr, nkn + 1) || upload(M, m.rowptr, M.h_rowptr.data(), nkn + 1);
     rc |= dalloc(M, &m.cols, M.nnz) || upload(M, m.cols, M.h_cols.data(), M.nnz);
     rc |= dalloc(M, &m.ijp, M.h_ijp.size()) || upload(M, m.ijp, M.h_ijp.data(), M.h_ijp.size());
     rc |= dalloc(M, &M.s.aval, M.nnz);
+    rc |= dalloc_s(M, &M.s.zrow, (size_t)12 * nel);
     // block-ILU(0) schedule
     {
         int brows = M.p.block_rows > 0 ? M.p.block_rows : std::min(nkn, 256);

@@ -35,6 +35,7 @@ struct ConzArgs {
     const int *nen3v, *ilhv, *ilhkv, *iextb, *ibnd;
     const double *geo, *dis;
     const int *nkc_ptr, *nkc_el;
+    const int *inc_cnt, *inc_elc;    // incidence slots (colour order)
     const int *tvd_el, *tvd_kn;
     const double *tvd_w;
     const S *utn, *vtn, *uto, *vto, *uln, *vln;
@@ -247,6 +248,8 @@ __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
         if (k >= nkn) continue;
         const int lmaxk = a.ilhkv[k];
         if (l > lmaxk) continue;
+        IncSlots inc;     // incident elements in colour order
+        inc.load(a.inc_cnt, a.inc_elc, nkn, k);
         // node column around l (zero outside 1..lmaxk)
         double ckw[5], hokw[5], hnkw[5];
         for (int j = 0; j < 5; j++) {
@@ -304,20 +307,34 @@ __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
         // incident elements in colour order (conz3d_element's scatter)
         const double hoc = hokw[2], hnc = hnkw[2], clc_l = ckw[2];
         double cn = 0., cd = 0., clo = 0., chi = 0.;
-        for (int q = a.nkc_ptr[k]; q < a.nkc_ptr[k + 1]; q++) {
-            const int pk = a.nkc_el[q];
-            const int ie = pk >> 2, ii = pk & 3;
-            const int ilev = a.ilhv[ie];
-            if (ilev < l) continue;
+        auto add = [&](int ilev, double aj4, double hf) {
+            if (ilev < l) return;
             const int v = min(ilev - l, 2);
-            const double aj4 = 4. * a.geo[(size_t)6 * nel + ie];
-            const double hf = (double)a.hbuf[((size_t)(l - 1) * 3 + ii) * nel + ie];
             const double cexpl = aj4 * (hoc * clc_l + dt * (0. + hf - fwv[v] - fdv[v]));
             cn += cexpl;
             clo += aj4 * dt * clmv[v];
             chi += aj4 * dt * clpv[v];
             cd += aj4 * dt * clcv[v] + aj4 * (1. + dt * 0.) * hnc;
+        };
+        int slv[kIncWidth];
+        double saj[kIncWidth], shf[kIncWidth];
+#pragma unroll
+        for (int s = 0; s < kIncWidth; s++) {
+            const int ie = inc.pk[s] >> 2, ii = inc.pk[s] & 3;
+            slv[s] = a.ilhv[ie];
+            saj[s] = 4. * a.geo[(size_t)6 * nel + ie];
+            shf[s] = (double)a.hbuf[((size_t)(l - 1) * 3 + ii) * nel + ie];
         }
+#pragma unroll
+        for (int s = 0; s < kIncWidth; s++)
+            if (s < inc.cnt) add(slv[s], saj[s], shf[s]);
+        if (inc.cnt > kIncWidth)
+            for (int q = a.nkc_ptr[k] + kIncWidth; q < a.nkc_ptr[k + 1]; q++) {
+                const int pk = a.nkc_el[q];
+                const int ie = pk >> 2, ii = pk & 3;
+                add(a.ilhv[ie], 4. * a.geo[(size_t)6 * nel + ie],
+                    (double)a.hbuf[((size_t)(l - 1) * 3 + ii) * nel + ie]);
+            }
         // conz3d_nodes: boundary mass flux
         if (a.mflux) {
             const size_t o = (size_t)(l - 1) * nkn + k;
@@ -366,7 +383,7 @@ static int conz_nl(Model &M, int itr, int istot, int isact, int clip, hipStream_
     a.az = M.p.azpar; a.ad = M.p.adpar; a.aa = M.p.aapar; a.difmol = M.p.difmol;
     a.nen3v = M.m.nen3v; a.ilhv = M.m.ilhv; a.ilhkv = M.m.ilhkv; a.iextb = M.m.iextb; a.ibnd = M.m.ibnd;
     a.geo = M.m.geo; a.dis = M.m.dis;
-    a.nkc_ptr = M.m.nk_ptr; a.nkc_el = M.m.nkc_el;
+    a.nkc_ptr = M.m.nk_ptr; a.nkc_el = M.m.nkc_el; a.inc_cnt = M.m.inc_cnt; a.inc_elc = M.m.inc_elc;
     a.tvd_el = M.m.tvd_el; a.tvd_kn = M.m.tvd_kn; a.tvd_w = (const double *)M.m.tvd_w;
     a.utn = (const S *)M.s.utn; a.vtn = (const S *)M.s.vtn; a.uto = (const S *)M.s.uto; a.vto = (const S *)M.s.vto;
     a.uln = (const S *)M.s.uln; a.vln = (const S *)M.s.vln;

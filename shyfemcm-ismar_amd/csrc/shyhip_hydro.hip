@@ -28,6 +28,7 @@ struct HydroArgs {
     const S *hm3v, *hev, *hlv, *hldv, *areakv, *fcorv, *czv;
     const int *ibnd, *ibtyp, *zbound;
     const int *nk_ptr, *nk_el;
+    const int *inc_cnt, *inc_el;     // incidence slots (element order)
     const int *rowptr, *ijp;
     S *zn, *zo, *zen, *zeo;
     S *utn, *vtn, *uto, *vto, *uln, *vln;
@@ -38,6 +39,7 @@ struct HydroArgs {
     S *rfric, *ddx, *ddy, *visv;
     S *taux, *tauy, *rzv, *rqv, *mflux;
     double *zel, *aval, *rhs, *x;
+    S *zrow;
     int *err;
 };
 
@@ -57,6 +59,7 @@ static HydroArgs<S> make_args(Model &M) {
     a.fcorv = (const S *)M.m.fcorv; a.czv = (const S *)M.m.czv;
     a.ibnd = M.m.ibnd; a.ibtyp = M.m.ibtyp; a.zbound = M.m.zbound;
     a.nk_ptr = M.m.nk_ptr; a.nk_el = M.m.nk_el; a.rowptr = M.m.rowptr; a.ijp = M.m.ijp;
+    a.inc_cnt = M.m.inc_cnt; a.inc_el = M.m.inc_el; a.zrow = (S *)M.s.zrow;
     a.zn = (S *)M.s.zn; a.zo = (S *)M.s.zo; a.zen = (S *)M.s.zen; a.zeo = (S *)M.s.zeo;
     a.utn = (S *)M.s.utn; a.vtn = (S *)M.s.vtn; a.uto = (S *)M.s.uto; a.vto = (S *)M.s.vto;
     a.uln = (S *)M.s.uln; a.vln = (S *)M.s.vln;
@@ -363,7 +366,7 @@ __global__ __launch_bounds__(kThreads) void k_hydro_intern(HydroArgs<S> a) {
 // element matrix of hydro_zeta (hydrodynamic.f90:600-728) for element ie,
 // returned row i (after level boundary conditions and the excluded-area rule)
 template <typename S>
-__device__ inline void zeta_row(const HydroArgs<S> &a, int ie, int i, S hia_row[3], S &hik_i) {
+__device__ inline void zeta_elem(const HydroArgs<S> &a, int ie, S hia[3][3], S hik[3]) {
     using R = S;
     const int nel = a.nel;
     const double dt = (double)(R)a.dt;
@@ -380,7 +383,6 @@ __device__ inline void zeta_row(const HydroArgs<S> &a, int ie, int i, S hia_row[
     }
     const double dbb = a.zel[ie], dbc = a.zel[(size_t)nel + ie], dcb = a.zel[(size_t)2 * nel + ie];
     const double dcc = a.zel[(size_t)3 * nel + ie], ut = a.zel[(size_t)4 * nel + ie], vt = a.zel[(size_t)5 * nel + ie];
-    R hia[3][3], hik[3];
     for (int n = 0; n < 3; n++) {
         for (int m = 0; m < 3; m++) {
             const double abn = b[n] * (b[m] * dbb + c[m] * dbc);
@@ -418,33 +420,82 @@ __device__ inline void zeta_row(const HydroArgs<S> &a, int ie, int i, S hia_row[
             }
         }
     }
-    for (int m = 0; m < 3; m++) hia_row[m] = hia[i][m];
-    hik_i = hik[i];
 }
 
-// system_init + hydro_zeta + system_assemble + system_add_rhs as a row
-// gather: row k sums its incident elements' rows in element order, the
-// order of the reference's scatter into c2coo / rvec2d.
+// hydro_zeta's element matrix and vector (all three rows) of every element
+template <typename S>
+__global__ __launch_bounds__(kThreads) void k_zeta_elem(HydroArgs<S> a) {
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ie >= a.nel) return;
+    S hia[3][3], hik[3];
+    zeta_elem(a, ie, hia, hik);
+    const size_t nel = a.nel;
+    for (int n = 0; n < 3; n++) {
+        for (int m = 0; m < 3; m++) a.zrow[(3 * n + m) * nel + ie] = hia[n][m];
+        a.zrow[(9 + n) * nel + ie] = hik[n];
+    }
+}
+
+// system_init + system_assemble + system_add_rhs as a row gather: row k
+// sums its incident elements' rows in element order, the order of the
+// reference's scatter into c2coo / rvec2d.  The row is accumulated in
+// registers by position in the row (all element loads issued first); rows
+// longer than kEllWidth or nodes with more than kIncWidth elements
+// accumulate in place.
 template <typename S>
 __global__ __launch_bounds__(kThreads) void k_zeta_rows(HydroArgs<S> a) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= a.nkn) return;
-    const int r0 = a.rowptr[k], r1 = a.rowptr[k + 1];
-    for (int q = r0; q < r1; q++) a.aval[q] = 0.;
-    double rhs = 0.;
-    for (int q = a.nk_ptr[k]; q < a.nk_ptr[k + 1]; q++) {
-        const int pk = a.nk_el[q];
-        const int ie = pk >> 2, i = pk & 3;
-        S row[3], hik;
-        zeta_row(a, ie, i, row, hik);
-        for (int j = 0; j < 3; j++) {
-            const int slot = a.ijp[9 * ie + 3 * i + j];
-            a.aval[slot] = a.aval[slot] + (double)row[j];
-        }
-        rhs = rhs + (double)hik;
-    }
+    const size_t nel = a.nel;
+    const int r0 = a.rowptr[k], len = a.rowptr[k + 1] - r0;
+    IncSlots in;
+    in.load(a.inc_cnt, a.inc_el, a.nkn, k);
     const S dt = (S)a.dt;
-    rhs = rhs + (double)(dt * a.rqv[k]);
+    const S rq = a.rqv[k];
+    double rhs = 0.;
+    if (len <= kEllWidth && in.cnt <= kIncWidth) {
+        S hv[kIncWidth][3], hk[kIncWidth];
+        int pos[kIncWidth][3];
+#pragma unroll
+        for (int s = 0; s < kIncWidth; s++) {
+            const int ie = in.pk[s] >> 2, i = in.pk[s] & 3;
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                hv[s][j] = a.zrow[(3 * i + j) * nel + ie];
+                pos[s][j] = a.ijp[9 * (size_t)ie + 3 * i + j] - r0;
+            }
+            hk[s] = a.zrow[(9 + i) * nel + ie];
+        }
+        double acc[kEllWidth];
+#pragma unroll
+        for (int t = 0; t < kEllWidth; t++) acc[t] = 0.;
+#pragma unroll
+        for (int s = 0; s < kIncWidth; s++) {
+            if (s < in.cnt) {
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+#pragma unroll
+                    for (int t = 0; t < kEllWidth; t++)
+                        if (t == pos[s][j]) acc[t] = acc[t] + (double)hv[s][j];
+                rhs = rhs + (double)hk[s];
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < kEllWidth; t++)
+            if (t < len) a.aval[r0 + t] = acc[t];
+    } else {
+        for (int q = r0; q < r0 + len; q++) a.aval[q] = 0.;
+        for (int q = a.nk_ptr[k]; q < a.nk_ptr[k + 1]; q++) {
+            const int pk = a.nk_el[q];
+            const int ie = pk >> 2, i = pk & 3;
+            for (int j = 0; j < 3; j++) {
+                const int slot = a.ijp[9 * (size_t)ie + 3 * i + j];
+                a.aval[slot] = a.aval[slot] + (double)a.zrow[(3 * i + j) * nel + ie];
+            }
+            rhs = rhs + (double)a.zrow[(9 + i) * nel + ie];
+        }
+    }
+    rhs = rhs + (double)(dt * rq);
     a.rhs[k] = rhs;
 }
 
@@ -521,12 +572,26 @@ template <typename S>
 __global__ __launch_bounds__(kThreads) void k_setnod(HydroArgs<S> a) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= a.nkn) return;
-    double win = 0.;
-    for (int q = a.nk_ptr[k]; q < a.nk_ptr[k + 1]; q++) {
-        const int pk = a.nk_el[q];
-        const int ie = pk >> 2, ii = pk & 3;
-        if (a.iweg[ie] == 0) win = win + a.ang[(size_t)ii * a.nel + ie];
+    IncSlots in;
+    in.load(a.inc_cnt, a.inc_el, a.nkn, k);
+    int wg[kIncWidth];
+    double an[kIncWidth];
+#pragma unroll
+    for (int s = 0; s < kIncWidth; s++) {
+        const int ie = in.pk[s] >> 2, ii = in.pk[s] & 3;
+        wg[s] = a.iweg[ie];
+        an[s] = a.ang[(size_t)ii * a.nel + ie];
     }
+    double win = 0.;
+#pragma unroll
+    for (int s = 0; s < kIncWidth; s++)
+        if (s < in.cnt && wg[s] == 0) win = win + an[s];
+    if (in.cnt > kIncWidth)
+        for (int q = a.nk_ptr[k] + kIncWidth; q < a.nk_ptr[k + 1]; q++) {
+            const int pk = a.nk_el[q];
+            const int ie = pk >> 2, ii = pk & 3;
+            if (a.iweg[ie] == 0) win = win + a.ang[(size_t)ii * a.nel + ie];
+        }
     int inod;
     if (win > 359.99) inod = 0;
     else if (win == 0.) inod = -2;
@@ -747,14 +812,10 @@ __global__ __launch_bounds__(kThreads) void k_setdepth_node(HydroArgs<S> a, cons
 #pragma unroll
     for (int l = 0; l < NL; l++) acc[l] = 0;
     bool bad = false;
-    for (int q = a.nk_ptr[k]; q < a.nk_ptr[k + 1]; q++) {
-        const int pk = a.nk_el[q];
-        const int ie = pk >> 2, ii = pk & 3;
-        const R areael = (R)(12. * a.geo[(size_t)6 * nel + ie]);
+    // the element terms, added in list order; the slots' loads are issued
+    // at once
+    auto add = [&](R areael, int lmax, R htot, R z) {
         const R areafv = areael / (R)3;
-        const int lmax = a.ilhv[ie];
-        const R htot = a.hm3v[(size_t)ii * nel + ie];
-        const R z = zen[(size_t)ii * nel + ie];
         if (lmax == 1) {
             acc[0] = acc[0] + areafv * (htot + z);
         } else {
@@ -767,7 +828,29 @@ __global__ __launch_bounds__(kThreads) void k_setdepth_node(HydroArgs<S> a, cons
                 if (l == lmax) acc[l - 1] = acc[l - 1] + areafv * hlast;
             }
         }
+    };
+    IncSlots in;
+    in.load(a.inc_cnt, a.inc_el, nkn, k);
+    R sarea[kIncWidth], shtot[kIncWidth], sz[kIncWidth];
+    int slmax[kIncWidth];
+#pragma unroll
+    for (int s = 0; s < kIncWidth; s++) {
+        const int ie = in.pk[s] >> 2, ii = in.pk[s] & 3;
+        sarea[s] = (R)(12. * a.geo[(size_t)6 * nel + ie]);
+        slmax[s] = a.ilhv[ie];
+        shtot[s] = a.hm3v[(size_t)ii * nel + ie];
+        sz[s] = zen[(size_t)ii * nel + ie];
     }
+#pragma unroll
+    for (int s = 0; s < kIncWidth; s++)
+        if (s < in.cnt) add(sarea[s], slmax[s], shtot[s], sz[s]);
+    if (in.cnt > kIncWidth)
+        for (int q = a.nk_ptr[k] + kIncWidth; q < a.nk_ptr[k + 1]; q++) {
+            const int pk = a.nk_el[q];
+            const int ie = pk >> 2, ii = pk & 3;
+            add((R)(12. * a.geo[(size_t)6 * nel + ie]), a.ilhv[ie], a.hm3v[(size_t)ii * nel + ie],
+                zen[(size_t)ii * nel + ie]);
+        }
     const int lmaxk = a.ilhkv[k];
     bool stop = false;
 #pragma unroll
@@ -807,10 +890,8 @@ __global__ __launch_bounds__(kThreads) void k_hydro_vertical(HydroArgs<S> a) {
         const int kk = idx % TN, l = idx / TN + 1, k = k0 + kk;
         if (k >= nkn || l > a.ilhkv[k]) continue;
         R vf = 0, va = 0;
-        for (int q = a.nk_ptr[k]; q < a.nk_ptr[k + 1]; q++) {
-            const int pk = a.nk_el[q];
-            const int ie = pk >> 2, ii = pk & 3;
-            if (a.ilhv[ie] < l) continue;
+        auto add = [&](int ie, int ii) {
+            if (a.ilhv[ie] < l) return;
             const R aj = (R)(4. * a.geo[(size_t)6 * nel + ie]);
             const R b = (R)a.geo[(size_t)ii * nel + ie], c = (R)a.geo[(size_t)(3 + ii) * nel + ie];
             const size_t o = (size_t)(l - 1) * nel + ie;
@@ -819,7 +900,35 @@ __global__ __launch_bounds__(kThreads) void k_hydro_vertical(HydroArgs<S> a) {
             const R ff = ffn * az + ffo * azt;
             vf = vf + (R)3 * aj * ff;
             va = va + aj;
+        };
+        IncSlots in;
+        in.load(a.inc_cnt, a.inc_el, nkn, k);
+        // slot operands loaded at once (layer l of every slot's element),
+        // then added in list order
+        int slv[kIncWidth];
+        R saj[kIncWidth], sb[kIncWidth], sc[kIncWidth], sun[kIncWidth], svn[kIncWidth], suo[kIncWidth], svo[kIncWidth];
+#pragma unroll
+        for (int s = 0; s < kIncWidth; s++) {
+            const int ie = in.pk[s] >> 2, ii = in.pk[s] & 3;
+            const size_t o = (size_t)(l - 1) * nel + ie;
+            slv[s] = a.ilhv[ie];
+            saj[s] = (R)(4. * a.geo[(size_t)6 * nel + ie]);
+            sb[s] = (R)a.geo[(size_t)ii * nel + ie];
+            sc[s] = (R)a.geo[(size_t)(3 + ii) * nel + ie];
+            sun[s] = a.utn[o]; svn[s] = a.vtn[o]; suo[s] = a.uto[o]; svo[s] = a.vto[o];
         }
+#pragma unroll
+        for (int s = 0; s < kIncWidth; s++) {
+            if (s < in.cnt && slv[s] >= l) {
+                const R ffn = sun[s] * sb[s] + svn[s] * sc[s];
+                const R ffo = suo[s] * sb[s] + svo[s] * sc[s];
+                const R ff = ffn * az + ffo * azt;
+                vf = vf + (R)3 * saj[s] * ff;
+                va = va + saj[s];
+            }
+        }
+        if (in.cnt > kIncWidth)
+            for (int q = a.nk_ptr[k] + kIncWidth; q < a.nk_ptr[k + 1]; q++) add(a.nk_el[q] >> 2, a.nk_el[q] & 3);
         vfL[l - 1][kk] = vf;
         vaL[l - 1][kk] = va;
     }
@@ -1019,6 +1128,7 @@ int launch_hydro_intern(Model &M) { return M.prec == SHYHIP_FP64 ? hydro_intern_
 template <typename S>
 static int zeta_t(Model &M) {
     auto a = make_args<S>(M);
+    LAUNCH(M, k_zeta_elem<S>, nblocks(M.nel), a);
     LAUNCH(M, k_zeta_rows<S>, nblocks(M.nkn), a);
     return 0;
 }

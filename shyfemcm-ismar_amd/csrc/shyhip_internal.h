@@ -23,6 +23,7 @@ constexpr double kDrittl = (double)(1.0f / 3.0f);    // default-real 1./3. in do
 constexpr int kThreads = 256;
 constexpr int kMaxBlockRows = 1024;                  // block-ILU(0) rows per workgroup (LDS)
 constexpr int kEllWidth = 10;                        // matrix entries per row held in ELL slots
+constexpr int kIncWidth = 10;                        // incident elements per node held in slots
 
 // error codes written by kernels into Model::d_err[0]
 enum DevErr { ERR_NONE = 0, ERR_SETNOD = 1, ERR_SETUVD = 3, ERR_SETDEPTH = 4, ERR_VOLUME = 5,
@@ -41,6 +42,10 @@ struct Mesh {                 // device copies of init-time data
     int *nk_ptr = nullptr;    // node -> incident (element<<2 | vertex), element order
     int *nk_el = nullptr;
     int *nkc_el = nullptr;    // same lists in colour order
+    // the first kIncWidth entries of both lists at [s*nkn + k] (padded with
+    // 0, a valid element), and the list lengths, so that a node's element
+    // loads can all be issued at once
+    int *inc_cnt = nullptr, *inc_el = nullptr, *inc_elc = nullptr;
     int *rowptr = nullptr, *cols = nullptr, *ijp = nullptr;   // zeta CSR; ijp[ie*9+i*3+j]
     int *bnd_nodes = nullptr; int *bnd_id = nullptr; int nbnd_nodes = 0;
     // block-ILU(0)
@@ -84,6 +89,7 @@ struct State {
     std::vector<void *> tracer_bc;   // boundary values per tracer (-999 = none)
     double *zel = nullptr;    // [6*nel] dbb dbc dcb dcc ut vt (hydro_zeta inputs)
     double *aval = nullptr;   // [nnz] assembled zeta matrix (FP64, as c2coo)
+    void *zrow = nullptr;     // S [12*nel] element matrices hia (9) and vectors hik (3)
     double *rhs = nullptr;    // [nkn] (as rvec2d)
     double *x = nullptr;      // [nkn] solution (FP64)
     void *hbuf = nullptr;     // conz3d horizontal terms [l][ii][ie]
@@ -148,6 +154,19 @@ struct Model {
     std::vector<int> tev_open;          // groups begun and not yet ended (nesting)
     double t_ms[T_NTIMERS] = {};
     long t_cnt[T_NTIMERS] = {};
+};
+
+// The incident elements of node k held in registers (element<<2 | vertex):
+// slots s < cnt hold list entries s; padding slots hold element 0.  Entries
+// from kIncWidth on are read from the CSR list (tail()).
+struct IncSlots {
+    int pk[kIncWidth];
+    int cnt = 0;
+    __device__ inline void load(const int *inc_cnt, const int *inc_el, int nkn, int k) {
+        cnt = inc_cnt[k];
+#pragma unroll
+        for (int s = 0; s < kIncWidth; s++) pk[s] = inc_el[(size_t)s * nkn + k];
+    }
 };
 
 // event marks around a launch group (no-op unless timing is enabled)
