@@ -92,6 +92,37 @@ def _kernel_bytes(b, p, brows):
     }
 
 
+# kernel names of the timed groups as rocprofv3 prints them (value type V)
+KERNEL_NAMES = {"bicg_a": "k_bicg_a<{v}, true>", "bicg_b": "k_bicg_b<{v}, true>",
+                "bilu_factor": "k_bilu_factor<{v}, true>", "conz3d_elem": "k_conz_elem<{v}>",
+                "conz3d_node": "k_conz_node<{v}, 12>", "hydro_vertical": "k_hydro_vertical<{v}, 12>",
+                "hydro_intern": "k_hydro_intern<{v}, 12, false>"}
+
+
+def _measured_traffic(path, group, vtype):
+    """HBM bytes per launch of a timed group's kernel from the committed
+    FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or None."""
+    from shyhip import profsum
+    t = profsum.read_traffic(path)
+    if not t or group not in KERNEL_NAMES:
+        return None
+    k = t["kernels"].get(KERNEL_NAMES[group].format(v=vtype))
+    return None if k is None else k["traffic_bytes"]
+
+
+def _copy_calibration(mib):
+    """One plain device-to-device copy of `mib` MiB (its counters calibrate
+    FETCH_SIZE / WRITE_SIZE against a known byte count)."""
+    import torch
+    n = mib * 1024 * 1024 // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
 def run_gpu(b, tr, prec, args):
     import shyhip
     from shyhip import mesh
@@ -239,6 +270,10 @@ def main():
     ap.add_argument("--srv-requests", type=int, default=12)
     ap.add_argument("--srv-max-nodes", type=int, default=100000)
     ap.add_argument("--srv-slots", type=int, default=4)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run one plain device copy of MIB MiB (calibrates HBM counters)")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "lagoon50k_pmc_traffic.md"),
+                    help="committed FETCH_SIZE/WRITE_SIZE summary that fills roofline.traffic")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design note, "
@@ -261,6 +296,9 @@ def main():
     dom = max((k for k in kb if avg.get(k, 0.0) > 0), key=lambda k: per_step[k])
     achieved = kb[dom] / (avg[dom] * 1e-3) / 1e9
     kernel_gbs = {k: round(kb[k] / (avg[k] * 1e-3) / 1e9, 1) for k in kb if avg.get(k, 0.0) > 0}
+    traffic = _measured_traffic(args.traffic, dom, "float")
+    if args.copy_calibration > 0:
+        _copy_calibration(args.copy_calibration)
 
     out = {
         "metric": "elem*layer*steps/s (FP32 + FP64-refined path; plain FP64 path, batched ensemble and "
@@ -290,7 +328,7 @@ def main():
                                     if v[0] > 0},
         "kernel_algorithmic_GBps_fp32": kernel_gbs,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_launch": kb[dom], "avg_launch_ms": round(avg[dom], 5)},
     }
     if not args.no_ensemble:

@@ -119,12 +119,24 @@ __device__ inline void block_sum(double (&v)[NQ]) {
     __syncthreads();
 }
 
-// per-workgroup partials: part[(region*kRegion + q)*nparts + workgroup]
+// per-workgroup partials: part[(region*kRegion + q)*nparts + slot], slot =
+// the workgroup's block (or its index)
 template <int NQ>
-__device__ inline void store_partials(double (&vals)[NQ], double *part, int region, int nparts) {
+__device__ inline void store_partials(double (&vals)[NQ], double *part, int region, int nparts, int slot) {
     block_sum<NQ>(vals);
     if (threadIdx.x == 0)
-        for (int q = 0; q < NQ; q++) part[((size_t)region * kRegion + q) * nparts + blockIdx.x] = vals[q];
+        for (int q = 0; q < NQ; q++) part[((size_t)region * kRegion + q) * nparts + slot] = vals[q];
+}
+
+// Block of a workgroup.  Workgroups are dealt to the XCDs round-robin
+// (workgroup w to XCD w % 8); each XCD is given a contiguous range of
+// blocks, so the blocks a block couples to (its neighbours in node order)
+// and its own factors are mostly produced on the same XCD and read from
+// its L2.  A bijection of [0, n).
+constexpr int kXcds = 8;
+__device__ inline int xcd_block(int w, int n) {
+    const int x = w % kXcds, slot = w / kXcds, per = n / kXcds, rem = n % kXcds;
+    return x * per + min(x, rem) + slot;
 }
 
 // total of the partials of the previous kernel (n workgroups), computed by
@@ -334,7 +346,7 @@ template <typename V, bool STAGE>
 __global__ __launch_bounds__(kThreads) void k_bilu_factor(SolvArgs<V> a, int sched) {
     extern __shared__ double smem[];
     V *val = reinterpret_cast<V *>(smem);
-    const int blk = blockIdx.x, b0 = blk * a.brows;
+    const int blk = xcd_block(blockIdx.x, gridDim.x), b0 = blk * a.brows;
     const int nb = min(a.n, b0 + a.brows) - b0;
     const int q0 = a.brp[b0], q1 = a.brp[b0 + nb], ne = q1 - q0;
     const int p0 = a.cp_ptr[q0], np = a.cp_ptr[q1] - p0;
@@ -434,7 +446,7 @@ __global__ __launch_bounds__(kThreads) void k_bicg_init(SolvArgs<V> a, int zero_
     if (a.ctrl[CT_REFCONV]) return;
     extern __shared__ double smem[];
     V *y = reinterpret_cast<V *>(smem), *w = y + a.brows;
-    const int blk = blockIdx.x, b0 = blk * a.brows;
+    const int blk = xcd_block(blockIdx.x, gridDim.x), b0 = blk * a.brows;
     const int nb = min(a.n, b0 + a.brows) - b0;
     BlockStage<V, STAGE> st;
     st.load(a, blk);
@@ -457,7 +469,7 @@ __global__ __launch_bounds__(kThreads) void k_bicg_init(SolvArgs<V> a, int zero_
         loc[0] += (double)r * (double)r;
     }
     loc[1] = loc[0];
-    store_partials<2>(loc, a.part, PR_C, a.nparts);
+    store_partials<2>(loc, a.part, PR_C, a.nparts, blk);
     if (blk == 0 && threadIdx.x == 0) {
         a.ctrl[CT_STOPIT] = 0; a.ctrl[CT_CONV] = 0; a.ctrl[CT_FAIL] = 0; a.ctrl[CT_MAXIT] = 0; a.ctrl[CT_NMV] = 1;
     }
@@ -481,7 +493,7 @@ __global__ __launch_bounds__(kThreads) void k_bicg_a(SolvArgs<V> a, int it) {
     V *y = reinterpret_cast<V *>(smem), *w = y + a.brows;
     const int par = it & 1, pp = par ^ 1;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    const int blk = blockIdx.x, b0 = blk * a.brows;
+    const int blk = xcd_block(blockIdx.x, gridDim.x), b0 = blk * a.brows;
     const int nb = min(a.n, b0 + a.brows) - b0;
     const V *pold = pp ? a.p1 : a.p0;     // direction of iteration it-1 (it = 0: unused)
     const V *vold = pp ? a.v1 : a.v0;
@@ -639,7 +651,7 @@ __global__ __launch_bounds__(kThreads) void k_bicg_a(SolvArgs<V> a, int it) {
             loc[0] += (double)a.r0[k] * (double)z[li];
         }
     }
-    store_partials<1>(loc, a.part, PR_A, a.nparts);
+    store_partials<1>(loc, a.part, PR_A, a.nparts, blk);
 }
 
 // Iteration it, second kernel: alpha = rho / (r0,v); s = r - alpha v
@@ -653,7 +665,7 @@ __global__ __launch_bounds__(kThreads) void k_bicg_b(SolvArgs<V> a, int it) {
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     const V *vnew = par ? a.v1 : a.v0;
     const V *rv = it == 0 ? a.r0 : a.r;
-    const int blk = blockIdx.x, b0 = blk * a.brows;
+    const int blk = xcd_block(blockIdx.x, gridDim.x), b0 = blk * a.brows;
     const int nb = min(a.n, b0 + a.brows) - b0;
     const bool pre = a.brows <= kThreads;
     const int i = b0 + threadIdx.x;
@@ -736,7 +748,7 @@ __global__ __launch_bounds__(kThreads) void k_bicg_b(SolvArgs<V> a, int it) {
             loc[4] += r0 * tv;
         }
     }
-    store_partials<5>(loc, a.part, PR_B, a.nparts);
+    store_partials<5>(loc, a.part, PR_B, a.nparts, blk);
 }
 
 // x = x0 + dx (tidycg) into the FP64 solution; with add, x += dx unless the
@@ -785,7 +797,7 @@ __global__ __launch_bounds__(kThreads) void k_resid64(const int *rowptr, const i
         loc[0] = r * r;
         loc[1] = b[i] * b[i];
     }
-    store_partials<2>(loc, part, PR_R, nparts);
+    store_partials<2>(loc, part, PR_R, nparts, blockIdx.x);
 }
 
 __global__ __launch_bounds__(kThreads) void k_resid_check(const double *part, int nparts, int n, double *scal,
