@@ -112,15 +112,24 @@ def _measured_traffic(path, group, vtype):
 
 def _copy_calibration(mib):
     """One plain device-to-device copy of `mib` MiB (its counters calibrate
-    FETCH_SIZE / WRITE_SIZE against a known byte count)."""
-    import torch
-    n = mib * 1024 * 1024 // 4
-    src = torch.ones(n, dtype=torch.float32, device="cuda")
-    dst = torch.empty_like(src)
-    torch.cuda.synchronize()
-    dst.copy_(src)
-    torch.cuda.synchronize()
-    del src, dst
+    FETCH_SIZE / WRITE_SIZE against a known byte count).  Made through the
+    HIP runtime libshyhip itself links (the same loaded instance), never a
+    second runtime in the process."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    nb = mib * 1024 * 1024
+    src, dst = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(src), C.c_size_t(nb)) or hip.hipMalloc(C.byref(dst), C.c_size_t(nb)):
+        raise RuntimeError("copy calibration: hipMalloc failed")
+    try:
+        hip.hipMemset(src, 1, C.c_size_t(nb))
+        hip.hipDeviceSynchronize()
+        if hip.hipMemcpy(dst, src, C.c_size_t(nb), 3):      # hipMemcpyDeviceToDevice
+            raise RuntimeError("copy calibration: hipMemcpy failed")
+        hip.hipDeviceSynchronize()
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
 
 
 def run_gpu(b, tr, prec, args):
