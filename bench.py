@@ -136,6 +136,7 @@ def run_gpu(b, tr, prec, args):
     import shyhip
     from shyhip import mesh
     m = shyhip.Model(b, prec, params=PARAMS, visv=1e-3, difv=1e-4, tracers=tr)
+    m.set_deferred_checks(True)     # device errors surface at the next wait (m.sync below)
     m.set_wind(*mesh.wind_stress(*WIND))
     t = 0.0
     for _ in range(args.warmup):

@@ -155,6 +155,13 @@ int  shyhip_step(shyhip_model *h, double dtime, int istot);
 /* scal3sh for tracer itr (0-based); clip_small = 1 for non-temperature */
 int  shyhip_scal3sh(shyhip_model *h, int itr, int istot, int clip_small);
 int  shyhip_sync(shyhip_model *h);
+/* Deferred checks (no counterpart in the reference, which runs on the host
+ * and stops at once): with on = 1, shyhip_step returns without waiting for
+ * the device at its end, so the next step's launches queue behind it.  A
+ * device error raised by a step's final phase (depths, transport) is then
+ * returned by the next call that waits for the device: the next step's
+ * solve, shyhip_sync, or a pull.  Off by default. */
+int  shyhip_set_deferred_checks(shyhip_model *h, int on);
 /* kernel-group timing with HIP events on the handle's stream (off by
  * default; enabling it adds a synchronisation at the end of every step).
  * id = 0 .. 17: set_dry, copy_uvz, hydro_intern, setnod, zeta_assemble,

@@ -298,6 +298,7 @@ int shyhip_destroy(shyhip_model *h) {
     for (hipStream_t st : M.tr_stream) (void)hipStreamDestroy(st);
     for (hipEvent_t ev : M.tr_done) (void)hipEventDestroy(ev);
     if (M.ev_hydro) (void)hipEventDestroy(M.ev_hydro);
+    if (M.ev_levels) (void)hipEventDestroy(M.ev_levels);
     for (void *q : M.tr_hbuf) dfree(q);
     dfree(M.d_err);
     dfree(M.d_levels);
@@ -606,6 +607,7 @@ int shyhip_set_boundaries(shyhip_model *h, int nbnd, const int *ibnd, const int 
     dfree(M.m.ibtyp);
     dfree(M.m.bnd_nodes);
     dfree(M.m.bnd_id);
+    HIPCHK(M, hipStreamSynchronize(M.stream));     // no upload from h_levels still queued
     dfree(M.d_levels);
     if (M.h_levels) (void)hipHostFree(M.h_levels);
     M.d_levels = nullptr;
@@ -792,8 +794,13 @@ int shyhip_set_boundary_levels(shyhip_model *h, int n, const float *levels) {
     if (!h) return fail(nullptr, "null handle");
     Model &M = h->M;
     if (n != M.nbnd || !M.d_levels) return fail(&M, "set_boundary_levels: n must equal nbnd");
+    // the staging buffer is rewritten only after its previous upload ran
+    // (steps may be queued ahead of the device)
+    if (!M.ev_levels) HIPCHK(M, hipEventCreateWithFlags(&M.ev_levels, hipEventDisableTiming));
+    else HIPCHK(M, hipEventSynchronize(M.ev_levels));
     std::copy(levels, levels + n, M.h_levels);
     HIPCHK(M, hipMemcpyAsync(M.d_levels, M.h_levels, n * sizeof(float), hipMemcpyHostToDevice, M.stream));
+    HIPCHK(M, hipEventRecord(M.ev_levels, M.stream));
     return launch_set_rzv_all(M);
 }
 
@@ -884,17 +891,18 @@ static int hydro_impl(Model &M, double dtime, bool final_check) {
         if (launch_zeta_assemble(M)) return 1;      // system_init + hydro_zeta + system_add_rhs
         timer_end(M);
         int its = 0;
+        // the solve, then system_get + setweg(1) behind it: one host
+        // read-back returns the solver flags, the device error word and the
+        // number of elements whose wet/dry state changed
+        const int bnew = M.dtime != M.dtime_old;
         timer_begin(M, T_SOLVE);
         int ref = 0;
-        const int rc = solver_solve(M, &its, &ref);
+        const int rc = solver_solve(M, &its, &ref, bnew);
         M.last_refine = ref;
         timer_end(M);
         its_total += its;
         if (rc) return rc;
-        const int bnew = M.dtime != M.dtime_old;
-        timer_begin(M, T_WETDRY);
-        if (launch_system_get_setweg1(M, bnew, &changed)) return 1;
-        timer_end(M);
+        changed = M.h_pinned[1];
         M.dtime_old = M.dtime;
         if (changed == 0) break;
     }
@@ -988,7 +996,14 @@ int shyhip_step(shyhip_model *h, double dtime, int istot) {
             if (shyhip_scal3sh(h, t, istot, t == 0 ? 0 : 1)) return 1;
     }
     if (timer_flush(M)) return 1;
+    if (M.deferred && !M.timing) return 0;
     return check_dev_err(M, "step");
+}
+
+int shyhip_set_deferred_checks(shyhip_model *h, int on) {
+    if (!h) return fail(nullptr, "null handle");
+    h->M.deferred = on != 0;
+    return 0;
 }
 
 int shyhip_set_timing(shyhip_model *h, int on) {

@@ -503,8 +503,10 @@ __global__ __launch_bounds__(kThreads) void k_zeta_rows(HydroArgs<S> a) {
 // setweg(iweich), wetdry.f90:128-347, one work-item per element; the number
 // of elements that changed state is accumulated in *cnt.
 template <typename S>
-__global__ __launch_bounds__(kThreads) void k_setweg(HydroArgs<S> a, int iweich, int bnewtime, int *cnt) {
+__global__ __launch_bounds__(kThreads) void k_setweg(HydroArgs<S> a, int iweich, int bnewtime, int *cnt,
+                                                     const int *gate) {
     using R = S;
+    if (gate && !*gate) return;
     const int ie = blockIdx.x * blockDim.x + threadIdx.x;
     if (ie >= a.nel) return;
     const int nel = a.nel;
@@ -614,9 +616,12 @@ __global__ __launch_bounds__(kThreads) void k_setnod(HydroArgs<S> a) {
     }
 }
 
-// system_get: z = real(x)
+// system_get: z = real(x).  With a gate (a solver flag), nothing happens
+// unless the flag is set: the kernel can be enqueued before the host knows
+// whether the solve has finished.
 template <typename S>
-__global__ __launch_bounds__(kThreads) void k_system_get(HydroArgs<S> a, int *cnt) {
+__global__ __launch_bounds__(kThreads) void k_system_get(HydroArgs<S> a, int *cnt, const int *gate) {
+    if (gate && !*gate) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k == 0) *cnt = 0;     // counter of the setweg that follows
     if (k >= a.nkn) return;
@@ -1081,7 +1086,7 @@ static int set_dry_t(Model &M, int iweich, int bnewtime, int *changed, bool cnt_
     auto a = make_args<S>(M);
     int *cnt = M.d_err + 1;
     if (changed && !cnt_zeroed) HIPCHK(M, hipMemsetAsync(cnt, 0, sizeof(int), M.stream));
-    LAUNCH(M, k_setweg<S>, nblocks(M.nel), a, iweich, bnewtime, cnt);
+    LAUNCH(M, k_setweg<S>, nblocks(M.nel), a, iweich, bnewtime, cnt, (const int *)nullptr);
     if (changed) {
         HIPCHK(M, hipMemcpyAsync(M.h_pinned, M.d_err, 2 * sizeof(int), hipMemcpyDeviceToHost, M.stream));
         HIPCHK(M, hipStreamSynchronize(M.stream));
@@ -1135,13 +1140,14 @@ static int zeta_t(Model &M) {
 int launch_zeta_assemble(Model &M) { return M.prec == SHYHIP_FP64 ? zeta_t<double>(M) : zeta_t<float>(M); }
 
 template <typename S>
-static int sysget_t(Model &M, int bnewtime, int *changed) {
+static int sysget_t(Model &M, int bnewtime, const int *gate) {
     auto a = make_args<S>(M);
-    LAUNCH(M, k_system_get<S>, nblocks(M.nkn), a, M.d_err + 1);
-    return set_dry_t<S>(M, 1, bnewtime, changed, true);
+    LAUNCH(M, k_system_get<S>, nblocks(M.nkn), a, M.d_err + 1, gate);
+    LAUNCH(M, k_setweg<S>, nblocks(M.nel), a, 1, bnewtime, M.d_err + 1, gate);
+    return 0;
 }
-int launch_system_get_setweg1(Model &M, int bnewtime, int *changed) {
-    return M.prec == SHYHIP_FP64 ? sysget_t<double>(M, bnewtime, changed) : sysget_t<float>(M, bnewtime, changed);
+int launch_system_get_setweg1(Model &M, int bnewtime, const int *gate) {
+    return M.prec == SHYHIP_FP64 ? sysget_t<double>(M, bnewtime, gate) : sysget_t<float>(M, bnewtime, gate);
 }
 
 template <typename S>

@@ -147,6 +147,8 @@ struct Model {
     std::vector<double> h_ev;             // [19*nel] reference ev layout
     std::vector<double> h_coo, h_rvec;    // plugin accumulation (c2coo, rvec2d)
     size_t esize() const { return prec == SHYHIP_FP64 ? 8 : 4; }
+    bool deferred = false;    // shyhip_step does not wait for the device at its end
+    hipEvent_t ev_levels = nullptr;   // the last upload from h_levels
     // timing
     bool timing = false;
     struct TEv { int id; hipEvent_t a, b; };
@@ -222,7 +224,10 @@ int launch_set_dry(Model &M, int iweich, int bnewtime, int *changed);
 int launch_setnod(Model &M);
 int launch_hydro_intern(Model &M);
 int launch_zeta_assemble(Model &M);
-int launch_system_get_setweg1(Model &M, int bnewtime, int *changed);
+// system_get + setweg(1), enqueued only: gate (a solver flag, may be null)
+// switches both off unless set; the changed-element count is left in
+// d_err[1] for the caller's read-back
+int launch_system_get_setweg1(Model &M, int bnewtime, const int *gate);
 int launch_transports_final(Model &M);
 int launch_setzev_setuvd(Model &M);
 int launch_setdepth(Model &M, bool old_level);
@@ -236,7 +241,12 @@ int launch_fill(Model &M, void *ptr, double value, size_t n);
 int launch_volume(Model &M, int mode, double *vol);
 int launch_tracer_mass(Model &M, int itr, double *mass);
 int solver_setup(Model &M);
-int solver_solve(Model &M, int *iters, int *refinements);
+// Zeta solve.  wetdry_bnew >= 0: system_get + setweg(1) (bnewtime =
+// wetdry_bnew) are enqueued behind the solve, switched on by its
+// convergence flag, before each host read of the solver flags, so one
+// read-back returns the solver state, d_err[0] and the changed-element
+// count (in h_pinned[0..1]).
+int solver_solve(Model &M, int *iters, int *refinements, int wetdry_bnew = -1);
 int launch_conz3d(Model &M, int itr, int istot, int isact, int clip_small, hipStream_t st, void *hbuf);
 int launch_bndo(Model &M, int itr, hipStream_t st);
 

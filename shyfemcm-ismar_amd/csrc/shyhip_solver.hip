@@ -29,6 +29,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 
 #include "shyhip_internal.h"
 
@@ -756,7 +757,7 @@ __global__ __launch_bounds__(kThreads) void k_bicg_b(SolvArgs<V> a, int it) {
 template <typename V>
 __global__ void k_bicg_finish(const V *x0, const V *dx, double *x, int n, int add, const int *ctrl) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || (add && ctrl[CT_REFCONV])) return;
+    if (i >= n || (add && ctrl[CT_REFCONV]) || (!add && !ctrl[CT_CONV])) return;
     if (add) x[i] = x[i] + (double)dx[i];
     else x[i] = (double)(x0[i] + dx[i]);
 }
@@ -815,10 +816,13 @@ __global__ __launch_bounds__(kThreads) void k_resid_check(const double *part, in
 }
 
 // ---------------------------------------------------------------- host driver
+// one read-back: the solver's control words (into h_pinned[16..]) and the
+// device error word and counter d_err[0..1] (into h_pinned[0..1])
 static int read_ctrl(Model &M, int *out, int n) {
-    HIPCHK(M, hipMemcpyAsync(M.h_pinned, M.sv.ctrl, sizeof(int) * n, hipMemcpyDeviceToHost, M.stream));
+    HIPCHK(M, hipMemcpyAsync(M.h_pinned + 16, M.sv.ctrl, sizeof(int) * n, hipMemcpyDeviceToHost, M.stream));
+    HIPCHK(M, hipMemcpyAsync(M.h_pinned, M.d_err, 2 * sizeof(int), hipMemcpyDeviceToHost, M.stream));
     HIPCHK(M, hipStreamSynchronize(M.stream));
-    for (int i = 0; i < n; i++) out[i] = M.h_pinned[i];
+    for (int i = 0; i < n; i++) out[i] = M.h_pinned[16 + i];
     return 0;
 }
 
@@ -872,9 +876,12 @@ static int factor(Model &M, SolvArgs<V> &a) {
 // enqueued in chunks between host checks of the device flags;
 // the first chunk is the iteration count of the previous solve of the same
 // kind (`slot`), so a converging solve usually needs a single round trip.
-// Kernels enqueued after convergence return immediately.
+// Kernels enqueued after convergence return immediately.  `tail` enqueues
+// the work that follows a converged solve, switched on by its flag, before
+// each check.
 template <typename V, bool STAGE>
-static int bicgstab_t(Model &M, SolvArgs<V> &a, int *iters, int slot, int zero_x0) {
+static int bicgstab_t(Model &M, SolvArgs<V> &a, int *iters, int slot, int zero_x0,
+                      const std::function<int()> &tail) {
     const int g = a.nblk;
     const size_t lds = apply_lds<V>(M);
     hipLaunchKernelGGL((k_bicg_init<V, STAGE>), dim3(g), dim3(kThreads), lds, M.stream, a, zero_x0);
@@ -888,6 +895,7 @@ static int bicgstab_t(Model &M, SolvArgs<V> &a, int *iters, int slot, int zero_x
             launch_timed(M, T_BICG_B, k_bicg_b<V, STAGE>, dim3(g), dim3(kThreads), lds, M.stream, a, it);
         }
         HIPCHK(M, hipGetLastError());
+        if (tail && tail()) return 1;
         if (read_ctrl(M, c, CT_WORDS)) return 1;
         if (c[CT_CONV] || c[CT_FAIL] || c[CT_MAXIT] || it >= maxit) break;
         chunk = 2;
@@ -910,9 +918,9 @@ static int bicgstab_t(Model &M, SolvArgs<V> &a, int *iters, int slot, int zero_x
 }
 
 template <typename V>
-static int bicgstab(Model &M, SolvArgs<V> &a, int *iters, int slot, int zero_x0) {
-    return stage_fits<V>(M) ? bicgstab_t<V, true>(M, a, iters, slot, zero_x0)
-                            : bicgstab_t<V, false>(M, a, iters, slot, zero_x0);
+static int bicgstab(Model &M, SolvArgs<V> &a, int *iters, int slot, int zero_x0, const std::function<int()> &tail) {
+    return stage_fits<V>(M) ? bicgstab_t<V, true>(M, a, iters, slot, zero_x0, tail)
+                            : bicgstab_t<V, false>(M, a, iters, slot, zero_x0, tail);
 }
 
 // one inner FP32 solve of the refinement, enqueued without host checks:
@@ -938,10 +946,18 @@ static int enqueue_inner(Model &M, SolvArgs<V> &a, int chunk) {
 
 int solver_setup(Model &M) { (void)M; return 0; }
 
-int solver_solve(Model &M, int *iters, int *refinements) {
+int solver_solve(Model &M, int *iters, int *refinements, int wetdry_bnew) {
     const int n = M.nkn;
     const int g1 = (n + kThreads - 1) / kThreads;
     int it_total = 0, nref = 0;
+    // system_get + setweg(1) behind the solve, gated by its convergence flag
+    auto wetdry = [&](const int *gate) -> int {
+        if (wetdry_bnew < 0) return 0;
+        timer_begin(M, T_WETDRY);
+        const int rc = launch_system_get_setweg1(M, wetdry_bnew, gate);
+        timer_end(M);
+        return rc;
+    };
     if (M.prec == SHYHIP_FP64) {
         // initial guess = current level (system_solve passes znv); the Krylov
         // x0 vector is the FP64 solution array itself
@@ -953,12 +969,15 @@ int solver_solve(Model &M, int *iters, int *refinements) {
         auto a = solv_args<double>(M, M.s.aval, M.s.rhs, M.s.x);
         if (factor<double>(M, a)) return 1;
         int its = 0;
-        const int rc = bicgstab<double>(M, a, &its, 0, 0);
+        auto tail = [&]() -> int {
+            hipLaunchKernelGGL(k_bicg_finish<double>, dim3(g1), dim3(kThreads), 0, M.stream, (const double *)M.s.x,
+                               (const double *)a.dx, M.s.x, n, 0, (const int *)M.sv.ctrl);
+            HIPCHK(M, hipGetLastError());
+            return wetdry(M.sv.ctrl + CT_CONV);
+        };
+        const int rc = bicgstab<double>(M, a, &its, 0, 0, tail);
         it_total = its;
         if (rc) return rc;
-        hipLaunchKernelGGL(k_bicg_finish<double>, dim3(g1), dim3(kThreads), 0, M.stream, (const double *)M.s.x,
-                           (const double *)a.dx, M.s.x, n, 0, (const int *)M.sv.ctrl);
-        HIPCHK(M, hipGetLastError());
     } else {
         // FP32 BiCGStab on the FP32-rounded matrix inside FP64 refinement:
         // x <- x + A32^-1 (b - A64 x) until |b - A64 x| <= refine_rtol |b|;
@@ -997,6 +1016,7 @@ int solver_solve(Model &M, int *iters, int *refinements) {
                                    (const float *)a.dx, M.s.x, n, 1, (const int *)M.sv.ctrl);
             }
             if (resid()) return 1;
+            if (wetdry(M.sv.ctrl + CT_REFCONV)) return 1;
             if (read_ctrl(M, c, CT_WORDS)) return 1;
             if (c[CT_FACTOR] != 0) {
                         M.err = "zeta solver: zero pivot in the block factorisation (row " + std::to_string(c[CT_FACTOR]) + ")";

@@ -191,6 +191,12 @@ module shyhip_capi
       type(c_ptr), value :: h
       integer(c_int) :: rc
     end function
+    function shyhip_set_deferred_checks(h, on) bind(C, name="shyhip_set_deferred_checks") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      integer(c_int), value :: on
+      integer(c_int) :: rc
+    end function
 
     function shyhip_volume(h, mode, vol) bind(C, name="shyhip_volume") result(rc)
       import :: c_ptr, c_int, c_double

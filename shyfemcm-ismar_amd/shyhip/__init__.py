@@ -228,6 +228,11 @@ class Model:
     def sync(self):
         self._chk(self.lib.shyhip_sync(self.h), "sync")
 
+    def set_deferred_checks(self, on=True):
+        """Steps return without waiting for the device; a device error of a
+        step's final phase is raised by the next call that waits."""
+        self._chk(self.lib.shyhip_set_deferred_checks(self.h, 1 if on else 0), "set_deferred_checks")
+
     def stats(self):
         a, b_, c = C.c_int(), C.c_int(), C.c_int()
         self._chk(self.lib.shyhip_stats(self.h, C.byref(a), C.byref(b_), C.byref(c)), "stats")
