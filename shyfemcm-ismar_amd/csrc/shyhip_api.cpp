@@ -288,7 +288,8 @@ int shyhip_destroy(shyhip_model *h) {
                     (void *)m.blk_cp_a, (void *)m.blk_cp_b, (void *)m.blk_jd, (void *)m.blk_pcol, (void *)m.blk_prp,
                     (void *)m.blk_pdg, (void *)m.blk_elp, (void *)m.blk_elr, (void *)m.blk_enlev, (void *)m.ell_col, (void *)m.ell_len, (void *)m.csr_ell,
                     (void *)m.fwd_rows, (void *)m.fwd_lptr, (void *)m.fwd_nlev, (void *)m.bwd_rows,
-                    (void *)m.bwd_lptr, (void *)m.bwd_nlev, (void *)m.tvd_el, (void *)m.tvd_kn, m.tvd_w})
+                    (void *)m.bwd_lptr, (void *)m.bwd_nlev, (void *)m.tvd_el, (void *)m.tvd_kn, m.tvd_w,
+                    (void *)m.tvd_lev, (void *)m.elem_iext})
         dfree(q);
     Solver &v = M.sv;
     for (void *q : {v.lu, v.ell_a, v.r0, v.r, v.p0, v.p1, v.v0, v.v1, v.s, v.t, v.dx, v.b, v.x0, (void *)v.part,
@@ -578,6 +579,7 @@ int shyhip_set_basin(shyhip_model *h, const int *nen3v, const float *xgv, const 
     // defaults: no open boundary, no external flags
     std::vector<int> zeros(nkn, 0), ty(2, 0);
     rc |= dalloc(M, &m.ibnd, nkn) || dalloc(M, &m.zbound, nkn) || dalloc(M, &m.iextb, nkn);
+    rc |= dalloc(M, &m.elem_iext, nel);
     rc |= dalloc(M, &m.ibtyp, 2);
     rc |= dalloc_s(M, &m.fcorv, nel) || dalloc_s(M, &m.czv, nel);
     HIPCHK(M, hipStreamSynchronize(M.stream));
@@ -620,6 +622,12 @@ int shyhip_set_boundaries(shyhip_model *h, int nbnd, const int *ibnd, const int 
     int rc = 0;
     rc |= upload(M, M.m.ibnd, ibnd, nkn) || upload(M, M.m.zbound, zb.data(), nkn) ||
           upload(M, M.m.iextb, M.h_iextb.data(), nkn);
+    {
+        std::vector<int> ext(M.nel, 0);
+        for (int ie = 0; ie < M.nel; ie++)
+            for (int ii = 0; ii < 3; ii++) ext[ie] += M.h_iextb[M.h_nen3v[3 * (size_t)ie + ii]] != 0;
+        rc |= upload(M, M.m.elem_iext, ext.data(), ext.size());
+    }
     rc |= dalloc(M, &M.m.ibtyp, nbnd + 1) || upload(M, M.m.ibtyp, ibtyp, nbnd + 1);
     rc |= dalloc(M, &M.m.bnd_nodes, nodes.size()) || upload(M, M.m.bnd_nodes, nodes.data(), nodes.size());
     rc |= dalloc(M, &M.m.bnd_id, ids.size()) || upload(M, M.m.bnd_id, ids.data(), ids.size());
@@ -697,9 +705,12 @@ int shyhip_tvd_init(shyhip_model *h) {
                 }
             }
     }
-    dfree(M.m.tvd_el); dfree(M.m.tvd_kn); dfree(M.m.tvd_w);
+    std::vector<int> tlev(6 * nel, 0);
+    for (size_t q = 0; q < tel.size(); q++) tlev[q] = tel[q] >= 0 ? M.h_ilhv[tel[q]] : 0;
+    dfree(M.m.tvd_el); dfree(M.m.tvd_kn); dfree(M.m.tvd_w); dfree(M.m.tvd_lev);
     int rc = 0;
     rc |= dalloc(M, &M.m.tvd_el, tel.size()) || upload(M, M.m.tvd_el, tel.data(), tel.size());
+    rc |= dalloc(M, &M.m.tvd_lev, tlev.size()) || upload(M, M.m.tvd_lev, tlev.data(), tlev.size());
     rc |= dalloc(M, &M.m.tvd_kn, tkn.size()) || upload(M, M.m.tvd_kn, tkn.data(), tkn.size());
     double *w = nullptr;
     rc |= dalloc(M, &w, tw.size()) || upload(M, w, tw.data(), tw.size());

@@ -36,7 +36,7 @@ struct ConzArgs {
     const double *geo, *dis;
     const int *nkc_ptr, *nkc_el;
     const int *inc_cnt, *inc_elc;    // incidence slots (colour order)
-    const int *tvd_el, *tvd_kn;
+    const int *tvd_el, *tvd_kn, *tvd_lev, *elem_iext;
     const double *tvd_w;
     const S *utn, *vtn, *uto, *vto, *uln, *vln;
     const S *hdkn, *hdko, *hden, *hdeo, *wln, *difv, *mflux;
@@ -62,10 +62,22 @@ __device__ inline S cval_s(const ConzArgs<S> &a, int l, int k) {
 
 // ------------------------------------------------------------------ element phase
 // One workgroup per tile of kElemTile consecutive elements.  The tile's
-// geometry, levels and TVD upwind tables are staged in LDS once; then the
-// work-items sweep the tile's element-layers (kElemTile elements x 4 layers
-// at a time), each reading only its layer's transports and concentrations.
+// geometry, levels and TVD upwind tables are staged in LDS once, four
+// work-items per element, every load independent of the others (the upwind
+// elements' levels and the elements' external-node counts are precomputed
+// tables).  Then each work-item takes up to kElemBatch element-layers of
+// the tile at once: the loads of all of them are issued together (the
+// layer's transports and node concentrations, then the upwind
+// concentrations of the two pairs the flow direction selects), before any
+// flux is computed.
 constexpr int kElemTile = 64;
+constexpr int kElemBatch = 1;
+
+// v[i] for a run-time i in 0..2 without indexing a register array
+template <typename T>
+__device__ inline T sel3(const T (&v)[3], int i) {
+    return i == 0 ? v[0] : (i == 1 ? v[1] : v[2]);
+}
 
 struct ElemTileLds {
     double b[3][kElemTile], c[3][kElemTile], dis[3][kElemTile];
@@ -79,36 +91,36 @@ template <typename S>
 __global__ __launch_bounds__(kThreads) void k_conz_elem(ConzArgs<S> a) {
     using R = S;
     __shared__ ElemTileLds T;
-    const int nel = a.nel, e0 = blockIdx.x * kElemTile;
+    const int nel = a.nel, nkn = a.nkn, e0 = blockIdx.x * kElemTile;
     const int ne = min(kElemTile, nel - e0);
     const bool use_tvd = a.itvd > 0;
     __shared__ int lmax_tile;
     if (threadIdx.x == 0) lmax_tile = 0;
     __syncthreads();
-    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
-        const int ie = e0 + e;
-        int iext = 0;
-        for (int ii = 0; ii < 3; ii++) {
-            const int k = a.nen3v[3 * ie + ii];
-            T.kn[ii][e] = k;
-            T.b[ii][e] = a.geo[(size_t)ii * nel + ie];
-            T.c[ii][e] = a.geo[(size_t)(3 + ii) * nel + ie];
-            T.dis[ii][e] = a.dis[(size_t)ii * nel + ie];
-            if (a.iextb[k]) iext++;
-        }
-        T.iext[e] = iext;
-        T.ajr[e] = a.geo[(size_t)6 * nel + ie];
-        const int lv = a.ilhv[ie];
-        T.ilev[e] = lv;
-        atomicMax(&lmax_tile, lv);
-        if (use_tvd && iext == 0) {
-            for (int p = 0; p < 6; p++) {
-                const int up = a.tvd_el[(size_t)p * nel + ie];
-                T.tel[p][e] = up;
-                T.tlev[p][e] = up >= 0 ? a.ilhv[up] : 0;
-                for (int q = 0; q < 3; q++) {
-                    T.tkn[q][p][e] = a.tvd_kn[((size_t)q * 6 + p) * nel + ie];
-                    T.tw[q][p][e] = a.tvd_w[((size_t)q * 6 + p) * nel + ie];
+    {
+        const int e = threadIdx.x % kElemTile, part = threadIdx.x / kElemTile;   // kThreads = 4 * kElemTile
+        if (e < ne) {
+            const int ie = e0 + e;
+            if (part == 0) {
+                for (int ii = 0; ii < 3; ii++) {
+                    T.kn[ii][e] = a.nen3v[3 * ie + ii];
+                    T.b[ii][e] = a.geo[(size_t)ii * nel + ie];
+                    T.c[ii][e] = a.geo[(size_t)(3 + ii) * nel + ie];
+                    T.dis[ii][e] = a.dis[(size_t)ii * nel + ie];
+                }
+                T.iext[e] = a.elem_iext[ie];
+                T.ajr[e] = a.geo[(size_t)6 * nel + ie];
+                const int lv = a.ilhv[ie];
+                T.ilev[e] = lv;
+                atomicMax(&lmax_tile, lv);
+            } else if (use_tvd) {
+                for (int p = 2 * (part - 1); p < 2 * part; p++) {
+                    T.tel[p][e] = a.tvd_el[(size_t)p * nel + ie];
+                    T.tlev[p][e] = a.tvd_lev[(size_t)p * nel + ie];
+                    for (int q = 0; q < 3; q++) {
+                        T.tkn[q][p][e] = a.tvd_kn[((size_t)q * 6 + p) * nel + ie];
+                        T.tw[q][p][e] = a.tvd_w[((size_t)q * 6 + p) * nel + ie];
+                    }
                 }
             }
         }
@@ -116,79 +128,124 @@ __global__ __launch_bounds__(kThreads) void k_conz_elem(ConzArgs<S> a) {
     __syncthreads();
     const int lmt = lmax_tile;
     const double azt = 1. - a.az;
-    for (int idx = threadIdx.x; idx < kElemTile * lmt; idx += blockDim.x) {
-        const int e = idx % kElemTile, l = idx / kElemTile + 1;
-        if (e >= ne || l > T.ilev[e]) continue;
-        const int ie = e0 + e;
-        int kn[3];
-        double b[3], c[3];
-        for (int ii = 0; ii < 3; ii++) { kn[ii] = T.kn[ii][e]; b[ii] = T.b[ii][e]; c[ii] = T.c[ii][e]; }
-        const bool tvd = use_tvd && T.iext[e] == 0;
-        const R ajr = (R)(24. * T.ajr[e]);
-        const size_t o = (size_t)(l - 1) * nel + ie;
-        const double us = a.az * (double)a.utn[o] + azt * (double)a.uto[o];
-        const double vs = a.az * (double)a.vtn[o] + azt * (double)a.vto[o];
-        double f[3], cl[3], fl[3];
-        int itot = 0, isum = 0;
-        for (int ii = 0; ii < 3; ii++) {
-            f[ii] = us * b[ii] + vs * c[ii];
-            if (f[ii] < 0.) { itot++; isum += ii + 1; }
-            cl[ii] = cval(a, l, kn[ii]);
+    auto clip = [&](S v) -> S { return (a.clip && v < (S)1.e-15) ? (S)0 : v; };
+    for (int base = 0; base < kElemTile * lmt; base += kElemBatch * kThreads) {
+        bool on[kElemBatch], tvd[kElemBatch];
+        int ev[kElemBatch], lv[kElemBatch];
+        S un[kElemBatch], uo[kElemBatch], vn[kElemBatch], vo[kElemBatch], ul[kElemBatch], vl[kElemBatch];
+        S cr[kElemBatch][3];
+        // round 1: the layer's transports, velocities and node concentrations
+#pragma unroll
+        for (int j = 0; j < kElemBatch; j++) {
+            const int idx = base + j * kThreads + threadIdx.x;
+            const int e = idx % kElemTile, l = idx / kElemTile + 1;
+            on[j] = idx < kElemTile * lmt && e < ne && l <= T.ilev[e];
+            ev[j] = e; lv[j] = l;
+            tvd[j] = false;
+            if (on[j]) {
+                const size_t o = (size_t)(l - 1) * nel + e0 + e;
+                un[j] = a.utn[o]; uo[j] = a.uto[o]; vn[j] = a.vtn[o]; vo[j] = a.vto[o];
+                tvd[j] = use_tvd && T.iext[e] == 0;
+                if (tvd[j]) { ul[j] = a.uln[o]; vl[j] = a.vln[o]; }
+                for (int ii = 0; ii < 3; ii++) cr[j][ii] = a.c[(size_t)(l - 1) * nkn + T.kn[ii][e]];
+            }
         }
-        if (itot == 1) {
-            for (int ii = 0; ii < 3; ii++) fl[ii] = f[ii] * cl[isum - 1];
-        } else if (itot == 2) {
-            const int is2 = 6 - isum;
-            for (int ii = 0; ii < 3; ii++) fl[ii] = f[ii] * cl[ii];
-            fl[is2 - 1] = 0.;
-            fl[is2 - 1] = -(fl[0] + fl[1] + fl[2]);
-        } else {
-            fl[0] = fl[1] = fl[2] = 0.;
-        }
-        if (tvd) {
-            // tvd_fluxes, tvd_admin.f90:673-823
-            fl[0] = fl[1] = fl[2] = 0.;
-            if (itot >= 1 && itot <= 2) {
-                const int itot2 = itot - 1, itot1 = 2 - itot;
-                const R u = a.uln[o], v = a.vln[o];
-                const int ip = itot2 * (6 - isum) + itot1 * isum;
-                for (int ii = 1; ii <= 3; ii++) {
-                    if (ii == ip) continue;
+        double f[kElemBatch][3], fl[kElemBatch][3];
+        int itot[kElemBatch], isum[kElemBatch];
+        // upwind rule; the TVD pairs of the flow direction and their loads (round 2)
+        int pii[kElemBatch][2], pic[kElemBatch][2], pid[kElemBatch][2], pix[kElemBatch][2];
+        bool pup[kElemBatch][2];
+        S cu[kElemBatch][2][3];
+#pragma unroll
+        for (int j = 0; j < kElemBatch; j++) {
+            if (!on[j]) continue;
+            const int e = ev[j];
+            const double us = a.az * (double)un[j] + azt * (double)uo[j];
+            const double vs = a.az * (double)vn[j] + azt * (double)vo[j];
+            itot[j] = 0; isum[j] = 0;
+            double cl[3];
+            for (int ii = 0; ii < 3; ii++) {
+                f[j][ii] = us * T.b[ii][e] + vs * T.c[ii][e];
+                if (f[j][ii] < 0.) { itot[j]++; isum[j] += ii + 1; }
+                cl[ii] = (double)clip(cr[j][ii]);
+            }
+            if (itot[j] == 1) {
+                const double cu1 = sel3(cl, isum[j] - 1);
+                for (int ii = 0; ii < 3; ii++) fl[j][ii] = f[j][ii] * cu1;
+            } else if (itot[j] == 2) {
+                const int is2 = 6 - isum[j];
+                for (int ii = 0; ii < 3; ii++) fl[j][ii] = (ii == is2 - 1) ? 0. : f[j][ii] * cl[ii];
+                const double sum = fl[j][0] + fl[j][1] + fl[j][2];
+                for (int ii = 0; ii < 3; ii++)
+                    if (ii == is2 - 1) fl[j][ii] = -sum;
+            } else {
+                fl[j][0] = fl[j][1] = fl[j][2] = 0.;
+            }
+            pup[j][0] = pup[j][1] = false;
+            if (tvd[j] && itot[j] >= 1 && itot[j] <= 2) {
+                const int itot2 = itot[j] - 1, itot1 = 2 - itot[j];
+                const int ip = itot2 * (6 - isum[j]) + itot1 * isum[j];
+#pragma unroll
+                for (int k = 0; k < 2; k++) {
+                    // the vertices other than ip, ascending
+                    const int ii = k == 0 ? (ip == 1 ? 2 : 1) : (ip == 3 ? 2 : 3);
                     const int ic = itot2 * ii + itot1 * ip;
                     const int id = itot2 * ip + itot1 * ii;
-                    const R fact = (R)(-itot2 + itot1);
-                    const R conc = (R)cl[ic - 1], cond = (R)cl[id - 1];
-                    const int iop = 6 - (id + ic);
-                    const int tet1 = 1 + iop % 3;
-                    R dx = (R)((double)ajr * c[iop - 1]);
-                    dx = (R)(-2 * (tet1 == id ? 1 : 0)) * dx + dx;
-                    R dy = (R)((double)ajr * b[iop - 1]);
-                    dy = (R)(-2 * (tet1 == ic ? 1 : 0)) * dy + dy;
-                    const R dis = (R)T.dis[iop - 1][e];
-                    const R vel = (R)fabs((double)(u * dx + v * dy)) / dis;
-                    const R alfa = (R)((a.dt * (double)vel) / (double)dis);
-                    // upwind value: pair index of (ic,id)
                     const int pidx = 2 * (ic - 1) + ((id == (ic % 3) + 1) ? 0 : 1);
-                    R conu = cond;
-                    if (T.tel[pidx][e] >= 0 && T.tlev[pidx][e] >= l) {
-                        double zh = 0.;
-                        for (int q = 0; q < 3; q++) zh = zh + (double)cval_s(a, l, T.tkn[q][pidx][e]) * T.tw[q][pidx][e];
-                        conu = (R)zh;
-                    }
-                    const R grad = cond - conu;
-                    R rf;
-                    if ((R)fabs((double)(conc - cond)) < (R)1.e-8) rf = -1;
-                    else rf = grad / (cond - conc) - (R)1;
-                    const double rfd = (double)rf;
-                    const R psi = (R)fmax(0., fmax(fmin(1., 2. * rfd), fmin(2., rfd)));
-                    const R conf = conc + (R)0.5 * psi * (cond - conc) * ((R)1 - alfa);
-                    const R term = (R)((double)(fact * conf) * f[ii - 1]);
-                    fl[ic - 1] = fl[ic - 1] - (double)term;
-                    fl[id - 1] = fl[id - 1] + (double)term;
+                    pii[j][k] = ii; pic[j][k] = ic; pid[j][k] = id; pix[j][k] = pidx;
+                    pup[j][k] = T.tel[pidx][e] >= 0 && T.tlev[pidx][e] >= lv[j];
+                    if (pup[j][k])
+                        for (int q = 0; q < 3; q++) cu[j][k][q] = a.c[(size_t)(lv[j] - 1) * nkn + T.tkn[q][pidx][e]];
                 }
             }
         }
-        for (int ii = 0; ii < 3; ii++) a.hbuf[((size_t)(l - 1) * 3 + ii) * nel + ie] = (S)(3. * fl[ii]);
+        // TVD fluxes (tvd_fluxes, tvd_admin.f90:673-823) and the write-out
+#pragma unroll
+        for (int j = 0; j < kElemBatch; j++) {
+            if (!on[j]) continue;
+            const int e = ev[j], ie = e0 + e, l = lv[j];
+            if (tvd[j]) {
+                fl[j][0] = fl[j][1] = fl[j][2] = 0.;
+                if (itot[j] >= 1 && itot[j] <= 2) {
+                    const int itot2 = itot[j] - 1, itot1 = 2 - itot[j];
+                    const R ajr = (R)(24. * T.ajr[e]);
+                    const R u = ul[j], v = vl[j];
+                    for (int k = 0; k < 2; k++) {
+                        const int ii = pii[j][k], ic = pic[j][k], id = pid[j][k], pidx = pix[j][k];
+                        const R fact = (R)(-itot2 + itot1);
+                        const R conc = clip(sel3(cr[j], ic - 1)), cond = clip(sel3(cr[j], id - 1));
+                        const int iop = 6 - (id + ic);
+                        const int tet1 = 1 + iop % 3;
+                        R dx = (R)((double)ajr * T.c[iop - 1][e]);
+                        dx = (R)(-2 * (tet1 == id ? 1 : 0)) * dx + dx;
+                        R dy = (R)((double)ajr * T.b[iop - 1][e]);
+                        dy = (R)(-2 * (tet1 == ic ? 1 : 0)) * dy + dy;
+                        const R dis = (R)T.dis[iop - 1][e];
+                        const R vel = (R)fabs((double)(u * dx + v * dy)) / dis;
+                        const R alfa = (R)((a.dt * (double)vel) / (double)dis);
+                        R conu = cond;
+                        if (pup[j][k]) {
+                            double zh = 0.;
+                            for (int q = 0; q < 3; q++) zh = zh + (double)clip(cu[j][k][q]) * T.tw[q][pidx][e];
+                            conu = (R)zh;
+                        }
+                        const R grad = cond - conu;
+                        R rf;
+                        if ((R)fabs((double)(conc - cond)) < (R)1.e-8) rf = -1;
+                        else rf = grad / (cond - conc) - (R)1;
+                        const double rfd = (double)rf;
+                        const R psi = (R)fmax(0., fmax(fmin(1., 2. * rfd), fmin(2., rfd)));
+                        const R conf = conc + (R)0.5 * psi * (cond - conc) * ((R)1 - alfa);
+                        const R term = (R)((double)(fact * conf) * sel3(f[j], ii - 1));
+                        for (int t = 0; t < 3; t++) {
+                            if (t == ic - 1) fl[j][t] = fl[j][t] - (double)term;
+                            if (t == id - 1) fl[j][t] = fl[j][t] + (double)term;
+                        }
+                    }
+                }
+            }
+            for (int ii = 0; ii < 3; ii++) a.hbuf[((size_t)(l - 1) * 3 + ii) * nel + ie] = (S)(3. * fl[j][ii]);
+        }
     }
 }
 
@@ -385,6 +442,7 @@ static int conz_nl(Model &M, int itr, int istot, int isact, int clip, hipStream_
     a.geo = M.m.geo; a.dis = M.m.dis;
     a.nkc_ptr = M.m.nk_ptr; a.nkc_el = M.m.nkc_el; a.inc_cnt = M.m.inc_cnt; a.inc_elc = M.m.inc_elc;
     a.tvd_el = M.m.tvd_el; a.tvd_kn = M.m.tvd_kn; a.tvd_w = (const double *)M.m.tvd_w;
+    a.tvd_lev = M.m.tvd_lev; a.elem_iext = M.m.elem_iext;
     a.utn = (const S *)M.s.utn; a.vtn = (const S *)M.s.vtn; a.uto = (const S *)M.s.uto; a.vto = (const S *)M.s.vto;
     a.uln = (const S *)M.s.uln; a.vln = (const S *)M.s.vln;
     a.hdkn = (const S *)M.s.hdkn; a.hdko = (const S *)M.s.hdko; a.hden = (const S *)M.s.hden;

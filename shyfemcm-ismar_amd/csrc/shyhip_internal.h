@@ -68,6 +68,8 @@ struct Mesh {                 // device copies of init-time data
     int *tvd_el = nullptr;    // upwind element or -1
     int *tvd_kn = nullptr;    // [3][6][nel] its nodes
     void *tvd_w = nullptr;    // double [3][6][nel] interpolation weights (adjusted xi)
+    int *tvd_lev = nullptr;   // [6][nel] bottom layer of the upwind element (0 if none)
+    int *elem_iext = nullptr; // [nel] vertices of the element flagged external (iextb)
     // scalar open boundary (bndo_init): nodes, inward normals, inner neighbours
     int nbndo = 0;
     int *bndo_k = nullptr, *bndo_ptr = nullptr, *bndo_nb = nullptr;
