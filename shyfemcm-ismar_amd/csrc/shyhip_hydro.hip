@@ -836,6 +836,13 @@ __global__ __launch_bounds__(kThreads) void k_setdepth_node(HydroArgs<S> a, cons
     };
     IncSlots in;
     in.load(a.inc_cnt, a.inc_el, nkn, k);
+    // the node's layer areas, loaded with the slots (used after the sums)
+    R sar[NL];
+    {
+        const int lk = a.ilhkv[k];
+#pragma unroll
+        for (int l = 0; l < NL; l++) sar[l] = l < lk ? a.areakv[(size_t)l * nkn + k] : (R)0;
+    }
     R sarea[kIncWidth], shtot[kIncWidth], sz[kIncWidth];
     int slmax[kIncWidth];
 #pragma unroll
@@ -861,7 +868,7 @@ __global__ __launch_bounds__(kThreads) void k_setdepth_node(HydroArgs<S> a, cons
 #pragma unroll
     for (int l = 1; l <= NL; l++) {
         if (l <= lmaxk && !stop) {
-            const R ar = a.areakv[(size_t)(l - 1) * nkn + k];
+            const R ar = sar[l - 1];
             if (ar > (R)0) acc[l - 1] = acc[l - 1] / ar;
             else stop = true;
         }
@@ -886,56 +893,78 @@ template <typename S, int NL>
 __global__ __launch_bounds__(kThreads) void k_hydro_vertical(HydroArgs<S> a) {
     using R = S;
     constexpr int TN = VertTile<NL>::TN;
-    __shared__ R vfL[NL][TN], vaL[NL][TN];
+    __shared__ R vfL[NL][TN], vaL[NL][TN], dvL[NL][TN];
     const int nel = a.nel, nkn = a.nkn, nlvdi = a.nlvdi;
     const int k0 = blockIdx.x * TN;
     const R az = (R)a.az, azt = (R)1 - az, dt = (R)a.dt;
     const int nlv = min(a.nlv, NL);
-    for (int idx = threadIdx.x; idx < TN * nlv; idx += blockDim.x) {
-        const int kk = idx % TN, l = idx / TN + 1, k = k0 + kk;
-        if (k >= nkn || l > a.ilhkv[k]) continue;
-        R vf = 0, va = 0;
-        auto add = [&](int ie, int ii) {
-            if (a.ilhv[ie] < l) return;
-            const R aj = (R)(4. * a.geo[(size_t)6 * nel + ie]);
-            const R b = (R)a.geo[(size_t)ii * nel + ie], c = (R)a.geo[(size_t)(3 + ii) * nel + ie];
-            const size_t o = (size_t)(l - 1) * nel + ie;
-            const R ffn = a.utn[o] * b + a.vtn[o] * c;
-            const R ffo = a.uto[o] * b + a.vto[o] * c;
-            const R ff = ffn * az + ffo * azt;
-            vf = vf + (R)3 * aj * ff;
-            va = va + aj;
-        };
+    // phase 1: a work-item takes one node of the tile (TN divides kThreads)
+    // and every (kThreads/TN)-th layer of it; the node's incidence slots and
+    // their elements' geometry are loaded once for all its layers
+    static_assert(kThreads % TN == 0, "tile must divide the workgroup");
+    {
+        const int kk = threadIdx.x % TN, k = k0 + kk;
+        const int lstep = kThreads / TN;
+        const int lim = k < nkn ? min(nlv, a.ilhkv[k]) : 0;
+        int slv[kIncWidth], sie[kIncWidth];
+        R saj[kIncWidth], sb[kIncWidth], sc[kIncWidth];
         IncSlots in;
-        in.load(a.inc_cnt, a.inc_el, nkn, k);
-        // slot operands loaded at once (layer l of every slot's element),
-        // then added in list order
-        int slv[kIncWidth];
-        R saj[kIncWidth], sb[kIncWidth], sc[kIncWidth], sun[kIncWidth], svn[kIncWidth], suo[kIncWidth], svo[kIncWidth];
+        if (lim > 0) {
+            in.load(a.inc_cnt, a.inc_el, nkn, k);
 #pragma unroll
-        for (int s = 0; s < kIncWidth; s++) {
-            const int ie = in.pk[s] >> 2, ii = in.pk[s] & 3;
-            const size_t o = (size_t)(l - 1) * nel + ie;
-            slv[s] = a.ilhv[ie];
-            saj[s] = (R)(4. * a.geo[(size_t)6 * nel + ie]);
-            sb[s] = (R)a.geo[(size_t)ii * nel + ie];
-            sc[s] = (R)a.geo[(size_t)(3 + ii) * nel + ie];
-            sun[s] = a.utn[o]; svn[s] = a.vtn[o]; suo[s] = a.uto[o]; svo[s] = a.vto[o];
-        }
-#pragma unroll
-        for (int s = 0; s < kIncWidth; s++) {
-            if (s < in.cnt && slv[s] >= l) {
-                const R ffn = sun[s] * sb[s] + svn[s] * sc[s];
-                const R ffo = suo[s] * sb[s] + svo[s] * sc[s];
-                const R ff = ffn * az + ffo * azt;
-                vf = vf + (R)3 * saj[s] * ff;
-                va = va + saj[s];
+            for (int s = 0; s < kIncWidth; s++) {
+                const int ie = in.pk[s] >> 2, ii = in.pk[s] & 3;
+                sie[s] = ie;
+                slv[s] = a.ilhv[ie];
+                saj[s] = (R)(4. * a.geo[(size_t)6 * nel + ie]);
+                sb[s] = (R)a.geo[(size_t)ii * nel + ie];
+                sc[s] = (R)a.geo[(size_t)(3 + ii) * nel + ie];
             }
         }
-        if (in.cnt > kIncWidth)
-            for (int q = a.nk_ptr[k] + kIncWidth; q < a.nk_ptr[k + 1]; q++) add(a.nk_el[q] >> 2, a.nk_el[q] & 3);
-        vfL[l - 1][kk] = vf;
-        vaL[l - 1][kk] = va;
+        for (int l = threadIdx.x / TN + 1; l <= lim; l += lstep) {
+            R vf = 0, va = 0;
+            // layer l of every slot's element loaded at once, then added in
+            // list order
+            R sun[kIncWidth], svn[kIncWidth], suo[kIncWidth], svo[kIncWidth];
+#pragma unroll
+            for (int s = 0; s < kIncWidth; s++) {
+                const size_t o = (size_t)(l - 1) * nel + sie[s];
+                sun[s] = a.utn[o]; svn[s] = a.vtn[o]; suo[s] = a.uto[o]; svo[s] = a.vto[o];
+            }
+            const size_t on = (size_t)(l - 1) * nkn + k;
+            const R ar = a.areakv[on];
+            const R hn = a.hdkn[on], ho = a.hdko[on];
+            const R qq = a.mflux ? a.mflux[on] : (R)0;
+#pragma unroll
+            for (int s = 0; s < kIncWidth; s++) {
+                if (s < in.cnt && slv[s] >= l) {
+                    const R ffn = sun[s] * sb[s] + svn[s] * sc[s];
+                    const R ffo = suo[s] * sb[s] + svo[s] * sc[s];
+                    const R ff = ffn * az + ffo * azt;
+                    vf = vf + (R)3 * saj[s] * ff;
+                    va = va + saj[s];
+                }
+            }
+            if (in.cnt > kIncWidth)
+                for (int q = a.nk_ptr[k] + kIncWidth; q < a.nk_ptr[k + 1]; q++) {
+                    const int ie = a.nk_el[q] >> 2, ii = a.nk_el[q] & 3;
+                    if (a.ilhv[ie] < l) continue;
+                    const R aj = (R)(4. * a.geo[(size_t)6 * nel + ie]);
+                    const R b = (R)a.geo[(size_t)ii * nel + ie], c = (R)a.geo[(size_t)(3 + ii) * nel + ie];
+                    const size_t o = (size_t)(l - 1) * nel + ie;
+                    const R ffn = a.utn[o] * b + a.vtn[o] * c;
+                    const R ffo = a.uto[o] * b + a.vto[o] * c;
+                    const R ff = ffn * az + ffo * azt;
+                    vf = vf + (R)3 * aj * ff;
+                    va = va + aj;
+                }
+            // the column terms of this layer (read here, so that the column
+            // scan below waits on LDS only)
+            const R voln = hn * ar, volo = ho * ar;
+            vfL[l - 1][kk] = vf + qq;
+            vaL[l - 1][kk] = va;
+            dvL[l - 1][kk] = (voln - volo) / dt;
+        }
     }
     __syncthreads();
     const int kk = threadIdx.x, k = k0 + kk;
@@ -945,12 +974,8 @@ __global__ __launch_bounds__(kThreads) void k_hydro_vertical(HydroArgs<S> a) {
     // bottom-up: w(l-1) = w(l) + div - dV/dt, then divide by the top area
     R wb = 0;
     for (int l = lmax; l >= 1; l--) {
-        const size_t o = (size_t)(l - 1) * nkn + k;
-        const R ar = a.areakv[o];
-        const R voln = a.hdkn[o] * ar, volo = a.hdko[o] * ar;
-        const R dvdt = (voln - volo) / dt;
-        const R qq = a.mflux ? a.mflux[o] : (R)0;
-        const R wdiv = vfL[l - 1][kk] + qq;
+        const R wdiv = vfL[l - 1][kk];
+        const R dvdt = dvL[l - 1][kk];
         const R wl = wb + wdiv - dvdt;
         R wout = wl;
         if (l == 1) wout = 0;
