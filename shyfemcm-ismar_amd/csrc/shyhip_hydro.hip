@@ -148,10 +148,29 @@ __global__ __launch_bounds__(kThreads) void k_hydro_intern(HydroArgs<S> a) {
     a.rfric[ie] = rfric;
 
     // layer thickness, inverse thickness, element viscosity
+    // The column's loads are issued at once, unconditionally: rows below the
+    // bottom layer read the bottom layer again (a valid address) and are
+    // replaced by zeros or not used.
     R hact[NL + 2], rhact[NL + 2], alev[NL + 1];
-    hact[0] = 0;
+    R vsum[NL + 1];
+    R uoc[NL], voc[NL];
 #pragma unroll
-    for (int l = 1; l <= NL + 1; l++) hact[l] = (l <= ilevel) ? a.hdeo[(size_t)(l - 1) * nel + ie] : (R)0;
+    for (int l = 1; l <= NL; l++) {
+        const size_t o = (size_t)(min(l, ilevel) - 1) * nel + ie;
+        hact[l] = a.hdeo[o];
+        uoc[l - 1] = a.uto[o];
+        voc[l - 1] = a.vto[o];
+    }
+#pragma unroll
+    for (int l = 0; l <= NL; l++) {
+        const size_t o = (size_t)min(l, ilevel) * nkn;
+        vsum[l] = a.visv[o + k1] + a.visv[o + k2] + a.visv[o + k3];
+    }
+    hact[0] = 0;
+    hact[NL + 1] = 0;
+#pragma unroll
+    for (int l = 1; l <= NL; l++)
+        if (l > ilevel) hact[l] = 0;
     if (a.iclin) hact[1] = (R)(hact[1] - zmm);
 #pragma unroll
     for (int l = 0; l <= NL + 1; l++) rhact[l] = (hact[l] <= (R)0) ? (R)0 : (R)1 / hact[l];
@@ -159,8 +178,7 @@ __global__ __launch_bounds__(kThreads) void k_hydro_intern(HydroArgs<S> a) {
     for (int l = 0; l <= NL; l++) {
         if (l <= ilevel) {
             double vis = (double)a.vismol;
-            const R sum = a.visv[(size_t)l * nkn + k1] + a.visv[(size_t)l * nkn + k2] + a.visv[(size_t)l * nkn + k3];
-            vis = vis + (double)(sum / (R)3);
+            vis = vis + (double)(vsum[l] / (R)3);
             alev[l] = (R)vis;
         } else {
             alev[l] = 0;
@@ -174,12 +192,12 @@ __global__ __launch_bounds__(kThreads) void k_hydro_intern(HydroArgs<S> a) {
         for (int l = 1; l <= NL; l++) {
             if (l > ilevel) continue;
             const bool bfirst = (l == 1), blast = (l == ilevel);
-            const int lp = (l + 1 < ilevel) ? l + 1 : ilevel;
-            const int lm = (l - 1 > 1) ? l - 1 : 1;
-            const R uui = a.uto[(size_t)(l - 1) * nel + ie], uuip = a.uto[(size_t)(lp - 1) * nel + ie];
-            const R uuim = a.uto[(size_t)(lm - 1) * nel + ie];
-            const R vvi = a.vto[(size_t)(l - 1) * nel + ie], vvip = a.vto[(size_t)(lp - 1) * nel + ie];
-            const R vvim = a.vto[(size_t)(lm - 1) * nel + ie];
+            // layers min(l+1, ilevel) and max(l-1, 1), from registers
+            const bool below = l < NL && l + 1 <= ilevel;
+            const R uui = uoc[l - 1], uuip = below ? uoc[l < NL ? l : l - 1] : uoc[l - 1];
+            const R uuim = l > 1 ? uoc[l > 1 ? l - 2 : 0] : uoc[0];
+            const R vvi = voc[l - 1], vvip = below ? voc[l < NL ? l : l - 1] : voc[l - 1];
+            const R vvim = l > 1 ? voc[l > 1 ? l - 2 : 0] : voc[0];
             const R hhi = hact[l], hhip = hact[l + 1], hhim = hact[l - 1];
             R rhp = 0, rhm = 0;
             if (hhi > (R)0) {
