@@ -293,7 +293,7 @@ int shyhip_destroy(shyhip_model *h) {
         dfree(q);
     Solver &v = M.sv;
     for (void *q : {v.lu, v.ell_a, v.r0, v.r, v.p0, v.p1, v.v0, v.v1, v.s, v.t, v.dx, v.b, v.x0, (void *)v.part,
-                    (void *)v.scal, (void *)v.ctrl, v.a32, (void *)v.r64})
+                    (void *)v.scal, (void *)v.ctrl, v.a32, (void *)v.r64, v.zprev})
         dfree(q);
     for (hipStream_t st : M.tr_stream) (void)hipStreamSynchronize(st);
     for (hipStream_t st : M.tr_stream) (void)hipStreamDestroy(st);
@@ -574,6 +574,7 @@ int shyhip_set_basin(shyhip_model *h, const int *nen3v, const float *xgv, const 
         rc |= dalloc(M, &v.ctrl, 16);
         rc |= dalloc(M, &v.r64, (size_t)nkn);
         if (M.prec == SHYHIP_FP32) rc |= dalloc(M, (float **)&v.a32, (size_t)M.nnz);
+        if (M.prec == SHYHIP_FP32) rc |= dalloc(M, (float **)&v.zprev, (size_t)nkn);
     }
     if (rc) return fail(&M, "basin upload failed: " + M.err);
     // defaults: no open boundary, no external flags
@@ -722,6 +723,7 @@ int shyhip_tvd_init(shyhip_model *h) {
 int shyhip_init_state(shyhip_model *h, float zconst) {
     if (!h) return fail(nullptr, "null handle");
     Model &M = h->M;
+    M.sv.have_prev = false;
     HIPCHK(M, hipMemsetAsync(M.d_err, 0, 8 * sizeof(int), M.stream));
     for (void *q : {M.s.utn, M.s.vtn, M.s.uto, M.s.vto})
         if (launch_fill(M, q, 0., (size_t)M.nlvdi * M.nel)) return 1;
@@ -752,6 +754,7 @@ int shyhip_push(shyhip_model *h, int field, const void *host, int dtype, size_t 
     const Shape sh = shape_of(M, f.kind);
     if (n != sh.n) return fail(&M, "push: size mismatch for field " + std::to_string(field));
     if (f.kind == K_NODE3 && field == SHYHIP_MFLUXV) M.has_mflux = true;
+    M.sv.have_prev = false;     // a pushed state starts a new level history
     if (f.isint) {
         if (dtype != SHYHIP_I32) return fail(&M, "integer field needs SHYHIP_I32");
         return upload(M, (int *)*f.ptr, (const int *)host, n);

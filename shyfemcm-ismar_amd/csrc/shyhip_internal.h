@@ -109,6 +109,8 @@ struct Solver {               // Krylov work space (value type V = S of the mode
     double *scal = nullptr;   // [16] device scalars
     int *ctrl = nullptr;      // [16] control words of the solve (flags, counters)
     void *a32 = nullptr;      // FP32 copy of the matrix values (FP32 path)
+    void *zprev = nullptr;    // FP32 path: level at the start of the previous step
+    bool have_prev = false;   // zprev holds a level of this run's time series
     double *r64 = nullptr;    // FP64 residual (refinement)
 };
 

@@ -782,6 +782,20 @@ __global__ void k_guess(const S *zn, double *x, int n, int *ctrl) {
     if (i < n) x[i] = (double)zn[i];
 }
 
+// FP32 path: initial guess extrapolated linearly in time from the levels at
+// the start of this step (zn) and of the previous one (zprev), x = 2 zn -
+// zprev (zn alone without a history); zprev <- zn.  The refinement's target
+// is absolute (a fraction of |b|), so a closer start saves rounds; the
+// solution it converges to does not depend on the start.
+__global__ void k_guess_extrap(const float *zn, float *zprev, double *x, int n, int *ctrl, int have_prev) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) { ctrl[CT_FACTOR] = 0; ctrl[CT_ITS] = 0; ctrl[CT_NREF] = 0; ctrl[CT_REFCONV] = 0; }
+    if (i >= n) return;
+    const double z = (double)zn[i];
+    x[i] = have_prev ? 2. * z - (double)zprev[i] : z;
+    zprev[i] = zn[i];
+}
+
 // FP64 residual r64 = b - A x with partial |r64|^2 and |b|^2 (refinement)
 __global__ __launch_bounds__(kThreads) void k_resid64(const int *rowptr, const int *cols, const double *A,
                                                        const double *b, const double *x, double *r64, float *r32,
@@ -985,8 +999,9 @@ int solver_solve(Model &M, int *iters, int *refinements, int wetdry_bnew) {
         // `chunk` iterations (a partial correction still refines).  The
         // rounds of the previous solve are enqueued at once; the flags are
         // read after them.
-        hipLaunchKernelGGL(k_guess<float>, dim3(g1), dim3(kThreads), 0, M.stream, (const float *)M.s.zn, M.s.x, n,
-                           M.sv.ctrl);
+        hipLaunchKernelGGL(k_guess_extrap, dim3(g1), dim3(kThreads), 0, M.stream, (const float *)M.s.zn,
+                           (float *)M.sv.zprev, M.s.x, n, M.sv.ctrl, M.sv.have_prev ? 1 : 0);
+        M.sv.have_prev = true;
         float *a32 = (float *)M.sv.a32;
         hipLaunchKernelGGL(k_prep_matrix<float>, dim3((M.nnz + kThreads - 1) / kThreads), dim3(kThreads), 0,
                            M.stream, (const double *)M.s.aval, (const int *)M.m.csr_ell, (float *)M.sv.ell_a, a32,
