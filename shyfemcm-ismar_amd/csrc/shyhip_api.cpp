@@ -755,6 +755,7 @@ int shyhip_push(shyhip_model *h, int field, const void *host, int dtype, size_t 
     if (n != sh.n) return fail(&M, "push: size mismatch for field " + std::to_string(field));
     if (f.kind == K_NODE3 && field == SHYHIP_MFLUXV) M.has_mflux = true;
     M.sv.have_prev = false;     // a pushed state starts a new level history
+    M.sv.fed = false;           // and any pushed matrix or level is the solve's input
     if (f.isint) {
         if (dtype != SHYHIP_I32) return fail(&M, "integer field needs SHYHIP_I32");
         return upload(M, (int *)*f.ptr, (const int *)host, n);
@@ -923,20 +924,17 @@ static int hydro_impl(Model &M, double dtime, bool final_check) {
     M.last_nloop = iloop;
     M.last_iters = its_total;
     if (report_dev_err(M, M.h_pinned[0], "hydro loop")) return 3;   // read with the setweg counter
+    // one element kernel: transports_final, setzev + setuvd + baro2l, the
+    // element depths of make_new_depth, compute_velocities (ulnv/vlnv,
+    // unv/vnv) -- then the node depths (+ check_volume) and hydro_vertical
     timer_begin(M, T_FINAL);
-    if (launch_transports_final(M)) return 1;
-    timer_end(M);
-    timer_begin(M, T_SETUVD);
-    if (launch_setzev_setuvd(M)) return 1;          // setzev + setuvd + baro2l
+    if (launch_elem_after_solve(M)) return 1;
     timer_end(M);
     timer_begin(M, T_DEPTH);
-    if (launch_setdepth(M, false)) return 1;        // make_new_depth (+ check_volume)
+    if (launch_setdepth_node(M)) return 1;
     timer_end(M);
     timer_begin(M, T_VERT);
     if (launch_hydro_vertical(M)) return 1;
-    timer_end(M);
-    timer_begin(M, T_TTOV);
-    if (launch_ttov_uvint(M)) return 1;             // compute_velocities: ulnv/vlnv, unv/vnv
     timer_end(M);
     return final_check ? check_dev_err(M, "hydro") : 0;
 }
@@ -1118,6 +1116,7 @@ int shyhip_system_solve(shyhip_model *h, int n, const float *z) {
     if (!h) return fail(nullptr, "null handle");
     Model &M = h->M;
     if (n != M.nkn || M.h_coo.size() != (size_t)M.nnz) return fail(&M, "system_solve: size mismatch or no assembly");
+    M.sv.fed = false;      // the system comes from the host: the solve prepares its own inputs
     if (upload(M, M.s.aval, M.h_coo.data(), M.nnz) || upload(M, M.s.rhs, M.h_rvec.data(), n)) return 1;
     std::vector<double> zz(z, z + n);
     if (upload_s(M, M.s.zn, zz.data(), n)) return 1;     // initial guess

@@ -40,6 +40,12 @@ struct HydroArgs {
     S *taux, *tauy, *rzv, *rqv, *mflux;
     double *zel, *aval, *rhs, *x;
     S *zrow;
+    // inputs of the zeta solve written by the row assembly (null: not fed)
+    S *sv_ell;          // ELL slot values
+    float *sv_a32;      // FP32 copy of the values (FP32 path)
+    float *sv_zprev;    // FP32 path: level of the previous step (extrapolated guess)
+    int sv_have_prev;
+    int *sv_ctrl;
     int *err;
 };
 
@@ -515,6 +521,24 @@ __global__ __launch_bounds__(kThreads) void k_zeta_rows(HydroArgs<S> a) {
     }
     rhs = rhs + (double)(dt * rq);
     a.rhs[k] = rhs;
+    if (a.sv_ell) {
+        // the solve's inputs: the row in the ELL slots (and in FP32), the
+        // initial guess (FP32 path: extrapolated in time), cleared words
+        for (int q = r0; q < r0 + len; q++) {
+            const double v = a.aval[q];
+            if (q - r0 < kEllWidth) a.sv_ell[(size_t)(q - r0) * a.nkn + k] = (S)v;
+            if (a.sv_a32) a.sv_a32[q] = (float)v;
+        }
+        const double z = (double)a.zn[k];
+        if (a.sv_zprev) {
+            a.x[k] = a.sv_have_prev ? 2. * z - (double)a.sv_zprev[k] : z;
+            a.sv_zprev[k] = (float)a.zn[k];
+        } else {
+            a.x[k] = z;
+        }
+        if (k == 0)
+            for (int w = CT_ITS; w <= CT_NREF; w++) a.sv_ctrl[w] = 0;
+    }
 }
 
 // ------------------------------------------------------------------ wet/dry
@@ -637,23 +661,29 @@ __global__ __launch_bounds__(kThreads) void k_setnod(HydroArgs<S> a) {
 // system_get: z = real(x).  With a gate (a solver flag), nothing happens
 // unless the flag is set: the kernel can be enqueued before the host knows
 // whether the solve has finished.
+// With dx (FP64 path) the Krylov correction is added first, x = x + dx
+// (tidycg), and kept in x.
 template <typename S>
-__global__ __launch_bounds__(kThreads) void k_system_get(HydroArgs<S> a, int *cnt, const int *gate) {
+__global__ __launch_bounds__(kThreads) void k_system_get(HydroArgs<S> a, int *cnt, const int *gate,
+                                                         const double *dx) {
     if (gate && !*gate) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k == 0) *cnt = 0;     // counter of the setweg that follows
     if (k >= a.nkn) return;
-    a.zn[k] = (S)a.x[k];
+    double v = a.x[k];
+    if (dx) {
+        v = v + dx[k];
+        a.x[k] = v;
+    }
+    a.zn[k] = (S)v;
 }
 
 // ------------------------------------------------------------------ final transports
 // hydro_transports_final, hydrodynamic.f90:1468-1578; without Coriolis the
 // cross terms ddx(jv), ddy(ju) are zero and are not read
 template <typename S, bool CORI>
-__global__ __launch_bounds__(kThreads) void k_transports_final(HydroArgs<S> a) {
+__device__ inline void transports_final_el(const HydroArgs<S> &a, int ie) {
     using R = S;
-    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ie >= a.nel) return;
     const int nel = a.nel;
     const R dt = (R)a.dt;
     const double beta = (double)(dt * (R)a.grav) * (double)a.am;
@@ -687,16 +717,19 @@ __global__ __launch_bounds__(kThreads) void k_transports_final(HydroArgs<S> a) {
     a.un[ie] = (S)um;
     a.vn[ie] = (S)vm;
 }
+template <typename S, bool CORI>
+__global__ __launch_bounds__(kThreads) void k_transports_final(HydroArgs<S> a) {
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ie < a.nel) transports_final_el<S, CORI>(a, ie);
+}
 
 // ------------------------------------------------------------------ setzev/setuvd/baro2l
 // setzev (wetdry.f90:594), setuvd (wetdry.f90:350-590), baro2l
 // (transforms.f90:437-481).  The layer thickness baro2l reads is the one
 // before make_new_depth, i.e. the old level's.
 template <typename S>
-__global__ __launch_bounds__(kThreads) void k_setzev_setuvd(HydroArgs<S> a) {
+__device__ inline void setzev_setuvd_el(const HydroArgs<S> &a, int ie) {
     using R = S;
-    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ie >= a.nel) return;
     const int nel = a.nel;
     if (a.iweg[ie] == 0) {
         for (int ii = 0; ii < 3; ii++) a.zen[(size_t)ii * nel + ie] = a.zn[a.nen3v[3 * ie + ii]];
@@ -791,14 +824,17 @@ __global__ __launch_bounds__(kThreads) void k_setzev_setuvd(HydroArgs<S> a) {
         a.vtn[(size_t)(l - 1) * nel + ie] = v * weight;
     }
 }
+template <typename S>
+__global__ __launch_bounds__(kThreads) void k_setzev_setuvd(HydroArgs<S> a) {
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ie < a.nel) setzev_setuvd_el<S>(a, ie);
+}
 
 // ------------------------------------------------------------------ setdepth
 // setdepth for z-layers (elems_dealing.f90:930-1156): element thickness
 template <typename S>
-__global__ __launch_bounds__(kThreads) void k_setdepth_elem(HydroArgs<S> a, const S *zen, S *hden) {
+__device__ inline void setdepth_elem_el(const HydroArgs<S> &a, int ie, const S *zen, S *hden) {
     using R = S;
-    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ie >= a.nel) return;
     const int nel = a.nel, lmax = a.ilhv[ie];
     R zmed = 0;
     for (int ii = 0; ii < 3; ii++) zmed = zmed + zen[(size_t)ii * nel + ie];
@@ -821,6 +857,11 @@ __global__ __launch_bounds__(kThreads) void k_setdepth_elem(HydroArgs<S> a, cons
         hden[(size_t)(l - 1) * nel + ie] = h;
     }
     if (bad) atomicCAS(a.err, 0, ERR_SETDEPTH);
+}
+template <typename S>
+__global__ __launch_bounds__(kThreads) void k_setdepth_elem(HydroArgs<S> a, const S *zen, S *hden) {
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ie < a.nel) setdepth_elem_el<S>(a, ie, zen, hden);
 }
 
 // node thickness: volume contributions of the incident elements (element
@@ -1006,9 +1047,7 @@ __global__ __launch_bounds__(kThreads) void k_hydro_vertical(HydroArgs<S> a) {
 
 // ttov (transforms.f90:98-122) + uvint (transforms.f90:336-362)
 template <typename S>
-__global__ __launch_bounds__(kThreads) void k_ttov_uvint(HydroArgs<S> a) {
-    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ie >= a.nel) return;
+__device__ inline void ttov_el(const HydroArgs<S> &a, int ie) {
     const int nel = a.nel;
     double u = 0., v = 0.;
     // layers below the element's bottom keep their zero velocities
@@ -1024,6 +1063,25 @@ __global__ __launch_bounds__(kThreads) void k_ttov_uvint(HydroArgs<S> a) {
     }
     a.un[ie] = (S)u;
     a.vn[ie] = (S)v;
+}
+template <typename S>
+__global__ __launch_bounds__(kThreads) void k_ttov_uvint(HydroArgs<S> a) {
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ie < a.nel) ttov_el<S>(a, ie);
+}
+
+// The element work of hydro after the solve, one work-item per element:
+// hydro_transports_final, setzev + setuvd + baro2l, the element depths of
+// make_new_depth, compute_velocities (ttov + uvint).  Each reads the final
+// node levels and its own element's data only, in the reference's order.
+template <typename S, bool CORI>
+__global__ __launch_bounds__(kThreads) void k_elem_after_solve(HydroArgs<S> a) {
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ie >= a.nel) return;
+    transports_final_el<S, CORI>(a, ie);
+    setzev_setuvd_el<S>(a, ie);
+    setdepth_elem_el<S>(a, ie, a.zen, a.hden);
+    ttov_el<S>(a, ie);
 }
 
 // setznv (wetdry.f90:619-700) at start-up
@@ -1176,21 +1234,28 @@ int launch_hydro_intern(Model &M) { return M.prec == SHYHIP_FP64 ? hydro_intern_
 template <typename S>
 static int zeta_t(Model &M) {
     auto a = make_args<S>(M);
+    a.sv_ell = (S *)M.sv.ell_a;
+    a.sv_a32 = (float *)M.sv.a32;
+    a.sv_zprev = (float *)M.sv.zprev;
+    a.sv_have_prev = M.sv.have_prev ? 1 : 0;
+    a.sv_ctrl = M.sv.ctrl;
     LAUNCH(M, k_zeta_elem<S>, nblocks(M.nel), a);
     LAUNCH(M, k_zeta_rows<S>, nblocks(M.nkn), a);
+    M.sv.fed = true;
+    if (M.sv.zprev) M.sv.have_prev = true;
     return 0;
 }
 int launch_zeta_assemble(Model &M) { return M.prec == SHYHIP_FP64 ? zeta_t<double>(M) : zeta_t<float>(M); }
 
 template <typename S>
-static int sysget_t(Model &M, int bnewtime, const int *gate) {
+static int sysget_t(Model &M, int bnewtime, const int *gate, const double *dx) {
     auto a = make_args<S>(M);
-    LAUNCH(M, k_system_get<S>, nblocks(M.nkn), a, M.d_err + 1, gate);
+    LAUNCH(M, k_system_get<S>, nblocks(M.nkn), a, M.d_err + 1, gate, dx);
     LAUNCH(M, k_setweg<S>, nblocks(M.nel), a, 1, bnewtime, M.d_err + 1, gate);
     return 0;
 }
-int launch_system_get_setweg1(Model &M, int bnewtime, const int *gate) {
-    return M.prec == SHYHIP_FP64 ? sysget_t<double>(M, bnewtime, gate) : sysget_t<float>(M, bnewtime, gate);
+int launch_system_get_setweg1(Model &M, int bnewtime, const int *gate, const double *dx) {
+    return M.prec == SHYHIP_FP64 ? sysget_t<double>(M, bnewtime, gate, dx) : sysget_t<float>(M, bnewtime, gate, dx);
 }
 
 template <typename S>
@@ -1211,31 +1276,45 @@ static int setuvd_t(Model &M) {
 int launch_setzev_setuvd(Model &M) { return M.prec == SHYHIP_FP64 ? setuvd_t<double>(M) : setuvd_t<float>(M); }
 
 template <typename S, int NL>
-static int setdepth_nl(Model &M, bool old_level) {
+static int setdepth_nl(Model &M, bool old_level, bool node_only) {
     auto a = make_args<S>(M);
     const S *zen = old_level ? a.zeo : a.zen;
     S *hden = old_level ? a.hdeo : a.hden;
     S *hdkn = old_level ? a.hdko : a.hdkn;
-    LAUNCH(M, k_setdepth_elem<S>, nblocks(M.nel), a, zen, hden);
+    if (!node_only) LAUNCH(M, k_setdepth_elem<S>, nblocks(M.nel), a, zen, hden);
     LAUNCH(M, (k_setdepth_node<S, NL>), nblocks(M.nkn), a, zen, hdkn);
     return 0;
 }
 template <typename S>
-static int setdepth_t(Model &M, bool o) {
+static int setdepth_t(Model &M, bool o, bool n) {
     switch (M.nl_tmpl) {
-    case 1: return setdepth_nl<S, 1>(M, o);
-    case 2: return setdepth_nl<S, 2>(M, o);
-    case 4: return setdepth_nl<S, 4>(M, o);
-    case 8: return setdepth_nl<S, 8>(M, o);
-    case 12: return setdepth_nl<S, 12>(M, o);
-    case 16: return setdepth_nl<S, 16>(M, o);
-    case 20: return setdepth_nl<S, 20>(M, o);
-    case 24: return setdepth_nl<S, 24>(M, o);
-    default: return setdepth_nl<S, 32>(M, o);
+    case 1: return setdepth_nl<S, 1>(M, o, n);
+    case 2: return setdepth_nl<S, 2>(M, o, n);
+    case 4: return setdepth_nl<S, 4>(M, o, n);
+    case 8: return setdepth_nl<S, 8>(M, o, n);
+    case 12: return setdepth_nl<S, 12>(M, o, n);
+    case 16: return setdepth_nl<S, 16>(M, o, n);
+    case 20: return setdepth_nl<S, 20>(M, o, n);
+    case 24: return setdepth_nl<S, 24>(M, o, n);
+    default: return setdepth_nl<S, 32>(M, o, n);
     }
 }
 int launch_setdepth(Model &M, bool old_level) {
-    return M.prec == SHYHIP_FP64 ? setdepth_t<double>(M, old_level) : setdepth_t<float>(M, old_level);
+    return M.prec == SHYHIP_FP64 ? setdepth_t<double>(M, old_level, false) : setdepth_t<float>(M, old_level, false);
+}
+int launch_setdepth_node(Model &M) {
+    return M.prec == SHYHIP_FP64 ? setdepth_t<double>(M, false, true) : setdepth_t<float>(M, false, true);
+}
+
+template <typename S>
+static int after_solve_t(Model &M) {
+    auto a = make_args<S>(M);
+    if (M.has_coriolis) LAUNCH(M, (k_elem_after_solve<S, true>), nblocks(M.nel), a);
+    else LAUNCH(M, (k_elem_after_solve<S, false>), nblocks(M.nel), a);
+    return 0;
+}
+int launch_elem_after_solve(Model &M) {
+    return M.prec == SHYHIP_FP64 ? after_solve_t<double>(M) : after_solve_t<float>(M);
 }
 
 template <typename S, int NL>
@@ -1331,7 +1410,7 @@ static int init_levels_t(Model &M, float zconst) {
     LAUNCH(M, k_fill<S>, nblocks(3 * M.nel), a.zen, (S)zconst, (size_t)3 * M.nel);
     if (set_dry_t<S>(M, 0, 0, nullptr)) return 1;
     LAUNCH(M, k_setznv<S>, nblocks(M.nkn), a);
-    if (setdepth_t<S>(M, false)) return 1;
+    if (setdepth_t<S>(M, false, false)) return 1;
     return 0;
 }
 int launch_init_levels(Model &M, float zconst) {

@@ -97,6 +97,14 @@ struct State {
     void *hbuf = nullptr;     // conz3d horizontal terms [l][ii][ie]
 };
 
+// zeta-solver control words (Solver::ctrl): iteration (+1) at which the
+// current Krylov solve stopped, its convergence (same stamp) / breakdown /
+// iteration-limit flags, its matvec count, iterations of all inner solves of
+// this zeta solve, FP64 refinement converged, factorisation failure,
+// refinements done; words CT_ITS..CT_NREF are cleared at the start of a solve
+enum { CT_STOPIT = 0, CT_CONV = 1, CT_FAIL = 2, CT_NMV = 3, CT_ITS = 4, CT_REFCONV = 5, CT_FACTOR = 6,
+       CT_NREF = 7, CT_MAXIT = 8, CT_WORDS = 16 };
+
 struct Solver {               // Krylov work space (value type V = S of the model)
     int nblk = 0, brows = 0, max_blk_nnz = 0, max_blk_pairs = 0, nparts = 0, max_elev = 0;
     int last_its[8] = {};     // iteration count of the previous solve, per solve kind
@@ -111,6 +119,8 @@ struct Solver {               // Krylov work space (value type V = S of the mode
     void *a32 = nullptr;      // FP32 copy of the matrix values (FP32 path)
     void *zprev = nullptr;    // FP32 path: level at the start of the previous step
     bool have_prev = false;   // zprev holds a level of this run's time series
+    bool fed = false;         // the assembly already wrote the solve's inputs (ELL
+                              // slots, FP32 copy, initial guess, cleared words)
     double *r64 = nullptr;    // FP64 residual (refinement)
 };
 
@@ -229,12 +239,16 @@ int launch_setnod(Model &M);
 int launch_hydro_intern(Model &M);
 int launch_zeta_assemble(Model &M);
 // system_get + setweg(1), enqueued only: gate (a solver flag, may be null)
-// switches both off unless set; the changed-element count is left in
-// d_err[1] for the caller's read-back
-int launch_system_get_setweg1(Model &M, int bnewtime, const int *gate);
+// switches both off unless set; dx (may be null) is a correction added to
+// the FP64 solution first; the changed-element count is left in d_err[1]
+// for the caller's read-back
+int launch_system_get_setweg1(Model &M, int bnewtime, const int *gate, const double *dx = nullptr);
 int launch_transports_final(Model &M);
 int launch_setzev_setuvd(Model &M);
 int launch_setdepth(Model &M, bool old_level);
+int launch_setdepth_node(Model &M);                 // node depths only (element depths done)
+// transports_final + setzev/setuvd/baro2l + element depths + ttov/uvint, fused
+int launch_elem_after_solve(Model &M);
 int launch_hydro_vertical(Model &M);
 int launch_ttov_uvint(Model &M);
 int launch_init_levels(Model &M, float zconst);

@@ -39,12 +39,6 @@ namespace shy {
 // kernel never overwrites a value that another workgroup of it still reads
 enum { SC_TARGET = 0, SC_NORM0 = 1, SC_RHO = 2, SC_ALPHA = 4, SC_OMEGA = 6, SC_RNORM = 8, SC_RES64 = 9,
        SC_BNORM = 10 };
-// control words: iteration (+1) at which the current Krylov solve stopped,
-// its convergence (same stamp) / breakdown / iteration-limit flags, its
-// matvec count, iterations of all inner solves of this zeta solve, FP64
-// refinement converged, factorisation failure, refinements done
-enum { CT_STOPIT = 0, CT_CONV = 1, CT_FAIL = 2, CT_NMV = 3, CT_ITS = 4, CT_REFCONV = 5, CT_FACTOR = 6,
-       CT_NREF = 7, CT_MAXIT = 8, CT_WORDS = 16 };
 // partial-sum regions: [region][q][workgroup]
 enum { PR_A = 0, PR_B = 1, PR_C = 2, PR_R = 3 };
 
@@ -961,33 +955,40 @@ static int enqueue_inner(Model &M, SolvArgs<V> &a, int chunk) {
 int solver_setup(Model &M) { (void)M; return 0; }
 
 int solver_solve(Model &M, int *iters, int *refinements, int wetdry_bnew) {
+    // inputs already written by the row assembly of this step's system?
+    const bool fed = M.sv.fed;
+    M.sv.fed = false;
     const int n = M.nkn;
     const int g1 = (n + kThreads - 1) / kThreads;
     int it_total = 0, nref = 0;
     // system_get + setweg(1) behind the solve, gated by its convergence flag
-    auto wetdry = [&](const int *gate) -> int {
+    auto wetdry = [&](const int *gate, const double *dx) -> int {
         if (wetdry_bnew < 0) return 0;
         timer_begin(M, T_WETDRY);
-        const int rc = launch_system_get_setweg1(M, wetdry_bnew, gate);
+        const int rc = launch_system_get_setweg1(M, wetdry_bnew, gate, dx);
         timer_end(M);
         return rc;
     };
     if (M.prec == SHYHIP_FP64) {
         // initial guess = current level (system_solve passes znv); the Krylov
         // x0 vector is the FP64 solution array itself
-        hipLaunchKernelGGL(k_guess<double>, dim3(g1), dim3(kThreads), 0, M.stream, (const double *)M.s.zn, M.s.x, n,
-                           M.sv.ctrl);
-        hipLaunchKernelGGL(k_prep_matrix<double>, dim3((M.nnz + kThreads - 1) / kThreads), dim3(kThreads), 0,
-                           M.stream, (const double *)M.s.aval, (const int *)M.m.csr_ell, (double *)M.sv.ell_a,
-                           (float *)nullptr, M.nnz);
+        if (!fed) {
+            hipLaunchKernelGGL(k_guess<double>, dim3(g1), dim3(kThreads), 0, M.stream, (const double *)M.s.zn, M.s.x,
+                               n, M.sv.ctrl);
+            hipLaunchKernelGGL(k_prep_matrix<double>, dim3((M.nnz + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                               M.stream, (const double *)M.s.aval, (const int *)M.m.csr_ell, (double *)M.sv.ell_a,
+                               (float *)nullptr, M.nnz);
+        }
         auto a = solv_args<double>(M, M.s.aval, M.s.rhs, M.s.x);
         if (factor<double>(M, a)) return 1;
         int its = 0;
         auto tail = [&]() -> int {
+            // x = x0 + dx: inside system_get when it follows, else on its own
+            if (wetdry_bnew >= 0) return wetdry(M.sv.ctrl + CT_CONV, a.dx);
             hipLaunchKernelGGL(k_bicg_finish<double>, dim3(g1), dim3(kThreads), 0, M.stream, (const double *)M.s.x,
                                (const double *)a.dx, M.s.x, n, 0, (const int *)M.sv.ctrl);
             HIPCHK(M, hipGetLastError());
-            return wetdry(M.sv.ctrl + CT_CONV);
+            return 0;
         };
         const int rc = bicgstab<double>(M, a, &its, 0, 0, tail);
         it_total = its;
@@ -999,13 +1000,15 @@ int solver_solve(Model &M, int *iters, int *refinements, int wetdry_bnew) {
         // `chunk` iterations (a partial correction still refines).  The
         // rounds of the previous solve are enqueued at once; the flags are
         // read after them.
-        hipLaunchKernelGGL(k_guess_extrap, dim3(g1), dim3(kThreads), 0, M.stream, (const float *)M.s.zn,
-                           (float *)M.sv.zprev, M.s.x, n, M.sv.ctrl, M.sv.have_prev ? 1 : 0);
-        M.sv.have_prev = true;
         float *a32 = (float *)M.sv.a32;
-        hipLaunchKernelGGL(k_prep_matrix<float>, dim3((M.nnz + kThreads - 1) / kThreads), dim3(kThreads), 0,
-                           M.stream, (const double *)M.s.aval, (const int *)M.m.csr_ell, (float *)M.sv.ell_a, a32,
-                           M.nnz);
+        if (!fed) {
+            hipLaunchKernelGGL(k_guess_extrap, dim3(g1), dim3(kThreads), 0, M.stream, (const float *)M.s.zn,
+                               (float *)M.sv.zprev, M.s.x, n, M.sv.ctrl, M.sv.have_prev ? 1 : 0);
+            M.sv.have_prev = true;
+            hipLaunchKernelGGL(k_prep_matrix<float>, dim3((M.nnz + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                               M.stream, (const double *)M.s.aval, (const int *)M.m.csr_ell, (float *)M.sv.ell_a, a32,
+                               M.nnz);
+        }
         float *b32 = (float *)M.sv.b, *x0 = (float *)M.sv.x0;
         auto a = solv_args<float>(M, a32, b32, x0);
         if (factor<float>(M, a)) return 1;
@@ -1031,7 +1034,7 @@ int solver_solve(Model &M, int *iters, int *refinements, int wetdry_bnew) {
                                    (const float *)a.dx, M.s.x, n, 1, (const int *)M.sv.ctrl);
             }
             if (resid()) return 1;
-            if (wetdry(M.sv.ctrl + CT_REFCONV)) return 1;
+            if (wetdry(M.sv.ctrl + CT_REFCONV, nullptr)) return 1;
             if (read_ctrl(M, c, CT_WORDS)) return 1;
             if (c[CT_FACTOR] != 0) {
                         M.err = "zeta solver: zero pivot in the block factorisation (row " + std::to_string(c[CT_FACTOR]) + ")";
