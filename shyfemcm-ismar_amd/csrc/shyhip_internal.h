@@ -102,13 +102,15 @@ struct State {
 // iteration-limit flags, its matvec count, iterations of all inner solves of
 // this zeta solve, FP64 refinement converged, factorisation failure,
 // refinements done; words CT_ITS..CT_NREF are cleared at the start of a solve
+// CT_RITS.. hold the iterations of each refinement round (FP32 path)
 enum { CT_STOPIT = 0, CT_CONV = 1, CT_FAIL = 2, CT_NMV = 3, CT_ITS = 4, CT_REFCONV = 5, CT_FACTOR = 6,
-       CT_NREF = 7, CT_MAXIT = 8, CT_WORDS = 16 };
+       CT_NREF = 7, CT_MAXIT = 8, CT_RITS = 9, CT_NRITS = 6, CT_WORDS = 16 };
 
 struct Solver {               // Krylov work space (value type V = S of the model)
     int nblk = 0, brows = 0, max_blk_nnz = 0, max_blk_pairs = 0, nparts = 0, max_elev = 0;
     int last_its[8] = {};     // iteration count of the previous solve, per solve kind
     int last_nref = 0;        // FP64 refinement rounds of the previous solve
+    int round_its[CT_NRITS] = {};   // iterations of each round of the previous solve
     void *lu = nullptr;       // block factors, fixed stride max_blk_nnz per block
     void *ell_a = nullptr;    // matrix values in the ELL slots (zero padding)
     void *r0 = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr, *v0 = nullptr, *v1 = nullptr,

@@ -815,6 +815,13 @@ __global__ __launch_bounds__(kThreads) void k_resid_check(const double *part, in
     double rs[2];
     sum_partials<2>(part, PR_R, nparts, n, rs);
     if (threadIdx.x == 0) {
+        // iterations of the inner solve just finished (round nref - 1),
+        // negative when it stopped without converging (cut short)
+        const int r = ctrl[CT_NREF];
+        if (r >= 1 && r <= CT_NRITS) {
+            const int its = (ctrl[CT_NMV] - 1) / 2;
+            ctrl[CT_RITS + r - 1] = ctrl[CT_CONV] ? its : -its;
+        }
         scal[SC_RES64] = sqrt(rs[0]);
         scal[SC_BNORM] = sqrt(rs[1]);
         const int conv = (sqrt(rs[0]) <= tol * sqrt(rs[1])) ? 1 : 0;
@@ -1016,7 +1023,15 @@ int solver_solve(Model &M, int *iters, int *refinements, int wetdry_bnew) {
         a.atol = 0.0;
         const int maxref = 12;
         const int itmax_inner = M.p.itermax > 0 ? (M.p.itermax + 1) / 2 : 500;
-        const int chunk = std::min(itmax_inner, std::max(4, M.sv.last_its[1] + 3));
+        // iterations enqueued for round r: that round's count in the previous
+        // solve plus one (kernels beyond convergence return at once but
+        // still cost a launch); a round cut short by its chunk leaves a
+        // partial correction and the next round continues from it
+        auto chunk_of = [&](int r) {
+            const int prev = r < CT_NRITS ? M.sv.round_its[r] : 0;
+            const int c0 = prev > 0 ? prev + 1 : std::max(4, M.sv.last_its[1] + 3);
+            return std::min(itmax_inner, std::max(2, c0));
+        };
         auto resid = [&]() -> int {
             hipLaunchKernelGGL(k_resid64, dim3(g1), dim3(kThreads), 0, M.stream, M.m.rowptr, M.m.cols,
                                (const double *)M.s.aval, (const double *)M.s.rhs, (const double *)M.s.x, M.sv.r64,
@@ -1029,7 +1044,7 @@ int solver_solve(Model &M, int *iters, int *refinements, int wetdry_bnew) {
         int plan = std::max(1, M.sv.last_nref), enq = 0, c[CT_WORDS] = {};
         for (;;) {
             for (int r = 0; r < plan && enq < maxref; r++, enq++) {
-                if (resid() || enqueue_inner<float>(M, a, chunk)) return 1;
+                if (resid() || enqueue_inner<float>(M, a, chunk_of(enq))) return 1;
                 hipLaunchKernelGGL(k_bicg_finish<float>, dim3(g1), dim3(kThreads), 0, M.stream, (const float *)x0,
                                    (const float *)a.dx, M.s.x, n, 1, (const int *)M.sv.ctrl);
             }
@@ -1051,6 +1066,11 @@ int solver_solve(Model &M, int *iters, int *refinements, int wetdry_bnew) {
         it_total = c[CT_ITS];
         M.sv.last_nref = nref;
         M.sv.last_its[1] = nref > 0 ? (it_total + nref - 1) / nref : 0;
+        for (int r = 0; r < CT_NRITS; r++) {
+            const int its_r = r < nref ? c[CT_RITS + r] : 0;
+            // a round cut short by its chunk gets more next time
+            M.sv.round_its[r] = its_r < 0 ? 2 - its_r : its_r;
+        }
     }
     if (iters) *iters = it_total;
     if (refinements) *refinements = nref;
