@@ -984,11 +984,11 @@ static int tracer_streams(Model &M) {
 int shyhip_step(shyhip_model *h, double dtime, int istot) {
     if (!h) return fail(nullptr, "null handle");
     Model &M = h->M;
+    // set_dry (setweg 2 and 3) and copy_uvz/copy_depth in one launch
     timer_begin(M, T_SETDRY);
-    if (shyhip_set_dry(h)) return 1;
-    timer_end(M);
-    timer_begin(M, T_COPY);
-    if (launch_copy_old(M)) return 1;
+    const int bnew = M.dtime != M.dtime_old;
+    if (launch_step_start(M, bnew)) return 1;
+    M.dtime_old = M.dtime;
     timer_end(M);
     if (hydro_impl(M, dtime, false)) return M.err.empty() ? 1 : 3;
     if (M.ntr > 1 && !M.timing) {

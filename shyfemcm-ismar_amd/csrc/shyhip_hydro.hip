@@ -544,16 +544,13 @@ __global__ __launch_bounds__(kThreads) void k_zeta_rows(HydroArgs<S> a) {
 // ------------------------------------------------------------------ wet/dry
 // setweg(iweich), wetdry.f90:128-347, one work-item per element; the number
 // of elements that changed state is accumulated in *cnt.
+// one setweg(iweich) update of element ie's state (weg, wet); returns the
+// number of state changes it counts
 template <typename S>
-__global__ __launch_bounds__(kThreads) void k_setweg(HydroArgs<S> a, int iweich, int bnewtime, int *cnt,
-                                                     const int *gate) {
+__device__ inline int setweg_el(const HydroArgs<S> &a, int ie, int iweich, int bnewtime, int &weg, int &wet) {
     using R = S;
-    if (gate && !*gate) return;
-    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ie >= a.nel) return;
     const int nel = a.nel;
     int iwh = 0;
-    int weg = a.iweg[ie], wet = a.iwet[ie];
     const int kn[3] = {a.nen3v[3 * ie], a.nen3v[3 * ie + 1], a.nen3v[3 * ie + 2]};
     if (iweich == -1) {
         weg = 0;
@@ -606,9 +603,39 @@ __global__ __launch_bounds__(kThreads) void k_setweg(HydroArgs<S> a, int iweich,
         if (weg == 0) wet++;
         else wet--;
     }
+    return iwh;
+}
+
+template <typename S>
+__global__ __launch_bounds__(kThreads) void k_setweg(HydroArgs<S> a, int iweich, int bnewtime, int *cnt,
+                                                     const int *gate) {
+    if (gate && !*gate) return;
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ie >= a.nel) return;
+    int weg = a.iweg[ie], wet = a.iwet[ie];
+    const int iwh = setweg_el<S>(a, ie, iweich, bnewtime, weg, wet);
     a.iweg[ie] = weg;
     a.iwet[ie] = wet;
     if (iwh && cnt) atomicAdd(cnt, iwh);
+}
+
+// The start of a step in one launch: set_dry (setweg(2) then setweg(3),
+// per element), then the copies of copy_uvz/copy_depth that follow it
+// (zov <- znv per node, zeov <- zenv per element); work-item i handles
+// element i and node i.
+template <typename S>
+__global__ __launch_bounds__(kThreads) void k_step_start(HydroArgs<S> a, int bnewtime) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t nel = a.nel;
+    if (i < a.nel) {
+        int weg = a.iweg[i], wet = a.iwet[i];
+        setweg_el<S>(a, i, 2, bnewtime, weg, wet);
+        setweg_el<S>(a, i, 3, 0, weg, wet);
+        a.iweg[i] = weg;
+        a.iwet[i] = wet;
+        for (int ii = 0; ii < 3; ii++) a.zeo[ii * nel + i] = a.zen[ii * nel + i];
+    }
+    if (i < a.nkn) a.zo[i] = a.zn[i];
 }
 
 // setnod (topo_update.f90:166-270) + adjust_mass_flux (bnd_routines.f90:1185)
@@ -1369,6 +1396,23 @@ static int copy_old_t(Model &M) {
     return 0;
 }
 int launch_copy_old(Model &M) { return M.prec == SHYHIP_FP64 ? copy_old_t<double>(M) : copy_old_t<float>(M); }
+
+template <typename S>
+static int step_start_t(Model &M, int bnewtime) {
+    std::swap(M.s.utn, M.s.uto);
+    std::swap(M.s.vtn, M.s.vto);
+    std::swap(M.s.un, M.s.uo);
+    std::swap(M.s.vn, M.s.vo);
+    std::swap(M.s.wln, M.s.wlo);
+    std::swap(M.s.hdkn, M.s.hdko);
+    std::swap(M.s.hden, M.s.hdeo);
+    auto a = make_args<S>(M);
+    LAUNCH(M, k_step_start<S>, nblocks(std::max(M.nel, M.nkn)), a, bnewtime);
+    return 0;
+}
+int launch_step_start(Model &M, int bnewtime) {
+    return M.prec == SHYHIP_FP64 ? step_start_t<double>(M, bnewtime) : step_start_t<float>(M, bnewtime);
+}
 
 template <typename S>
 static int set_rzv_t(Model &M, int ibc, float level) {

@@ -255,6 +255,8 @@ int launch_hydro_vertical(Model &M);
 int launch_ttov_uvint(Model &M);
 int launch_init_levels(Model &M, float zconst);
 int launch_copy_old(Model &M);
+// set_dry + copy_uvz/copy_depth of a time step in one launch
+int launch_step_start(Model &M, int bnewtime);
 int launch_set_rzv(Model &M, int ibc, float level);
 int launch_set_rzv_all(Model &M);
 int launch_fill(Model &M, void *ptr, double value, size_t n);
