@@ -70,6 +70,8 @@ int fail(Model *M, const std::string &msg, int code = 1) {
     return code;
 }
 
+int pipe_drain(Model &M);
+
 template <typename T>
 int dalloc(Model &M, T **p, size_t n) {
     *p = nullptr;
@@ -191,6 +193,17 @@ Shape shape_of(Model &M, Kind k) {
     return {0, 0, 0};
 }
 
+// wait for any transport still running (every entry point but the step
+// and the forcing setters calls this first)
+int pipe_drain(Model &M) {
+    if (!M.pipe_live) return 0;
+    if (M.tr0_stream) HIPCHK(M, hipStreamSynchronize(M.tr0_stream));
+    for (hipStream_t st : M.tr_stream) HIPCHK(M, hipStreamSynchronize(st));
+    M.pipe_live = false;
+    M.pipe_rec[0] = M.pipe_rec[1] = false;
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -300,6 +313,13 @@ int shyhip_destroy(shyhip_model *h) {
     for (hipEvent_t ev : M.tr_done) (void)hipEventDestroy(ev);
     if (M.ev_hydro) (void)hipEventDestroy(M.ev_hydro);
     if (M.ev_levels) (void)hipEventDestroy(M.ev_levels);
+    if (M.tr0_stream) (void)hipStreamSynchronize(M.tr0_stream);
+    if (M.tr0_stream) (void)hipStreamDestroy(M.tr0_stream);
+    for (int p = 0; p < 2; p++)
+        for (hipEvent_t ev : M.pipe_ev[p]) (void)hipEventDestroy(ev);
+    for (void *q : {M.spare_ut, M.spare_vt, M.spare_wl, M.spare_hdk, M.spare_hde, M.uln_b, M.vln_b,
+                    (void *)M.iweg_tr[0], (void *)M.iweg_tr[1]})
+        dfree(q);
     for (void *q : M.tr_hbuf) dfree(q);
     dfree(M.d_err);
     dfree(M.d_levels);
@@ -334,6 +354,7 @@ int shyhip_set_params(shyhip_model *h, const shyhip_params *p) {
 int shyhip_set_basin(shyhip_model *h, const int *nen3v, const float *xgv, const float *ygv, const float *hm3v,
                      const float *hev, const float *hlv, const float *hldv, const int *ilhv, const int *ilhkv) {
     if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
     Model &M = h->M;
     const int nkn = M.nkn, nel = M.nel, L = M.nlvdi;
     M.h_nen3v.resize(3 * (size_t)nel);
@@ -589,6 +610,7 @@ int shyhip_set_basin(shyhip_model *h, const int *nen3v, const float *xgv, const 
 
 int shyhip_set_boundaries(shyhip_model *h, int nbnd, const int *ibnd, const int *ibtyp, const int *iextb) {
     if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
     Model &M = h->M;
     if (!M.m.ibnd) return fail(&M, "call shyhip_set_basin first");
     const int nkn = M.nkn;
@@ -658,6 +680,7 @@ int shyhip_set_boundaries(shyhip_model *h, int nbnd, const int *ibnd, const int 
 
 int shyhip_set_coefficients(shyhip_model *h, const float *fcorv, const float *czv) {
     if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
     Model &M = h->M;
     if (!M.m.fcorv) return fail(&M, "call shyhip_set_basin first");
     std::vector<double> t(M.nel, 0.);
@@ -683,6 +706,7 @@ int shyhip_set_coefficients(shyhip_model *h, const float *fcorv, const float *cz
 
 int shyhip_tvd_init(shyhip_model *h) {
     if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
     Model &M = h->M;
     // upwind tables per member (a member's upwind points lie in its own elements)
     const size_t nel = M.nel;
@@ -722,6 +746,7 @@ int shyhip_tvd_init(shyhip_model *h) {
 
 int shyhip_init_state(shyhip_model *h, float zconst) {
     if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
     Model &M = h->M;
     M.sv.have_prev = false;
     HIPCHK(M, hipMemsetAsync(M.d_err, 0, 8 * sizeof(int), M.stream));
@@ -748,6 +773,7 @@ int shyhip_init_state(shyhip_model *h, float zconst) {
 
 int shyhip_push(shyhip_model *h, int field, const void *host, int dtype, size_t n) {
     if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
     Model &M = h->M;
     Field f;
     if (!field_of(M, field, f)) return fail(&M, "unknown field id " + std::to_string(field));
@@ -772,6 +798,7 @@ int shyhip_push(shyhip_model *h, int field, const void *host, int dtype, size_t 
 
 int shyhip_pull(shyhip_model *h, int field, void *host, int dtype, size_t n) {
     if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
     Model &M = h->M;
     Field f;
     if (!field_of(M, field, f)) return fail(&M, "unknown field id " + std::to_string(field));
@@ -821,6 +848,7 @@ int shyhip_set_boundary_levels(shyhip_model *h, int n, const float *levels) {
 
 int shyhip_set_members(shyhip_model *h, int nmem, const int *node_start, const int *elem_start) {
     if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
     Model &M = h->M;
     if (M.m.nen3v) return fail(&M, "set_members must precede set_basin");
     if (nmem < 1 || node_start[0] != 0 || elem_start[0] != 0 || node_start[nmem] != M.nkn ||
@@ -843,6 +871,7 @@ int shyhip_set_wind_uniform(shyhip_model *h, float taux, float tauy) {
 
 int shyhip_set_dry(shyhip_model *h) {
     if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
     Model &M = h->M;
     const int bnew = M.dtime != M.dtime_old;
     if (launch_set_dry(M, 2, bnew, nullptr)) return 1;
@@ -853,16 +882,30 @@ int shyhip_set_dry(shyhip_model *h) {
 
 int shyhip_copy_uvz(shyhip_model *h) {
     if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
     return launch_copy_old(h->M);
 }
 
-int shyhip_hydro_transports(shyhip_model *h) { return h ? launch_hydro_intern(h->M) : fail(nullptr, "null handle"); }
-int shyhip_hydro_zeta(shyhip_model *h) { return h ? launch_zeta_assemble(h->M) : fail(nullptr, "null handle"); }
-int shyhip_hydro_transports_final(shyhip_model *h) { return h ? launch_transports_final(h->M) : fail(nullptr, "null handle"); }
-int shyhip_hydro_vertical(shyhip_model *h) { return h ? launch_hydro_vertical(h->M) : fail(nullptr, "null handle"); }
+int shyhip_hydro_transports(shyhip_model *h) {
+    if (!h) return fail(nullptr, "null handle");
+    return pipe_drain(h->M) || launch_hydro_intern(h->M);
+}
+int shyhip_hydro_zeta(shyhip_model *h) {
+    if (!h) return fail(nullptr, "null handle");
+    return pipe_drain(h->M) || launch_zeta_assemble(h->M);
+}
+int shyhip_hydro_transports_final(shyhip_model *h) {
+    if (!h) return fail(nullptr, "null handle");
+    return pipe_drain(h->M) || launch_transports_final(h->M);
+}
+int shyhip_hydro_vertical(shyhip_model *h) {
+    if (!h) return fail(nullptr, "null handle");
+    return pipe_drain(h->M) || launch_hydro_vertical(h->M);
+}
 
 int shyhip_system_solve_device(shyhip_model *h, int *iters) {
     if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
     Model &M = h->M;
     int ref = 0;
     const int rc = solver_solve(M, iters, &ref);
@@ -941,6 +984,7 @@ static int hydro_impl(Model &M, double dtime, bool final_check) {
 
 int shyhip_hydro(shyhip_model *h, double dtime) {
     if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
     return hydro_impl(h->M, dtime, true);
 }
 
@@ -960,6 +1004,7 @@ static int scal3sh_on(Model &M, int itr, int istot, int clip_small, hipStream_t 
 
 int shyhip_scal3sh(shyhip_model *h, int itr, int istot, int clip_small) {
     if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
     return scal3sh_on(h->M, itr, istot, clip_small, h->M.stream, h->M.s.hbuf);
 }
 
@@ -981,16 +1026,69 @@ static int tracer_streams(Model &M) {
     return 0;
 }
 
+// buffers, streams and events of the pipelined transport (created once)
+static int pipe_setup(Model &M) {
+    if (M.tr0_stream) return 0;
+    if (tracer_streams(M)) return 1;
+    if (!M.ev_hydro) HIPCHK(M, hipEventCreateWithFlags(&M.ev_hydro, hipEventDisableTiming));
+    HIPCHK(M, hipStreamCreate(&M.tr0_stream));
+    for (int p = 0; p < 2; p++)
+        for (int t = 0; t < M.ntr; t++) {
+            hipEvent_t ev = nullptr;
+            HIPCHK(M, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            M.pipe_ev[p].push_back(ev);
+        }
+    const size_t L = M.nlvdi, ne = M.nel, nk = M.nkn;
+    int rc = 0;
+    rc |= dalloc_s(M, &M.spare_ut, L * ne) || dalloc_s(M, &M.spare_vt, L * ne) || dalloc_s(M, &M.spare_hde, L * ne);
+    rc |= dalloc_s(M, &M.spare_wl, (L + 1) * nk) || dalloc_s(M, &M.spare_hdk, L * nk);
+    rc |= dalloc_s(M, &M.uln_b, L * ne) || dalloc_s(M, &M.vln_b, L * ne);
+    rc |= dalloc(M, &M.iweg_tr[0], ne) || dalloc(M, &M.iweg_tr[1], ne);
+    return rc;
+}
+
+
 int shyhip_step(shyhip_model *h, double dtime, int istot) {
     if (!h) return fail(nullptr, "null handle");
     Model &M = h->M;
+    const bool pipe = M.deferred && !M.timing && !M.has_mflux && M.ntr >= 1;
+    int par = 0;
+    if (!pipe) {
+        if (pipe_drain(M)) return 1;
+    } else {
+        if (pipe_setup(M)) return 1;
+        par = M.pipe_k & 1;
+        // the transport of two steps back read the buffers this step rewrites
+        if (M.pipe_rec[par])
+            for (hipEvent_t ev : M.pipe_ev[par]) HIPCHK(M, hipStreamWaitEvent(M.stream, ev, 0));
+    }
+    M.iweg_snap = pipe ? M.iweg_tr[par] : nullptr;
     // set_dry (setweg 2 and 3) and copy_uvz/copy_depth in one launch
     timer_begin(M, T_SETDRY);
     const int bnew = M.dtime != M.dtime_old;
-    if (launch_step_start(M, bnew)) return 1;
+    if (launch_step_start(M, bnew, pipe)) return 1;
     M.dtime_old = M.dtime;
     timer_end(M);
     if (hydro_impl(M, dtime, false)) return M.err.empty() ? 1 : 3;
+    M.iweg_snap = nullptr;
+    if (pipe) {
+        // every tracer on its own stream after this step's hydro; the model's
+        // stream goes on with the next step
+        HIPCHK(M, hipEventRecord(M.ev_hydro, M.stream));
+        M.iweg_conz = M.iweg_tr[par];
+        for (int t = 0; t < M.ntr; t++) {
+            hipStream_t st = t == 0 ? M.tr0_stream : M.tr_stream[t - 1];
+            void *hb = t == 0 ? M.s.hbuf : M.tr_hbuf[t - 1];
+            HIPCHK(M, hipStreamWaitEvent(st, M.ev_hydro, 0));
+            if (scal3sh_on(M, t, istot, t == 0 ? 0 : 1, st, hb)) { M.iweg_conz = nullptr; return 1; }
+            HIPCHK(M, hipEventRecord(M.pipe_ev[par][t], st));
+        }
+        M.iweg_conz = nullptr;
+        M.pipe_rec[par] = true;
+        M.pipe_live = true;
+        M.pipe_k++;
+        return 0;
+    }
     if (M.ntr > 1 && !M.timing) {
         // the tracers are transported concurrently: each after hydro, each
         // on its own stream, joined back into the model's stream
@@ -1014,12 +1112,14 @@ int shyhip_step(shyhip_model *h, double dtime, int istot) {
 
 int shyhip_set_deferred_checks(shyhip_model *h, int on) {
     if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
     h->M.deferred = on != 0;
     return 0;
 }
 
 int shyhip_set_timing(shyhip_model *h, int on) {
     if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
     Model &M = h->M;
     if (timer_flush(M)) return 1;
     M.timing = on != 0;
@@ -1043,17 +1143,20 @@ int shyhip_get_timing(shyhip_model *h, int id, const char **name, double *total_
 
 int shyhip_sync(shyhip_model *h) {
     if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
     HIPCHK(h->M, hipStreamSynchronize(h->M.stream));
     return check_dev_err(h->M, "sync");
 }
 
 int shyhip_volume(shyhip_model *h, int mode, double *vol) {
     if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
     return launch_volume(h->M, mode, vol);
 }
 
 int shyhip_tracer_mass(shyhip_model *h, int itr, double *mass) {
     if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
     if (itr < 0 || itr >= h->M.ntr) return fail(&h->M, "tracer index out of range");
     return launch_tracer_mass(h->M, itr, mass);
 }
@@ -1068,6 +1171,7 @@ int shyhip_stats(shyhip_model *h, int *nloop, int *solver_iters, int *refinement
 
 int shyhip_get_pattern(shyhip_model *h, int *nnz, int *ntg, int *cols, int *ncolor, int *color) {
     if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
     Model &M = h->M;
     if (nnz) *nnz = M.nnz;
     if (ncolor) *ncolor = M.ncolor;
@@ -1080,6 +1184,7 @@ int shyhip_get_pattern(shyhip_model *h, int *nnz, int *ntg, int *cols, int *ncol
 // ---- zeta-system plugin (simsys_spkaux.f90 interface) ----------------------
 int shyhip_system_init(shyhip_model *h) {
     if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
     Model &M = h->M;
     M.h_coo.assign(M.nnz, 0.);
     M.h_rvec.assign(M.nkn, 0.);
@@ -1088,6 +1193,7 @@ int shyhip_system_init(shyhip_model *h) {
 
 int shyhip_system_assemble(shyhip_model *h, int ie, const int *kn, const float *hia, const float *hik) {
     if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
     Model &M = h->M;
     if (ie < 1 || ie > M.nel) return fail(&M, "system_assemble: element out of range");
     if (M.h_coo.size() != (size_t)M.nnz) return fail(&M, "system_assemble before system_init");
@@ -1106,6 +1212,7 @@ int shyhip_system_assemble(shyhip_model *h, int ie, const int *kn, const float *
 
 int shyhip_system_add_rhs(shyhip_model *h, float dt, int n, const float *array) {
     if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
     Model &M = h->M;
     if (n != M.nkn || M.h_rvec.size() != (size_t)n) return fail(&M, "system_add_rhs: size mismatch");
     for (int k = 0; k < n; k++) M.h_rvec[k] = M.h_rvec[k] + (double)(dt * array[k]);
@@ -1114,6 +1221,7 @@ int shyhip_system_add_rhs(shyhip_model *h, float dt, int n, const float *array) 
 
 int shyhip_system_solve(shyhip_model *h, int n, const float *z) {
     if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
     Model &M = h->M;
     if (n != M.nkn || M.h_coo.size() != (size_t)M.nnz) return fail(&M, "system_solve: size mismatch or no assembly");
     M.sv.fed = false;      // the system comes from the host: the solve prepares its own inputs
@@ -1126,6 +1234,7 @@ int shyhip_system_solve(shyhip_model *h, int n, const float *z) {
 
 int shyhip_system_get(shyhip_model *h, int n, float *z) {
     if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
     Model &M = h->M;
     if (n != M.nkn) return fail(&M, "system_get: size mismatch");
     std::vector<double> x(n);

@@ -558,7 +558,8 @@ static int bndo_t(Model &M, int itr, hipStream_t st) {
     if (M.nlv > kBndoLanes) return hip_check(M, hipErrorInvalidValue, "bndo: more layers than lanes");
     hipLaunchKernelGGL(k_bndo<S>, dim3(M.m.nbndo), dim3(kBndoLanes), 0, st, M.m.nbndo, M.m.bndo_k,
                        (const S *)M.m.bndo_nx, (const S *)M.m.bndo_ny, M.m.bndo_ptr, M.m.bndo_nb,
-                       (const S *)M.m.bndo_w, M.m.ibnd, M.m.ibtyp, M.m.ilhkv, M.m.ilhv, M.s.iweg, M.m.nk_ptr,
+                       (const S *)M.m.bndo_w, M.m.ibnd, M.m.ibtyp, M.m.ilhkv, M.m.ilhv,
+                       M.iweg_conz ? M.iweg_conz : M.s.iweg, M.m.nk_ptr,
                        M.m.nk_el, M.m.geo, (const S *)M.s.uln, (const S *)M.s.vln, (S *)M.s.tracer[itr],
                        (const S *)M.s.tracer_bc[itr], M.nkn, M.nel, M.d_err);
     HIPCHK(M, hipGetLastError());

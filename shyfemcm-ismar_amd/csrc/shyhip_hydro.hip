@@ -41,6 +41,7 @@ struct HydroArgs {
     double *zel, *aval, *rhs, *x;
     S *zrow;
     // inputs of the zeta solve written by the row assembly (null: not fed)
+    int *iweg_snap;     // copy of the final wet/dry mask for the transport (may be null)
     S *sv_ell;          // ELL slot values
     float *sv_a32;      // FP32 copy of the values (FP32 path)
     float *sv_zprev;    // FP32 path: level of the previous step (extrapolated guess)
@@ -1109,6 +1110,7 @@ __global__ __launch_bounds__(kThreads) void k_elem_after_solve(HydroArgs<S> a) {
     setzev_setuvd_el<S>(a, ie);
     setdepth_elem_el<S>(a, ie, a.zen, a.hden);
     ttov_el<S>(a, ie);
+    if (a.iweg_snap) a.iweg_snap[ie] = a.iweg[ie];
 }
 
 // setznv (wetdry.f90:619-700) at start-up
@@ -1336,6 +1338,7 @@ int launch_setdepth_node(Model &M) {
 template <typename S>
 static int after_solve_t(Model &M) {
     auto a = make_args<S>(M);
+    a.iweg_snap = M.iweg_snap;
     if (M.has_coriolis) LAUNCH(M, (k_elem_after_solve<S, true>), nblocks(M.nel), a);
     else LAUNCH(M, (k_elem_after_solve<S, false>), nblocks(M.nel), a);
     return 0;
@@ -1397,21 +1400,39 @@ static int copy_old_t(Model &M) {
 }
 int launch_copy_old(Model &M) { return M.prec == SHYHIP_FP64 ? copy_old_t<double>(M) : copy_old_t<float>(M); }
 
+// new <- spare, old <- new, spare <- old
+static void rotate3(void *&nw, void *&old, void *&spare) {
+    void *t = old;
+    old = nw;
+    nw = spare;
+    spare = t;
+}
+
 template <typename S>
-static int step_start_t(Model &M, int bnewtime) {
-    std::swap(M.s.utn, M.s.uto);
-    std::swap(M.s.vtn, M.s.vto);
+static int step_start_t(Model &M, int bnewtime, bool rot) {
+    if (rot) {
+        rotate3(M.s.utn, M.s.uto, M.spare_ut);
+        rotate3(M.s.vtn, M.s.vto, M.spare_vt);
+        rotate3(M.s.wln, M.s.wlo, M.spare_wl);
+        rotate3(M.s.hdkn, M.s.hdko, M.spare_hdk);
+        rotate3(M.s.hden, M.s.hdeo, M.spare_hde);
+        std::swap(M.s.uln, M.uln_b);
+        std::swap(M.s.vln, M.vln_b);
+    } else {
+        std::swap(M.s.utn, M.s.uto);
+        std::swap(M.s.vtn, M.s.vto);
+        std::swap(M.s.wln, M.s.wlo);
+        std::swap(M.s.hdkn, M.s.hdko);
+        std::swap(M.s.hden, M.s.hdeo);
+    }
     std::swap(M.s.un, M.s.uo);
     std::swap(M.s.vn, M.s.vo);
-    std::swap(M.s.wln, M.s.wlo);
-    std::swap(M.s.hdkn, M.s.hdko);
-    std::swap(M.s.hden, M.s.hdeo);
     auto a = make_args<S>(M);
     LAUNCH(M, k_step_start<S>, nblocks(std::max(M.nel, M.nkn)), a, bnewtime);
     return 0;
 }
-int launch_step_start(Model &M, int bnewtime) {
-    return M.prec == SHYHIP_FP64 ? step_start_t<double>(M, bnewtime) : step_start_t<float>(M, bnewtime);
+int launch_step_start(Model &M, int bnewtime, bool rot) {
+    return M.prec == SHYHIP_FP64 ? step_start_t<double>(M, bnewtime, rot) : step_start_t<float>(M, bnewtime, rot);
 }
 
 template <typename S>

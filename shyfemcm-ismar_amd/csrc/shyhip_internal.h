@@ -164,6 +164,21 @@ struct Model {
     std::vector<double> h_coo, h_rvec;    // plugin accumulation (c2coo, rvec2d)
     size_t esize() const { return prec == SHYHIP_FP64 ? 8 : 4; }
     bool deferred = false;    // shyhip_step does not wait for the device at its end
+    // Pipelined transport (deferred checks, no timing, no mass-flux field):
+    // the tracers of step k are transported on streams of their own while
+    // step k+1's hydro runs.  The arrays the transport reads are rotated over
+    // three buffers (two for ulnv/vlnv and the wet/dry mask it sees), so a
+    // step rewrites only buffers whose readers, two steps back, are done.
+    bool pipe_live = false;            // a transport may still be running
+    int pipe_k = 0;                    // pipelined step counter (parity)
+    bool pipe_rec[2] = {false, false}; // events of parity p recorded
+    hipStream_t tr0_stream = nullptr;  // tracer 0's stream when pipelined
+    std::vector<hipEvent_t> pipe_ev[2];
+    void *spare_ut = nullptr, *spare_vt = nullptr, *spare_wl = nullptr, *spare_hdk = nullptr, *spare_hde = nullptr;
+    void *uln_b = nullptr, *vln_b = nullptr;
+    int *iweg_tr[2] = {nullptr, nullptr};
+    int *iweg_snap = nullptr;          // the element kernel of this step copies iwegv here
+    const int *iweg_conz = nullptr;    // mask the open-boundary treatment reads (null: iwegv)
     hipEvent_t ev_levels = nullptr;   // the last upload from h_levels
     // timing
     bool timing = false;
@@ -255,8 +270,9 @@ int launch_hydro_vertical(Model &M);
 int launch_ttov_uvint(Model &M);
 int launch_init_levels(Model &M, float zconst);
 int launch_copy_old(Model &M);
-// set_dry + copy_uvz/copy_depth of a time step in one launch
-int launch_step_start(Model &M, int bnewtime);
+// set_dry + copy_uvz/copy_depth of a time step in one launch; rotate3:
+// the arrays of the pipelined transport rotate over three buffers
+int launch_step_start(Model &M, int bnewtime, bool rotate3 = false);
 int launch_set_rzv(Model &M, int ibc, float level);
 int launch_set_rzv_all(Model &M);
 int launch_fill(Model &M, void *ptr, double value, size_t n);

@@ -1,7 +1,10 @@
 """Deferred checks through the C ABI: steps that do not wait for the device
 at their end (shyhip_set_deferred_checks) run the same kernels in the same
-order, so every field is bit-identical to steps that wait; the ensemble
-path (pinned boundary-level staging) is covered as well."""
+order, so every field is bit-identical to steps that wait.  In this mode a
+step's tracer transport runs on streams of its own while the next step's
+hydro runs (the arrays it reads rotate over three buffers): the cases
+below cover one and two tracers, switching the mode during a run, pulls
+between steps, and the ensemble path (pinned boundary-level staging)."""
 import numpy as np
 import pytest
 
@@ -58,3 +61,47 @@ def test_deferred_ensemble_levels(gpu):
         ens.close()
     for a, c in zip(*res):
         assert np.array_equal(a, c)
+
+
+def _lag():
+    b = mesh.lagoon(2500, seed=41, lx=7000.0, ly=3500.0, nlv=5, tile=16)
+    return b, dict(salt=mesh.smooth_tracer(b, 42, 25.0, 38.0))
+
+
+def _lev(t):
+    return mesh.tide_level(t, amps=(0.4, 0.1), periods=(mesh.M2_PERIOD, mesh.K1_PERIOD), phases=(0.3, 0.7))
+
+
+def _steps(m, t0, n, pulls=False):
+    t, seen = t0, []
+    for _ in range(n):
+        t += PARAMS["dt"]
+        m.step(_lev(t))
+        if pulls:
+            seen.append(m.pull("salt"))       # waits for the transport still running
+    return t, seen
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_pipelined_one_tracer_modes_and_pulls(gpu, prec):
+    b, tr = _lag()
+    ref = gpu.Model(b, prec, params=PARAMS, visv=1e-3, difv=1e-4, tracers=tr)
+    ref.set_wind(*mesh.wind_stress(12.0, 1.1))
+    _, ref_seen = _steps(ref, 0.0, 18, pulls=True)
+    ref_end = {k: ref.pull(k) for k in ("znv", "utlnv", "salt")}
+    ref.close()
+    m = gpu.Model(b, prec, params=PARAMS, visv=1e-3, difv=1e-4, tracers=tr)
+    m.set_wind(*mesh.wind_stress(12.0, 1.1))
+    m.set_deferred_checks(True)
+    t, seen = _steps(m, 0.0, 6, pulls=True)          # pipelined, drained by every pull
+    t, _ = _steps(m, t, 5)                            # pipelined, nothing waits
+    m.set_deferred_checks(False)                      # drains; buffers back to two
+    t, _ = _steps(m, t, 3)
+    m.set_deferred_checks(True)
+    t, _ = _steps(m, t, 4)
+    m.sync()
+    for a, c in zip(seen, ref_seen[:6]):
+        assert np.array_equal(a, c)
+    for k, v in ref_end.items():
+        assert np.array_equal(m.pull(k), v), k
+    m.close()
