@@ -206,7 +206,7 @@ def ensemble_members(args):
 
 def run_ensemble(mems, prec, steps, warmup):
     from shyhip import ensemble
-    ens = ensemble.Ensemble(mems, prec, params=PARAMS, visv=1e-3, difv=1e-4)
+    ens = ensemble.Ensemble(mems, prec, params=PARAMS, visv=1e-3, difv=1e-4, deferred=True)
     for _ in range(warmup):
         ens.step()
     ens.sync()
@@ -227,6 +227,7 @@ def run_members_alone(mems, prec, steps, warmup):
     tot_el, tot_t = 0, 0.0
     for mem in mems:
         m = shyhip.Model(mem["basin"], prec, params=PARAMS, visv=1e-3, difv=1e-4, tracers=mem["tracers"])
+        m.set_deferred_checks(True)
         m.set_wind(*mem["wind"])
         t = 0.0
         for _ in range(warmup):
@@ -249,7 +250,7 @@ def run_serving(args):
     reqs = serve.poisson_stream(args.srv_requests, rate=4.0, seed=args.seed + 505,
                                 sizes=(5000, args.srv_max_nodes), steps=(120, 360), nlv=8)
     params = dict(dt=60.0, ireib=5, block_rows=256, tri_sweeps=3)
-    srv = serve.Server("fp32", params=params, slots=args.srv_slots, hbm_budget=64 << 30)
+    srv = serve.Server("fp32", params=params, slots=args.srv_slots, hbm_budget=64 << 30, deferred=True)
     done, wall = srv.serve(reqs)
     s = serve.summary(done, wall)
     s.update(slots=args.srv_slots, rate_per_s=4.0,

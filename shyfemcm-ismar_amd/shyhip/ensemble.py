@@ -74,7 +74,7 @@ class Ensemble:
     float function for one boundary), tracers (dict name -> (nkn, nlv)
     array, same names for every member)."""
 
-    def __init__(self, members, precision="fp32", params=None, czv=2.5e-3, visv=0.0, difv=0.0):
+    def __init__(self, members, precision="fp32", params=None, czv=2.5e-3, visv=0.0, difv=0.0, deferred=False):
         import shyhip
         self.members = members
         self.u = union_basin([m["basin"] for m in members])
@@ -93,6 +93,8 @@ class Ensemble:
             ty[self.u.node_start[m]:self.u.node_start[m + 1]] = w[1]
         self.model.push("tauxnv", tx)
         self.model.push("tauynv", ty)
+        if deferred:    # steps queue behind each other; transport overlaps the next hydro
+            self.model.set_deferred_checks(True)
         self.nbnd = int(self.u.bnd_start[-1])
         self.t = 0.0
 

@@ -70,11 +70,16 @@ def forcing(req: Request, n: int, dt: float):
     return taux, tauy, level
 
 
-def run_request(req: Request, precision: str, params: dict, model_kw: dict, keep=("znv",)) -> dict:
-    """Create the model of `req`, advance it nsteps, return the kept fields."""
+def run_request(req: Request, precision: str, params: dict, model_kw: dict, keep=("znv",),
+                deferred: bool = False) -> dict:
+    """Create the model of `req`, advance it nsteps, return the kept fields
+    (deferred: the steps do not wait for the device; errors surface at the
+    final pull)."""
     import shyhip
     m = shyhip.Model(req.basin, precision, params=params, tracers=req.tracers, **model_kw)
     try:
+        if deferred:
+            m.set_deferred_checks(True)
         dt = m.params_dict["dt"]
         for n in range(1, req.nsteps + 1):
             taux, tauy, level = forcing(req, n, dt)
@@ -89,8 +94,9 @@ def run_request(req: Request, precision: str, params: dict, model_kw: dict, keep
 class Server:
     """Admission by HBM footprint and slots; worker threads step the runs."""
 
-    def __init__(self, precision="fp32", params=None, model_kw=None, hbm_budget=64 << 30, slots=8):
+    def __init__(self, precision="fp32", params=None, model_kw=None, hbm_budget=64 << 30, slots=8, deferred=False):
         self.precision = precision
+        self.deferred = deferred
         self.params = dict(params or {})
         self.model_kw = dict(model_kw or {})
         self.budget = hbm_budget
@@ -127,7 +133,7 @@ class Server:
                 self._admit(need)
                 req.t_admit = time.perf_counter() - t0
                 try:
-                    req.result = run_request(req, self.precision, self.params, self.model_kw, keep)
+                    req.result = run_request(req, self.precision, self.params, self.model_kw, keep, self.deferred)
                 except Exception as e:   # the error is reported with the request
                     req.error = str(e)
                 finally:
