@@ -299,13 +299,13 @@ int shyhip_destroy(shyhip_model *h) {
                     (void *)m.rowptr, (void *)m.cols, (void *)m.ijp, (void *)m.bnd_nodes, (void *)m.bnd_id,
                     (void *)m.blk_rowptr, (void *)m.blk_cols, (void *)m.blk_src, (void *)m.blk_diag, (void *)m.blk_cp_ptr,
                     (void *)m.blk_cp_a, (void *)m.blk_cp_b, (void *)m.blk_jd, (void *)m.blk_pcol, (void *)m.blk_prp,
-                    (void *)m.blk_pdg, (void *)m.blk_elp, (void *)m.blk_elr, (void *)m.blk_enlev, (void *)m.ell_col, (void *)m.ell_len, (void *)m.csr_ell,
+                    (void *)m.blk_pdg, (void *)m.fs_col, (void *)m.fs_meta, (void *)m.blk_elp, (void *)m.blk_elr, (void *)m.blk_enlev, (void *)m.ell_col, (void *)m.ell_len, (void *)m.csr_ell,
                     (void *)m.fwd_rows, (void *)m.fwd_lptr, (void *)m.fwd_nlev, (void *)m.bwd_rows,
                     (void *)m.bwd_lptr, (void *)m.bwd_nlev, (void *)m.tvd_el, (void *)m.tvd_kn, m.tvd_w,
                     (void *)m.tvd_lev, (void *)m.elem_iext})
         dfree(q);
     Solver &v = M.sv;
-    for (void *q : {v.lu, v.ell_a, v.r0, v.r, v.p0, v.p1, v.v0, v.v1, v.s, v.t, v.dx, v.b, v.x0, (void *)v.part,
+    for (void *q : {v.lu, v.ell_a, v.fs_val, v.r0, v.r, v.p0, v.p1, v.v0, v.v1, v.s, v.t, v.dx, v.b, v.x0, (void *)v.part,
                     (void *)v.scal, (void *)v.ctrl, v.a32, (void *)v.r64, v.zprev})
         dfree(q);
     for (hipStream_t st : M.tr_stream) (void)hipStreamSynchronize(st);
@@ -578,6 +578,27 @@ int shyhip_set_basin(shyhip_model *h, const int *nen3v, const float *xgv, const 
                     ecol[slot] = k < len ? M.h_cols[q0 + k] : i;
                     if (k < len) cse[q0 + k] = (int)slot;
                 }
+            }
+            // block factor rows slot-major, when every block row fits the slots
+            int maxrow = 0;
+            for (int i = 0; i < nkn; i++) maxrow = std::max(maxrow, bs.rowptr[i + 1] - bs.rowptr[i]);
+            if (maxrow <= kEllWidth) {
+                std::vector<int> fcol((size_t)kEllWidth * nkn), fmeta(nkn);
+                for (int i = 0; i < nkn; i++) {
+                    // slot 0 the diagonal, slots 1..nl the L entries, then U
+                    const int li = i % brows, q0 = bs.rowptr[i], cnt = bs.rowptr[i + 1] - q0;
+                    int nl = 0;
+                    for (int p = 0; p < cnt; p++) nl += bs.cols[q0 + p] < li;
+                    for (int k = 0; k < kEllWidth; k++) fcol[(size_t)k * nkn + i] = li;
+                    for (int p = 0; p < cnt; p++) {
+                        const int slot = p < nl ? p + 1 : (p == nl ? 0 : p);
+                        fcol[(size_t)slot * nkn + i] = bs.cols[q0 + p];
+                    }
+                    fmeta[i] = nl | (cnt << 8);
+                }
+                rc |= dalloc(M, &m.fs_col, fcol.size()) || upload(M, m.fs_col, fcol.data(), fcol.size());
+                rc |= dalloc(M, &m.fs_meta, fmeta.size()) || upload(M, m.fs_meta, fmeta.data(), fmeta.size());
+                rc |= dalloc_s(M, &M.sv.fs_val, (size_t)kEllWidth * nkn);
             }
             rc |= dalloc(M, &m.ell_col, ecol.size()) || upload(M, m.ell_col, ecol.data(), ecol.size());
             rc |= dalloc(M, &m.ell_len, elen.size()) || upload(M, m.ell_len, elen.data(), elen.size());

@@ -62,6 +62,10 @@ struct Mesh {                 // device copies of init-time data
     // at [k*nkn + i] (columns padded with i), the row length, and for each
     // CSR entry its slot (or -1 beyond the width)
     int *ell_col = nullptr, *ell_len = nullptr, *csr_ell = nullptr;
+    // block factor rows slot-major: local columns [k*nkn + i] (padding: the
+    // row itself) and nL | cnt << 8 per row (absent if a block row is longer
+    // than kEllWidth)
+    int *fs_col = nullptr, *fs_meta = nullptr;
     int *fwd_rows = nullptr, *fwd_lptr = nullptr, *fwd_nlev = nullptr;
     int *bwd_rows = nullptr, *bwd_lptr = nullptr, *bwd_nlev = nullptr;
     // TVD upwind tables: per element 6 ordered pairs (ic,id); [p*nel + ie]
@@ -113,6 +117,7 @@ struct Solver {               // Krylov work space (value type V = S of the mode
     int round_its[CT_NRITS] = {};   // iterations of each round of the previous solve
     void *lu = nullptr;       // block factors, fixed stride max_blk_nnz per block
     void *ell_a = nullptr;    // matrix values in the ELL slots (zero padding)
+    void *fs_val = nullptr;   // block factor rows, slot-major (with fs_col)
     void *r0 = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr, *v0 = nullptr, *v1 = nullptr,
          *s = nullptr, *t = nullptr, *dx = nullptr, *b = nullptr, *x0 = nullptr;
     double *part = nullptr;   // [32*nparts] per-workgroup partial sums (4 regions x 8)

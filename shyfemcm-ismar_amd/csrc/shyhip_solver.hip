@@ -55,6 +55,11 @@ struct SolvArgs {
     int edmax;
     const int *ell_col, *ell_len;    // ELL slices of the matrix rows
     const V *ell_a;
+    // block factor rows slot-major ([k*n + i]: slot 0 the inverted pivot,
+    // slots 1..nL the L entries, then the U entries; local columns) and per
+    // row nL | cnt << 8; fs_val null: not used
+    const int *fs_col, *fs_meta;
+    V *fs_val;
     V *lu;                           // block factors at a fixed stride per block
     const double *aval;
     V *b, *x0, *r0, *r, *p0, *p1, *v0, *v1, *s, *t, *dx;
@@ -78,6 +83,9 @@ static SolvArgs<V> solv_args(Model &M, const V *A, V *b, V *x0) {
     a.pcol = M.m.blk_pcol; a.prp = M.m.blk_prp; a.pdg = M.m.blk_pdg;
     a.elp = M.m.blk_elp; a.elr = M.m.blk_elr; a.enlev = M.m.blk_enlev; a.edmax = M.sv.max_elev;
     a.ell_col = M.m.ell_col; a.ell_len = M.m.ell_len; a.ell_a = (const V *)M.sv.ell_a;
+    a.fs_col = M.m.fs_col; a.fs_meta = M.m.fs_meta;
+    // factor rows in registers: one row per work-item and Jacobi sweeps
+    a.fs_val = (M.p.tri_sweeps > 0 && M.sv.brows <= kThreads) ? (V *)M.sv.fs_val : nullptr;
     a.lu = (V *)M.sv.lu; a.aval = M.s.aval;
     a.b = b; a.x0 = x0;
     a.r0 = (V *)M.sv.r0; a.r = (V *)M.sv.r; a.p0 = (V *)M.sv.p0; a.p1 = (V *)M.sv.p1;
@@ -327,6 +335,68 @@ __device__ inline V *block_lusol(const SolvArgs<V> &a, const BlockView<V> &B, in
     return y;
 }
 
+// One row of the block factors held in registers by the work-item that
+// owns the row (blocks of at most one row per work-item): its inverted
+// pivot (slot 0), L entries (slots 1..nl) and U entries (slots nl+1..cnt-1)
+// with their local columns, loaded at kernel start from the slot-major copy,
+// so that the Jacobi sweeps read only the neighbour values from LDS.  Same
+// operations in the same order as block_lusol's sweeps.
+constexpr int kFacWidth = kEllWidth;
+
+template <typename V>
+struct FacRow {
+    V v[kFacWidth];
+    int c[kFacWidth];
+    int nl = 0, cnt = 0;
+    __device__ __attribute__((always_inline)) void load(const SolvArgs<V> &a, int i, bool valid) {
+        if (!valid) return;
+        const int meta = a.fs_meta[i];
+        nl = meta & 255;
+        cnt = meta >> 8;
+#pragma unroll
+        for (int k = 0; k < kFacWidth; k++) {
+            v[k] = a.fs_val[(size_t)k * a.n + i];
+            c[k] = a.fs_col[(size_t)k * a.n + i];
+        }
+    }
+    // (LU)^-1 y for this row, a.sweeps > 0 Jacobi sweeps per factor
+    // alternating between y and w; returns the buffer holding the result
+    __device__ __attribute__((always_inline)) V *lusol(int sweeps, bool mine, int li, V *y, V *w) const {
+        V *src = y, *dst = w;
+        V bl = mine ? y[li] : (V)0;
+        for (int sw = 1; sw < sweeps; sw++) {
+            if (mine) {
+                V acc = bl;
+#pragma unroll
+                for (int k = 1; k < kFacWidth; k++)
+                    if (k <= nl) acc = acc - v[k] * src[c[k]];
+                dst[li] = acc;
+            }
+            __syncthreads();
+            V *t = src; src = dst; dst = t;
+        }
+        const V dinv = v[0];
+        if (mine) {
+            bl = src[li];
+            dst[li] = dinv * bl;
+        }
+        __syncthreads();
+        { V *t = src; src = dst; dst = t; }
+        for (int sw = 1; sw < sweeps; sw++) {
+            if (mine) {
+                V acc = bl;
+#pragma unroll
+                for (int k = 0; k < kFacWidth; k++)
+                    if (k > nl && k < cnt) acc = acc - v[k] * src[c[k]];
+                dst[li] = dinv * acc;
+            }
+            __syncthreads();
+            V *t = src; src = dst; dst = t;
+        }
+        return src;
+    }
+};
+
 // ILU(0) of each diagonal block (the pivot is stored inverted).  Row i,
 // entries in column order:
 //   l_ij = (a_ij - sum_{k<j} l_ik u_kj) * (1/u_jj)   (j < i)
@@ -392,6 +462,15 @@ __global__ __launch_bounds__(kThreads) void k_bilu_factor(SolvArgs<V> a, int sch
         __syncthreads();
     }
     for (int e = threadIdx.x; e < ne; e += blockDim.x) a.lu[(size_t)blk * a.mx + e] = val[e];
+    if (a.fs_val) {
+        const int *rp = a.prp + (size_t)blk * (a.brows + 1);
+        for (int li = threadIdx.x; li < nb; li += blockDim.x)
+            for (int e = rp[li]; e < rp[li + 1]; e++) {
+                const int pos = e - rp[li], nl = a.fs_meta[b0 + li] & 255;
+                const int slot = pos < nl ? pos + 1 : (pos == nl ? 0 : pos);
+                a.fs_val[(size_t)slot * a.n + b0 + li] = val[e];
+            }
+    }
 }
 
 // The operands of one matrix row -- values and NV gathered vectors at the
@@ -482,8 +561,9 @@ __global__ __launch_bounds__(kThreads) void k_bicg_init(SolvArgs<V> a, int zero_
 // work-item, every load that depends on no other load (control words,
 // scalars, the block's factors, the row's operands, the partial sums) is
 // issued before the first test.
-template <typename V, bool STAGE>
+template <typename V, int MODE>
 __global__ __launch_bounds__(kThreads) void k_bicg_a(SolvArgs<V> a, int it) {
+    constexpr bool STAGE = MODE == 1;      // block factors staged in LDS (0: read from global memory)
     extern __shared__ double smem[];
     V *y = reinterpret_cast<V *>(smem), *w = y + a.brows;
     const int par = it & 1, pp = par ^ 1;
@@ -501,8 +581,11 @@ __global__ __launch_bounds__(kThreads) void k_bicg_a(SolvArgs<V> a, int it) {
     // independent loads first
     const int stop = a.ctrl[CT_STOPIT], fail = a.ctrl[CT_FAIL], refconv = a.ctrl[CT_REFCONV];
     const double alpha_prev = a.scal[SC_ALPHA + pp], target_prev = a.scal[SC_TARGET], rho_prev = a.scal[SC_RHO + pp];
+    constexpr bool regs = MODE == 2;       // factor rows in registers, no block staging
     BlockStage<V, STAGE> st;
-    st.load(a, blk);
+    FacRow<V> fr;
+    if constexpr (regs) fr.load(a, i, mine);
+    else st.load(a, blk);
     Partials<5> ps;
     ps.load(a.part, region, a.nparts, a.nblk, nq);
     RowOps<V, 4> ro;
@@ -516,7 +599,7 @@ __global__ __launch_bounds__(kThreads) void k_bicg_a(SolvArgs<V> a, int it) {
         }
     }
     if (fail || refconv || (stop != 0 && stop <= it)) return;
-    const BlockView<V> B = st.store(a, blk, reinterpret_cast<unsigned char *>(w + a.brows));
+    const BlockView<V> B = regs ? BlockView<V>{} : st.store(a, blk, reinterpret_cast<unsigned char *>(w + a.brows));
     double q[5];   // it = 0: (r0,r0) (r0,r0); else (t,t) (t,s) (s,s) (r0,s) (r0,t)
     ps.total(a.part, region, a.nparts, a.nblk, nq, q);
     double rn, rho, omega = 0., alpha = 0.;
@@ -632,7 +715,7 @@ __global__ __launch_bounds__(kThreads) void k_bicg_a(SolvArgs<V> a, int it) {
         }
     }
     __syncthreads();
-    const V *z = block_lusol(a, B, blk, b0, nb, y, w);
+    const V *z = regs ? fr.lusol(a.sweeps, mine, threadIdx.x, y, w) : block_lusol(a, B, blk, b0, nb, y, w);
     double loc[1] = {0.};
     if (pre) {
         if (mine) {
@@ -652,8 +735,9 @@ __global__ __launch_bounds__(kThreads) void k_bicg_a(SolvArgs<V> a, int it) {
 // Iteration it, second kernel: alpha = rho / (r0,v); s = r - alpha v
 // (fused); t = M^-1 A s; partials (t,t) (t,s) (s,s) (r0,s) (r0,t).
 // Independent loads are issued first, as in k_bicg_a.
-template <typename V, bool STAGE>
+template <typename V, int MODE>
 __global__ __launch_bounds__(kThreads) void k_bicg_b(SolvArgs<V> a, int it) {
+    constexpr bool STAGE = MODE == 1;      // block factors staged in LDS (0: read from global memory)
     extern __shared__ double smem[];
     V *y = reinterpret_cast<V *>(smem), *w = y + a.brows;
     const int par = it & 1;
@@ -667,8 +751,11 @@ __global__ __launch_bounds__(kThreads) void k_bicg_b(SolvArgs<V> a, int it) {
     const bool mine = pre && (int)threadIdx.x < nb;
     const int stop = a.ctrl[CT_STOPIT], fail = a.ctrl[CT_FAIL], refconv = a.ctrl[CT_REFCONV];
     const double rho = a.scal[SC_RHO + par];
+    constexpr bool regs = MODE == 2;
     BlockStage<V, STAGE> st;
-    st.load(a, blk);
+    FacRow<V> fr;
+    if constexpr (regs) fr.load(a, i, mine);
+    else st.load(a, blk);
     Partials<1> ps;
     ps.load(a.part, PR_A, a.nparts, a.nblk, 1);
     RowOps<V, 2> ro;
@@ -678,7 +765,7 @@ __global__ __launch_bounds__(kThreads) void k_bicg_b(SolvArgs<V> a, int it) {
         if (mine) r0_i = a.r0[i];
     }
     if (stop || fail || refconv) return;
-    const BlockView<V> B = st.store(a, blk, reinterpret_cast<unsigned char *>(w + a.brows));
+    const BlockView<V> B = regs ? BlockView<V>{} : st.store(a, blk, reinterpret_cast<unsigned char *>(w + a.brows));
     double d[1];
     ps.total(a.part, PR_A, a.nparts, a.nblk, 1, d);
     if (breakdown(d[0])) {
@@ -719,7 +806,7 @@ __global__ __launch_bounds__(kThreads) void k_bicg_b(SolvArgs<V> a, int it) {
         }
     }
     __syncthreads();
-    const V *z = block_lusol(a, B, blk, b0, nb, y, w);
+    const V *z = regs ? fr.lusol(a.sweeps, mine, threadIdx.x, y, w) : block_lusol(a, B, blk, b0, nb, y, w);
     double loc[5] = {0., 0., 0., 0., 0.};
     if (pre) {
         if (mine) {
@@ -894,20 +981,22 @@ static int factor(Model &M, SolvArgs<V> &a) {
 // Kernels enqueued after convergence return immediately.  `tail` enqueues
 // the work that follows a converged solve, switched on by its flag, before
 // each check.
-template <typename V, bool STAGE>
+// MODE of the iteration kernels: 0 block factors read from global memory,
+// 1 staged in LDS, 2 factor rows in registers (the init kernel stages)
+template <typename V, int MODE>
 static int bicgstab_t(Model &M, SolvArgs<V> &a, int *iters, int slot, int zero_x0,
                       const std::function<int()> &tail) {
     const int g = a.nblk;
     const size_t lds = apply_lds<V>(M);
-    hipLaunchKernelGGL((k_bicg_init<V, STAGE>), dim3(g), dim3(kThreads), lds, M.stream, a, zero_x0);
+    hipLaunchKernelGGL((k_bicg_init<V, MODE != 0>), dim3(g), dim3(kThreads), lds, M.stream, a, zero_x0);
     HIPCHK(M, hipGetLastError());
     int it = 0, c[CT_WORDS] = {};
     const int maxit = M.p.itermax > 0 ? (M.p.itermax + 1) / 2 + 1 : 100000;
     int chunk = std::max(2, M.sv.last_its[slot] + 1);
     for (;;) {
         for (int q = 0; q < chunk; q++, it++) {
-            launch_timed(M, T_BICG_A, k_bicg_a<V, STAGE>, dim3(g), dim3(kThreads), lds, M.stream, a, it);
-            launch_timed(M, T_BICG_B, k_bicg_b<V, STAGE>, dim3(g), dim3(kThreads), lds, M.stream, a, it);
+            launch_timed(M, T_BICG_A, k_bicg_a<V, MODE>, dim3(g), dim3(kThreads), lds, M.stream, a, it);
+            launch_timed(M, T_BICG_B, k_bicg_b<V, MODE>, dim3(g), dim3(kThreads), lds, M.stream, a, it);
         }
         HIPCHK(M, hipGetLastError());
         if (tail && tail()) return 1;
@@ -933,22 +1022,31 @@ static int bicgstab_t(Model &M, SolvArgs<V> &a, int *iters, int slot, int zero_x
 }
 
 template <typename V>
+static int iter_mode(const Model &M, const SolvArgs<V> &a) {
+    if (!stage_fits<V>(M)) return 0;
+    return (a.fs_val && a.brows <= kThreads) ? 2 : 1;
+}
+
+template <typename V>
 static int bicgstab(Model &M, SolvArgs<V> &a, int *iters, int slot, int zero_x0, const std::function<int()> &tail) {
-    return stage_fits<V>(M) ? bicgstab_t<V, true>(M, a, iters, slot, zero_x0, tail)
-                            : bicgstab_t<V, false>(M, a, iters, slot, zero_x0, tail);
+    switch (iter_mode(M, a)) {
+    case 2: return bicgstab_t<V, 2>(M, a, iters, slot, zero_x0, tail);
+    case 1: return bicgstab_t<V, 1>(M, a, iters, slot, zero_x0, tail);
+    default: return bicgstab_t<V, 0>(M, a, iters, slot, zero_x0, tail);
+    }
 }
 
 // one inner FP32 solve of the refinement, enqueued without host checks:
 // init from a zero correction, then `chunk` iterations (no-ops once
 // converged or once the refinement has converged)
-template <typename V, bool STAGE>
+template <typename V, int MODE>
 static int enqueue_inner_t(Model &M, SolvArgs<V> &a, int chunk) {
     const int g = a.nblk;
     const size_t lds = apply_lds<V>(M);
-    hipLaunchKernelGGL((k_bicg_init<V, STAGE>), dim3(g), dim3(kThreads), lds, M.stream, a, 1);
+    hipLaunchKernelGGL((k_bicg_init<V, MODE != 0>), dim3(g), dim3(kThreads), lds, M.stream, a, 1);
     for (int it = 0; it < chunk; it++) {
-        launch_timed(M, T_BICG_A, k_bicg_a<V, STAGE>, dim3(g), dim3(kThreads), lds, M.stream, a, it);
-        launch_timed(M, T_BICG_B, k_bicg_b<V, STAGE>, dim3(g), dim3(kThreads), lds, M.stream, a, it);
+        launch_timed(M, T_BICG_A, k_bicg_a<V, MODE>, dim3(g), dim3(kThreads), lds, M.stream, a, it);
+        launch_timed(M, T_BICG_B, k_bicg_b<V, MODE>, dim3(g), dim3(kThreads), lds, M.stream, a, it);
     }
     HIPCHK(M, hipGetLastError());
     return 0;
@@ -956,7 +1054,11 @@ static int enqueue_inner_t(Model &M, SolvArgs<V> &a, int chunk) {
 
 template <typename V>
 static int enqueue_inner(Model &M, SolvArgs<V> &a, int chunk) {
-    return stage_fits<V>(M) ? enqueue_inner_t<V, true>(M, a, chunk) : enqueue_inner_t<V, false>(M, a, chunk);
+    switch (iter_mode(M, a)) {
+    case 2: return enqueue_inner_t<V, 2>(M, a, chunk);
+    case 1: return enqueue_inner_t<V, 1>(M, a, chunk);
+    default: return enqueue_inner_t<V, 0>(M, a, chunk);
+    }
 }
 
 int solver_setup(Model &M) { (void)M; return 0; }
