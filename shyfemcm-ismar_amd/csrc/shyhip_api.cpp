@@ -299,7 +299,8 @@ int shyhip_destroy(shyhip_model *h) {
                     (void *)m.rowptr, (void *)m.cols, (void *)m.ijp, (void *)m.bnd_nodes, (void *)m.bnd_id,
                     (void *)m.blk_rowptr, (void *)m.blk_cols, (void *)m.blk_src, (void *)m.blk_diag, (void *)m.blk_cp_ptr,
                     (void *)m.blk_cp_a, (void *)m.blk_cp_b, (void *)m.blk_jd, (void *)m.blk_pcol, (void *)m.blk_prp,
-                    (void *)m.blk_pdg, (void *)m.fs_col, (void *)m.fs_meta, (void *)m.blk_elp, (void *)m.blk_elr, (void *)m.blk_enlev, (void *)m.ell_col, (void *)m.ell_len, (void *)m.csr_ell,
+                    (void *)m.blk_pdg, (void *)m.fs_col, (void *)m.fs_meta, (void *)m.blk_fdesc, (void *)m.blk_fpair,
+                    (void *)m.blk_elp, (void *)m.blk_elr, (void *)m.blk_enlev, (void *)m.ell_col, (void *)m.ell_len, (void *)m.csr_ell,
                     (void *)m.fwd_rows, (void *)m.fwd_lptr, (void *)m.fwd_nlev, (void *)m.bwd_rows,
                     (void *)m.bwd_lptr, (void *)m.bwd_nlev, (void *)m.tvd_el, (void *)m.tvd_kn, m.tvd_w,
                     (void *)m.tvd_lev, (void *)m.elem_iext})
@@ -561,6 +562,26 @@ int shyhip_set_basin(shyhip_model *h, const int *nen3v, const float *xgv, const 
                 std::vector<int> pos(lp, lp + nl);
                 for (int e = 0; e < ne; e++) elr[(size_t)b * mx + pos[bs.ent_lev[q0 + e]]++] = e;
             }
+            // descriptors in level order and packed pairs, relative to the block
+            std::vector<int> fdesc((size_t)nbk * mx * 4, 0), fpair(2 * bs.cp_a.size() + 2, 0);
+            for (int b = 0; b < nbk; b++) {
+                const int b0 = b * brows, nb = std::min(nkn, b0 + brows) - b0;
+                const int q0 = bs.rowptr[b0], ne = bs.rowptr[b0 + nb] - q0, p0 = bs.cp_ptr[q0];
+                for (int t = 0; t < ne; t++) {
+                    const int e = elr[(size_t)b * mx + t], q = q0 + e;
+                    int *d = &fdesc[((size_t)b * mx + t) * 4];
+                    d[0] = e;
+                    d[1] = bs.cp_ptr[q] - p0;
+                    d[2] = bs.cp_ptr[q + 1] - p0;
+                    d[3] = bs.ent_jd[q] >= 0 ? bs.ent_jd[q] - q0 : bs.ent_jd[q];
+                }
+                for (int p = p0; p < bs.cp_ptr[q0 + ne]; p++) {
+                    fpair[2 * (size_t)p] = bs.cp_a[p] - q0;
+                    fpair[2 * (size_t)p + 1] = bs.cp_b[p] - q0;
+                }
+            }
+            rc |= dalloc(M, &m.blk_fdesc, fdesc.size()) || upload(M, m.blk_fdesc, fdesc.data(), fdesc.size());
+            rc |= dalloc(M, &m.blk_fpair, fpair.size()) || upload(M, m.blk_fpair, fpair.data(), fpair.size());
             rc |= dalloc(M, &m.blk_elp, elp.size()) || upload(M, m.blk_elp, elp.data(), elp.size());
             rc |= dalloc(M, &m.blk_elr, elr.size()) || upload(M, m.blk_elr, elr.data(), elr.size());
             rc |= dalloc(M, &m.blk_enlev, bs.ent_nlev.size()) ||

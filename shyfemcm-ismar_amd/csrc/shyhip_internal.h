@@ -58,6 +58,10 @@ struct Mesh {                 // device copies of init-time data
     // entry-level factorisation schedule: level starts [blk*(max_elev+1) + lv]
     // and the block's entries by level [blk*max_blk_nnz + t]; levels per block
     int *blk_elp = nullptr, *blk_elr = nullptr, *blk_enlev = nullptr;
+    // the same entries as descriptors {entry, pairs begin, end, pivot} (int4,
+    // [blk*max_blk_nnz + t]) and the update pairs packed {a, b} (int2,
+    // indexed like cp_a), all relative to the block
+    int *blk_fdesc = nullptr, *blk_fpair = nullptr;
     // zeta matrix rows sliced ELL-style: the first kEllWidth entries of row i
     // at [k*nkn + i] (columns padded with i), the row length, and for each
     // CSR entry its slot (or -1 beyond the width)

@@ -52,6 +52,8 @@ struct SolvArgs {
     const int *cp_ptr, *cp_a, *cp_b, *bjd;
     const int *pcol, *prp, *pdg;     // block pattern at a fixed stride per block
     const int *elp, *elr, *enlev;    // entry-level factorisation schedule
+    const int4 *fdesc;               // per entry in level order: {entry, pairs begin, end, pivot}
+    const int2 *fpair;               // update pairs {l_ik, u_kj} entries, relative to the block
     int edmax;
     const int *ell_col, *ell_len;    // ELL slices of the matrix rows
     const V *ell_a;
@@ -82,6 +84,7 @@ static SolvArgs<V> solv_args(Model &M, const V *A, V *b, V *x0) {
     a.cp_ptr = M.m.blk_cp_ptr; a.cp_a = M.m.blk_cp_a; a.cp_b = M.m.blk_cp_b; a.bjd = M.m.blk_jd;
     a.pcol = M.m.blk_pcol; a.prp = M.m.blk_prp; a.pdg = M.m.blk_pdg;
     a.elp = M.m.blk_elp; a.elr = M.m.blk_elr; a.enlev = M.m.blk_enlev; a.edmax = M.sv.max_elev;
+    a.fdesc = (const int4 *)M.m.blk_fdesc; a.fpair = (const int2 *)M.m.blk_fpair;
     a.ell_col = M.m.ell_col; a.ell_len = M.m.ell_len; a.ell_a = (const V *)M.sv.ell_a;
     a.fs_col = M.m.fs_col; a.fs_meta = M.m.fs_meta;
     // factor rows in registers: one row per work-item and Jacobi sweeps
@@ -401,62 +404,58 @@ struct FacRow {
 // entries in column order:
 //   l_ij = (a_ij - sum_{k<j} l_ik u_kj) * (1/u_jj)   (j < i)
 //   u_ij =  a_ij - sum_{k<i} l_ik u_kj               (j >= i)
-// with the (k ascending) update pairs of entry q in cp_a/cp_b[cp_ptr[q]..].
-// Entries are computed level by level of the entry dependence graph (an
-// entry's level is one more than that of every entry it reads), one
-// work-item per entry, a barrier per level; each entry is the same
-// expression in the same order as in a row-by-row factorisation.  With
-// sched the level lists are copied to LDS; STAGE copies the update lists.
-template <typename V, bool STAGE>
-__global__ __launch_bounds__(kThreads) void k_bilu_factor(SolvArgs<V> a, int sched) {
+// with the (k ascending) update pairs of each entry.  Entries are computed
+// level by level of the entry dependence graph (an entry's level is one
+// more than that of every entry it reads), one work-item per entry, a
+// barrier per level; each entry is the same expression in the same order
+// as in a row-by-row factorisation.  One wavefront per block: a level has
+// a few tens of entries, and a one-wavefront barrier costs next to nothing.
+// Per entry, in level order, one descriptor {entry, first pair, end of
+// pairs, pivot entry (>= 0) / -1 (U) / -2 - row (diagonal)} and the pairs'
+// two entries packed together, all relative to the block; with stage the
+// descriptors and pairs are copied to LDS.
+constexpr int kFactorThreads = 64;
+
+template <typename V>
+__global__ __launch_bounds__(kFactorThreads) void k_bilu_factor(SolvArgs<V> a, int stage) {
     extern __shared__ double smem[];
     V *val = reinterpret_cast<V *>(smem);
-    const int blk = xcd_block(blockIdx.x, gridDim.x), b0 = blk * a.brows;
+    const int blk = blockIdx.x, b0 = blk * a.brows;
     const int nb = min(a.n, b0 + a.brows) - b0;
     const int q0 = a.brp[b0], q1 = a.brp[b0 + nb], ne = q1 - q0;
     const int p0 = a.cp_ptr[q0], np = a.cp_ptr[q1] - p0;
     const int nl = a.enlev[blk];
     const int *elp = a.elp + (size_t)blk * (a.edmax + 1);   // level starts
-    const int *elr = a.elr + (size_t)blk * a.mx;             // entries by level (relative)
-    int *lfree = reinterpret_cast<int *>(val + a.mx);
-    if (sched) {
-        int *l_p = lfree, *l_r = l_p + a.edmax + 1;
-        for (int t = threadIdx.x; t <= nl; t += blockDim.x) l_p[t] = elp[t];
-        for (int t = threadIdx.x; t < ne; t += blockDim.x) l_r[t] = elr[t];
-        elp = l_p; elr = l_r;
-        lfree = l_r + a.mx;
-    }
-    const int *cpp = a.cp_ptr + q0, *cpa = a.cp_a + p0, *cpb = a.cp_b + p0, *jdv = a.bjd + q0;
-    int poff = p0;                  // pair index base of the lists in use
-    if (STAGE) {
-        int *lp = lfree;
-        int *la = lp + a.mx + 1, *lb = la + a.mxp, *lj = lb + a.mxp;
-        for (int e = threadIdx.x; e <= ne; e += blockDim.x) {
-            lp[e] = a.cp_ptr[q0 + e];
-            if (e < ne) lj[e] = a.bjd[q0 + e];
-        }
-        for (int p = threadIdx.x; p < np; p += blockDim.x) {
-            la[p] = a.cp_a[p0 + p];
-            lb[p] = a.cp_b[p0 + p];
-        }
-        cpp = lp; cpa = la; cpb = lb; jdv = lj;
+    const int4 *desc = a.fdesc + (size_t)blk * a.mx;         // entries by level
+    const int2 *pairs = a.fpair + p0;
+    // the lists start 16-byte aligned after the values
+    unsigned char *lfree = reinterpret_cast<unsigned char *>(smem) + ((a.mx * sizeof(V) + 15) & ~(size_t)15);
+    if (stage) {
+        int4 *l_d = reinterpret_cast<int4 *>(lfree);
+        int2 *l_p = reinterpret_cast<int2 *>(l_d + a.mx);
+        int *l_e = reinterpret_cast<int *>(l_p + a.mxp);
+        for (int t = threadIdx.x; t < ne; t += blockDim.x) l_d[t] = desc[t];
+        for (int p = threadIdx.x; p < np; p += blockDim.x) l_p[p] = pairs[p];
+        for (int t = threadIdx.x; t <= nl; t += blockDim.x) l_e[t] = elp[t];
+        desc = l_d; pairs = l_p; elp = l_e;
     }
     for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) val[q - q0] = (V)a.aval[a.bsrc[q]];
     __syncthreads();
     for (int lv = 0; lv < nl; lv++) {
         for (int t = elp[lv] + threadIdx.x; t < elp[lv + 1]; t += blockDim.x) {
-            const int e = elr[t];
-            V acc = val[e];
-            for (int p = cpp[e] - poff; p < cpp[e + 1] - poff; p++)
-                acc = acc - val[cpa[p] - q0] * val[cpb[p] - q0];
-            const int jd = jdv[e];
-            if (jd >= 0) {
-                val[e] = acc * val[jd - q0];
-            } else if (jd == -1) {
-                val[e] = acc;
+            const int4 d = desc[t];
+            V acc = val[d.x];
+            for (int p = d.y; p < d.z; p++) {
+                const int2 pr = pairs[p];
+                acc = acc - val[pr.x] * val[pr.y];
+            }
+            if (d.w >= 0) {
+                val[d.x] = acc * val[d.w];
+            } else if (d.w == -1) {
+                val[d.x] = acc;
             } else {
-                if (acc == (V)0) atomicCAS(a.ctrl + CT_FACTOR, 0, 1 + b0 + (-2 - jd));
-                val[e] = (V)1 / acc;
+                if (acc == (V)0) atomicCAS(a.ctrl + CT_FACTOR, 0, 1 + b0 + (-2 - d.w));
+                val[d.x] = (V)1 / acc;
             }
         }
         __syncthreads();
@@ -466,8 +465,8 @@ __global__ __launch_bounds__(kThreads) void k_bilu_factor(SolvArgs<V> a, int sch
         const int *rp = a.prp + (size_t)blk * (a.brows + 1);
         for (int li = threadIdx.x; li < nb; li += blockDim.x)
             for (int e = rp[li]; e < rp[li + 1]; e++) {
-                const int pos = e - rp[li], nl = a.fs_meta[b0 + li] & 255;
-                const int slot = pos < nl ? pos + 1 : (pos == nl ? 0 : pos);
+                const int pos = e - rp[li], nlr = a.fs_meta[b0 + li] & 255;
+                const int slot = pos < nlr ? pos + 1 : (pos == nlr ? 0 : pos);
                 a.fs_val[(size_t)slot * a.n + b0 + li] = val[e];
             }
     }
@@ -948,28 +947,16 @@ static size_t apply_lds(const Model &M) {
 // wet/dry state changes rows of the zeta matrix from one step to the next
 // (dry-element rows, level boundaries), and factors of an earlier matrix
 // then no longer precondition it.
-static size_t factor_lds_sched(const Model &M) {
-    return (size_t)(M.sv.max_elev + 1 + M.sv.max_blk_nnz) * sizeof(int);
-}
-
-static size_t factor_lds_lists(const Model &M) {
-    return (size_t)(2 * M.sv.max_blk_nnz + 1 + 2 * M.sv.max_blk_pairs) * sizeof(int);
-}
-
-// LDS of the factorisation: the values always; the row schedule and the
-// update lists when they fit (schedule first)
+// LDS of the factorisation: the block's values, and when they fit the
+// descriptors, pairs and level starts
 template <typename V>
 static int factor(Model &M, SolvArgs<V> &a) {
-    const size_t vals = (size_t)M.sv.max_blk_nnz * sizeof(V), sch = factor_lds_sched(M), lst = factor_lds_lists(M);
-    const dim3 g(M.sv.nblk), b(kThreads);
-    if (vals + sch + lst <= kLdsLimit)
-        launch_timed(M, T_FACTOR, k_bilu_factor<V, true>, g, b, vals + sch + lst, M.stream, a, 1);
-    else if (vals + lst <= kLdsLimit)
-        launch_timed(M, T_FACTOR, k_bilu_factor<V, true>, g, b, vals + lst, M.stream, a, 0);
-    else if (vals + sch <= kLdsLimit)
-        launch_timed(M, T_FACTOR, k_bilu_factor<V, false>, g, b, vals + sch, M.stream, a, 1);
-    else
-        launch_timed(M, T_FACTOR, k_bilu_factor<V, false>, g, b, vals, M.stream, a, 0);
+    const size_t vals = ((size_t)M.sv.max_blk_nnz * sizeof(V) + 15) & ~(size_t)15;
+    const size_t lists = (size_t)M.sv.max_blk_nnz * sizeof(int4) + (size_t)M.sv.max_blk_pairs * sizeof(int2) +
+                         (size_t)(M.sv.max_elev + 1) * sizeof(int);
+    const int stage = vals + lists <= kLdsLimit ? 1 : 0;
+    launch_timed(M, T_FACTOR, k_bilu_factor<V>, dim3(M.sv.nblk), dim3(kFactorThreads), stage ? vals + lists : vals,
+                 M.stream, a, stage);
     HIPCHK(M, hipGetLastError());
     return 0;
 }
