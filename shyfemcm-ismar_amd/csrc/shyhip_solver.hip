@@ -105,9 +105,31 @@ static SolvArgs<V> solv_args(Model &M, const V *A, V *b, V *x0) {
 constexpr int kWaves = kThreads / 64;
 constexpr int kRegion = 8;            // partial-sum slots per region
 
+// v from the DPP-permuted lane (CTRL: a GFX9 DPP control code whose source
+// lane always lies in the same row of 16, so every lane is written)
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ inline double readlane_f64(double v, int lane) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                            __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
+// Sum over the 64 lanes, the same in every lane: exchanges within rows of
+// 16 by DPP (lanes i^1, i^2 by quad permutes, then the mirrored halves of
+// 8 and of 16 lanes, which hold equal values by then), then the four row
+// sums read from lanes 0, 16, 32, 48.  Each exchange adds two values that
+// the partner lane adds in the other order, so all lanes agree bitwise.
 __device__ inline double wave_sum(double v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    v = v + dpp_f64<0xB1>(v);      // quad_perm [1,0,3,2]
+    v = v + dpp_f64<0x4E>(v);      // quad_perm [2,3,0,1]
+    v = v + dpp_f64<0x141>(v);     // row_half_mirror
+    v = v + dpp_f64<0x140>(v);     // row_mirror
+    return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
 }
 
 template <int NQ>
