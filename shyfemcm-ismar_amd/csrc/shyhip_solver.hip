@@ -430,13 +430,15 @@ struct FacRow {
 // level by level of the entry dependence graph (an entry's level is one
 // more than that of every entry it reads), one work-item per entry, a
 // barrier per level; each entry is the same expression in the same order
-// as in a row-by-row factorisation.  One wavefront per block: a level has
-// a few tens of entries, and a one-wavefront barrier costs next to nothing.
+// as in a row-by-row factorisation.  Four wavefronts per block: staging
+// and the wide early levels spread over 256 work-items (measured on the
+// 50k-node lagoon, 256-row blocks: 56 us against 60 us with two
+// wavefronts and 75 us with one).
 // Per entry, in level order, one descriptor {entry, first pair, end of
 // pairs, pivot entry (>= 0) / -1 (U) / -2 - row (diagonal)} and the pairs'
 // two entries packed together, all relative to the block; with stage the
 // descriptors and pairs are copied to LDS.
-constexpr int kFactorThreads = 64;
+constexpr int kFactorThreads = 256;
 
 template <typename V>
 __global__ __launch_bounds__(kFactorThreads) void k_bilu_factor(SolvArgs<V> a, int stage) {
