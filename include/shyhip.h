@@ -140,6 +140,46 @@ int  shyhip_set_boundary_levels(shyhip_model *h, int n, const float *levels);
 /* uniform wind stress per unit density (meteo_forcing.f90:973) */
 int  shyhip_set_wind_uniform(shyhip_model *h, float taux, float tauy);
 
+/* Forcing series held on the device.  Replaces the forcing files that
+ * sp111(2) reads for each step (bnd_routines.f90:444-596, the boundary
+ * files opened at :330-356) and meteo_forcing_fem reads for the wind
+ * (meteo_forcing.f90:301-555), with their time interpolation
+ * (iff_read_and_interpolate + iff_time_interpolate, intp_fem_file.f90:
+ * 2288-2510, rd_intp_neville, utils/generic/intp.f90:317-345): records
+ * time[0..nrec-1] (strictly increasing, seconds as dtime), interpolated with
+ * intpol = 2 or 4 records (0: the reference's default, 4 for a level
+ * boundary, 2 for the wind), the first ring chosen for start time dtime0 as
+ * iff_populate_records does (:1047-1144).  One record = a constant field.
+ * A record value of -999 inside the window gives 0.  Once installed, every
+ * shyhip_step evaluates the series at its dtime on the device before hydro
+ * (as sp111(2) precedes hydro, shyfem_subs.f90:592-604); times must not
+ * decrease between steps, and a time outside the records is an error
+ * (status 2), as the reference stops (intp_fem_file.f90:2344-2352).
+ *
+ * Level boundary ibc (ibtyp 1): val[r*nval + i], nval = 1 (one level for the
+ * whole boundary) or the boundary's node count (its nodes in ascending node
+ * order); the level is multiplied by zfact (adjust_bound,
+ * bnd_routines.f90:1003-1031) and written to rzv. */
+int  shyhip_set_boundary_series(shyhip_model *h, int ibc, int nrec, const double *time, const float *val, int nval,
+                                int intpol, float zfact, double dtime0);
+/* wind conversion settings (meteo_set_wind_data, meteo_forcing.f90:700-802;
+ * STR defaults def_para.f90:782-799: iwtype 1, itdrag 0, dragco 2.5e-3,
+ * roluft 1.225, wslim -1, wsmax 50); rowass is the model parameter */
+typedef struct {
+    int iwtype;      /* 1: wind velocity (m/s), 2: stress (N/m2)                       */
+    int itdrag;      /* 0: dragco, 1: Smith and Banke (1975), 2: Large and Pond (1981)   */
+    float dragco, roluft, wslim, wsmax;
+} shyhip_wind_params;
+/* Wind series: wx/wy[r*nval + i], nval = 1 (one point for the whole basin)
+ * or nkn (node values); each step converts it to the stress per unit density
+ * in tauxnv/tauynv (meteo_convert_wind_data, meteo_forcing.f90:864-1000, no
+ * ice cover).  A speed above wsmax is reported as a device error at the next
+ * check.  nrec = 0 removes the series. */
+int  shyhip_set_wind_series(shyhip_model *h, int nrec, const double *time, const float *wx, const float *wy, int nval,
+                            int intpol, double dtime0, const shyhip_wind_params *wp);
+/* sp111(2) + meteo_force on the device at time dtime, outside a step */
+int  shyhip_forcing(shyhip_model *h, double dtime);
+
 /* time step -------------------------------------------------------------- */
 int  shyhip_set_dry(shyhip_model *h);
 int  shyhip_copy_uvz(shyhip_model *h);                 /* copy_uvz + copy_depth */

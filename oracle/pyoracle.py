@@ -298,3 +298,53 @@ class OracleModel:
     def tracer_mass(self, name, mode=1):
         self.lib.oc_massconc.restype = C.c_double
         return self.lib.oc_massconc(C.byref(self.mesh), C.byref(self.state), ptr(self.tracers[name]), mode)
+
+
+# ------------------------------------------------------------------ forcing
+def intp_neville(xa, ya, x):
+    """rd_intp_neville over the points in the given order (f32 data)."""
+    lib = load("f32")
+    lib.oc_intp_neville.restype = C.c_double
+    xa = np.ascontiguousarray(xa, np.float64)
+    ya = np.ascontiguousarray(ya, np.float32)
+    return lib.oc_intp_neville(len(xa), ptr(xa), ptr(ya), C.c_double(x))
+
+
+def iff_first_record(time, nintp, dtime0):
+    lib = load("f32")
+    t = np.ascontiguousarray(time, np.float64)
+    return lib.oc_iff_first_record(len(t), ptr(t), nintp, C.c_double(dtime0))
+
+
+def iff_series(time, vals, nintp, dtime0, times, comp=0, vdefault=0.0):
+    """Values of component comp of a record series at each of `times`, as a
+    run with one interpolation per step gets them.  vals: (nrec, nval) or
+    (nrec, ncomp, nval).  Returns (out (nt, nval) f32, newest record per
+    time), or None when a time is outside the records."""
+    lib = load("f32")
+    t = np.ascontiguousarray(time, np.float64)
+    v = np.asarray(vals, np.float32)
+    if v.ndim == 2:
+        v = v[:, None, :]
+    v = np.ascontiguousarray(v)
+    nrec, ncomp, nval = v.shape
+    ts = np.ascontiguousarray(times, np.float64)
+    out = np.zeros((len(ts), nval), np.float32)
+    newest = np.zeros(len(ts), np.int32)
+    rc = lib.oc_iff_series(nrec, ptr(t), ptr(v), ncomp, comp, nval, nintp, C.c_double(dtime0),
+                           C.c_float(vdefault), len(ts), ptr(ts), ptr(out), ptr(newest))
+    return None if rc else (out, newest)
+
+
+def convert_wind(wx, wy, iwtype=1, itdrag=0, dragco=2.5e-3, roluft=1.225, rowass=1025.0, wslim=-1.0, wsmax=50.0):
+    """meteo_convert_wind_data: returns (tx, ty, ws, wx, wy, rc) (f32 arrays;
+    wx/wy after the limit; rc = 1 when a speed exceeds wsmax)."""
+    lib = load("f32")
+    x = np.array(wx, np.float32, copy=True).ravel()
+    y = np.array(wy, np.float32, copy=True).ravel()
+    n = len(x)
+    tx, ty, ws = (np.zeros(n, np.float32) for _ in range(3))
+    f = C.c_float
+    rc = lib.oc_convert_wind(n, ptr(x), ptr(y), iwtype, itdrag, f(dragco), f(roluft), f(rowass), f(wslim), f(wsmax),
+                             ptr(tx), ptr(ty), ptr(ws))
+    return tx, ty, ws, x, y, rc

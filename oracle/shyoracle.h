@@ -183,6 +183,15 @@ int  oc_bndo_init(const oc_mesh *m, const real *xgv, const real *ygv, int nb, co
 void oc_uvtopr(const oc_mesh *m, const oc_state *s, real *uprv, real *vprv);
 int  oc_bndo_setbc(const oc_mesh *m, real *cv, const real *rbc, const real *uprv, const real *vprv);
 
+/* forcing (shyoracle_forcing.c): record series and wind conversion, always
+ * in the reference's own types (f32 data, f64 times and interpolation) */
+double oc_intp_neville(int nintp, const double *xa, const float *ya, double x);
+int  oc_iff_first_record(int nrec, const double *time, int nintp, double dtime0);
+int  oc_iff_series(int nrec, const double *time, const float *vals, int ncomp, int comp, int nval, int nintp,
+                   double dtime0, float vdefault, int nt, const double *t, float *out, int *newest);
+int  oc_convert_wind(int n, float *wx, float *wy, int iwtype, int itdrag, float dragco, float roluft, float rowass,
+                     float wslim, float wsmax, float *tx, float *ty, float *ws);
+
 #ifdef __cplusplus
 }
 #endif

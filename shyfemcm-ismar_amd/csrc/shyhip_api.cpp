@@ -71,6 +71,8 @@ int fail(Model *M, const std::string &msg, int code = 1) {
 }
 
 int pipe_drain(Model &M);
+void forcing_free_bnd(Model &M);
+void forcing_free_wind(Model &M);
 
 template <typename T>
 int dalloc(Model &M, T **p, size_t n) {
@@ -322,6 +324,8 @@ int shyhip_destroy(shyhip_model *h) {
                     (void *)M.iweg_tr[0], (void *)M.iweg_tr[1]})
         dfree(q);
     for (void *q : M.tr_hbuf) dfree(q);
+    forcing_free_bnd(M);
+    forcing_free_wind(M);
     dfree(M.d_err);
     dfree(M.d_levels);
     if (M.h_levels) (void)hipHostFree(M.h_levels);
@@ -656,6 +660,8 @@ int shyhip_set_boundaries(shyhip_model *h, int nbnd, const int *ibnd, const int 
     Model &M = h->M;
     if (!M.m.ibnd) return fail(&M, "call shyhip_set_basin first");
     const int nkn = M.nkn;
+    HIPCHK(M, hipStreamSynchronize(M.stream));
+    forcing_free_bnd(M);                            // series belong to the previous boundaries
     M.nbnd = nbnd;
     M.h_ibnd.assign(ibnd, ibnd + nkn);
     M.h_ibtyp.assign(ibtyp, ibtyp + nbnd + 1);
@@ -911,6 +917,177 @@ int shyhip_set_wind_uniform(shyhip_model *h, float taux, float tauy) {
     return launch_fill(M, M.s.tauy, tauy, M.nkn);
 }
 
+// ------------------------------------------------------------ forcing series
+}  // extern "C"
+
+namespace {
+
+// first record of the initial ring for start time dtime0 (the rule of
+// iff_populate_records, intp_fem_file.f90:1065-1102): records are passed
+// while the next one (plus the record spacing for the cubic window) is
+// before dtime0; -1 when the records cannot serve dtime0
+int first_record(int nrec, const double *time, int nintp, double dtime0) {
+    const double ddt = nintp == 4 ? time[1] - time[0] : 0.0;
+    int j = 0;
+    while (time[j + 1] + ddt < dtime0) {
+        if (++j + 1 >= nrec) return -1;
+    }
+    if (dtime0 < time[0] || j + nintp > nrec) return -1;
+    return j;
+}
+
+// validates a series' times and interpolation order; fills jstart, nintp,
+// and the time range [lo, hi] the ring covers
+int series_setup(Model &M, const char *what, int nrec, const double *time, int intpol, double dtime0, SeriesDev &s,
+                 double &lo, double &hi) {
+    if (nrec < 1 || !time) return fail(&M, std::string(what) + ": no records");
+    for (int r = 1; r < nrec; r++)
+        if (!(time[r] > time[r - 1])) return fail(&M, std::string(what) + ": record times not increasing");
+    if (nrec == 1) {
+        s.nintp = 0;
+        s.jstart = 0;
+        lo = -HUGE_VAL;
+        hi = HUGE_VAL;
+        return 0;
+    }
+    if (intpol != 2 && intpol != 4) return fail(&M, std::string(what) + ": interpolation must use 2 or 4 records");
+    if (nrec < intpol) return fail(&M, std::string(what) + ": fewer records than the interpolation uses");
+    s.nintp = intpol;
+    s.jstart = first_record(nrec, time, intpol, dtime0);
+    if (s.jstart < 0) return fail(&M, std::string(what) + ": the records do not cover the start time");
+    lo = time[s.jstart];
+    hi = time[nrec - 1];
+    return 0;
+}
+
+void forcing_free_bnd(Model &M) {
+    Forcing &f = M.f;
+    for (void *q : f.bbuf) dfree(q);
+    for (void *q : {(void *)f.d_bser, (void *)f.d_zfact, (void *)f.bnd_pos}) dfree(q);
+    f.bbuf.clear(); f.bser.clear(); f.zfact.clear(); f.t_lo.clear(); f.t_hi.clear();
+    f.d_bser = nullptr; f.d_zfact = nullptr; f.bnd_pos = nullptr;
+    f.nseries = 0;
+}
+
+void forcing_free_wind(Model &M) {
+    Forcing &f = M.f;
+    dfree(f.wbuf[0]); dfree(f.wbuf[1]);
+    f.wbuf[0] = f.wbuf[1] = nullptr;
+    f.wind = SeriesDev{};
+    f.wind_on = false;
+}
+
+// every series must cover dtime (the reference stops otherwise,
+// iff_time_interpolate, intp_fem_file.f90:2344-2352)
+int forcing_check(Model &M, double t) {
+    Forcing &f = M.f;
+    for (size_t b = 0; b < f.bser.size(); b++)
+        if (f.bser[b].time && (t < f.t_lo[b] || t > f.t_hi[b]))
+            return fail(&M, "boundary " + std::to_string(b + 1) + ": time " + std::to_string(t) +
+                                " outside the records of its series", 2);
+    if (f.wind_on && (t < f.w_lo || t > f.w_hi))
+        return fail(&M, "wind: time " + std::to_string(t) + " outside the records of its series", 2);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int shyhip_set_boundary_series(shyhip_model *h, int ibc, int nrec, const double *time, const float *val, int nval,
+                               int intpol, float zfact, double dtime0) {
+    if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
+    Model &M = h->M;
+    if (ibc < 1 || ibc > M.nbnd) return fail(&M, "set_boundary_series: boundary out of range");
+    if (M.h_ibtyp[ibc] != 1) return fail(&M, "set_boundary_series: only level boundaries (ibtyp 1) take a series");
+    Forcing &f = M.f;
+    const int nbn = M.m.nbnd_nodes;
+    if (f.bser.empty()) {
+        // per-boundary tables and each node's position in its boundary
+        f.bser.assign(M.nbnd, SeriesDev{});
+        f.zfact.assign(M.nbnd, 1.f);
+        f.t_lo.assign(M.nbnd, 0.);
+        f.t_hi.assign(M.nbnd, 0.);
+        f.bbuf.assign(2 * (size_t)M.nbnd, nullptr);
+        std::vector<int> pos, cnt(M.nbnd + 1, 0);
+        for (int k = 0; k < M.nkn; k++)
+            if (M.h_ibnd[k] > 0) pos.push_back(cnt[M.h_ibnd[k]]++);
+        if (dalloc(M, &f.d_bser, M.nbnd) || dalloc(M, &f.d_zfact, M.nbnd) || dalloc(M, &f.bnd_pos, nbn) ||
+            upload(M, f.bnd_pos, pos.data(), pos.size()))
+            return 1;
+    }
+    int nk = 0;
+    for (int k = 0; k < M.nkn; k++) nk += M.h_ibnd[k] == ibc;
+    if (nval != 1 && nval != nk)
+        return fail(&M, "set_boundary_series: nval must be 1 or the boundary's node count (" + std::to_string(nk) + ")");
+    SeriesDev s{};
+    double lo, hi;
+    if (series_setup(M, "set_boundary_series", nrec, time, intpol > 0 ? intpol : 4, dtime0, s, lo, hi)) return 1;
+    if (!val) return fail(&M, "set_boundary_series: no values");
+    HIPCHK(M, hipStreamSynchronize(M.stream));     // no queued step still reads the old series
+    const int b = ibc - 1;
+    dfree(f.bbuf[2 * b]); dfree(f.bbuf[2 * b + 1]);
+    f.bbuf[2 * b] = f.bbuf[2 * b + 1] = nullptr;
+    double *dt_ = nullptr;
+    float *dv = nullptr;
+    if (dalloc(M, &dt_, nrec) || dalloc(M, &dv, (size_t)nrec * nval)) return 1;
+    f.bbuf[2 * b] = dt_;
+    f.bbuf[2 * b + 1] = dv;
+    if (upload(M, dt_, time, nrec) || upload(M, dv, val, (size_t)nrec * nval)) return 1;
+    s.time = dt_; s.val = dv; s.nrec = nrec; s.nval = nval; s.ncomp = 1;
+    if (!f.bser[b].time) f.nseries++;
+    f.bser[b] = s;
+    f.zfact[b] = zfact;
+    f.t_lo[b] = lo;
+    f.t_hi[b] = hi;
+    return upload(M, f.d_bser, f.bser.data(), f.bser.size()) || upload(M, f.d_zfact, f.zfact.data(), f.zfact.size());
+}
+
+int shyhip_set_wind_series(shyhip_model *h, int nrec, const double *time, const float *wx, const float *wy, int nval,
+                           int intpol, double dtime0, const shyhip_wind_params *wp) {
+    if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
+    Model &M = h->M;
+    HIPCHK(M, hipStreamSynchronize(M.stream));     // no queued step still reads the old series
+    forcing_free_wind(M);
+    if (nrec == 0) return 0;
+    if (!wp) return fail(&M, "set_wind_series: no parameters");
+    if (wp->iwtype != 1 && wp->iwtype != 2) return fail(&M, "set_wind_series: iwtype must be 1 (wind) or 2 (stress)");
+    if (wp->itdrag < 0 || wp->itdrag > 2) return fail(&M, "set_wind_series: itdrag must be 0, 1 or 2");
+    if (!(wp->roluft > 0.f) || !(wp->wsmax > 0.f)) return fail(&M, "set_wind_series: roluft and wsmax must be > 0");
+    if (wp->iwtype == 1 && wp->itdrag == 0 && !(wp->dragco > 0.f))
+        return fail(&M, "set_wind_series: dragco must be > 0");
+    if (nval != 1 && nval != M.nkn) return fail(&M, "set_wind_series: nval must be 1 or nkn");
+    if (!wx || !wy) return fail(&M, "set_wind_series: no values");
+    Forcing &f = M.f;
+    SeriesDev s{};
+    if (series_setup(M, "set_wind_series", nrec, time, intpol > 0 ? intpol : 2, dtime0, s, f.w_lo, f.w_hi)) return 1;
+    std::vector<float> v(2 * (size_t)nrec * nval);
+    for (int r = 0; r < nrec; r++) {
+        std::copy(wx + (size_t)r * nval, wx + (size_t)(r + 1) * nval, v.begin() + 2 * (size_t)r * nval);
+        std::copy(wy + (size_t)r * nval, wy + (size_t)(r + 1) * nval, v.begin() + (2 * (size_t)r + 1) * nval);
+    }
+    double *dt_ = nullptr;
+    float *dv = nullptr;
+    if (dalloc(M, &dt_, nrec) || dalloc(M, &dv, v.size())) return 1;
+    f.wbuf[0] = dt_;
+    f.wbuf[1] = dv;
+    if (upload(M, dt_, time, nrec) || upload(M, dv, v.data(), v.size())) return 1;
+    s.time = dt_; s.val = dv; s.nrec = nrec; s.nval = nval; s.ncomp = 2;
+    f.wind = s;
+    f.wpar = WindPar{wp->iwtype, wp->itdrag, wp->dragco, wp->roluft, M.p.rowass, wp->wslim, wp->wsmax};
+    f.wind_on = true;
+    return 0;
+}
+
+int shyhip_forcing(shyhip_model *h, double dtime) {
+    if (!h) return fail(nullptr, "null handle");
+    Model &M = h->M;
+    if (forcing_check(M, dtime)) return 2;
+    return launch_forcing(M, dtime);
+}
+
 int shyhip_set_dry(shyhip_model *h) {
     if (!h) return fail(nullptr, "null handle");
     if (pipe_drain(h->M)) return 1;
@@ -963,8 +1140,9 @@ static int report_dev_err(Model &M, int e, const char *where) {
     HIPCHK(M, hipStreamSynchronize(M.stream));
     static const char *names[] = {"", "setnod: open boundary node out of the system", "", "setuvd: z or u/v computation",
                                   "setdepth: non-positive layer thickness", "check_volume: negative volume", "",
-                                  "conz3d: zero diagonal in a column", "penta: singular vertical column"};
-    return fail(&M, std::string(where) + ": " + (e >= 0 && e <= 8 ? names[e] : "device error"), 3);
+                                  "conz3d: zero diagonal in a column", "penta: singular vertical column",
+                                  "meteo_convert_wind_data: wind speed above wsmax"};
+    return fail(&M, std::string(where) + ": " + (e >= 0 && e <= 9 ? names[e] : "device error"), 3);
 }
 
 static int check_dev_err(Model &M, const char *where) {
@@ -1111,6 +1289,11 @@ int shyhip_step(shyhip_model *h, double dtime, int istot) {
     if (launch_step_start(M, bnew, pipe)) return 1;
     M.dtime_old = M.dtime;
     timer_end(M);
+    // sp111(2): boundary levels and wind of the installed series at dtime
+    if (M.f.nseries > 0 || M.f.wind_on) {
+        if (forcing_check(M, dtime)) return 2;
+        if (launch_forcing(M, dtime)) return 1;
+    }
     if (hydro_impl(M, dtime, false)) return M.err.empty() ? 1 : 3;
     M.iweg_snap = nullptr;
     if (pipe) {

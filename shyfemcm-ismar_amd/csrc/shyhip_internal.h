@@ -27,7 +27,7 @@ constexpr int kIncWidth = 10;                        // incident elements per no
 
 // error codes written by kernels into Model::d_err[0]
 enum DevErr { ERR_NONE = 0, ERR_SETNOD = 1, ERR_SETUVD = 3, ERR_SETDEPTH = 4, ERR_VOLUME = 5,
-              ERR_CONZ = 7, ERR_PENTA = 8 };
+              ERR_CONZ = 7, ERR_PENTA = 8, ERR_WIND = 9 };
 
 struct Mesh {                 // device copies of init-time data
     int *nen3v = nullptr;     // [3*nel] element-major (ie*3+ii), 0-based
@@ -139,6 +139,39 @@ struct Solver {               // Krylov work space (value type V = S of the mode
 enum TimerId { T_SETDRY = 0, T_COPY, T_INTERN, T_SETNOD, T_ZETA, T_SOLVE, T_WETDRY, T_FINAL, T_SETUVD, T_DEPTH,
                T_VERT, T_TTOV, T_CONZ_ELEM, T_CONZ_NODE, T_BNDO, T_FACTOR, T_BICG_A, T_BICG_B, T_NTIMERS };
 
+// A record series resident on the device (time interpolation of forcing
+// files): times [nrec], values record-major [nrec][ncomp][nval]; nintp
+// points per interpolation (0: one record, a constant field), the ring of
+// records starting at record jstart; time null: no series.
+struct SeriesDev {
+    const double *time;
+    const float *val;
+    int nrec, nval, ncomp, nintp, jstart;
+};
+// meteo_convert_wind_data settings (wind forcing)
+struct WindPar {
+    int iwtype, itdrag;
+    float dragco, roluft, rowass, wslim, wsmax;
+};
+struct Forcing {
+    // boundary-level series per open boundary, its factor (zfact) and the
+    // time range its ring covers
+    std::vector<SeriesDev> bser;
+    std::vector<float> zfact;
+    std::vector<double> t_lo, t_hi;
+    std::vector<void *> bbuf;          // device buffers of each boundary's series (2 per boundary)
+    SeriesDev *d_bser = nullptr;       // [nbnd]
+    float *d_zfact = nullptr;          // [nbnd]
+    int *bnd_pos = nullptr;            // [nbnd_nodes] position of the node in its boundary's list
+    int nseries = 0;                   // boundaries with a series
+    // wind series (node values or one point) and conversion settings
+    SeriesDev wind{};
+    WindPar wpar{};
+    double w_lo = 0., w_hi = 0.;
+    void *wbuf[2] = {nullptr, nullptr};
+    bool wind_on = false;
+};
+
 struct Model {
     int prec = SHYHIP_FP64;
     int nkn = 0, nel = 0, nlvdi = 0, nlv = 0, ntr = 0;
@@ -189,6 +222,7 @@ struct Model {
     int *iweg_snap = nullptr;          // the element kernel of this step copies iwegv here
     const int *iweg_conz = nullptr;    // mask the open-boundary treatment reads (null: iwegv)
     hipEvent_t ev_levels = nullptr;   // the last upload from h_levels
+    Forcing f;                         // forcing series on the device
     // timing
     bool timing = false;
     struct TEv { int id; hipEvent_t a, b; };
@@ -284,6 +318,10 @@ int launch_copy_old(Model &M);
 int launch_step_start(Model &M, int bnewtime, bool rotate3 = false);
 int launch_set_rzv(Model &M, int ibc, float level);
 int launch_set_rzv_all(Model &M);
+// sp111(2) and meteo_force on the device at time dtime: boundary levels of
+// the boundaries with a series into rzv, the wind series into the stress
+// (shyhip_forcing.hip); the host has checked that dtime is covered
+int launch_forcing(Model &M, double dtime);
 int launch_fill(Model &M, void *ptr, double value, size_t n);
 int launch_volume(Model &M, int mode, double *vol);
 int launch_tracer_mass(Model &M, int itr, double *mass);

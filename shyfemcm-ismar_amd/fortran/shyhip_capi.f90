@@ -47,6 +47,12 @@ module shyhip_capi
     integer(c_int) :: itvd, itvdv
   end type shyhip_params
 
+  ! wind conversion settings of shyhip_set_wind_series
+  type, bind(C) :: shyhip_wind_params
+    integer(c_int) :: iwtype, itdrag
+    real(c_float)  :: dragco, roluft, wslim, wsmax
+  end type shyhip_wind_params
+
   interface
 
     function shyhip_last_error(h) bind(C, name="shyhip_last_error") result(msg)
@@ -149,6 +155,39 @@ module shyhip_capi
       import :: c_ptr, c_int, c_float
       type(c_ptr), value :: h
       real(c_float), value :: taux, tauy
+      integer(c_int) :: rc
+    end function
+
+    ! the boundary file of level boundary ibc and the wind file, held on the
+    ! device and evaluated by every step (sp111(2))
+    function shyhip_set_boundary_series(h, ibc, nrec, time, val, nval, intpol, zfact, dtime0) &
+        bind(C, name="shyhip_set_boundary_series") result(rc)
+      import :: c_ptr, c_int, c_float, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: ibc, nrec, nval, intpol
+      real(c_double), intent(in) :: time(*)
+      real(c_float), intent(in) :: val(*)
+      real(c_float), value :: zfact
+      real(c_double), value :: dtime0
+      integer(c_int) :: rc
+    end function
+
+    function shyhip_set_wind_series(h, nrec, time, wx, wy, nval, intpol, dtime0, wp) &
+        bind(C, name="shyhip_set_wind_series") result(rc)
+      import :: c_ptr, c_int, c_float, c_double, shyhip_wind_params
+      type(c_ptr), value :: h
+      integer(c_int), value :: nrec, nval, intpol
+      real(c_double), intent(in) :: time(*)
+      real(c_float), intent(in) :: wx(*), wy(*)
+      real(c_double), value :: dtime0
+      type(shyhip_wind_params), intent(in) :: wp
+      integer(c_int) :: rc
+    end function
+
+    function shyhip_forcing(h, dtime) bind(C, name="shyhip_forcing") result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: h
+      real(c_double), value :: dtime
       integer(c_int) :: rc
     end function
 

@@ -50,6 +50,14 @@ DEFAULTS = dict(dt=300.0, azpar=0.5, ampar=0.5, afpar=0.5, atpar=1.0, adpar=1.0,
                 iclin=0, rtol=float(np.float32(1.0e-8)), atol=float(np.float32(1.0e-8)), itermax=1000,
                 refine_rtol=1.0e-10, block_rows=0, tri_sweeps=0, inner_rtol=1.0e-5, rkpar=0.0, difmol=1.0e-6, itvd=0, itvdv=0)
 
+# wind conversion settings (shyhip_wind_params; STR defaults, def_para.f90:782-799)
+class WindParams(C.Structure):
+    _fields_ = [("iwtype", C.c_int), ("itdrag", C.c_int), ("dragco", C.c_float), ("roluft", C.c_float),
+                ("wslim", C.c_float), ("wsmax", C.c_float)]
+
+
+WIND_DEFAULTS = dict(iwtype=1, itdrag=0, dragco=2.5e-3, roluft=1.225, wslim=-1.0, wsmax=50.0)
+
 _lib = None
 
 
@@ -217,6 +225,43 @@ class Model:
 
     def set_wind(self, taux, tauy):
         self._chk(self.lib.shyhip_set_wind_uniform(self.h, C.c_float(taux), C.c_float(tauy)), "set_wind_uniform")
+
+    def set_boundary_series(self, ibc, time, values, intpol=0, zfact=1.0, dtime0=None):
+        """Boundary-level file of level boundary ibc held on the device:
+        time (nrec,) in seconds, values (nrec,) for one level on the whole
+        boundary or (nrec, nk) per node (the boundary's nodes in ascending
+        order); intpol 2 or 4 records (0: 4); the first ring is chosen for
+        dtime0 (default: the model's current time).  Every step then sets
+        the level from it (sp111(2))."""
+        t = np.ascontiguousarray(time, dtype=np.float64)
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        nval = 1 if v.ndim == 1 else v.shape[1]
+        d0 = self.t if dtime0 is None else dtime0
+        self._chk(self.lib.shyhip_set_boundary_series(self.h, ibc, len(t), _p(t), _p(v), nval, intpol,
+                                                      C.c_float(zfact), C.c_double(d0)), "set_boundary_series")
+
+    def set_wind_series(self, time, wx, wy, intpol=0, dtime0=None, **wind):
+        """Wind file held on the device: time (nrec,), wx/wy (nrec,) for one
+        point or (nrec, nkn) node values; wind settings as WIND_DEFAULTS
+        (iwtype 1: wind velocity, 2: stress).  Every step converts it to the
+        stress (meteo_convert_wind_data).  time=None removes the series."""
+        if time is None:
+            self._chk(self.lib.shyhip_set_wind_series(self.h, 0, None, None, None, 1, 0, C.c_double(0.0), None),
+                      "set_wind_series")
+            return
+        w = dict(WIND_DEFAULTS)
+        w.update(wind)
+        t = np.ascontiguousarray(time, dtype=np.float64)
+        x = np.ascontiguousarray(wx, dtype=np.float32)
+        y = np.ascontiguousarray(wy, dtype=np.float32)
+        nval = 1 if x.ndim == 1 else x.shape[1]
+        d0 = self.t if dtime0 is None else dtime0
+        self._chk(self.lib.shyhip_set_wind_series(self.h, len(t), _p(t), _p(x), _p(y), nval, intpol, C.c_double(d0),
+                                                  C.byref(WindParams(**w))), "set_wind_series")
+
+    def forcing(self, dtime):
+        """Evaluate the installed series at dtime (sp111(2) outside a step)."""
+        self._chk(self.lib.shyhip_forcing(self.h, C.c_double(dtime)), "forcing")
 
     def step(self, bnd_level=None, istot=1):
         """set_dry, set_timestep, copy_uvz/copy_depth, sp111 level, hydro, scal3sh."""
