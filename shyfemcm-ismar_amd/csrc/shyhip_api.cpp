@@ -975,6 +975,7 @@ void forcing_free_wind(Model &M) {
     f.wbuf[0] = f.wbuf[1] = nullptr;
     f.wind = SeriesDev{};
     f.wind_on = false;
+    f.wind_applied = false;
 }
 
 // every series must cover dtime (the reference stops otherwise,

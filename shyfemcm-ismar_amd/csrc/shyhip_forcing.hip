@@ -147,7 +147,10 @@ template <typename S>
 static int forcing_t(Model &M, double t) {
     Forcing &f = M.f;
     const int nb = f.nseries > 0 ? M.m.nbnd_nodes : 0;
-    const int nw = f.wind_on ? M.nkn : 0;
+    // a constant wind (one record) is converted on its first evaluation
+    // only, as meteo_forcing_fem converts it on the first call
+    // (meteo_forcing.f90:508-511)
+    const int nw = (f.wind_on && !(f.wind.nrec == 1 && f.wind_applied)) ? M.nkn : 0;
     if (nb + nw == 0) return 0;
     const int g = (nb + nw + kForceThreads - 1) / kForceThreads;
     SeriesDev wind = f.wind;
@@ -157,6 +160,7 @@ static int forcing_t(Model &M, double t) {
                        (const int *)f.bnd_pos, (const int *)M.m.ibtyp, (const SeriesDev *)f.d_bser,
                        (const float *)f.d_zfact, nb, wind, f.wpar, nw, t);
     HIPCHK(M, hipGetLastError());
+    if (nw > 0) f.wind_applied = true;
     return 0;
 }
 

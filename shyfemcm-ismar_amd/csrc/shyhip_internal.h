@@ -170,6 +170,7 @@ struct Forcing {
     double w_lo = 0., w_hi = 0.;
     void *wbuf[2] = {nullptr, nullptr};
     bool wind_on = false;
+    bool wind_applied = false;         // a one-record (constant) wind is converted once
 };
 
 struct Model {

@@ -17,6 +17,8 @@ from __future__ import annotations
 
 from dataclasses import dataclass
 
+import math
+
 import numpy as np
 
 from .mesh import Basin
@@ -97,6 +99,7 @@ class Ensemble:
             self.model.set_deferred_checks(True)
         self.nbnd = int(self.u.bnd_start[-1])
         self.t = 0.0
+        self.level_series = False
 
     def levels(self, t):
         lv = np.zeros(self.nbnd, np.float32)
@@ -109,9 +112,24 @@ class Ensemble:
             lv[b0:b1] = v if len(v) == b1 - b0 else v[0]
         return lv
 
+    def set_level_series(self, t_end):
+        """Every member's boundary levels held on the device as boundary
+        files: records at every time step from the current time to t_end,
+        interpolated with four records (the default for a level boundary);
+        the steps then upload no levels."""
+        dt = self.model.params_dict["dt"]
+        n = int(math.ceil((t_end - self.t) / dt))
+        rec_t = self.t + dt * np.arange(-2, n + 4)
+        lv = np.stack([self.levels(t) for t in rec_t])
+        ibtyp = self.u.basin.ibtyp
+        for b in range(self.nbnd):
+            if ibtyp[b + 1] == 1:
+                self.model.set_boundary_series(b + 1, rec_t, lv[:, b], intpol=4, dtime0=self.t)
+        self.level_series = True
+
     def step(self, istot=1):
         dt = self.model.params_dict["dt"]
-        if self.nbnd:
+        if self.nbnd and not self.level_series:
             self.model.set_boundary_levels(self.levels(self.t + dt))
         self.model.step(None, istot)
         self.t = self.model.t

@@ -150,3 +150,30 @@ def test_wind_above_wsmax_is_an_error(gpu):
     with pytest.raises(gpu.ShyhipError, match="wsmax"):
         m.step(None)                                    # 120 s: 70 m/s
     m.close()
+
+
+def test_ensemble_level_series_equal_host_levels(gpu):
+    """Ensemble members' boundary levels as device series sampled at every
+    step: interpolation at a record's time returns the record, so the run
+    equals the one that uploads the same levels each step."""
+    from shyhip import ensemble
+    mems = []
+    for n, s, phase in ((1200, 71, 0.0), (1800, 72, 2.0)):
+        b = mesh.lagoon(n, seed=s, lx=6000.0, ly=3000.0, nlv=4, tile=16)
+        mems.append(dict(basin=b, wind=mesh.wind_stress(6.0, phase),
+                         level=(lambda t, ph=phase: mesh.tide_level(t, amps=(0.4, 0.1),
+                                                                   periods=(mesh.M2_PERIOD, mesh.K1_PERIOD),
+                                                                   phases=(ph, 0.7)) - 0.2),
+                         tracers=dict(salt=mesh.smooth_tracer(b, s + 1, 25.0, 38.0).astype(np.float32))))
+    res = []
+    for series in (False, True):
+        ens = ensemble.Ensemble(mems, "fp32", params=PARAMS, visv=1e-3, difv=1e-4, deferred=True)
+        if series:
+            ens.set_level_series(PARAMS["dt"] * 20)
+        for _ in range(15):
+            ens.step()
+        ens.sync()
+        res.append([(ens.pull("znv", i), ens.pull("salt", i)) for i in range(len(mems))])
+        ens.close()
+    for (za, sa), (zb, sb) in zip(*res):
+        assert np.array_equal(za, zb) and np.array_equal(sa, sb)
