@@ -79,6 +79,7 @@ enum {
     SHYHIP_VISV, SHYHIP_DIFV, SHYHIP_DIFHV,
     SHYHIP_TAUXNV, SHYHIP_TAUYNV, SHYHIP_RZV, SHYHIP_RQV, SHYHIP_MFLUXV,
     SHYHIP_AREAKV, SHYHIP_CSR_VAL, SHYHIP_RHS,
+    SHYHIP_RHOV,               /* density anomaly rho - rowass (rhov), node 3D          */
     SHYHIP_TRACER0 = 100,      /* tracer itr: SHYHIP_TRACER0 + itr, node 3D            */
     SHYHIP_TRACER_BC0 = 200    /* its open-boundary value (-999 = none), node 3D        */
 };
@@ -179,6 +180,20 @@ int  shyhip_set_wind_series(shyhip_model *h, int nrec, const double *time, const
                             int intpol, double dtime0, const shyhip_wind_params *wp);
 /* sp111(2) + meteo_force on the device at time dtime, outside a step */
 int  shyhip_forcing(shyhip_model *h, double dtime);
+
+/* baroclinic coupling (z-layers) -------------------------------------------
+ * ibarcl = 1 (barocl, baroclinic.f90:195-600; ibarcl = 0 switches it off):
+ * tracers itemp and isalt are temperature and salinity.  Starts the density
+ * anomaly as barocl(0) does (rhov = 0, rhoset three times, :398-404); then
+ * every hydro adds the baroclinic pressure gradient to the explicit
+ * momentum terms (set_explicit -> set_barocl_new_interface,
+ * explicit.f90:193,1226-1530) and every shyhip_step ends with one rhoset
+ * (rhoset_shell(1), :542-548) from the transported T/S. */
+int  shyhip_set_baroclinic(shyhip_model *h, int ibarcl, int itemp, int isalt);
+/* rhoset (baroclinic.f90:654-720) niter times: the density anomaly from
+ * T, S and the mid-layer pressure, with the UNESCO 1980 equation of state
+ * (sigma_stp, eq_state.f90:154-235) */
+int  shyhip_rhoset(shyhip_model *h, int niter);
 
 /* time step -------------------------------------------------------------- */
 int  shyhip_set_dry(shyhip_model *h);

@@ -274,6 +274,27 @@ class OracleModel:
         self.a["tauxnv"][:] = taux
         self.a["tauynv"][:] = tauy
 
+    def set_baroclinic(self, temp="temp", salt="salt"):
+        """ibarcl = 1: barocl(0)'s start (rhov = 0, rhoset three times,
+        baroclinic.f90:398-404); then every step adds the baroclinic term to
+        the explicit momentum terms before hydro (set_explicit,
+        explicit.f90:193) and updates rhov after the transport
+        (rhoset_shell(1), baroclinic.f90:542-548)."""
+        R = self.real
+        self.bcl = (temp, salt)
+        self.rhov = np.zeros((self.basin.nkn, self.nlvdi), dtype=R)
+        self.a["fxv"] = np.zeros((self.basin.nel, self.nlvdi), dtype=R)
+        self.a["fyv"] = np.zeros((self.basin.nel, self.nlvdi), dtype=R)
+        self.state.fxv = ptr(self.a["fxv"])
+        self.state.fyv = ptr(self.a["fyv"])
+        for _ in range(3):
+            self.rhoset()
+
+    def rhoset(self):
+        t, sname = self.bcl
+        self.lib.oc_rhoset(C.byref(self.mesh), C.byref(self.params), ptr(self.tracers[t]),
+                           ptr(self.tracers[sname]), ptr(self.rhov))
+
     def step(self, bnd_level, istot=1):
         lib, m, s, p = self.lib, C.byref(self.mesh), C.byref(self.state), C.byref(self.params)
         lib.oc_set_dry(m, s, p)
@@ -282,6 +303,8 @@ class OracleModel:
         lib.oc_copy_uvz(m, s)
         lib.oc_copy_depth(m, s)
         self.set_boundary(bnd_level)
+        if getattr(self, "bcl", None):
+            lib.oc_barocl(m, p, ptr(self.rhov), ptr(self.a["fxv"]), ptr(self.a["fyv"]))
         rc = lib.oc_hydro(m, s, p, C.byref(self.stats))
         if rc != 0:
             raise RuntimeError(f"oracle hydro failed with code {rc}")
@@ -289,6 +312,8 @@ class OracleModel:
             rc = lib.oc_scal3sh(m, s, p, ptr(arr), ptr(self.tracer_bc[name]), istot, 0 if name == "temp" else 1)
             if rc != 0:
                 raise RuntimeError(f"oracle scal3sh({name}) failed with code {rc}")
+        if getattr(self, "bcl", None):
+            self.rhoset()
         return self.stats
 
     def volume(self, mode=1):
@@ -348,3 +373,14 @@ def convert_wind(wx, wy, iwtype=1, itdrag=0, dragco=2.5e-3, roluft=1.225, rowass
     rc = lib.oc_convert_wind(n, ptr(x), ptr(y), iwtype, itdrag, f(dragco), f(roluft), f(rowass), f(wslim), f(wsmax),
                              ptr(tx), ptr(ty), ptr(ws))
     return tx, ty, ws, x, y, rc
+
+
+# ------------------------------------------------------------------ baroclinic
+def sigma_stp(s, t, p, precision="f32"):
+    """sigma = rho - 1000 of the UNESCO 1980 equation of state (p in bar)."""
+    lib = load(precision)
+    T = C.c_float if precision == "f32" else C.c_double
+    f = lib.oc_sigma_stp
+    f.restype = T
+    f.argtypes = [T, T, T]
+    return f(s, t, p)

@@ -183,6 +183,11 @@ int  oc_bndo_init(const oc_mesh *m, const real *xgv, const real *ygv, int nb, co
 void oc_uvtopr(const oc_mesh *m, const oc_state *s, real *uprv, real *vprv);
 int  oc_bndo_setbc(const oc_mesh *m, real *cv, const real *rbc, const real *uprv, const real *vprv);
 
+/* baroclinic coupling, z-layers (shyoracle_barocl.c) */
+real oc_sigma_stp(real s, real t, real p);
+void oc_rhoset(const oc_mesh *m, const oc_params *p, const real *temp, const real *salt, real *rhov);
+void oc_barocl(const oc_mesh *m, const oc_params *p, const real *rhov, real *fxv, real *fyv);
+
 /* forcing (shyoracle_forcing.c): record series and wind conversion, always
  * in the reference's own types (f32 data, f64 times and interpolation) */
 double oc_intp_neville(int nintp, const double *xa, const float *ya, double x);

@@ -164,6 +164,7 @@ bool field_of(Model &M, int id, Field &f) {
     case SHYHIP_RZV: return S(&M.s.rzv, K_NODE2);
     case SHYHIP_RQV: return S(&M.s.rqv, K_NODE2);
     case SHYHIP_MFLUXV: return S(&M.s.mflux, K_NODE3);
+    case SHYHIP_RHOV: return S(&M.s.rhov, K_NODE3);
     case SHYHIP_AREAKV: return S(&M.m.areakv, K_NODE3);
     case SHYHIP_CSR_VAL: f = Field{(void **)&M.s.aval, K_CSR, false, true}; return true;
     case SHYHIP_RHS: f = Field{(void **)&M.s.rhs, K_RHS, false, true}; return true;
@@ -253,7 +254,7 @@ int shyhip_create(shyhip_model **out, int precision, int nkn, int nel, int nlvdi
     for (void **q : {&s.utn, &s.vtn, &s.uto, &s.vto, &s.uln, &s.vln, &s.hden, &s.hdeo, &s.difhv}) rc |= dalloc_s(M, q, L * ne);
     for (void **q : {&s.un, &s.vn, &s.uo, &s.vo, &s.rfric}) rc |= dalloc_s(M, q, ne);
     for (void **q : {&s.wln, &s.wlo, &s.visv, &s.difv}) rc |= dalloc_s(M, q, (L + 1) * nk);
-    for (void **q : {&s.hdkn, &s.hdko, &s.mflux}) rc |= dalloc_s(M, q, L * nk);
+    for (void **q : {&s.hdkn, &s.hdko, &s.mflux, &s.rhov}) rc |= dalloc_s(M, q, L * nk);
     for (void **q : {&s.taux, &s.tauy, &s.rzv, &s.rqv}) rc |= dalloc_s(M, q, nk);
     rc |= dalloc_s(M, &s.ddx, 2 * L * ne); rc |= dalloc_s(M, &s.ddy, 2 * L * ne);
     rc |= dalloc(M, &s.iweg, ne); rc |= dalloc(M, &s.iwet, ne); rc |= dalloc(M, &s.inod, nk);
@@ -286,7 +287,7 @@ int shyhip_destroy(shyhip_model *h) {
     State &s = M.s;
     for (void *q : {s.zn, s.zo, s.zen, s.zeo, s.utn, s.vtn, s.uto, s.vto, s.uln, s.vln, s.un, s.vn, s.uo, s.vo,
                     s.wln, s.wlo, s.hdkn, s.hdko, s.hden, s.hdeo, s.rfric, s.ddx, s.ddy, s.visv, s.difv, s.difhv,
-                    s.taux, s.tauy, s.rzv, s.rqv, s.mflux, s.hbuf})
+                    s.taux, s.tauy, s.rzv, s.rqv, s.mflux, s.rhov, s.hbuf})
         dfree(q);
     dfree(s.iweg); dfree(s.iwet); dfree(s.inod); dfree(s.zel); dfree(s.aval); dfree(s.zrow); dfree(s.rhs); dfree(s.x);
     for (void *t : s.tracer) dfree(t);
@@ -1082,6 +1083,36 @@ int shyhip_set_wind_series(shyhip_model *h, int nrec, const double *time, const 
     return 0;
 }
 
+int shyhip_set_baroclinic(shyhip_model *h, int ibarcl, int itemp, int isalt) {
+    if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
+    Model &M = h->M;
+    if (ibarcl == 0) {
+        M.bcl_on = false;
+        return 0;
+    }
+    if (ibarcl != 1) return fail(&M, "set_baroclinic: ibarcl must be 0 or 1 (prognostic T/S)");
+    if (itemp < 0 || itemp >= M.ntr || isalt < 0 || isalt >= M.ntr || itemp == isalt)
+        return fail(&M, "set_baroclinic: itemp/isalt must be two different tracers of the model");
+    M.bcl_on = true;
+    M.bcl_temp = itemp;
+    M.bcl_salt = isalt;
+    // barocl(0): rhov = 0, then rhoset three times (baroclinic.f90:398-404)
+    if (launch_fill(M, M.s.rhov, 0., (size_t)M.nlvdi * M.nkn)) return 1;
+    if (launch_rhoset(M, 3, M.stream)) return 1;
+    HIPCHK(M, hipStreamSynchronize(M.stream));
+    return 0;
+}
+
+int shyhip_rhoset(shyhip_model *h, int niter) {
+    if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
+    Model &M = h->M;
+    if (!M.bcl_on) return fail(&M, "rhoset: call shyhip_set_baroclinic first");
+    if (niter < 1) return fail(&M, "rhoset: niter must be >= 1");
+    return launch_rhoset(M, niter, M.stream);
+}
+
 int shyhip_forcing(shyhip_model *h, double dtime) {
     if (!h) return fail(nullptr, "null handle");
     Model &M = h->M;
@@ -1272,7 +1303,9 @@ static int pipe_setup(Model &M) {
 int shyhip_step(shyhip_model *h, double dtime, int istot) {
     if (!h) return fail(nullptr, "null handle");
     Model &M = h->M;
-    const bool pipe = M.deferred && !M.timing && !M.has_mflux && M.ntr >= 1;
+    // the transport overlaps the next step's hydro unless that hydro needs
+    // this step's density (baroclinic runs)
+    const bool pipe = M.deferred && !M.timing && !M.has_mflux && M.ntr >= 1 && !M.bcl_on;
     int par = 0;
     if (!pipe) {
         if (pipe_drain(M)) return 1;
@@ -1331,6 +1364,9 @@ int shyhip_step(shyhip_model *h, double dtime, int istot) {
         for (int t = 0; t < M.ntr; t++)
             if (shyhip_scal3sh(h, t, istot, t == 0 ? 0 : 1)) return 1;
     }
+    // barocl: the density of the transported T/S (rhoset_shell(1),
+    // baroclinic.f90:542-548) for the next step's baroclinic term
+    if (launch_rhoset(M, 1, M.stream)) return 1;
     if (timer_flush(M)) return 1;
     if (M.deferred && !M.timing) return 0;
     return check_dev_err(M, "step");

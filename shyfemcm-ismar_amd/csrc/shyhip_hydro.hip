@@ -38,6 +38,7 @@ struct HydroArgs {
     int *iweg, *iwet, *inod;
     S *rfric, *ddx, *ddy, *visv;
     S *taux, *tauy, *rzv, *rqv, *mflux;
+    const S *rhov;      // density anomaly (baroclinic runs), null otherwise
     double *zel, *aval, *rhs, *x;
     S *zrow;
     // inputs of the zeta solve written by the row assembly (null: not fed)
@@ -77,6 +78,7 @@ static HydroArgs<S> make_args(Model &M) {
     a.rfric = (S *)M.s.rfric; a.ddx = (S *)M.s.ddx; a.ddy = (S *)M.s.ddy; a.visv = (S *)M.s.visv;
     a.taux = (S *)M.s.taux; a.tauy = (S *)M.s.tauy; a.rzv = (S *)M.s.rzv; a.rqv = (S *)M.s.rqv;
     a.mflux = M.has_mflux ? (S *)M.s.mflux : nullptr;
+    a.rhov = M.bcl_on ? (const S *)M.s.rhov : nullptr;
     a.zel = M.s.zel; a.aval = M.s.aval; a.rhs = M.s.rhs; a.x = M.s.x;
     a.err = M.d_err;
     return a;
@@ -192,6 +194,45 @@ __global__ __launch_bounds__(kThreads) void k_hydro_intern(HydroArgs<S> a) {
         }
     }
 
+    // explicit terms: the baroclinic pressure gradient (set_explicit ->
+    // set_barocl_new_interface, explicit.f90:193,1226-1530, z-layers) when
+    // the density anomaly is carried, else zero.  Per layer the anomaly
+    // gradient from the element's P1 gradients, averaged onto the interface
+    // above (the first layer uses its own), integrated down over the
+    // interface thicknesses, times g/rho0 and the nominal layer thickness;
+    // rounded to the reference's real as fxv/fyv are.
+    R xex[NL], yex[NL];
+    if (a.rhov) {
+        const double raux = (double)(grav / rowass);
+        double px = 0., py = 0., brup = 0., crup = 0., hhup = 0.;
+#pragma unroll
+        for (int l = 1; l <= NL; l++) {
+            const size_t o = (size_t)(min(l, ilevel) - 1) * nkn;
+            const R r[3] = {a.rhov[o + k1], a.rhov[o + k2], a.rhov[o + k3]};
+            const double hlayer = (double)a.hldv[l - 1];
+            const double hh = 0.5 * hlayer;
+            const double hint = hh + hhup;
+            double br = 0., cr = 0.;
+#pragma unroll
+            for (int ii = 0; ii < 3; ii++) {
+                br = br + b[ii] * (double)r[ii];
+                cr = cr + c[ii] * (double)r[ii];
+            }
+            const double brint = l == 1 ? br : 0.5 * (br + brup);
+            const double crint = l == 1 ? cr : 0.5 * (cr + crup);
+            brup = br;
+            crup = cr;
+            hhup = hh;
+            px = px + hint * brint;
+            py = py + hint * crint;
+            xex[l - 1] = (R)(0.0 + raux * hlayer * px);
+            yex[l - 1] = (R)(0.0 + raux * hlayer * py);
+        }
+    } else {
+#pragma unroll
+        for (int l = 0; l < NL; l++) xex[l] = yex[l] = 0;
+    }
+
     double sxu[NL], sxv[NL], sh[NL];      // solutions: -F^x, -F^y and H rows
     if constexpr (!CORI) {
         double xd[NL], up[NL], gx[NL], gy[NL], gh[NL];
@@ -223,7 +264,7 @@ __global__ __launch_bounds__(kThreads) void k_hydro_intern(HydroArgs<S> a) {
             if (blast) { aa = aa + dt * rfric; aat = aat + rfric; }
             const R gravx = (R)(1.0 * grav * hhi * bz);
             const R gravy = (R)(1.0 * grav * hhi * cz);
-            const R presx = 0, presy = 0, xexpl = 0, yexpl = 0;
+            const R presx = 0, presy = 0, xexpl = xex[l - 1], yexpl = yex[l - 1];
             const R ppx_aux = aat * uui - bbt * uuip - cct * uuim - gammat * vvi + gravx + (hhi / rowass) * presx + xexpl;
             const R ppy_aux = aat * vvi - bbt * vvip - cct * vvim + gammat * uui + gravy + (hhi / rowass) * presy + yexpl;
             ppx = ppx + (double)ppx_aux;
@@ -294,8 +335,9 @@ __global__ __launch_bounds__(kThreads) void k_hydro_intern(HydroArgs<S> a) {
             if (blast) { aa = aa + dt * rfric; aat = aat + rfric; }
             const R gravx = (R)(1.0 * grav * hhi * bz);
             const R gravy = (R)(1.0 * grav * hhi * cz);
-            const R ppx_aux = aat * uui - bbt * uuip - cct * uuim - gammat * vvi + gravx;
-            const R ppy_aux = aat * vvi - bbt * vvip - cct * vvim + gammat * uui + gravy;
+            const R presx = 0, presy = 0, xexpl = xex[l - 1], yexpl = yex[l - 1];
+            const R ppx_aux = aat * uui - bbt * uuip - cct * uuim - gammat * vvi + gravx + (hhi / rowass) * presx + xexpl;
+            const R ppy_aux = aat * vvi - bbt * vvip - cct * vvim + gammat * uui + gravy + (hhi / rowass) * presy + yexpl;
             ppx = ppx + (double)ppx_aux;
             ppy = ppy + (double)ppy_aux;
             const int ju = 2 * l - 2, jv = 2 * l - 1;
@@ -1164,6 +1206,56 @@ __global__ void k_set_rzv_all(S *rzv, const int *nodes, const int *bid, const in
     if (i < n && ibtyp[bid[i]] == 1) rzv[nodes[i]] = (S)level[bid[i] - 1];
 }
 
+// sigma_stp, eq_state.f90:154-235: the UNESCO 1980 equation of state (its
+// published coefficients) in double precision; sigma = rho - 1000 (kg/m3)
+// for salinity s, temperature t (deg C) and pressure p (bar)
+template <typename R>
+__device__ inline R sigma_stp(R s, R t, R p) {
+    const double t1 = t, t2 = t1 * t1, t3 = t2 * t1, t4 = t3 * t1, t5 = t4 * t1;
+    const double s1 = s, s2 = s1 * s1, s32 = sqrt(s2 * s1);
+    const double p1 = p, p2 = p1 * p1;
+    const double rho0 = 999.842594 + 6.793952e-2 * t1 + -9.095290e-3 * t2 + 1.001685e-4 * t3 + -1.120083e-6 * t4 +
+                        6.536332e-9 * t5 +
+                        (8.24493e-1 + -4.0899e-3 * t1 + 7.6438e-5 * t2 + -8.2467e-7 * t3 + 5.3875e-9 * t4) * s1 +
+                        (-5.72466e-3 + 1.0227e-4 * t1 + -1.6546e-6 * t2) * s32 + 4.8314e-4 * s2;
+    const double sbmk = 19652.21 + 148.4206 * t1 + -2.327105 * t2 + 1.360477e-2 * t3 + -5.155288e-5 * t4 +
+                        (3.239908 + 1.43713e-3 * t1 + 1.16092e-4 * t2 + -5.77905e-7 * t3) * p1 +
+                        (8.50935e-5 + -6.12293e-6 * t1 + 5.2787e-8 * t2) * p2 +
+                        (54.6746 + -0.603459 * t1 + 1.09987e-2 * t2 + -6.1670e-5 * t3) * s1 +
+                        (7.944e-2 + 1.6483e-2 * t1 + -5.3009e-4 * t2 + 1.91075e-4 * p1) * s32 +
+                        (2.2838e-3 + -1.0981e-5 * t1 + -1.6078e-6 * t2) * p1 * s1 +
+                        (-9.9348e-7 + 2.0816e-8 * t1 + 9.1697e-10 * t2) * p2 * s1;
+    return (R)(rho0 / (1.0 - (double)p / sbmk) - 1000.0);
+}
+
+// rhoset, baroclinic.f90:654-720, z-layers: one work-item per node column,
+// top down; the mid-layer pressure (bar) is the barotropic rho0 g depth plus
+// the baroclinic part integrated from the anomaly of the previous pass, and
+// the new anomaly is sigma(max(S, 0), T, p) - (rho0 - 1000)
+template <typename S>
+__global__ __launch_bounds__(kThreads) void k_rhoset(HydroArgs<S> a, const S *temp, const S *salt, S *rhov) {
+    using R = S;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nkn) return;
+    const R grav = (R)a.grav, rho0 = (R)a.rowass;
+    const R sigma0 = rho0 - (R)1000.;
+    R depth = 0, presbc = 0;
+    const int lmax = a.ilhkv[k];
+    for (int l = 1; l <= lmax; l++) {
+        const size_t o = (size_t)(l - 1) * a.nkn + k;
+        const R hh = (R)0.5 * a.hldv[l - 1];
+        depth = depth + hh;
+        const R dpresc = rhov[o] * grav * hh;
+        presbc = presbc + dpresc;
+        const R presbt = rho0 * grav * depth;
+        const R pres = (R)1.e-5 * (presbt + presbc);
+        const R sl = salt[o] > (R)0 ? salt[o] : (R)0;
+        rhov[o] = sigma_stp<R>(sl, temp[o], pres) - sigma0;
+        depth = depth + hh;
+        presbc = presbc + dpresc;
+    }
+}
+
 // FP64 total volume (masscont, elems_dealing.f90:1191-1220): per-block sums
 // in a fixed tree order, then a single-block pass over the partials.
 template <typename S>
@@ -1494,6 +1586,20 @@ static int volume_t(Model &M, const void *hdk, const void *conc, double *out) {
     HIPCHK(M, hipStreamSynchronize(M.stream));
     return 0;
 }
+template <typename S>
+static int rhoset_t(Model &M, int niter, hipStream_t st) {
+    auto a = make_args<S>(M);
+    for (int i = 0; i < niter; i++)
+        hipLaunchKernelGGL(k_rhoset<S>, dim3(nblocks(M.nkn)), dim3(kThreads), 0, st, a,
+                           (const S *)M.s.tracer[M.bcl_temp], (const S *)M.s.tracer[M.bcl_salt], (S *)M.s.rhov);
+    HIPCHK(M, hipGetLastError());
+    return 0;
+}
+int launch_rhoset(Model &M, int niter, hipStream_t st) {
+    if (!M.bcl_on) return 0;
+    return M.prec == SHYHIP_FP64 ? rhoset_t<double>(M, niter, st) : rhoset_t<float>(M, niter, st);
+}
+
 int launch_volume(Model &M, int mode, double *vol) {
     const void *hdk = mode > 0 ? M.s.hdkn : M.s.hdko;
     return M.prec == SHYHIP_FP64 ? volume_t<double>(M, hdk, nullptr, vol) : volume_t<float>(M, hdk, nullptr, vol);

@@ -95,6 +95,7 @@ struct State {
     void *rfric = nullptr, *ddx = nullptr, *ddy = nullptr;
     void *visv = nullptr, *difv = nullptr, *difhv = nullptr;
     void *taux = nullptr, *tauy = nullptr, *rzv = nullptr, *rqv = nullptr, *mflux = nullptr;
+    void *rhov = nullptr;     // density anomaly rho - rowass [l*nkn + k] (baroclinic runs)
     std::vector<void *> tracer;
     std::vector<void *> tracer_bc;   // boundary values per tracer (-999 = none)
     double *zel = nullptr;    // [6*nel] dbb dbc dcb dcc ut vt (hydro_zeta inputs)
@@ -187,6 +188,9 @@ struct Model {
     int *h_pinned = nullptr;  // pinned host mirror for small readbacks
     double dtime = 0., dtime_old = -1.;
     bool has_coriolis = false, has_mflux = false, tvd_ready = false;
+    // baroclinic coupling (ibarcl = 1): tracers holding T and S
+    bool bcl_on = false;
+    int bcl_temp = -1, bcl_salt = -1;
     int last_nloop = 0, last_iters = 0, last_refine = 0;
     // independent basins held in one model: member m owns nodes
     // [mem_node[m], mem_node[m+1]) and elements [mem_elem[m], mem_elem[m+1])
@@ -323,6 +327,9 @@ int launch_set_rzv_all(Model &M);
 // the boundaries with a series into rzv, the wind series into the stress
 // (shyhip_forcing.hip); the host has checked that dtime is covered
 int launch_forcing(Model &M, double dtime);
+// rhoset (baroclinic.f90:654-720) niter times: the density anomaly of every
+// node-layer from T, S and the mid-layer pressure (shyhip_hydro.hip)
+int launch_rhoset(Model &M, int niter, hipStream_t st);
 int launch_fill(Model &M, void *ptr, double value, size_t n);
 int launch_volume(Model &M, int mode, double *vol);
 int launch_tracer_mass(Model &M, int itr, double *mass);

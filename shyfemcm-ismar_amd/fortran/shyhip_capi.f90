@@ -27,6 +27,7 @@ module shyhip_capi
   integer(c_int), parameter :: SHYHIP_TAUXNV = 30, SHYHIP_TAUYNV = 31, SHYHIP_RZV = 32
   integer(c_int), parameter :: SHYHIP_RQV = 33, SHYHIP_MFLUXV = 34
   integer(c_int), parameter :: SHYHIP_AREAKV = 35, SHYHIP_CSR_VAL = 36, SHYHIP_RHS = 37
+  integer(c_int), parameter :: SHYHIP_RHOV = 38
   integer(c_int), parameter :: SHYHIP_TRACER0 = 100, SHYHIP_TRACER_BC0 = 200
 
   type, bind(C) :: shyhip_params
@@ -181,6 +182,21 @@ module shyhip_capi
       real(c_float), intent(in) :: wx(*), wy(*)
       real(c_double), value :: dtime0
       type(shyhip_wind_params), intent(in) :: wp
+      integer(c_int) :: rc
+    end function
+
+    ! baroclinic coupling: tracers itemp/isalt are T and S (ibarcl 1)
+    function shyhip_set_baroclinic(h, ibarcl, itemp, isalt) bind(C, name="shyhip_set_baroclinic") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      integer(c_int), value :: ibarcl, itemp, isalt
+      integer(c_int) :: rc
+    end function
+
+    function shyhip_rhoset(h, niter) bind(C, name="shyhip_rhoset") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      integer(c_int), value :: niter
       integer(c_int) :: rc
     end function
 

@@ -28,7 +28,7 @@ F32, F64, I32 = 0, 1, 2
 FIELDS = dict(znv=1, zov=2, zenv=3, zeov=4, utlnv=5, vtlnv=6, utlov=7, vtlov=8, unv=9, vnv=10, uov=11, vov=12,
               ulnv=13, vlnv=14, wlnv=15, wlov=16, hdknv=17, hdkov=18, hdenv=19, hdeov=20, iwegv=21, iwetv=22,
               inodv=23, rfricv=24, ddxv=25, ddyv=26, visv=27, difv=28, difhv=29, tauxnv=30, tauynv=31, rzv=32,
-              rqv=33, mfluxv=34, areakv=35, csr_val=36, rhs=37)
+              rqv=33, mfluxv=34, areakv=35, csr_val=36, rhs=37, rhov=38)
 TRACER0 = 100
 TRACER_BC0 = 200
 INT_FIELDS = {"iwegv", "iwetv", "inodv"}
@@ -162,7 +162,7 @@ class Model:
                       difv=(b.nkn, L + 1), iwegv=(b.nel,), iwetv=(b.nel,), inodv=(b.nkn,), rfricv=(b.nel,),
                       ddxv=(b.nel, 2 * L), ddyv=(b.nel, 2 * L), tauxnv=(b.nkn,), tauynv=(b.nkn,), rzv=(b.nkn,),
                       rqv=(b.nkn,), hdknv=(b.nkn, L), hdkov=(b.nkn, L), mfluxv=(b.nkn, L), areakv=(b.nkn, L),
-                      rhs=(b.nkn,))
+                      rhs=(b.nkn,), rhov=(b.nkn, L))
         if name in shapes:
             return shapes[name]
         if name in ("utlnv", "vtlnv", "utlov", "vtlov", "ulnv", "vlnv", "hdenv", "hdeov", "difhv"):
@@ -258,6 +258,18 @@ class Model:
         d0 = self.t if dtime0 is None else dtime0
         self._chk(self.lib.shyhip_set_wind_series(self.h, len(t), _p(t), _p(x), _p(y), nval, intpol, C.c_double(d0),
                                                   C.byref(WindParams(**w))), "set_wind_series")
+
+    def set_baroclinic(self, temp="temp", salt="salt", ibarcl=1):
+        """Baroclinic coupling with the named tracers as T and S (ibarcl 1;
+        0 switches it off): starts rhov as barocl(0) does, then every step
+        adds the baroclinic term before hydro and updates rhov after the
+        transport."""
+        it = self.tracer_names.index(temp) if ibarcl else 0
+        isl = self.tracer_names.index(salt) if ibarcl else 0
+        self._chk(self.lib.shyhip_set_baroclinic(self.h, ibarcl, it, isl), "set_baroclinic")
+
+    def rhoset(self, niter=1):
+        self._chk(self.lib.shyhip_rhoset(self.h, niter), "rhoset")
 
     def forcing(self, dtime):
         """Evaluate the installed series at dtime (sp111(2) outside a step)."""
