@@ -147,17 +147,20 @@ def install_forcing(m, nsteps):
     m.set_wind_series([0.0], [np.float32(sp * math.cos(d))], [np.float32(sp * math.sin(d))])
 
 
-def run_gpu(b, tr, prec, args):
+def run_gpu(b, tr, prec, args, steps=None, baroclinic=False):
     import shyhip
+    steps = args.steps if steps is None else steps
     m = shyhip.Model(b, prec, params=PARAMS, visv=1e-3, difv=1e-4, tracers=tr)
     m.set_deferred_checks(True)     # device errors surface at the next wait (m.sync below)
-    nt = max(5, min(20, args.steps))
-    install_forcing(m, args.warmup + args.steps + nt)
+    if baroclinic:                  # T and S drive the momentum equations (ibarcl = 1)
+        m.set_baroclinic("temp", "salt")
+    nt = max(5, min(20, steps))
+    install_forcing(m, args.warmup + steps + nt)
     for _ in range(args.warmup):
         m.step(None)
     m.sync()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         m.step(None)
     m.sync()
     elapsed = time.perf_counter() - t0
@@ -303,6 +306,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-baroclinic", action="store_true")
+    ap.add_argument("--bcl-steps", type=int, default=360)
     ap.add_argument("--ens-members", type=int, default=16)
     ap.add_argument("--ens-min", type=int, default=5000)
     ap.add_argument("--ens-max", type=int, default=40000)
@@ -373,6 +378,16 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_launch": kb[dom], "avg_launch_ms": round(avg[dom], 5)},
     }
+    if not args.no_baroclinic:
+        bcl = {p: run_gpu(b, tr, p, args, steps=args.bcl_steps, baroclinic=True) for p in ("fp64", "fp32")}
+        out["baroclinic"] = {
+            "workload": "the same lagoon with T and S driving the momentum equations (ibarcl = 1: UNESCO "
+                        "density after each transport, baroclinic pressure gradient in the element columns)",
+            "steps": args.bcl_steps,
+            **{f"{p}_ms_per_step": round(1e3 * r["elapsed"] / args.bcl_steps, 4) for p, r in bcl.items()},
+            **{f"{p}_value": round(EL * args.bcl_steps / r["elapsed"], 1) for p, r in bcl.items()},
+            "solver": {p: r["stats"] for p, r in bcl.items()},
+        }
     if not args.no_ensemble:
         mems = ensemble_members(args)
         e32, t32, z32 = run_ensemble(mems, "fp32", args.ens_steps, 5)
