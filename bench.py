@@ -307,7 +307,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-baroclinic", action="store_true")
-    ap.add_argument("--bcl-steps", type=int, default=360)
+    ap.add_argument("--bcl-steps", type=int, default=60)
     ap.add_argument("--ens-members", type=int, default=16)
     ap.add_argument("--ens-min", type=int, default=5000)
     ap.add_argument("--ens-max", type=int, default=40000)
@@ -379,15 +379,30 @@ def main():
                      "bytes_per_launch": kb[dom], "avg_launch_ms": round(avg[dom], 5)},
     }
     if not args.no_baroclinic:
-        bcl = {p: run_gpu(b, tr, p, args, steps=args.bcl_steps, baroclinic=True) for p in ("fp64", "fp32")}
-        out["baroclinic"] = {
-            "workload": "the same lagoon with T and S driving the momentum equations (ibarcl = 1: UNESCO "
-                        "density after each transport, baroclinic pressure gradient in the element columns)",
-            "steps": args.bcl_steps,
-            **{f"{p}_ms_per_step": round(1e3 * r["elapsed"] / args.bcl_steps, 4) for p, r in bcl.items()},
-            **{f"{p}_value": round(EL * args.bcl_steps / r["elapsed"], 1) for p, r in bcl.items()},
-            "solver": {p: r["stats"] for p, r in bcl.items()},
-        }
+        # T and S drive the momentum equations.  With the workload's fields
+        # (15-25 C, 25-38 psu, salinity falling with depth) the added forcing
+        # drains the 1 m top layer of multi-layer elements at this shallow
+        # lagoon's inlet within tens of steps, which stops the model as it
+        # stops the reference (setdepth).  The coupled run uses a stable
+        # stratification (T -0.2 C and S +0.3 psu per layer) under 1/10 of
+        # the horizontal contrasts, and is short: the oracle run of it stops
+        # after about 160 steps the same way.
+        lev = np.arange(b.nlv)[None, :]
+        trb = {k: (v[:, :1].mean() + 0.1 * (v[:, :1] - v[:, :1].mean())
+                   + (-0.2 if k == "temp" else 0.3) * lev).astype(np.float32) for k, v in tr.items()}
+        try:
+            bcl = {p: run_gpu(b, trb, p, args, steps=args.bcl_steps, baroclinic=True) for p in ("fp64", "fp32")}
+            out["baroclinic"] = {
+                "workload": "the same lagoon with T and S driving the momentum equations (ibarcl = 1: UNESCO "
+                            "density after each transport, baroclinic pressure gradient in the element columns), "
+                            "stable stratification, 1/10 of the T/S horizontal contrasts",
+                "steps": args.bcl_steps,
+                **{f"{p}_ms_per_step": round(1e3 * r["elapsed"] / args.bcl_steps, 4) for p, r in bcl.items()},
+                **{f"{p}_value": round(EL * args.bcl_steps / r["elapsed"], 1) for p, r in bcl.items()},
+                "solver": {p: r["stats"] for p, r in bcl.items()},
+            }
+        except Exception as e:      # a model stop is a result of the run, reported, not a crash of the bench
+            out["baroclinic"] = {"error": str(e)}
     if not args.no_ensemble:
         mems = ensemble_members(args)
         e32, t32, z32 = run_ensemble(mems, "fp32", args.ens_steps, 5)
