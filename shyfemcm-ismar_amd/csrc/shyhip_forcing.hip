@@ -31,10 +31,23 @@ namespace shy {
 constexpr int kForceThreads = 256;
 constexpr float kNoData = -999.f;     // record value meaning "no data" (the files' flag)
 
-// newest record of the window for time t
+// newest record of the window for time t: the smallest j in [lo, hi] whose
+// check time time[j + off] is not before t (hi when there is none).  The
+// first guess assumes evenly spaced records and is confirmed by two loads
+// issued together; a binary search over what remains covers the rest.
 __device__ inline int window_last(const SeriesDev &s, double t) {
     const int off = 1 - s.nintp / 2;
     int lo = s.jstart + s.nintp - 1, hi = s.nrec - 1;
+    const double t0 = s.time[lo + off], t1 = s.time[hi + off];
+    if (t <= t0) return lo;
+    if (t > t1) return hi;
+    // here the answer lies in (lo, hi]
+    int g = lo + (int)ceil((t - t0) / (t1 - t0) * (double)(hi - lo));
+    g = min(hi, max(lo + 1, g));
+    const double tg = s.time[g + off], tp = s.time[g - 1 + off];
+    if (tg >= t && tp < t) return g;
+    if (tg >= t) hi = g - 1;     // time[g-1+off] >= t as well
+    else lo = g + 1;             // time[g+off] < t
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
         if (s.time[mid + off] >= t) hi = mid;
