@@ -181,6 +181,34 @@ int  shyhip_set_wind_series(shyhip_model *h, int nrec, const double *time, const
 /* sp111(2) + meteo_force on the device at time dtime, outside a step */
 int  shyhip_forcing(shyhip_model *h, double dtime);
 
+/* output in the SHY format ----------------------------------------------------
+ * (io_shy.f90, format version 14; the file SHYFEM's post-processing reads).
+ * shyhip_shy_open writes the header (shy_write_header, io_shy.f90:1550-1606)
+ * from host arrays in the reference's layouts: nen3v(3,nel) 1-based,
+ * hm3v(3,nel), external numbers ipev/ipv and area codes iarv/iarnv (null:
+ * 1..n and 0), simpar(4) = hzmin, hzoff, nzadapt, dtaver (null: zeros).
+ * ftype 1: hydro (npr 3, nvar 4), 2: scalars on nodes (npr 1).
+ * shyhip_shy_write_record writes one data record (shy_write_record,
+ * :1610-1681): c(nlvddi, n*m), a 3D record (lmax > 1) packed over each
+ * node's/element's own layers.  Status 0 = ok; shyhip_shy_error gives the
+ * text.  Host code only: no GPU is needed. */
+typedef struct shyhip_shyfile shyhip_shyfile;
+int  shyhip_shy_open(const char *path, int ftype, int nkn, int nel, int npr, int nlv, int nvar, int date, int time,
+                     const char *title, const char *femver, const float *simpar, const int *nen3v, const int *ipev,
+                     const int *ipv, const int *iarv, const int *iarnv, const float *xgv, const float *ygv,
+                     const float *hm3v, const float *hlv, const int *ilhv, const int *ilhkv, shyhip_shyfile **f);
+int  shyhip_shy_write_record(shyhip_shyfile *f, double dtime, int ivar, int belem, int n, int m, int lmax, int nlvddi,
+                             const float *c);
+int  shyhip_shy_close(shyhip_shyfile *f);
+const char *shyhip_shy_error(const shyhip_shyfile *f);
+/* one hydro time record of the model's device state: znv, zenv, utlnv,
+ * vtlnv (shy_write_hydro_records, shy_util.f90:1210-1245, as called by the
+ * hydro output, ous_admin.f90:122-127) */
+int  shyhip_shy_write_hydro(shyhip_model *h, shyhip_shyfile *f, double dtime);
+/* one node 3D field of the model (e.g. SHYHIP_TRACER0 + itr) as a scalar
+ * record with variable id ivar */
+int  shyhip_shy_write_node3d(shyhip_model *h, shyhip_shyfile *f, double dtime, int ivar, int field);
+
 /* baroclinic coupling (z-layers) -------------------------------------------
  * ibarcl = 1 (barocl, baroclinic.f90:195-600; ibarcl = 0 switches it off):
  * tracers itemp and isalt are temperature and salinity.  Starts the density

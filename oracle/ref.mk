@@ -3,7 +3,10 @@
 # PENTA_FACT/PENTA_SOLVE/TRIA_MULTI), compiled from their sources where they
 # lie in the reference tree with AMD flang, into one shared library used
 # only by tests/test_oracle_ref.py to check the C restatement in this
-# directory.  Nothing is copied; the build writes only to oracle/_ref/.
+# directory; and the reference's SHY file reader (io_shy.f90 with
+# to_linear.f90) linked with shy_dump.f90 from this directory into
+# _ref/shy_dump, used only by tests/test_shy_cpu.py to read the files the
+# library writes.  Nothing is copied; the build writes only to oracle/_ref/.
 #   make -f ref.mk            (needs the reference tree at REF)
 REF   ?= /root/reference
 FLANG ?= /opt/rocm/llvm/bin/flang
@@ -12,7 +15,14 @@ SRC   := $(SPK)/spk_ilut.f90 $(SPK)/spk_iters.f90 $(SPK)/spk_itaux.f90 $(SPK)/sp
          $(SPK)/spk_matvec.f90 $(SPK)/penta.f90
 OUT   := _ref
 
-all: $(OUT)/libspkref.so
+SHYSRC := $(REF)/src/utils/generic/to_linear.f90 $(REF)/src/utils/shyutil/io_shy.f90
+
+all: $(OUT)/libspkref.so $(OUT)/shy_dump
+
+$(OUT)/shy_dump: $(SHYSRC) shy_dump.f90
+	@mkdir -p $(OUT)/shy
+	cd $(OUT)/shy && for f in $(SHYSRC); do $(FLANG) -O1 -c $$f || exit 1; done
+	$(FLANG) -O1 -module-dir $(OUT)/shy -o $@ shy_dump.f90 $(addprefix $(OUT)/shy/,$(notdir $(SHYSRC:.f90=.o)))
 
 $(OUT)/libspkref.so: $(SRC)
 	@mkdir -p $(OUT)

@@ -171,6 +171,14 @@ class Model:
             return (b.nkn, L)
         raise KeyError(name)
 
+    def field_id(self, name):
+        """Field id of a state array or a tracer (by name)."""
+        if isinstance(name, int):
+            return name
+        if name in self.tracer_names:
+            return TRACER0 + self.tracer_names.index(name)
+        return FIELDS[name]
+
     def push(self, name, arr):
         fid = name if isinstance(name, int) else FIELDS[name]
         if name in INT_FIELDS:
