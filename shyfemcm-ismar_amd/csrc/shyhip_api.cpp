@@ -632,6 +632,7 @@ int shyhip_set_basin(shyhip_model *h, const int *nen3v, const float *xgv, const 
         }
         Solver &v = M.sv;
         rc |= dalloc_s(M, &v.lu, (size_t)bs.nblk * mx);
+        v.have_lu = false;
         rc |= dalloc_s(M, &v.ell_a, (size_t)kEllWidth * nkn);
         for (void **q : {&v.r0, &v.r, &v.p0, &v.p1, &v.v0, &v.v1, &v.s, &v.t, &v.dx, &v.b, &v.x0})
             rc |= dalloc_s(M, q, (size_t)nkn);

@@ -131,6 +131,9 @@ struct Solver {               // Krylov work space (value type V = S of the mode
     void *a32 = nullptr;      // FP32 copy of the matrix values (FP32 path)
     void *zprev = nullptr;    // FP32 path: level at the start of the previous step
     bool have_prev = false;   // zprev holds a level of this run's time series
+    bool have_lu = false;     // lu holds the block factors of an earlier matrix of this pattern
+    bool last_warm = false;   // the last factorisation refined earlier factors
+    int solves_since_exact = 0;
     bool fed = false;         // the assembly already wrote the solve's inputs (ELL
                               // slots, FP32 copy, initial guess, cleared words)
     double *r64 = nullptr;    // FP64 residual (refinement)

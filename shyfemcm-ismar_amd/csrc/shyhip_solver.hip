@@ -439,9 +439,17 @@ struct FacRow {
 // two entries packed together, all relative to the block; with stage the
 // descriptors and pairs are copied to LDS.
 constexpr int kFactorThreads = 256;
+// Warm start (sweeps > 0): the factors of the previous solve, whose matrix
+// differs little, refined by synchronous sweeps over all entries at once
+// (Chow and Patel, fine-grained parallel ILU, SIAM J. Sci. Comput. 37,
+// 2015): every entry is recomputed from the current values of the others,
+// in the same operations as above; after k sweeps every entry within
+// dependence depth k is exact.  kWarmEntries bounds the entries a
+// work-item keeps the matrix values of (larger blocks run cold).
+constexpr int kWarmEntries = 8;
 
 template <typename V>
-__global__ __launch_bounds__(kFactorThreads) void k_bilu_factor(SolvArgs<V> a, int stage) {
+__global__ __launch_bounds__(kFactorThreads) void k_bilu_factor(SolvArgs<V> a, int stage, int sweeps) {
     extern __shared__ double smem[];
     V *val = reinterpret_cast<V *>(smem);
     const int blk = blockIdx.x, b0 = blk * a.brows;
@@ -452,8 +460,10 @@ __global__ __launch_bounds__(kFactorThreads) void k_bilu_factor(SolvArgs<V> a, i
     const int *elp = a.elp + (size_t)blk * (a.edmax + 1);   // level starts
     const int4 *desc = a.fdesc + (size_t)blk * a.mx;         // entries by level
     const int2 *pairs = a.fpair + p0;
-    // the lists start 16-byte aligned after the values
-    unsigned char *lfree = reinterpret_cast<unsigned char *>(smem) + ((a.mx * sizeof(V) + 15) & ~(size_t)15);
+    // the lists start 16-byte aligned after the values (two value buffers
+    // when warm)
+    const size_t vreg = (a.mx * sizeof(V) + 15) & ~(size_t)15;
+    unsigned char *lfree = reinterpret_cast<unsigned char *>(smem) + vreg * (sweeps > 0 ? 2 : 1);
     if (stage) {
         int4 *l_d = reinterpret_cast<int4 *>(lfree);
         int2 *l_p = reinterpret_cast<int2 *>(l_d + a.mx);
@@ -463,9 +473,55 @@ __global__ __launch_bounds__(kFactorThreads) void k_bilu_factor(SolvArgs<V> a, i
         for (int t = threadIdx.x; t <= nl; t += blockDim.x) l_e[t] = elp[t];
         desc = l_d; pairs = l_p; elp = l_e;
     }
-    for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) val[q - q0] = (V)a.aval[a.bsrc[q]];
-    __syncthreads();
-    for (int lv = 0; lv < nl; lv++) {
+    if (sweeps > 0) {
+        // this work-item's entries: descriptors and matrix values in registers
+        V av[kWarmEntries];
+        int4 dd[kWarmEntries];
+        const V *prev = a.lu + (size_t)blk * a.mx;
+        V *nxt = reinterpret_cast<V *>(reinterpret_cast<unsigned char *>(smem) + vreg);
+#pragma unroll
+        for (int k = 0; k < kWarmEntries; k++) {
+            const int t = threadIdx.x + k * blockDim.x;
+            if (t < ne) {
+                dd[k] = desc[t];
+                av[k] = (V)a.aval[a.bsrc[q0 + dd[k].x]];
+            }
+        }
+        for (int e = threadIdx.x; e < ne; e += blockDim.x) val[e] = prev[e];
+        __syncthreads();
+        V *cur = val;
+        for (int sw = 0; sw < sweeps; sw++) {
+#pragma unroll
+            for (int k = 0; k < kWarmEntries; k++) {
+                const int t = threadIdx.x + k * blockDim.x;
+                if (t >= ne) break;
+                const int4 d = dd[k];
+                V acc = av[k];
+                for (int p = d.y; p < d.z; p++) {
+                    const int2 pr = pairs[p];
+                    acc = acc - cur[pr.x] * cur[pr.y];
+                }
+                V out;
+                if (d.w >= 0) out = acc * cur[d.w];
+                else if (d.w == -1) out = acc;
+                else {
+                    if (acc == (V)0) atomicCAS(a.ctrl + CT_FACTOR, 0, 1 + b0 + (-2 - d.w));
+                    out = (V)1 / acc;
+                }
+                nxt[d.x] = out;
+            }
+            __syncthreads();
+            V *tmp = cur; cur = nxt; nxt = tmp;
+        }
+        if (cur != val) {
+            for (int e = threadIdx.x; e < ne; e += blockDim.x) val[e] = cur[e];
+            __syncthreads();
+        }
+    } else {
+        for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) val[q - q0] = (V)a.aval[a.bsrc[q]];
+        __syncthreads();
+    }
+    for (int lv = 0; lv < (sweeps > 0 ? 0 : nl); lv++) {
         for (int t = elp[lv] + threadIdx.x; t < elp[lv + 1]; t += blockDim.x) {
             const int4 d = desc[t];
             V acc = val[d.x];
@@ -973,15 +1029,26 @@ static size_t apply_lds(const Model &M) {
 // then no longer precondition it.
 // LDS of the factorisation: the block's values, and when they fit the
 // descriptors, pairs and level starts
+// sweeps of a warm start (four reach the iteration counts of exact factors
+// on the 50k-node lagoon, two do not), and exact factorisations at least
+// every kWarmRefresh solves
+constexpr int kWarmSweeps = 4;
+constexpr int kWarmRefresh = 16;
+
 template <typename V>
 static int factor(Model &M, SolvArgs<V> &a) {
-    const size_t vals = ((size_t)M.sv.max_blk_nnz * sizeof(V) + 15) & ~(size_t)15;
+    const int sweeps = (kWarmSweeps > 0 && M.sv.have_lu && M.sv.max_blk_nnz <= kWarmEntries * kFactorThreads)
+                           ? kWarmSweeps : 0;
+    M.sv.last_warm = sweeps > 0;
+    if (sweeps == 0) M.sv.solves_since_exact = 0;
+    const size_t vals = (((size_t)M.sv.max_blk_nnz * sizeof(V) + 15) & ~(size_t)15) * (sweeps > 0 ? 2 : 1);
     const size_t lists = (size_t)M.sv.max_blk_nnz * sizeof(int4) + (size_t)M.sv.max_blk_pairs * sizeof(int2) +
                          (size_t)(M.sv.max_elev + 1) * sizeof(int);
     const int stage = vals + lists <= kLdsLimit ? 1 : 0;
     launch_timed(M, T_FACTOR, k_bilu_factor<V>, dim3(M.sv.nblk), dim3(kFactorThreads), stage ? vals + lists : vals,
-                 M.stream, a, stage);
+                 M.stream, a, stage, sweeps);
     HIPCHK(M, hipGetLastError());
+    M.sv.have_lu = true;
     return 0;
 }
 
@@ -1074,10 +1141,16 @@ static int enqueue_inner(Model &M, SolvArgs<V> &a, int chunk) {
 
 int solver_setup(Model &M) { (void)M; return 0; }
 
-int solver_solve(Model &M, int *iters, int *refinements, int wetdry_bnew) {
-    // inputs already written by the row assembly of this step's system?
-    const bool fed = M.sv.fed;
-    M.sv.fed = false;
+// the words of one zeta solve, for a second attempt at it
+__global__ void k_ctrl_restart(int *ctrl) {
+    if (threadIdx.x == 0) {
+        ctrl[CT_FACTOR] = 0; ctrl[CT_ITS] = 0; ctrl[CT_NREF] = 0; ctrl[CT_REFCONV] = 0;
+    }
+}
+
+// one zeta solve; fed: the row assembly already wrote its inputs (or a
+// first attempt did)
+static int solve_once(Model &M, int *iters, int *refinements, int wetdry_bnew, bool fed) {
     const int n = M.nkn;
     const int g1 = (n + kThreads - 1) / kThreads;
     int it_total = 0, nref = 0;
@@ -1188,6 +1261,26 @@ int solver_solve(Model &M, int *iters, int *refinements, int wetdry_bnew) {
     if (iters) *iters = it_total;
     if (refinements) *refinements = nref;
     return 0;
+}
+
+// The block factors are refined from the previous solve's (warm) except on
+// the first solve and every kWarmRefresh-th, which factorise exactly.  A
+// solve that fails with refined factors (a matrix that changed too much for
+// the sweeps: rows switched by wetting and drying) is done again with exact
+// factors from where it stopped, so the warm start can cost iterations but
+// never a solve.
+int solver_solve(Model &M, int *iters, int *refinements, int wetdry_bnew) {
+    // inputs already written by the row assembly of this step's system?
+    const bool fed = M.sv.fed;
+    M.sv.fed = false;
+    if (++M.sv.solves_since_exact >= kWarmRefresh) M.sv.have_lu = false;
+    const int rc = solve_once(M, iters, refinements, wetdry_bnew, fed);
+    if (rc != 2 || !M.sv.last_warm) return rc;
+    M.sv.have_lu = false;
+    hipLaunchKernelGGL(k_ctrl_restart, dim3(1), dim3(64), 0, M.stream, M.sv.ctrl);
+    HIPCHK(M, hipGetLastError());
+    M.err.clear();
+    return solve_once(M, iters, refinements, wetdry_bnew, true);
 }
 
 }  // namespace shy
