@@ -11,8 +11,9 @@ resident on the device before the timed region.
 A "step" is one model time step: set_dry, copy_uvz/copy_depth, hydro (column
 solves, zeta assembly, sparse solve, wet/dry loop, final transports, depths,
 vertical velocity) and scal3sh for T and S.  Two device paths are timed:
-plain FP64, and FP32 state/transport with the FP32 Krylov solve inside FP64
-iterative refinement (the headline value).  Beside them, unless switched
+plain FP64 (the headline value: the workload is the plain FP64 GPU run), and
+FP32 state/transport with the FP32 Krylov solve inside FP64 iterative
+refinement, reported beside it.  Beside them, unless switched
 off: a batched ensemble of uneven lagoons (all members in one model) against
 the same members run one after the other on the plain FP64 path, a serving
 stream with p50/p99 latency, and the CPU oracle on a bounded sample.
@@ -334,46 +335,48 @@ def main():
     thr = {p: EL * args.steps / r["elapsed"] for p, r in res.items()}
     dz = np.abs(res["fp32"]["znv"] - res["fp64"]["znv"])
 
-    # roofline of the headline path's dominant kernel (largest time per step)
-    head = res["fp32"]
-    kb = _kernel_bytes(b, 4, PARAMS["block_rows"])
+    # roofline of the headline path's dominant kernel (largest time per step).
+    # The headline workload is the lagoon on the plain FP64 GPU path; the
+    # FP32 + FP64-refined path runs the same workload beside it.
+    head = res["fp64"]
+    kb = _kernel_bytes(b, 8, PARAMS["block_rows"])
     per_step = {k: v[0] / head["nt"] for k, v in head["timing"].items()}
     avg = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in head["timing"].items()}
     dom = max((k for k in kb if avg.get(k, 0.0) > 0), key=lambda k: per_step[k])
     achieved = kb[dom] / (avg[dom] * 1e-3) / 1e9
     kernel_gbs = {k: round(kb[k] / (avg[k] * 1e-3) / 1e9, 1) for k in kb if avg.get(k, 0.0) > 0}
-    traffic = _measured_traffic(args.traffic, dom, "float")
+    traffic = _measured_traffic(args.traffic, dom, "double")
     if args.copy_calibration > 0:
         _copy_calibration(args.copy_calibration)
 
     out = {
-        "metric": "elem*layer*steps/s (FP32 + FP64-refined path; plain FP64 path, batched ensemble and "
-                  "serving p50/p99 reported beside it)",
-        "value": round(thr["fp32"], 1),
+        "metric": "elem*layer*steps/s (3D lagoon, plain FP64 GPU path; the FP32 + FP64-refined path, the batched "
+                  "FP32+refine ensemble against plain FP64, and serving p50/p99 reported beside it)",
+        "value": round(thr["fp64"], 1),
         "unit": "elem*layer*steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * res["fp32"]["elapsed"] / args.steps, 4),
+        "ms_per_step": round(1e3 * res["fp64"]["elapsed"] / args.steps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 state/transport + f64 refinement",
+        "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"3D lagoon: {b.nkn} nodes, {b.nel} elements, {args.layers} z-layers, T+S (TVD), "
                                f"dt=60 s, M2+K1 tide (boundary file at every step, cubic interpolation on the "
                                f"device), 8 m/s wind, seed {args.seed}",
                    "elem_layers": EL, "parallelism": "single GPU"},
-        "fp64_value": round(thr["fp64"], 1),
-        "fp64_ms_per_step": round(1e3 * res["fp64"]["elapsed"] / args.steps, 4),
+        "fp32_refined_value": round(thr["fp32"], 1),
+        "fp32_refined_ms_per_step": round(1e3 * res["fp32"]["elapsed"] / args.steps, 4),
         "fp32_over_fp64": round(thr["fp32"] / thr["fp64"], 3),
         "zeta_err_fp32_vs_fp64_m": {"max": float(dz.max()), "p99": float(np.percentile(dz, 99)),
                                     "rms": float(np.sqrt(np.mean(dz ** 2)))},
         "solver": {p: res[p]["stats"] for p in res},
-        "kernel_ms_per_step_fp32": {k: round(v, 4) for k, v in per_step.items() if v > 0},
-        "kernel_ms_per_step_fp64": {k: round(v[0] / res["fp64"]["nt"], 4) for k, v in res["fp64"]["timing"].items()
+        "kernel_ms_per_step_fp64": {k: round(v, 4) for k, v in per_step.items() if v > 0},
+        "kernel_ms_per_step_fp32": {k: round(v[0] / res["fp32"]["nt"], 4) for k, v in res["fp32"]["timing"].items()
                                     if v[0] > 0},
-        "kernel_algorithmic_GBps_fp32": kernel_gbs,
+        "kernel_algorithmic_GBps_fp64": kernel_gbs,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_launch": kb[dom], "avg_launch_ms": round(avg[dom], 5)},
