@@ -1304,6 +1304,8 @@ static int pipe_setup(Model &M) {
 int shyhip_step(shyhip_model *h, double dtime, int istot) {
     if (!h) return fail(nullptr, "null handle");
     Model &M = h->M;
+    // the forcing series must cover dtime; checked before the step starts
+    if ((M.f.nseries > 0 || M.f.wind_on) && forcing_check(M, dtime)) return 2;
     // the transport overlaps the next step's hydro unless that hydro needs
     // this step's density (baroclinic runs)
     const bool pipe = M.deferred && !M.timing && !M.has_mflux && M.ntr >= 1 && !M.bcl_on;
@@ -1325,10 +1327,7 @@ int shyhip_step(shyhip_model *h, double dtime, int istot) {
     M.dtime_old = M.dtime;
     timer_end(M);
     // sp111(2): boundary levels and wind of the installed series at dtime
-    if (M.f.nseries > 0 || M.f.wind_on) {
-        if (forcing_check(M, dtime)) return 2;
-        if (launch_forcing(M, dtime)) return 1;
-    }
+    if ((M.f.nseries > 0 || M.f.wind_on) && launch_forcing(M, dtime)) return 1;
     if (hydro_impl(M, dtime, false)) return M.err.empty() ? 1 : 3;
     M.iweg_snap = nullptr;
     if (pipe) {

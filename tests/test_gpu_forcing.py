@@ -177,3 +177,22 @@ def test_ensemble_level_series_equal_host_levels(gpu):
         ens.close()
     for (za, sa), (zb, sb) in zip(*res):
         assert np.array_equal(za, zb) and np.array_equal(sa, sb)
+
+
+def test_step_outside_records_leaves_state(gpu):
+    """A step whose time the series do not cover fails before it starts
+    (status 2, as the reference stops in iff_time_interpolate): the state
+    is the one of the last good step."""
+    b = mesh.lagoon(1500, seed=64, lx=5000.0, ly=2500.0, nlv=3, tile=16)
+    m = gpu.Model(b, "fp64", params=PARAMS, visv=1e-3)
+    t1, v1 = _level_records()
+    m.set_boundary_series(1, t1, v1, intpol=4, dtime0=0.0)
+    for _ in range(3):
+        m.step(None)
+    before = {f: m.pull(f) for f in ("znv", "utlnv", "zenv")}
+    m.t = float(t1[-1])                  # the next step's time lies past the last record
+    with pytest.raises(gpu.ShyhipError, match="outside the records"):
+        m.step(None)
+    for f, v in before.items():
+        assert np.array_equal(m.pull(f), v), f
+    m.close()
