@@ -209,6 +209,22 @@ int  shyhip_shy_write_hydro(shyhip_model *h, shyhip_shyfile *f, double dtime);
  * record with variable id ivar */
 int  shyhip_shy_write_node3d(shyhip_model *h, shyhip_shyfile *f, double dtime, int ivar, int field);
 
+/* debug dumps (shympi_debug, utils/shympi/mpi_debug.f90:114-146,367-415;
+ * read and compared by check_shympi_debug): a time block is
+ * shyhip_dbg_time, records of arrays (lmax, nn) in the reference's layout
+ * (type 1 int32, 2 real, 3 double), shyhip_dbg_end_time.  Host code. */
+typedef struct shyhip_dbgfile shyhip_dbgfile;
+int  shyhip_dbg_open(const char *path, shyhip_dbgfile **d);
+int  shyhip_dbg_time(shyhip_dbgfile *d, double dtime);
+int  shyhip_dbg_record(shyhip_dbgfile *d, const char *name, int nn, int lmax, int type, const void *array);
+int  shyhip_dbg_end_time(shyhip_dbgfile *d);
+int  shyhip_dbg_close(shyhip_dbgfile *d);
+const char *shyhip_dbg_error(const shyhip_dbgfile *d);
+/* the records of the model's state inside a time block: znv, zenv, utlnv,
+ * vtlnv, wlnv, hdknv, hdenv, iwegv (the arrays this path updates, as
+ * shyfem_subs.f90:1210-1265 writes them) */
+int  shyhip_dbg_write_state(shyhip_model *h, shyhip_dbgfile *d);
+
 /* baroclinic coupling (z-layers) -------------------------------------------
  * ibarcl = 1 (barocl, baroclinic.f90:195-600; ibarcl = 0 switches it off):
  * tracers itemp and isalt are temperature and salinity.  Starts the density

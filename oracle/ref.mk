@@ -6,7 +6,8 @@
 # directory; and the reference's SHY file reader (io_shy.f90 with
 # to_linear.f90) linked with shy_dump.f90 from this directory into
 # _ref/shy_dump, used only by tests/test_shy_cpu.py to read the files the
-# library writes.  Nothing is copied; the build writes only to oracle/_ref/.
+# library writes; and the reference's debug-file checker (check_shympi_debug
+# with clo.f90) into _ref/check_dbg, used only by the .dbg tests.  Nothing is copied; the build writes only to oracle/_ref/.
 #   make -f ref.mk            (needs the reference tree at REF)
 REF   ?= /root/reference
 FLANG ?= /opt/rocm/llvm/bin/flang
@@ -17,7 +18,15 @@ OUT   := _ref
 
 SHYSRC := $(REF)/src/utils/generic/to_linear.f90 $(REF)/src/utils/shyutil/io_shy.f90
 
-all: $(OUT)/libspkref.so $(OUT)/shy_dump
+DBGSRC := $(REF)/src/utils/generic/clo.f90 $(REF)/src/utils/shympi/check_shympi_debug.f90
+
+all: $(OUT)/libspkref.so $(OUT)/shy_dump $(OUT)/check_dbg
+
+# the reference's debug-file checker (lists one file, compares two)
+$(OUT)/check_dbg: $(DBGSRC)
+	@mkdir -p $(OUT)/dbg
+	cd $(OUT)/dbg && for f in $(DBGSRC); do $(FLANG) -O1 -c $$f || exit 1; done
+	$(FLANG) -O1 -o $@ $(addprefix $(OUT)/dbg/,$(notdir $(DBGSRC:.f90=.o)))
 
 $(OUT)/shy_dump: $(SHYSRC) shy_dump.f90
 	@mkdir -p $(OUT)/shy

@@ -41,21 +41,23 @@ struct Piece {
     size_t n;
 };
 
-int put_record(shyhip_shyfile *f, std::initializer_list<Piece> parts) {
+int put_record(FILE *fp, std::string &err, std::initializer_list<Piece> parts) {
     size_t total = 0;
     for (const Piece &q : parts) total += q.n;
     if (total > 0x7fffffffu) {
-        f->err = "shy: record larger than 2 GiB";
+        err = "record larger than 2 GiB";
         return 1;
     }
     const int32_t len = (int32_t)total;
-    bool ok = fwrite(&len, 4, 1, f->fp) == 1;
+    bool ok = fwrite(&len, 4, 1, fp) == 1;
     for (const Piece &q : parts)
-        if (q.n) ok = ok && fwrite(q.p, 1, q.n, f->fp) == q.n;
-    ok = ok && fwrite(&len, 4, 1, f->fp) == 1;
-    if (!ok) f->err = "shy: write error";
+        if (q.n) ok = ok && fwrite(q.p, 1, q.n, fp) == q.n;
+    ok = ok && fwrite(&len, 4, 1, fp) == 1;
+    if (!ok) err = "write error";
     return ok ? 0 : 1;
 }
+
+int put_record(shyhip_shyfile *f, std::initializer_list<Piece> parts) { return put_record(f->fp, f->err, parts); }
 
 void text80(char *dst, const char *src) {
     std::memset(dst, ' ', 80);
@@ -189,5 +191,97 @@ int shyhip_shy_write_node3d(shyhip_model *h, shyhip_shyfile *f, double dtime, in
     if (shyhip_pull(h, field, c.data(), SHYHIP_F32, c.size())) return 1;
     return shyhip_shy_write_record(f, dtime, ivar, 0, M.nkn, 1, M.nlvdi, M.nlvdi, c.data());
 }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- debug files
+// The debug dump (shympi_debug, utils/shympi/mpi_debug.f90:114-146,367-415):
+// per time block the record dtime, then per array three records --
+// nn lmax type (1 integer, 2 real, 3 double), the name (80 chars), the array
+// (lmax, nn) -- and the record 0 0 0 closing the block.  check_shympi_debug
+// (utils/shympi/check_shympi_debug.f90) lists and compares such files.
+struct shyhip_dbgfile {
+    FILE *fp = nullptr;
+    std::string err;
+};
+
+namespace {
+
+int put_dbg(shyhip_dbgfile *d, std::initializer_list<Piece> parts) { return put_record(d->fp, d->err, parts); }
+
+}  // namespace
+
+extern "C" {
+
+int shyhip_dbg_open(const char *path, shyhip_dbgfile **out) {
+    if (!out || !path) return 1;
+    auto *d = new shyhip_dbgfile;
+    d->fp = std::fopen(path, "wb");
+    if (!d->fp) {
+        delete d;
+        *out = nullptr;
+        return 1;
+    }
+    *out = d;
+    return 0;
+}
+
+int shyhip_dbg_time(shyhip_dbgfile *d, double dtime) {
+    if (!d || !d->fp) return 1;
+    return put_dbg(d, {{&dtime, sizeof dtime}});
+}
+
+// one array (lmax, nn), type 1 int32, 2 float, 3 double
+int shyhip_dbg_record(shyhip_dbgfile *d, const char *name, int nn, int lmax, int type, const void *array) {
+    if (!d || !d->fp || !array || nn <= 0 || lmax <= 0 || type < 1 || type > 3) return 1;
+    const int head[3] = {nn, lmax, type};
+    char t80[80];
+    text80(t80, name);
+    const size_t w = type == 3 ? 8 : 4;
+    return put_dbg(d, {{head, sizeof head}}) || put_dbg(d, {{t80, 80}}) ||
+           put_dbg(d, {{array, w * (size_t)nn * lmax}});
+}
+
+int shyhip_dbg_end_time(shyhip_dbgfile *d) {
+    if (!d || !d->fp) return 1;
+    const int z[3] = {0, 0, 0};
+    return put_dbg(d, {{z, sizeof z}});
+}
+
+int shyhip_dbg_close(shyhip_dbgfile *d) {
+    if (!d) return 0;
+    const int rc = (d->fp && std::fclose(d->fp) != 0) ? 1 : 0;
+    delete d;
+    return rc;
+}
+
+// the records of the model's state that the reference's debug output
+// writes after a step (shyfem_subs.f90:1210-1265, the arrays this path
+// updates): znv, zenv, utlnv, vtlnv, wlnv, hdknv, hdenv, iwegv; the caller
+// opens and closes the time block
+int shyhip_dbg_write_state(shyhip_model *h, shyhip_dbgfile *d) {
+    if (!h || !d) return 1;
+    shy::Model &M = h->M;
+    const int L = M.nlvdi, nk = M.nkn, ne = M.nel;
+    std::vector<float> z(nk), ze(3 * (size_t)ne), u((size_t)L * ne), v((size_t)L * ne), w((size_t)(L + 1) * nk),
+        hk((size_t)L * nk), he((size_t)L * ne);
+    std::vector<int> iw(ne);
+    if (shyhip_pull(h, SHYHIP_ZNV, z.data(), SHYHIP_F32, z.size()) ||
+        shyhip_pull(h, SHYHIP_ZENV, ze.data(), SHYHIP_F32, ze.size()) ||
+        shyhip_pull(h, SHYHIP_UTLNV, u.data(), SHYHIP_F32, u.size()) ||
+        shyhip_pull(h, SHYHIP_VTLNV, v.data(), SHYHIP_F32, v.size()) ||
+        shyhip_pull(h, SHYHIP_WLNV, w.data(), SHYHIP_F32, w.size()) ||
+        shyhip_pull(h, SHYHIP_HDKNV, hk.data(), SHYHIP_F32, hk.size()) ||
+        shyhip_pull(h, SHYHIP_HDENV, he.data(), SHYHIP_F32, he.size()) ||
+        shyhip_pull(h, SHYHIP_IWEGV, iw.data(), SHYHIP_I32, iw.size()))
+        return 1;
+    return shyhip_dbg_record(d, "znv", nk, 1, 2, z.data()) ||
+           shyhip_dbg_record(d, "zenv", ne, 3, 2, ze.data()) || shyhip_dbg_record(d, "utlnv", ne, L, 2, u.data()) ||
+           shyhip_dbg_record(d, "vtlnv", ne, L, 2, v.data()) || shyhip_dbg_record(d, "wlnv", nk, L + 1, 2, w.data()) ||
+           shyhip_dbg_record(d, "hdknv", nk, L, 2, hk.data()) || shyhip_dbg_record(d, "hdenv", ne, L, 2, he.data()) ||
+           shyhip_dbg_record(d, "iwegv", ne, 1, 1, iw.data());
+}
+
+const char *shyhip_dbg_error(const shyhip_dbgfile *d) { return d ? d->err.c_str() : "null file"; }
 
 }  // extern "C"

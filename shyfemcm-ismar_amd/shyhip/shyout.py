@@ -75,3 +75,59 @@ class ShyWriter:
             self.f = None
             if rc:
                 raise ShyhipError("shy_close: could not close the file")
+
+
+class DbgWriter:
+    """Debug dumps in the reference's format (shympi_debug), the files
+    check_shympi_debug lists and compares."""
+
+    TYPES = {np.dtype(np.int32): 1, np.dtype(np.float32): 2, np.dtype(np.float64): 3}
+
+    def __init__(self, path):
+        self.lib = lib = load_library()
+        lib.shyhip_dbg_error.restype = C.c_char_p
+        d = C.c_void_p()
+        if lib.shyhip_dbg_open(str(path).encode(), C.byref(d)):
+            raise ShyhipError(f"shyhip_dbg_open({path}) failed")
+        self.d = d
+
+    def _chk(self, rc, what):
+        if rc:
+            raise ShyhipError(f"{what}: {self.lib.shyhip_dbg_error(self.d).decode()}")
+
+    def time(self, dtime):
+        self._chk(self.lib.shyhip_dbg_time(self.d, C.c_double(dtime)), "dbg_time")
+
+    def record(self, name, values):
+        """values: (nn,) or (nn, lmax) of int32, float32 or float64."""
+        a = np.ascontiguousarray(values)
+        if a.dtype not in self.TYPES:
+            a = np.ascontiguousarray(a, dtype=np.float32)
+        nn = a.shape[0]
+        lmax = 1 if a.ndim == 1 else a.shape[1]
+        self._chk(self.lib.shyhip_dbg_record(self.d, name.encode(), nn, lmax, self.TYPES[a.dtype], _p(a)),
+                  "dbg_record")
+
+    def end_time(self):
+        self._chk(self.lib.shyhip_dbg_end_time(self.d), "dbg_end_time")
+
+    def state(self, model):
+        """The model's state records (inside a time block)."""
+        self._chk(self.lib.shyhip_dbg_write_state(model.h, self.d), "dbg_write_state")
+
+    def header(self, basin):
+        """The basin records the reference's first debug block holds
+        (shyfem_subs.f90:1198-1204): external numbers, coordinates, depths."""
+        meta = getattr(basin, "meta", {}) or {}
+        self.record("ipv", np.asarray(meta.get("node_ext", np.arange(1, basin.nkn + 1)), np.int32))
+        self.record("ipev", np.asarray(meta.get("elem_ext", np.arange(1, basin.nel + 1)), np.int32))
+        self.record("xgv", np.asarray(basin.xgv, np.float32))
+        self.record("ygv", np.asarray(basin.ygv, np.float32))
+        self.record("hm3v", np.asarray(basin.hm3v, np.float32))
+
+    def close(self):
+        if self.d:
+            rc = self.lib.shyhip_dbg_close(self.d)
+            self.d = None
+            if rc:
+                raise ShyhipError("dbg_close: could not close the file")
