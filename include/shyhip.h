@@ -239,6 +239,21 @@ int  shyhip_set_baroclinic(shyhip_model *h, int ibarcl, int itemp, int isalt);
  * (sigma_stp, eq_state.f90:154-235) */
 int  shyhip_rhoset(shyhip_model *h, int niter);
 
+/* discharges (set_mass_flux, bnd_routines.f90:1035-1183) --------------------
+ * The point sources: one entry per node of the flux (ibtyp 2) and inlet
+ * (ibtyp 3) boundaries, in the reference's loop order (boundary by boundary,
+ * kbnds order), with the boundary's levmin/levmax (get_bnd_ipar); node
+ * 1-based.  nsrc = 0: distributed sources only. */
+int  shyhip_set_sources(shyhip_model *h, int nsrc, const int *node, const int *levmin, const int *levmax);
+/* set_mass_flux: mfluxv (per node and layer) and rqv on the device from the
+ * discharge of every source (rflux(kindex) of sp111, bnd_routines.f90:
+ * 532-544, m3/s, in shyhip_set_sources' order) and the distributed sources
+ * rqdsv[nkn] (rain/evaporation, m3/s, NULL = none), using the current layer
+ * volumes.  Call it where the reference calls set_mass_flux, before the step
+ * (or before shyhip_hydro).  Errors (status 3): layer limits outside the
+ * column, a zero source volume (the reference's error stop). */
+int  shyhip_set_mass_flux(shyhip_model *h, const float *rflux, const float *rqdsv);
+
 /* time step -------------------------------------------------------------- */
 int  shyhip_set_dry(shyhip_model *h);
 int  shyhip_copy_uvz(shyhip_model *h);                 /* copy_uvz + copy_depth */

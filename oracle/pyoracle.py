@@ -375,6 +375,26 @@ def convert_wind(wx, wy, iwtype=1, itdrag=0, dragco=2.5e-3, roluft=1.225, rowass
     return tx, ty, ws, x, y, rc
 
 
+def set_mass_flux(ilhkv, hdknv, areakv, nodes, levmin, levmax, rflux, rqdsv=None, nlv=None, precision="f32"):
+    """set_mass_flux: hdknv/areakv (nkn, nlvdi) node-major; sources with
+    0-based nodes.  Returns (mfluxv (nkn, nlvdi), rqv) or None on the
+    error stop."""
+    lib = load(precision)
+    dt = np.float32 if precision == "f32" else np.float64
+    hd = np.ascontiguousarray(hdknv, dtype=dt)
+    ar = np.ascontiguousarray(areakv, dtype=dt)
+    nkn, nlvdi = hd.shape
+    il = np.ascontiguousarray(ilhkv, dtype=np.int32)
+    nd, lo, hi = (np.ascontiguousarray(a, dtype=np.int32) for a in (nodes, levmin, levmax))
+    q = np.ascontiguousarray(rflux, dtype=np.float32)
+    rq = None if rqdsv is None else np.ascontiguousarray(rqdsv, dtype=np.float32)
+    mf = np.zeros((nkn, nlvdi), dt)
+    rqv = np.zeros(nkn, dt)
+    rc = lib.oc_set_mass_flux(nkn, nlvdi, nlvdi if nlv is None else nlv, ptr(il), ptr(hd), ptr(ar), len(nd), ptr(nd),
+                              ptr(lo), ptr(hi), ptr(q), ptr(rq), ptr(mf), ptr(rqv))
+    return None if rc else (mf, rqv)
+
+
 # ------------------------------------------------------------------ baroclinic
 def sigma_stp(s, t, p, precision="f32"):
     """sigma = rho - 1000 of the UNESCO 1980 equation of state (p in bar)."""

@@ -196,6 +196,11 @@ int  oc_iff_series(int nrec, const double *time, const float *vals, int ncomp, i
                    double dtime0, float vdefault, int nt, const double *t, float *out, int *newest);
 int  oc_convert_wind(int n, float *wx, float *wy, int iwtype, int itdrag, float dragco, float roluft, float rowass,
                      float wslim, float wsmax, float *tx, float *ty, float *ws);
+/* set_mass_flux (shyoracle_forcing.c): sources in the reference's loop
+ * order, node 0-based; rqdsv may be NULL; returns 1 on the error stop */
+int  oc_set_mass_flux(int nkn, int nlvdi, int nlv, const int *ilhkv, const real *hdknv, const real *areakv,
+                      int nsrc, const int *node, const int *levmin, const int *levmax, const float *rflux,
+                      const float *rqdsv, real *mfluxv, real *rqv);
 
 #ifdef __cplusplus
 }

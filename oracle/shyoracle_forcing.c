@@ -18,6 +18,14 @@
  * The bits of the interpolated value depend on the ring-slot order of the
  * Neville points, which only the restated order fixes: parity unpinned at
  * the last-bit level (the device is compared with this restatement).
+ *
+ * set_mass_flux (discharges into the layers of their nodes) is pinned in
+ * tests/test_mass_flux_cpu.py by what the routine must conserve and by
+ * hand-computed cases: a column's parts sum to its discharge, parts follow
+ * the layer volumes, levmin/levmax select the layers (negative: counted
+ * from the bottom), two sources at one node add, rain goes to the top
+ * layer, rqv is the column sum, and a bad layer range or a zero volume is
+ * the error stop.
  */
 #include <math.h>
 #include <stddef.h>
@@ -206,4 +214,41 @@ int oc_convert_wind(int n, float *wx, float *wy, int iwtype, int itdrag, float d
         wxymax = wslim;
     }
     return wxymax > wsmax ? 1 : 0;
+}
+
+/* set_mass_flux, bnd_routines.f90:1035-1183 (mode +1: volnode = hdknv *
+ * areakv, elems_dealing.f90:741-800; jlhkv = 1 for z-layers).  Volumes are
+ * reals summed in double, the share is double and every layer's part is
+ * rounded to real, as the reference declares them. */
+int oc_set_mass_flux(int nkn, int nlvdi, int nlv, const int *ilhkv, const real *hdknv, const real *areakv,
+                     int nsrc, const int *node, const int *levmin, const int *levmax, const float *rflux,
+                     const float *rqdsv, real *mfluxv, real *rqv) {
+    for (size_t i = 0; i < (size_t)nlvdi * nkn; i++) mfluxv[i] = 0;
+    for (int i = 0; i < nsrc; i++) {
+        const int k = node[i];
+        int lmax = ilhkv[k], lmin = 1;
+        if (levmax[i] > 0 && levmax[i] < lmax) lmax = levmax[i];
+        if (levmax[i] < 0 && lmax + 1 + levmax[i] < lmax) lmax = lmax + 1 + levmax[i];
+        if (levmin[i] > 0 && levmin[i] > lmin) lmin = levmin[i];
+        if (levmin[i] < 0 && lmax + 1 + levmin[i] > lmin) lmin = lmax + 1 + levmin[i];
+        if (lmin > lmax || lmin < 1 || lmax > nlv) return 1;
+        double dvols[256], dvoltot = 0.;
+        for (int l = lmin; l <= lmax; l++) {
+            const real vol = hdknv[(l - 1) + (size_t)nlvdi * k] * areakv[(l - 1) + (size_t)nlvdi * k];
+            dvols[l - 1] = (double)vol;
+            dvoltot += dvols[l - 1];
+        }
+        if (dvoltot <= 0.) return 1;
+        for (int l = lmin; l <= lmax; l++) {
+            const real part = (real)((double)rflux[i] * (dvols[l - 1] / dvoltot));
+            mfluxv[(l - 1) + (size_t)nlvdi * k] += part;
+        }
+    }
+    for (int k = 0; k < nkn; k++) {
+        if (rqdsv) mfluxv[(size_t)nlvdi * k] += (real)rqdsv[k];
+        double col = 0.;
+        for (int l = 1; l <= ilhkv[k]; l++) col += (double)mfluxv[(l - 1) + (size_t)nlvdi * k];
+        rqv[k] = (real)col;
+    }
+    return 0;
 }

@@ -73,6 +73,7 @@ int fail(Model *M, const std::string &msg, int code = 1) {
 int pipe_drain(Model &M);
 void forcing_free_bnd(Model &M);
 void forcing_free_wind(Model &M);
+void forcing_free_sources(Model &M);
 
 template <typename T>
 int dalloc(Model &M, T **p, size_t n) {
@@ -327,6 +328,7 @@ int shyhip_destroy(shyhip_model *h) {
     for (void *q : M.tr_hbuf) dfree(q);
     forcing_free_bnd(M);
     forcing_free_wind(M);
+    forcing_free_sources(M);
     dfree(M.d_err);
     dfree(M.d_levels);
     if (M.h_levels) (void)hipHostFree(M.h_levels);
@@ -971,6 +973,14 @@ void forcing_free_bnd(Model &M) {
     f.nseries = 0;
 }
 
+void forcing_free_sources(Model &M) {
+    Forcing &f = M.f;
+    dfree(f.src_ptr); dfree(f.src_lev); dfree(f.src_idx); dfree(f.src_q); dfree(f.rqds);
+    f.src_ptr = f.src_lev = f.src_idx = nullptr;
+    f.src_q = f.rqds = nullptr;
+    f.nsrc = 0;
+}
+
 void forcing_free_wind(Model &M) {
     Forcing &f = M.f;
     dfree(f.wbuf[0]); dfree(f.wbuf[1]);
@@ -1121,6 +1131,55 @@ int shyhip_forcing(shyhip_model *h, double dtime) {
     return launch_forcing(M, dtime);
 }
 
+static int check_dev_err(Model &M, const char *where);
+
+int shyhip_set_sources(shyhip_model *h, int nsrc, const int *node, const int *levmin, const int *levmax) {
+    if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
+    Model &M = h->M;
+    if (nsrc < 0 || (nsrc > 0 && (!node || !levmin || !levmax))) return fail(&M, "set_sources: bad arguments");
+    for (int i = 0; i < nsrc; i++)
+        if (node[i] < 1 || node[i] > M.nkn)
+            return fail(&M, "set_sources: node " + std::to_string(node[i]) + " out of range");
+    forcing_free_sources(M);
+    Forcing &f = M.f;
+    if (dalloc(M, &f.rqds, (size_t)M.nkn)) return 1;
+    if (nsrc == 0) return 0;
+    // the sources of each node, in the caller's order (a counting sort)
+    std::vector<int> ptr(M.nkn + 1, 0), idx(nsrc), lev(2 * (size_t)nsrc);
+    for (int i = 0; i < nsrc; i++) ptr[node[i]]++;
+    for (int k = 0; k < M.nkn; k++) ptr[k + 1] += ptr[k];
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (int i = 0; i < nsrc; i++) {
+        const int p = fill[node[i] - 1]++;
+        idx[p] = i;
+        lev[2 * p] = levmin[i];
+        lev[2 * p + 1] = levmax[i];
+    }
+    if (dalloc(M, &f.src_ptr, ptr.size()) || dalloc(M, &f.src_lev, lev.size()) || dalloc(M, &f.src_idx, idx.size()) ||
+        dalloc(M, &f.src_q, (size_t)nsrc))
+        return 1;
+    if (upload(M, f.src_ptr, ptr.data(), ptr.size()) || upload(M, f.src_lev, lev.data(), lev.size()) ||
+        upload(M, f.src_idx, idx.data(), idx.size()))
+        return 1;
+    f.nsrc = nsrc;
+    return 0;
+}
+
+int shyhip_set_mass_flux(shyhip_model *h, const float *rflux, const float *rqdsv) {
+    if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
+    Model &M = h->M;
+    Forcing &f = M.f;
+    if (!f.rqds) return fail(&M, "set_mass_flux: call shyhip_set_sources first");
+    if (f.nsrc > 0 && !rflux) return fail(&M, "set_mass_flux: rflux is needed for the sources");
+    if (f.nsrc > 0 && upload(M, f.src_q, rflux, (size_t)f.nsrc)) return 1;
+    if (rqdsv && upload(M, f.rqds, rqdsv, (size_t)M.nkn)) return 1;
+    M.has_mflux = true;
+    if (launch_mass_flux(M, rqdsv != nullptr)) return 1;
+    return check_dev_err(M, "set_mass_flux");
+}
+
 int shyhip_set_dry(shyhip_model *h) {
     if (!h) return fail(nullptr, "null handle");
     if (pipe_drain(h->M)) return 1;
@@ -1174,8 +1233,9 @@ static int report_dev_err(Model &M, int e, const char *where) {
     static const char *names[] = {"", "setnod: open boundary node out of the system", "", "setuvd: z or u/v computation",
                                   "setdepth: non-positive layer thickness", "check_volume: negative volume", "",
                                   "conz3d: zero diagonal in a column", "penta: singular vertical column",
-                                  "meteo_convert_wind_data: wind speed above wsmax"};
-    return fail(&M, std::string(where) + ": " + (e >= 0 && e <= 9 ? names[e] : "device error"), 3);
+                                  "meteo_convert_wind_data: wind speed above wsmax",
+                                  "set_mass_flux: source layers out of bounds or zero volume"};
+    return fail(&M, std::string(where) + ": " + (e >= 0 && e <= 10 ? names[e] : "device error"), 3);
 }
 
 static int check_dev_err(Model &M, const char *where) {

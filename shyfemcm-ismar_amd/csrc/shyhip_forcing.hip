@@ -181,4 +181,73 @@ int launch_forcing(Model &M, double dtime) {
     return M.prec == SHYHIP_FP64 ? forcing_t<double>(M, dtime) : forcing_t<float>(M, dtime);
 }
 
+// set_mass_flux (bnd_routines.f90:1035-1183), one work-item per node.  The
+// node's point sources, in the caller's (the reference's boundary loop)
+// order, spread their discharge over layers lmin..lmax (levmin/levmax: > 0
+// a layer, < 0 counted from the bottom) in proportion to the layer volumes
+// hdknv*areakv (volnode with mode +1); the distributed source rqdsv goes to
+// the top layer; rqv is the column sum, added in double.  Types follow the
+// reference: volumes of the storage type summed in double, the share in
+// double, each layer's part rounded to the storage type.
+template <typename S>
+__global__ void __launch_bounds__(kForceThreads)
+k_mass_flux(S *__restrict__ mflux, S *__restrict__ rqv, int *__restrict__ err, const S *__restrict__ hdkn,
+            const S *__restrict__ areakv, const int *__restrict__ ilhkv, const int *__restrict__ src_ptr,
+            const int *__restrict__ src_lev, const int *__restrict__ src_idx, const float *__restrict__ src_q,
+            const float *__restrict__ rqds, int nkn, int nlv) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nkn) return;
+    const int lk = ilhkv[k];
+    for (int l = 0; l < nlv; l++) mflux[(size_t)l * nkn + k] = (S)0;
+    const int p0 = src_ptr ? src_ptr[k] : 0, p1 = src_ptr ? src_ptr[k + 1] : 0;
+    for (int p = p0; p < p1; p++) {
+        const int levmin = src_lev[2 * p], levmax = src_lev[2 * p + 1];
+        int lmax = lk, lmin = 1;
+        if (levmax > 0) lmax = min(lmax, levmax);
+        if (levmax < 0) lmax = min(lmax, lmax + 1 + levmax);
+        if (levmin > 0) lmin = max(lmin, levmin);
+        if (levmin < 0) lmin = max(lmin, lmax + 1 + levmin);
+        if (lmin > lmax || lmin < 1 || lmax > nlv) {
+            atomicCAS(err, 0, ERR_MFLUX);
+            return;
+        }
+        double voltot = 0.;
+        for (int l = lmin - 1; l < lmax; l++) {
+            const size_t o = (size_t)l * nkn + k;
+            voltot += (double)(S)(hdkn[o] * areakv[o]);
+        }
+        if (voltot <= 0.) {
+            atomicCAS(err, 0, ERR_MFLUX);
+            return;
+        }
+        const float q = src_q[src_idx[p]];
+        for (int l = lmin - 1; l < lmax; l++) {
+            const size_t o = (size_t)l * nkn + k;
+            const double share = (double)(S)(hdkn[o] * areakv[o]) / voltot;
+            mflux[o] += (S)((double)q * share);
+        }
+    }
+    if (rqds) mflux[k] += (S)rqds[k];
+    double col = 0.;
+    for (int l = 0; l < lk; l++) col += (double)mflux[(size_t)l * nkn + k];
+    rqv[k] = (S)col;
+}
+
+template <typename S>
+static int mass_flux_t(Model &M, bool with_rqds) {
+    Forcing &f = M.f;
+    const int g = (M.nkn + kForceThreads - 1) / kForceThreads;
+    hipLaunchKernelGGL(k_mass_flux<S>, dim3(g), dim3(kForceThreads), 0, M.stream, (S *)M.s.mflux, (S *)M.s.rqv,
+                       M.d_err, (const S *)M.s.hdkn, (const S *)M.m.areakv, (const int *)M.m.ilhkv,
+                       (const int *)(f.nsrc > 0 ? f.src_ptr : nullptr), (const int *)f.src_lev,
+                       (const int *)f.src_idx, (const float *)f.src_q, with_rqds ? (const float *)f.rqds : nullptr,
+                       M.nkn, M.nlv);
+    HIPCHK(M, hipGetLastError());
+    return 0;
+}
+
+int launch_mass_flux(Model &M, bool with_rqds) {
+    return M.prec == SHYHIP_FP64 ? mass_flux_t<double>(M, with_rqds) : mass_flux_t<float>(M, with_rqds);
+}
+
 }  // namespace shy

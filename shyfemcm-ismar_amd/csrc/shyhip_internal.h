@@ -27,7 +27,7 @@ constexpr int kIncWidth = 10;                        // incident elements per no
 
 // error codes written by kernels into Model::d_err[0]
 enum DevErr { ERR_NONE = 0, ERR_SETNOD = 1, ERR_SETUVD = 3, ERR_SETDEPTH = 4, ERR_VOLUME = 5,
-              ERR_CONZ = 7, ERR_PENTA = 8, ERR_WIND = 9 };
+              ERR_CONZ = 7, ERR_PENTA = 8, ERR_WIND = 9, ERR_MFLUX = 10 };
 
 struct Mesh {                 // device copies of init-time data
     int *nen3v = nullptr;     // [3*nel] element-major (ie*3+ii), 0-based
@@ -175,6 +175,13 @@ struct Forcing {
     void *wbuf[2] = {nullptr, nullptr};
     bool wind_on = false;
     bool wind_applied = false;         // a one-record (constant) wind is converted once
+    // point sources of set_mass_flux: node k's sources are [src_ptr[k],
+    // src_ptr[k+1]) in the caller's order; per source its layer limits
+    // (levmin, levmax) and its index into the discharge array
+    int nsrc = 0;
+    int *src_ptr = nullptr, *src_lev = nullptr, *src_idx = nullptr;
+    float *src_q = nullptr;            // [nsrc] discharges (rflux)
+    float *rqds = nullptr;             // [nkn] distributed sources (rqdsv)
 };
 
 struct Model {
@@ -330,6 +337,7 @@ int launch_set_rzv_all(Model &M);
 // the boundaries with a series into rzv, the wind series into the stress
 // (shyhip_forcing.hip); the host has checked that dtime is covered
 int launch_forcing(Model &M, double dtime);
+int launch_mass_flux(Model &M, bool with_rqds);
 // rhoset (baroclinic.f90:654-720) niter times: the density anomaly of every
 // node-layer from T, S and the mid-layer pressure (shyhip_hydro.hip)
 int launch_rhoset(Model &M, int niter, hipStream_t st);

@@ -200,6 +200,21 @@ module shyhip_capi
       integer(c_int) :: rc
     end function
 
+    function shyhip_set_sources(h, nsrc, node, levmin, levmax) bind(C, name="shyhip_set_sources") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      integer(c_int), value :: nsrc
+      integer(c_int), intent(in) :: node(*), levmin(*), levmax(*)
+      integer(c_int) :: rc
+    end function
+
+    function shyhip_set_mass_flux(h, rflux, rqdsv) bind(C, name="shyhip_set_mass_flux") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      type(c_ptr), value :: rflux, rqdsv
+      integer(c_int) :: rc
+    end function
+
     function shyhip_forcing(h, dtime) bind(C, name="shyhip_forcing") result(rc)
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: h

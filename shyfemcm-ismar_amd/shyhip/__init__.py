@@ -279,6 +279,28 @@ class Model:
     def rhoset(self, niter=1):
         self._chk(self.lib.shyhip_rhoset(self.h, niter), "rhoset")
 
+    def set_sources(self, nodes, levmin=None, levmax=None):
+        """Point sources of set_mass_flux: 0-based nodes in the reference's
+        boundary loop order, with their boundary's levmin/levmax (0: the
+        whole column).  An empty list keeps the distributed sources only."""
+        n = np.ascontiguousarray(np.asarray(nodes, np.int64) + 1, dtype=np.int32)
+        lo = np.ascontiguousarray(np.zeros(len(n)) if levmin is None else levmin, dtype=np.int32)
+        hi = np.ascontiguousarray(np.zeros(len(n)) if levmax is None else levmax, dtype=np.int32)
+        if not (len(lo) == len(hi) == len(n)):
+            raise ValueError("set_sources: nodes, levmin and levmax differ in length")
+        self._src = (n, lo, hi)
+        self._chk(self.lib.shyhip_set_sources(self.h, len(n), _p(n), _p(lo), _p(hi)), "set_sources")
+
+    def set_mass_flux(self, rflux=None, rqdsv=None):
+        """set_mass_flux on the device: discharges per source (m3/s) and
+        distributed sources per node (None: none) into mfluxv and rqv."""
+        q = None if rflux is None else np.ascontiguousarray(rflux, dtype=np.float32)
+        d = None if rqdsv is None else np.ascontiguousarray(rqdsv, dtype=np.float32)
+        if d is not None and d.shape != (self.basin.nkn,):
+            raise ValueError("set_mass_flux: rqdsv must hold one value per node")
+        self._chk(self.lib.shyhip_set_mass_flux(self.h, None if q is None else _p(q), None if d is None else _p(d)),
+                  "set_mass_flux")
+
     def forcing(self, dtime):
         """Evaluate the installed series at dtime (sp111(2) outside a step)."""
         self._chk(self.lib.shyhip_forcing(self.h, C.c_double(dtime)), "forcing")
