@@ -7,7 +7,10 @@
 # to_linear.f90) linked with shy_dump.f90 from this directory into
 # _ref/shy_dump, used only by tests/test_shy_cpu.py to read the files the
 # library writes; and the reference's debug-file checker (check_shympi_debug
-# with clo.f90) into _ref/check_dbg, used only by the .dbg tests.  Nothing is copied; the build writes only to oracle/_ref/.
+# with clo.f90) into _ref/check_dbg, used only by the .dbg tests; and the
+# reference's basin reader (basin.f90 with file.f90) linked with
+# bas_dump.f90 from this directory into _ref/bas_dump, used only by
+# tests/test_bas_cpu.py.  Nothing is copied; the build writes only to oracle/_ref/.
 #   make -f ref.mk            (needs the reference tree at REF)
 REF   ?= /root/reference
 FLANG ?= /opt/rocm/llvm/bin/flang
@@ -20,7 +23,15 @@ SHYSRC := $(REF)/src/utils/generic/to_linear.f90 $(REF)/src/utils/shyutil/io_shy
 
 DBGSRC := $(REF)/src/utils/generic/clo.f90 $(REF)/src/utils/shympi/check_shympi_debug.f90
 
-all: $(OUT)/libspkref.so $(OUT)/shy_dump $(OUT)/check_dbg
+BASSRC := $(REF)/src/utils/generic/file.f90 $(REF)/src/utils/shyutil/basin.f90
+
+all: $(OUT)/libspkref.so $(OUT)/shy_dump $(OUT)/check_dbg $(OUT)/bas_dump
+
+# the reference's basin (.bas) reader
+$(OUT)/bas_dump: $(BASSRC) bas_dump.f90
+	@mkdir -p $(OUT)/bas
+	cd $(OUT)/bas && for f in $(BASSRC); do $(FLANG) -O1 -c $$f || exit 1; done
+	$(FLANG) -O1 -module-dir $(OUT)/bas -o $@ bas_dump.f90 $(addprefix $(OUT)/bas/,$(notdir $(BASSRC:.f90=.o)))
 
 # the reference's debug-file checker (lists one file, compares two)
 $(OUT)/check_dbg: $(DBGSRC)

@@ -45,6 +45,7 @@ class Grd:
     elem_depth: np.ndarray         # (nel,) f32, -999 where not given
     lines: list = field(default_factory=list)   # (number, type, external nodes, depth)
     comments: int = 0
+    vertex_depth: np.ndarray | None = None      # (nel, 3) f32 depth at each vertex (.bas files)
 
 
 def _numbers(line, lineno):
@@ -145,7 +146,11 @@ def basin_from_grd(g: Grd, levels, bounds=(), name: str = "") -> Basin:
     nen[cw] = nen[cw][:, [0, 2, 1]]
     eorder = np.lexsort((nen.max(axis=1), nen.min(axis=1)))
     nen = nen[eorder].astype(np.int32)
-    if np.all(g.elem_depth != NO_DEPTH):
+    if g.vertex_depth is not None:
+        vd = np.array(g.vertex_depth, np.float32)
+        vd[cw] = vd[cw][:, [0, 2, 1]]
+        hm3v = vd[eorder]
+    elif np.all(g.elem_depth != NO_DEPTH):
         hm3v = np.repeat(g.elem_depth[eorder][:, None], 3, axis=1).astype(np.float32)
     elif np.all(g.node_depth != NO_DEPTH):
         hm3v = g.node_depth[order][nen].astype(np.float32)
