@@ -10,7 +10,11 @@
 # with clo.f90) into _ref/check_dbg, used only by the .dbg tests; and the
 # reference's basin reader (basin.f90 with file.f90) linked with
 # bas_dump.f90 from this directory into _ref/bas_dump, used only by
-# tests/test_bas_cpu.py.  Nothing is copied; the build writes only to oracle/_ref/.
+# tests/test_bas_cpu.py; and the reference's set_ev, element colouring,
+# quad-tree element search and femintp_xi (evgeom.f90, subset.f90,
+# quad_tree.f90 with what they use) linked with geom_ref.f90 from this
+# directory into _ref/geom_ref, used only by tests/test_oracle_geom.py.
+# Nothing is copied; the build writes only to oracle/_ref/.
 #   make -f ref.mk            (needs the reference tree at REF)
 REF   ?= /root/reference
 FLANG ?= /opt/rocm/llvm/bin/flang
@@ -25,7 +29,20 @@ DBGSRC := $(REF)/src/utils/generic/clo.f90 $(REF)/src/utils/shympi/check_shympi_
 
 BASSRC := $(REF)/src/utils/generic/file.f90 $(REF)/src/utils/shyutil/basin.f90
 
-all: $(OUT)/libspkref.so $(OUT)/shy_dump $(OUT)/check_dbg $(OUT)/bas_dump
+# set_ev, the colouring, the quad tree and femintp_xi with what they use
+GEOSRC := $(REF)/src/utils/generic/file.f90 $(REF)/src/utils/shyutil/basin.f90 \
+          $(REF)/src/utils/shyutil/geom.f90 $(REF)/src/utils/shyutil/evgeom.f90 \
+          $(REF)/src/utils/generic/stack.f90 $(REF)/src/utils/shyutil/quad_tree.f90 \
+          $(REF)/src/utils/shyutil/find.f90 $(REF)/src/utils/shyutil/regular.f90 \
+          $(REF)/src/utils/generic/round.f90 $(REF)/src/shyfem/subset.f90
+
+all: $(OUT)/libspkref.so $(OUT)/shy_dump $(OUT)/check_dbg $(OUT)/bas_dump $(OUT)/geom_ref
+
+# the reference's geometry, colouring and upwind-point search
+$(OUT)/geom_ref: $(GEOSRC) geom_ref.f90
+	@mkdir -p $(OUT)/geo
+	cd $(OUT)/geo && for f in $(GEOSRC); do $(FLANG) -O1 -c $$f || exit 1; done
+	$(FLANG) -O1 -module-dir $(OUT)/geo -o $@ geom_ref.f90 $(addprefix $(OUT)/geo/,$(notdir $(GEOSRC:.f90=.o)))
 
 # the reference's basin (.bas) reader
 $(OUT)/bas_dump: $(BASSRC) bas_dump.f90

@@ -74,6 +74,17 @@ static real tvd_upwind(const oc_mesh *m, int ie, int l, int ic, int id, real cu,
     return (real)zh;
 }
 
+/* tvd_upwind's value at layer l for every (id, ic, ie), out[(id-1) +
+ * 3(ic-1) + 9 ie], 0 where there is no upwind element or id == ic (the
+ * tests compare it with the reference's femintp_xi) */
+void oc_tvd_upwind_table(const oc_mesh *m, int l, const real *cv, real *out)
+{
+    for (int ie = 0; ie < m->nel; ie++)
+        for (int ic = 1; ic <= 3; ic++)
+            for (int id = 1; id <= 3; id++)
+                out[(id - 1) + 3 * (ic - 1) + 9 * ie] = id == ic ? (real)0 : tvd_upwind(m, ie, l, ic, id, (real)0, cv);
+}
+
 static int smartdelta(int a, int b)
 {
     return (int)((float)((a + b) - abs(a - b)) / (float)((a + b) + abs(a - b)));

@@ -304,36 +304,58 @@ void oc_set_area(const oc_mesh *m, real *areakv)
     }
 }
 
-/* point-in-triangle with inclusive edges; the lowest element index among the
- * containing ones is returned (quad_tree_search tie-breaking is not restated:
- * the interpolated value is continuous across shared edges). */
+/* in_element (src/utils/shyutil/find.f90:365-400) with intri
+ * (src/utils/shyutil/regular.f90:1252-1282): the element's bounding box in
+ * default reals, then each edge's outward test in double precision of
+ * default-real differences; a point on an edge is inside. */
+static int in_element_ref(const int *kn, const real *xgv, const real *ygv, float xp, float yp)
+{
+    float x[3], y[3];
+    for (int ii = 0; ii < 3; ii++) { x[ii] = (float)xgv[kn[ii]]; y[ii] = (float)ygv[kn[ii]]; }
+    float xmin = x[0], xmax = x[0], ymin = y[0], ymax = y[0];
+    for (int ii = 1; ii < 3; ii++) {
+        if (x[ii] < xmin) xmin = x[ii];
+        if (x[ii] > xmax) xmax = x[ii];
+        if (y[ii] < ymin) ymin = y[ii];
+        if (y[ii] > ymax) ymax = y[ii];
+    }
+    if (!(xp >= xmin && xp <= xmax && yp >= ymin && yp <= ymax)) return 0;
+    for (int k1 = 0; k1 < 3; k1++) {
+        int k2 = (k1 + 1) % 3;
+        double x21 = (double)(float)(x[k2] - x[k1]);
+        double y21 = (double)(float)(y[k2] - y[k1]);
+        double yn = x21, xn = -y21;
+        double scal = (double)(float)(xp - x[k1]) * xn + (double)(float)(yp - y[k1]) * yn;
+        if (scal < 0.) return 0;
+    }
+    return 1;
+}
+
+/* quad_tree_search (src/utils/shyutil/quad_tree.f90:313-397): of the
+ * elements containing the point (in_element), the one with the highest
+ * external number, here the element index (the basin's own numbering); -1
+ * if none.  The quad tree only narrows the candidates: every element whose
+ * bounding box holds the point is in the leaf the search reaches, so a
+ * uniform grid of cells gives the same answer. */
 static int find_elem(const oc_mesh *m, const real *xgv, const real *ygv,
                      const int *cell_ptr, const int *cell_el, double x0, double y0,
-                     double dxc, double dyc, int ncx, int ncy, double px, double py)
+                     double dxc, double dyc, int ncx, int ncy, float px, float py)
 {
-    int cx = (int)floor((px - x0) / dxc), cy = (int)floor((py - y0) / dyc);
+    int cx = (int)floor(((double)px - x0) / dxc), cy = (int)floor(((double)py - y0) / dyc);
     if (cx < 0 || cy < 0 || cx >= ncx || cy >= ncy) return -1;
     int best = -1;
     int cell = cx + ncx * cy;
     for (int q = cell_ptr[cell]; q < cell_ptr[cell + 1]; q++) {
         int ie = cell_el[q];
-        if (best >= 0 && ie > best) continue;
-        const int *kn = m->nen3v + 3 * ie;
-        double x1 = xgv[kn[0]], y1 = ygv[kn[0]], x2 = xgv[kn[1]], y2 = ygv[kn[1]];
-        double x3 = xgv[kn[2]], y3 = ygv[kn[2]];
-        double d = (x2 - x1) * (y3 - y1) - (x3 - x1) * (y2 - y1);
-        double l1 = ((x2 - px) * (y3 - py) - (x3 - px) * (y2 - py)) / d;
-        double l2 = ((x3 - px) * (y1 - py) - (x1 - px) * (y3 - py)) / d;
-        double l3 = 1. - l1 - l2;
-        const double tol = -1e-12;
-        if (l1 >= tol && l2 >= tol && l3 >= tol) best = ie;
+        if (ie < best) continue;
+        if (in_element_ref(m->nen3v + 3 * ie, xgv, ygv, px, py)) best = ie;
     }
     return best;
 }
 
 /* tvd_upwind_init_elem: tvd_admin.f90:361-474.  For vertex ic of element ie
  * and the other vertex id, the upwind point is 2*x(ic) - x(id), stored as a
- * default real, with the element containing it (or -1). */
+ * default real, with the element quad_tree_search finds there (or -1). */
 int oc_tvd_init(const oc_mesh *m, const real *xgv, const real *ygv, int *tvd_ie, real *tvd_x, real *tvd_y)
 {
     int nkn = m->nkn, nel = m->nel;
@@ -390,9 +412,9 @@ int oc_tvd_init(const oc_mesh *m, const real *xgv, const real *ygv, int *tvd_ie,
                 int j = (t == 0) ? (ii % 3) + 1 : ((ii + 1) % 3) + 1;
                 int kd = m->nen3v[3 * ie + j - 1];
                 double xd = xgv[kd], yd = ygv[kd];
-                real x = (real)(2 * xc - xd), y = (real)(2 * yc - yd);
-                int ienew = find_elem(m, xgv, ygv, cell_ptr, cell_el, xmin, ymin, dxc, dyc, ncx, ncy,
-                                      (double)x, (double)y);
+                /* the point is a default real, as the reference stores it */
+                float x = (float)(2 * xc - xd), y = (float)(2 * yc - yd);
+                int ienew = find_elem(m, xgv, ygv, cell_ptr, cell_el, xmin, ymin, dxc, dyc, ncx, ncy, x, y);
                 int idx = (j - 1) + 3 * (ii - 1) + 9 * ie;
                 tvd_ie[idx] = ienew;
                 tvd_x[idx] = x;

@@ -220,8 +220,8 @@ static bool natural_coords(const double *ev, int ie, double x, double y, double 
 
 // For each ordered vertex pair (ic,id) of an element the upwind point is
 // 2*x(ic) - x(id) (stored as a default real); the element containing it is
-// found by a uniform-grid search (lowest element index on ties) and the
-// interpolation weights are stored.  Pair p = 2*(ic-1) + t enumerates id =
+// found as quad_tree_search finds it (highest element number on shared edges)
+// and the interpolation weights are stored.  Pair p = 2*(ic-1) + t enumerates id =
 // mod(ic,3)+1 (t=0) and mod(ic+1,3)+1 (t=1).
 void host_tvd_tables(int nkn, int nel, const int *nen3v0, const float *xg, const float *yg, const double *ev,
                      std::vector<int> &tel, std::vector<int> &tkn, std::vector<double> &tw) {
@@ -249,21 +249,33 @@ void host_tvd_tables(int nkn, int nel, const int *nen3v0, const float *xg, const
         for (int cy = cy0; cy <= cy1; cy++)
             for (int cx = cx0; cx <= cx1; cx++) cell[(size_t)cx + (size_t)nc * cy].push_back(ie);
     }
-    auto locate = [&](double px, double py) {
-        const int cx = (int)std::floor((px - xmin) / dxc), cy = (int)std::floor((py - ymin) / dyc);
+    // quad_tree_search (src/utils/shyutil/quad_tree.f90:313-397): of the
+    // elements containing the point by in_element (find.f90:365-400: bounding
+    // box in default reals, then intri's edge tests, regular.f90:1252-1282, in
+    // double of default-real differences, edges inside), the one with the
+    // highest number.  Every element whose bounding box holds the point is in
+    // the point's cell, so the cells give the quad tree's answer.
+    auto inside = [&](int ie, float xp, float yp) {
+        const int *kn = nen3v0 + 3 * ie;
+        float x[3], y[3];
+        for (int ii = 0; ii < 3; ii++) { x[ii] = xg[kn[ii]]; y[ii] = yg[kn[ii]]; }
+        const float x0 = std::min(x[0], std::min(x[1], x[2])), x1 = std::max(x[0], std::max(x[1], x[2]));
+        const float y0 = std::min(y[0], std::min(y[1], y[2])), y1 = std::max(y[0], std::max(y[1], y[2]));
+        if (!(xp >= x0 && xp <= x1 && yp >= y0 && yp <= y1)) return false;
+        for (int k1 = 0; k1 < 3; k1++) {
+            const int k2 = (k1 + 1) % 3;
+            const double ex = (double)(float)(x[k2] - x[k1]), ey = (double)(float)(y[k2] - y[k1]);
+            const double scal = (double)(float)(xp - x[k1]) * -ey + (double)(float)(yp - y[k1]) * ex;
+            if (scal < 0.) return false;
+        }
+        return true;
+    };
+    auto locate = [&](float px, float py) {
+        const int cx = (int)std::floor(((double)px - xmin) / dxc), cy = (int)std::floor(((double)py - ymin) / dyc);
         if (cx < 0 || cy < 0 || cx >= nc || cy >= nc) return -1;
         int best = -1;
-        for (int ie : cell[(size_t)cx + (size_t)nc * cy]) {
-            if (best >= 0 && ie > best) continue;
-            const int *kn = nen3v0 + 3 * ie;
-            const double x1 = xg[kn[0]], y1 = yg[kn[0]], x2 = xg[kn[1]], y2 = yg[kn[1]];
-            const double x3 = xg[kn[2]], y3 = yg[kn[2]];
-            const double d = (x2 - x1) * (y3 - y1) - (x3 - x1) * (y2 - y1);
-            const double l1 = ((x2 - px) * (y3 - py) - (x3 - px) * (y2 - py)) / d;
-            const double l2 = ((x3 - px) * (y1 - py) - (x1 - px) * (y3 - py)) / d;
-            const double l3 = 1. - l1 - l2;
-            if (l1 >= -1e-12 && l2 >= -1e-12 && l3 >= -1e-12) best = ie;
-        }
+        for (int ie : cell[(size_t)cx + (size_t)nc * cy])
+            if (ie > best && inside(ie, px, py)) best = ie;
         return best;
     };
     tel.assign(6 * (size_t)nel, -1);
