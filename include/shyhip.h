@@ -96,6 +96,11 @@ int  shyhip_destroy(shyhip_model *h);
  * defaults, default_parms in para.f90 / def_para.f90) */
 int  shyhip_get_params(const shyhip_model *h, shyhip_params *p);
 int  shyhip_set_params(shyhip_model *h, const shyhip_params *p);
+/* the same checks without a model and without a GPU (the text is in
+ * shyhip_last_error(NULL)): a run the device path cannot do is refused,
+ * as the reference stops on an unknown option (e.g. bottom_friction,
+ * chezy.f90:373-376, for ireib outside 0-10) */
+int  shyhip_check_params(const shyhip_params *p);
 
 /* several independent basins in one model (an ensemble): member m owns the
  * nodes node_start[m] .. node_start[m+1]-1 and the elements

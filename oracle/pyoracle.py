@@ -265,6 +265,32 @@ class OracleModel:
         znv[dry] = v1[dry] / v2[dry]
         a["znv"][:] = znv
 
+    def set_initial_level(self, znv):
+        """A non-uniform start level on an all-wet basin: znv and zenv, the
+        new depths from it (setdepth), then copy_uvz/copy_depth so the old
+        time level equals the new."""
+        lib, m, s = self.lib, C.byref(self.mesh), C.byref(self.state)
+        a, b = self.a, self.basin
+        a["znv"][:] = np.asarray(znv, dtype=self.real)
+        a["zenv"][:] = a["znv"][b.nen3v]
+        rc = lib.oc_setdepth(m, ptr(a["hdknv"]), ptr(a["hdenv"]), ptr(a["zenv"]))
+        assert rc == 0, "setdepth failed"
+        lib.oc_copy_uvz(m, s)
+        lib.oc_copy_depth(m, s)
+
+    def transports(self):
+        """hydro_transports alone (bottom friction + the element columns) on
+        the current state: utlnv, vtlnv, ddxv, ddyv, rfricv."""
+        self.lib.oc_hydro_transports(C.byref(self.mesh), C.byref(self.state), C.byref(self.params))
+
+    def scalar_stability(self, dt=None, rstol=1.0):
+        """scalar_stability on the current state: (istot, rindex)."""
+        ri = self.real_c(0)
+        dt = self.params_dict["dt"] if dt is None else dt
+        istot = self.lib.oc_scalar_stability(C.byref(self.mesh), C.byref(self.state), C.byref(self.params),
+                                             self.real_c(dt), self.real_c(rstol), C.byref(ri))
+        return istot, ri.value
+
     # ---------------------------------------------------------------- step
     def set_boundary(self, level):
         self.a["rzv"][:] = FLAG

@@ -134,6 +134,8 @@ int  oc_color(int nkn, int nel, const int *nen3v, int *color, int *color_ptr, in
 void oc_set_area(const oc_mesh *m, real *areakv);
 int  oc_tvd_init(const oc_mesh *m, const real *xgv, const real *ygv, int *tvd_ie, real *tvd_x, real *tvd_y);
 void oc_tvd_upwind_table(const oc_mesh *m, int l, const real *cv, real *out);
+int  oc_scalar_stability(const oc_mesh *m, const oc_state *s, const oc_params *p, real dt, real rstol,
+                         real *rindex);
 
 /* band solvers (penta.f90) */
 int  oc_penta_fact(int n, double *val);           /* val[(j+2) + 5*(i-1)], j=-2..2 */

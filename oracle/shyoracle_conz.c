@@ -534,10 +534,149 @@ int oc_bndo_setbc(const oc_mesh *m, real *cv, const real *rbc, const real *uprv,
     return 0;
 }
 
-/* scal3sh substep loop: concentration.f90:565-810 (istot given) */
+/* scalar_stability (stability.f90:220-260): scalar_compute_stability
+ * (:117-177) runs conzstab (concentration.f90:1573-2097) for one second
+ * (ddt = 1, istot = isact = 1) with difmol = 0 and without nudging or
+ * sinking.  conzstab sums per node and layer the advective flux (the larger
+ * of the last in- and out-flux seen over the element's vertices so far, as
+ * the reference carries them from vertex to vertex), horizontal and explicit
+ * vertical diffusion and point sources, divides by the node volume and takes
+ * the maximum over the layers of the nodes that are not level boundaries,
+ * scaled by 1/rstol.  Then rindex = dt * that and istot = 1 + rindex,
+ * truncated.  Returns istot; *rindex receives rindex. */
+int oc_scalar_stability(const oc_mesh *m, const oc_state *s, const oc_params *p, real dt, real rstol,
+                        real *rindex)
+{
+    const int nlvdi = m->nlvdi, nkn = m->nkn;
+    double az = p->azpar, ad = p->adpar, aa = p->aapar;
+    double azt = 1. - az, adt = 1. - ad, aat = 1. - aa;
+    const double ddt = 1.;
+    const double rstot = 1., rso = 0. / rstot, rsn = 1. / rstot;
+    const double rsot = 1. - rso, rsnt = 1. - rsn;
+    const double dtt = ddt / rstot;
+    const real difmol = 0, rkpar = p->rkpar;
+    size_t nn = (size_t)nlvdi * nkn;
+    double *cn = (double *)calloc(nn, sizeof(double));
+    double *co = (double *)calloc(nn, sizeof(double));
+    double *cdiag = (double *)calloc(nn, sizeof(double));
+    double *clow = (double *)calloc(nn, sizeof(double));
+    double *chigh = (double *)calloc(nn, sizeof(double));
+    int L2 = nlvdi + 2;
+    double *hdv = (double *)calloc((size_t)L2, sizeof(double));
+    double *presentl = (double *)calloc((size_t)L2, sizeof(double));
+    double *hold = (double *)calloc((size_t)3 * L2, sizeof(double));
+    double *hnew = (double *)calloc((size_t)3 * L2, sizeof(double));
+    double *wl = (double *)calloc((size_t)3 * L2, sizeof(double));
+#define H2(a, l, ii) (a)[(l) + L2 * ((ii) - 1)]
+    if (s->mfluxv)
+        for (int k = 0; k < nkn; k++)
+            for (int l = 1; l <= m->nlv; l++)
+                if (A3K(s->mfluxv, l, k) > 0) A3K(co, l, k) = A3K(s->mfluxv, l, k);
+    /* the layer arrays keep what earlier elements left below this one's
+     * bottom, as the reference's do; `present` is zero there */
+    for (int ie = 0; ie < m->nel; ie++) {
+        int kn[3];
+        double b[3], c[3];
+        for (int ii = 0; ii < 3; ii++) {
+            kn[ii] = m->nen3v[3 * ie + ii];
+            b[ii] = EV(ii + 4, ie);
+            c[ii] = EV(ii + 7, ie);
+        }
+        double aj = EV(10, ie), aj4 = 4. * aj;
+        int ilevel = m->ilhv[ie], jlevel = 1;
+        for (int l = jlevel; l <= ilevel; l++) {
+            hdv[l] = A3E(s->hdeov, l, ie);
+            presentl[l] = 1.;
+            for (int ii = 1; ii <= 3; ii++) {
+                int k = kn[ii - 1];
+                double hn = A3K(s->hdknv, l, k), ho = A3K(s->hdkov, l, k);
+                H2(hold, l, ii) = rso * hn + rsot * ho;
+                H2(hnew, l, ii) = rsn * hn + rsnt * ho;
+                H2(wl, l, ii) = AW(s->wlnv, l, k);
+            }
+        }
+        for (int l = ilevel + 1; l <= m->nlv; l++) presentl[l] = 0.;
+        for (int ii = 1; ii <= 3; ii++) H2(wl, ilevel, ii) = 0.;
+        for (int l = jlevel; l <= ilevel; l++) {
+            double us = az * A3E(s->utlnv, l, ie) + azt * A3E(s->utlov, l, ie);
+            double vs = az * A3E(s->vtlnv, l, ie) + azt * A3E(s->vtlov, l, ie);
+            double rk3 = 3. * rkpar * (s->difhv ? A3E(s->difhv, l, ie) : (real)0);
+            double f[3], fwin[3], fwout[3], fd[3], wdiff[3];
+            for (int ii = 1; ii <= 3; ii++) {
+                int k = kn[ii - 1];
+                wdiff[ii - 1] = m->wdifhv ? m->wdifhv[(ii - 1) + 3 * (ii - 1) + 9 * ie] : 0.;
+                f[ii - 1] = us * b[ii - 1] + vs * c[ii - 1];
+                fwin[ii - 1] = 0.;
+                fwout[ii - 1] = 0.;
+                double w = H2(wl, l - 1, ii);
+                if (w > 0.) fwout[ii - 1] = fwout[ii - 1] + aat * w;
+                else fwin[ii - 1] = fwin[ii - 1] - aat * w;
+                w = H2(wl, l, ii);
+                if (w > 0.) fwin[ii - 1] = fwin[ii - 1] + aat * w;
+                else fwout[ii - 1] = fwout[ii - 1] - aat * w;
+                double rvptop = (double)(AW(s->difv, l - 1, k) + difmol);
+                double rvpbot = (double)(AW(s->difv, l, k) + difmol);
+                double hmtop = 2. * rvptop * presentl[l - 1] / (hdv[l - 1] + hdv[l]);
+                double hmbot = 2. * rvpbot * presentl[l + 1] / (hdv[l] + hdv[l + 1]);
+                fd[ii - 1] = adt * (hmtop + hmbot);
+            }
+            double coadv = 0., ciadv = 0.;
+            for (int ii = 1; ii <= 3; ii++) {
+                int k = kn[ii - 1];
+                double hmed = H2(hold, l, ii);
+                double chdiff = dtt * aj4 * rk3 * hmed * wdiff[ii - 1];
+                double cvdiff = dtt * aj4 * fd[ii - 1];
+                double chadv = dtt * aj4 * 3. * f[ii - 1];
+                double cviadv = dtt * aj4 * fwin[ii - 1];
+                double cvoadv = dtt * aj4 * fwout[ii - 1];
+                if (chadv < 0.) coadv = cvoadv - chadv;
+                else ciadv = cviadv + chadv;
+                double cadv = dmax(coadv, ciadv);
+                A3K(chigh, l, k) += cadv;
+                A3K(clow, l, k) += chdiff;
+                A3K(cn, l, k) += cvdiff;
+            }
+        }
+        for (int l = jlevel; l <= ilevel; l++)
+            for (int ii = 1; ii <= 3; ii++) {
+                int k = kn[ii - 1];
+                double hmed = dmin(H2(hold, l, ii), H2(hnew, l, ii));
+                A3K(cdiag, l, k) += aj4 * hmed;
+            }
+    }
+    double stabind = 0.;
+    for (int k = 0; k < nkn; k++) {
+        int ibc = m->ibnd ? m->ibnd[k] : 0;
+        if (ibc > 0 && m->ibtyp[ibc] == 1) continue;          /* is_zeta_bound */
+        for (int l = 1; l <= m->ilhkv[k]; l++) {
+            double voltot = A3K(cdiag, l, k);
+            double flxtot = A3K(chigh, l, k) + A3K(clow, l, k) + A3K(cn, l, k) + A3K(co, l, k);
+            if (voltot > 0.) stabind = dmax(stabind, flxtot / voltot);
+        }
+    }
+    float raux = (float)stabind;                             /* shympi_max of a real */
+    stabind = raux;
+    real sindex = (real)(stabind / (double)rstol);
+    real ri = dt * sindex;
+    free(cn); free(co); free(cdiag); free(clow); free(chigh);
+    free(hdv); free(presentl); free(hold); free(hnew); free(wl);
+#undef H2
+    *rindex = ri;
+    return (int)((real)1 + ri);
+}
+
+/* scal3sh substep loop: concentration.f90:565-810.  istot <= 0: the
+ * number of substeps from scalar_stability (rstol = 1, the STR default), as
+ * the reference sets it (:678-700); more than istot_max = 1000 is the
+ * reference's error stop (returns 3). */
 int oc_scal3sh(const oc_mesh *m, oc_state *s, const oc_params *p, real *cnv,
                const real *rcv, int istot, int clip_small)
 {
+    if (istot <= 0) {
+        real rindex;
+        istot = oc_scalar_stability(m, s, p, (real)p->dt, (real)1, &rindex);
+        if (istot > 1000) return 3;
+    }
     size_t nn = (size_t)m->nlvdi * m->nkn;
     real *sbconz = (real *)calloc(nn, sizeof(real));
     int ret = 0;

@@ -49,6 +49,14 @@ void oc_bottom_friction(const oc_mesh *m, oc_state *s, const oc_params *p)
         case 3: rcd = p->grav / (rfric * rfric); rr = rcd * uv / (hzg * hzg); break;
         case 4: rr = rfric / hzg; break;
         case 5: rcd = rfric; rr = rcd * uv / (hzg * hzg); break;
+        case 6: {
+            /* rfric is z0: cdf (chezy.f90:402-419), kappa = 0.4; the log is
+             * taken in double and rounded (the device does the same) */
+            real cds = (real)0.4 / (real)log((double)((rfric + (real)0.5 * hzg) / rfric));
+            rcd = cds * cds;
+            rr = rcd * uv / (hzg * hzg);
+            break;
+        }
         case 7:
             if (rfric >= 1) rcd = p->grav / ((rfric * rfric) * (real)pow((double)hzg, (double)drittl));
             else rcd = rfric;

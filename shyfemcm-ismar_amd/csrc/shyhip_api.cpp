@@ -344,17 +344,33 @@ int shyhip_get_params(const shyhip_model *h, shyhip_params *p) {
     return 0;
 }
 
+// the parameter checks of shyhip_set_params, against a model (M) or alone
+static int check_params(Model *M, const shyhip_params *p) {
+    if (p->dt <= 0.) return fail(M, "dt must be positive");
+    if (p->itvdv > 0 && p->aapar != 0.f)
+        return fail(M, "implicit vertical advection (aapar != 0) cannot be combined with itvdv > 0");
+    if (p->ireib == 8 || p->ireib == 9)
+        return fail(M, "ireib = 8/9 (roughness from the sediment or fluid-mud modules) is not supported by the device path");
+    if (p->ireib < 0 || (p->ireib > 7 && p->ireib != 10))
+        return fail(M, "unknown friction (ireib), as bottom_friction stops (chezy.f90:373-376)");
+    if (p->rkpar != 0.f) return fail(M, "horizontal diffusion (rkpar != 0) is not supported by the device path");
+    if (p->itvd == 1) return fail(M, "itvd = 1 (gradient TVD) is not supported by the device path; use 0 or 2");
+    if (p->itvd < 0 || p->itvd > 2 || p->itvdv < 0 || p->itvdv > 1) return fail(M, "itvd must be 0 or 2, itvdv 0 or 1");
+    if (p->block_rows < 0 || p->block_rows > kMaxBlockRows) return fail(M, "block_rows must be in 0..1024");
+    if (p->tri_sweeps < 0 || p->tri_sweeps > 64) return fail(M, "tri_sweeps must be in 0..64");
+    if (!(p->inner_rtol > 0. && p->inner_rtol < 1.)) return fail(M, "inner_rtol must be in (0,1)");
+    return 0;
+}
+
+int shyhip_check_params(const shyhip_params *p) {
+    if (!p) return fail(nullptr, "null argument");
+    return check_params(nullptr, p);
+}
+
 int shyhip_set_params(shyhip_model *h, const shyhip_params *p) {
     if (!h || !p) return fail(nullptr, "null argument");
     Model &M = h->M;
-    if (p->dt <= 0.) return fail(&M, "dt must be positive");
-    if (p->itvdv > 0 && p->aapar != 0.f)
-        return fail(&M, "implicit vertical advection (aapar != 0) cannot be combined with itvdv > 0");
-    if (p->rkpar != 0.f) return fail(&M, "horizontal diffusion (rkpar != 0) is not supported by the device path");
-    if (p->itvd == 1) return fail(&M, "itvd = 1 (gradient TVD) is not supported by the device path; use 0 or 2");
-    if (p->block_rows < 0 || p->block_rows > kMaxBlockRows) return fail(&M, "block_rows must be in 0..1024");
-    if (p->tri_sweeps < 0 || p->tri_sweeps > 64) return fail(&M, "tri_sweeps must be in 0..64");
-    if (!(p->inner_rtol > 0. && p->inner_rtol < 1.)) return fail(&M, "inner_rtol must be in (0,1)");
+    if (check_params(&M, p)) return 1;
     M.p = *p;
     return 0;
 }

@@ -87,7 +87,8 @@ static HydroArgs<S> make_args(Model &M) {
 static inline int nblocks(int n) { return (n + kThreads - 1) / kThreads; }
 
 // ------------------------------------------------------------------ friction
-// bottom_friction, chezy.f90:239-398 (formulations without roughness length)
+// bottom_friction, chezy.f90:239-398: ireib 0-7 and 10 (8 and 9 take z0 from
+// the sediment and fluid-mud modules and are rejected by shyhip_set_params)
 template <typename S>
 __device__ inline S friction(const HydroArgs<S> &a, int ie, int lmax) {
     using R = S;
@@ -105,6 +106,14 @@ __device__ inline S friction(const HydroArgs<S> &a, int ie, int lmax) {
     case 3: rcd = grav / (rfric * rfric); rr = rcd * uv / (hzg * hzg); break;
     case 4: rr = rfric / hzg; break;
     case 5: rcd = rfric; rr = rcd * uv / (hzg * hzg); break;
+    case 6: {
+        // rfric is the roughness length z0: cd = (kappa / log((z0 + h/2) / z0))^2,
+        // kappa = 0.4 (cdf, chezy.f90:402-419); log in double, rounded
+        const R cds = (R)0.4 / (R)log((double)((rfric + (R)0.5 * hzg) / rfric));
+        rcd = cds * cds;
+        rr = rcd * uv / (hzg * hzg);
+        break;
+    }
     case 7:
         rcd = (rfric >= (R)1) ? grav / ((rfric * rfric) * (R)pow((double)hzg, (double)((R)1 / (R)3))) : rfric;
         rr = rcd * uv / (hzg * hzg);

@@ -83,6 +83,17 @@ def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def check_params(params: dict | None = None) -> None:
+    """Raises ShyhipError if the device path refuses these parameters (the
+    checks of shyhip_set_params, run without a GPU)."""
+    lib = load_library()
+    p = dict(DEFAULTS)
+    if params:
+        p.update(params)
+    if lib.shyhip_check_params(C.byref(ShyParams(**p))):
+        raise ShyhipError(f"check_params: {lib.shyhip_last_error(None).decode()}")
+
+
 class Model:
     """One basin resident on the GPU, stepped through libshyhip."""
 
