@@ -37,9 +37,9 @@ sys.path.insert(0, os.path.join(ROOT, "shyfemcm-ismar_amd"))
 HBM_PEAK_GBS = 8000.0
 
 
-def _lagoon(args):
+def _lagoon(args, hmin=0.05):
     from shyhip import mesh
-    b = mesh.lagoon(args.nodes, seed=args.seed, lx=40000.0, ly=20000.0, nlv=args.layers, tile=32)
+    b = mesh.lagoon(args.nodes, seed=args.seed, lx=40000.0, ly=20000.0, nlv=args.layers, tile=32, hmin=hmin)
     tr = dict(temp=mesh.smooth_tracer(b, args.seed + 1, 15.0, 25.0),
               salt=mesh.smooth_tracer(b, args.seed + 2, 25.0, 38.0))
     return b, tr
@@ -51,6 +51,9 @@ def _level(t, phase=0.0):
 
 
 PARAMS = dict(dt=60.0, ireib=5, itvd=2, itvdv=1, aapar=0.0, itpre=2, block_rows=256, tri_sweeps=3)
+# scal3sh substeps: 0 = chosen every step by scalar_stability, as the
+# reference chooses them (stability.f90:220-260); on this lagoon 3-4
+ISTOT = 0
 WIND = (8.0, 0.9)
 
 
@@ -158,31 +161,85 @@ def run_gpu(b, tr, prec, args, steps=None, baroclinic=False):
     nt = max(5, min(20, steps))
     install_forcing(m, args.warmup + steps + nt)
     for _ in range(args.warmup):
-        m.step(None)
+        m.step(None, istot=ISTOT)
     m.sync()
     t0 = time.perf_counter()
+    substeps = []
     for _ in range(steps):
-        m.step(None)
+        m.step(None, istot=ISTOT)
+        substeps.append(m.last_istot())
     m.sync()
     elapsed = time.perf_counter() - t0
     zn = m.pull("znv")
+    iweg = m.pull("iwegv")
     stats = m.stats()
+    stats["istot_mean"] = round(float(np.mean(substeps)), 3)
+    stats["istot_max"] = int(np.max(substeps))
     # kernel timing pass (events on the library's stream), outside the timed region
     m.set_timing(True)
     for _ in range(nt):
-        m.step(None)
+        m.step(None, istot=ISTOT)
     timing = m.timing()
     m.set_timing(False)
     m.close()
-    return dict(elapsed=elapsed, timing=timing, nt=nt, znv=zn, stats=stats)
+    return dict(elapsed=elapsed, timing=timing, nt=nt, znv=zn, iwegv=iweg, stats=stats)
 
 
-def cpu_baseline(b, tr, args):
-    """The oracle (reference precision policy, ILUT + BiCGStab) on the same
-    basin, a bounded number of steps, one host core."""
+def _zeta_err(b, z32, z64, w32, w64):
+    """FP32-vs-FP64 zeta differences: over all nodes, and over the nodes
+    more than two element rings away from every element whose wet/dry state
+    differs between the two runs."""
+    dz = np.abs(np.asarray(z32, np.float64) - np.asarray(z64, np.float64))
+    near = np.zeros(b.nkn, bool)
+    near[b.nen3v[np.nonzero(w32 != w64)[0]].reshape(-1)] = True
+    for _ in range(2):
+        near[b.nen3v[near[b.nen3v].any(axis=1)].reshape(-1)] = True
+    far = dz[~near]
+    return {"max": float(dz.max()), "p99": float(np.percentile(dz, 99)), "rms": float(np.sqrt(np.mean(dz ** 2))),
+            "elements_wetdry_differ": int((w32 != w64).sum()),
+            "max_beyond_2_rings_of_differing_elements": float(far.max()) if far.size else None}
+
+
+def accuracy_run(b, tr, steps):
+    """The FP32 + FP64-refined path and the plain FP64 path side by side
+    (same basin, forcing and substeps rule): the zeta difference after
+    `steps` steps."""
+    import shyhip
+    from shyhip import mesh
+    ms = {p: shyhip.Model(b, p, params=PARAMS, visv=1e-3, difv=1e-4, tracers=tr) for p in ("fp32", "fp64")}
+    for m in ms.values():
+        m.set_wind(*mesh.wind_stress(*WIND))
+    for n in range(steps):
+        lev = _level((n + 1) * PARAMS["dt"])
+        for m in ms.values():
+            m.step(lev, istot=ISTOT)
+    z = {p: m.pull("znv") for p, m in ms.items()}
+    w = {p: m.pull("iwegv") for p, m in ms.items()}
+    for m in ms.values():
+        m.close()
+    return _zeta_err(b, z["fp32"], z["fp64"], w["fp32"], w["fp64"])
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def cpu_baseline(b, tr, args, threads):
+    """The oracle (reference precision policy, ILUT + BiCGStab, substeps from
+    scalar_stability) on the same basin for a bounded time, on `threads`
+    host threads: OpenMP over the element columns and over conz3d's colours
+    and nodes, where the reference runs in parallel; the solve is serial, as
+    SPARSKIT's is.  Returns (steps, seconds)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     from shyhip import mesh
+    pyoracle.load("f32").oc_set_threads(threads)
     om = pyoracle.OracleModel(b, "f32", params=PARAMS, visv=1e-3, difv=1e-4, tracers=tr)
     om.set_wind(*mesh.wind_stress(*WIND))
     t = 0.0
@@ -190,11 +247,13 @@ def cpu_baseline(b, tr, args):
     n = 0
     while n < args.cpu_steps or (time.perf_counter() - t0) < args.cpu_seconds and n < 200:
         t += PARAMS["dt"]
-        om.step(_level(t))
+        om.step(_level(t), istot=ISTOT)
         n += 1
         if time.perf_counter() - t0 > 4 * args.cpu_seconds:
             break
-    return n, time.perf_counter() - t0
+    el = time.perf_counter() - t0
+    pyoracle.load("f32").oc_set_threads(1)
+    return n, el
 
 
 def ensemble_members(args):
@@ -223,11 +282,11 @@ def run_ensemble(mems, prec, steps, warmup):
     ens = ensemble.Ensemble(mems, prec, params=PARAMS, visv=1e-3, difv=1e-4, deferred=True)
     ens.set_level_series(PARAMS["dt"] * (warmup + steps + 2))     # each member's boundary file on the device
     for _ in range(warmup):
-        ens.step()
+        ens.step(istot=ISTOT)
     ens.sync()
     t0 = time.perf_counter()
     for _ in range(steps):
-        ens.step()
+        ens.step(istot=ISTOT)
     ens.sync()
     el = time.perf_counter() - t0
     z = [(ens.pull("znv", i), ens.pull("iwegv", i)) for i in range(len(mems))]
@@ -267,11 +326,11 @@ def run_members_alone(mems, prec, steps, warmup):
         rec_t = PARAMS["dt"] * np.arange(-2, warmup + steps + 4)
         m.set_boundary_series(1, rec_t, np.array([mem["level"](t) for t in rec_t], np.float32), intpol=4, dtime0=0.0)
         for _ in range(warmup):
-            m.step(None)
+            m.step(None, istot=ISTOT)
         m.sync()
         t0 = time.perf_counter()
         for _ in range(steps):
-            m.step(None)
+            m.step(None, istot=ISTOT)
         m.sync()
         tot_t += time.perf_counter() - t0
         tot_el += int(mem["basin"].ilhv.sum()) * steps
@@ -308,6 +367,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-baroclinic", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--acc-steps", type=int, default=120)
     ap.add_argument("--bcl-steps", type=int, default=60)
     ap.add_argument("--ens-members", type=int, default=16)
     ap.add_argument("--ens-min", type=int, default=5000)
@@ -333,7 +394,6 @@ def main():
     for prec in ("fp64", "fp32"):
         res[prec] = run_gpu(b, tr, prec, args)
     thr = {p: EL * args.steps / r["elapsed"] for p, r in res.items()}
-    dz = np.abs(res["fp32"]["znv"] - res["fp64"]["znv"])
 
     # roofline of the headline path's dominant kernel (largest time per step).
     # The headline workload is the lagoon on the plain FP64 GPU path; the
@@ -363,15 +423,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"3D lagoon: {b.nkn} nodes, {b.nel} elements, {args.layers} z-layers, T+S (TVD), "
+        "config": {"workload": f"3D lagoon: {b.nkn} nodes, {b.nel} elements, {args.layers} z-layers, T+S (TVD, "
+                               f"substeps from scalar_stability as the reference chooses them), "
                                f"dt=60 s, M2+K1 tide (boundary file at every step, cubic interpolation on the "
                                f"device), 8 m/s wind, seed {args.seed}",
                    "elem_layers": EL, "parallelism": "single GPU"},
         "fp32_refined_value": round(thr["fp32"], 1),
         "fp32_refined_ms_per_step": round(1e3 * res["fp32"]["elapsed"] / args.steps, 4),
         "fp32_over_fp64": round(thr["fp32"] / thr["fp64"], 3),
-        "zeta_err_fp32_vs_fp64_m": {"max": float(dz.max()), "p99": float(np.percentile(dz, 99)),
-                                    "rms": float(np.sqrt(np.mean(dz ** 2)))},
+        "zeta_err_fp32_vs_fp64_m": _zeta_err(b, res["fp32"]["znv"], res["fp64"]["znv"], res["fp32"]["iwegv"],
+                                             res["fp64"]["iwegv"]),
         "solver": {p: res[p]["stats"] for p in res},
         "kernel_ms_per_step_fp64": {k: round(v, 4) for k, v in per_step.items() if v > 0},
         "kernel_ms_per_step_fp32": {k: round(v[0] / res["fp32"]["nt"], 4) for k, v in res["fp32"]["timing"].items()
@@ -381,6 +442,16 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_launch": kb[dom], "avg_launch_ms": round(avg[dom], 5)},
     }
+    if not args.no_accuracy:
+        # The workload clips depths at 0.05 m, the drying threshold hzoff:
+        # at the rest level thousands of nodes sit exactly on it, and their
+        # elements' wet/dry state follows round-off (the reference's own
+        # FP32 and FP64 runs disagree on some of them).  The same lagoon with
+        # depths clipped at 0.06 m has no such ties.
+        bt, trt = _lagoon(args, hmin=0.06)
+        out["zeta_err_fp32_vs_fp64_no_threshold_ties_m"] = dict(
+            accuracy_run(bt, trt, args.acc_steps), steps=args.acc_steps,
+            workload="the same lagoon with depths clipped at 0.06 m instead of 0.05 m (hzoff)")
     if not args.no_baroclinic:
         # T and S drive the momentum equations.  With the workload's fields
         # (15-25 C, 25-38 psu, salinity falling with depth) the added forcing
@@ -426,12 +497,19 @@ def main():
     if not args.no_serving:
         out["serving"] = run_serving(args)
     if not args.no_cpu:
-        n, el = cpu_baseline(b, tr, args)
-        out["cpu_baseline"] = {"value": round(EL * n / el, 1), "unit": "elem*layer*steps/s", "cores": 1,
+        # all the host threads this process may use (the box's share), and one
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        n, el = cpu_baseline(b, tr, args, threads)
+        n1, el1 = cpu_baseline(b, tr, args, 1)
+        out["cpu_baseline"] = {"value": round(EL * n / el, 1), "unit": "elem*layer*steps/s", "cores": threads,
                                "kind": "port",
                                "sample": f"{n} steps of the same workload through the C oracle "
-                                         f"(f32 state, ILUT(5,1e-2) + BiCGStab), {el:.1f} s on "
-                                         f"{platform.processor() or platform.machine()}"}
+                                         f"(f32 state, ILUT(5,1e-2) + BiCGStab, substeps from "
+                                         f"scalar_stability; OpenMP over element columns and conz3d "
+                                         f"colours/nodes, serial solve), {el:.1f} s on {threads} threads of "
+                                         f"{_cpu_model()} (nproc {os.cpu_count()})",
+                               "one_core_value": round(EL * n1 / el1, 1),
+                               "one_core_sample": f"{n1} steps in {el1:.1f} s on 1 thread"}
     print(json.dumps(out))
 
 

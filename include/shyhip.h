@@ -92,6 +92,8 @@ int  shyhip_device_count(int *n);
 /* life cycle ------------------------------------------------------------ */
 int  shyhip_create(shyhip_model **h, int precision, int nkn, int nel, int nlvdi, int nlv, int ntracer);
 int  shyhip_destroy(shyhip_model *h);
+/* the sizes the model was created with (any pointer may be NULL) */
+int  shyhip_get_sizes(const shyhip_model *h, int *nkn, int *nel, int *nlvdi, int *nlv, int *ntracer);
 /* the handle's parameters (defaults after create: the reference's STR
  * defaults, default_parms in para.f90 / def_para.f90) */
 int  shyhip_get_params(const shyhip_model *h, shyhip_params *p);
@@ -271,8 +273,36 @@ int  shyhip_hydro_vertical(shyhip_model *h);
 /* one full model step: set_dry, set_timestep, copy_uvz/copy_depth, hydro,
  * and scal3sh for every tracer with istot substeps */
 int  shyhip_step(shyhip_model *h, double dtime, int istot);
-/* scal3sh for tracer itr (0-based); clip_small = 1 for non-temperature */
+/* scal3sh for tracer itr (0-based); clip_small = 1 for non-temperature.
+ * istot <= 0: the number of substeps is chosen as the reference does,
+ * by scalar_stability with rstol = 1 (concentration.f90:678-700; more than
+ * 1000 is an error, status 3); shyhip_step does the same for all tracers. */
 int  shyhip_scal3sh(shyhip_model *h, int itr, int istot, int clip_small);
+/* conz3d_omp (src/shyfem/concentration_omp.f90:65-340): one substep isact of
+ * istot of tracer itr on the device state -- the element and node phases
+ * only; the clipping of small values and the open-boundary treatment around
+ * it are scal3sh's (shyhip_scal3sh).  cbound(nlvddi,nkn): the source
+ * concentration of scal3sh's make_scal_flux (sbconz), NULL = zero. */
+int  shyhip_conz3d(shyhip_model *h, int itr, const float *cbound, int istot, int isact);
+/* the open-boundary treatment scal3sh applies after each substep
+ * (bndo_setbc, src/shyfem/bndo.f90:620-890) on tracer itr, with the boundary
+ * values of SHYHIP_TRACER_BC0 + itr */
+int  shyhip_bndo_setbc(shyhip_model *h, int itr);
+/* scalar_stability (src/shyfem/stability.f90:220-260, conzstab
+ * src/shyfem/concentration.f90:1573-2097) on the current state: the
+ * stability index of the scalar transport for the step dt, rindex, and
+ * istot = 1 + rindex (an ensemble: the largest over its members) */
+int  shyhip_scalar_stability(shyhip_model *h, float rstol, int *istot, double *rindex);
+/* the number of substeps the last scal3sh or step ran (an ensemble: the
+ * largest of its members') */
+int  shyhip_get_istot(shyhip_model *h, int *istot);
+/* an ensemble's members (shyhip_set_members) each choose their istot as they
+ * do alone; the substeps each member ran in the last step (nmem = the
+ * number of members) */
+int  shyhip_member_istot(shyhip_model *h, int nmem, int *istot);
+/* zeta solves since create that failed with warm-started block factors and
+ * were done again with exact ones (the retry is counted, never hidden) */
+int  shyhip_get_retries(shyhip_model *h, long *retries);
 int  shyhip_sync(shyhip_model *h);
 /* Deferred checks (no counterpart in the reference, which runs on the host
  * and stops at once): with on = 1, shyhip_step returns without waiting for
@@ -302,6 +332,10 @@ int  shyhip_get_pattern(shyhip_model *h, int *nnz, int *ntg, int *cols, int *nco
  * calling system_assemble(ie) with kn/hia/hik; the plugin accumulates on the
  * host in element order and solves on the device. */
 int  shyhip_system_init(shyhip_model *h);
+/* system_set_explicit (simsys_spkaux.f90:160-168): from now on the system
+ * is the lumped one, diagonal and right-hand side only, and the solve is
+ * their quotient (:475-479); an off-diagonal entry is an error */
+int  shyhip_system_set_explicit(shyhip_model *h);
 int  shyhip_system_assemble(shyhip_model *h, int ie, const int *kn, const float *hia, const float *hik);
 int  shyhip_system_add_rhs(shyhip_model *h, float dt, int n, const float *array);
 int  shyhip_system_solve(shyhip_model *h, int n, const float *z);

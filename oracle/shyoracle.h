@@ -127,6 +127,7 @@ typedef struct {
     char msg[160];
 } oc_stats;
 
+int  oc_set_threads(int n);
 /* geometry / topology (init) */
 void oc_set_ev(int nkn, int nel, const int *nen3v, const real *xgv, const real *ygv, double *ev);
 int  oc_node_list(int nkn, int nel, const int *nen3v, int *ntg, int *cols, int maxnnz, int *ijp);

@@ -153,6 +153,13 @@ int oc_conz3d(const oc_mesh *m, const oc_state *s, const oc_params *p,
     double *clow = (double *)calloc(nn, sizeof(double));
     double *chigh = (double *)calloc(nn, sizeof(double));
     int L2 = nlvdi + 2;
+#define H2(a, l, ii) (a)[(l) + L2 * ((ii) - 1)]
+    /* colour by colour; inside a colour no two elements share a node, so
+     * its elements may run in parallel (as conz3d_omp runs them) and every
+     * node still receives its contributions in colour order */
+    for (int ccol = 0; ccol < m->ncolor; ccol++) {
+#pragma omp parallel
+    {
     double *hold = (double *)malloc(sizeof(double) * 3 * L2);
     double *hnew = (double *)malloc(sizeof(double) * 3 * L2);
     double *cl = (double *)malloc(sizeof(double) * 3 * L2);
@@ -163,8 +170,8 @@ int oc_conz3d(const oc_mesh *m, const oc_state *s, const oc_params *p,
     double *clc = (double *)malloc(sizeof(double) * 3 * L2);
     double *clm = (double *)malloc(sizeof(double) * 3 * L2);
     double *clp = (double *)malloc(sizeof(double) * 3 * L2);
-#define H2(a, l, ii) (a)[(l) + L2 * ((ii) - 1)]
-    for (int q = 0; q < m->nel; q++) {
+#pragma omp for schedule(static)
+    for (int q = m->color_ptr[ccol]; q < m->color_ptr[ccol + 1]; q++) {
         int ie = m->color_elems[q];
         memset(hold, 0, sizeof(double) * 3 * L2);
         memset(hnew, 0, sizeof(double) * 3 * L2);
@@ -297,9 +304,14 @@ int oc_conz3d(const oc_mesh *m, const oc_state *s, const oc_params *p,
             }
         }
     }
-    /* conz3d_nodes */
+    free(hold); free(hnew); free(cl); free(wl); free(vflux); free(haver); free(presentl);
+    free(clc); free(clm); free(clp);
+    }
+    }
+    /* conz3d_nodes: every node on its own */
     int ret = 0;
-    for (int k = 0; k < nkn && !ret; k++) {
+#pragma omp parallel for schedule(static) reduction(| : ret)
+    for (int k = 0; k < nkn; k++) {
         int ilevel = m->ilhkv[k], jlevel = 1;
         for (int l = jlevel; l <= ilevel; l++) {
             double cconz = cbound ? A3K(cbound, l, k) : 0.;
@@ -329,8 +341,6 @@ int oc_conz3d(const oc_mesh *m, const oc_state *s, const oc_params *p,
         for (size_t i = 0; i < nn; i++) cn1[i] = (real)cn[i];
 #undef H2
     free(cn); free(cdiag); free(clow); free(chigh);
-    free(hold); free(hnew); free(cl); free(wl); free(vflux); free(haver); free(presentl);
-    free(clc); free(clm); free(clp);
     return ret;
 }
 

@@ -215,12 +215,18 @@ static int hydro_intern(const oc_mesh *m, oc_state *s, const oc_params *p, int i
 void oc_hydro_transports(const oc_mesh *m, oc_state *s, const oc_params *p)
 {
     int nlvdi = m->nlvdi;
-    double *smat = (double *)malloc(sizeof(double) * (size_t)(10 * nlvdi + 10));
-    double *rvec = (double *)malloc(sizeof(double) * (size_t)(6 * nlvdi + 6));
-    double *solv = (double *)malloc(sizeof(double) * (size_t)(6 * nlvdi + 6));
     oc_bottom_friction(m, s, p);
-    for (int ie = 0; ie < m->nel; ie++) hydro_intern(m, s, p, ie, smat, rvec, solv);
-    free(smat); free(rvec); free(solv);
+    /* element columns are independent (the reference's OMP PARALLEL DO,
+     * hydrodynamic.f90:837-850) */
+#pragma omp parallel
+    {
+        double *smat = (double *)malloc(sizeof(double) * (size_t)(10 * nlvdi + 10));
+        double *rvec = (double *)malloc(sizeof(double) * (size_t)(6 * nlvdi + 6));
+        double *solv = (double *)malloc(sizeof(double) * (size_t)(6 * nlvdi + 6));
+#pragma omp for schedule(static)
+        for (int ie = 0; ie < m->nel; ie++) hydro_intern(m, s, p, ie, smat, rvec, solv);
+        free(smat); free(rvec); free(solv);
+    }
 }
 
 /* ---------------- hydro_zeta: hydrodynamic.f90:488-754 ---------------- */

@@ -14,6 +14,23 @@
 
 #define EV(i, ie) ev[((i) - 1) + 19 * (ie)]
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* host threads of the parallel loops (element columns, conz3d colours and
+ * nodes); 1 = serial.  The results do not depend on it. */
+int oc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
 /* adjust_bc: evgeom.f90:1007-1042 -- make b (or c) sum to exactly zero */
 static int adjust_bc(double *v1, double *v2, double *v3)
 {

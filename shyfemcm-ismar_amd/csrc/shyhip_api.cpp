@@ -268,7 +268,7 @@ int shyhip_create(shyhip_model **out, int precision, int nkn, int nel, int nlvdi
         rc |= dalloc_s(M, &s.tracer[t], L * nk);
         rc |= dalloc_s(M, &s.tracer_bc[t], L * nk);
     }
-    rc |= dalloc(M, &M.d_err, 8);
+    rc |= dalloc(M, &M.d_err, 16);
     if (rc) {
         std::string e = M.err;
         shyhip_destroy(h);
@@ -330,6 +330,9 @@ int shyhip_destroy(shyhip_model *h) {
     forcing_free_wind(M);
     forcing_free_sources(M);
     dfree(M.d_err);
+    dfree(M.d_cbound);
+    dfree(M.d_node_mem); dfree(M.d_elem_mem); dfree(M.d_mem_istot); dfree(M.d_mem_stab);
+    if (M.h_mem_stab) (void)hipHostFree(M.h_mem_stab);
     dfree(M.d_levels);
     if (M.h_levels) (void)hipHostFree(M.h_levels);
     if (M.h_pinned) (void)hipHostFree(M.h_pinned);
@@ -412,6 +415,29 @@ int shyhip_set_basin(shyhip_model *h, const int *nen3v, const float *xgv, const 
                 if (k < M.mem_node[m] || k >= M.mem_node[m + 1])
                     return fail(&M, "element " + std::to_string(ie + 1) + " uses a node of another member");
             }
+    // maps for members that choose their own scal3sh substeps
+    {
+        const int nmem = (int)M.mem_node.size() - 1;
+        dfree(M.d_node_mem); dfree(M.d_elem_mem); dfree(M.d_mem_istot); dfree(M.d_mem_stab);
+        M.d_node_mem = M.d_elem_mem = M.d_mem_istot = nullptr;
+        M.d_mem_stab = nullptr;
+        if (M.h_mem_stab) (void)hipHostFree(M.h_mem_stab);
+        M.h_mem_stab = nullptr;
+        M.nmem_dev = nmem;
+        if (nmem > 1) {
+            std::vector<int> nm(nkn), em(nel);
+            for (int m = 0; m < nmem; m++) {
+                for (int k = M.mem_node[m]; k < M.mem_node[m + 1]; k++) nm[k] = m;
+                for (int ie = M.mem_elem[m]; ie < M.mem_elem[m + 1]; ie++) em[ie] = m;
+            }
+            int rc = dalloc(M, &M.d_node_mem, nkn) || upload(M, M.d_node_mem, nm.data(), nkn) ||
+                     dalloc(M, &M.d_elem_mem, nel) || upload(M, M.d_elem_mem, em.data(), nel) ||
+                     dalloc(M, &M.d_mem_istot, 2 * (size_t)nmem) || dalloc(M, &M.d_mem_stab, 3 * (size_t)nmem);
+            if (rc) return 1;
+            HIPCHK(M, hipHostMalloc((void **)&M.h_mem_stab, 2 * (size_t)nmem * sizeof(unsigned long long)));
+            M.cur_mem_istot = M.d_mem_istot;
+        }
+    }
     // element colouring of every member on its own (domain_clusterization_ggu)
     M.h_color.assign(nel, 0);
     M.ncolor = 0;
@@ -848,8 +874,12 @@ int shyhip_push(shyhip_model *h, int field, const void *host, int dtype, size_t 
     const Shape sh = shape_of(M, f.kind);
     if (n != sh.n) return fail(&M, "push: size mismatch for field " + std::to_string(field));
     if (f.kind == K_NODE3 && field == SHYHIP_MFLUXV) M.has_mflux = true;
-    M.sv.have_prev = false;     // a pushed state starts a new level history
-    M.sv.fed = false;           // and any pushed matrix or level is the solve's input
+    // a pushed level starts a new level history (the FP32 solve's
+    // extrapolated start), and a pushed level, matrix or right-hand side is
+    // the next solve's input; tracers, coefficients and other fields leave both
+    if (field == SHYHIP_ZNV || field == SHYHIP_ZOV || field == SHYHIP_ZENV || field == SHYHIP_ZEOV)
+        M.sv.have_prev = false;
+    if (field == SHYHIP_ZNV || field == SHYHIP_CSR_VAL || field == SHYHIP_RHS) M.sv.fed = false;
     if (f.isint) {
         if (dtype != SHYHIP_I32) return fail(&M, "integer field needs SHYHIP_I32");
         return upload(M, (int *)*f.ptr, (const int *)host, n);
@@ -1317,6 +1347,86 @@ int shyhip_hydro(shyhip_model *h, double dtime) {
     return hydro_impl(h->M, dtime, true);
 }
 
+// istot of scal3sh: the caller's (1..1000), or for istot <= 0 the
+// reference's choice, scalar_stability with rstol = 1 (the STR default),
+// stopping above istot_max = 1000 (concentration.f90:678-700)
+static int resolve_istot(Model &M, int *istot) {
+    M.mem_istot_on = false;
+    if (*istot > 1000) return fail(&M, "istot must be at most 1000");
+    if (*istot >= 1) {
+        M.last_istot = *istot;
+        return 0;
+    }
+    // the members of an ensemble choose their own istot, as each does alone;
+    // the step runs the largest, and members past their own last substep
+    // are left alone by the kernels
+    const int nmem = M.nmem_dev > 1 ? M.nmem_dev : 1;
+    if (nmem > 1) M.cur_mem_istot = M.d_mem_istot + (size_t)(M.pipe_k & 1) * nmem;
+    std::vector<int> ist(nmem);
+    std::vector<double> ri(nmem);
+    if (launch_scalar_stability(M, 1.0f, ist.data(), ri.data())) return 1;
+    int mx = 1, mn = 1 << 30;
+    for (int m = 0; m < nmem; m++) {
+        if (ist[m] > 1000 || ist[m] < 1)
+            return fail(&M, "scal3sh: istot index too high (" + std::to_string(ist[m]) + ", sindex " +
+                                std::to_string(ri[m]) + (nmem > 1 ? ", member " + std::to_string(m) : "") + ")", 3);
+        mx = std::max(mx, ist[m]);
+        mn = std::min(mn, ist[m]);
+    }
+    *istot = mx;
+    M.mem_istot = ist;
+    M.mem_istot_on = nmem > 1 && mn != mx;
+    M.last_istot = mx;
+    return 0;
+}
+
+int shyhip_member_istot(shyhip_model *h, int nmem, int *istot) {
+    if (!h || !istot) return fail(nullptr, "null argument");
+    const Model &M = h->M;
+    const int n = M.nmem_dev > 1 ? M.nmem_dev : 1;
+    if (nmem != n) return fail(&h->M, "member_istot: the model has " + std::to_string(n) + " members");
+    for (int m = 0; m < n; m++) istot[m] = (M.mem_istot.size() == (size_t)n) ? M.mem_istot[m] : M.last_istot;
+    return 0;
+}
+
+int shyhip_get_sizes(const shyhip_model *h, int *nkn, int *nel, int *nlvdi, int *nlv, int *ntracer) {
+    if (!h) return fail(nullptr, "null handle");
+    const Model &M = h->M;
+    if (nkn) *nkn = M.nkn;
+    if (nel) *nel = M.nel;
+    if (nlvdi) *nlvdi = M.nlvdi;
+    if (nlv) *nlv = M.nlv;
+    if (ntracer) *ntracer = M.ntr;
+    return 0;
+}
+
+int shyhip_get_retries(shyhip_model *h, long *retries) {
+    if (!h || !retries) return fail(nullptr, "null argument");
+    *retries = h->M.solver_retries;
+    return 0;
+}
+
+int shyhip_get_istot(shyhip_model *h, int *istot) {
+    if (!h || !istot) return fail(nullptr, "null argument");
+    *istot = h->M.last_istot;
+    return 0;
+}
+
+int shyhip_scalar_stability(shyhip_model *h, float rstol, int *istot, double *rindex) {
+    if (!h || !istot || !rindex) return fail(nullptr, "null argument");
+    Model &M = h->M;
+    if (pipe_drain(M)) return 1;
+    if (!(rstol > 0.f)) return fail(&M, "rstol must be positive");
+    const int nmem = M.nmem_dev > 1 ? M.nmem_dev : 1;
+    if (nmem > 1) M.cur_mem_istot = M.d_mem_istot + (size_t)(M.pipe_k & 1) * nmem;
+    std::vector<int> ist(nmem);
+    std::vector<double> ri(nmem);
+    if (launch_scalar_stability(M, rstol, ist.data(), ri.data())) return 1;
+    *istot = *std::max_element(ist.begin(), ist.end());
+    *rindex = *std::max_element(ri.begin(), ri.end());
+    return 0;
+}
+
 // scal3sh of tracer itr on stream st with conz3d scratch hbuf
 static int scal3sh_on(Model &M, int itr, int istot, int clip_small, hipStream_t st, void *hbuf) {
     if (itr < 0 || itr >= M.ntr) return fail(&M, "tracer index out of range");
@@ -1325,15 +1435,47 @@ static int scal3sh_on(Model &M, int itr, int istot, int clip_small, hipStream_t 
     for (int isact = 1; isact <= istot; isact++) {
         if (launch_conz3d(M, itr, istot, isact, clip_small, st, hbuf)) return 1;
         timer_begin(M, T_BNDO);
-        if (launch_bndo(M, itr, st)) return 1;       // bndo_setbc
+        if (launch_bndo(M, itr, st, isact)) return 1;       // bndo_setbc
         timer_end(M);
     }
     return 0;
 }
 
+int shyhip_conz3d(shyhip_model *h, int itr, const float *cbound, int istot, int isact) {
+    if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
+    Model &M = h->M;
+    if (itr < 0 || itr >= M.ntr) return fail(&M, "tracer index out of range");
+    if (istot < 1 || istot > 1000 || isact < 1 || isact > istot) return fail(&M, "conz3d: need 1 <= isact <= istot <= 1000");
+    if (M.p.itvd == 2 && !M.tvd_ready) return fail(&M, "itvd = 2 needs shyhip_tvd_init");
+    void *cb = nullptr;
+    if (cbound) {
+        if (!M.d_cbound && dalloc_s(M, &M.d_cbound, (size_t)M.nlvdi * M.nkn)) return 1;
+        std::vector<double> t((size_t)M.nlvdi * M.nkn);
+        for (int k = 0; k < M.nkn; k++)
+            for (int l = 0; l < M.nlvdi; l++) t[(size_t)l * M.nkn + k] = cbound[(size_t)k * M.nlvdi + l];
+        if (upload_s(M, M.d_cbound, t.data(), t.size())) return 1;
+        cb = M.d_cbound;
+    }
+    M.mem_istot_on = false;
+    if (launch_conz3d(M, itr, istot, isact, 0, M.stream, M.s.hbuf, cb)) return 1;
+    return check_dev_err(M, "conz3d");
+}
+
+int shyhip_bndo_setbc(shyhip_model *h, int itr) {
+    if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
+    Model &M = h->M;
+    if (itr < 0 || itr >= M.ntr) return fail(&M, "tracer index out of range");
+    M.mem_istot_on = false;
+    if (launch_bndo(M, itr, M.stream)) return 1;
+    return check_dev_err(M, "bndo_setbc");
+}
+
 int shyhip_scal3sh(shyhip_model *h, int itr, int istot, int clip_small) {
     if (!h) return fail(nullptr, "null handle");
     if (pipe_drain(h->M)) return 1;
+    if (resolve_istot(h->M, &istot)) return 1;
     return scal3sh_on(h->M, itr, istot, clip_small, h->M.stream, h->M.s.hbuf);
 }
 
@@ -1406,6 +1548,8 @@ int shyhip_step(shyhip_model *h, double dtime, int istot) {
     if ((M.f.nseries > 0 || M.f.wind_on) && launch_forcing(M, dtime)) return 1;
     if (hydro_impl(M, dtime, false)) return M.err.empty() ? 1 : 3;
     M.iweg_snap = nullptr;
+    // one istot for all tracers: conzstab does not depend on the tracer
+    if (M.ntr > 0 && resolve_istot(M, &istot)) return 1;
     if (pipe) {
         // every tracer on its own stream after this step's hydro; the model's
         // stream goes on with the next step
@@ -1526,6 +1670,13 @@ int shyhip_system_init(shyhip_model *h) {
     Model &M = h->M;
     M.h_coo.assign(M.nnz, 0.);
     M.h_rvec.assign(M.nkn, 0.);
+    M.h_raux.assign(M.nkn, 0.);
+    return 0;
+}
+
+int shyhip_system_set_explicit(shyhip_model *h) {
+    if (!h) return fail(nullptr, "null handle");
+    h->M.sys_explicit = true;
     return 0;
 }
 
@@ -1536,6 +1687,20 @@ int shyhip_system_assemble(shyhip_model *h, int ie, const int *kn, const float *
     if (ie < 1 || ie > M.nel) return fail(&M, "system_assemble: element out of range");
     if (M.h_coo.size() != (size_t)M.nnz) return fail(&M, "system_assemble before system_init");
     const int e = ie - 1;
+    if (M.sys_explicit) {
+        // the lumped system: diagonal and right-hand side only; an
+        // off-diagonal entry is the reference's error stop (:561-592)
+        for (int i = 0; i < 3; i++) {
+            const int k = kn[i] - 1;
+            if (k < 0 || k >= M.nkn) return fail(&M, "system_assemble: node out of range");
+            M.h_raux[k] = M.h_raux[k] + (double)hia[i + 3 * i];
+            M.h_rvec[k] = M.h_rvec[k] + (double)hik[i];
+            for (int j = 0; j < 3; j++)
+                if (i != j && hia[i + 3 * j] != 0.f)
+                    return fail(&M, "system_assemble: non diag elems /= 0 (element " + std::to_string(ie) + ")");
+        }
+        return 0;
+    }
     for (int i = 0; i < 3; i++) {
         const int k = kn[i] - 1;
         if (k != M.h_nen3v[3 * e + i]) return fail(&M, "system_assemble: kn does not match nen3v");
@@ -1562,6 +1727,12 @@ int shyhip_system_solve(shyhip_model *h, int n, const float *z) {
     if (pipe_drain(h->M)) return 1;
     Model &M = h->M;
     if (n != M.nkn || M.h_coo.size() != (size_t)M.nnz) return fail(&M, "system_solve: size mismatch or no assembly");
+    if (M.sys_explicit) {
+        // explicit: rvec2d / raux2d (simsys_spkaux.f90:475-479)
+        std::vector<double> x(n);
+        for (int k = 0; k < n; k++) x[k] = M.h_rvec[k] / M.h_raux[k];
+        return upload(M, M.s.x, x.data(), n);
+    }
     M.sv.fed = false;      // the system comes from the host: the solve prepares its own inputs
     if (upload(M, M.s.aval, M.h_coo.data(), M.nnz) || upload(M, M.s.rhs, M.h_rvec.data(), n)) return 1;
     std::vector<double> zz(z, z + n);

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstring>
 
 #include "shyhip_internal.h"
 
@@ -44,6 +45,13 @@ struct ConzArgs {
     const S *cbound; // boundary concentration (may be null)
     S *hbuf;
     int *err;
+    // ensembles whose members run different numbers of substeps: member
+    // istot (null: all run istot), node/element -> member, this substep and
+    // the step as (double)(S)dt; members already past their last substep
+    // are left alone
+    const int *mem_istot, *node_mem, *elem_mem;
+    int isact;
+    double dtp;
 };
 
 template <typename S>
@@ -140,6 +148,7 @@ __global__ __launch_bounds__(kThreads) void k_conz_elem(ConzArgs<S> a) {
             const int idx = base + j * kThreads + threadIdx.x;
             const int e = idx % kElemTile, l = idx / kElemTile + 1;
             on[j] = idx < kElemTile * lmt && e < ne && l <= T.ilev[e];
+            if (on[j] && a.mem_istot) on[j] = a.isact <= a.mem_istot[a.elem_mem[e0 + e]];
             ev[j] = e; lv[j] = l;
             tvd[j] = false;
             if (on[j]) {
@@ -222,7 +231,8 @@ __global__ __launch_bounds__(kThreads) void k_conz_elem(ConzArgs<S> a) {
                         dy = (R)(-2 * (tet1 == ic ? 1 : 0)) * dy + dy;
                         const R dis = (R)T.dis[iop - 1][e];
                         const R vel = (R)fabs((double)(u * dx + v * dy)) / dis;
-                        const R alfa = (R)((a.dt * (double)vel) / (double)dis);
+                        const double dte = a.mem_istot ? a.dtp / (double)a.mem_istot[a.elem_mem[ie]] : a.dt;
+                        const R alfa = (R)((dte * (double)vel) / (double)dis);
                         R conu = cond;
                         if (pup[j][k]) {
                             double zh = 0.;
@@ -294,8 +304,7 @@ __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
     __shared__ double cnL[NL][TN], cdL[NL][TN], cloL[NL][TN], chiL[NL][TN];
     const int nkn = a.nkn, nel = a.nel;
     const int k0 = blockIdx.x * TN;
-    const double dt = a.dt, ad = a.ad, aa = a.aa, adt = 1. - a.ad, aat = 1. - a.aa;
-    const double rso = a.rso, rsn = a.rsn, rsot = 1. - rso, rsnt = 1. - rsn;
+    const double ad = a.ad, aa = a.aa, adt = 1. - a.ad, aat = 1. - a.aa;
     const bool btvdv = a.itvdv > 0;
     const int nlv = min(a.nlv, NL);
 
@@ -305,6 +314,16 @@ __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
         if (k >= nkn) continue;
         const int lmaxk = a.ilhkv[k];
         if (l > lmaxk) continue;
+        double dt = a.dt, rso = a.rso, rsn = a.rsn;
+        if (a.mem_istot) {
+            const int ist = a.mem_istot[a.node_mem[k]];
+            if (a.isact > ist) continue;
+            const double rstot = ist;
+            rso = (a.isact - 1) / rstot;
+            rsn = a.isact / rstot;
+            dt = a.dtp / rstot;
+        }
+        const double rsot = 1. - rso, rsnt = 1. - rsn;
         IncSlots inc;     // incident elements in colour order
         inc.load(a.inc_cnt, a.inc_elc, nkn, k);
         // node column around l (zero outside 1..lmaxk)
@@ -407,6 +426,7 @@ __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
     // phase 2: the column solve of conz3d_nodes, one work-item per node
     const int kk = threadIdx.x, k = k0 + kk;
     if (kk >= TN || k >= nkn) return;
+    if (a.mem_istot && a.isact > a.mem_istot[a.node_mem[k]]) return;
     const int lmaxk = a.ilhkv[k];
     bool bad = false;
     if ((aa == 0. && ad == 0.) || a.nlv == 1) {
@@ -429,7 +449,7 @@ __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
 }
 
 template <typename S, int NL>
-static int conz_nl(Model &M, int itr, int istot, int isact, int clip, hipStream_t st, void *hbuf) {
+static int conz_nl(Model &M, int itr, int istot, int isact, int clip, hipStream_t st, void *hbuf, const void *cb) {
     ConzArgs<S> a{};
     a.nkn = M.nkn; a.nel = M.nel; a.nlvdi = M.nlvdi; a.nlv = M.nlv;
     a.itvd = M.p.itvd; a.itvdv = M.p.itvdv; a.clip = clip;
@@ -449,7 +469,12 @@ static int conz_nl(Model &M, int itr, int istot, int isact, int clip, hipStream_
     a.hdeo = (const S *)M.s.hdeo; a.wln = (const S *)M.s.wln; a.difv = (const S *)M.s.difv;
     a.mflux = M.has_mflux ? (const S *)M.s.mflux : nullptr;
     a.c = (S *)M.s.tracer[itr];
-    a.cbound = nullptr;
+    a.cbound = (const S *)cb;
+    a.isact = isact;
+    a.dtp = (double)(S)M.p.dt;
+    if (M.mem_istot_on) {
+        a.mem_istot = M.cur_mem_istot; a.node_mem = M.d_node_mem; a.elem_mem = M.d_elem_mem;
+    }
     a.hbuf = (S *)hbuf;
     a.err = M.d_err;
     const int ge = (M.nel + kElemTile - 1) / kElemTile;
@@ -462,17 +487,17 @@ static int conz_nl(Model &M, int itr, int istot, int isact, int clip, hipStream_
 }
 
 template <typename S>
-static int conz_t(Model &M, int itr, int istot, int isact, int clip, hipStream_t st, void *hbuf) {
+static int conz_t(Model &M, int itr, int istot, int isact, int clip, hipStream_t st, void *hbuf, const void *cb) {
     switch (M.nl_tmpl) {
-    case 1: return conz_nl<S, 1>(M, itr, istot, isact, clip, st, hbuf);
-    case 2: return conz_nl<S, 2>(M, itr, istot, isact, clip, st, hbuf);
-    case 4: return conz_nl<S, 4>(M, itr, istot, isact, clip, st, hbuf);
-    case 8: return conz_nl<S, 8>(M, itr, istot, isact, clip, st, hbuf);
-    case 12: return conz_nl<S, 12>(M, itr, istot, isact, clip, st, hbuf);
-    case 16: return conz_nl<S, 16>(M, itr, istot, isact, clip, st, hbuf);
-    case 20: return conz_nl<S, 20>(M, itr, istot, isact, clip, st, hbuf);
-    case 24: return conz_nl<S, 24>(M, itr, istot, isact, clip, st, hbuf);
-    default: return conz_nl<S, 32>(M, itr, istot, isact, clip, st, hbuf);
+    case 1: return conz_nl<S, 1>(M, itr, istot, isact, clip, st, hbuf, cb);
+    case 2: return conz_nl<S, 2>(M, itr, istot, isact, clip, st, hbuf, cb);
+    case 4: return conz_nl<S, 4>(M, itr, istot, isact, clip, st, hbuf, cb);
+    case 8: return conz_nl<S, 8>(M, itr, istot, isact, clip, st, hbuf, cb);
+    case 12: return conz_nl<S, 12>(M, itr, istot, isact, clip, st, hbuf, cb);
+    case 16: return conz_nl<S, 16>(M, itr, istot, isact, clip, st, hbuf, cb);
+    case 20: return conz_nl<S, 20>(M, itr, istot, isact, clip, st, hbuf, cb);
+    case 24: return conz_nl<S, 24>(M, itr, istot, isact, clip, st, hbuf, cb);
+    default: return conz_nl<S, 32>(M, itr, istot, isact, clip, st, hbuf, cb);
     }
 }
 
@@ -491,7 +516,8 @@ __global__ __launch_bounds__(kBndoLanes) void k_bndo(int nbndo, const int *bk, c
                                                     const int *ibtyp, const int *ilhkv, const int *ilhv,
                                                     const int *iweg, const int *nk_ptr, const int *nk_el,
                                                     const double *geo, const S *uln, const S *vln, S *cv,
-                                                    const S *rbc, int nkn, int nel, int *err) {
+                                                    const S *rbc, int nkn, int nel, int *err,
+                                                    const int *mem_istot, const int *node_mem, int isact) {
     using R = S;
     __shared__ R val[kBndoLanes];
     __shared__ int status[kBndoLanes];
@@ -499,6 +525,7 @@ __global__ __launch_bounds__(kBndoLanes) void k_bndo(int nbndo, const int *bk, c
     if (i >= nbndo) return;
     const int k = bk[i];
     if (ibtyp[ibnd[k]] != 1) return;
+    if (mem_istot && isact > mem_istot[node_mem[k]]) return;
     const int lane = threadIdx.x, l = lane + 1;
     const int lmax = ilhkv[k];
     const R flag = (R)-999.;
@@ -553,7 +580,7 @@ __global__ __launch_bounds__(kBndoLanes) void k_bndo(int nbndo, const int *bk, c
 }
 
 template <typename S>
-static int bndo_t(Model &M, int itr, hipStream_t st) {
+static int bndo_t(Model &M, int itr, hipStream_t st, int isact) {
     if (M.m.nbndo == 0) return 0;
     if (M.nlv > kBndoLanes) return hip_check(M, hipErrorInvalidValue, "bndo: more layers than lanes");
     hipLaunchKernelGGL(k_bndo<S>, dim3(M.m.nbndo), dim3(kBndoLanes), 0, st, M.m.nbndo, M.m.bndo_k,
@@ -561,18 +588,145 @@ static int bndo_t(Model &M, int itr, hipStream_t st) {
                        (const S *)M.m.bndo_w, M.m.ibnd, M.m.ibtyp, M.m.ilhkv, M.m.ilhv,
                        M.iweg_conz ? M.iweg_conz : M.s.iweg, M.m.nk_ptr,
                        M.m.nk_el, M.m.geo, (const S *)M.s.uln, (const S *)M.s.vln, (S *)M.s.tracer[itr],
-                       (const S *)M.s.tracer_bc[itr], M.nkn, M.nel, M.d_err);
+                       (const S *)M.s.tracer_bc[itr], M.nkn, M.nel, M.d_err,
+                       M.mem_istot_on ? M.cur_mem_istot : nullptr, M.d_node_mem, isact);
     HIPCHK(M, hipGetLastError());
     return 0;
 }
 
-int launch_bndo(Model &M, int itr, hipStream_t st) {
-    return M.prec == SHYHIP_FP64 ? bndo_t<double>(M, itr, st) : bndo_t<float>(M, itr, st);
+int launch_bndo(Model &M, int itr, hipStream_t st, int isact) {
+    return M.prec == SHYHIP_FP64 ? bndo_t<double>(M, itr, st, isact) : bndo_t<float>(M, itr, st, isact);
 }
 
-int launch_conz3d(Model &M, int itr, int istot, int isact, int clip_small, hipStream_t st, void *hbuf) {
-    return M.prec == SHYHIP_FP64 ? conz_t<double>(M, itr, istot, isact, clip_small, st, hbuf)
-                                 : conz_t<float>(M, itr, istot, isact, clip_small, st, hbuf);
+// ------------------------------------------------------------------ stability
+// scalar_stability (stability.f90:220-260): conzstab (concentration.f90:
+// 1573-2097) for one second (one substep) with difmol = 0, no nudging and no
+// sinking, then rindex = dt * (index / rstol) and istot = 1 + rindex.  One
+// work-item per node-layer gathers its incident elements in element order --
+// the order conzstab accumulates them -- with each element's advective term
+// for this vertex (the larger of the last out- and in-flux seen over the
+// element's vertices up to this one, as conzstab carries them), its explicit
+// vertical diffusion and its share of the node volume; point sources add
+// their inflow.  The node-layer ratios are non-negative, so their IEEE bit
+// patterns order as unsigned integers and one atomic max finds the largest.
+// Horizontal diffusion (rkpar) is zero on the device path and adds nothing.
+template <typename S>
+__global__ void __launch_bounds__(kThreads) k_conz_stab(int nkn, int nel, int nlv, const int *ilhkv, const int *ilhv,
+                                                        const int *zbound, const int *nk_ptr, const int *nk_el,
+                                                        const int *nen3v, const double *geo, double az, double ad,
+                                                        double aa, const S *utn, const S *uto, const S *vtn,
+                                                        const S *vto, const S *wln, const S *difv, const S *hdeo,
+                                                        const S *hdkn, const S *hdko, const S *mflux,
+                                                        const int *node_mem, unsigned long long *dmax) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nlv * nkn) return;
+    const int l = i / nkn + 1, k = i - (l - 1) * nkn;
+    if (l > ilhkv[k] || zbound[k]) return;
+    const double azt = 1. - az, adt = 1. - ad, aat = 1. - aa;
+    double chigh = 0., cn = 0., cdiag = 0.;
+    double co = 0.;
+    if (mflux) {
+        const S q = mflux[(size_t)(l - 1) * nkn + k];
+        if (q > (S)0) co = (double)q;
+    }
+    for (int p = nk_ptr[k]; p < nk_ptr[k + 1]; p++) {
+        const int ie = nk_el[p] >> 2, iv = nk_el[p] & 3;
+        const int ilevel = ilhv[ie];
+        if (l > ilevel) continue;
+        const double aj4 = 4. * geo[(size_t)6 * nel + ie];
+        const size_t le = (size_t)(l - 1) * nel + ie;
+        const double us = az * (double)utn[le] + azt * (double)uto[le];
+        const double vs = az * (double)vtn[le] + azt * (double)vto[le];
+        double coadv = 0., ciadv = 0., cadv = 0.;
+        for (int jj = 0; jj <= iv; jj++) {
+            const int kj = nen3v[3 * ie + jj];
+            const double f = us * geo[(size_t)jj * nel + ie] + vs * geo[(size_t)(3 + jj) * nel + ie];
+            double fwin = 0., fwout = 0.;
+            const double wt = l > 1 ? (double)wln[(size_t)(l - 1) * nkn + kj] : 0.;
+            const double wb = l < ilevel ? (double)wln[(size_t)l * nkn + kj] : 0.;
+            if (wt > 0.) fwout = fwout + aat * wt;
+            else fwin = fwin - aat * wt;
+            if (wb > 0.) fwin = fwin + aat * wb;
+            else fwout = fwout - aat * wb;
+            const double chadv = aj4 * 3. * f;
+            const double cviadv = aj4 * fwin, cvoadv = aj4 * fwout;
+            if (chadv < 0.) coadv = cvoadv - chadv;
+            else ciadv = cviadv + chadv;
+            cadv = coadv > ciadv ? coadv : ciadv;
+        }
+        chigh = chigh + cadv;
+        const double hdv = (double)hdeo[le];
+        double hmtop = 0., hmbot = 0.;
+        if (l > 1)
+            hmtop = 2. * (double)difv[(size_t)(l - 1) * nkn + k] / ((double)hdeo[le - nel] + hdv);
+        if (l < ilevel)
+            hmbot = 2. * (double)difv[(size_t)l * nkn + k] / (hdv + (double)hdeo[le + nel]);
+        cn = cn + aj4 * (adt * (hmtop + hmbot));
+        const double hn = (double)hdkn[(size_t)(l - 1) * nkn + k], ho = (double)hdko[(size_t)(l - 1) * nkn + k];
+        cdiag = cdiag + aj4 * (ho < hn ? ho : hn);
+    }
+    if (cdiag > 0.) {
+        const double r = ((chigh + 0.) + cn + co) / cdiag;
+        if (r > 0.) atomicMax(dmax + (node_mem ? node_mem[k] : 0), (unsigned long long)__double_as_longlong(r));
+    }
+}
+
+// the index of each member rounded through a default real (the reference's
+// shympi_max of a real), rindex = dt * index / rstol in the model's real,
+// istot = 1 + rindex truncated.  w holds nmem maxima, then nmem istot and
+// nmem rindex bits (as double); istot also goes to mem_istot (may be null);
+// the maxima are cleared for the next call.  One work-item per member.
+template <typename S>
+__global__ void k_conz_stab_final(unsigned long long *w, int nmem, S dt, float rstol, int *mem_istot) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmem) return;
+    double stabind = __longlong_as_double((long long)w[m]);
+    stabind = (double)(float)stabind;
+    const S sindex = (S)(stabind / (double)rstol);
+    const S ri = dt * sindex;
+    const int ist = (int)((S)1 + ri);
+    w[nmem + m] = (unsigned long long)(long long)ist;
+    w[2 * nmem + m] = (unsigned long long)__double_as_longlong((double)ri);
+    if (mem_istot) mem_istot[m] = ist;
+    w[m] = 0ull;
+}
+
+// per member (nmem = 1: the model): istot[m] and rindex[m] on the host
+template <typename S>
+static int stab_t(Model &M, float rstol, int *istot, double *rindex) {
+    const int nmem = M.nmem_dev > 1 ? M.nmem_dev : 1;
+    unsigned long long *w = nmem > 1 ? M.d_mem_stab : (unsigned long long *)(M.d_err + 8);
+    const int n = M.nlv * M.nkn;
+    hipLaunchKernelGGL(k_conz_stab<S>, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, M.stream, M.nkn,
+                       M.nel, M.nlv, M.m.ilhkv, M.m.ilhv, M.m.zbound, M.m.nk_ptr, M.m.nk_el, M.m.nen3v, M.m.geo,
+                       (double)M.p.azpar, (double)M.p.adpar, (double)M.p.aapar, (const S *)M.s.utn,
+                       (const S *)M.s.uto, (const S *)M.s.vtn, (const S *)M.s.vto, (const S *)M.s.wln,
+                       (const S *)M.s.difv, (const S *)M.s.hdeo, (const S *)M.s.hdkn, (const S *)M.s.hdko,
+                       M.has_mflux ? (const S *)M.s.mflux : (const S *)nullptr,
+                       nmem > 1 ? M.d_node_mem : nullptr, w);
+    HIPCHK(M, hipGetLastError());
+    hipLaunchKernelGGL(k_conz_stab_final<S>, dim3((nmem + 63) / 64), dim3(64), 0, M.stream, w, nmem, (S)M.p.dt,
+                       rstol, nmem > 1 ? M.cur_mem_istot : nullptr);
+    HIPCHK(M, hipGetLastError());
+    unsigned long long *hbuf = nmem > 1 ? M.h_mem_stab : (unsigned long long *)(M.h_pinned + 8);
+    HIPCHK(M, hipMemcpyAsync(hbuf, w + nmem, 2 * nmem * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             M.stream));
+    HIPCHK(M, hipStreamSynchronize(M.stream));
+    for (int m = 0; m < nmem; m++) {
+        istot[m] = (int)(long long)hbuf[m];
+        std::memcpy(&rindex[m], &hbuf[nmem + m], sizeof(double));
+    }
+    return 0;
+}
+
+int launch_scalar_stability(Model &M, float rstol, int *istot, double *rindex) {
+    return M.prec == SHYHIP_FP64 ? stab_t<double>(M, rstol, istot, rindex) : stab_t<float>(M, rstol, istot, rindex);
+}
+
+int launch_conz3d(Model &M, int itr, int istot, int isact, int clip_small, hipStream_t st, void *hbuf,
+                  const void *cbound) {
+    return M.prec == SHYHIP_FP64 ? conz_t<double>(M, itr, istot, isact, clip_small, st, hbuf, cbound)
+                                 : conz_t<float>(M, itr, istot, isact, clip_small, st, hbuf, cbound);
 }
 
 }  // namespace shy

@@ -10,6 +10,9 @@
 // disables FP contraction, so per-kernel results reproduce the CPU path.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <string>
+
 #include <cmath>
 
 #include "shyhip_internal.h"
@@ -1268,14 +1271,15 @@ __global__ __launch_bounds__(kThreads) void k_rhoset(HydroArgs<S> a, const S *te
 // FP64 total volume (masscont, elems_dealing.f90:1191-1220): per-block sums
 // in a fixed tree order, then a single-block pass over the partials.
 template <typename S>
-__global__ __launch_bounds__(kThreads) void k_volume_part(HydroArgs<S> a, const S *hdk, const S *conc, double *part) {
+__global__ __launch_bounds__(kThreads) void k_volume_part(int nkn, const int *ilhkv, const S *areakv, const S *hdk,
+                                                          const S *conc, double *part) {
     __shared__ double red[kThreads];
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     double t = 0.;
-    if (k < a.nkn) {
-        for (int l = 1; l <= a.ilhkv[k]; l++) {
-            const size_t o = (size_t)(l - 1) * a.nkn + k;
-            const S vol = hdk[o] * a.areakv[o];
+    if (k < nkn) {
+        for (int l = 1; l <= ilhkv[k]; l++) {
+            const size_t o = (size_t)(l - 1) * nkn + k;
+            const S vol = hdk[o] * areakv[o];
             t += conc ? (double)conc[o] * (double)vol : (double)vol;
         }
     }
@@ -1588,7 +1592,7 @@ static int volume_t(Model &M, const void *hdk, const void *conc, double *out) {
     auto a = make_args<S>(M);
     const int nb = nblocks(M.nkn);
     double *part = M.sv.part;   // large enough: >= nblocks(nkn)
-    LAUNCH(M, k_volume_part<S>, nb, a, (const S *)hdk, (const S *)conc, part);
+    LAUNCH(M, k_volume_part<S>, nb, a.nkn, a.ilhkv, a.areakv, (const S *)hdk, (const S *)conc, part);
     hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kThreads), 0, M.stream, part, nb, M.sv.scal + 15);
     HIPCHK(M, hipGetLastError());
     HIPCHK(M, hipMemcpyAsync(out, M.sv.scal + 15, sizeof(double), hipMemcpyDeviceToHost, M.stream));

@@ -194,14 +194,26 @@ struct Model {
     State s;
     Solver sv;
     int nnz = 0, ncolor = 0, nbnd = 0;
-    int *d_err = nullptr;     // [8] device error words + counters
+    int *d_err = nullptr;     // [16] device error words + counters; [8..13] the
+                              // scalar-stability maximum and result (64-bit words)
     int *h_pinned = nullptr;  // pinned host mirror for small readbacks
     double dtime = 0., dtime_old = -1.;
     bool has_coriolis = false, has_mflux = false, tvd_ready = false;
     // baroclinic coupling (ibarcl = 1): tracers holding T and S
     bool bcl_on = false;
     int bcl_temp = -1, bcl_salt = -1;
-    int last_nloop = 0, last_iters = 0, last_refine = 0;
+    int last_nloop = 0, last_iters = 0, last_refine = 0, last_istot = 1;
+    long solver_retries = 0;   // zeta solves redone with exact factors after a warm start failed
+    // members with their own scal3sh substeps (ensembles, set at set_basin
+    // for more than one member): node/element -> member maps, each member's
+    // istot of the current step, the stability words (3 per member) and
+    // their pinned host mirror; mem_istot_on while a step's transport uses them
+    int nmem_dev = 0;
+    int *d_node_mem = nullptr, *d_elem_mem = nullptr, *d_mem_istot = nullptr;   // mem_istot: 2 slots of nmem
+    int *cur_mem_istot = nullptr;    // the slot of the current step (pipelined steps alternate)
+    unsigned long long *d_mem_stab = nullptr, *h_mem_stab = nullptr;
+    std::vector<int> mem_istot;
+    bool mem_istot_on = false;
     // independent basins held in one model: member m owns nodes
     // [mem_node[m], mem_node[m+1]) and elements [mem_elem[m], mem_elem[m+1])
     std::vector<int> mem_node, mem_elem;
@@ -219,6 +231,9 @@ struct Model {
     std::vector<float> h_x, h_y;
     std::vector<double> h_ev;             // [19*nel] reference ev layout
     std::vector<double> h_coo, h_rvec;    // plugin accumulation (c2coo, rvec2d)
+    std::vector<double> h_raux;           // explicit system: the diagonal (raux2d)
+    bool sys_explicit = false;            // system_set_explicit was called
+    void *d_cbound = nullptr;             // S [nlvdi*nkn] source concentrations of shyhip_conz3d
     size_t esize() const { return prec == SHYHIP_FP64 ? 8 : 4; }
     bool deferred = false;    // shyhip_step does not wait for the device at its end
     // Pipelined transport (deferred checks, no timing, no mass-flux field):
@@ -351,8 +366,13 @@ int solver_setup(Model &M);
 // read-back returns the solver state, d_err[0] and the changed-element
 // count (in h_pinned[0..1]).
 int solver_solve(Model &M, int *iters, int *refinements, int wetdry_bnew = -1);
-int launch_conz3d(Model &M, int itr, int istot, int isact, int clip_small, hipStream_t st, void *hbuf);
-int launch_bndo(Model &M, int itr, hipStream_t st);
+int launch_conz3d(Model &M, int itr, int istot, int isact, int clip_small, hipStream_t st, void *hbuf,
+                  const void *cbound = nullptr);
+int launch_bndo(Model &M, int itr, hipStream_t st, int isact = 1);
+// scalar_stability on the current state: istot and rindex of each member
+// (one entry per member when the model holds several, else one)
+// (synchronises)
+int launch_scalar_stability(Model &M, float rstol, int *istot, double *rindex);
 
 // helpers
 int hip_check(Model &M, hipError_t e, const char *what);

@@ -1276,6 +1276,7 @@ int solver_solve(Model &M, int *iters, int *refinements, int wetdry_bnew) {
     if (++M.sv.solves_since_exact >= kWarmRefresh) M.sv.have_lu = false;
     const int rc = solve_once(M, iters, refinements, wetdry_bnew, fed);
     if (rc != 2 || !M.sv.last_warm) return rc;
+    M.solver_retries++;
     M.sv.have_lu = false;
     hipLaunchKernelGGL(k_ctrl_restart, dim3(1), dim3(64), 0, M.stream, M.sv.ctrl);
     HIPCHK(M, hipGetLastError());

@@ -1,0 +1,73 @@
+! conz3d_shyhip.f90 -- conz3d_omp on the GPU, with the reference's name and
+! argument list (src/shyfem/concentration_omp.f90:65-151), so that scal3sh
+! (concentration.f90:748-760) calls it unchanged for each substep isact of
+! istot.  The hydrodynamic state it transports with is the one the device
+! model holds (after shyhip_hydro or a device step).  One call: push cn1 and
+! difv into the model, run the element and node phases of substep isact on
+! the device, pull cn1 back.  The model's tracer slot shy_conz_slot
+! (shyhip_shyfem) holds the concentration meanwhile.
+!
+! Supported as on the device path: horizontal diffusion (rkpar), nudging
+! (robs), settling (wsink) and loads (rload) zero, itvd 0 or 2.  Any other
+! value ends the run with an error stop, as the reference stops on options
+! it cannot run.  azpar, adpar, aapar, difmol, itvd and itvdv are taken
+! over into the model's parameters; ddt must be the model's time step.
+! The reference runs the T and S calls in concurrent OpenMP tasks
+! (baroclinic.f90:466-500); here they take turns on the one model.
+
+subroutine conz3d_omp(cn1, co1, ddt, rkpar, difhv, difv, difmol, cbound, itvd, itvdv, gradxv, gradyv, &
+                      cobs, robs, rtauv, wsink, wsinkv, rload, load, azpar, adpar, aapar, istot, isact, &
+                      nlvddi, nlev)
+  use iso_c_binding
+  use shyhip_capi
+  use shyhip_shyfem
+  implicit none
+  integer, intent(in) :: nlvddi, nlev, itvd, itvdv, istot, isact
+  real, intent(in) :: difmol, robs, wsink, rload, ddt, rkpar
+  real, intent(in) :: azpar, adpar, aapar
+  real, intent(inout), target :: cn1(nlvddi,*)
+  real, intent(in) :: co1(nlvddi,*)
+  real, intent(in), target :: cbound(nlvddi,*)
+  real, intent(in) :: difhv(nlvddi,*)
+  real, intent(in) :: gradxv(nlvddi,*), gradyv(nlvddi,*)
+  real, intent(in) :: cobs(nlvddi,*), rtauv(nlvddi,*)
+  real, intent(inout) :: load(nlvddi,*)
+  real, intent(in), target :: difv(0:nlvddi,*)
+  real, intent(in) :: wsinkv(0:nlvddi,*)
+
+  type(shyhip_params) :: p
+  integer(c_int) :: nkn, nel, nlvdi, nlv, ntr
+  logical :: change
+
+  if (rkpar /= 0.) stop 'error stop conz3d_omp: horizontal diffusion (rkpar /= 0) is not on the device'
+  if (robs /= 0.) stop 'error stop conz3d_omp: nudging (robs /= 0) is not on the device'
+  if (wsink /= 0.) stop 'error stop conz3d_omp: settling (wsink /= 0) is not on the device'
+  if (rload /= 0.) stop 'error stop conz3d_omp: loads (rload /= 0) are not on the device'
+  if (itvd == 1) stop 'error stop conz3d_omp: itvd = 1 is not on the device'
+
+!$omp critical (shyhip_conz3d)
+  call shyhip_check(shy, shyhip_get_sizes(shy, nkn, nel, nlvdi, nlv, ntr), 'conz3d_omp')
+  if (nlvddi /= nlvdi .or. nlev /= nlv) stop 'error stop conz3d_omp: layers differ from the device model'
+  if (shy_conz_slot < 0 .or. shy_conz_slot >= ntr) stop 'error stop conz3d_omp: no tracer slot in the device model'
+  call shyhip_check(shy, shyhip_get_params(shy, p), 'conz3d_omp')
+  if (real(p%dt) /= ddt) stop 'error stop conz3d_omp: ddt differs from the time step of the device model'
+  change = p%azpar /= azpar .or. p%adpar /= adpar .or. p%aapar /= aapar .or. p%difmol /= difmol &
+           .or. p%itvd /= itvd .or. p%itvdv /= itvdv
+  if (change) then
+    p%azpar = azpar
+    p%adpar = adpar
+    p%aapar = aapar
+    p%difmol = difmol
+    p%itvd = itvd
+    p%itvdv = itvdv
+    call shyhip_check(shy, shyhip_set_params(shy, p), 'conz3d_omp')
+  end if
+  call shyhip_check(shy, shyhip_push(shy, SHYHIP_TRACER0 + shy_conz_slot, c_loc(cn1), SHYHIP_F32, &
+                                     int(nlvddi, c_size_t) * nkn), 'conz3d_omp')
+  call shyhip_check(shy, shyhip_push(shy, SHYHIP_DIFV, c_loc(difv), SHYHIP_F32, &
+                                     int(nlvddi + 1, c_size_t) * nkn), 'conz3d_omp')
+  call shyhip_check(shy, shyhip_conz3d(shy, shy_conz_slot, c_loc(cbound), istot, isact), 'conz3d_omp')
+  call shyhip_check(shy, shyhip_pull(shy, SHYHIP_TRACER0 + shy_conz_slot, c_loc(cn1), SHYHIP_F32, &
+                                     int(nlvddi, c_size_t) * nkn), 'conz3d_omp')
+!$omp end critical (shyhip_conz3d)
+end subroutine conz3d_omp

@@ -290,6 +290,51 @@ module shyhip_capi
       integer(c_int) :: rc
     end function
 
+    function shyhip_system_set_explicit(h) bind(C, name="shyhip_system_set_explicit") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      integer(c_int) :: rc
+    end function
+
+    function shyhip_conz3d(h, itr, cbound, istot, isact) bind(C, name="shyhip_conz3d") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      integer(c_int), value :: itr
+      type(c_ptr), value :: cbound
+      integer(c_int), value :: istot, isact
+      integer(c_int) :: rc
+    end function
+
+    function shyhip_bndo_setbc(h, itr) bind(C, name="shyhip_bndo_setbc") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      integer(c_int), value :: itr
+      integer(c_int) :: rc
+    end function
+
+    function shyhip_get_sizes(h, nkn, nel, nlvdi, nlv, ntracer) bind(C, name="shyhip_get_sizes") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      integer(c_int), intent(out) :: nkn, nel, nlvdi, nlv, ntracer
+      integer(c_int) :: rc
+    end function
+
+    function shyhip_get_istot(h, istot) bind(C, name="shyhip_get_istot") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      integer(c_int), intent(out) :: istot
+      integer(c_int) :: rc
+    end function
+
+    function shyhip_scalar_stability(h, rstol, istot, rindex) bind(C, name="shyhip_scalar_stability") result(rc)
+      import :: c_ptr, c_int, c_float, c_double
+      type(c_ptr), value :: h
+      real(c_float), value :: rstol
+      integer(c_int), intent(out) :: istot
+      real(c_double), intent(out) :: rindex
+      integer(c_int) :: rc
+    end function
+
     function shyhip_system_assemble(h, ie, kn, hia, hik) bind(C, name="shyhip_system_assemble") result(rc)
       import :: c_ptr, c_int, c_float
       type(c_ptr), value :: h

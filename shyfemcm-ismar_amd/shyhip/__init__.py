@@ -317,7 +317,9 @@ class Model:
         self._chk(self.lib.shyhip_forcing(self.h, C.c_double(dtime)), "forcing")
 
     def step(self, bnd_level=None, istot=1):
-        """set_dry, set_timestep, copy_uvz/copy_depth, sp111 level, hydro, scal3sh."""
+        """set_dry, set_timestep, copy_uvz/copy_depth, sp111 level, hydro,
+        scal3sh (istot <= 0: substeps from scalar_stability, as the
+        reference chooses them)."""
         if bnd_level is not None:
             self.set_boundary(bnd_level)
         self.t += self.params_dict["dt"]
@@ -331,10 +333,34 @@ class Model:
         step's final phase is raised by the next call that waits."""
         self._chk(self.lib.shyhip_set_deferred_checks(self.h, 1 if on else 0), "set_deferred_checks")
 
+    def last_istot(self):
+        """Substeps the last step's scal3sh ran (a host-side value)."""
+        d = C.c_int()
+        self._chk(self.lib.shyhip_get_istot(self.h, C.byref(d)), "get_istot")
+        return d.value
+
+    def member_istot(self, nmem):
+        """Substeps each ensemble member ran in the last step."""
+        out = np.zeros(nmem, np.int32)
+        self._chk(self.lib.shyhip_member_istot(self.h, nmem, _p(out)), "member_istot")
+        return out
+
+    def scalar_stability(self, rstol=1.0):
+        """scalar_stability on the current state: (istot, rindex)."""
+        n = C.c_int()
+        r = C.c_double()
+        self._chk(self.lib.shyhip_scalar_stability(self.h, C.c_float(rstol), C.byref(n), C.byref(r)),
+                  "scalar_stability")
+        return n.value, r.value
+
     def stats(self):
-        a, b_, c = C.c_int(), C.c_int(), C.c_int()
+        a, b_, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         self._chk(self.lib.shyhip_stats(self.h, C.byref(a), C.byref(b_), C.byref(c)), "stats")
-        return dict(nloop=a.value, solver_iters=b_.value, refinements=c.value)
+        self._chk(self.lib.shyhip_get_istot(self.h, C.byref(d)), "get_istot")
+        r = C.c_long()
+        self._chk(self.lib.shyhip_get_retries(self.h, C.byref(r)), "get_retries")
+        return dict(nloop=a.value, solver_iters=b_.value, refinements=c.value, istot=d.value,
+                    solver_retries=r.value)
 
     def set_timing(self, on=True):
         self._chk(self.lib.shyhip_set_timing(self.h, 1 if on else 0), "set_timing")

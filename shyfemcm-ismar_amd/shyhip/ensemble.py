@@ -128,6 +128,8 @@ class Ensemble:
         self.level_series = True
 
     def step(self, istot=1):
+        """One step of every member; istot <= 0: each member's transport
+        takes the substeps scalar_stability gives it alone."""
         dt = self.model.params_dict["dt"]
         if self.nbnd and not self.level_series:
             self.model.set_boundary_levels(self.levels(self.t + dt))
@@ -143,6 +145,9 @@ class Ensemble:
         if a.shape[0] == self.u.basin.nel:
             return a[self.u.elem_start[m]:self.u.elem_start[m + 1]]
         raise ValueError(f"{name}: not a node or element field ({a.shape}, member nkn {b.nkn})")
+
+    def member_istot(self):
+        return self.model.member_istot(len(self.members))
 
     def sync(self):
         self.model.sync()

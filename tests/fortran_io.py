@@ -1,4 +1,5 @@
-"""Case and result files of the Fortran host program fortran/shyhip_run
+"""Case and result files of the Fortran host program
+shyfemcm-ismar_amd/fortran/shyhip_run, for tests/test_gpu_fortran_host.py
 (stream records in native byte order, layouts as in the reference: Fortran
 column-major, 1-based connectivity)."""
 from __future__ import annotations
@@ -7,7 +8,9 @@ import os
 
 import numpy as np
 
-RUN_EXE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "fortran", "shyhip_run")
+FORTRAN_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "shyfemcm-ismar_amd", "fortran")
+RUN_EXE = os.path.join(FORTRAN_DIR, "shyhip_run")
+DROPIN_EXE = os.path.join(FORTRAN_DIR, "shyhip_dropin_check")
 
 
 def write_case(path, basin, precision: str, dt: float, levels, ireib=5, itvd=0, itvdv=0, aapar=0.0,

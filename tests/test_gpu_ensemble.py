@@ -58,3 +58,38 @@ def test_ensemble_members_match_alone(gpu, prec, ztol):
         assert dz < ztol                                   # m: solver tolerance of the union solve
         assert ds < 1e-5 * 38.0                            # relative to the field scale
     ens.close()
+
+
+def test_ensemble_members_choose_own_substeps(gpu):
+    """With istot from scalar_stability, each member runs the substeps it
+    chooses alone (the kernels leave a member alone past its last substep),
+    so the members still match their stand-alone runs; the members here
+    choose different counts."""
+    mems = _members()
+    nsteps = 8
+    ens = ensemble.Ensemble(mems, "fp64", params=PARAMS, visv=1e-3, difv=1e-4)
+    seen = []
+    for _ in range(nsteps):
+        ens.step(istot=0)
+        seen.append(tuple(ens.member_istot()))
+    print("member istot per step:", seen)
+    assert any(len(set(s)) > 1 for s in seen)
+    for i, mem in enumerate(mems):
+        m = gpu.Model(mem["basin"], "fp64", params=PARAMS, visv=1e-3, difv=1e-4, tracers=mem["tracers"])
+        m.set_wind(*mem["wind"])
+        t = 0.0
+        alone = []
+        for _ in range(nsteps):
+            t += PARAMS["dt"]
+            m.step(mem["level"](t), istot=0)
+            alone.append(m.last_istot())
+        assert alone == [s[i] for s in seen]
+        assert np.array_equal(ens.pull("iwegv", i), m.pull("iwegv"))
+        b = mem["basin"]
+        valid = np.arange(b.nlv)[None, :] < b.ilhkv[:, None]
+        dz = float(np.max(np.abs(ens.pull("znv", i) - m.pull("znv"))))
+        ds = float(np.max(np.abs(ens.pull("salt", i) - m.pull("salt"))[valid]))
+        print("member", i, "|dz| =", dz, "|dsalt| =", ds)
+        assert dz < 1e-6 and ds < 1e-5 * 38.0
+        m.close()
+    ens.close()

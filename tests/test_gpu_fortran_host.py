@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from shyhip import mesh
-from shyhip import fortran_io
+import fortran_io
 
 pytestmark = pytest.mark.gpu
 

@@ -403,3 +403,23 @@ def test_reference_precision_tracks_fp64():
         o32.step(lev)
         o64.step(lev)
     assert np.abs(o32.a["znv"] - o64.a["znv"]).max() < 2e-5
+
+
+def test_threaded_oracle_bit_identical():
+    """The oracle's parallel loops (element columns; conz3d's elements colour
+    by colour and its nodes) give the same bits on any number of threads."""
+    b = mesh.lagoon(1500, seed=7, lx=6000.0, ly=3000.0, nlv=4, tile=16)
+    salt = mesh.smooth_tracer(b, 8, 25.0, 38.0)
+    params = dict(dt=60.0, ireib=5, itpre=1, itvd=2, itvdv=1, aapar=0.0)
+    out = []
+    for nt in (1, 4):
+        lib = pyoracle.load("f64")
+        lib.oc_set_threads(nt)
+        om = pyoracle.OracleModel(b, "f64", params=params, visv=1e-3, difv=1e-4, tracers=dict(salt=salt))
+        om.set_wind(*mesh.wind_stress(10.0, 0.3))
+        for n in range(4):
+            om.step(mesh.tide_level((n + 1) * 60.0) - 0.2, istot=0)
+        out.append((om.a["znv"].copy(), om.a["utlnv"].copy(), om.tracers["salt"].copy()))
+    lib.oc_set_threads(1)
+    for a, c in zip(out[0], out[1]):
+        assert np.array_equal(a, c)
