@@ -320,6 +320,20 @@ int  shyhip_set_deferred_checks(shyhip_model *h, int on);
 int  shyhip_set_timing(shyhip_model *h, int on);
 int  shyhip_get_timing(shyhip_model *h, int id, const char **name, double *total_ms, long *count);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the FP64 volume (conc = NULL) or tracer mass of node 3D
+ * device arrays in the library's layout a[l*nkn + k], element type dtype
+ * (SHYHIP_F32 or SHYHIP_F64), summed exactly as shyhip_volume and
+ * shyhip_tracer_mass sum them (same kernels and order, so the same bits),
+ * into the device double *out, asynchronously on stream (a hipStream_t,
+ * NULL = the default stream).  ilhkv: device int[nkn].  Allocates nothing:
+ * the partial sums live in one static 512 KiB device array per GPU, so nkn
+ * must be at most 16,777,216 (larger is an error); the array keeps the last
+ * call's partials, and calls that overlap on one GPU race on it. */
+int  shyhip_masscont_noalloc(int dtype, int nkn, const int *ilhkv, const void *hdk, const void *areakv,
+                             const void *conc, double *out, void *stream);
+
 /* diagnostics ------------------------------------------------------------ */
 int  shyhip_volume(shyhip_model *h, int mode, double *vol);     /* FP64 total volume */
 int  shyhip_tracer_mass(shyhip_model *h, int itr, double *mass);

@@ -1389,6 +1389,14 @@ int shyhip_member_istot(shyhip_model *h, int nmem, int *istot) {
     return 0;
 }
 
+int shyhip_masscont_noalloc(int dtype, int nkn, const int *ilhkv, const void *hdk, const void *areakv,
+                            const void *conc, double *out, void *stream) {
+    std::string err;
+    if (noalloc_masscont(dtype, nkn, ilhkv, hdk, areakv, conc, out, (hipStream_t)stream, err))
+        return fail(nullptr, "masscont_noalloc: " + err);
+    return 0;
+}
+
 int shyhip_get_sizes(const shyhip_model *h, int *nkn, int *nel, int *nlvdi, int *nlv, int *ntracer) {
     if (!h) return fail(nullptr, "null handle");
     const Model &M = h->M;

@@ -1613,6 +1613,59 @@ int launch_rhoset(Model &M, int niter, hipStream_t st) {
     return M.prec == SHYHIP_FP64 ? rhoset_t<double>(M, niter, st) : rhoset_t<float>(M, niter, st);
 }
 
+// ------------------------------------------------------------------ allocation-free volume / mass
+// The partial sums of the allocation-free variant of the volume and tracer
+// mass sums live in this array, one copy per GPU, held until the process
+// ends.  Host code never uses its name as an address: the address comes from
+// hipGetSymbolAddress, once per device (the first call on that device), and
+// is kept per device.
+constexpr int kNoallocParts = 65536;                 // 512 KiB: nkn <= 65536 * kThreads
+static __device__ __align__(256) double scratch[kNoallocParts];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[64] = {};
+
+int noalloc_masscont(int dtype, int nkn, const int *ilhkv, const void *hdk, const void *areakv, const void *conc,
+                     double *out, hipStream_t st, std::string &err) {
+    if (dtype != SHYHIP_F32 && dtype != SHYHIP_F64) { err = "dtype must be SHYHIP_F32 or SHYHIP_F64"; return 1; }
+    if (nkn <= 0 || !ilhkv || !hdk || !areakv || !out) { err = "invalid arguments"; return 1; }
+    const int nb = nblocks(nkn);
+    if (nb > kNoallocParts) {
+        err = "nkn = " + std::to_string(nkn) + " exceeds the allocation-free limit of " +
+              std::to_string((long long)kNoallocParts * kThreads) + " nodes";
+        return 1;
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) { err = "no current HIP device"; return 1; }
+    double *part = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch_ptr[dev]) {
+            void *ptr = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !ptr) {
+                err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+                return 1;
+            }
+            g_scratch_ptr[dev] = ptr;
+        }
+        part = (double *)g_scratch_ptr[dev];
+    }
+    if (dtype == SHYHIP_F64)
+        hipLaunchKernelGGL(k_volume_part<double>, dim3(nb), dim3(kThreads), 0, st, nkn, ilhkv, (const double *)areakv,
+                           (const double *)hdk, (const double *)conc, part);
+    else
+        hipLaunchKernelGGL(k_volume_part<float>, dim3(nb), dim3(kThreads), 0, st, nkn, ilhkv, (const float *)areakv,
+                           (const float *)hdk, (const float *)conc, part);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kThreads), 0, st, part, nb, out);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) { err = std::string("launch: ") + hipGetErrorString(e); return 1; }
+    return 0;
+}
+
 int launch_volume(Model &M, int mode, double *vol) {
     const void *hdk = mode > 0 ? M.s.hdkn : M.s.hdko;
     return M.prec == SHYHIP_FP64 ? volume_t<double>(M, hdk, nullptr, vol) : volume_t<float>(M, hdk, nullptr, vol);

@@ -358,6 +358,9 @@ int launch_mass_flux(Model &M, bool with_rqds);
 int launch_rhoset(Model &M, int niter, hipStream_t st);
 int launch_fill(Model &M, void *ptr, double value, size_t n);
 int launch_volume(Model &M, int mode, double *vol);
+// the allocation-free volume / tracer mass of caller device arrays
+int noalloc_masscont(int dtype, int nkn, const int *ilhkv, const void *hdk, const void *areakv, const void *conc,
+                     double *out, hipStream_t st, std::string &err);
 int launch_tracer_mass(Model &M, int itr, double *mass);
 int solver_setup(Model &M);
 // Zeta solve.  wetdry_bnew >= 0: system_get + setweg(1) (bnewtime =
