@@ -66,10 +66,11 @@ def _block_nnz(b, brows):
     return int(2 * inside.sum() + b.nkn), int(2 * len(e) + b.nkn)
 
 
-def _kernel_bytes(b, p, brows):
+def _kernel_bytes(b, p, brows, ntr=2):
     """Algorithmic HBM bytes per launch of each timed kernel: every listed
     array read or written once at its stored width p (gathers that hit
-    cache are not counted)."""
+    cache are not counted).  The transport kernels carry ntr tracers per
+    launch (T and S together)."""
     EL = int(b.ilhv.sum())
     NL = int(b.ilhkv.sum())
     E, N = b.nel, b.nkn
@@ -80,10 +81,11 @@ def _kernel_bytes(b, p, brows):
         # utlov vtlov hdeov read, utlnv vtlnv written, ddxv ddyv (2 planes) written,
         # visv at the interfaces, per element geometry/levels/summaries
         "hydro_intern": 9 * EL * p + (NL + N) * p + E * (12 + 56 + 6 * p + 4 + 48),
-        # utlnv vtlnv utlov vtlov ulnv vlnv read, 3 horizontal terms written, c gathered
-        "conz3d_elem": 9 * EL * p + NL * p + E * (12 + 56 + 4),
-        # horizontal terms read; c hdknv hdkov wlnv difv read, c written
-        "conz3d_node": 3 * EL * p + 6 * NL * p + 2 * N * p,
+        # utlnv vtlnv utlov vtlov ulnv vlnv read, 3 horizontal terms per tracer
+        # written, each tracer gathered
+        "conz3d_elem": (6 + 3 * ntr) * EL * p + ntr * NL * p + E * (12 + 56 + 4),
+        # horizontal terms read; hdknv hdkov wlnv difv read, each tracer read and written
+        "conz3d_node": 3 * ntr * EL * p + (4 + 2 * ntr) * NL * p + 2 * N * p,
         "transports_final": 8 * EL * p + E * (12 + 56 + 3 * p + 2 * p),
         "hydro_vertical": 4 * EL * p + 4 * NL * p + N * p,
         "setdepth": EL * p + NL * p + E * 3 * p,
@@ -100,8 +102,8 @@ def _kernel_bytes(b, p, brows):
 # kernel-name prefixes (template arguments after the value type vary with
 # the iteration-kernel variant) of each timed group's kernel
 KERNEL_NAMES = {"bicg_a": "k_bicg_a<{v},", "bicg_b": "k_bicg_b<{v},",
-                "bilu_factor": "k_bilu_factor<{v}>", "conz3d_elem": "k_conz_elem<{v}>",
-                "conz3d_node": "k_conz_node<{v}, 12>", "hydro_vertical": "k_hydro_vertical<{v}, 12>",
+                "bilu_factor": "k_bilu_factor<{v}>", "conz3d_elem": "k_conz_elem<{v}, 2>",
+                "conz3d_node": "k_conz_node<{v}, 12, 2>", "hydro_vertical": "k_hydro_vertical<{v}, 12>",
                 "hydro_intern": "k_hydro_intern<{v}, 12, false>"}
 
 

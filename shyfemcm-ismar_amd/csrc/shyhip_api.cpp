@@ -325,7 +325,7 @@ int shyhip_destroy(shyhip_model *h) {
     for (void *q : {M.spare_ut, M.spare_vt, M.spare_wl, M.spare_hdk, M.spare_hde, M.uln_b, M.vln_b,
                     (void *)M.iweg_tr[0], (void *)M.iweg_tr[1]})
         dfree(q);
-    for (void *q : M.tr_hbuf) dfree(q);
+    for (void *q : M.grp_hbuf) dfree(q);
     forcing_free_bnd(M);
     forcing_free_wind(M);
     forcing_free_sources(M);
@@ -1436,15 +1436,42 @@ int shyhip_scalar_stability(shyhip_model *h, float rstol, int *istot, double *ri
 }
 
 // scal3sh of tracer itr on stream st with conz3d scratch hbuf
-static int scal3sh_on(Model &M, int itr, int istot, int clip_small, hipStream_t st, void *hbuf) {
-    if (itr < 0 || itr >= M.ntr) return fail(&M, "tracer index out of range");
+static int scal3sh_on(Model &M, const TracerSet &ts, int istot, hipStream_t st, void *hbuf) {
+    for (int q = 0; q < ts.n; q++)
+        if (ts.itr[q] < 0 || ts.itr[q] >= M.ntr) return fail(&M, "tracer index out of range");
     if (istot < 1 || istot > 1000) return fail(&M, "istot must be in 1..1000");
     if (M.p.itvd == 2 && !M.tvd_ready) return fail(&M, "itvd = 2 needs shyhip_tvd_init");
     for (int isact = 1; isact <= istot; isact++) {
-        if (launch_conz3d(M, itr, istot, isact, clip_small, st, hbuf)) return 1;
+        if (launch_conz3d(M, ts, istot, isact, st, hbuf)) return 1;
         timer_begin(M, T_BNDO);
-        if (launch_bndo(M, itr, st, isact)) return 1;       // bndo_setbc
+        if (launch_bndo(M, ts, st, isact)) return 1;       // bndo_setbc
         timer_end(M);
+    }
+    return 0;
+}
+
+// The tracers of a step are transported in groups: pairs (0,1), (2,3), ...
+// and a last single one; each group's launches carry both its tracers.
+// Slot 0 (temperature) is not clipped, the others are (scal3sh's
+// 'temp' / non-'temp' rule).
+static int ngroups(const Model &M) { return (M.ntr + 1) / 2; }
+static TracerSet group_of(const Model &M, int g) {
+    TracerSet ts;
+    const int t0 = 2 * g;
+    ts.n = t0 + 1 < M.ntr ? 2 : 1;
+    for (int q = 0; q < ts.n; q++) {
+        ts.itr[q] = t0 + q;
+        ts.clip[q] = t0 + q == 0 ? 0 : 1;
+    }
+    return ts;
+}
+// the element-term scratch of each group (sized for its tracers; created once)
+static int group_buffers(Model &M) {
+    if ((int)M.grp_hbuf.size() == ngroups(M)) return 0;
+    for (int g = 0; g < ngroups(M); g++) {
+        void *hb = nullptr;
+        if (dalloc_s(M, &hb, (size_t)group_of(M, g).n * 3 * M.nlvdi * M.nel)) return 1;
+        M.grp_hbuf.push_back(hb);
     }
     return 0;
 }
@@ -1484,23 +1511,24 @@ int shyhip_scal3sh(shyhip_model *h, int itr, int istot, int clip_small) {
     if (!h) return fail(nullptr, "null handle");
     if (pipe_drain(h->M)) return 1;
     if (resolve_istot(h->M, &istot)) return 1;
-    return scal3sh_on(h->M, itr, istot, clip_small, h->M.stream, h->M.s.hbuf);
+    TracerSet ts;
+    ts.itr[0] = itr;
+    ts.clip[0] = clip_small;
+    return scal3sh_on(h->M, ts, istot, h->M.stream, h->M.s.hbuf);
 }
 
-// streams, events and scratch of the tracers after the first (created once)
+// streams and events of the tracer groups after the first (created once)
 static int tracer_streams(Model &M) {
-    if (!M.tr_stream.empty() || M.ntr < 2) return 0;
-    HIPCHK(M, hipEventCreateWithFlags(&M.ev_hydro, hipEventDisableTiming));
-    for (int t = 1; t < M.ntr; t++) {
+    if (group_buffers(M)) return 1;
+    if (!M.tr_stream.empty() || ngroups(M) < 2) return 0;
+    if (!M.ev_hydro) HIPCHK(M, hipEventCreateWithFlags(&M.ev_hydro, hipEventDisableTiming));
+    for (int g = 1; g < ngroups(M); g++) {
         hipStream_t st = nullptr;
         hipEvent_t ev = nullptr;
         HIPCHK(M, hipStreamCreate(&st));
         M.tr_stream.push_back(st);
         HIPCHK(M, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
         M.tr_done.push_back(ev);
-        void *hb = nullptr;
-        if (dalloc_s(M, &hb, 3 * (size_t)M.nlvdi * M.nel)) return 1;
-        M.tr_hbuf.push_back(hb);
     }
     return 0;
 }
@@ -1512,7 +1540,7 @@ static int pipe_setup(Model &M) {
     if (!M.ev_hydro) HIPCHK(M, hipEventCreateWithFlags(&M.ev_hydro, hipEventDisableTiming));
     HIPCHK(M, hipStreamCreate(&M.tr0_stream));
     for (int p = 0; p < 2; p++)
-        for (int t = 0; t < M.ntr; t++) {
+        for (int g = 0; g < ngroups(M); g++) {
             hipEvent_t ev = nullptr;
             HIPCHK(M, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
             M.pipe_ev[p].push_back(ev);
@@ -1559,16 +1587,15 @@ int shyhip_step(shyhip_model *h, double dtime, int istot) {
     // one istot for all tracers: conzstab does not depend on the tracer
     if (M.ntr > 0 && resolve_istot(M, &istot)) return 1;
     if (pipe) {
-        // every tracer on its own stream after this step's hydro; the model's
+        // every tracer group on its own stream after this step's hydro; the model's
         // stream goes on with the next step
         HIPCHK(M, hipEventRecord(M.ev_hydro, M.stream));
         M.iweg_conz = M.iweg_tr[par];
-        for (int t = 0; t < M.ntr; t++) {
-            hipStream_t st = t == 0 ? M.tr0_stream : M.tr_stream[t - 1];
-            void *hb = t == 0 ? M.s.hbuf : M.tr_hbuf[t - 1];
+        for (int g = 0; g < ngroups(M); g++) {
+            hipStream_t st = g == 0 ? M.tr0_stream : M.tr_stream[g - 1];
             HIPCHK(M, hipStreamWaitEvent(st, M.ev_hydro, 0));
-            if (scal3sh_on(M, t, istot, t == 0 ? 0 : 1, st, hb)) { M.iweg_conz = nullptr; return 1; }
-            HIPCHK(M, hipEventRecord(M.pipe_ev[par][t], st));
+            if (scal3sh_on(M, group_of(M, g), istot, st, M.grp_hbuf[g])) { M.iweg_conz = nullptr; return 1; }
+            HIPCHK(M, hipEventRecord(M.pipe_ev[par][g], st));
         }
         M.iweg_conz = nullptr;
         M.pipe_rec[par] = true;
@@ -1576,21 +1603,23 @@ int shyhip_step(shyhip_model *h, double dtime, int istot) {
         M.pipe_k++;
         return 0;
     }
-    if (M.ntr > 1 && !M.timing) {
-        // the tracers are transported concurrently: each after hydro, each
-        // on its own stream, joined back into the model's stream
+    if (M.ntr > 0) {
         if (tracer_streams(M)) return 1;
-        HIPCHK(M, hipEventRecord(M.ev_hydro, M.stream));
-        for (int t = 1; t < M.ntr; t++) HIPCHK(M, hipStreamWaitEvent(M.tr_stream[t - 1], M.ev_hydro, 0));
-        if (scal3sh_on(M, 0, istot, 0, M.stream, M.s.hbuf)) return 1;
-        for (int t = 1; t < M.ntr; t++) {
-            if (scal3sh_on(M, t, istot, 1, M.tr_stream[t - 1], M.tr_hbuf[t - 1])) return 1;
-            HIPCHK(M, hipEventRecord(M.tr_done[t - 1], M.tr_stream[t - 1]));
-            HIPCHK(M, hipStreamWaitEvent(M.stream, M.tr_done[t - 1], 0));
+        if (ngroups(M) > 1 && !M.timing) {
+            // the groups are transported concurrently: each after hydro, on
+            // its own stream, joined back into the model's stream
+            HIPCHK(M, hipEventRecord(M.ev_hydro, M.stream));
+            for (int g = 1; g < ngroups(M); g++) HIPCHK(M, hipStreamWaitEvent(M.tr_stream[g - 1], M.ev_hydro, 0));
+            if (scal3sh_on(M, group_of(M, 0), istot, M.stream, M.grp_hbuf[0])) return 1;
+            for (int g = 1; g < ngroups(M); g++) {
+                if (scal3sh_on(M, group_of(M, g), istot, M.tr_stream[g - 1], M.grp_hbuf[g])) return 1;
+                HIPCHK(M, hipEventRecord(M.tr_done[g - 1], M.tr_stream[g - 1]));
+                HIPCHK(M, hipStreamWaitEvent(M.stream, M.tr_done[g - 1], 0));
+            }
+        } else {
+            for (int g = 0; g < ngroups(M); g++)
+                if (scal3sh_on(M, group_of(M, g), istot, M.stream, M.grp_hbuf[g])) return 1;
         }
-    } else {
-        for (int t = 0; t < M.ntr; t++)
-            if (shyhip_scal3sh(h, t, istot, t == 0 ? 0 : 1)) return 1;
     }
     // barocl: the density of the transported T/S (rhoset_shell(1),
     // baroclinic.f90:542-548) for the next step's baroclinic term

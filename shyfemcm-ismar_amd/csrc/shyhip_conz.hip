@@ -28,9 +28,16 @@
 
 namespace shy {
 
+// Up to kMaxTracers tracers are transported by one launch of each phase:
+// they share every hydrodynamic load, the element flux coefficients and the
+// node column matrix; each keeps its own arithmetic, identical to its
+// transport alone.
+constexpr int kMaxTracers = 2;
+
 template <typename S>
 struct ConzArgs {
-    int nkn, nel, nlvdi, nlv, itvd, itvdv, clip;
+    int nkn, nel, nlvdi, nlv, itvd, itvdv;
+    int clip[kMaxTracers];
     double dt, az, ad, aa, rso, rsn;
     float difmol;
     const int *nen3v, *ilhv, *ilhkv, *iextb, *ibnd;
@@ -41,9 +48,9 @@ struct ConzArgs {
     const double *tvd_w;
     const S *utn, *vtn, *uto, *vto, *uln, *vln;
     const S *hdkn, *hdko, *hden, *hdeo, *wln, *difv, *mflux;
-    S *c;            // tracer (in/out)
-    const S *cbound; // boundary concentration (may be null)
-    S *hbuf;
+    S *c[kMaxTracers];               // tracers (in/out)
+    const S *cbound[kMaxTracers];    // source concentrations (may be null)
+    S *hbuf;                         // element terms [((l-1)*NT + t)*3 + ii][ie]
     int *err;
     // ensembles whose members run different numbers of substeps: member
     // istot (null: all run istot), node/element -> member, this substep and
@@ -55,17 +62,10 @@ struct ConzArgs {
 };
 
 template <typename S>
-__device__ inline double cval(const ConzArgs<S> &a, int l, int k) {
-    const S v = a.c[(size_t)(l - 1) * a.nkn + k];
-    if (a.clip && v < (S)1.e-15) return 0.;
+__device__ inline double cval(const ConzArgs<S> &a, int t, int l, int k) {
+    const S v = a.c[t][(size_t)(l - 1) * a.nkn + k];
+    if (a.clip[t] && v < (S)1.e-15) return 0.;
     return (double)v;
-}
-
-template <typename S>
-__device__ inline S cval_s(const ConzArgs<S> &a, int l, int k) {
-    const S v = a.c[(size_t)(l - 1) * a.nkn + k];
-    if (a.clip && v < (S)1.e-15) return (S)0;
-    return v;
 }
 
 // ------------------------------------------------------------------ element phase
@@ -73,13 +73,10 @@ __device__ inline S cval_s(const ConzArgs<S> &a, int l, int k) {
 // geometry, levels and TVD upwind tables are staged in LDS once, four
 // work-items per element, every load independent of the others (the upwind
 // elements' levels and the elements' external-node counts are precomputed
-// tables).  Then each work-item takes up to kElemBatch element-layers of
-// the tile at once: the loads of all of them are issued together (the
-// layer's transports and node concentrations, then the upwind
-// concentrations of the two pairs the flow direction selects), before any
-// flux is computed.
+// tables).  Then one work-item per element-layer of the tile: the layer's
+// transports and every tracer's node concentrations, the flux coefficients
+// once, and per tracer the upwind rule or the TVD limiter.
 constexpr int kElemTile = 64;
-constexpr int kElemBatch = 1;
 
 // v[i] for a run-time i in 0..2 without indexing a register array
 template <typename T>
@@ -95,7 +92,7 @@ struct ElemTileLds {
     double ajr[kElemTile];
 };
 
-template <typename S>
+template <typename S, int NT>
 __global__ __launch_bounds__(kThreads) void k_conz_elem(ConzArgs<S> a) {
     using R = S;
     __shared__ ElemTileLds T;
@@ -136,107 +133,88 @@ __global__ __launch_bounds__(kThreads) void k_conz_elem(ConzArgs<S> a) {
     __syncthreads();
     const int lmt = lmax_tile;
     const double azt = 1. - a.az;
-    auto clip = [&](S v) -> S { return (a.clip && v < (S)1.e-15) ? (S)0 : v; };
-    for (int base = 0; base < kElemTile * lmt; base += kElemBatch * kThreads) {
-        bool on[kElemBatch], tvd[kElemBatch];
-        int ev[kElemBatch], lv[kElemBatch];
-        S un[kElemBatch], uo[kElemBatch], vn[kElemBatch], vo[kElemBatch], ul[kElemBatch], vl[kElemBatch];
-        S cr[kElemBatch][3];
-        // round 1: the layer's transports, velocities and node concentrations
+    for (int idx = threadIdx.x; idx < kElemTile * lmt; idx += kThreads) {
+        const int e = idx % kElemTile, l = idx / kElemTile + 1;
+        if (e >= ne || l > T.ilev[e]) continue;
+        const int ie = e0 + e;
+        if (a.mem_istot && a.isact > a.mem_istot[a.elem_mem[ie]]) continue;
+        // the layer's transports, velocities and node concentrations
+        const size_t o = (size_t)(l - 1) * nel + ie;
+        const S un = a.utn[o], uo = a.uto[o], vn = a.vtn[o], vo = a.vto[o];
+        const bool tvd = use_tvd && T.iext[e] == 0;
+        S ul = 0, vl = 0;
+        if (tvd) { ul = a.uln[o]; vl = a.vln[o]; }
+        S cr[NT][3];
 #pragma unroll
-        for (int j = 0; j < kElemBatch; j++) {
-            const int idx = base + j * kThreads + threadIdx.x;
-            const int e = idx % kElemTile, l = idx / kElemTile + 1;
-            on[j] = idx < kElemTile * lmt && e < ne && l <= T.ilev[e];
-            if (on[j] && a.mem_istot) on[j] = a.isact <= a.mem_istot[a.elem_mem[e0 + e]];
-            ev[j] = e; lv[j] = l;
-            tvd[j] = false;
-            if (on[j]) {
-                const size_t o = (size_t)(l - 1) * nel + e0 + e;
-                un[j] = a.utn[o]; uo[j] = a.uto[o]; vn[j] = a.vtn[o]; vo[j] = a.vto[o];
-                tvd[j] = use_tvd && T.iext[e] == 0;
-                if (tvd[j]) { ul[j] = a.uln[o]; vl[j] = a.vln[o]; }
-                for (int ii = 0; ii < 3; ii++) cr[j][ii] = a.c[(size_t)(l - 1) * nkn + T.kn[ii][e]];
+        for (int t = 0; t < NT; t++)
+            for (int ii = 0; ii < 3; ii++) cr[t][ii] = a.c[t][(size_t)(l - 1) * nkn + T.kn[ii][e]];
+        auto clip = [&](int t, S v) -> S { return (a.clip[t] && v < (S)1.e-15) ? (S)0 : v; };
+        // upwind rule (the flux coefficients are the tracers' common part)
+        const double us = a.az * (double)un + azt * (double)uo;
+        const double vs = a.az * (double)vn + azt * (double)vo;
+        int itot = 0, isum = 0;
+        double f[3];
+        for (int ii = 0; ii < 3; ii++) {
+            f[ii] = us * T.b[ii][e] + vs * T.c[ii][e];
+            if (f[ii] < 0.) { itot++; isum += ii + 1; }
+        }
+        double fl[NT][3];
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            double cl[3];
+            for (int ii = 0; ii < 3; ii++) cl[ii] = (double)clip(t, cr[t][ii]);
+            if (itot == 1) {
+                const double cu1 = sel3(cl, isum - 1);
+                for (int ii = 0; ii < 3; ii++) fl[t][ii] = f[ii] * cu1;
+            } else if (itot == 2) {
+                const int is2 = 6 - isum;
+                for (int ii = 0; ii < 3; ii++) fl[t][ii] = (ii == is2 - 1) ? 0. : f[ii] * cl[ii];
+                const double sum = fl[t][0] + fl[t][1] + fl[t][2];
+                for (int ii = 0; ii < 3; ii++)
+                    if (ii == is2 - 1) fl[t][ii] = -sum;
+            } else {
+                fl[t][0] = fl[t][1] = fl[t][2] = 0.;
             }
         }
-        double f[kElemBatch][3], fl[kElemBatch][3];
-        int itot[kElemBatch], isum[kElemBatch];
-        // upwind rule; the TVD pairs of the flow direction and their loads (round 2)
-        int pii[kElemBatch][2], pic[kElemBatch][2], pid[kElemBatch][2], pix[kElemBatch][2];
-        bool pup[kElemBatch][2];
-        S cu[kElemBatch][2][3];
+        // TVD fluxes (tvd_fluxes, tvd_admin.f90:673-823): the pairs of the
+        // flow direction, then each tracer's upwind values and limiter
+        if (tvd) {
 #pragma unroll
-        for (int j = 0; j < kElemBatch; j++) {
-            if (!on[j]) continue;
-            const int e = ev[j];
-            const double us = a.az * (double)un[j] + azt * (double)uo[j];
-            const double vs = a.az * (double)vn[j] + azt * (double)vo[j];
-            itot[j] = 0; isum[j] = 0;
-            double cl[3];
-            for (int ii = 0; ii < 3; ii++) {
-                f[j][ii] = us * T.b[ii][e] + vs * T.c[ii][e];
-                if (f[j][ii] < 0.) { itot[j]++; isum[j] += ii + 1; }
-                cl[ii] = (double)clip(cr[j][ii]);
-            }
-            if (itot[j] == 1) {
-                const double cu1 = sel3(cl, isum[j] - 1);
-                for (int ii = 0; ii < 3; ii++) fl[j][ii] = f[j][ii] * cu1;
-            } else if (itot[j] == 2) {
-                const int is2 = 6 - isum[j];
-                for (int ii = 0; ii < 3; ii++) fl[j][ii] = (ii == is2 - 1) ? 0. : f[j][ii] * cl[ii];
-                const double sum = fl[j][0] + fl[j][1] + fl[j][2];
-                for (int ii = 0; ii < 3; ii++)
-                    if (ii == is2 - 1) fl[j][ii] = -sum;
-            } else {
-                fl[j][0] = fl[j][1] = fl[j][2] = 0.;
-            }
-            pup[j][0] = pup[j][1] = false;
-            if (tvd[j] && itot[j] >= 1 && itot[j] <= 2) {
-                const int itot2 = itot[j] - 1, itot1 = 2 - itot[j];
-                const int ip = itot2 * (6 - isum[j]) + itot1 * isum[j];
-#pragma unroll
+            for (int t = 0; t < NT; t++) fl[t][0] = fl[t][1] = fl[t][2] = 0.;
+            if (itot >= 1 && itot <= 2) {
+                const int itot2 = itot - 1, itot1 = 2 - itot;
+                const int ip = itot2 * (6 - isum) + itot1 * isum;
+                const R ajr = (R)(24. * T.ajr[e]);
+                const double dte = a.mem_istot ? a.dtp / (double)a.mem_istot[a.elem_mem[ie]] : a.dt;
                 for (int k = 0; k < 2; k++) {
                     // the vertices other than ip, ascending
                     const int ii = k == 0 ? (ip == 1 ? 2 : 1) : (ip == 3 ? 2 : 3);
                     const int ic = itot2 * ii + itot1 * ip;
                     const int id = itot2 * ip + itot1 * ii;
                     const int pidx = 2 * (ic - 1) + ((id == (ic % 3) + 1) ? 0 : 1);
-                    pii[j][k] = ii; pic[j][k] = ic; pid[j][k] = id; pix[j][k] = pidx;
-                    pup[j][k] = T.tel[pidx][e] >= 0 && T.tlev[pidx][e] >= lv[j];
-                    if (pup[j][k])
-                        for (int q = 0; q < 3; q++) cu[j][k][q] = a.c[(size_t)(lv[j] - 1) * nkn + T.tkn[q][pidx][e]];
-                }
-            }
-        }
-        // TVD fluxes (tvd_fluxes, tvd_admin.f90:673-823) and the write-out
+                    const bool pup = T.tel[pidx][e] >= 0 && T.tlev[pidx][e] >= l;
+                    S cu[NT][3];
+                    if (pup)
 #pragma unroll
-        for (int j = 0; j < kElemBatch; j++) {
-            if (!on[j]) continue;
-            const int e = ev[j], ie = e0 + e, l = lv[j];
-            if (tvd[j]) {
-                fl[j][0] = fl[j][1] = fl[j][2] = 0.;
-                if (itot[j] >= 1 && itot[j] <= 2) {
-                    const int itot2 = itot[j] - 1, itot1 = 2 - itot[j];
-                    const R ajr = (R)(24. * T.ajr[e]);
-                    const R u = ul[j], v = vl[j];
-                    for (int k = 0; k < 2; k++) {
-                        const int ii = pii[j][k], ic = pic[j][k], id = pid[j][k], pidx = pix[j][k];
-                        const R fact = (R)(-itot2 + itot1);
-                        const R conc = clip(sel3(cr[j], ic - 1)), cond = clip(sel3(cr[j], id - 1));
-                        const int iop = 6 - (id + ic);
-                        const int tet1 = 1 + iop % 3;
-                        R dx = (R)((double)ajr * T.c[iop - 1][e]);
-                        dx = (R)(-2 * (tet1 == id ? 1 : 0)) * dx + dx;
-                        R dy = (R)((double)ajr * T.b[iop - 1][e]);
-                        dy = (R)(-2 * (tet1 == ic ? 1 : 0)) * dy + dy;
-                        const R dis = (R)T.dis[iop - 1][e];
-                        const R vel = (R)fabs((double)(u * dx + v * dy)) / dis;
-                        const double dte = a.mem_istot ? a.dtp / (double)a.mem_istot[a.elem_mem[ie]] : a.dt;
-                        const R alfa = (R)((dte * (double)vel) / (double)dis);
+                        for (int t = 0; t < NT; t++)
+                            for (int q = 0; q < 3; q++) cu[t][q] = a.c[t][(size_t)(l - 1) * nkn + T.tkn[q][pidx][e]];
+                    const R fact = (R)(-itot2 + itot1);
+                    const int iop = 6 - (id + ic);
+                    const int tet1 = 1 + iop % 3;
+                    R dx = (R)((double)ajr * T.c[iop - 1][e]);
+                    dx = (R)(-2 * (tet1 == id ? 1 : 0)) * dx + dx;
+                    R dy = (R)((double)ajr * T.b[iop - 1][e]);
+                    dy = (R)(-2 * (tet1 == ic ? 1 : 0)) * dy + dy;
+                    const R dis = (R)T.dis[iop - 1][e];
+                    const R vel = (R)fabs((double)(ul * dx + vl * dy)) / dis;
+                    const R alfa = (R)((dte * (double)vel) / (double)dis);
+#pragma unroll
+                    for (int t = 0; t < NT; t++) {
+                        const R conc = clip(t, sel3(cr[t], ic - 1)), cond = clip(t, sel3(cr[t], id - 1));
                         R conu = cond;
-                        if (pup[j][k]) {
+                        if (pup) {
                             double zh = 0.;
-                            for (int q = 0; q < 3; q++) zh = zh + (double)clip(cu[j][k][q]) * T.tw[q][pidx][e];
+                            for (int q = 0; q < 3; q++) zh = zh + (double)clip(t, cu[t][q]) * T.tw[q][pidx][e];
                             conu = (R)zh;
                         }
                         const R grad = cond - conu;
@@ -246,16 +224,19 @@ __global__ __launch_bounds__(kThreads) void k_conz_elem(ConzArgs<S> a) {
                         const double rfd = (double)rf;
                         const R psi = (R)fmax(0., fmax(fmin(1., 2. * rfd), fmin(2., rfd)));
                         const R conf = conc + (R)0.5 * psi * (cond - conc) * ((R)1 - alfa);
-                        const R term = (R)((double)(fact * conf) * sel3(f[j], ii - 1));
-                        for (int t = 0; t < 3; t++) {
-                            if (t == ic - 1) fl[j][t] = fl[j][t] - (double)term;
-                            if (t == id - 1) fl[j][t] = fl[j][t] + (double)term;
+                        const R term = (R)((double)(fact * conf) * sel3(f, ii - 1));
+                        for (int q = 0; q < 3; q++) {
+                            if (q == ic - 1) fl[t][q] = fl[t][q] - (double)term;
+                            if (q == id - 1) fl[t][q] = fl[t][q] + (double)term;
                         }
                     }
                 }
             }
-            for (int ii = 0; ii < 3; ii++) a.hbuf[((size_t)(l - 1) * 3 + ii) * nel + ie] = (S)(3. * fl[j][ii]);
         }
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+            for (int ii = 0; ii < 3; ii++)
+                a.hbuf[(((size_t)(l - 1) * NT + t) * 3 + ii) * nel + ie] = (S)(3. * fl[t][ii]);
     }
 }
 
@@ -297,11 +278,11 @@ struct NodeTile {
     static constexpr int TN = NL <= 16 ? 64 : 32;    // nodes per workgroup
 };
 
-template <typename S, int NL>
+template <typename S, int NL, int NT>
 __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
     using R = S;
     constexpr int TN = NodeTile<NL>::TN;
-    __shared__ double cnL[NL][TN], cdL[NL][TN], cloL[NL][TN], chiL[NL][TN];
+    __shared__ double cnL[NT][NL][TN], cdL[NL][TN], cloL[NL][TN], chiL[NL][TN];
     const int nkn = a.nkn, nel = a.nel;
     const int k0 = blockIdx.x * TN;
     const double ad = a.ad, aa = a.aa, adt = 1. - a.ad, aat = 1. - a.aa;
@@ -327,12 +308,15 @@ __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
         IncSlots inc;     // incident elements in colour order
         inc.load(a.inc_cnt, a.inc_elc, nkn, k);
         // node column around l (zero outside 1..lmaxk)
-        double ckw[5], hokw[5], hnkw[5];
+        double ckw[NT][5], hokw[5], hnkw[5];
         for (int j = 0; j < 5; j++) {
             const int x = l - 2 + j;
-            ckw[j] = 0.; hokw[j] = 0.; hnkw[j] = 0.;
+            hokw[j] = 0.; hnkw[j] = 0.;
+#pragma unroll
+            for (int t = 0; t < NT; t++) ckw[t][j] = 0.;
             if (x >= 1 && x <= lmaxk) {
-                ckw[j] = cval(a, x, k);
+#pragma unroll
+                for (int t = 0; t < NT; t++) ckw[t][j] = cval(a, t, x, k);
                 if (j >= 1 && j <= 3) {
                     const size_t o = (size_t)(x - 1) * nkn + k;
                     const double hn = (double)a.hdkn[o], ho = (double)a.hdko[o];
@@ -347,15 +331,13 @@ __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
         const double wbot = (double)a.wln[(size_t)l * nkn + k];
         // element terms of the three column truncations: bottom layer l,
         // l+1, or deeper (nothing below l+2 is read)
-        double fdv[3], fwv[3], clcv[3], clmv[3], clpv[3];
+        double fdv[NT][3], fwv[NT][3], clcv[3], clmv[3], clpv[3];
         for (int v = 0; v < 3; v++) {
             const int lmax = l + v;
-            double clt[5], holdt[5];
+            double holdt[5];
             for (int j = 0; j < 5; j++) {
                 const int x = l - 2 + j;
-                const bool in = x >= 1 && x <= lmax;
-                clt[j] = in ? ckw[j] : 0.;
-                holdt[j] = in ? hokw[j] : 0.;
+                holdt[j] = (x >= 1 && x <= lmax) ? hokw[j] : 0.;
             }
             const double pm = (l - 1 >= 1) ? 1. : 0.;
             const double pp = (l + 1 <= lmax) ? 1. : 0.;
@@ -365,7 +347,6 @@ __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
             const double hmobot = 2. * rvpbot * pp / (hoc + hop);
             const double hmntop = 2. * rvptop * pm / (hnm + hnc);
             const double hmnbot = 2. * rvpbot * pp / (hnc + hnp);
-            fdv[v] = adt * ((clt[2] - clt[3]) * hmobot - (clt[1] - clt[2]) * hmotop);
             double clc = 0. + ad * (hmntop + hmnbot);
             double clm = 0. - ad * hmntop;
             double clp = 0. - ad * hmnbot;
@@ -375,31 +356,47 @@ __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
             w = (l == lmax) ? 0. : wbot;
             if (w > 0.) clp -= aa * w;
             else clc -= aa * w;
-            const double vf_prev = vflux_win(l - 1, l, lmax, wtop, clt, holdt, dt, btvdv);
-            const double vf_here = vflux_win(l, l, lmax, wbot, clt, holdt, dt, btvdv);
-            fwv[v] = aat * (vf_prev - vf_here);
             clcv[v] = clc; clmv[v] = clm; clpv[v] = clp;
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                double clt[5];
+                for (int j = 0; j < 5; j++) {
+                    const int x = l - 2 + j;
+                    clt[j] = (x >= 1 && x <= lmax) ? ckw[t][j] : 0.;
+                }
+                fdv[t][v] = adt * ((clt[2] - clt[3]) * hmobot - (clt[1] - clt[2]) * hmotop);
+                const double vf_prev = vflux_win(l - 1, l, lmax, wtop, clt, holdt, dt, btvdv);
+                const double vf_here = vflux_win(l, l, lmax, wbot, clt, holdt, dt, btvdv);
+                fwv[t][v] = aat * (vf_prev - vf_here);
+            }
         }
         // incident elements in colour order (conz3d_element's scatter)
-        const double hoc = hokw[2], hnc = hnkw[2], clc_l = ckw[2];
-        double cn = 0., cd = 0., clo = 0., chi = 0.;
-        auto add = [&](int ilev, double aj4, double hf) {
+        const double hoc = hokw[2], hnc = hnkw[2];
+        double cn[NT], cd = 0., clo = 0., chi = 0.;
+#pragma unroll
+        for (int t = 0; t < NT; t++) cn[t] = 0.;
+        auto add = [&](int ilev, double aj4, const double *hf) {
             if (ilev < l) return;
             const int v = min(ilev - l, 2);
-            const double cexpl = aj4 * (hoc * clc_l + dt * (0. + hf - fwv[v] - fdv[v]));
-            cn += cexpl;
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                const double cexpl = aj4 * (hoc * ckw[t][2] + dt * (0. + hf[t] - fwv[t][v] - fdv[t][v]));
+                cn[t] += cexpl;
+            }
             clo += aj4 * dt * clmv[v];
             chi += aj4 * dt * clpv[v];
             cd += aj4 * dt * clcv[v] + aj4 * (1. + dt * 0.) * hnc;
         };
         int slv[kIncWidth];
-        double saj[kIncWidth], shf[kIncWidth];
+        double saj[kIncWidth], shf[kIncWidth][NT];
 #pragma unroll
         for (int s = 0; s < kIncWidth; s++) {
             const int ie = inc.pk[s] >> 2, ii = inc.pk[s] & 3;
             slv[s] = a.ilhv[ie];
             saj[s] = 4. * a.geo[(size_t)6 * nel + ie];
-            shf[s] = (double)a.hbuf[((size_t)(l - 1) * 3 + ii) * nel + ie];
+#pragma unroll
+            for (int t = 0; t < NT; t++)
+                shf[s][t] = (double)a.hbuf[(((size_t)(l - 1) * NT + t) * 3 + ii) * nel + ie];
         }
 #pragma unroll
         for (int s = 0; s < kIncWidth; s++)
@@ -408,22 +405,30 @@ __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
             for (int q = a.nkc_ptr[k] + kIncWidth; q < a.nkc_ptr[k + 1]; q++) {
                 const int pk = a.nkc_el[q];
                 const int ie = pk >> 2, ii = pk & 3;
-                add(a.ilhv[ie], 4. * a.geo[(size_t)6 * nel + ie],
-                    (double)a.hbuf[((size_t)(l - 1) * 3 + ii) * nel + ie]);
+                double hf[NT];
+#pragma unroll
+                for (int t = 0; t < NT; t++) hf[t] = (double)a.hbuf[(((size_t)(l - 1) * NT + t) * 3 + ii) * nel + ie];
+                add(a.ilhv[ie], 4. * a.geo[(size_t)6 * nel + ie], hf);
             }
         // conz3d_nodes: boundary mass flux
         if (a.mflux) {
             const size_t o = (size_t)(l - 1) * nkn + k;
-            double cconz = a.cbound ? (double)a.cbound[o] : 0.;
             const double qflux = (double)a.mflux[o];
-            if (qflux < 0. && a.ibnd[k] > 0) cconz = clc_l;
-            cn += dt * (qflux * cconz);
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                double cconz = a.cbound[t] ? (double)a.cbound[t][o] : 0.;
+                if (qflux < 0. && a.ibnd[k] > 0) cconz = ckw[t][2];
+                cn[t] += dt * (qflux * cconz);
+            }
         }
-        cnL[l - 1][kk] = cn; cdL[l - 1][kk] = cd; cloL[l - 1][kk] = clo; chiL[l - 1][kk] = chi;
+#pragma unroll
+        for (int t = 0; t < NT; t++) cnL[t][l - 1][kk] = cn[t];
+        cdL[l - 1][kk] = cd; cloL[l - 1][kk] = clo; chiL[l - 1][kk] = chi;
     }
     __syncthreads();
 
-    // phase 2: the column solve of conz3d_nodes, one work-item per node
+    // phase 2: the column solve of conz3d_nodes, one work-item per node; the
+    // tracers share the column matrix and its elimination
     const int kk = threadIdx.x, k = k0 + kk;
     if (kk >= TN || k >= nkn) return;
     if (a.mem_istot && a.isact > a.mem_istot[a.node_mem[k]]) return;
@@ -431,28 +436,37 @@ __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
     bool bad = false;
     if ((aa == 0. && ad == 0.) || a.nlv == 1) {
         for (int l = 1; l <= lmaxk; l++)
-            if (cdL[l - 1][kk] != 0.) cnL[l - 1][kk] = cnL[l - 1][kk] / cdL[l - 1][kk];
+            if (cdL[l - 1][kk] != 0.)
+#pragma unroll
+                for (int t = 0; t < NT; t++) cnL[t][l - 1][kk] = cnL[t][l - 1][kk] / cdL[l - 1][kk];
     } else {
         double aux = 1. / cdL[0][kk];
         chiL[0][kk] *= aux;
-        cnL[0][kk] *= aux;
+#pragma unroll
+        for (int t = 0; t < NT; t++) cnL[t][0][kk] *= aux;
         for (int l = 2; l <= lmaxk; l++) {
             if (cdL[l - 1][kk] == 0.) bad = true;
             aux = 1. / (cdL[l - 1][kk] - cloL[l - 1][kk] * chiL[l - 2][kk]);
             chiL[l - 1][kk] *= aux;
-            cnL[l - 1][kk] = (cnL[l - 1][kk] - cloL[l - 1][kk] * cnL[l - 2][kk]) * aux;
+#pragma unroll
+            for (int t = 0; t < NT; t++)
+                cnL[t][l - 1][kk] = (cnL[t][l - 1][kk] - cloL[l - 1][kk] * cnL[t][l - 2][kk]) * aux;
         }
-        for (int l = lmaxk - 1; l >= 1; l--) cnL[l - 1][kk] = cnL[l - 1][kk] - cnL[l][kk] * chiL[l - 1][kk];
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+            for (int l = lmaxk - 1; l >= 1; l--) cnL[t][l - 1][kk] = cnL[t][l - 1][kk] - cnL[t][l][kk] * chiL[l - 1][kk];
     }
     if (bad) { atomicCAS(a.err, 0, ERR_CONZ); return; }
-    for (int l = 1; l <= a.nlvdi; l++) a.c[(size_t)(l - 1) * nkn + k] = (S)(l <= lmaxk ? cnL[l - 1][kk] : 0.);
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+        for (int l = 1; l <= a.nlvdi; l++) a.c[t][(size_t)(l - 1) * nkn + k] = (S)(l <= lmaxk ? cnL[t][l - 1][kk] : 0.);
 }
 
-template <typename S, int NL>
-static int conz_nl(Model &M, int itr, int istot, int isact, int clip, hipStream_t st, void *hbuf, const void *cb) {
+template <typename S, int NL, int NT>
+static int conz_nl(Model &M, const TracerSet &ts, int istot, int isact, hipStream_t st, void *hbuf) {
     ConzArgs<S> a{};
     a.nkn = M.nkn; a.nel = M.nel; a.nlvdi = M.nlvdi; a.nlv = M.nlv;
-    a.itvd = M.p.itvd; a.itvdv = M.p.itvdv; a.clip = clip;
+    a.itvd = M.p.itvd; a.itvdv = M.p.itvdv;
     const double rstot = istot;
     a.rso = (isact - 1) / rstot;
     a.rsn = isact / rstot;
@@ -468,8 +482,11 @@ static int conz_nl(Model &M, int itr, int istot, int isact, int clip, hipStream_
     a.hdkn = (const S *)M.s.hdkn; a.hdko = (const S *)M.s.hdko; a.hden = (const S *)M.s.hden;
     a.hdeo = (const S *)M.s.hdeo; a.wln = (const S *)M.s.wln; a.difv = (const S *)M.s.difv;
     a.mflux = M.has_mflux ? (const S *)M.s.mflux : nullptr;
-    a.c = (S *)M.s.tracer[itr];
-    a.cbound = (const S *)cb;
+    for (int t = 0; t < NT; t++) {
+        a.c[t] = (S *)M.s.tracer[ts.itr[t]];
+        a.cbound[t] = (const S *)ts.cbound[t];
+        a.clip[t] = ts.clip[t];
+    }
     a.isact = isact;
     a.dtp = (double)(S)M.p.dt;
     if (M.mem_istot_on) {
@@ -478,26 +495,26 @@ static int conz_nl(Model &M, int itr, int istot, int isact, int clip, hipStream_
     a.hbuf = (S *)hbuf;
     a.err = M.d_err;
     const int ge = (M.nel + kElemTile - 1) / kElemTile;
-    launch_timed(M, T_CONZ_ELEM, k_conz_elem<S>, dim3(ge), dim3(kThreads), 0, st, a);
+    launch_timed(M, T_CONZ_ELEM, k_conz_elem<S, NT>, dim3(ge), dim3(kThreads), 0, st, a);
     HIPCHK(M, hipGetLastError());
-    launch_timed(M, T_CONZ_NODE, k_conz_node<S, NL>, dim3((M.nkn + NodeTile<NL>::TN - 1) / NodeTile<NL>::TN),
+    launch_timed(M, T_CONZ_NODE, k_conz_node<S, NL, NT>, dim3((M.nkn + NodeTile<NL>::TN - 1) / NodeTile<NL>::TN),
                  dim3(kThreads), 0, st, a);
     HIPCHK(M, hipGetLastError());
     return 0;
 }
 
-template <typename S>
-static int conz_t(Model &M, int itr, int istot, int isact, int clip, hipStream_t st, void *hbuf, const void *cb) {
+template <typename S, int NT>
+static int conz_t(Model &M, const TracerSet &ts, int istot, int isact, hipStream_t st, void *hbuf) {
     switch (M.nl_tmpl) {
-    case 1: return conz_nl<S, 1>(M, itr, istot, isact, clip, st, hbuf, cb);
-    case 2: return conz_nl<S, 2>(M, itr, istot, isact, clip, st, hbuf, cb);
-    case 4: return conz_nl<S, 4>(M, itr, istot, isact, clip, st, hbuf, cb);
-    case 8: return conz_nl<S, 8>(M, itr, istot, isact, clip, st, hbuf, cb);
-    case 12: return conz_nl<S, 12>(M, itr, istot, isact, clip, st, hbuf, cb);
-    case 16: return conz_nl<S, 16>(M, itr, istot, isact, clip, st, hbuf, cb);
-    case 20: return conz_nl<S, 20>(M, itr, istot, isact, clip, st, hbuf, cb);
-    case 24: return conz_nl<S, 24>(M, itr, istot, isact, clip, st, hbuf, cb);
-    default: return conz_nl<S, 32>(M, itr, istot, isact, clip, st, hbuf, cb);
+    case 1: return conz_nl<S, 1, NT>(M, ts, istot, isact, st, hbuf);
+    case 2: return conz_nl<S, 2, NT>(M, ts, istot, isact, st, hbuf);
+    case 4: return conz_nl<S, 4, NT>(M, ts, istot, isact, st, hbuf);
+    case 8: return conz_nl<S, 8, NT>(M, ts, istot, isact, st, hbuf);
+    case 12: return conz_nl<S, 12, NT>(M, ts, istot, isact, st, hbuf);
+    case 16: return conz_nl<S, 16, NT>(M, ts, istot, isact, st, hbuf);
+    case 20: return conz_nl<S, 20, NT>(M, ts, istot, isact, st, hbuf);
+    case 24: return conz_nl<S, 24, NT>(M, ts, istot, isact, st, hbuf);
+    default: return conz_nl<S, 32, NT>(M, ts, istot, isact, st, hbuf);
     }
 }
 
@@ -517,10 +534,12 @@ __global__ __launch_bounds__(kBndoLanes) void k_bndo(int nbndo, const int *bk, c
                                                     const int *iweg, const int *nk_ptr, const int *nk_el,
                                                     const double *geo, const S *uln, const S *vln, S *cv,
                                                     const S *rbc, int nkn, int nel, int *err,
-                                                    const int *mem_istot, const int *node_mem, int isact) {
+                                                    const int *mem_istot, const int *node_mem, int isact,
+                                                    S *cv1, const S *rbc1) {
     using R = S;
     __shared__ R val[kBndoLanes];
     __shared__ int status[kBndoLanes];
+    if (blockIdx.y == 1) { cv = cv1; rbc = rbc1; }      // the second tracer of a pair
     const int i = blockIdx.x;
     if (i >= nbndo) return;
     const int k = bk[i];
@@ -580,22 +599,49 @@ __global__ __launch_bounds__(kBndoLanes) void k_bndo(int nbndo, const int *bk, c
 }
 
 template <typename S>
-static int bndo_t(Model &M, int itr, hipStream_t st, int isact) {
+static int bndo_t(Model &M, const TracerSet &ts, hipStream_t st, int isact) {
     if (M.m.nbndo == 0) return 0;
     if (M.nlv > kBndoLanes) return hip_check(M, hipErrorInvalidValue, "bndo: more layers than lanes");
-    hipLaunchKernelGGL(k_bndo<S>, dim3(M.m.nbndo), dim3(kBndoLanes), 0, st, M.m.nbndo, M.m.bndo_k,
+    const int t1 = ts.n > 1 ? ts.itr[1] : ts.itr[0];
+    hipLaunchKernelGGL(k_bndo<S>, dim3(M.m.nbndo, ts.n), dim3(kBndoLanes), 0, st, M.m.nbndo, M.m.bndo_k,
                        (const S *)M.m.bndo_nx, (const S *)M.m.bndo_ny, M.m.bndo_ptr, M.m.bndo_nb,
                        (const S *)M.m.bndo_w, M.m.ibnd, M.m.ibtyp, M.m.ilhkv, M.m.ilhv,
                        M.iweg_conz ? M.iweg_conz : M.s.iweg, M.m.nk_ptr,
-                       M.m.nk_el, M.m.geo, (const S *)M.s.uln, (const S *)M.s.vln, (S *)M.s.tracer[itr],
-                       (const S *)M.s.tracer_bc[itr], M.nkn, M.nel, M.d_err,
-                       M.mem_istot_on ? M.cur_mem_istot : nullptr, M.d_node_mem, isact);
+                       M.m.nk_el, M.m.geo, (const S *)M.s.uln, (const S *)M.s.vln, (S *)M.s.tracer[ts.itr[0]],
+                       (const S *)M.s.tracer_bc[ts.itr[0]], M.nkn, M.nel, M.d_err,
+                       M.mem_istot_on ? M.cur_mem_istot : nullptr, M.d_node_mem, isact, (S *)M.s.tracer[t1],
+                       (const S *)M.s.tracer_bc[t1]);
     HIPCHK(M, hipGetLastError());
     return 0;
 }
 
+int launch_bndo(Model &M, const TracerSet &ts, hipStream_t st, int isact) {
+    return M.prec == SHYHIP_FP64 ? bndo_t<double>(M, ts, st, isact) : bndo_t<float>(M, ts, st, isact);
+}
+
 int launch_bndo(Model &M, int itr, hipStream_t st, int isact) {
-    return M.prec == SHYHIP_FP64 ? bndo_t<double>(M, itr, st, isact) : bndo_t<float>(M, itr, st, isact);
+    TracerSet ts;
+    ts.n = 1;
+    ts.itr[0] = itr;
+    return launch_bndo(M, ts, st, isact);
+}
+
+int launch_conz3d(Model &M, const TracerSet &ts, int istot, int isact, hipStream_t st, void *hbuf) {
+    if (ts.n == 2)
+        return M.prec == SHYHIP_FP64 ? conz_t<double, 2>(M, ts, istot, isact, st, hbuf)
+                                     : conz_t<float, 2>(M, ts, istot, isact, st, hbuf);
+    return M.prec == SHYHIP_FP64 ? conz_t<double, 1>(M, ts, istot, isact, st, hbuf)
+                                 : conz_t<float, 1>(M, ts, istot, isact, st, hbuf);
+}
+
+int launch_conz3d(Model &M, int itr, int istot, int isact, int clip_small, hipStream_t st, void *hbuf,
+                  const void *cbound) {
+    TracerSet ts;
+    ts.n = 1;
+    ts.itr[0] = itr;
+    ts.clip[0] = clip_small;
+    ts.cbound[0] = cbound;
+    return launch_conz3d(M, ts, istot, isact, st, hbuf);
 }
 
 // ------------------------------------------------------------------ stability
@@ -721,12 +767,6 @@ static int stab_t(Model &M, float rstol, int *istot, double *rindex) {
 
 int launch_scalar_stability(Model &M, float rstol, int *istot, double *rindex) {
     return M.prec == SHYHIP_FP64 ? stab_t<double>(M, rstol, istot, rindex) : stab_t<float>(M, rstol, istot, rindex);
-}
-
-int launch_conz3d(Model &M, int itr, int istot, int isact, int clip_small, hipStream_t st, void *hbuf,
-                  const void *cbound) {
-    return M.prec == SHYHIP_FP64 ? conz_t<double>(M, itr, istot, isact, clip_small, st, hbuf, cbound)
-                                 : conz_t<float>(M, itr, istot, isact, clip_small, st, hbuf, cbound);
 }
 
 }  // namespace shy

@@ -224,7 +224,7 @@ struct Model {
     std::vector<hipStream_t> tr_stream;
     std::vector<hipEvent_t> tr_done;
     hipEvent_t ev_hydro = nullptr;
-    std::vector<void *> tr_hbuf;
+    std::vector<void *> grp_hbuf;     // conz3d element terms of each tracer group
     std::string err;
     // host-side copies needed for init / plugin / checks
     std::vector<int> h_nen3v, h_rowptr, h_cols, h_ijp, h_color, h_ilhv, h_ilhkv, h_ibnd, h_ibtyp, h_iextb;
@@ -369,9 +369,20 @@ int solver_setup(Model &M);
 // read-back returns the solver state, d_err[0] and the changed-element
 // count (in h_pinned[0..1]).
 int solver_solve(Model &M, int *iters, int *refinements, int wetdry_bnew = -1);
+// the tracers one conz3d launch transports together (1 or 2): slots, whether
+// small values are clipped (not temperature), source concentrations (device,
+// may be null)
+struct TracerSet {
+    int n = 1;
+    int itr[2] = {0, 0};
+    int clip[2] = {0, 0};
+    const void *cbound[2] = {nullptr, nullptr};
+};
+int launch_conz3d(Model &M, const TracerSet &ts, int istot, int isact, hipStream_t st, void *hbuf);
 int launch_conz3d(Model &M, int itr, int istot, int isact, int clip_small, hipStream_t st, void *hbuf,
                   const void *cbound = nullptr);
 int launch_bndo(Model &M, int itr, hipStream_t st, int isact = 1);
+int launch_bndo(Model &M, const TracerSet &ts, hipStream_t st, int isact);
 // scalar_stability on the current state: istot and rindex of each member
 // (one entry per member when the model holds several, else one)
 // (synchronises)
