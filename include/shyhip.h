@@ -64,6 +64,9 @@ typedef struct {
     /* scalars */
     float rkpar, difmol;
     int itvd, itvdv;
+    /* transport kernels */
+    int node_tile;                              /* nodes per workgroup of the conz3d node phase:
+                                                   0 = auto, else a power of two <= 256 */
 } shyhip_params;
 
 /* field ids for shyhip_push / shyhip_pull */

@@ -247,6 +247,7 @@ int shyhip_create(shyhip_model **out, int precision, int nkn, int nel, int nlvdi
     p.hzmin = 0.01f; p.hzoff = 0.05f; p.hzon = 0.10f; p.volmin = 1.f; p.iclin = 0;
     p.rtol = (double)1.0e-8f; p.atol = (double)1.0e-8f; p.itermax = 1000; p.refine_rtol = 1e-10;
     p.block_rows = 0; p.tri_sweeps = 0; p.inner_rtol = 1e-5; p.rkpar = 0.f; p.difmol = 1.0e-6f; p.itvd = 0; p.itvdv = 0;
+    p.node_tile = 0;
     const size_t ne = nel, nk = nkn, L = nlvdi;
     State &s = M.s;
     int rc = 0;
@@ -361,6 +362,8 @@ static int check_params(Model *M, const shyhip_params *p) {
     if (p->itvd < 0 || p->itvd > 2 || p->itvdv < 0 || p->itvdv > 1) return fail(M, "itvd must be 0 or 2, itvdv 0 or 1");
     if (p->block_rows < 0 || p->block_rows > kMaxBlockRows) return fail(M, "block_rows must be in 0..1024");
     if (p->tri_sweeps < 0 || p->tri_sweeps > 64) return fail(M, "tri_sweeps must be in 0..64");
+    if (p->node_tile < 0 || p->node_tile > 256 || (p->node_tile & (p->node_tile - 1)) != 0)
+        return fail(M, "node_tile must be 0 or a power of two <= 256");
     if (!(p->inner_rtol > 0. && p->inner_rtol < 1.)) return fail(M, "inner_rtol must be in (0,1)");
     return 0;
 }

@@ -59,6 +59,7 @@ struct ConzArgs {
     const int *mem_istot, *node_mem, *elem_mem;
     int isact;
     double dtp;
+    int node_tile;                   // nodes per workgroup of the node phase
 };
 
 template <typename S>
@@ -96,7 +97,7 @@ template <typename S, int NT>
 __global__ __launch_bounds__(kThreads) void k_conz_elem(ConzArgs<S> a) {
     using R = S;
     __shared__ ElemTileLds T;
-    const int nel = a.nel, nkn = a.nkn, e0 = blockIdx.x * kElemTile;
+    const int nel = a.nel, nkn = a.nkn, e0 = xcd_block() * kElemTile;
     const int ne = min(kElemTile, nel - e0);
     const bool use_tvd = a.itvd > 0;
     __shared__ int lmax_tile;
@@ -273,18 +274,29 @@ __device__ inline double vflux_win(int lp, int l, int lmax, double w, const doub
     return w * conf;
 }
 
-template <int NL>
-struct NodeTile {
-    static constexpr int TN = NL <= 16 ? 64 : 32;    // nodes per workgroup
-};
+// Nodes per workgroup of the node phase (a.node_tile, a power of two).  The
+// default was the fastest measured on the 12-layer lagoon (tiles of 8 to
+// 64 nodes): phase 2's serial column solves are then shared by the most
+// nodes for the LDS a workgroup holds.
+static int conz_node_tile(int nlv, int requested) {
+    if (requested > 0) return requested;
+    return nlv <= 16 ? 64 : 32;
+}
+// LDS of one node-phase workgroup: right-hand sides of NT tracers and the
+// three diagonals of the column matrix, [NL][TN] each
+static size_t conz_node_lds(int NL, int NT, int TN) { return (size_t)(NT + 3) * NL * TN * sizeof(double); }
 
 template <typename S, int NL, int NT>
 __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
     using R = S;
-    constexpr int TN = NodeTile<NL>::TN;
-    __shared__ double cnL[NT][NL][TN], cdL[NL][TN], cloL[NL][TN], chiL[NL][TN];
+    const int TN = a.node_tile;
+    extern __shared__ double node_lds[];
+    double *const cnP = node_lds, *const cdP = cnP + NT * NL * TN, *const cloP = cdP + NL * TN,
+                  *const chiP = cloP + NL * TN;
+    // entry (layer index l0, node kk of the tile) of a [NL][TN] array
+    auto at = [TN](double *p, int l0, int kk) -> double & { return p[l0 * TN + kk]; };
     const int nkn = a.nkn, nel = a.nel;
-    const int k0 = blockIdx.x * TN;
+    const int k0 = xcd_block() * TN;
     const double ad = a.ad, aa = a.aa, adt = 1. - a.ad, aat = 1. - a.aa;
     const bool btvdv = a.itvdv > 0;
     const int nlv = min(a.nlv, NL);
@@ -422,8 +434,8 @@ __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
             }
         }
 #pragma unroll
-        for (int t = 0; t < NT; t++) cnL[t][l - 1][kk] = cn[t];
-        cdL[l - 1][kk] = cd; cloL[l - 1][kk] = clo; chiL[l - 1][kk] = chi;
+        for (int t = 0; t < NT; t++) at(cnP + t * NL * TN, l - 1, kk) = cn[t];
+        at(cdP, l - 1, kk) = cd; at(cloP, l - 1, kk) = clo; at(chiP, l - 1, kk) = chi;
     }
     __syncthreads();
 
@@ -433,33 +445,36 @@ __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
     if (kk >= TN || k >= nkn) return;
     if (a.mem_istot && a.isact > a.mem_istot[a.node_mem[k]]) return;
     const int lmaxk = a.ilhkv[k];
+    auto cn = [&](int t, int l0) -> double & { return at(cnP + t * NL * TN, l0, kk); };
+    auto cd = [&](int l0) -> double & { return at(cdP, l0, kk); };
+    auto clo = [&](int l0) -> double & { return at(cloP, l0, kk); };
+    auto chi = [&](int l0) -> double & { return at(chiP, l0, kk); };
     bool bad = false;
     if ((aa == 0. && ad == 0.) || a.nlv == 1) {
         for (int l = 1; l <= lmaxk; l++)
-            if (cdL[l - 1][kk] != 0.)
+            if (cd(l - 1) != 0.)
 #pragma unroll
-                for (int t = 0; t < NT; t++) cnL[t][l - 1][kk] = cnL[t][l - 1][kk] / cdL[l - 1][kk];
+                for (int t = 0; t < NT; t++) cn(t, l - 1) = cn(t, l - 1) / cd(l - 1);
     } else {
-        double aux = 1. / cdL[0][kk];
-        chiL[0][kk] *= aux;
+        double aux = 1. / cd(0);
+        chi(0) *= aux;
 #pragma unroll
-        for (int t = 0; t < NT; t++) cnL[t][0][kk] *= aux;
+        for (int t = 0; t < NT; t++) cn(t, 0) *= aux;
         for (int l = 2; l <= lmaxk; l++) {
-            if (cdL[l - 1][kk] == 0.) bad = true;
-            aux = 1. / (cdL[l - 1][kk] - cloL[l - 1][kk] * chiL[l - 2][kk]);
-            chiL[l - 1][kk] *= aux;
+            if (cd(l - 1) == 0.) bad = true;
+            aux = 1. / (cd(l - 1) - clo(l - 1) * chi(l - 2));
+            chi(l - 1) *= aux;
 #pragma unroll
-            for (int t = 0; t < NT; t++)
-                cnL[t][l - 1][kk] = (cnL[t][l - 1][kk] - cloL[l - 1][kk] * cnL[t][l - 2][kk]) * aux;
+            for (int t = 0; t < NT; t++) cn(t, l - 1) = (cn(t, l - 1) - clo(l - 1) * cn(t, l - 2)) * aux;
         }
 #pragma unroll
         for (int t = 0; t < NT; t++)
-            for (int l = lmaxk - 1; l >= 1; l--) cnL[t][l - 1][kk] = cnL[t][l - 1][kk] - cnL[t][l][kk] * chiL[l - 1][kk];
+            for (int l = lmaxk - 1; l >= 1; l--) cn(t, l - 1) = cn(t, l - 1) - cn(t, l) * chi(l - 1);
     }
     if (bad) { atomicCAS(a.err, 0, ERR_CONZ); return; }
 #pragma unroll
     for (int t = 0; t < NT; t++)
-        for (int l = 1; l <= a.nlvdi; l++) a.c[t][(size_t)(l - 1) * nkn + k] = (S)(l <= lmaxk ? cnL[t][l - 1][kk] : 0.);
+        for (int l = 1; l <= a.nlvdi; l++) a.c[t][(size_t)(l - 1) * nkn + k] = (S)(l <= lmaxk ? cn(t, l - 1) : 0.);
 }
 
 template <typename S, int NL, int NT>
@@ -494,11 +509,15 @@ static int conz_nl(Model &M, const TracerSet &ts, int istot, int isact, hipStrea
     }
     a.hbuf = (S *)hbuf;
     a.err = M.d_err;
+    a.node_tile = conz_node_tile(M.nlv, M.p.node_tile);
     const int ge = (M.nel + kElemTile - 1) / kElemTile;
     launch_timed(M, T_CONZ_ELEM, k_conz_elem<S, NT>, dim3(ge), dim3(kThreads), 0, st, a);
     HIPCHK(M, hipGetLastError());
-    launch_timed(M, T_CONZ_NODE, k_conz_node<S, NL, NT>, dim3((M.nkn + NodeTile<NL>::TN - 1) / NodeTile<NL>::TN),
-                 dim3(kThreads), 0, st, a);
+    const int tn = a.node_tile;
+    if (conz_node_lds(NL, NT, tn) > 65536)
+        return hip_check(M, hipErrorInvalidValue, "conz3d: node_tile too large for the LDS of one workgroup");
+    launch_timed(M, T_CONZ_NODE, k_conz_node<S, NL, NT>, dim3((M.nkn + tn - 1) / tn),
+                 dim3(std::min(kThreads, (tn * M.nlv + 63) / 64 * 64)), conz_node_lds(NL, NT, tn), st, a);
     HIPCHK(M, hipGetLastError());
     return 0;
 }
@@ -653,21 +672,41 @@ int launch_conz3d(Model &M, int itr, int istot, int isact, int clip_small, hipSt
 // for this vertex (the larger of the last out- and in-flux seen over the
 // element's vertices up to this one, as conzstab carries them), its explicit
 // vertical diffusion and its share of the node volume; point sources add
-// their inflow.  The node-layer ratios are non-negative, so their IEEE bit
-// patterns order as unsigned integers and one atomic max finds the largest.
-// Horizontal diffusion (rkpar) is zero on the device path and adds nothing.
+// their inflow.  Horizontal diffusion (rkpar) is zero on the device path and
+// adds nothing.  The node-layer ratios are non-negative, so their IEEE bit
+// patterns order as unsigned integers; their maximum is taken across the
+// wavefront and the workgroup first (exact in any order), and each workgroup
+// then issues one atomic max per member it touches, none when the value
+// already held is not smaller.
+
+// maximum of v over the wavefront, in every lane
+__device__ inline unsigned long long wave_max_u64(unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned lo = (unsigned)__shfl_xor((int)(unsigned)v, off);
+        const unsigned hi = (unsigned)__shfl_xor((int)(unsigned)(v >> 32), off);
+        const unsigned long long o = ((unsigned long long)hi << 32) | lo;
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// atomic max that skips the atomic when the value held is already at least v
+// (a stale read is never larger than the value held, so skipping is exact)
+__device__ inline void max_into(unsigned long long *p, unsigned long long v) {
+    if (v == 0ull) return;
+    const unsigned long long cur = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (v > cur) atomicMax(p, v);
+}
+
+// the ratio of one node-layer (l, k) as the bits of a non-negative double,
+// 0 when it adds nothing
 template <typename S>
-__global__ void __launch_bounds__(kThreads) k_conz_stab(int nkn, int nel, int nlv, const int *ilhkv, const int *ilhv,
-                                                        const int *zbound, const int *nk_ptr, const int *nk_el,
-                                                        const int *nen3v, const double *geo, double az, double ad,
-                                                        double aa, const S *utn, const S *uto, const S *vtn,
-                                                        const S *vto, const S *wln, const S *difv, const S *hdeo,
-                                                        const S *hdkn, const S *hdko, const S *mflux,
-                                                        const int *node_mem, unsigned long long *dmax) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nlv * nkn) return;
-    const int l = i / nkn + 1, k = i - (l - 1) * nkn;
-    if (l > ilhkv[k] || zbound[k]) return;
+__device__ inline unsigned long long stab_ratio(int l, int k, int nkn, int nel, const int *ilhv, const int *nk_ptr,
+                                                const int *nk_el, const int *nen3v, const double *geo, double az,
+                                                double ad, double aa, const S *utn, const S *uto, const S *vtn,
+                                                const S *vto, const S *wln, const S *difv, const S *hdeo,
+                                                const S *hdkn, const S *hdko, const S *mflux) {
     const double azt = 1. - az, adt = 1. - ad, aat = 1. - aa;
     double chigh = 0., cn = 0., cdiag = 0.;
     double co = 0.;
@@ -713,7 +752,56 @@ __global__ void __launch_bounds__(kThreads) k_conz_stab(int nkn, int nel, int nl
     }
     if (cdiag > 0.) {
         const double r = ((chigh + 0.) + cn + co) / cdiag;
-        if (r > 0.) atomicMax(dmax + (node_mem ? node_mem[k] : 0), (unsigned long long)__double_as_longlong(r));
+        if (r > 0.) return (unsigned long long)__double_as_longlong(r);
+    }
+    return 0ull;
+}
+
+template <typename S>
+__global__ void __launch_bounds__(kThreads) k_conz_stab(int nkn, int nel, int nlv, const int *ilhkv, const int *ilhv,
+                                                        const int *zbound, const int *nk_ptr, const int *nk_el,
+                                                        const int *nen3v, const double *geo, double az, double ad,
+                                                        double aa, const S *utn, const S *uto, const S *vtn,
+                                                        const S *vto, const S *wln, const S *difv, const S *hdeo,
+                                                        const S *hdkn, const S *hdko, const S *mflux,
+                                                        const int *node_mem, unsigned long long *dmax) {
+    // (blocks in launch order: the layer-major index puts the work of the
+    // upper layers in the first blocks, which round-robin spreads over XCDs)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long val = 0ull;    // this node-layer's ratio as bits (0: none)
+    int key = 0;                      // its member
+    if (i < nlv * nkn) {
+        const int l = i / nkn + 1, k = i - (l - 1) * nkn;
+        if (node_mem) key = node_mem[k];
+        if (l <= ilhkv[k] && !zbound[k]) val = stab_ratio(l, k, nkn, nel, ilhv, nk_ptr, nk_el, nen3v, geo, az, ad,
+                                                         aa, utn, uto, vtn, vto, wln, difv, hdeo, hdkn, hdko, mflux);
+    }
+    // the wavefront's maximum when its node-layers belong to one member
+    // (always, for a single model); lanes of mixed waves go one by one
+    __shared__ unsigned long long wval[kThreads / 64];
+    __shared__ int wkey[kThreads / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int key0 = __shfl(key, 0);
+    if (__all(key == key0)) {
+        val = wave_max_u64(val);
+        if (lane == 0) { wval[wv] = val; wkey[wv] = key0; }
+    } else {
+        max_into(dmax + key, val);
+        if (lane == 0) { wval[wv] = 0ull; wkey[wv] = key0; }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // one atomic per distinct member of the workgroup's waves
+        for (int w = 0; w < kThreads / 64; w++) {
+            if (wval[w] == 0ull) continue;
+            unsigned long long m = wval[w];
+            for (int q = w + 1; q < kThreads / 64; q++)
+                if (wkey[q] == wkey[w] && wval[q] != 0ull) {
+                    m = wval[q] > m ? wval[q] : m;
+                    wval[q] = 0ull;
+                }
+            max_into(dmax + wkey[w], m);
+        }
     }
 }
 

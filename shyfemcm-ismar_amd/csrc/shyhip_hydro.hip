@@ -1036,7 +1036,7 @@ __global__ __launch_bounds__(kThreads) void k_hydro_vertical(HydroArgs<S> a) {
     constexpr int TN = VertTile<NL>::TN;
     __shared__ R vfL[NL][TN], vaL[NL][TN], dvL[NL][TN];
     const int nel = a.nel, nkn = a.nkn, nlvdi = a.nlvdi;
-    const int k0 = blockIdx.x * TN;
+    const int k0 = xcd_block() * TN;
     const R az = (R)a.az, azt = (R)1 - az, dt = (R)a.dt;
     const int nlv = min(a.nlv, NL);
     // phase 1: a work-item takes one node of the tile (TN divides kThreads)

@@ -265,6 +265,19 @@ struct Model {
 // The incident elements of node k held in registers (element<<2 | vertex):
 // slots s < cnt hold list entries s; padding slots hold element 0.  Entries
 // from kIncWidth on are read from the CSR list (tail()).
+// Workgroups are dispatched to the chip's 8 XCDs in turn (workgroup b to XCD
+// b % 8), and each XCD has its own L2.  xcd_block() renumbers them so that
+// each XCD takes one contiguous range of tiles: tiles that gather the same
+// data (neighbouring elements and nodes) then share one L2 instead of
+// fetching it into several.
+// A bijection of [0, nb).
+constexpr int kXcds = 8;
+__device__ inline int xcd_block(int b, int nb) {
+    const int x = b % kXcds, i = b / kXcds, per = nb / kXcds, rem = nb % kXcds;
+    return x * per + min(x, rem) + i;
+}
+__device__ inline int xcd_block() { return xcd_block(blockIdx.x, gridDim.x); }
+
 struct IncSlots {
     int pk[kIncWidth];
     int cnt = 0;

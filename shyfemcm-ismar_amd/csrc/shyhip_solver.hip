@@ -156,16 +156,10 @@ __device__ inline void store_partials(double (&vals)[NQ], double *part, int regi
         for (int q = 0; q < NQ; q++) part[((size_t)region * kRegion + q) * nparts + slot] = vals[q];
 }
 
-// Block of a workgroup.  Workgroups are dealt to the XCDs round-robin
-// (workgroup w to XCD w % 8); each XCD is given a contiguous range of
+// Block of a workgroup: xcd_block(w, n) gives each XCD a contiguous range of
 // blocks, so the blocks a block couples to (its neighbours in node order)
 // and its own factors are mostly produced on the same XCD and read from
-// its L2.  A bijection of [0, n).
-constexpr int kXcds = 8;
-__device__ inline int xcd_block(int w, int n) {
-    const int x = w % kXcds, slot = w / kXcds, per = n / kXcds, rem = n % kXcds;
-    return x * per + min(x, rem) + slot;
-}
+// its L2.
 
 // total of the partials of the previous kernel (n workgroups), computed by
 // every workgroup in the same order; the result is returned to all threads

@@ -46,6 +46,7 @@ module shyhip_capi
     real(c_double) :: inner_rtol
     real(c_float)  :: rkpar, difmol
     integer(c_int) :: itvd, itvdv
+    integer(c_int) :: node_tile
   end type shyhip_params
 
   ! wind conversion settings of shyhip_set_wind_series
