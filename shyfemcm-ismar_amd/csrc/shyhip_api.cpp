@@ -296,11 +296,15 @@ int shyhip_destroy(shyhip_model *h) {
     for (void *t : s.tracer_bc) dfree(t);
     dfree(M.m.bndo_k); dfree(M.m.bndo_ptr); dfree(M.m.bndo_nb); dfree(M.m.bndo_nx); dfree(M.m.bndo_ny);
     dfree(M.m.bndo_w);
+    for (void *q : {(void *)M.m.bndo_open, (void *)M.m.bndo_lmax, (void *)M.m.bndo_eptr, (void *)M.m.bndo_el,
+                    (void *)M.m.bndo_elev, (void *)M.m.bndo_nblev, (void *)M.m.bndo_eaj})
+        dfree(q);
     Mesh &m = M.m;
     for (void *q : {(void *)m.nen3v, (void *)m.geo, (void *)m.ang, (void *)m.dis, (void *)m.ilhv, (void *)m.ilhkv,
                     m.hm3v, m.hev, m.hlv, m.hldv, m.areakv, (void *)m.ibnd, (void *)m.zbound, (void *)m.iextb,
                     (void *)m.ibtyp, m.fcorv, m.czv, (void *)m.nk_ptr, (void *)m.nk_el, (void *)m.nkc_el,
-                    (void *)m.inc_cnt, (void *)m.inc_el, (void *)m.inc_elc,
+                    (void *)m.inc_cnt, (void *)m.inc_el, (void *)m.inc_elc, (void *)m.slot_of, (void *)m.slot_lev,
+                    (void *)m.slot_aj4,
                     (void *)m.rowptr, (void *)m.cols, (void *)m.ijp, (void *)m.bnd_nodes, (void *)m.bnd_id,
                     (void *)m.blk_rowptr, (void *)m.blk_cols, (void *)m.blk_src, (void *)m.blk_diag, (void *)m.blk_cp_ptr,
                     (void *)m.blk_cp_a, (void *)m.blk_cp_b, (void *)m.blk_jd, (void *)m.blk_pcol, (void *)m.blk_prp,
@@ -519,14 +523,23 @@ int shyhip_set_basin(shyhip_model *h, const int *nen3v, const float *xgv, const 
                 el[pos[k]++] = (ie << 2) | ii;
             }
         // colour order: colours ascending, elements ascending within a colour
+        std::vector<int> sof(3 * (size_t)nel), slev(3 * (size_t)nel);
+        std::vector<double> saj4(3 * (size_t)nel);
         for (int c = 1; c <= M.ncolor; c++)
             for (int ie = 0; ie < nel; ie++) {
                 if (M.h_color[ie] != c) continue;
                 for (int ii = 0; ii < 3; ii++) {
                     const int k = M.h_nen3v[3 * ie + ii];
-                    elc[pos2[k]++] = (ie << 2) | ii;
+                    const int q = pos2[k]++;
+                    elc[q] = (ie << 2) | ii;
+                    sof[(size_t)ii * nel + ie] = q;
+                    slev[q] = ilhv[ie];
+                    saj4[q] = 4. * M.h_ev[19 * (size_t)ie + 9];
                 }
             }
+        rc |= dalloc(M, &m.slot_of, sof.size()) || upload(M, m.slot_of, sof.data(), sof.size());
+        rc |= dalloc(M, &m.slot_lev, slev.size()) || upload(M, m.slot_lev, slev.data(), slev.size());
+        rc |= dalloc(M, &m.slot_aj4, saj4.size()) || upload(M, m.slot_aj4, saj4.data(), saj4.size());
         rc |= dalloc(M, &m.nk_ptr, nkn + 1) || upload(M, m.nk_ptr, ptr.data(), ptr.size());
         rc |= dalloc(M, &m.nk_el, el.size()) || upload(M, m.nk_el, el.data(), el.size());
         rc |= dalloc(M, &m.nkc_el, elc.size()) || upload(M, m.nkc_el, elc.data(), elc.size());
@@ -765,7 +778,43 @@ int shyhip_set_boundaries(shyhip_model *h, int nbnd, const int *ibnd, const int 
         return fail(&M, "open boundary with a single node is not permitted");
     dfree(M.m.bndo_k); dfree(M.m.bndo_ptr); dfree(M.m.bndo_nb); dfree(M.m.bndo_nx); dfree(M.m.bndo_ny);
     dfree(M.m.bndo_w);
+    for (void *q : {(void *)M.m.bndo_open, (void *)M.m.bndo_lmax, (void *)M.m.bndo_eptr, (void *)M.m.bndo_el,
+                    (void *)M.m.bndo_elev, (void *)M.m.bndo_nblev, (void *)M.m.bndo_eaj})
+        dfree(q);
     M.m.nbndo = (int)bnodes.size();
+    {
+        // static per-node data of the open-boundary treatment
+        std::vector<int> open, lmax, eptr(1, 0), el, elev, nblev;
+        std::vector<double> eaj;
+        for (int k : bnodes) {
+            open.push_back(ibtyp[ibnd[k]] == 1 ? 1 : 0);
+            lmax.push_back(M.h_ilhkv[k]);
+        }
+        std::vector<std::vector<int>> inc(bnodes.size());
+        std::vector<int> slot(nkn, -1);
+        for (size_t i = 0; i < bnodes.size(); i++) slot[bnodes[i]] = (int)i;
+        for (int ie = 0; ie < M.nel; ie++)              // element order
+            for (int ii = 0; ii < 3; ii++) {
+                const int s = slot[M.h_nen3v[3 * (size_t)ie + ii]];
+                if (s >= 0) inc[s].push_back(ie);
+            }
+        for (auto &v : inc) {
+            for (int ie : v) {
+                el.push_back(ie);
+                elev.push_back(M.h_ilhv[ie]);
+                eaj.push_back(M.h_ev[19 * (size_t)ie + 9]);
+            }
+            eptr.push_back((int)el.size());
+        }
+        for (int ko : bnb) nblev.push_back(M.h_ilhkv[ko]);
+        auto up_i = [&](int **d, const std::vector<int> &h) {
+            return dalloc(M, d, std::max<size_t>(1, h.size())) || (!h.empty() && upload(M, *d, h.data(), h.size()));
+        };
+        rc |= up_i(&M.m.bndo_open, open) || up_i(&M.m.bndo_lmax, lmax) || up_i(&M.m.bndo_eptr, eptr) ||
+              up_i(&M.m.bndo_el, el) || up_i(&M.m.bndo_elev, elev) || up_i(&M.m.bndo_nblev, nblev);
+        rc |= dalloc(M, &M.m.bndo_eaj, std::max<size_t>(1, eaj.size())) ||
+              (!eaj.empty() && upload(M, M.m.bndo_eaj, eaj.data(), eaj.size()));
+    }
     rc |= dalloc(M, &M.m.bndo_k, bnodes.size()) || upload(M, M.m.bndo_k, bnodes.data(), bnodes.size());
     rc |= dalloc(M, &M.m.bndo_ptr, bptr.size()) || upload(M, M.m.bndo_ptr, bptr.data(), bptr.size());
     rc |= dalloc(M, &M.m.bndo_nb, bnb.size()) || upload(M, M.m.bndo_nb, bnb.data(), bnb.size());

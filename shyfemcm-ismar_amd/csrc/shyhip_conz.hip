@@ -43,14 +43,16 @@ struct ConzArgs {
     const int *nen3v, *ilhv, *ilhkv, *iextb, *ibnd;
     const double *geo, *dis;
     const int *nkc_ptr, *nkc_el;
-    const int *inc_cnt, *inc_elc;    // incidence slots (colour order)
+    const int *slot_of, *slot_lev;   // colour-order list positions (see Mesh)
+    const double *slot_aj4;
+    int nslot;                       // 3 * nel
     const int *tvd_el, *tvd_kn, *tvd_lev, *elem_iext;
     const double *tvd_w;
     const S *utn, *vtn, *uto, *vto, *uln, *vln;
     const S *hdkn, *hdko, *hden, *hdeo, *wln, *difv, *mflux;
     S *c[kMaxTracers];               // tracers (in/out)
     const S *cbound[kMaxTracers];    // source concentrations (may be null)
-    S *hbuf;                         // element terms [((l-1)*NT + t)*3 + ii][ie]
+    S *hbuf;                         // element terms [(l-1)*NT + t][slot_of[ii][ie]]
     int *err;
     // ensembles whose members run different numbers of substeps: member
     // istot (null: all run istot), node/element -> member, this substep and
@@ -88,7 +90,7 @@ __device__ inline T sel3(const T (&v)[3], int i) {
 struct ElemTileLds {
     double b[3][kElemTile], c[3][kElemTile], dis[3][kElemTile];
     double tw[3][6][kElemTile];
-    int kn[3][kElemTile], tkn[3][6][kElemTile], tel[6][kElemTile], tlev[6][kElemTile];
+    int kn[3][kElemTile], so[3][kElemTile], tkn[3][6][kElemTile], tel[6][kElemTile], tlev[6][kElemTile];
     int ilev[kElemTile], iext[kElemTile];
     double ajr[kElemTile];
 };
@@ -110,6 +112,7 @@ __global__ __launch_bounds__(kThreads) void k_conz_elem(ConzArgs<S> a) {
             if (part == 0) {
                 for (int ii = 0; ii < 3; ii++) {
                     T.kn[ii][e] = a.nen3v[3 * ie + ii];
+                    T.so[ii][e] = a.slot_of[(size_t)ii * nel + ie];
                     T.b[ii][e] = a.geo[(size_t)ii * nel + ie];
                     T.c[ii][e] = a.geo[(size_t)(3 + ii) * nel + ie];
                     T.dis[ii][e] = a.dis[(size_t)ii * nel + ie];
@@ -132,13 +135,31 @@ __global__ __launch_bounds__(kThreads) void k_conz_elem(ConzArgs<S> a) {
         }
     }
     __syncthreads();
-    const int lmt = lmax_tile;
+    // the tile's element-layers that exist (members still substepping),
+    // layer by layer with the elements ascending in each, listed by the
+    // first wavefront (work-item e owns element e of the tile)
+    static_assert(kElemTile == 64, "one wavefront lists the tile");
+    __shared__ unsigned short items[kElemTile * 32];      // e | l << 8
+    __shared__ int nitems_s;
+    if (threadIdx.x < kElemTile) {
+        const int e = threadIdx.x;
+        int lev = e < ne ? T.ilev[e] : 0;
+        if (lev > 0 && a.mem_istot && a.isact > a.mem_istot[a.elem_mem[e0 + e]]) lev = 0;
+        const unsigned long long below = e == 0 ? 0ull : (~0ull >> (64 - e));
+        int off = 0;
+        for (int l = 1; l <= lmax_tile; l++) {
+            const unsigned long long m = __ballot(lev >= l);
+            if (lev >= l) items[off + __popcll(m & below)] = (unsigned short)(e | (l << 8));
+            off += __popcll(m);
+        }
+        if (e == 0) nitems_s = off;
+    }
+    __syncthreads();
+    const int nitems = nitems_s;
     const double azt = 1. - a.az;
-    for (int idx = threadIdx.x; idx < kElemTile * lmt; idx += kThreads) {
-        const int e = idx % kElemTile, l = idx / kElemTile + 1;
-        if (e >= ne || l > T.ilev[e]) continue;
+    for (int it = threadIdx.x; it < nitems; it += kThreads) {
+        const int e = items[it] & 0xff, l = items[it] >> 8;
         const int ie = e0 + e;
-        if (a.mem_istot && a.isact > a.mem_istot[a.elem_mem[ie]]) continue;
         // the layer's transports, velocities and node concentrations
         const size_t o = (size_t)(l - 1) * nel + ie;
         const S un = a.utn[o], uo = a.uto[o], vn = a.vtn[o], vo = a.vto[o];
@@ -237,7 +258,7 @@ __global__ __launch_bounds__(kThreads) void k_conz_elem(ConzArgs<S> a) {
 #pragma unroll
         for (int t = 0; t < NT; t++)
             for (int ii = 0; ii < 3; ii++)
-                a.hbuf[(((size_t)(l - 1) * NT + t) * 3 + ii) * nel + ie] = (S)(3. * fl[t][ii]);
+                a.hbuf[((size_t)(l - 1) * NT + t) * a.nslot + T.so[ii][e]] = (S)(3. * fl[t][ii]);
     }
 }
 
@@ -283,8 +304,11 @@ static int conz_node_tile(int nlv, int requested) {
     return nlv <= 16 ? 64 : 32;
 }
 // LDS of one node-phase workgroup: right-hand sides of NT tracers and the
-// three diagonals of the column matrix, [NL][TN] each
-static size_t conz_node_lds(int NL, int NT, int TN) { return (size_t)(NT + 3) * NL * TN * sizeof(double); }
+// three diagonals of the column matrix, [NL][TN] each, and the list of the
+// tile's node-layers
+static size_t conz_node_lds(int NL, int NT, int TN) {
+    return (size_t)(NT + 3) * NL * TN * sizeof(double) + (size_t)NL * TN * sizeof(int);
+}
 
 template <typename S, int NL, int NT>
 __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
@@ -295,30 +319,70 @@ __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
                   *const chiP = cloP + NL * TN;
     // entry (layer index l0, node kk of the tile) of a [NL][TN] array
     auto at = [TN](double *p, int l0, int kk) -> double & { return p[l0 * TN + kk]; };
-    const int nkn = a.nkn, nel = a.nel;
+    const int nkn = a.nkn;
     const int k0 = xcd_block() * TN;
     const double ad = a.ad, aa = a.aa, adt = 1. - a.ad, aat = 1. - a.aa;
     const bool btvdv = a.itvdv > 0;
     const int nlv = min(a.nlv, NL);
 
+    // The tile's node-layers that exist (l <= ilhkv of the node, members
+    // still substepping), layer by layer with the nodes ascending in each:
+    // phase 1 takes exactly these, so no work-item idles below a node's
+    // bottom and a layer's loads stay in node order.  Work-item kk < TN owns
+    // node kk; each wavefront counts its nodes per layer with a ballot.
+    constexpr int kWaves = kThreads / 64;
+    __shared__ int wpos[kWaves][NL + 1];
+    __shared__ int nitems_s;
+    int *const items = (int *)(chiP + NL * TN);     // kk | l << 16
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = (TN + 63) / 64;
+    int lmk = 0;
+    if ((int)threadIdx.x < TN && k0 + (int)threadIdx.x < nkn) {
+        const int k = k0 + threadIdx.x;
+        lmk = min(a.ilhkv[k], nlv);
+        if (a.mem_istot && a.isact > a.mem_istot[a.node_mem[k]]) lmk = 0;
+    }
+    if (wv < nwv)
+        for (int l = 1; l <= nlv; l++) {
+            const unsigned long long m = __ballot(lmk >= l);
+            if (lane == 0) wpos[wv][l] = __popcll(m);
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int off = 0;
+        for (int l = 1; l <= nlv; l++)
+            for (int w = 0; w < nwv; w++) {
+                const int c = wpos[w][l];
+                wpos[w][l] = off;
+                off += c;
+            }
+        nitems_s = off;
+    }
+    __syncthreads();
+    if (wv < nwv) {
+        const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        for (int l = 1; l <= nlv; l++) {
+            const unsigned long long m = __ballot(lmk >= l);
+            if (lmk >= l) items[wpos[wv][l] + __popcll(m & below)] = (int)threadIdx.x | (l << 16);
+        }
+    }
+    __syncthreads();
+    const int nitems = nitems_s;
+
     // phase 1: one work-item per node-layer
-    for (int idx = threadIdx.x; idx < TN * nlv; idx += blockDim.x) {
-        const int kk = idx % TN, l = idx / TN + 1, k = k0 + kk;
-        if (k >= nkn) continue;
+    for (int it = threadIdx.x; it < nitems; it += blockDim.x) {
+        const int kk = items[it] & 0xffff, l = items[it] >> 16, k = k0 + kk;
         const int lmaxk = a.ilhkv[k];
-        if (l > lmaxk) continue;
         double dt = a.dt, rso = a.rso, rsn = a.rsn;
         if (a.mem_istot) {
             const int ist = a.mem_istot[a.node_mem[k]];
-            if (a.isact > ist) continue;
             const double rstot = ist;
             rso = (a.isact - 1) / rstot;
             rsn = a.isact / rstot;
             dt = a.dtp / rstot;
         }
         const double rsot = 1. - rso, rsnt = 1. - rsn;
-        IncSlots inc;     // incident elements in colour order
-        inc.load(a.inc_cnt, a.inc_elc, nkn, k);
+        // incident elements in colour order: list positions q0 .. q0+cnt-1
+        const int q0 = a.nkc_ptr[k], cnt = a.nkc_ptr[k + 1] - q0;
         // node column around l (zero outside 1..lmaxk)
         double ckw[NT][5], hokw[5], hnkw[5];
         for (int j = 0; j < 5; j++) {
@@ -399,29 +463,28 @@ __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
             chi += aj4 * dt * clpv[v];
             cd += aj4 * dt * clcv[v] + aj4 * (1. + dt * 0.) * hnc;
         };
+        // the first kIncWidth positions are read unconditionally (clamped
+        // to the list), contiguous for the node; the rest one by one
         int slv[kIncWidth];
         double saj[kIncWidth], shf[kIncWidth][NT];
+        const size_t hl = (size_t)(l - 1) * NT * a.nslot;
 #pragma unroll
         for (int s = 0; s < kIncWidth; s++) {
-            const int ie = inc.pk[s] >> 2, ii = inc.pk[s] & 3;
-            slv[s] = a.ilhv[ie];
-            saj[s] = 4. * a.geo[(size_t)6 * nel + ie];
+            const int q = min(q0 + s, a.nslot - 1);
+            slv[s] = a.slot_lev[q];
+            saj[s] = a.slot_aj4[q];
 #pragma unroll
-            for (int t = 0; t < NT; t++)
-                shf[s][t] = (double)a.hbuf[(((size_t)(l - 1) * NT + t) * 3 + ii) * nel + ie];
+            for (int t = 0; t < NT; t++) shf[s][t] = (double)a.hbuf[hl + (size_t)t * a.nslot + q];
         }
 #pragma unroll
         for (int s = 0; s < kIncWidth; s++)
-            if (s < inc.cnt) add(slv[s], saj[s], shf[s]);
-        if (inc.cnt > kIncWidth)
-            for (int q = a.nkc_ptr[k] + kIncWidth; q < a.nkc_ptr[k + 1]; q++) {
-                const int pk = a.nkc_el[q];
-                const int ie = pk >> 2, ii = pk & 3;
-                double hf[NT];
+            if (s < cnt) add(slv[s], saj[s], shf[s]);
+        for (int q = q0 + kIncWidth; q < q0 + cnt; q++) {
+            double hf[NT];
 #pragma unroll
-                for (int t = 0; t < NT; t++) hf[t] = (double)a.hbuf[(((size_t)(l - 1) * NT + t) * 3 + ii) * nel + ie];
-                add(a.ilhv[ie], 4. * a.geo[(size_t)6 * nel + ie], hf);
-            }
+            for (int t = 0; t < NT; t++) hf[t] = (double)a.hbuf[hl + (size_t)t * a.nslot + q];
+            add(a.slot_lev[q], a.slot_aj4[q], hf);
+        }
         // conz3d_nodes: boundary mass flux
         if (a.mflux) {
             const size_t o = (size_t)(l - 1) * nkn + k;
@@ -456,20 +519,40 @@ __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
 #pragma unroll
                 for (int t = 0; t < NT; t++) cn(t, l - 1) = cn(t, l - 1) / cd(l - 1);
     } else {
-        double aux = 1. / cd(0);
-        chi(0) *= aux;
+        // the elimination's chain runs through registers (chip, cnp); LDS
+        // supplies the coefficients and keeps the forward results for the
+        // back substitution
+        const double aux0 = 1. / cd(0);
+        double chip = chi(0) * aux0, cnp[NT];
+        chi(0) = chip;
 #pragma unroll
-        for (int t = 0; t < NT; t++) cn(t, 0) *= aux;
+        for (int t = 0; t < NT; t++) {
+            cnp[t] = cn(t, 0) * aux0;
+            cn(t, 0) = cnp[t];
+        }
         for (int l = 2; l <= lmaxk; l++) {
-            if (cd(l - 1) == 0.) bad = true;
-            aux = 1. / (cd(l - 1) - clo(l - 1) * chi(l - 2));
-            chi(l - 1) *= aux;
+            const double cdl = cd(l - 1), clol = clo(l - 1), chil = chi(l - 1);
+            double cnl[NT];
 #pragma unroll
-            for (int t = 0; t < NT; t++) cn(t, l - 1) = (cn(t, l - 1) - clo(l - 1) * cn(t, l - 2)) * aux;
+            for (int t = 0; t < NT; t++) cnl[t] = cn(t, l - 1);
+            if (cdl == 0.) bad = true;
+            const double aux = 1. / (cdl - clol * chip);
+            chip = chil * aux;
+            chi(l - 1) = chip;
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                cnp[t] = (cnl[t] - clol * cnp[t]) * aux;
+                cn(t, l - 1) = cnp[t];
+            }
         }
 #pragma unroll
-        for (int t = 0; t < NT; t++)
-            for (int l = lmaxk - 1; l >= 1; l--) cn(t, l - 1) = cn(t, l - 1) - cn(t, l) * chi(l - 1);
+        for (int t = 0; t < NT; t++) {
+            double nx = cnp[t];
+            for (int l = lmaxk - 1; l >= 1; l--) {
+                nx = cn(t, l - 1) - nx * chi(l - 1);
+                cn(t, l - 1) = nx;
+            }
+        }
     }
     if (bad) { atomicCAS(a.err, 0, ERR_CONZ); return; }
 #pragma unroll
@@ -489,7 +572,8 @@ static int conz_nl(Model &M, const TracerSet &ts, int istot, int isact, hipStrea
     a.az = M.p.azpar; a.ad = M.p.adpar; a.aa = M.p.aapar; a.difmol = M.p.difmol;
     a.nen3v = M.m.nen3v; a.ilhv = M.m.ilhv; a.ilhkv = M.m.ilhkv; a.iextb = M.m.iextb; a.ibnd = M.m.ibnd;
     a.geo = M.m.geo; a.dis = M.m.dis;
-    a.nkc_ptr = M.m.nk_ptr; a.nkc_el = M.m.nkc_el; a.inc_cnt = M.m.inc_cnt; a.inc_elc = M.m.inc_elc;
+    a.nkc_ptr = M.m.nk_ptr; a.nkc_el = M.m.nkc_el;
+    a.slot_of = M.m.slot_of; a.slot_lev = M.m.slot_lev; a.slot_aj4 = M.m.slot_aj4; a.nslot = 3 * M.nel;
     a.tvd_el = M.m.tvd_el; a.tvd_kn = M.m.tvd_kn; a.tvd_w = (const double *)M.m.tvd_w;
     a.tvd_lev = M.m.tvd_lev; a.elem_iext = M.m.elem_iext;
     a.utn = (const S *)M.s.utn; a.vtn = (const S *)M.s.vtn; a.uto = (const S *)M.s.uto; a.vto = (const S *)M.s.vto;
@@ -543,40 +627,52 @@ static int conz_t(Model &M, const TracerSet &ts, int istot, int isact, hipStream
 // neighbours.  One wavefront per open-boundary node, one lane per layer; a
 // layer with no inner neighbour deep enough takes the final value of the
 // layer above, resolved top-down after the lanes finish.  Each wavefront
-// writes only its own column and reads only inner nodes.
+// writes only its own column and reads only inner nodes.  The node's static
+// data (open flag, bottom layer, incident elements in element order with
+// their bottom layers and areas, the inner neighbours' bottom layers) comes
+// from tables built at set-up, so a lane's loads depend on one another at
+// most two deep.
 constexpr int kBndoLanes = 64;
 
 template <typename S>
-__global__ __launch_bounds__(kBndoLanes) void k_bndo(int nbndo, const int *bk, const S *bnx, const S *bny,
-                                                    const int *bptr, const int *bnb, const S *bw, const int *ibnd,
-                                                    const int *ibtyp, const int *ilhkv, const int *ilhv,
-                                                    const int *iweg, const int *nk_ptr, const int *nk_el,
-                                                    const double *geo, const S *uln, const S *vln, S *cv,
-                                                    const S *rbc, int nkn, int nel, int *err,
-                                                    const int *mem_istot, const int *node_mem, int isact,
-                                                    S *cv1, const S *rbc1) {
+struct BndoArgs {
+    int nbndo, nkn, nel;
+    const int *bk, *open, *lmax, *eptr, *el, *elev, *bptr, *bnb, *nblev;
+    const double *eaj;
+    const S *bnx, *bny, *bw, *uln, *vln;
+    const int *iweg;
+    S *cv[2];               // the tracers of the launch (grid row 0 and 1)
+    const S *rbc[2];        // their boundary values (may be null)
+    int *err;
+    const int *mem_istot, *node_mem;
+    int isact;
+};
+
+template <typename S>
+__global__ __launch_bounds__(kBndoLanes) void k_bndo(BndoArgs<S> a) {
     using R = S;
     __shared__ R val[kBndoLanes];
     __shared__ int status[kBndoLanes];
-    if (blockIdx.y == 1) { cv = cv1; rbc = rbc1; }      // the second tracer of a pair
     const int i = blockIdx.x;
-    if (i >= nbndo) return;
-    const int k = bk[i];
-    if (ibtyp[ibnd[k]] != 1) return;
-    if (mem_istot && isact > mem_istot[node_mem[k]]) return;
+    if (i >= a.nbndo || !a.open[i]) return;
+    const int k = a.bk[i];
+    if (a.mem_istot && a.isact > a.mem_istot[a.node_mem[k]]) return;
+    S *const cv = a.cv[blockIdx.y];
+    const S *const rbc = a.rbc[blockIdx.y];
     const int lane = threadIdx.x, l = lane + 1;
-    const int lmax = ilhkv[k];
+    const int lmax = a.lmax[i];
+    const int nkn = a.nkn, nel = a.nel;
     const R flag = (R)-999.;
     status[lane] = 0;
     if (l <= lmax) {
         R vv = 0, up = 0, vp = 0;
-        for (int q = nk_ptr[k]; q < nk_ptr[k + 1]; q++) {
-            const int ie = nk_el[q] >> 2;
-            if (iweg[ie] != 0 || l > ilhv[ie]) continue;
-            const R aj = (R)geo[(size_t)6 * nel + ie];
+        for (int q = a.eptr[i]; q < a.eptr[i + 1]; q++) {
+            const int ie = a.el[q];
+            if (a.iweg[ie] != 0 || l > a.elev[q]) continue;
+            const R aj = (R)a.eaj[q];
             vv = vv + aj;
-            up = up + aj * uln[(size_t)(l - 1) * nel + ie];
-            vp = vp + aj * vln[(size_t)(l - 1) * nel + ie];
+            up = up + aj * a.uln[(size_t)(l - 1) * nel + ie];
+            vp = vp + aj * a.vln[(size_t)(l - 1) * nel + ie];
         }
         const size_t o = (size_t)(l - 1) * nkn + k;
         R cur = cv[o];
@@ -586,21 +682,20 @@ __global__ __launch_bounds__(kBndoLanes) void k_bndo(int nbndo, const int *bk, c
         }
         R u = up, v = vp;
         if (vv > (R)0) { u = u / vv; v = v / vv; }
-        const R scal = bnx[i] * u + bny[i] * v;
+        const R scal = a.bnx[i] * u + a.bny[i] * v;
         const bool binside = scal <= (R)0 || cur == flag;
         R bc = 0, tw = 0;
         int ntbc = 0;
         if (binside) {
-            for (int j = bptr[i]; j < bptr[i + 1]; j++) {
-                const int ko = bnb[j];
-                if (l <= ilhkv[ko]) {
+            for (int j = a.bptr[i]; j < a.bptr[i + 1]; j++) {
+                if (l <= a.nblev[j]) {
                     ntbc++;
-                    tw = tw + bw[j];
-                    bc = bc + bw[j] * cv[(size_t)(l - 1) * nkn + ko];
+                    tw = tw + a.bw[j];
+                    bc = bc + a.bw[j] * cv[(size_t)(l - 1) * nkn + a.bnb[j]];
                 }
             }
         } else {
-            ntbc = bptr[i + 1] - bptr[i];
+            ntbc = a.bptr[i + 1] - a.bptr[i];
             tw = 1;
             bc = cur;
         }
@@ -611,7 +706,7 @@ __global__ __launch_bounds__(kBndoLanes) void k_bndo(int nbndo, const int *bk, c
     if (lane == 0) {
         for (int m = 1; m < lmax && m < kBndoLanes; m++)
             if (status[m] == 2) val[m] = val[m - 1];
-        if (status[0] == 3) atomicCAS(err, 0, ERR_CONZ);
+        if (status[0] == 3) atomicCAS(a.err, 0, ERR_CONZ);
     }
     __syncthreads();
     if (l <= lmax && status[0] != 3) cv[(size_t)(l - 1) * nkn + k] = val[lane];
@@ -621,15 +716,23 @@ template <typename S>
 static int bndo_t(Model &M, const TracerSet &ts, hipStream_t st, int isact) {
     if (M.m.nbndo == 0) return 0;
     if (M.nlv > kBndoLanes) return hip_check(M, hipErrorInvalidValue, "bndo: more layers than lanes");
-    const int t1 = ts.n > 1 ? ts.itr[1] : ts.itr[0];
-    hipLaunchKernelGGL(k_bndo<S>, dim3(M.m.nbndo, ts.n), dim3(kBndoLanes), 0, st, M.m.nbndo, M.m.bndo_k,
-                       (const S *)M.m.bndo_nx, (const S *)M.m.bndo_ny, M.m.bndo_ptr, M.m.bndo_nb,
-                       (const S *)M.m.bndo_w, M.m.ibnd, M.m.ibtyp, M.m.ilhkv, M.m.ilhv,
-                       M.iweg_conz ? M.iweg_conz : M.s.iweg, M.m.nk_ptr,
-                       M.m.nk_el, M.m.geo, (const S *)M.s.uln, (const S *)M.s.vln, (S *)M.s.tracer[ts.itr[0]],
-                       (const S *)M.s.tracer_bc[ts.itr[0]], M.nkn, M.nel, M.d_err,
-                       M.mem_istot_on ? M.cur_mem_istot : nullptr, M.d_node_mem, isact, (S *)M.s.tracer[t1],
-                       (const S *)M.s.tracer_bc[t1]);
+    BndoArgs<S> a{};
+    a.nbndo = M.m.nbndo; a.nkn = M.nkn; a.nel = M.nel;
+    a.bk = M.m.bndo_k; a.open = M.m.bndo_open; a.lmax = M.m.bndo_lmax; a.eptr = M.m.bndo_eptr; a.el = M.m.bndo_el;
+    a.elev = M.m.bndo_elev; a.bptr = M.m.bndo_ptr; a.bnb = M.m.bndo_nb; a.nblev = M.m.bndo_nblev;
+    a.eaj = M.m.bndo_eaj;
+    a.bnx = (const S *)M.m.bndo_nx; a.bny = (const S *)M.m.bndo_ny; a.bw = (const S *)M.m.bndo_w;
+    a.uln = (const S *)M.s.uln; a.vln = (const S *)M.s.vln;
+    a.iweg = M.iweg_conz ? M.iweg_conz : M.s.iweg;
+    for (int t = 0; t < 2; t++) {
+        const int itr = ts.itr[t < ts.n ? t : 0];
+        a.cv[t] = (S *)M.s.tracer[itr];
+        a.rbc[t] = (const S *)M.s.tracer_bc[itr];
+    }
+    a.err = M.d_err;
+    if (M.mem_istot_on) { a.mem_istot = M.cur_mem_istot; a.node_mem = M.d_node_mem; }
+    a.isact = isact;
+    hipLaunchKernelGGL(k_bndo<S>, dim3(M.m.nbndo, ts.n), dim3(kBndoLanes), 0, st, a);
     HIPCHK(M, hipGetLastError());
     return 0;
 }

@@ -46,6 +46,12 @@ struct Mesh {                 // device copies of init-time data
     // 0, a valid element), and the list lengths, so that a node's element
     // loads can all be issued at once
     int *inc_cnt = nullptr, *inc_el = nullptr, *inc_elc = nullptr;
+    // positions of the colour-order lists (nk_ptr / nkc_el): the position of
+    // vertex ii of element ie ([ii][ie]), and each position's element bottom
+    // layer and 4 * area (the conz3d element terms are stored by position, so
+    // a node reads its incident elements' terms contiguously)
+    int *slot_of = nullptr, *slot_lev = nullptr;
+    double *slot_aj4 = nullptr;
     int *rowptr = nullptr, *cols = nullptr, *ijp = nullptr;   // zeta CSR; ijp[ie*9+i*3+j]
     int *bnd_nodes = nullptr; int *bnd_id = nullptr; int nbnd_nodes = 0;
     // block-ILU(0)
@@ -82,6 +88,13 @@ struct Mesh {                 // device copies of init-time data
     int nbndo = 0;
     int *bndo_k = nullptr, *bndo_ptr = nullptr, *bndo_nb = nullptr;
     void *bndo_nx = nullptr, *bndo_ny = nullptr, *bndo_w = nullptr;   // S
+    // the same nodes' static data, resolved at set-up so that a launch reads
+    // it directly: open (type 1) flag, bottom layer, incident elements in
+    // element order with their bottom layers and areas, and the bottom
+    // layer of each inner neighbour
+    int *bndo_open = nullptr, *bndo_lmax = nullptr, *bndo_eptr = nullptr, *bndo_el = nullptr, *bndo_elev = nullptr;
+    int *bndo_nblev = nullptr;
+    double *bndo_eaj = nullptr;
 };
 
 struct State {
