@@ -82,10 +82,14 @@ def _kernel_bytes(b, p, brows, ntr=2):
         # visv at the interfaces, per element geometry/levels/summaries
         "hydro_intern": 9 * EL * p + (NL + N) * p + E * (12 + 56 + 6 * p + 4 + 48),
         # utlnv vtlnv utlov vtlov ulnv vlnv read, 3 horizontal terms per tracer
-        # written, each tracer gathered
-        "conz3d_elem": (6 + 3 * ntr) * EL * p + ntr * NL * p + E * (12 + 56 + 4),
-        # horizontal terms read; hdknv hdkov wlnv difv read, each tracer read and written
-        "conz3d_node": 3 * ntr * EL * p + (4 + 2 * ntr) * NL * p + 2 * N * p,
+        # written, each tracer gathered; per element: nodes, list positions,
+        # geometry (7 doubles), distances (3), levels, external flags and the
+        # TVD upwind tables (6 points: element, level, 3 nodes, 3 weights)
+        "conz3d_elem": (6 + 3 * ntr) * EL * p + ntr * NL * p + E * (12 + 12 + 56 + 24 + 4 + 4 + 264),
+        # horizontal terms read; hdknv hdkov wlnv difv read, each tracer read
+        # and written; per list position its bottom layer and 4 * area, per
+        # node its list start and bottom layer
+        "conz3d_node": 3 * ntr * EL * p + (4 + 2 * ntr) * NL * p + 2 * N * p + 3 * E * 12 + 2 * N * 4,
         "transports_final": 8 * EL * p + E * (12 + 56 + 3 * p + 2 * p),
         "hydro_vertical": 4 * EL * p + 4 * NL * p + N * p,
         "setdepth": EL * p + NL * p + E * 3 * p,
@@ -153,10 +157,10 @@ def install_forcing(m, nsteps):
     m.set_wind_series([0.0], [np.float32(sp * math.cos(d))], [np.float32(sp * math.sin(d))])
 
 
-def run_gpu(b, tr, prec, args, steps=None, baroclinic=False):
+def run_gpu(b, tr, prec, args, steps=None, baroclinic=False, params=None):
     import shyhip
     steps = args.steps if steps is None else steps
-    m = shyhip.Model(b, prec, params=PARAMS, visv=1e-3, difv=1e-4, tracers=tr)
+    m = shyhip.Model(b, prec, params=params or PARAMS, visv=1e-3, difv=1e-4, tracers=tr)
     m.set_deferred_checks(True)     # device errors surface at the next wait (m.sync below)
     if baroclinic:                  # T and S drive the momentum equations (ibarcl = 1)
         m.set_baroclinic("temp", "salt")
@@ -185,6 +189,88 @@ def run_gpu(b, tr, prec, args, steps=None, baroclinic=False):
     m.set_timing(False)
     m.close()
     return dict(elapsed=elapsed, timing=timing, nt=nt, znv=zn, iwegv=iweg, stats=stats)
+
+
+# the settings the autotuner chooses among (all solve and transport the
+# same way; they trade iterations and launch shapes)
+TUNE_GRID = dict(block_rows=(128, 256, 512), tri_sweeps=(3, 4), node_tile=(32, 64))
+
+
+def autotune_params(b, tr, prec, steps=20):
+    """PARAMS with the block size, triangular sweeps and node tile that run
+    the fastest on this basin (shyhip.autotune, timed on the workload
+    itself); returns (params, tuner report)."""
+    from shyhip import autotune
+
+    def forcing(m, n):
+        if n == 0:
+            install_forcing(m, 4 * steps + 40)
+        m.step(None, istot=ISTOT)
+
+    rep = autotune.tune(b, prec, PARAMS, forcing, steps=steps, warmup=5, grid=TUNE_GRID, cache=False,
+                        visv=1e-3, difv=1e-4, tracers=tr)
+    return dict(PARAMS, **rep["choice"]), rep
+
+
+def c3_run(args, params):
+    """The mixed-precision configuration at 500k nodes: the FP32 + FP64-refined
+    path and the plain FP64 path on the same basin and forcing; per-step
+    rates and the FP32 fields' differences from the FP64 run."""
+    import shyhip
+    from shyhip import mesh
+    scale = math.sqrt(args.c3_nodes / 50000.0)      # the same node density as the 50k lagoon
+    b = mesh.lagoon(args.c3_nodes, seed=args.seed + 3, lx=40000.0 * scale, ly=20000.0 * scale,
+                    nlv=args.c3_layers, tile=32)
+    tr = dict(temp=mesh.smooth_tracer(b, args.seed + 4, 15.0, 25.0),
+              salt=mesh.smooth_tracer(b, args.seed + 5, 25.0, 38.0))
+    EL = int(b.ilhv.sum())
+    out = {"workload": f"3D lagoon: {b.nkn} nodes, {b.nel} elements, {args.c3_layers} z-layers, same generator, "
+                       f"depth law, tide, wind and tracers as the headline lagoon; {args.c3_steps} steps",
+           "elem_layers": EL}
+    fields = {}
+    for prec in ("fp64", "fp32"):
+        m = shyhip.Model(b, prec, params=params[prec], visv=1e-3, difv=1e-4, tracers=tr)
+        m.set_deferred_checks(True)
+        install_forcing(m, args.c3_steps + 10)
+        for _ in range(3):
+            m.step(None, istot=ISTOT)
+        m.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.c3_steps):
+            m.step(None, istot=ISTOT)
+        m.sync()
+        el = time.perf_counter() - t0
+        out[f"{prec}_ms_per_step"] = round(1e3 * el / args.c3_steps, 4)
+        out[f"{prec}_value"] = round(EL * args.c3_steps / el, 1)
+        out[f"{prec}_solver"] = m.stats()
+        fields[prec] = {k: m.pull(k) for k in ("znv", "iwegv", "temp", "salt")}
+        fields[prec]["volume"] = m.volume()
+        if prec == "fp64":
+            m.set_timing(True)
+            for _ in range(5):
+                m.step(None, istot=ISTOT)
+            t = m.timing()
+            m.set_timing(False)
+            kb = _kernel_bytes(b, 8, params[prec]["block_rows"])
+            per_step = {k: v[0] / 5 for k, v in t.items()}
+            avg = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in t.items()}
+            dom = max((k for k in kb if avg.get(k, 0.0) > 0), key=lambda k: per_step[k])
+            ach = kb[dom] / (avg[dom] * 1e-3) / 1e9
+            out["roofline_fp64"] = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                                    "bytes_per_launch": kb[dom], "avg_launch_ms": round(avg[dom], 5)}
+            out["kernel_ms_per_step_fp64"] = {k: round(v, 4) for k, v in per_step.items() if v > 0}
+        m.close()
+    f32, f64 = fields["fp32"], fields["fp64"]
+    valid = np.arange(b.nlv)[None, :] < b.ilhkv[:, None]
+    out["fp32_vs_fp64"] = {
+        "zeta": _zeta_err(b, f32["znv"], f64["znv"], f32["iwegv"], f64["iwegv"]),
+        "temp_max_abs": float(np.max(np.abs(f32["temp"] - f64["temp"])[valid])),
+        "salt_max_abs": float(np.max(np.abs(f32["salt"] - f64["salt"])[valid])),
+        "volume_rel": abs(f32["volume"] - f64["volume"]) / abs(f64["volume"]),
+    }
+    out["fp32_over_fp64"] = round(out["fp32_value"] / out["fp64_value"], 3)
+    return out
 
 
 def _zeta_err(b, z32, z64, w32, w64):
@@ -380,6 +466,11 @@ def main():
     ap.add_argument("--srv-requests", type=int, default=12)
     ap.add_argument("--srv-max-nodes", type=int, default=100000)
     ap.add_argument("--srv-slots", type=int, default=4)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--c3-nodes", type=int, default=500000)
+    ap.add_argument("--c3-layers", type=int, default=20)
+    ap.add_argument("--c3-steps", type=int, default=30)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one plain device copy of MIB MiB (calibrates HBM counters)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "lagoon50k_pmc_traffic.md"),
@@ -392,16 +483,25 @@ def main():
 
     b, tr = _lagoon(args)
     EL = int(b.ilhv.sum())
+    # per precision: the settings the autotuner picks on this basin
+    params, tuned = {}, {}
+    for prec in ("fp64", "fp32"):
+        if args.no_autotune:
+            params[prec] = dict(PARAMS)
+        else:
+            params[prec], rep = autotune_params(b, tr, prec)
+            tuned[prec] = {"choice": rep["choice"], "ms_per_step": {
+                ",".join(f"{k}={t[k]}" for k in TUNE_GRID): round(t["ms"], 4) for t in rep["timings"]}}
     res = {}
     for prec in ("fp64", "fp32"):
-        res[prec] = run_gpu(b, tr, prec, args)
+        res[prec] = run_gpu(b, tr, prec, args, params=params[prec])
     thr = {p: EL * args.steps / r["elapsed"] for p, r in res.items()}
 
     # roofline of the headline path's dominant kernel (largest time per step).
     # The headline workload is the lagoon on the plain FP64 GPU path; the
     # FP32 + FP64-refined path runs the same workload beside it.
     head = res["fp64"]
-    kb = _kernel_bytes(b, 8, PARAMS["block_rows"])
+    kb = _kernel_bytes(b, 8, params["fp64"]["block_rows"])
     per_step = {k: v[0] / head["nt"] for k, v in head["timing"].items()}
     avg = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in head["timing"].items()}
     dom = max((k for k in kb if avg.get(k, 0.0) > 0), key=lambda k: per_step[k])
@@ -436,6 +536,8 @@ def main():
         "zeta_err_fp32_vs_fp64_m": _zeta_err(b, res["fp32"]["znv"], res["fp64"]["znv"], res["fp32"]["iwegv"],
                                              res["fp64"]["iwegv"]),
         "solver": {p: res[p]["stats"] for p in res},
+        "settings": {p: {k: params[p][k] for k in TUNE_GRID} for p in params},
+        "autotune": tuned or None,
         "kernel_ms_per_step_fp64": {k: round(v, 4) for k, v in per_step.items() if v > 0},
         "kernel_ms_per_step_fp32": {k: round(v[0] / res["fp32"]["nt"], 4) for k, v in res["fp32"]["timing"].items()
                                     if v[0] > 0},
@@ -498,6 +600,8 @@ def main():
         }
     if not args.no_serving:
         out["serving"] = run_serving(args)
+    if not args.no_c3:
+        out["c3"] = c3_run(args, params)
     if not args.no_cpu:
         # all the host threads this process may use (the box's share), and one
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
