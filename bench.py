@@ -193,7 +193,7 @@ def run_gpu(b, tr, prec, args, steps=None, baroclinic=False, params=None):
 
 # the settings the autotuner chooses among (all solve and transport the
 # same way; they trade iterations and launch shapes)
-TUNE_GRID = dict(block_rows=(128, 256, 512), tri_sweeps=(3, 4), node_tile=(32, 64))
+TUNE_GRID = dict(block_rows=(128, 256, 512), tri_sweeps=(3, 4), node_tile=(64, 256))
 
 
 def autotune_params(b, tr, prec, steps=20):
@@ -536,7 +536,7 @@ def main():
         "zeta_err_fp32_vs_fp64_m": _zeta_err(b, res["fp32"]["znv"], res["fp64"]["znv"], res["fp32"]["iwegv"],
                                              res["fp64"]["iwegv"]),
         "solver": {p: res[p]["stats"] for p in res},
-        "settings": {p: {k: params[p][k] for k in TUNE_GRID} for p in params},
+        "settings": {p: {k: params[p].get(k, 0) for k in TUNE_GRID} for p in params},
         "autotune": tuned or None,
         "kernel_ms_per_step_fp64": {k: round(v, 4) for k, v in per_step.items() if v > 0},
         "kernel_ms_per_step_fp32": {k: round(v[0] / res["fp32"]["nt"], 4) for k, v in res["fp32"]["timing"].items()

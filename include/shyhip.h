@@ -65,8 +65,10 @@ typedef struct {
     float rkpar, difmol;
     int itvd, itvdv;
     /* transport kernels */
-    int node_tile;                              /* nodes per workgroup of the conz3d node phase:
-                                                   0 = auto, else a power of two <= 256 */
+    int node_tile;                              /* most nodes per workgroup of the conz3d node
+                                                   phase (a workgroup also holds at most 256
+                                                   node-layers): 0 = auto (256), else 1..256;
+                                                   read by shyhip_set_basin, like block_rows */
 } shyhip_params;
 
 /* field ids for shyhip_push / shyhip_pull */

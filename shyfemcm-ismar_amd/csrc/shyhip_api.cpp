@@ -312,7 +312,7 @@ int shyhip_destroy(shyhip_model *h) {
                     (void *)m.blk_elp, (void *)m.blk_elr, (void *)m.blk_enlev, (void *)m.ell_col, (void *)m.ell_len, (void *)m.csr_ell,
                     (void *)m.fwd_rows, (void *)m.fwd_lptr, (void *)m.fwd_nlev, (void *)m.bwd_rows,
                     (void *)m.bwd_lptr, (void *)m.bwd_nlev, (void *)m.tvd_el, (void *)m.tvd_kn, m.tvd_w,
-                    (void *)m.tvd_lev, (void *)m.elem_iext})
+                    (void *)m.tvd_lev, (void *)m.elem_iext, (void *)m.conz_eitem, (void *)m.conz_ntile})
         dfree(q);
     Solver &v = M.sv;
     for (void *q : {v.lu, v.ell_a, v.fs_val, v.r0, v.r, v.p0, v.p1, v.v0, v.v1, v.s, v.t, v.dx, v.b, v.x0, (void *)v.part,
@@ -366,8 +366,8 @@ static int check_params(Model *M, const shyhip_params *p) {
     if (p->itvd < 0 || p->itvd > 2 || p->itvdv < 0 || p->itvdv > 1) return fail(M, "itvd must be 0 or 2, itvdv 0 or 1");
     if (p->block_rows < 0 || p->block_rows > kMaxBlockRows) return fail(M, "block_rows must be in 0..1024");
     if (p->tri_sweeps < 0 || p->tri_sweeps > 64) return fail(M, "tri_sweeps must be in 0..64");
-    if (p->node_tile < 0 || p->node_tile > 256 || (p->node_tile & (p->node_tile - 1)) != 0)
-        return fail(M, "node_tile must be 0 or a power of two <= 256");
+    if (p->node_tile < 0 || p->node_tile > kNodeTileMax)
+        return fail(M, "node_tile must be in 0..256");
     if (!(p->inner_rtol > 0. && p->inner_rtol < 1.)) return fail(M, "inner_rtol must be in (0,1)");
     return 0;
 }
@@ -554,6 +554,40 @@ int shyhip_set_basin(shyhip_model *h, const int *nen3v, const float *xgv, const 
         rc |= dalloc(M, &m.inc_cnt, icnt.size()) || upload(M, m.inc_cnt, icnt.data(), icnt.size());
         rc |= dalloc(M, &m.inc_el, iel.size()) || upload(M, m.inc_el, iel.data(), iel.size());
         rc |= dalloc(M, &m.inc_elc, ielc.size()) || upload(M, m.inc_elc, ielc.data(), ielc.size());
+    }
+    // conz3d work division: the element-layer list (chunks of kConzChunk
+    // elements, layer by layer) and the node tiles (at most node_tile nodes
+    // and kNodeItems node-layers each)
+    {
+        std::vector<int> items;
+        items.reserve((size_t)nel * 2);
+        for (int c0 = 0; c0 < nel; c0 += kConzChunk) {
+            const int c1 = std::min(nel, c0 + kConzChunk);
+            int lmax = 0;
+            for (int ie = c0; ie < c1; ie++) lmax = std::max(lmax, ilhv[ie]);
+            for (int l = 1; l <= lmax; l++)
+                for (int ie = c0; ie < c1; ie++)
+                    if (ilhv[ie] >= l) items.push_back((ie << 5) | (l - 1));
+        }
+        if (nel >= (1 << 26)) return fail(&M, "at most 2^26 elements");
+        const int tn = M.p.node_tile > 0 ? std::min(M.p.node_tile, kNodeTileMax) : kNodeTileMax;
+        std::vector<int> tiles{0};
+        int cnt = 0, nl = 0;
+        for (int k = 0; k < nkn; k++) {
+            if (cnt == tn || nl + ilhkv[k] > kNodeItems) {
+                tiles.push_back(k);
+                cnt = 0;
+                nl = 0;
+            }
+            cnt++;
+            nl += ilhkv[k];
+        }
+        tiles.push_back(nkn);
+        m.n_eitem = (int)items.size();
+        m.n_ntile = (int)tiles.size() - 1;
+        m.ntile_tn = tn;
+        rc |= dalloc(M, &m.conz_eitem, items.size()) || upload(M, m.conz_eitem, items.data(), items.size());
+        rc |= dalloc(M, &m.conz_ntile, tiles.size()) || upload(M, m.conz_ntile, tiles.data(), tiles.size());
     }
     // CSR pattern
     rc |= dalloc(M, &m.rowptr, nkn + 1) || upload(M, m.rowptr, M.h_rowptr.data(), nkn + 1);

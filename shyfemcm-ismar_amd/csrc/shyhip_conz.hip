@@ -1,22 +1,23 @@
 // shyhip_conz.hip -- 3D scalar advection-diffusion (conz3d) on gfx950.
 //
 // Two kernels per substep:
-//   k_conz_elem  one workgroup per tile of elements, one work-item per
-//                element-layer: horizontal advective fluxes (upwind rule or
-//                the itvd=2 TVD scheme with precomputed upwind interpolation
-//                weights) -> 3*fl per vertex, written layer-major to a
-//                scratch array.
-//   k_conz_node  one workgroup per tile of consecutive nodes.  Phase 1, one
-//                work-item per node-layer: the vertical diffusion/advection
-//                terms of the node column (they depend on an element only
-//                through its bottom layer, so three variants cover every
-//                element), then the incident elements gathered in colour
-//                order -- the order in which conz3d_omp accumulates them
-//                (concentration_omp.f90:243-290) -- with the element terms of
-//                conz3d_element (:355-753), and the boundary mass flux of
-//                conz3d_nodes; the column coefficients go to LDS.  Phase 2,
-//                one work-item per node: the implicit vertical solve
-//                (:757-886) and the write-back.
+//   k_conz_elem  one work-item per element-layer of a precomputed work list
+//                (workgroups of equal work whatever the depth): horizontal
+//                advective fluxes (upwind rule or the itvd=2 TVD scheme with
+//                precomputed upwind interpolation weights) -> 3*fl per
+//                vertex, stored by the vertex's position in its node's
+//                colour-order incidence list.
+//   k_conz_node  one workgroup per node tile (at most 64 nodes and 256
+//                node-layers).  Phase 1, one work-item per node-layer: the
+//                vertical diffusion/advection terms of the node column (they
+//                depend on an element only through its bottom layer, so
+//                three variants cover every element), then the incident
+//                elements gathered in colour order -- the order in which
+//                conz3d_omp accumulates them (concentration_omp.f90:243-290)
+//                -- with the element terms of conz3d_element (:355-753), and
+//                the boundary mass flux of conz3d_nodes; the column
+//                coefficients go to LDS.  Phase 2, one work-item per node:
+//                the implicit vertical solve (:757-886) and the write-back.
 // Nudging, settling velocity and loads are not part of the supported
 // configurations; horizontal diffusion (rkpar) must be zero.
 #include <hip/hip_runtime.h>
@@ -61,7 +62,9 @@ struct ConzArgs {
     const int *mem_istot, *node_mem, *elem_mem;
     int isact;
     double dtp;
-    int node_tile;                   // nodes per workgroup of the node phase
+    // element-layer work list and node tiles (see Mesh)
+    const int *eitem, *ntile;
+    int n_eitem;
 };
 
 template <typename S>
@@ -72,14 +75,16 @@ __device__ inline double cval(const ConzArgs<S> &a, int t, int l, int k) {
 }
 
 // ------------------------------------------------------------------ element phase
-// One workgroup per tile of kElemTile consecutive elements.  The tile's
-// geometry, levels and TVD upwind tables are staged in LDS once, four
-// work-items per element, every load independent of the others (the upwind
-// elements' levels and the elements' external-node counts are precomputed
-// tables).  Then one work-item per element-layer of the tile: the layer's
-// transports and every tracer's node concentrations, the flux coefficients
-// once, and per tracer the upwind rule or the TVD limiter.
-constexpr int kElemTile = 64;
+// One work-item per entry of the element-layer work list (Mesh::conz_eitem):
+// chunks of kConzChunk consecutive elements, each chunk layer by layer with
+// its elements ascending, so that a layer's loads are contiguous.  A
+// workgroup takes kThreads consecutive entries, whatever the depth of the
+// elements they come from, so every workgroup does the same work.  An entry
+// loads its element's data itself (the element's other layers read the
+// same lines from L2); the independent loads -- nodes, list positions,
+// geometry, the layer's transports -- are issued at once, the tracers at
+// the nodes next, and for TVD the two upwind points the flow direction
+// picks, then their tracer values.
 
 // v[i] for a run-time i in 0..2 without indexing a register array
 template <typename T>
@@ -87,100 +92,60 @@ __device__ inline T sel3(const T (&v)[3], int i) {
     return i == 0 ? v[0] : (i == 1 ? v[1] : v[2]);
 }
 
-struct ElemTileLds {
-    double b[3][kElemTile], c[3][kElemTile], dis[3][kElemTile];
-    double tw[3][6][kElemTile];
-    int kn[3][kElemTile], so[3][kElemTile], tkn[3][6][kElemTile], tel[6][kElemTile], tlev[6][kElemTile];
-    int ilev[kElemTile], iext[kElemTile];
-    double ajr[kElemTile];
-};
-
 template <typename S, int NT>
-__global__ __launch_bounds__(kThreads) void k_conz_elem(ConzArgs<S> a) {
+__global__ __launch_bounds__(kThreads, 5) void k_conz_elem(ConzArgs<S> a) {
     using R = S;
-    __shared__ ElemTileLds T;
-    const int nel = a.nel, nkn = a.nkn, e0 = xcd_block() * kElemTile;
-    const int ne = min(kElemTile, nel - e0);
+    const int i = xcd_block() * kThreads + threadIdx.x;
+    if (i >= a.n_eitem) return;
+    const unsigned w = (unsigned)a.eitem[i];
+    const int ie = (int)(w >> 5), l = (int)(w & 31u) + 1;
+    if (a.mem_istot && a.isact > a.mem_istot[a.elem_mem[ie]]) return;
+    const int nel = a.nel, nkn = a.nkn;
     const bool use_tvd = a.itvd > 0;
-    __shared__ int lmax_tile;
-    if (threadIdx.x == 0) lmax_tile = 0;
-    __syncthreads();
-    {
-        const int e = threadIdx.x % kElemTile, part = threadIdx.x / kElemTile;   // kThreads = 4 * kElemTile
-        if (e < ne) {
-            const int ie = e0 + e;
-            if (part == 0) {
-                for (int ii = 0; ii < 3; ii++) {
-                    T.kn[ii][e] = a.nen3v[3 * ie + ii];
-                    T.so[ii][e] = a.slot_of[(size_t)ii * nel + ie];
-                    T.b[ii][e] = a.geo[(size_t)ii * nel + ie];
-                    T.c[ii][e] = a.geo[(size_t)(3 + ii) * nel + ie];
-                    T.dis[ii][e] = a.dis[(size_t)ii * nel + ie];
-                }
-                T.iext[e] = a.elem_iext[ie];
-                T.ajr[e] = a.geo[(size_t)6 * nel + ie];
-                const int lv = a.ilhv[ie];
-                T.ilev[e] = lv;
-                atomicMax(&lmax_tile, lv);
-            } else if (use_tvd) {
-                for (int p = 2 * (part - 1); p < 2 * part; p++) {
-                    T.tel[p][e] = a.tvd_el[(size_t)p * nel + ie];
-                    T.tlev[p][e] = a.tvd_lev[(size_t)p * nel + ie];
-                    for (int q = 0; q < 3; q++) {
-                        T.tkn[q][p][e] = a.tvd_kn[((size_t)q * 6 + p) * nel + ie];
-                        T.tw[q][p][e] = a.tvd_w[((size_t)q * 6 + p) * nel + ie];
-                    }
-                }
-            }
-        }
-    }
-    __syncthreads();
-    // the tile's element-layers that exist (members still substepping),
-    // layer by layer with the elements ascending in each, listed by the
-    // first wavefront (work-item e owns element e of the tile)
-    static_assert(kElemTile == 64, "one wavefront lists the tile");
-    __shared__ unsigned short items[kElemTile * 32];      // e | l << 8
-    __shared__ int nitems_s;
-    if (threadIdx.x < kElemTile) {
-        const int e = threadIdx.x;
-        int lev = e < ne ? T.ilev[e] : 0;
-        if (lev > 0 && a.mem_istot && a.isact > a.mem_istot[a.elem_mem[e0 + e]]) lev = 0;
-        const unsigned long long below = e == 0 ? 0ull : (~0ull >> (64 - e));
-        int off = 0;
-        for (int l = 1; l <= lmax_tile; l++) {
-            const unsigned long long m = __ballot(lev >= l);
-            if (lev >= l) items[off + __popcll(m & below)] = (unsigned short)(e | (l << 8));
-            off += __popcll(m);
-        }
-        if (e == 0) nitems_s = off;
-    }
-    __syncthreads();
-    const int nitems = nitems_s;
-    const double azt = 1. - a.az;
-    for (int it = threadIdx.x; it < nitems; it += kThreads) {
-        const int e = items[it] & 0xff, l = items[it] >> 8;
-        const int ie = e0 + e;
-        // the layer's transports, velocities and node concentrations
-        const size_t o = (size_t)(l - 1) * nel + ie;
-        const S un = a.utn[o], uo = a.uto[o], vn = a.vtn[o], vo = a.vto[o];
-        const bool tvd = use_tvd && T.iext[e] == 0;
-        S ul = 0, vl = 0;
-        if (tvd) { ul = a.uln[o]; vl = a.vln[o]; }
-        S cr[NT][3];
+    int kn[3], so[3];
+    double gb[3], gc[3];
 #pragma unroll
-        for (int t = 0; t < NT; t++)
-            for (int ii = 0; ii < 3; ii++) cr[t][ii] = a.c[t][(size_t)(l - 1) * nkn + T.kn[ii][e]];
-        auto clip = [&](int t, S v) -> S { return (a.clip[t] && v < (S)1.e-15) ? (S)0 : v; };
-        // upwind rule (the flux coefficients are the tracers' common part)
-        const double us = a.az * (double)un + azt * (double)uo;
-        const double vs = a.az * (double)vn + azt * (double)vo;
-        int itot = 0, isum = 0;
-        double f[3];
-        for (int ii = 0; ii < 3; ii++) {
-            f[ii] = us * T.b[ii][e] + vs * T.c[ii][e];
-            if (f[ii] < 0.) { itot++; isum += ii + 1; }
-        }
-        double fl[NT][3];
+    for (int ii = 0; ii < 3; ii++) {
+        kn[ii] = a.nen3v[3 * ie + ii];
+        so[ii] = a.slot_of[(size_t)ii * nel + ie];
+        gb[ii] = a.geo[(size_t)ii * nel + ie];
+        gc[ii] = a.geo[(size_t)(3 + ii) * nel + ie];
+    }
+    // the layer's transports, velocities and node concentrations
+    const size_t o = (size_t)(l - 1) * nel + ie;
+    const S un = a.utn[o], uo = a.uto[o], vn = a.vtn[o], vo = a.vto[o];
+    int iext = 1;
+    S ul = 0, vl = 0;
+    double dis3[3] = {0., 0., 0.}, ajr0 = 0.;
+    if (use_tvd) {
+        iext = a.elem_iext[ie];
+        ul = a.uln[o];
+        vl = a.vln[o];
+#pragma unroll
+        for (int ii = 0; ii < 3; ii++) dis3[ii] = a.dis[(size_t)ii * nel + ie];
+        ajr0 = a.geo[(size_t)6 * nel + ie];
+    }
+    const bool tvd = use_tvd && iext == 0;
+    S cr[NT][3];
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int ii = 0; ii < 3; ii++) cr[t][ii] = a.c[t][(size_t)(l - 1) * nkn + kn[ii]];
+    auto clip = [&](int t, S v) -> S { return (a.clip[t] && v < (S)1.e-15) ? (S)0 : v; };
+    // upwind rule (the flux coefficients are the tracers' common part)
+    const double azt = 1. - a.az;
+    const double us = a.az * (double)un + azt * (double)uo;
+    const double vs = a.az * (double)vn + azt * (double)vo;
+    int itot = 0, isum = 0;
+    double f[3];
+    for (int ii = 0; ii < 3; ii++) {
+        f[ii] = us * gb[ii] + vs * gc[ii];
+        if (f[ii] < 0.) { itot++; isum += ii + 1; }
+    }
+    double fl[NT][3];
+#pragma unroll
+    for (int t = 0; t < NT; t++) fl[t][0] = fl[t][1] = fl[t][2] = 0.;
+    if (!tvd) {
 #pragma unroll
         for (int t = 0; t < NT; t++) {
             double cl[3];
@@ -194,179 +159,166 @@ __global__ __launch_bounds__(kThreads) void k_conz_elem(ConzArgs<S> a) {
                 const double sum = fl[t][0] + fl[t][1] + fl[t][2];
                 for (int ii = 0; ii < 3; ii++)
                     if (ii == is2 - 1) fl[t][ii] = -sum;
-            } else {
-                fl[t][0] = fl[t][1] = fl[t][2] = 0.;
             }
         }
-        // TVD fluxes (tvd_fluxes, tvd_admin.f90:673-823): the pairs of the
-        // flow direction, then each tracer's upwind values and limiter
-        if (tvd) {
+    } else if (itot >= 1 && itot <= 2) {
+        // TVD fluxes (tvd_fluxes, tvd_admin.f90:673-823): the two pairs
+        // (ic, id) of the flow direction -- the vertices other than ip,
+        // ascending -- with their Courant numbers, then both pairs' upwind
+        // tables (independent loads), their tracer values, and each
+        // tracer's limiter
+        const int itot2 = itot - 1, itot1 = 2 - itot;
+        const int ip = itot2 * (6 - isum) + itot1 * isum;
+        const R ajr = (R)(24. * ajr0);
+        const double dte = a.mem_istot ? a.dtp / (double)a.mem_istot[a.elem_mem[ie]] : a.dt;
+        const R fact = (R)(-itot2 + itot1);
+        int icv[2], idv[2], pidx[2], tkn[2][3];
+        R alfa[2];
+        double fii[2];
 #pragma unroll
-            for (int t = 0; t < NT; t++) fl[t][0] = fl[t][1] = fl[t][2] = 0.;
-            if (itot >= 1 && itot <= 2) {
-                const int itot2 = itot - 1, itot1 = 2 - itot;
-                const int ip = itot2 * (6 - isum) + itot1 * isum;
-                const R ajr = (R)(24. * T.ajr[e]);
-                const double dte = a.mem_istot ? a.dtp / (double)a.mem_istot[a.elem_mem[ie]] : a.dt;
-                for (int k = 0; k < 2; k++) {
-                    // the vertices other than ip, ascending
-                    const int ii = k == 0 ? (ip == 1 ? 2 : 1) : (ip == 3 ? 2 : 3);
-                    const int ic = itot2 * ii + itot1 * ip;
-                    const int id = itot2 * ip + itot1 * ii;
-                    const int pidx = 2 * (ic - 1) + ((id == (ic % 3) + 1) ? 0 : 1);
-                    const bool pup = T.tel[pidx][e] >= 0 && T.tlev[pidx][e] >= l;
-                    S cu[NT][3];
-                    if (pup)
+        for (int k = 0; k < 2; k++) {
+            const int ii = k == 0 ? (ip == 1 ? 2 : 1) : (ip == 3 ? 2 : 3);
+            const int ic = itot2 * ii + itot1 * ip;
+            const int id = itot2 * ip + itot1 * ii;
+            icv[k] = ic;
+            idv[k] = id;
+            fii[k] = sel3(f, ii - 1);
+            pidx[k] = 2 * (ic - 1) + ((id == (ic % 3) + 1) ? 0 : 1);
+            const int iop = 6 - (id + ic);
+            const int tet1 = 1 + iop % 3;
+            R dx = (R)((double)ajr * sel3(gc, iop - 1));
+            dx = (R)(-2 * (tet1 == id ? 1 : 0)) * dx + dx;
+            R dy = (R)((double)ajr * sel3(gb, iop - 1));
+            dy = (R)(-2 * (tet1 == ic ? 1 : 0)) * dy + dy;
+            const R dis = (R)sel3(dis3, iop - 1);
+            const R vel = (R)fabs((double)(ul * dx + vl * dy)) / dis;
+            alfa[k] = (R)((dte * (double)vel) / (double)dis);
+        }
+        bool pup[2];
+        double tw[2][3];
 #pragma unroll
-                        for (int t = 0; t < NT; t++)
-                            for (int q = 0; q < 3; q++) cu[t][q] = a.c[t][(size_t)(l - 1) * nkn + T.tkn[q][pidx][e]];
-                    const R fact = (R)(-itot2 + itot1);
-                    const int iop = 6 - (id + ic);
-                    const int tet1 = 1 + iop % 3;
-                    R dx = (R)((double)ajr * T.c[iop - 1][e]);
-                    dx = (R)(-2 * (tet1 == id ? 1 : 0)) * dx + dx;
-                    R dy = (R)((double)ajr * T.b[iop - 1][e]);
-                    dy = (R)(-2 * (tet1 == ic ? 1 : 0)) * dy + dy;
-                    const R dis = (R)T.dis[iop - 1][e];
-                    const R vel = (R)fabs((double)(ul * dx + vl * dy)) / dis;
-                    const R alfa = (R)((dte * (double)vel) / (double)dis);
+        for (int k = 0; k < 2; k++) {
+            const size_t po = (size_t)pidx[k] * nel + ie;
+            pup[k] = a.tvd_el[po] >= 0 && a.tvd_lev[po] >= l;
 #pragma unroll
-                    for (int t = 0; t < NT; t++) {
-                        const R conc = clip(t, sel3(cr[t], ic - 1)), cond = clip(t, sel3(cr[t], id - 1));
-                        R conu = cond;
-                        if (pup) {
-                            double zh = 0.;
-                            for (int q = 0; q < 3; q++) zh = zh + (double)clip(t, cu[t][q]) * T.tw[q][pidx][e];
-                            conu = (R)zh;
-                        }
-                        const R grad = cond - conu;
-                        R rf;
-                        if ((R)fabs((double)(conc - cond)) < (R)1.e-8) rf = -1;
-                        else rf = grad / (cond - conc) - (R)1;
-                        const double rfd = (double)rf;
-                        const R psi = (R)fmax(0., fmax(fmin(1., 2. * rfd), fmin(2., rfd)));
-                        const R conf = conc + (R)0.5 * psi * (cond - conc) * ((R)1 - alfa);
-                        const R term = (R)((double)(fact * conf) * sel3(f, ii - 1));
-                        for (int q = 0; q < 3; q++) {
-                            if (q == ic - 1) fl[t][q] = fl[t][q] - (double)term;
-                            if (q == id - 1) fl[t][q] = fl[t][q] + (double)term;
-                        }
-                    }
+            for (int q = 0; q < 3; q++) {
+                tkn[k][q] = a.tvd_kn[((size_t)q * 6 + pidx[k]) * nel + ie];
+                tw[k][q] = a.tvd_w[((size_t)q * 6 + pidx[k]) * nel + ie];
+            }
+        }
+        S cu[2][NT][3];
+#pragma unroll
+        for (int k = 0; k < 2; k++)
+            if (pup[k])
+#pragma unroll
+                for (int t = 0; t < NT; t++)
+#pragma unroll
+                    for (int q = 0; q < 3; q++) cu[k][t][q] = a.c[t][(size_t)(l - 1) * nkn + tkn[k][q]];
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int ic = icv[k], id = idv[k];
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                const R conc = clip(t, sel3(cr[t], ic - 1)), cond = clip(t, sel3(cr[t], id - 1));
+                R conu = cond;
+                if (pup[k]) {
+                    double zh = 0.;
+                    for (int q = 0; q < 3; q++) zh = zh + (double)clip(t, cu[k][t][q]) * tw[k][q];
+                    conu = (R)zh;
+                }
+                const R grad = cond - conu;
+                R rf;
+                if ((R)fabs((double)(conc - cond)) < (R)1.e-8) rf = -1;
+                else rf = grad / (cond - conc) - (R)1;
+                const double rfd = (double)rf;
+                const R psi = (R)fmax(0., fmax(fmin(1., 2. * rfd), fmin(2., rfd)));
+                const R conf = conc + (R)0.5 * psi * (cond - conc) * ((R)1 - alfa[k]);
+                const R term = (R)((double)(fact * conf) * fii[k]);
+                for (int q = 0; q < 3; q++) {
+                    if (q == ic - 1) fl[t][q] = fl[t][q] - (double)term;
+                    if (q == id - 1) fl[t][q] = fl[t][q] + (double)term;
                 }
             }
         }
-#pragma unroll
-        for (int t = 0; t < NT; t++)
-            for (int ii = 0; ii < 3; ii++)
-                a.hbuf[((size_t)(l - 1) * NT + t) * a.nslot + T.so[ii][e]] = (S)(3. * fl[t][ii]);
     }
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int ii = 0; ii < 3; ii++)
+            a.hbuf[((size_t)(l - 1) * NT + t) * a.nslot + so[ii]] = (S)(3. * fl[t][ii]);
 }
 
 // ------------------------------------------------------------------ node phase
-// Vertical flux at interface lp of a column truncated at lmax
-// (vertical_flux_ie, tvd_admin.f90:930-1016).  cl and hold are windows of
-// the column around layer l (entry x - l + 2 holds layer x, x = l-2..l+2),
-// zero beyond lmax; w is the vertical velocity at the interface.
-__device__ inline double vflux_win(int lp, int l, int lmax, double w, const double *cl, const double *hold, double dt,
-                                   bool btvdv) {
-    if (lp <= 0 || lp >= lmax) return 0.;
-    const int o = 2 - l;       // window offset: layer x at cl[x + o]
-    const double conc = (w > 0.) ? cl[lp + 1 + o] : cl[lp + o];
-    double conf = conc;
-    if (btvdv) {
-        double cond, conu;
-        if (w > 0.) {
-            cond = cl[lp + o];
-            conu = cond;
-            if (lp + 2 <= lmax) conu = cl[lp + 2 + o];
-        } else {
-            cond = cl[lp + 1 + o];
-            conu = cond;
-            if (lp - 1 >= 1) conu = cl[lp - 1 + o];
-        }
-        const double hdis = 0.5 * (hold[lp + o] + hold[lp + 1 + o]);
-        const double alfa = dt * fabs(w) / hdis;
-        double rf;
-        if (fabs(conc - cond) < (double)1.e-8f) rf = -1.;
-        else rf = (cond - conu) / (cond - conc) - 1.;
-        const double psi = fmax(0., fmax(fmin(1., 2. * rf), fmin(2., rf)));
-        conf = conc + 0.5 * psi * (cond - conc) * (1. - alfa);
-    }
-    return w * conf;
+// The limited face value of the vertical TVD flux (vertical_flux_ie,
+// tvd_admin.f90:930-1016): centre conc, downstream cond, upstream conu,
+// Courant number alfa.
+__device__ inline double tvd_face(double conc, double cond, double conu, double alfa) {
+    double rf;
+    if (fabs(conc - cond) < (double)1.e-8f) rf = -1.;
+    else rf = (cond - conu) / (cond - conc) - 1.;
+    const double psi = fmax(0., fmax(fmin(1., 2. * rf), fmin(2., rf)));
+    return conc + 0.5 * psi * (cond - conc) * (1. - alfa);
 }
 
-// Nodes per workgroup of the node phase (a.node_tile, a power of two).  The
-// default was the fastest measured on the 12-layer lagoon (tiles of 8 to
-// 64 nodes): phase 2's serial column solves are then shared by the most
-// nodes for the LDS a workgroup holds.
-static int conz_node_tile(int nlv, int requested) {
-    if (requested > 0) return requested;
-    return nlv <= 16 ? 64 : 32;
-}
-// LDS of one node-phase workgroup: right-hand sides of NT tracers and the
-// three diagonals of the column matrix, [NL][TN] each, and the list of the
-// tile's node-layers
-static size_t conz_node_lds(int NL, int NT, int TN) {
-    return (size_t)(NT + 3) * NL * TN * sizeof(double) + (size_t)NL * TN * sizeof(int);
-}
-
+// Node tiles (Mesh::conz_ntile, built at set-up): consecutive nodes, at most
+// the node_tile setting (<= kNodeTileMax = the workgroup size) and at most
+// kNodeItems node-layers, so that phase 1 is one pass of the workgroup
+// whatever the depth of the tile's columns.
 template <typename S, int NL, int NT>
 __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
     using R = S;
-    const int TN = a.node_tile;
-    extern __shared__ double node_lds[];
-    double *const cnP = node_lds, *const cdP = cnP + NT * NL * TN, *const cloP = cdP + NL * TN,
-                  *const chiP = cloP + NL * TN;
-    // entry (layer index l0, node kk of the tile) of a [NL][TN] array
-    auto at = [TN](double *p, int l0, int kk) -> double & { return p[l0 * TN + kk]; };
+    // phase-1 results by list position: right-hand sides of NT tracers and
+    // the three diagonals of the column matrix
+    __shared__ double cnP[NT][kNodeItems], cdP[kNodeItems], cloP[kNodeItems], chiP[kNodeItems];
+    __shared__ int items[kNodeItems];                 // kk | l << 16
+    constexpr int kWaves = kThreads / 64;
+    // per wavefront and layer: its nodes that have the layer (bit per lane)
+    // and the list position of the first of them
+    __shared__ unsigned long long lmask[kWaves][NL + 1];
+    __shared__ int lbase[kWaves][NL + 1];
+    __shared__ int nitems_s;
     const int nkn = a.nkn;
-    const int k0 = xcd_block() * TN;
+    const int tile = xcd_block();
+    const int k0 = a.ntile[tile], tn = a.ntile[tile + 1] - k0;
     const double ad = a.ad, aa = a.aa, adt = 1. - a.ad, aat = 1. - a.aa;
     const bool btvdv = a.itvdv > 0;
     const int nlv = min(a.nlv, NL);
 
     // The tile's node-layers that exist (l <= ilhkv of the node, members
     // still substepping), layer by layer with the nodes ascending in each:
-    // phase 1 takes exactly these, so no work-item idles below a node's
-    // bottom and a layer's loads stay in node order.  Work-item kk < TN owns
-    // node kk; each wavefront counts its nodes per layer with a ballot.
-    constexpr int kWaves = kThreads / 64;
-    __shared__ int wpos[kWaves][NL + 1];
-    __shared__ int nitems_s;
-    int *const items = (int *)(chiP + NL * TN);     // kk | l << 16
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = (TN + 63) / 64;
+    // phase 1 takes exactly these, and a layer's loads stay in node order.
+    // Work-item kk owns node kk; each wavefront counts its nodes per layer
+    // with a ballot, one work-item turns the counts into list positions.
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     int lmk = 0;
-    if ((int)threadIdx.x < TN && k0 + (int)threadIdx.x < nkn) {
+    if ((int)threadIdx.x < tn) {
         const int k = k0 + threadIdx.x;
         lmk = min(a.ilhkv[k], nlv);
         if (a.mem_istot && a.isact > a.mem_istot[a.node_mem[k]]) lmk = 0;
     }
-    if (wv < nwv)
-        for (int l = 1; l <= nlv; l++) {
-            const unsigned long long m = __ballot(lmk >= l);
-            if (lane == 0) wpos[wv][l] = __popcll(m);
-        }
+    for (int l = 1; l <= nlv; l++) {
+        const unsigned long long m = __ballot(lmk >= l);
+        if (lane == 0) lmask[wv][l] = m;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         int off = 0;
         for (int l = 1; l <= nlv; l++)
-            for (int w = 0; w < nwv; w++) {
-                const int c = wpos[w][l];
-                wpos[w][l] = off;
-                off += c;
+            for (int w = 0; w < kWaves; w++) {
+                lbase[w][l] = off;
+                off += __popcll(lmask[w][l]);
             }
         nitems_s = off;
     }
     __syncthreads();
-    if (wv < nwv) {
-        const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-        for (int l = 1; l <= nlv; l++) {
-            const unsigned long long m = __ballot(lmk >= l);
-            if (lmk >= l) items[wpos[wv][l] + __popcll(m & below)] = (int)threadIdx.x | (l << 16);
-        }
-    }
-    __syncthreads();
     const int nitems = nitems_s;
+    if (nitems > kNodeItems) {      // the tile table guarantees otherwise
+        if (threadIdx.x == 0) atomicCAS(a.err, 0, ERR_CONZ);
+        return;
+    }
+    for (int l = 1; l <= lmk; l++) items[lbase[wv][l] + __popcll(lmask[wv][l] & below)] = (int)threadIdx.x | (l << 16);
+    __syncthreads();
 
     // phase 1: one work-item per node-layer
     for (int it = threadIdx.x; it < nitems; it += blockDim.x) {
@@ -385,6 +337,7 @@ __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
         const int q0 = a.nkc_ptr[k], cnt = a.nkc_ptr[k + 1] - q0;
         // node column around l (zero outside 1..lmaxk)
         double ckw[NT][5], hokw[5], hnkw[5];
+#pragma unroll
         for (int j = 0; j < 5; j++) {
             const int x = l - 2 + j;
             hokw[j] = 0.; hnkw[j] = 0.;
@@ -405,86 +358,131 @@ __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
         const double rvpbot = (double)(R)(a.difv[(size_t)l * nkn + k] + (R)a.difmol);
         const double wtop = (l == 1) ? 0. : (double)a.wln[(size_t)(l - 1) * nkn + k];
         const double wbot = (double)a.wln[(size_t)l * nkn + k];
-        // element terms of the three column truncations: bottom layer l,
-        // l+1, or deeper (nothing below l+2 is read)
-        double fdv[NT][3], fwv[NT][3], clcv[3], clmv[3], clpv[3];
-        for (int v = 0; v < 3; v++) {
-            const int lmax = l + v;
-            double holdt[5];
-            for (int j = 0; j < 5; j++) {
-                const int x = l - 2 + j;
-                holdt[j] = (x >= 1 && x <= lmax) ? hokw[j] : 0.;
-            }
-            const double pm = (l - 1 >= 1) ? 1. : 0.;
-            const double pp = (l + 1 <= lmax) ? 1. : 0.;
-            const double hom = holdt[1], hoc = holdt[2], hop = holdt[3];
-            const double hnm = (l - 1 >= 1) ? hnkw[1] : 0., hnc = hnkw[2], hnp = (l + 1 <= lmax) ? hnkw[3] : 0.;
-            const double hmotop = 2. * rvptop * pm / (hom + hoc);
-            const double hmobot = 2. * rvpbot * pp / (hoc + hop);
-            const double hmntop = 2. * rvptop * pm / (hnm + hnc);
-            const double hmnbot = 2. * rvpbot * pp / (hnc + hnp);
-            double clc = 0. + ad * (hmntop + hmnbot);
-            double clm = 0. - ad * hmntop;
-            double clp = 0. - ad * hmnbot;
+        // element terms of the three column truncations: bottom layer l
+        // (v = 0), l+1 (v = 1) or deeper (v = 2; nothing below l+2 is read).
+        // The truncation enters only through the layers below l: the
+        // diffusion and vertical flux across the bottom interface exist
+        // for v >= 1, and the upstream value beyond the next layer for v = 2.
+        // Each distinct term is computed once, with the arithmetic of the
+        // truncated column (vertical_flux_ie: zero at the column's ends).
+        const double hom = hokw[1], hoc = hokw[2], hop = hokw[3];
+        const double hnm = hnkw[1], hnc = hnkw[2], hnp = hnkw[3];
+        const double pm = (l - 1 >= 1) ? 1. : 0.;
+        const double hmotop = 2. * rvptop * pm / (hom + hoc);
+        const double hmntop = 2. * rvptop * pm / (hnm + hnc);
+        // [0]: no layer below l (pp = 0, the lower neighbour zero), [1]: one
+        const double hmobot[2] = {2. * rvpbot * 0. / (hoc + 0.), 2. * rvpbot * 1. / (hoc + hop)};
+        const double hmnbot[2] = {2. * rvpbot * 0. / (hnc + 0.), 2. * rvpbot * 1. / (hnc + hnp)};
+        // matrix coefficients: [0] for v = 0, [1] for v >= 1 (the lower
+        // diagonal does not depend on the truncation)
+        double clcv[2], clpv[2], clm = 0.;
+#pragma unroll
+        for (int v = 0; v < 2; v++) {
+            const double hmb = hmnbot[v];
+            double clc = 0. + ad * (hmntop + hmb);
+            clm = 0. - ad * hmntop;
+            double clp = 0. - ad * hmb;
             double w = wtop;
             if (w >= 0.) clc += aa * w;
             else clm += aa * w;
-            w = (l == lmax) ? 0. : wbot;
+            w = (v == 0) ? 0. : wbot;
             if (w > 0.) clp -= aa * w;
             else clc -= aa * w;
-            clcv[v] = clc; clmv[v] = clm; clpv[v] = clp;
+            clcv[v] = clc; clpv[v] = clp;
+        }
+        // Courant numbers of the two interfaces (tracer-independent)
+        double alfa_t = 0., alfa_b = 0.;
+        if (btvdv) {
+            alfa_t = dt * fabs(wtop) / (0.5 * (hom + hoc));
+            alfa_b = dt * fabs(wbot) / (0.5 * (hoc + hop));
+        }
+        // (one array per variant, indexed by the tracer only: a run-time
+        // variant index would put the terms in scratch memory)
+        double fd0[NT], fd1[NT], fw0[NT], fw1[NT], fw2[NT];
 #pragma unroll
-            for (int t = 0; t < NT; t++) {
-                double clt[5];
-                for (int j = 0; j < 5; j++) {
-                    const int x = l - 2 + j;
-                    clt[j] = (x >= 1 && x <= lmax) ? ckw[t][j] : 0.;
+        for (int t = 0; t < NT; t++) {
+            const double c0 = ckw[t][0], c1 = ckw[t][1], c2 = ckw[t][2], c3 = ckw[t][3], c4 = ckw[t][4];
+            fd0[t] = adt * ((c2 - 0.) * hmobot[0] - (c1 - c2) * hmotop);
+            fd1[t] = adt * ((c2 - c3) * hmobot[1] - (c1 - c2) * hmotop);
+            // flux across the top interface (none at the surface): [0]
+            // without a layer below l, [1] with one (they differ only in
+            // the upstream value of a rising flow)
+            double vt[2] = {0., 0.};
+            if (l >= 2) {
+                const double w = wtop;
+                if (!btvdv) {
+                    vt[0] = vt[1] = w * ((w > 0.) ? c2 : c1);
+                } else if (w > 0.) {
+                    vt[0] = w * tvd_face(c2, c1, c1, alfa_t);
+                    vt[1] = w * tvd_face(c2, c1, c3, alfa_t);
+                } else {
+                    vt[0] = vt[1] = w * tvd_face(c1, c2, (l - 2 >= 1) ? c0 : c2, alfa_t);
                 }
-                fdv[t][v] = adt * ((clt[2] - clt[3]) * hmobot - (clt[1] - clt[2]) * hmotop);
-                const double vf_prev = vflux_win(l - 1, l, lmax, wtop, clt, holdt, dt, btvdv);
-                const double vf_here = vflux_win(l, l, lmax, wbot, clt, holdt, dt, btvdv);
-                fwv[t][v] = aat * (vf_prev - vf_here);
             }
+            // flux across the bottom interface (none for v = 0): [0] for
+            // v = 1, [1] for v = 2 (they differ only in the upstream value
+            // of a rising flow)
+            double vb[2];
+            {
+                const double w = wbot;
+                if (!btvdv) {
+                    vb[0] = vb[1] = w * ((w > 0.) ? c3 : c2);
+                } else if (w > 0.) {
+                    vb[0] = w * tvd_face(c3, c2, c2, alfa_b);
+                    vb[1] = w * tvd_face(c3, c2, c4, alfa_b);
+                } else {
+                    vb[0] = vb[1] = w * tvd_face(c2, c3, (l - 1 >= 1) ? c1 : c3, alfa_b);
+                }
+            }
+            fw0[t] = aat * (vt[0] - 0.);
+            fw1[t] = aat * (vt[1] - vb[0]);
+            fw2[t] = aat * (vt[1] - vb[1]);
         }
         // incident elements in colour order (conz3d_element's scatter)
-        const double hoc = hokw[2], hnc = hnkw[2];
         double cn[NT], cd = 0., clo = 0., chi = 0.;
 #pragma unroll
         for (int t = 0; t < NT; t++) cn[t] = 0.;
-        auto add = [&](int ilev, double aj4, const double *hf) {
-            if (ilev < l) return;
-            const int v = min(ilev - l, 2);
-#pragma unroll
-            for (int t = 0; t < NT; t++) {
-                const double cexpl = aj4 * (hoc * ckw[t][2] + dt * (0. + hf[t] - fwv[t][v] - fdv[t][v]));
-                cn[t] += cexpl;
-            }
-            clo += aj4 * dt * clmv[v];
-            chi += aj4 * dt * clpv[v];
-            cd += aj4 * dt * clcv[v] + aj4 * (1. + dt * 0.) * hnc;
-        };
+        // One element's terms (bottom layer ilev, 4 * area aj4, horizontal
+        // terms hf0 and hf1 of tracers 0 and 1).  A macro rather than a
+        // lambda: the lambda's captures stayed in scratch memory.
+        static_assert(NT <= 2, "two tracers per launch");
+#define CONZ_ADD(ilev_, aj4_, hf0_, hf1_)                                                            \
+    do {                                                                                           \
+        const int ilev = (ilev_);                                                                  \
+        if (ilev >= l) {                                                                           \
+            const double aj4 = (aj4_), hfv[2] = {(hf0_), (hf1_)};                                  \
+            const bool deep = ilev > l, deeper = ilev > l + 1; /* v >= 1, v == 2 */                \
+            _Pragma("unroll") for (int t = 0; t < NT; t++) {                                       \
+                const double fw = deeper ? fw2[t] : (deep ? fw1[t] : fw0[t]);                      \
+                const double fd = deep ? fd1[t] : fd0[t];                                          \
+                const double cexpl = aj4 * (hoc * ckw[t][2] + dt * (0. + hfv[t] - fw - fd));       \
+                cn[t] += cexpl;                                                                    \
+            }                                                                                      \
+            clo += aj4 * dt * clm;                                                                 \
+            chi += aj4 * dt * (deep ? clpv[1] : clpv[0]);                                          \
+            cd += aj4 * dt * (deep ? clcv[1] : clcv[0]) + aj4 * (1. + dt * 0.) * hnc;              \
+        }                                                                                          \
+    } while (0)
         // the first kIncWidth positions are read unconditionally (clamped
         // to the list), contiguous for the node; the rest one by one
         int slv[kIncWidth];
-        double saj[kIncWidth], shf[kIncWidth][NT];
+        double saj[kIncWidth], shf0[kIncWidth], shf1[kIncWidth];
         const size_t hl = (size_t)(l - 1) * NT * a.nslot;
 #pragma unroll
         for (int s = 0; s < kIncWidth; s++) {
             const int q = min(q0 + s, a.nslot - 1);
             slv[s] = a.slot_lev[q];
             saj[s] = a.slot_aj4[q];
-#pragma unroll
-            for (int t = 0; t < NT; t++) shf[s][t] = (double)a.hbuf[hl + (size_t)t * a.nslot + q];
+            shf0[s] = (double)a.hbuf[hl + q];
+            shf1[s] = NT > 1 ? (double)a.hbuf[hl + (size_t)a.nslot + q] : 0.;
         }
 #pragma unroll
         for (int s = 0; s < kIncWidth; s++)
-            if (s < cnt) add(slv[s], saj[s], shf[s]);
-        for (int q = q0 + kIncWidth; q < q0 + cnt; q++) {
-            double hf[NT];
-#pragma unroll
-            for (int t = 0; t < NT; t++) hf[t] = (double)a.hbuf[hl + (size_t)t * a.nslot + q];
-            add(a.slot_lev[q], a.slot_aj4[q], hf);
-        }
+            if (s < cnt) CONZ_ADD(slv[s], saj[s], shf0[s], shf1[s]);
+        for (int q = q0 + kIncWidth; q < q0 + cnt; q++)
+            CONZ_ADD(a.slot_lev[q], a.slot_aj4[q], (double)a.hbuf[hl + q],
+                     NT > 1 ? (double)a.hbuf[hl + (size_t)a.nslot + q] : 0.);
+#undef CONZ_ADD
         // conz3d_nodes: boundary mass flux
         if (a.mflux) {
             const size_t o = (size_t)(l - 1) * nkn + k;
@@ -497,21 +495,24 @@ __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
             }
         }
 #pragma unroll
-        for (int t = 0; t < NT; t++) at(cnP + t * NL * TN, l - 1, kk) = cn[t];
-        at(cdP, l - 1, kk) = cd; at(cloP, l - 1, kk) = clo; at(chiP, l - 1, kk) = chi;
+        for (int t = 0; t < NT; t++) cnP[t][it] = cn[t];
+        cdP[it] = cd; cloP[it] = clo; chiP[it] = chi;
     }
     __syncthreads();
 
     // phase 2: the column solve of conz3d_nodes, one work-item per node; the
-    // tracers share the column matrix and its elimination
+    // tracers share the column matrix and its elimination.  Layer l of node
+    // kk sits at list position lstart[l] + (tile nodes before kk that have
+    // a layer l).
     const int kk = threadIdx.x, k = k0 + kk;
-    if (kk >= TN || k >= nkn) return;
+    if (kk >= tn) return;
     if (a.mem_istot && a.isact > a.mem_istot[a.node_mem[k]]) return;
     const int lmaxk = a.ilhkv[k];
-    auto cn = [&](int t, int l0) -> double & { return at(cnP + t * NL * TN, l0, kk); };
-    auto cd = [&](int l0) -> double & { return at(cdP, l0, kk); };
-    auto clo = [&](int l0) -> double & { return at(cloP, l0, kk); };
-    auto chi = [&](int l0) -> double & { return at(chiP, l0, kk); };
+    auto pos = [&](int l0) -> int { return lbase[wv][l0 + 1] + __popcll(lmask[wv][l0 + 1] & below); };
+    auto cn = [&](int t, int l0) -> double & { return cnP[t][pos(l0)]; };
+    auto cd = [&](int l0) -> double & { return cdP[pos(l0)]; };
+    auto clo = [&](int l0) -> double & { return cloP[pos(l0)]; };
+    auto chi = [&](int l0) -> double & { return chiP[pos(l0)]; };
     bool bad = false;
     if ((aa == 0. && ad == 0.) || a.nlv == 1) {
         for (int l = 1; l <= lmaxk; l++)
@@ -593,15 +594,12 @@ static int conz_nl(Model &M, const TracerSet &ts, int istot, int isact, hipStrea
     }
     a.hbuf = (S *)hbuf;
     a.err = M.d_err;
-    a.node_tile = conz_node_tile(M.nlv, M.p.node_tile);
-    const int ge = (M.nel + kElemTile - 1) / kElemTile;
+    if (!M.m.conz_eitem || !M.m.conz_ntile) return hip_check(M, hipErrorNotInitialized, "conz3d: no basin");
+    a.eitem = M.m.conz_eitem; a.n_eitem = M.m.n_eitem; a.ntile = M.m.conz_ntile;
+    const int ge = (M.m.n_eitem + kThreads - 1) / kThreads;
     launch_timed(M, T_CONZ_ELEM, k_conz_elem<S, NT>, dim3(ge), dim3(kThreads), 0, st, a);
     HIPCHK(M, hipGetLastError());
-    const int tn = a.node_tile;
-    if (conz_node_lds(NL, NT, tn) > 65536)
-        return hip_check(M, hipErrorInvalidValue, "conz3d: node_tile too large for the LDS of one workgroup");
-    launch_timed(M, T_CONZ_NODE, k_conz_node<S, NL, NT>, dim3((M.nkn + tn - 1) / tn),
-                 dim3(std::min(kThreads, (tn * M.nlv + 63) / 64 * 64)), conz_node_lds(NL, NT, tn), st, a);
+    launch_timed(M, T_CONZ_NODE, k_conz_node<S, NL, NT>, dim3(M.m.n_ntile), dim3(kThreads), 0, st, a);
     HIPCHK(M, hipGetLastError());
     return 0;
 }

@@ -24,6 +24,13 @@ constexpr int kThreads = 256;
 constexpr int kMaxBlockRows = 1024;                  // block-ILU(0) rows per workgroup (LDS)
 constexpr int kEllWidth = 10;                        // matrix entries per row held in ELL slots
 constexpr int kIncWidth = 10;                        // incident elements per node held in slots
+// conz3d work division: the element-layer list is built in chunks of
+// kConzChunk consecutive elements; a node tile holds at most kNodeTileMax
+// nodes (one work-item each) and kNodeItems node-layers (one pass of a
+// workgroup)
+constexpr int kConzChunk = 64;
+constexpr int kNodeTileMax = kThreads;
+constexpr int kNodeItems = kThreads;
 
 // error codes written by kernels into Model::d_err[0]
 enum DevErr { ERR_NONE = 0, ERR_SETNOD = 1, ERR_SETUVD = 3, ERR_SETDEPTH = 4, ERR_VOLUME = 5,
@@ -84,6 +91,11 @@ struct Mesh {                 // device copies of init-time data
     void *tvd_w = nullptr;    // double [3][6][nel] interpolation weights (adjusted xi)
     int *tvd_lev = nullptr;   // [6][nel] bottom layer of the upwind element (0 if none)
     int *elem_iext = nullptr; // [nel] vertices of the element flagged external (iextb)
+    // conz3d work division (set_basin): the element-layer list, ie << 5 |
+    // (l - 1), in chunks of kConzChunk elements taken layer by layer; node
+    // tiles as first nodes [n_ntile + 1], built for at most ntile_tn nodes
+    int *conz_eitem = nullptr, *conz_ntile = nullptr;
+    int n_eitem = 0, n_ntile = 0, ntile_tn = 0;
     // scalar open boundary (bndo_init): nodes, inward normals, inner neighbours
     int nbndo = 0;
     int *bndo_k = nullptr, *bndo_ptr = nullptr, *bndo_nb = nullptr;
