@@ -312,7 +312,7 @@ int shyhip_destroy(shyhip_model *h) {
                     (void *)m.blk_elp, (void *)m.blk_elr, (void *)m.blk_enlev, (void *)m.ell_col, (void *)m.ell_len, (void *)m.csr_ell,
                     (void *)m.fwd_rows, (void *)m.fwd_lptr, (void *)m.fwd_nlev, (void *)m.bwd_rows,
                     (void *)m.bwd_lptr, (void *)m.bwd_nlev, (void *)m.tvd_el, (void *)m.tvd_kn, m.tvd_w,
-                    (void *)m.tvd_lev, (void *)m.elem_iext, (void *)m.conz_eitem, (void *)m.conz_ntile})
+                    (void *)m.tvd_lev, (void *)m.elem_iext, (void *)m.conz_eitem, (void *)m.conz_ntile, (void *)m.conz_nitem})
         dfree(q);
     Solver &v = M.sv;
     for (void *q : {v.lu, v.ell_a, v.fs_val, v.r0, v.r, v.p0, v.p1, v.v0, v.v1, v.s, v.t, v.dx, v.b, v.x0, (void *)v.part,
@@ -331,6 +331,8 @@ int shyhip_destroy(shyhip_model *h) {
                     (void *)M.iweg_tr[0], (void *)M.iweg_tr[1]})
         dfree(q);
     for (void *q : M.grp_hbuf) dfree(q);
+    for (void *q : M.grp_bscal) dfree(q);
+    dfree(M.bscal1);
     forcing_free_bnd(M);
     forcing_free_wind(M);
     forcing_free_sources(M);
@@ -583,6 +585,19 @@ int shyhip_set_basin(shyhip_model *h, const int *nen3v, const float *xgv, const 
             nl += ilhkv[k];
         }
         tiles.push_back(nkn);
+        std::vector<int> nitems;
+        nitems.reserve((size_t)nkn * 2);
+        for (int c0 = 0; c0 < nkn; c0 += kConzChunk) {
+            const int c1 = std::min(nkn, c0 + kConzChunk);
+            int lmax = 0;
+            for (int k = c0; k < c1; k++) lmax = std::max(lmax, ilhkv[k]);
+            for (int l = 1; l <= lmax; l++)
+                for (int k = c0; k < c1; k++)
+                    if (ilhkv[k] >= l) nitems.push_back((k << 5) | (l - 1));
+        }
+        if (nkn >= (1 << 26)) return fail(&M, "at most 2^26 nodes");
+        m.n_nitem = (int)nitems.size();
+        rc |= dalloc(M, &m.conz_nitem, nitems.size()) || upload(M, m.conz_nitem, nitems.data(), nitems.size());
         m.n_eitem = (int)items.size();
         m.n_ntile = (int)tiles.size() - 1;
         m.ntile_tn = tn;
@@ -758,6 +773,10 @@ int shyhip_set_boundaries(shyhip_model *h, int nbnd, const int *ibnd, const int 
     const int nkn = M.nkn;
     HIPCHK(M, hipStreamSynchronize(M.stream));
     forcing_free_bnd(M);                            // series belong to the previous boundaries
+    for (void *q : M.grp_bscal) dfree(q);           // sized for the previous boundaries
+    M.grp_bscal.clear();
+    dfree(M.bscal1);
+    M.bscal1 = nullptr;
     M.nbnd = nbnd;
     M.h_ibnd.assign(ibnd, ibnd + nkn);
     M.h_ibtyp.assign(ibtyp, ibtyp + nbnd + 1);
@@ -1522,7 +1541,7 @@ int shyhip_scalar_stability(shyhip_model *h, float rstol, int *istot, double *ri
 }
 
 // scal3sh of tracer itr on stream st with conz3d scratch hbuf
-static int scal3sh_on(Model &M, const TracerSet &ts, int istot, hipStream_t st, void *hbuf) {
+static int scal3sh_on(Model &M, const TracerSet &ts, int istot, hipStream_t st, void *hbuf, void *bscal) {
     for (int q = 0; q < ts.n; q++)
         if (ts.itr[q] < 0 || ts.itr[q] >= M.ntr) return fail(&M, "tracer index out of range");
     if (istot < 1 || istot > 1000) return fail(&M, "istot must be in 1..1000");
@@ -1530,7 +1549,7 @@ static int scal3sh_on(Model &M, const TracerSet &ts, int istot, hipStream_t st, 
     for (int isact = 1; isact <= istot; isact++) {
         if (launch_conz3d(M, ts, istot, isact, st, hbuf)) return 1;
         timer_begin(M, T_BNDO);
-        if (launch_bndo(M, ts, st, isact)) return 1;       // bndo_setbc
+        if (launch_bndo(M, ts, st, isact, bscal)) return 1;       // bndo_setbc
         timer_end(M);
     }
     return 0;
@@ -1553,12 +1572,18 @@ static TracerSet group_of(const Model &M, int g) {
 }
 // the element-term scratch of each group (sized for its tracers; created once)
 static int group_buffers(Model &M) {
-    if ((int)M.grp_hbuf.size() == ngroups(M)) return 0;
-    for (int g = 0; g < ngroups(M); g++) {
-        void *hb = nullptr;
-        if (dalloc_s(M, &hb, (size_t)group_of(M, g).n * 3 * M.nlvdi * M.nel)) return 1;
-        M.grp_hbuf.push_back(hb);
-    }
+    if ((int)M.grp_hbuf.size() != ngroups(M))
+        for (int g = 0; g < ngroups(M); g++) {
+            void *hb = nullptr;
+            if (dalloc_s(M, &hb, (size_t)group_of(M, g).n * 3 * M.nlvdi * M.nel)) return 1;
+            M.grp_hbuf.push_back(hb);
+        }
+    if ((int)M.grp_bscal.size() != ngroups(M))
+        for (int g = 0; g < ngroups(M); g++) {
+            void *bs = nullptr;
+            if (dalloc_s(M, &bs, (size_t)std::max(1, M.m.nbndo) * M.nlvdi)) return 1;
+            M.grp_bscal.push_back(bs);
+        }
     return 0;
 }
 
@@ -1600,7 +1625,8 @@ int shyhip_scal3sh(shyhip_model *h, int itr, int istot, int clip_small) {
     TracerSet ts;
     ts.itr[0] = itr;
     ts.clip[0] = clip_small;
-    return scal3sh_on(h->M, ts, istot, h->M.stream, h->M.s.hbuf);
+    if (!h->M.bscal1 && dalloc_s(h->M, &h->M.bscal1, (size_t)std::max(1, h->M.m.nbndo) * h->M.nlvdi)) return 1;
+    return scal3sh_on(h->M, ts, istot, h->M.stream, h->M.s.hbuf, h->M.bscal1);
 }
 
 // streams and events of the tracer groups after the first (created once)
@@ -1680,7 +1706,7 @@ int shyhip_step(shyhip_model *h, double dtime, int istot) {
         for (int g = 0; g < ngroups(M); g++) {
             hipStream_t st = g == 0 ? M.tr0_stream : M.tr_stream[g - 1];
             HIPCHK(M, hipStreamWaitEvent(st, M.ev_hydro, 0));
-            if (scal3sh_on(M, group_of(M, g), istot, st, M.grp_hbuf[g])) { M.iweg_conz = nullptr; return 1; }
+            if (scal3sh_on(M, group_of(M, g), istot, st, M.grp_hbuf[g], M.grp_bscal[g])) { M.iweg_conz = nullptr; return 1; }
             HIPCHK(M, hipEventRecord(M.pipe_ev[par][g], st));
         }
         M.iweg_conz = nullptr;
@@ -1696,15 +1722,15 @@ int shyhip_step(shyhip_model *h, double dtime, int istot) {
             // its own stream, joined back into the model's stream
             HIPCHK(M, hipEventRecord(M.ev_hydro, M.stream));
             for (int g = 1; g < ngroups(M); g++) HIPCHK(M, hipStreamWaitEvent(M.tr_stream[g - 1], M.ev_hydro, 0));
-            if (scal3sh_on(M, group_of(M, 0), istot, M.stream, M.grp_hbuf[0])) return 1;
+            if (scal3sh_on(M, group_of(M, 0), istot, M.stream, M.grp_hbuf[0], M.grp_bscal[0])) return 1;
             for (int g = 1; g < ngroups(M); g++) {
-                if (scal3sh_on(M, group_of(M, g), istot, M.tr_stream[g - 1], M.grp_hbuf[g])) return 1;
+                if (scal3sh_on(M, group_of(M, g), istot, M.tr_stream[g - 1], M.grp_hbuf[g], M.grp_bscal[g])) return 1;
                 HIPCHK(M, hipEventRecord(M.tr_done[g - 1], M.tr_stream[g - 1]));
                 HIPCHK(M, hipStreamWaitEvent(M.stream, M.tr_done[g - 1], 0));
             }
         } else {
             for (int g = 0; g < ngroups(M); g++)
-                if (scal3sh_on(M, group_of(M, g), istot, M.stream, M.grp_hbuf[g])) return 1;
+                if (scal3sh_on(M, group_of(M, g), istot, M.stream, M.grp_hbuf[g], M.grp_bscal[g])) return 1;
         }
     }
     // barocl: the density of the transported T/S (rhoset_shell(1),

@@ -644,6 +644,12 @@ struct BndoArgs {
     int *err;
     const int *mem_istot, *node_mem;
     int isact;
+    // the outflow test's normal velocity scal per node and layer
+    // ([i * nlvdi + l - 1]): computed and kept by the first substep of a
+    // step (the velocities and the wet/dry mask hold for the whole step),
+    // read by the others; null: computed by every call
+    S *scal;
+    int nlvdi;
 };
 
 template <typename S>
@@ -663,24 +669,30 @@ __global__ __launch_bounds__(kBndoLanes) void k_bndo(BndoArgs<S> a) {
     const R flag = (R)-999.;
     status[lane] = 0;
     if (l <= lmax) {
-        R vv = 0, up = 0, vp = 0;
-        for (int q = a.eptr[i]; q < a.eptr[i + 1]; q++) {
-            const int ie = a.el[q];
-            if (a.iweg[ie] != 0 || l > a.elev[q]) continue;
-            const R aj = (R)a.eaj[q];
-            vv = vv + aj;
-            up = up + aj * a.uln[(size_t)(l - 1) * nel + ie];
-            vp = vp + aj * a.vln[(size_t)(l - 1) * nel + ie];
-        }
         const size_t o = (size_t)(l - 1) * nkn + k;
         R cur = cv[o];
         if (rbc) {
             const R rb = rbc[o];
             if (rb != flag) cur = rb;
         }
-        R u = up, v = vp;
-        if (vv > (R)0) { u = u / vv; v = v / vv; }
-        const R scal = a.bnx[i] * u + a.bny[i] * v;
+        R scal;
+        if (a.scal && a.isact > 1) {
+            scal = a.scal[(size_t)i * a.nlvdi + l - 1];
+        } else {
+            R vv = 0, up = 0, vp = 0;
+            for (int q = a.eptr[i]; q < a.eptr[i + 1]; q++) {
+                const int ie = a.el[q];
+                if (a.iweg[ie] != 0 || l > a.elev[q]) continue;
+                const R aj = (R)a.eaj[q];
+                vv = vv + aj;
+                up = up + aj * a.uln[(size_t)(l - 1) * nel + ie];
+                vp = vp + aj * a.vln[(size_t)(l - 1) * nel + ie];
+            }
+            R u = up, v = vp;
+            if (vv > (R)0) { u = u / vv; v = v / vv; }
+            scal = a.bnx[i] * u + a.bny[i] * v;
+            if (a.scal && blockIdx.y == 0) a.scal[(size_t)i * a.nlvdi + l - 1] = scal;
+        }
         const bool binside = scal <= (R)0 || cur == flag;
         R bc = 0, tw = 0;
         int ntbc = 0;
@@ -711,7 +723,7 @@ __global__ __launch_bounds__(kBndoLanes) void k_bndo(BndoArgs<S> a) {
 }
 
 template <typename S>
-static int bndo_t(Model &M, const TracerSet &ts, hipStream_t st, int isact) {
+static int bndo_t(Model &M, const TracerSet &ts, hipStream_t st, int isact, void *scal) {
     if (M.m.nbndo == 0) return 0;
     if (M.nlv > kBndoLanes) return hip_check(M, hipErrorInvalidValue, "bndo: more layers than lanes");
     BndoArgs<S> a{};
@@ -730,20 +742,22 @@ static int bndo_t(Model &M, const TracerSet &ts, hipStream_t st, int isact) {
     a.err = M.d_err;
     if (M.mem_istot_on) { a.mem_istot = M.cur_mem_istot; a.node_mem = M.d_node_mem; }
     a.isact = isact;
+    a.scal = (S *)scal;
+    a.nlvdi = M.nlvdi;
     hipLaunchKernelGGL(k_bndo<S>, dim3(M.m.nbndo, ts.n), dim3(kBndoLanes), 0, st, a);
     HIPCHK(M, hipGetLastError());
     return 0;
 }
 
-int launch_bndo(Model &M, const TracerSet &ts, hipStream_t st, int isact) {
-    return M.prec == SHYHIP_FP64 ? bndo_t<double>(M, ts, st, isact) : bndo_t<float>(M, ts, st, isact);
+int launch_bndo(Model &M, const TracerSet &ts, hipStream_t st, int isact, void *scal) {
+    return M.prec == SHYHIP_FP64 ? bndo_t<double>(M, ts, st, isact, scal) : bndo_t<float>(M, ts, st, isact, scal);
 }
 
 int launch_bndo(Model &M, int itr, hipStream_t st, int isact) {
     TracerSet ts;
     ts.n = 1;
     ts.itr[0] = itr;
-    return launch_bndo(M, ts, st, isact);
+    return launch_bndo(M, ts, st, isact, nullptr);
 }
 
 int launch_conz3d(Model &M, const TracerSet &ts, int istot, int isact, hipStream_t st, void *hbuf) {
@@ -801,13 +815,19 @@ __device__ inline void max_into(unsigned long long *p, unsigned long long v) {
 }
 
 // the ratio of one node-layer (l, k) as the bits of a non-negative double,
-// 0 when it adds nothing
+// 0 when it adds nothing.  The incident elements are taken in element order
+// kStabBatch at a time: each batch's loads are issued together (the list
+// entries, then every element's data and its vertices, then the vertices'
+// vertical velocities), and the batch is then added in order.
+constexpr int kStabBatch = 3;
+
 template <typename S>
-__device__ inline unsigned long long stab_ratio(int l, int k, int nkn, int nel, const int *ilhv, const int *nk_ptr,
-                                                const int *nk_el, const int *nen3v, const double *geo, double az,
-                                                double ad, double aa, const S *utn, const S *uto, const S *vtn,
-                                                const S *vto, const S *wln, const S *difv, const S *hdeo,
-                                                const S *hdkn, const S *hdko, const S *mflux) {
+__device__ inline unsigned long long stab_ratio(int l, int k, int nkn, int nel, int nlv, const int *ilhv,
+                                                const int *nk_ptr, const int *nk_el, const int *nen3v,
+                                                const double *geo, double az, double ad, double aa, const S *utn,
+                                                const S *uto, const S *vtn, const S *vto, const S *wln,
+                                                const S *difv, const S *hdeo, const S *hdkn, const S *hdko,
+                                                const S *mflux) {
     const double azt = 1. - az, adt = 1. - ad, aat = 1. - aa;
     double chigh = 0., cn = 0., cdiag = 0.;
     double co = 0.;
@@ -815,41 +835,75 @@ __device__ inline unsigned long long stab_ratio(int l, int k, int nkn, int nel, 
         const S q = mflux[(size_t)(l - 1) * nkn + k];
         if (q > (S)0) co = (double)q;
     }
-    for (int p = nk_ptr[k]; p < nk_ptr[k + 1]; p++) {
-        const int ie = nk_el[p] >> 2, iv = nk_el[p] & 3;
-        const int ilevel = ilhv[ie];
-        if (l > ilevel) continue;
-        const double aj4 = 4. * geo[(size_t)6 * nel + ie];
-        const size_t le = (size_t)(l - 1) * nel + ie;
-        const double us = az * (double)utn[le] + azt * (double)uto[le];
-        const double vs = az * (double)vtn[le] + azt * (double)vto[le];
-        double coadv = 0., ciadv = 0., cadv = 0.;
-        for (int jj = 0; jj <= iv; jj++) {
-            const int kj = nen3v[3 * ie + jj];
-            const double f = us * geo[(size_t)jj * nel + ie] + vs * geo[(size_t)(3 + jj) * nel + ie];
-            double fwin = 0., fwout = 0.;
-            const double wt = l > 1 ? (double)wln[(size_t)(l - 1) * nkn + kj] : 0.;
-            const double wb = l < ilevel ? (double)wln[(size_t)l * nkn + kj] : 0.;
-            if (wt > 0.) fwout = fwout + aat * wt;
-            else fwin = fwin - aat * wt;
-            if (wb > 0.) fwin = fwin + aat * wb;
-            else fwout = fwout - aat * wb;
-            const double chadv = aj4 * 3. * f;
-            const double cviadv = aj4 * fwin, cvoadv = aj4 * fwout;
-            if (chadv < 0.) coadv = cvoadv - chadv;
-            else ciadv = cviadv + chadv;
-            cadv = coadv > ciadv ? coadv : ciadv;
+    // the node's own data (the layer below exists for elements deeper than l)
+    const double dtop = l > 1 ? (double)difv[(size_t)(l - 1) * nkn + k] : 0.;
+    const double dbot = l < nlv ? (double)difv[(size_t)l * nkn + k] : 0.;
+    const double hn = (double)hdkn[(size_t)(l - 1) * nkn + k], ho = (double)hdko[(size_t)(l - 1) * nkn + k];
+    const int p0 = nk_ptr[k], p1 = nk_ptr[k + 1];
+    for (int pb = p0; pb < p1; pb += kStabBatch) {
+        int ie[kStabBatch], iv[kStabBatch], ilev[kStabBatch], kj[kStabBatch][3];
+        double aj4[kStabBatch], us[kStabBatch], vs[kStabBatch], hdv[kStabBatch], hdt[kStabBatch], hdb[kStabBatch];
+        double gb[kStabBatch][3], gc[kStabBatch][3], wt[kStabBatch][3], wb[kStabBatch][3];
+#pragma unroll
+        for (int b = 0; b < kStabBatch; b++) {
+            const int e = nk_el[min(pb + b, p1 - 1)];
+            ie[b] = e >> 2;
+            iv[b] = e & 3;
         }
-        chigh = chigh + cadv;
-        const double hdv = (double)hdeo[le];
-        double hmtop = 0., hmbot = 0.;
-        if (l > 1)
-            hmtop = 2. * (double)difv[(size_t)(l - 1) * nkn + k] / ((double)hdeo[le - nel] + hdv);
-        if (l < ilevel)
-            hmbot = 2. * (double)difv[(size_t)l * nkn + k] / (hdv + (double)hdeo[le + nel]);
-        cn = cn + aj4 * (adt * (hmtop + hmbot));
-        const double hn = (double)hdkn[(size_t)(l - 1) * nkn + k], ho = (double)hdko[(size_t)(l - 1) * nkn + k];
-        cdiag = cdiag + aj4 * (ho < hn ? ho : hn);
+#pragma unroll
+        for (int b = 0; b < kStabBatch; b++) {
+            const int el = ie[b];
+            const size_t le = (size_t)(l - 1) * nel + el;
+            ilev[b] = ilhv[el];
+            aj4[b] = 4. * geo[(size_t)6 * nel + el];
+            us[b] = az * (double)utn[le] + azt * (double)uto[le];
+            vs[b] = az * (double)vtn[le] + azt * (double)vto[le];
+            hdv[b] = (double)hdeo[le];
+            hdt[b] = l > 1 ? (double)hdeo[le - nel] : 0.;
+            hdb[b] = l < nlv ? (double)hdeo[le + nel] : 0.;
+#pragma unroll
+            for (int jj = 0; jj < 3; jj++) {
+                kj[b][jj] = nen3v[3 * el + jj];
+                gb[b][jj] = geo[(size_t)jj * nel + el];
+                gc[b][jj] = geo[(size_t)(3 + jj) * nel + el];
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < kStabBatch; b++)
+#pragma unroll
+            for (int jj = 0; jj < 3; jj++) {
+                wt[b][jj] = l > 1 ? (double)wln[(size_t)(l - 1) * nkn + kj[b][jj]] : 0.;
+                wb[b][jj] = l < nlv ? (double)wln[(size_t)l * nkn + kj[b][jj]] : 0.;
+            }
+#pragma unroll
+        for (int b = 0; b < kStabBatch; b++) {
+            if (pb + b >= p1 || l > ilev[b]) continue;
+            const bool below = l < ilev[b];
+            double coadv = 0., ciadv = 0., cadv = 0.;
+#pragma unroll
+            for (int jj = 0; jj < 3; jj++) {
+                if (jj > iv[b]) break;
+                const double f = us[b] * gb[b][jj] + vs[b] * gc[b][jj];
+                double fwin = 0., fwout = 0.;
+                const double w_t = wt[b][jj];
+                const double w_b = below ? wb[b][jj] : 0.;
+                if (w_t > 0.) fwout = fwout + aat * w_t;
+                else fwin = fwin - aat * w_t;
+                if (w_b > 0.) fwin = fwin + aat * w_b;
+                else fwout = fwout - aat * w_b;
+                const double chadv = aj4[b] * 3. * f;
+                const double cviadv = aj4[b] * fwin, cvoadv = aj4[b] * fwout;
+                if (chadv < 0.) coadv = cvoadv - chadv;
+                else ciadv = cviadv + chadv;
+                cadv = coadv > ciadv ? coadv : ciadv;
+            }
+            chigh = chigh + cadv;
+            double hmtop = 0., hmbot = 0.;
+            if (l > 1) hmtop = 2. * dtop / (hdt[b] + hdv[b]);
+            if (below) hmbot = 2. * dbot / (hdv[b] + hdb[b]);
+            cn = cn + aj4[b] * (adt * (hmtop + hmbot));
+            cdiag = cdiag + aj4[b] * (ho < hn ? ho : hn);
+        }
     }
     if (cdiag > 0.) {
         const double r = ((chigh + 0.) + cn + co) / cdiag;
@@ -858,24 +912,24 @@ __device__ inline unsigned long long stab_ratio(int l, int k, int nkn, int nel, 
     return 0ull;
 }
 
+// One work-item per entry of the node-layer list.
 template <typename S>
-__global__ void __launch_bounds__(kThreads) k_conz_stab(int nkn, int nel, int nlv, const int *ilhkv, const int *ilhv,
-                                                        const int *zbound, const int *nk_ptr, const int *nk_el,
-                                                        const int *nen3v, const double *geo, double az, double ad,
-                                                        double aa, const S *utn, const S *uto, const S *vtn,
-                                                        const S *vto, const S *wln, const S *difv, const S *hdeo,
-                                                        const S *hdkn, const S *hdko, const S *mflux,
+__global__ void __launch_bounds__(kThreads) k_conz_stab(int nkn, int nel, int nlv, int n_nitem, const int *nitem,
+                                                        const int *ilhv, const int *zbound, const int *nk_ptr,
+                                                        const int *nk_el, const int *nen3v, const double *geo,
+                                                        double az, double ad, double aa, const S *utn, const S *uto,
+                                                        const S *vtn, const S *vto, const S *wln, const S *difv,
+                                                        const S *hdeo, const S *hdkn, const S *hdko, const S *mflux,
                                                         const int *node_mem, unsigned long long *dmax) {
-    // (blocks in launch order: the layer-major index puts the work of the
-    // upper layers in the first blocks, which round-robin spreads over XCDs)
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = xcd_block() * blockDim.x + threadIdx.x;
     unsigned long long val = 0ull;    // this node-layer's ratio as bits (0: none)
     int key = 0;                      // its member
-    if (i < nlv * nkn) {
-        const int l = i / nkn + 1, k = i - (l - 1) * nkn;
+    if (i < n_nitem) {
+        const unsigned w = (unsigned)nitem[i];
+        const int k = (int)(w >> 5), l = (int)(w & 31u) + 1;
         if (node_mem) key = node_mem[k];
-        if (l <= ilhkv[k] && !zbound[k]) val = stab_ratio(l, k, nkn, nel, ilhv, nk_ptr, nk_el, nen3v, geo, az, ad,
-                                                         aa, utn, uto, vtn, vto, wln, difv, hdeo, hdkn, hdko, mflux);
+        if (!zbound[k]) val = stab_ratio(l, k, nkn, nel, nlv, ilhv, nk_ptr, nk_el, nen3v, geo, az, ad, aa, utn, uto,
+                                         vtn, vto, wln, difv, hdeo, hdkn, hdko, mflux);
     }
     // the wavefront's maximum when its node-layers belong to one member
     // (always, for a single model); lanes of mixed waves go one by one
@@ -931,9 +985,9 @@ template <typename S>
 static int stab_t(Model &M, float rstol, int *istot, double *rindex) {
     const int nmem = M.nmem_dev > 1 ? M.nmem_dev : 1;
     unsigned long long *w = nmem > 1 ? M.d_mem_stab : (unsigned long long *)(M.d_err + 8);
-    const int n = M.nlv * M.nkn;
+    const int n = M.m.n_nitem;
     hipLaunchKernelGGL(k_conz_stab<S>, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, M.stream, M.nkn,
-                       M.nel, M.nlv, M.m.ilhkv, M.m.ilhv, M.m.zbound, M.m.nk_ptr, M.m.nk_el, M.m.nen3v, M.m.geo,
+                       M.nel, M.nlv, n, M.m.conz_nitem, M.m.ilhv, M.m.zbound, M.m.nk_ptr, M.m.nk_el, M.m.nen3v, M.m.geo,
                        (double)M.p.azpar, (double)M.p.adpar, (double)M.p.aapar, (const S *)M.s.utn,
                        (const S *)M.s.uto, (const S *)M.s.vtn, (const S *)M.s.vto, (const S *)M.s.wln,
                        (const S *)M.s.difv, (const S *)M.s.hdeo, (const S *)M.s.hdkn, (const S *)M.s.hdko,

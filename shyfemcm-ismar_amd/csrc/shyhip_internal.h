@@ -96,6 +96,10 @@ struct Mesh {                 // device copies of init-time data
     // tiles as first nodes [n_ntile + 1], built for at most ntile_tn nodes
     int *conz_eitem = nullptr, *conz_ntile = nullptr;
     int n_eitem = 0, n_ntile = 0, ntile_tn = 0;
+    // the node-layer list, k << 5 | (l - 1), in chunks of kConzChunk nodes
+    // taken layer by layer (scalar_stability)
+    int *conz_nitem = nullptr;
+    int n_nitem = 0;
     // scalar open boundary (bndo_init): nodes, inward normals, inner neighbours
     int nbndo = 0;
     int *bndo_k = nullptr, *bndo_ptr = nullptr, *bndo_nb = nullptr;
@@ -250,6 +254,11 @@ struct Model {
     std::vector<hipEvent_t> tr_done;
     hipEvent_t ev_hydro = nullptr;
     std::vector<void *> grp_hbuf;     // conz3d element terms of each tracer group
+    // bndo_setbc's per-step outflow test of each tracer group, and of the
+    // single-tracer scal3sh entry point (nbndo * nlvdi values; freed when
+    // the boundaries change)
+    std::vector<void *> grp_bscal;
+    void *bscal1 = nullptr;
     std::string err;
     // host-side copies needed for init / plugin / checks
     std::vector<int> h_nen3v, h_rowptr, h_cols, h_ijp, h_color, h_ilhv, h_ilhkv, h_ibnd, h_ibtyp, h_iextb;
@@ -420,7 +429,10 @@ int launch_conz3d(Model &M, const TracerSet &ts, int istot, int isact, hipStream
 int launch_conz3d(Model &M, int itr, int istot, int isact, int clip_small, hipStream_t st, void *hbuf,
                   const void *cbound = nullptr);
 int launch_bndo(Model &M, int itr, hipStream_t st, int isact = 1);
-int launch_bndo(Model &M, const TracerSet &ts, hipStream_t st, int isact);
+// bndo_setbc of substep isact; scal: the step's outflow-test buffer of the
+// tracer group (nbndo * nlvdi storage values, written by substep 1, read by
+// the later ones; null: recomputed by every call)
+int launch_bndo(Model &M, const TracerSet &ts, hipStream_t st, int isact, void *scal);
 // scalar_stability on the current state: istot and rindex of each member
 // (one entry per member when the model holds several, else one)
 // (synchronises)
