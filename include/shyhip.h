@@ -316,6 +316,14 @@ int  shyhip_sync(shyhip_model *h);
  * returned by the next call that waits for the device: the next step's
  * solve, shyhip_sync, or a pull.  Off by default. */
 int  shyhip_set_deferred_checks(shyhip_model *h, int on);
+/* HIP graphs (no counterpart in the reference): each tracer group's
+ * transport substeps (conz3d and bndo_setbc launches of a step) and each
+ * chunk of zeta-solver iterations are captured once per distinct argument
+ * set and replayed from the instantiated graph, one launch instead of
+ * dozens.  On by default; the results are the same bits either way.
+ * shyhip_graph_stats: graphs instantiated and replays since create. */
+int  shyhip_set_graphs(shyhip_model *h, int on);
+int  shyhip_graph_stats(shyhip_model *h, long *builds, long *replays);
 /* kernel-group timing with HIP events on the handle's stream (off by
  * default; enabling it adds a synchronisation at the end of every step).
  * id = 0 .. 17: set_dry, copy_uvz, hydro_intern, setnod, zeta_assemble,

@@ -26,6 +26,38 @@ int hip_check(Model &M, hipError_t e, const char *what) {
     return 1;
 }
 
+int graph_run(Model &M, hipStream_t st, const std::string &key, const std::function<int()> &launches) {
+    if (!M.use_graphs || M.timing) return launches();
+    auto it = M.graphs.find(key);
+    if (it == M.graphs.end()) {
+        // a bounded cache: the argument sets of a run are few (rotating
+        // buffers, substep counts); start over if a run keeps making new ones
+        if (M.graphs.size() >= 256) {
+            for (auto &g : M.graphs) (void)hipGraphExecDestroy(g.second);
+            M.graphs.clear();
+        }
+        HIPCHK(M, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int rc = launches();
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        HIPCHK(M, e);
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPCHK(M, ei);
+        it = M.graphs.emplace(key, exec).first;
+        M.graph_builds++;
+    } else {
+        M.graph_replays++;
+    }
+    HIPCHK(M, hipGraphLaunch(it->second, st));
+    return 0;
+}
+
 void timer_begin(Model &M, int id) {
     if (!M.timing) return;
     Model::TEv t{id, nullptr, nullptr};
@@ -286,6 +318,9 @@ int shyhip_destroy(shyhip_model *h) {
     if (!h) return 0;
     Model &M = h->M;
     if (M.stream) (void)hipStreamSynchronize(M.stream);
+    (void)pipe_drain(M);
+    for (auto &g : M.graphs) (void)hipGraphExecDestroy(g.second);
+    M.graphs.clear();
     State &s = M.s;
     for (void *q : {s.zn, s.zo, s.zen, s.zeo, s.utn, s.vtn, s.uto, s.vto, s.uln, s.vln, s.un, s.vn, s.uo, s.vo,
                     s.wln, s.wlo, s.hdkn, s.hdko, s.hden, s.hdeo, s.rfric, s.ddx, s.ddy, s.visv, s.difv, s.difhv,
@@ -1546,13 +1581,19 @@ static int scal3sh_on(Model &M, const TracerSet &ts, int istot, hipStream_t st, 
         if (ts.itr[q] < 0 || ts.itr[q] >= M.ntr) return fail(&M, "tracer index out of range");
     if (istot < 1 || istot > 1000) return fail(&M, "istot must be in 1..1000");
     if (M.p.itvd == 2 && !M.tvd_ready) return fail(&M, "itvd = 2 needs shyhip_tvd_init");
-    for (int isact = 1; isact <= istot; isact++) {
-        if (launch_conz3d(M, ts, istot, isact, st, hbuf)) return 1;
-        timer_begin(M, T_BNDO);
-        if (launch_bndo(M, ts, st, isact, bscal)) return 1;       // bndo_setbc
-        timer_end(M);
-    }
-    return 0;
+    auto substeps = [&]() -> int {
+        for (int isact = 1; isact <= istot; isact++) {
+            if (launch_conz3d(M, ts, istot, isact, st, hbuf)) return 1;
+            timer_begin(M, T_BNDO);
+            if (launch_bndo(M, ts, st, isact, bscal)) return 1;       // bndo_setbc
+            timer_end(M);
+        }
+        return 0;
+    };
+    if (!M.use_graphs || M.timing) return substeps();
+    std::string key("scal3sh");
+    conz3d_key(M, ts, istot, hbuf, bscal, key);
+    return graph_run(M, st, key, substeps);
 }
 
 // The tracers of a step are transported in groups: pairs (0,1), (2,3), ...
@@ -1745,6 +1786,21 @@ int shyhip_set_deferred_checks(shyhip_model *h, int on) {
     if (!h) return fail(nullptr, "null handle");
     if (pipe_drain(h->M)) return 1;
     h->M.deferred = on != 0;
+    return 0;
+}
+
+int shyhip_set_graphs(shyhip_model *h, int on) {
+    if (!h) return fail(nullptr, "null handle");
+    if (pipe_drain(h->M)) return 1;
+    HIPCHK(h->M, hipStreamSynchronize(h->M.stream));
+    h->M.use_graphs = on != 0;
+    return 0;
+}
+
+int shyhip_graph_stats(shyhip_model *h, long *builds, long *replays) {
+    if (!h || !builds || !replays) return fail(h ? &h->M : nullptr, "null argument");
+    *builds = h->M.graph_builds;
+    *replays = h->M.graph_replays;
     return 0;
 }
 

@@ -561,9 +561,27 @@ __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
         for (int l = 1; l <= a.nlvdi; l++) a.c[t][(size_t)(l - 1) * nkn + k] = (S)(l <= lmaxk ? cn(t, l - 1) : 0.);
 }
 
+template <typename S>
+static ConzArgs<S> conz_args(Model &M, const TracerSet &ts, int istot, int isact, void *hbuf);
+
 template <typename S, int NL, int NT>
 static int conz_nl(Model &M, const TracerSet &ts, int istot, int isact, hipStream_t st, void *hbuf) {
-    ConzArgs<S> a{};
+    if (!M.m.conz_eitem || !M.m.conz_ntile) return hip_check(M, hipErrorNotInitialized, "conz3d: no basin");
+    const ConzArgs<S> a = conz_args<S>(M, ts, istot, isact, hbuf);
+    const int ge = (M.m.n_eitem + kThreads - 1) / kThreads;
+    launch_timed(M, T_CONZ_ELEM, k_conz_elem<S, NT>, dim3(ge), dim3(kThreads), 0, st, a);
+    HIPCHK(M, hipGetLastError());
+    launch_timed(M, T_CONZ_NODE, k_conz_node<S, NL, NT>, dim3(M.m.n_ntile), dim3(kThreads), 0, st, a);
+    HIPCHK(M, hipGetLastError());
+    return 0;
+}
+
+// The arguments of substep isact of istot (zeroed first, so that their
+// bytes, padding included, identify the launch).
+template <typename S>
+static ConzArgs<S> conz_args(Model &M, const TracerSet &ts, int istot, int isact, void *hbuf) {
+    ConzArgs<S> a;
+    std::memset(&a, 0, sizeof a);
     a.nkn = M.nkn; a.nel = M.nel; a.nlvdi = M.nlvdi; a.nlv = M.nlv;
     a.itvd = M.p.itvd; a.itvdv = M.p.itvdv;
     const double rstot = istot;
@@ -582,7 +600,7 @@ static int conz_nl(Model &M, const TracerSet &ts, int istot, int isact, hipStrea
     a.hdkn = (const S *)M.s.hdkn; a.hdko = (const S *)M.s.hdko; a.hden = (const S *)M.s.hden;
     a.hdeo = (const S *)M.s.hdeo; a.wln = (const S *)M.s.wln; a.difv = (const S *)M.s.difv;
     a.mflux = M.has_mflux ? (const S *)M.s.mflux : nullptr;
-    for (int t = 0; t < NT; t++) {
+    for (int t = 0; t < ts.n && t < kMaxTracers; t++) {
         a.c[t] = (S *)M.s.tracer[ts.itr[t]];
         a.cbound[t] = (const S *)ts.cbound[t];
         a.clip[t] = ts.clip[t];
@@ -594,14 +612,8 @@ static int conz_nl(Model &M, const TracerSet &ts, int istot, int isact, hipStrea
     }
     a.hbuf = (S *)hbuf;
     a.err = M.d_err;
-    if (!M.m.conz_eitem || !M.m.conz_ntile) return hip_check(M, hipErrorNotInitialized, "conz3d: no basin");
     a.eitem = M.m.conz_eitem; a.n_eitem = M.m.n_eitem; a.ntile = M.m.conz_ntile;
-    const int ge = (M.m.n_eitem + kThreads - 1) / kThreads;
-    launch_timed(M, T_CONZ_ELEM, k_conz_elem<S, NT>, dim3(ge), dim3(kThreads), 0, st, a);
-    HIPCHK(M, hipGetLastError());
-    launch_timed(M, T_CONZ_NODE, k_conz_node<S, NL, NT>, dim3(M.m.n_ntile), dim3(kThreads), 0, st, a);
-    HIPCHK(M, hipGetLastError());
-    return 0;
+    return a;
 }
 
 template <typename S, int NT>
@@ -723,10 +735,23 @@ __global__ __launch_bounds__(kBndoLanes) void k_bndo(BndoArgs<S> a) {
 }
 
 template <typename S>
+static BndoArgs<S> bndo_args(Model &M, const TracerSet &ts, int isact, void *scal);
+
+template <typename S>
 static int bndo_t(Model &M, const TracerSet &ts, hipStream_t st, int isact, void *scal) {
     if (M.m.nbndo == 0) return 0;
     if (M.nlv > kBndoLanes) return hip_check(M, hipErrorInvalidValue, "bndo: more layers than lanes");
-    BndoArgs<S> a{};
+    const BndoArgs<S> a = bndo_args<S>(M, ts, isact, scal);
+    hipLaunchKernelGGL(k_bndo<S>, dim3(M.m.nbndo, ts.n), dim3(kBndoLanes), 0, st, a);
+    HIPCHK(M, hipGetLastError());
+    return 0;
+}
+
+// (zeroed first, like conz_args)
+template <typename S>
+static BndoArgs<S> bndo_args(Model &M, const TracerSet &ts, int isact, void *scal) {
+    BndoArgs<S> a;
+    std::memset(&a, 0, sizeof a);
     a.nbndo = M.m.nbndo; a.nkn = M.nkn; a.nel = M.nel;
     a.bk = M.m.bndo_k; a.open = M.m.bndo_open; a.lmax = M.m.bndo_lmax; a.eptr = M.m.bndo_eptr; a.el = M.m.bndo_el;
     a.elev = M.m.bndo_elev; a.bptr = M.m.bndo_ptr; a.bnb = M.m.bndo_nb; a.nblev = M.m.bndo_nblev;
@@ -744,9 +769,7 @@ static int bndo_t(Model &M, const TracerSet &ts, hipStream_t st, int isact, void
     a.isact = isact;
     a.scal = (S *)scal;
     a.nlvdi = M.nlvdi;
-    hipLaunchKernelGGL(k_bndo<S>, dim3(M.m.nbndo, ts.n), dim3(kBndoLanes), 0, st, a);
-    HIPCHK(M, hipGetLastError());
-    return 0;
+    return a;
 }
 
 int launch_bndo(Model &M, const TracerSet &ts, hipStream_t st, int isact, void *scal) {
@@ -766,6 +789,22 @@ int launch_conz3d(Model &M, const TracerSet &ts, int istot, int isact, hipStream
                                      : conz_t<float, 2>(M, ts, istot, isact, st, hbuf);
     return M.prec == SHYHIP_FP64 ? conz_t<double, 1>(M, ts, istot, isact, st, hbuf)
                                  : conz_t<float, 1>(M, ts, istot, isact, st, hbuf);
+}
+
+void conz3d_key(Model &M, const TracerSet &ts, int istot, void *hbuf, void *bscal, std::string &key) {
+    key_add(key, istot);
+    key_add(key, M.nl_tmpl);
+    key_add(key, M.prec);
+    key_add(key, ts.n);
+    key_add(key, M.m.nbndo);
+    key_add(key, M.m.n_ntile);
+    if (M.prec == SHYHIP_FP64) {
+        key_add(key, conz_args<double>(M, ts, istot, 1, hbuf));
+        key_add(key, bndo_args<double>(M, ts, 1, bscal));
+    } else {
+        key_add(key, conz_args<float>(M, ts, istot, 1, hbuf));
+        key_add(key, bndo_args<float>(M, ts, 1, bscal));
+    }
 }
 
 int launch_conz3d(Model &M, int itr, int istot, int isact, int clip_small, hipStream_t st, void *hbuf,

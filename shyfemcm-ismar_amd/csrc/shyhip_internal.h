@@ -11,7 +11,9 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
 #include <cstdint>
+#include <functional>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/shyhip.h"
@@ -289,6 +291,11 @@ struct Model {
     Forcing f;                         // forcing series on the device
     // timing
     bool timing = false;
+    // launch sequences replayed from HIP graphs (graph_run): instantiated
+    // graphs by the bytes of their launch arguments; on by default
+    bool use_graphs = true;
+    std::unordered_map<std::string, hipGraphExec_t> graphs;
+    long graph_builds = 0, graph_replays = 0;
     struct TEv { int id; hipEvent_t a, b; };
     std::vector<TEv> tev;
     std::vector<int> tev_open;          // groups begun and not yet ended (nesting)
@@ -426,6 +433,9 @@ struct TracerSet {
     const void *cbound[2] = {nullptr, nullptr};
 };
 int launch_conz3d(Model &M, const TracerSet &ts, int istot, int isact, hipStream_t st, void *hbuf);
+// the bytes that identify the launches of a tracer group's substeps 1..istot
+// (conz3d and bndo_setbc), appended to key (a graph_run key)
+void conz3d_key(Model &M, const TracerSet &ts, int istot, void *hbuf, void *bscal, std::string &key);
 int launch_conz3d(Model &M, int itr, int istot, int isact, int clip_small, hipStream_t st, void *hbuf,
                   const void *cbound = nullptr);
 int launch_bndo(Model &M, int itr, hipStream_t st, int isact = 1);
@@ -440,6 +450,17 @@ int launch_scalar_stability(Model &M, float rstol, int *istot, double *rindex);
 
 // helpers
 int hip_check(Model &M, hipError_t e, const char *what);
+// A fixed sequence of kernel launches on stream st, replayed from a HIP
+// graph: key holds every argument of the launches (the bytes of their
+// argument structures and grids); the first call with a key captures the
+// launches on st and instantiates them, later calls launch the instantiated
+// graph.  The launches must be kernels only (no memset, copy or host
+// synchronisation).  Runs them directly when graphs are off or timing is on.
+int graph_run(Model &M, hipStream_t st, const std::string &key, const std::function<int()> &launches);
+template <typename T>
+inline void key_add(std::string &key, const T &v) {
+    key.append(reinterpret_cast<const char *>(&v), sizeof v);
+}
 #define HIPCHK(M, call) do { if (hip_check((M), (call), #call)) return 1; } while (0)
 
 }  // namespace shy

@@ -27,6 +27,8 @@
 // enqueues the expected number of rounds and reads the flags once.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include <algorithm>
 #include <cmath>
 #include <functional>
@@ -72,7 +74,9 @@ struct SolvArgs {
 
 template <typename V>
 static SolvArgs<V> solv_args(Model &M, const V *A, V *b, V *x0) {
-    SolvArgs<V> a{};
+    // zeroed first: the bytes, padding included, identify the launches
+    SolvArgs<V> a;
+    std::memset(&a, 0, sizeof a);
     a.n = M.nkn; a.nblk = M.sv.nblk; a.brows = M.sv.brows; a.mx = M.sv.max_blk_nnz; a.mxp = M.sv.max_blk_pairs;
     a.itermax = M.p.itermax;
     a.sweeps = M.p.tri_sweeps;
@@ -1060,17 +1064,32 @@ static int bicgstab_t(Model &M, SolvArgs<V> &a, int *iters, int slot, int zero_x
                       const std::function<int()> &tail) {
     const int g = a.nblk;
     const size_t lds = apply_lds<V>(M);
-    hipLaunchKernelGGL((k_bicg_init<V, MODE != 0>), dim3(g), dim3(kThreads), lds, M.stream, a, zero_x0);
-    HIPCHK(M, hipGetLastError());
     int it = 0, c[CT_WORDS] = {};
     const int maxit = M.p.itermax > 0 ? (M.p.itermax + 1) / 2 + 1 : 100000;
     int chunk = std::max(2, M.sv.last_its[slot] + 1);
     for (;;) {
-        for (int q = 0; q < chunk; q++, it++) {
-            launch_timed(M, T_BICG_A, k_bicg_a<V, MODE>, dim3(g), dim3(kThreads), lds, M.stream, a, it);
-            launch_timed(M, T_BICG_B, k_bicg_b<V, MODE>, dim3(g), dim3(kThreads), lds, M.stream, a, it);
-        }
-        HIPCHK(M, hipGetLastError());
+        // the chunk's iterations (after the initialisation, for the first)
+        // as one replayed launch sequence
+        const int it0 = it;
+        auto launches = [&]() -> int {
+            if (it0 == 0)
+                hipLaunchKernelGGL((k_bicg_init<V, MODE != 0>), dim3(g), dim3(kThreads), lds, M.stream, a, zero_x0);
+            for (int q = 0; q < chunk; q++) {
+                launch_timed(M, T_BICG_A, k_bicg_a<V, MODE>, dim3(g), dim3(kThreads), lds, M.stream, a, it0 + q);
+                launch_timed(M, T_BICG_B, k_bicg_b<V, MODE>, dim3(g), dim3(kThreads), lds, M.stream, a, it0 + q);
+            }
+            HIPCHK(M, hipGetLastError());
+            return 0;
+        };
+        std::string key("bicgstab");
+        key_add(key, a);
+        key_add(key, MODE);
+        key_add(key, lds);
+        key_add(key, it0);
+        key_add(key, chunk);
+        key_add(key, zero_x0);
+        if (graph_run(M, M.stream, key, launches)) return 1;
+        it += chunk;
         if (tail && tail()) return 1;
         if (read_ctrl(M, c, CT_WORDS)) return 1;
         if (c[CT_CONV] || c[CT_FAIL] || c[CT_MAXIT] || it >= maxit) break;
@@ -1115,13 +1134,22 @@ template <typename V, int MODE>
 static int enqueue_inner_t(Model &M, SolvArgs<V> &a, int chunk) {
     const int g = a.nblk;
     const size_t lds = apply_lds<V>(M);
-    hipLaunchKernelGGL((k_bicg_init<V, MODE != 0>), dim3(g), dim3(kThreads), lds, M.stream, a, 1);
-    for (int it = 0; it < chunk; it++) {
-        launch_timed(M, T_BICG_A, k_bicg_a<V, MODE>, dim3(g), dim3(kThreads), lds, M.stream, a, it);
-        launch_timed(M, T_BICG_B, k_bicg_b<V, MODE>, dim3(g), dim3(kThreads), lds, M.stream, a, it);
-    }
-    HIPCHK(M, hipGetLastError());
-    return 0;
+    auto launches = [&]() -> int {
+        hipLaunchKernelGGL((k_bicg_init<V, MODE != 0>), dim3(g), dim3(kThreads), lds, M.stream, a, 1);
+        for (int it = 0; it < chunk; it++) {
+            launch_timed(M, T_BICG_A, k_bicg_a<V, MODE>, dim3(g), dim3(kThreads), lds, M.stream, a, it);
+            launch_timed(M, T_BICG_B, k_bicg_b<V, MODE>, dim3(g), dim3(kThreads), lds, M.stream, a, it);
+        }
+        HIPCHK(M, hipGetLastError());
+        return 0;
+    };
+    // one replayed launch sequence per argument set and chunk
+    std::string key("inner");
+    key_add(key, a);
+    key_add(key, MODE);
+    key_add(key, lds);
+    key_add(key, chunk);
+    return graph_run(M, M.stream, key, launches);
 }
 
 template <typename V>

@@ -366,6 +366,17 @@ class Model:
     def set_timing(self, on=True):
         self._chk(self.lib.shyhip_set_timing(self.h, 1 if on else 0), "set_timing")
 
+    def set_graphs(self, on=True):
+        """Replay the transport substeps and solver iteration chunks from HIP
+        graphs (on by default); off launches every kernel directly."""
+        self._chk(self.lib.shyhip_set_graphs(self.h, 1 if on else 0), "set_graphs")
+
+    def graph_stats(self):
+        """(graphs instantiated, graph replays) since create."""
+        b, r = C.c_long(), C.c_long()
+        self._chk(self.lib.shyhip_graph_stats(self.h, C.byref(b), C.byref(r)), "graph_stats")
+        return b.value, r.value
+
     def timing(self):
         """{kernel group: (total_ms, launches)} since set_timing(True)."""
         out = {}
