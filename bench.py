@@ -81,11 +81,15 @@ def _kernel_bytes(b, p, brows, ntr=2):
         # utlov vtlov hdeov read, utlnv vtlnv written, ddxv ddyv (2 planes) written,
         # visv at the interfaces, per element geometry/levels/summaries
         "hydro_intern": 9 * EL * p + (NL + N) * p + E * (12 + 56 + 6 * p + 4 + 48),
-        # utlnv vtlnv utlov vtlov ulnv vlnv read, 3 horizontal terms per tracer
-        # written, each tracer gathered; per element: nodes, list positions,
-        # geometry (7 doubles), distances (3), levels, external flags and the
-        # TVD upwind tables (6 points: element, level, 3 nodes, 3 weights)
-        "conz3d_elem": (6 + 3 * ntr) * EL * p + ntr * NL * p + E * (12 + 12 + 56 + 24 + 4 + 4 + 264),
+        # once per step: utlnv vtlnv utlov vtlov ulnv vlnv read, per element
+        # nodes, list positions, geometry (7 doubles), distances (3), levels,
+        # external flags and the TVD upwind tables (6 points: element, level,
+        # 3 nodes, 3 weights) read; the element-layer record written (15
+        # ints, 11 doubles)
+        "conz3d_prep": 6 * EL * p + E * (12 + 12 + 56 + 24 + 4 + 4 + 264) + EL * 148,
+        # per substep: the record read, 3 horizontal terms per tracer
+        # written, each tracer gathered
+        "conz3d_elem": 3 * ntr * EL * p + ntr * NL * p + EL * 148,
         # horizontal terms read; hdknv hdkov wlnv difv read, each tracer read
         # and written; per list position its bottom layer and 4 * area, per
         # node its list start and bottom layer
@@ -107,6 +111,7 @@ def _kernel_bytes(b, p, brows, ntr=2):
 # the iteration-kernel variant) of each timed group's kernel
 KERNEL_NAMES = {"bicg_a": "k_bicg_a<{v},", "bicg_b": "k_bicg_b<{v},",
                 "bilu_factor": "k_bilu_factor<{v}>", "conz3d_elem": "k_conz_elem<{v}, 2>",
+                "conz3d_prep": "k_conz_prep<{v}>",
                 "conz3d_node": "k_conz_node<{v}, 12, 2>", "hydro_vertical": "k_hydro_vertical<{v}, 12>",
                 "hydro_intern": "k_hydro_intern<{v}, 12, false>"}
 

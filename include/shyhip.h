@@ -326,10 +326,11 @@ int  shyhip_set_graphs(shyhip_model *h, int on);
 int  shyhip_graph_stats(shyhip_model *h, long *builds, long *replays);
 /* kernel-group timing with HIP events on the handle's stream (off by
  * default; enabling it adds a synchronisation at the end of every step).
- * id = 0 .. 17: set_dry, copy_uvz, hydro_intern, setnod, zeta_assemble,
+ * id = 0 .. 18: set_dry, copy_uvz, hydro_intern, setnod, zeta_assemble,
  * zeta_solve, system_get_setweg, transports_final, setuvd_baro2l, setdepth,
- * hydro_vertical, ttov_uvint, conz3d_elem, conz3d_node, bndo_setbc, and
- * inside zeta_solve the single kernels bilu_factor, bicg_a, bicg_b */
+ * hydro_vertical, ttov_uvint, conz3d_elem, conz3d_node, bndo_setbc, inside
+ * zeta_solve the single kernels bilu_factor, bicg_a, bicg_b, and
+ * conz3d_prep (the step's element-layer records of the transport) */
 int  shyhip_set_timing(shyhip_model *h, int on);
 int  shyhip_get_timing(shyhip_model *h, int id, const char **name, double *total_ms, long *count);
 

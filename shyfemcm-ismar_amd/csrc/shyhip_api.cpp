@@ -368,6 +368,8 @@ int shyhip_destroy(shyhip_model *h) {
     for (void *q : M.grp_hbuf) dfree(q);
     for (void *q : M.grp_bscal) dfree(q);
     dfree(M.bscal1);
+    for (void *q : M.grp_rec) dfree(q);
+    dfree(M.rec1);
     forcing_free_bnd(M);
     forcing_free_wind(M);
     forcing_free_sources(M);
@@ -1576,14 +1578,15 @@ int shyhip_scalar_stability(shyhip_model *h, float rstol, int *istot, double *ri
 }
 
 // scal3sh of tracer itr on stream st with conz3d scratch hbuf
-static int scal3sh_on(Model &M, const TracerSet &ts, int istot, hipStream_t st, void *hbuf, void *bscal) {
+static int scal3sh_on(Model &M, const TracerSet &ts, int istot, hipStream_t st, void *hbuf, void *rec, void *bscal) {
     for (int q = 0; q < ts.n; q++)
         if (ts.itr[q] < 0 || ts.itr[q] >= M.ntr) return fail(&M, "tracer index out of range");
     if (istot < 1 || istot > 1000) return fail(&M, "istot must be in 1..1000");
     if (M.p.itvd == 2 && !M.tvd_ready) return fail(&M, "itvd = 2 needs shyhip_tvd_init");
     auto substeps = [&]() -> int {
         for (int isact = 1; isact <= istot; isact++) {
-            if (launch_conz3d(M, ts, istot, isact, st, hbuf)) return 1;
+            // the step's element-layer records before the first substep
+            if (launch_conz3d(M, ts, istot, isact, st, hbuf, rec, isact == 1)) return 1;
             timer_begin(M, T_BNDO);
             if (launch_bndo(M, ts, st, isact, bscal)) return 1;       // bndo_setbc
             timer_end(M);
@@ -1592,7 +1595,7 @@ static int scal3sh_on(Model &M, const TracerSet &ts, int istot, hipStream_t st, 
     };
     if (!M.use_graphs || M.timing) return substeps();
     std::string key("scal3sh");
-    conz3d_key(M, ts, istot, hbuf, bscal, key);
+    conz3d_key(M, ts, istot, hbuf, rec, bscal, key);
     return graph_run(M, st, key, substeps);
 }
 
@@ -1618,6 +1621,12 @@ static int group_buffers(Model &M) {
             void *hb = nullptr;
             if (dalloc_s(M, &hb, (size_t)group_of(M, g).n * 3 * M.nlvdi * M.nel)) return 1;
             M.grp_hbuf.push_back(hb);
+        }
+    if ((int)M.grp_rec.size() != ngroups(M))
+        for (int g = 0; g < ngroups(M); g++) {
+            char *rb = nullptr;
+            if (dalloc(M, &rb, conz_record_bytes(M))) return 1;
+            M.grp_rec.push_back(rb);
         }
     if ((int)M.grp_bscal.size() != ngroups(M))
         for (int g = 0; g < ngroups(M); g++) {
@@ -1645,7 +1654,8 @@ int shyhip_conz3d(shyhip_model *h, int itr, const float *cbound, int istot, int 
         cb = M.d_cbound;
     }
     M.mem_istot_on = false;
-    if (launch_conz3d(M, itr, istot, isact, 0, M.stream, M.s.hbuf, cb)) return 1;
+    if (!M.rec1 && dalloc(M, (char **)&M.rec1, conz_record_bytes(M))) return 1;
+    if (launch_conz3d(M, itr, istot, isact, 0, M.stream, M.s.hbuf, M.rec1, cb)) return 1;
     return check_dev_err(M, "conz3d");
 }
 
@@ -1667,7 +1677,8 @@ int shyhip_scal3sh(shyhip_model *h, int itr, int istot, int clip_small) {
     ts.itr[0] = itr;
     ts.clip[0] = clip_small;
     if (!h->M.bscal1 && dalloc_s(h->M, &h->M.bscal1, (size_t)std::max(1, h->M.m.nbndo) * h->M.nlvdi)) return 1;
-    return scal3sh_on(h->M, ts, istot, h->M.stream, h->M.s.hbuf, h->M.bscal1);
+    if (!h->M.rec1 && dalloc(h->M, (char **)&h->M.rec1, conz_record_bytes(h->M))) return 1;
+    return scal3sh_on(h->M, ts, istot, h->M.stream, h->M.s.hbuf, h->M.rec1, h->M.bscal1);
 }
 
 // streams and events of the tracer groups after the first (created once)
@@ -1747,7 +1758,7 @@ int shyhip_step(shyhip_model *h, double dtime, int istot) {
         for (int g = 0; g < ngroups(M); g++) {
             hipStream_t st = g == 0 ? M.tr0_stream : M.tr_stream[g - 1];
             HIPCHK(M, hipStreamWaitEvent(st, M.ev_hydro, 0));
-            if (scal3sh_on(M, group_of(M, g), istot, st, M.grp_hbuf[g], M.grp_bscal[g])) { M.iweg_conz = nullptr; return 1; }
+            if (scal3sh_on(M, group_of(M, g), istot, st, M.grp_hbuf[g], M.grp_rec[g], M.grp_bscal[g])) { M.iweg_conz = nullptr; return 1; }
             HIPCHK(M, hipEventRecord(M.pipe_ev[par][g], st));
         }
         M.iweg_conz = nullptr;
@@ -1763,15 +1774,15 @@ int shyhip_step(shyhip_model *h, double dtime, int istot) {
             // its own stream, joined back into the model's stream
             HIPCHK(M, hipEventRecord(M.ev_hydro, M.stream));
             for (int g = 1; g < ngroups(M); g++) HIPCHK(M, hipStreamWaitEvent(M.tr_stream[g - 1], M.ev_hydro, 0));
-            if (scal3sh_on(M, group_of(M, 0), istot, M.stream, M.grp_hbuf[0], M.grp_bscal[0])) return 1;
+            if (scal3sh_on(M, group_of(M, 0), istot, M.stream, M.grp_hbuf[0], M.grp_rec[0], M.grp_bscal[0])) return 1;
             for (int g = 1; g < ngroups(M); g++) {
-                if (scal3sh_on(M, group_of(M, g), istot, M.tr_stream[g - 1], M.grp_hbuf[g], M.grp_bscal[g])) return 1;
+                if (scal3sh_on(M, group_of(M, g), istot, M.tr_stream[g - 1], M.grp_hbuf[g], M.grp_rec[g], M.grp_bscal[g])) return 1;
                 HIPCHK(M, hipEventRecord(M.tr_done[g - 1], M.tr_stream[g - 1]));
                 HIPCHK(M, hipStreamWaitEvent(M.stream, M.tr_done[g - 1], 0));
             }
         } else {
             for (int g = 0; g < ngroups(M); g++)
-                if (scal3sh_on(M, group_of(M, g), istot, M.stream, M.grp_hbuf[g], M.grp_bscal[g])) return 1;
+                if (scal3sh_on(M, group_of(M, g), istot, M.stream, M.grp_hbuf[g], M.grp_rec[g], M.grp_bscal[g])) return 1;
         }
     }
     // barocl: the density of the transported T/S (rhoset_shell(1),
@@ -1818,7 +1829,7 @@ int shyhip_get_timing(shyhip_model *h, int id, const char **name, double *total_
     static const char *names[T_NTIMERS] = {"set_dry", "copy_uvz", "hydro_intern", "setnod", "zeta_assemble",
                                            "zeta_solve", "system_get_setweg", "transports_final", "setuvd_baro2l",
                                            "setdepth", "hydro_vertical", "ttov_uvint", "conz3d_elem", "conz3d_node",
-                                           "bndo_setbc", "bilu_factor", "bicg_a", "bicg_b"};
+                                           "bndo_setbc", "bilu_factor", "bicg_a", "bicg_b", "conz3d_prep"};
     if (!h) return fail(nullptr, "null handle");
     if (id < 0 || id >= T_NTIMERS) return fail(&h->M, "timer id out of range");
     if (timer_flush(h->M)) return 1;

@@ -65,6 +65,9 @@ struct ConzArgs {
     // element-layer work list and node tiles (see Mesh)
     const int *eitem, *ntile;
     int n_eitem;
+    // the step's element-layer records (k_conz_prep): int and double columns
+    int *rec_i;
+    double *rec_d;
 };
 
 template <typename S>
@@ -79,12 +82,19 @@ __device__ inline double cval(const ConzArgs<S> &a, int t, int l, int k) {
 // chunks of kConzChunk consecutive elements, each chunk layer by layer with
 // its elements ascending, so that a layer's loads are contiguous.  A
 // workgroup takes kThreads consecutive entries, whatever the depth of the
-// elements they come from, so every workgroup does the same work.  An entry
-// loads its element's data itself (the element's other layers read the
-// same lines from L2); the independent loads -- nodes, list positions,
-// geometry, the layer's transports -- are issued at once, the tracers at
-// the nodes next, and for TVD the two upwind points the flow direction
-// picks, then their tracer values.
+// elements they come from, so every workgroup does the same work.
+//
+// Everything of an element-layer that depends on the flow only -- the flux
+// coefficients f, the upwind case, and for TVD the two pairs of the flow
+// direction with their Courant numbers, upwind nodes and interpolation
+// weights -- holds for all the substeps of a step (the transports,
+// velocities and the substep length do not change within it).  k_conz_prep
+// computes it once per step into a record per list entry (ConzRec: columns
+// indexed by the entry, so every load is contiguous); k_conz_elem then
+// reads the record, gathers the tracers at the element's nodes and upwind
+// points, and applies the upwind rule or the limiter.  The arithmetic is
+// that of the element terms computed in one pass (the record holds the
+// values, not approximations of them).
 
 // v[i] for a run-time i in 0..2 without indexing a register array
 template <typename T>
@@ -92,15 +102,24 @@ __device__ inline T sel3(const T (&v)[3], int i) {
     return i == 0 ? v[0] : (i == 1 ? v[1] : v[2]);
 }
 
-template <typename S, int NT>
-__global__ __launch_bounds__(kThreads, 5) void k_conz_elem(ConzArgs<S> a) {
+// record columns (int: RI_*, double: RD_*), each n_eitem long
+enum { RI_IE = 0, RI_L, RI_KN, RI_SO = RI_KN + 3, RI_CODE = RI_SO + 3, RI_TKN, RI_N = RI_TKN + 6 };
+enum { RD_F = 0, RD_TW = 3, RD_ALFA = 9, RD_N = 11 };
+// the case word: itot (bits 0-1), isum (2-4), TVD pairs present (5), and per
+// pair k (bits 8 + 8k ..): ic (2 bits), id (2 bits), upwind point present (1)
+__host__ __device__ inline size_t conz_rec_bytes(int n) {
+    return ((size_t)RI_N * n * sizeof(int) + 7) / 8 * 8 + (size_t)RD_N * n * sizeof(double);
+}
+
+template <typename S>
+__global__ __launch_bounds__(kThreads) void k_conz_prep(ConzArgs<S> a) {
     using R = S;
     const int i = xcd_block() * kThreads + threadIdx.x;
     if (i >= a.n_eitem) return;
+    const int n = a.n_eitem;
     const unsigned w = (unsigned)a.eitem[i];
     const int ie = (int)(w >> 5), l = (int)(w & 31u) + 1;
-    if (a.mem_istot && a.isact > a.mem_istot[a.elem_mem[ie]]) return;
-    const int nel = a.nel, nkn = a.nkn;
+    const int nel = a.nel;
     const bool use_tvd = a.itvd > 0;
     int kn[3], so[3];
     double gb[3], gc[3];
@@ -111,7 +130,6 @@ __global__ __launch_bounds__(kThreads, 5) void k_conz_elem(ConzArgs<S> a) {
         gb[ii] = a.geo[(size_t)ii * nel + ie];
         gc[ii] = a.geo[(size_t)(3 + ii) * nel + ie];
     }
-    // the layer's transports, velocities and node concentrations
     const size_t o = (size_t)(l - 1) * nel + ie;
     const S un = a.utn[o], uo = a.uto[o], vn = a.vtn[o], vo = a.vto[o];
     int iext = 1;
@@ -125,14 +143,6 @@ __global__ __launch_bounds__(kThreads, 5) void k_conz_elem(ConzArgs<S> a) {
         for (int ii = 0; ii < 3; ii++) dis3[ii] = a.dis[(size_t)ii * nel + ie];
         ajr0 = a.geo[(size_t)6 * nel + ie];
     }
-    const bool tvd = use_tvd && iext == 0;
-    S cr[NT][3];
-#pragma unroll
-    for (int t = 0; t < NT; t++)
-#pragma unroll
-        for (int ii = 0; ii < 3; ii++) cr[t][ii] = a.c[t][(size_t)(l - 1) * nkn + kn[ii]];
-    auto clip = [&](int t, S v) -> S { return (a.clip[t] && v < (S)1.e-15) ? (S)0 : v; };
-    // upwind rule (the flux coefficients are the tracers' common part)
     const double azt = 1. - a.az;
     const double us = a.az * (double)un + azt * (double)uo;
     const double vs = a.az * (double)vn + azt * (double)vo;
@@ -142,10 +152,87 @@ __global__ __launch_bounds__(kThreads, 5) void k_conz_elem(ConzArgs<S> a) {
         f[ii] = us * gb[ii] + vs * gc[ii];
         if (f[ii] < 0.) { itot++; isum += ii + 1; }
     }
+    int code = itot | (isum << 2);
+    int *ri = a.rec_i;
+    double *rd = a.rec_d;
+    if (use_tvd && iext == 0 && itot >= 1 && itot <= 2) {
+        // TVD fluxes (tvd_fluxes, tvd_admin.f90:673-823): the two pairs
+        // (ic, id) of the flow direction -- the vertices other than ip,
+        // ascending -- and their Courant numbers and upwind points
+        code |= 1 << 5;
+        const int itot2 = itot - 1, itot1 = 2 - itot;
+        const int ip = itot2 * (6 - isum) + itot1 * isum;
+        const R ajr = (R)(24. * ajr0);
+        const double dte = a.mem_istot ? a.dtp / (double)a.mem_istot[a.elem_mem[ie]] : a.dt;
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int ii = k == 0 ? (ip == 1 ? 2 : 1) : (ip == 3 ? 2 : 3);
+            const int ic = itot2 * ii + itot1 * ip;
+            const int id = itot2 * ip + itot1 * ii;
+            const int pidx = 2 * (ic - 1) + ((id == (ic % 3) + 1) ? 0 : 1);
+            const int iop = 6 - (id + ic);
+            const int tet1 = 1 + iop % 3;
+            R dx = (R)((double)ajr * sel3(gc, iop - 1));
+            dx = (R)(-2 * (tet1 == id ? 1 : 0)) * dx + dx;
+            R dy = (R)((double)ajr * sel3(gb, iop - 1));
+            dy = (R)(-2 * (tet1 == ic ? 1 : 0)) * dy + dy;
+            const R dis = (R)sel3(dis3, iop - 1);
+            const R vel = (R)fabs((double)(ul * dx + vl * dy)) / dis;
+            const R alfa = (R)((dte * (double)vel) / (double)dis);
+            const size_t po = (size_t)pidx * nel + ie;
+            const bool pup = a.tvd_el[po] >= 0 && a.tvd_lev[po] >= l;
+            code |= (ic | (id << 2) | ((pup ? 1 : 0) << 4)) << (8 + 8 * k);
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                ri[(size_t)(RI_TKN + 3 * k + q) * n + i] = a.tvd_kn[((size_t)q * 6 + pidx) * nel + ie];
+                rd[(size_t)(RD_TW + 3 * k + q) * n + i] = a.tvd_w[((size_t)q * 6 + pidx) * nel + ie];
+            }
+            rd[(size_t)(RD_ALFA + k) * n + i] = (double)alfa;
+        }
+    }
+    ri[(size_t)RI_IE * n + i] = ie;
+    ri[(size_t)RI_L * n + i] = l;
+#pragma unroll
+    for (int ii = 0; ii < 3; ii++) {
+        ri[(size_t)(RI_KN + ii) * n + i] = kn[ii];
+        ri[(size_t)(RI_SO + ii) * n + i] = so[ii];
+        rd[(size_t)(RD_F + ii) * n + i] = f[ii];
+    }
+    ri[(size_t)RI_CODE * n + i] = code;
+}
+
+template <typename S, int NT>
+__global__ __launch_bounds__(kThreads) void k_conz_elem(ConzArgs<S> a) {
+    using R = S;
+    const int i = xcd_block() * kThreads + threadIdx.x;
+    if (i >= a.n_eitem) return;
+    const int n = a.n_eitem, nkn = a.nkn;
+    const int *ri = a.rec_i;
+    const double *rd = a.rec_d;
+    if (a.mem_istot && a.isact > a.mem_istot[a.elem_mem[ri[(size_t)RI_IE * n + i]]]) return;
+    const int l = ri[(size_t)RI_L * n + i], code = ri[(size_t)RI_CODE * n + i];
+    int kn[3], so[3];
+    double f[3];
+#pragma unroll
+    for (int ii = 0; ii < 3; ii++) {
+        kn[ii] = ri[(size_t)(RI_KN + ii) * n + i];
+        so[ii] = ri[(size_t)(RI_SO + ii) * n + i];
+        f[ii] = rd[(size_t)(RD_F + ii) * n + i];
+    }
+    const int itot = code & 3, isum = (code >> 2) & 7;
+    const bool tvd = (code >> 5) & 1;
+    S cr[NT][3];
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int ii = 0; ii < 3; ii++) cr[t][ii] = a.c[t][(size_t)(l - 1) * nkn + kn[ii]];
+    auto clip = [&](int t, S v) -> S { return (a.clip[t] && v < (S)1.e-15) ? (S)0 : v; };
     double fl[NT][3];
 #pragma unroll
     for (int t = 0; t < NT; t++) fl[t][0] = fl[t][1] = fl[t][2] = 0.;
     if (!tvd) {
+        // upwind rule (also the elements of a TVD run at an external
+        // boundary); no flux when no vertex or all take inflow
 #pragma unroll
         for (int t = 0; t < NT; t++) {
             double cl[3];
@@ -161,67 +248,43 @@ __global__ __launch_bounds__(kThreads, 5) void k_conz_elem(ConzArgs<S> a) {
                     if (ii == is2 - 1) fl[t][ii] = -sum;
             }
         }
-    } else if (itot >= 1 && itot <= 2) {
-        // TVD fluxes (tvd_fluxes, tvd_admin.f90:673-823): the two pairs
-        // (ic, id) of the flow direction -- the vertices other than ip,
-        // ascending -- with their Courant numbers, then both pairs' upwind
-        // tables (independent loads), their tracer values, and each
-        // tracer's limiter
+    } else {
+        // the two pairs' upwind values, then each tracer's limiter
         const int itot2 = itot - 1, itot1 = 2 - itot;
         const int ip = itot2 * (6 - isum) + itot1 * isum;
-        const R ajr = (R)(24. * ajr0);
-        const double dte = a.mem_istot ? a.dtp / (double)a.mem_istot[a.elem_mem[ie]] : a.dt;
         const R fact = (R)(-itot2 + itot1);
-        int icv[2], idv[2], pidx[2], tkn[2][3];
-        R alfa[2];
-        double fii[2];
-#pragma unroll
-        for (int k = 0; k < 2; k++) {
-            const int ii = k == 0 ? (ip == 1 ? 2 : 1) : (ip == 3 ? 2 : 3);
-            const int ic = itot2 * ii + itot1 * ip;
-            const int id = itot2 * ip + itot1 * ii;
-            icv[k] = ic;
-            idv[k] = id;
-            fii[k] = sel3(f, ii - 1);
-            pidx[k] = 2 * (ic - 1) + ((id == (ic % 3) + 1) ? 0 : 1);
-            const int iop = 6 - (id + ic);
-            const int tet1 = 1 + iop % 3;
-            R dx = (R)((double)ajr * sel3(gc, iop - 1));
-            dx = (R)(-2 * (tet1 == id ? 1 : 0)) * dx + dx;
-            R dy = (R)((double)ajr * sel3(gb, iop - 1));
-            dy = (R)(-2 * (tet1 == ic ? 1 : 0)) * dy + dy;
-            const R dis = (R)sel3(dis3, iop - 1);
-            const R vel = (R)fabs((double)(ul * dx + vl * dy)) / dis;
-            alfa[k] = (R)((dte * (double)vel) / (double)dis);
-        }
-        bool pup[2];
+        int tkn[2][3];
         double tw[2][3];
+        R alfa[2];
 #pragma unroll
         for (int k = 0; k < 2; k++) {
-            const size_t po = (size_t)pidx[k] * nel + ie;
-            pup[k] = a.tvd_el[po] >= 0 && a.tvd_lev[po] >= l;
 #pragma unroll
             for (int q = 0; q < 3; q++) {
-                tkn[k][q] = a.tvd_kn[((size_t)q * 6 + pidx[k]) * nel + ie];
-                tw[k][q] = a.tvd_w[((size_t)q * 6 + pidx[k]) * nel + ie];
+                tkn[k][q] = ri[(size_t)(RI_TKN + 3 * k + q) * n + i];
+                tw[k][q] = rd[(size_t)(RD_TW + 3 * k + q) * n + i];
             }
+            alfa[k] = (R)rd[(size_t)(RD_ALFA + k) * n + i];
         }
         S cu[2][NT][3];
 #pragma unroll
         for (int k = 0; k < 2; k++)
-            if (pup[k])
+            if ((code >> (12 + 8 * k)) & 1)
 #pragma unroll
                 for (int t = 0; t < NT; t++)
 #pragma unroll
                     for (int q = 0; q < 3; q++) cu[k][t][q] = a.c[t][(size_t)(l - 1) * nkn + tkn[k][q]];
 #pragma unroll
         for (int k = 0; k < 2; k++) {
-            const int ic = icv[k], id = idv[k];
+            const int ck = code >> (8 + 8 * k);
+            const int ic = ck & 3, id = (ck >> 2) & 3;
+            const bool pup = (ck >> 4) & 1;
+            const int ii = k == 0 ? (ip == 1 ? 2 : 1) : (ip == 3 ? 2 : 3);
+            const double fii = sel3(f, ii - 1);
 #pragma unroll
             for (int t = 0; t < NT; t++) {
                 const R conc = clip(t, sel3(cr[t], ic - 1)), cond = clip(t, sel3(cr[t], id - 1));
                 R conu = cond;
-                if (pup[k]) {
+                if (pup) {
                     double zh = 0.;
                     for (int q = 0; q < 3; q++) zh = zh + (double)clip(t, cu[k][t][q]) * tw[k][q];
                     conu = (R)zh;
@@ -233,7 +296,7 @@ __global__ __launch_bounds__(kThreads, 5) void k_conz_elem(ConzArgs<S> a) {
                 const double rfd = (double)rf;
                 const R psi = (R)fmax(0., fmax(fmin(1., 2. * rfd), fmin(2., rfd)));
                 const R conf = conc + (R)0.5 * psi * (cond - conc) * ((R)1 - alfa[k]);
-                const R term = (R)((double)(fact * conf) * fii[k]);
+                const R term = (R)((double)(fact * conf) * fii);
                 for (int q = 0; q < 3; q++) {
                     if (q == ic - 1) fl[t][q] = fl[t][q] - (double)term;
                     if (q == id - 1) fl[t][q] = fl[t][q] + (double)term;
@@ -562,13 +625,19 @@ __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
 }
 
 template <typename S>
-static ConzArgs<S> conz_args(Model &M, const TracerSet &ts, int istot, int isact, void *hbuf);
+static ConzArgs<S> conz_args(Model &M, const TracerSet &ts, int istot, int isact, void *hbuf, void *rec);
 
 template <typename S, int NL, int NT>
-static int conz_nl(Model &M, const TracerSet &ts, int istot, int isact, hipStream_t st, void *hbuf) {
+static int conz_nl(Model &M, const TracerSet &ts, int istot, int isact, hipStream_t st, void *hbuf, void *rec,
+                   bool prep) {
     if (!M.m.conz_eitem || !M.m.conz_ntile) return hip_check(M, hipErrorNotInitialized, "conz3d: no basin");
-    const ConzArgs<S> a = conz_args<S>(M, ts, istot, isact, hbuf);
+    if (!rec) return hip_check(M, hipErrorInvalidValue, "conz3d: no element-layer record buffer");
+    const ConzArgs<S> a = conz_args<S>(M, ts, istot, isact, hbuf, rec);
     const int ge = (M.m.n_eitem + kThreads - 1) / kThreads;
+    if (prep) {
+        launch_timed(M, T_CONZ_PREP, k_conz_prep<S>, dim3(ge), dim3(kThreads), 0, st, a);
+        HIPCHK(M, hipGetLastError());
+    }
     launch_timed(M, T_CONZ_ELEM, k_conz_elem<S, NT>, dim3(ge), dim3(kThreads), 0, st, a);
     HIPCHK(M, hipGetLastError());
     launch_timed(M, T_CONZ_NODE, k_conz_node<S, NL, NT>, dim3(M.m.n_ntile), dim3(kThreads), 0, st, a);
@@ -579,7 +648,7 @@ static int conz_nl(Model &M, const TracerSet &ts, int istot, int isact, hipStrea
 // The arguments of substep isact of istot (zeroed first, so that their
 // bytes, padding included, identify the launch).
 template <typename S>
-static ConzArgs<S> conz_args(Model &M, const TracerSet &ts, int istot, int isact, void *hbuf) {
+static ConzArgs<S> conz_args(Model &M, const TracerSet &ts, int istot, int isact, void *hbuf, void *rec) {
     ConzArgs<S> a;
     std::memset(&a, 0, sizeof a);
     a.nkn = M.nkn; a.nel = M.nel; a.nlvdi = M.nlvdi; a.nlv = M.nlv;
@@ -613,21 +682,28 @@ static ConzArgs<S> conz_args(Model &M, const TracerSet &ts, int istot, int isact
     a.hbuf = (S *)hbuf;
     a.err = M.d_err;
     a.eitem = M.m.conz_eitem; a.n_eitem = M.m.n_eitem; a.ntile = M.m.conz_ntile;
+    a.rec_i = (int *)rec;
+    a.rec_d = rec ? (double *)((char *)rec + conz_rec_bytes(M.m.n_eitem) -
+                               (size_t)RD_N * M.m.n_eitem * sizeof(double))
+                  : nullptr;
     return a;
 }
 
+size_t conz_record_bytes(const Model &M) { return conz_rec_bytes(M.m.n_eitem); }
+
 template <typename S, int NT>
-static int conz_t(Model &M, const TracerSet &ts, int istot, int isact, hipStream_t st, void *hbuf) {
+static int conz_t(Model &M, const TracerSet &ts, int istot, int isact, hipStream_t st, void *hbuf, void *rec,
+                  bool prep) {
     switch (M.nl_tmpl) {
-    case 1: return conz_nl<S, 1, NT>(M, ts, istot, isact, st, hbuf);
-    case 2: return conz_nl<S, 2, NT>(M, ts, istot, isact, st, hbuf);
-    case 4: return conz_nl<S, 4, NT>(M, ts, istot, isact, st, hbuf);
-    case 8: return conz_nl<S, 8, NT>(M, ts, istot, isact, st, hbuf);
-    case 12: return conz_nl<S, 12, NT>(M, ts, istot, isact, st, hbuf);
-    case 16: return conz_nl<S, 16, NT>(M, ts, istot, isact, st, hbuf);
-    case 20: return conz_nl<S, 20, NT>(M, ts, istot, isact, st, hbuf);
-    case 24: return conz_nl<S, 24, NT>(M, ts, istot, isact, st, hbuf);
-    default: return conz_nl<S, 32, NT>(M, ts, istot, isact, st, hbuf);
+    case 1: return conz_nl<S, 1, NT>(M, ts, istot, isact, st, hbuf, rec, prep);
+    case 2: return conz_nl<S, 2, NT>(M, ts, istot, isact, st, hbuf, rec, prep);
+    case 4: return conz_nl<S, 4, NT>(M, ts, istot, isact, st, hbuf, rec, prep);
+    case 8: return conz_nl<S, 8, NT>(M, ts, istot, isact, st, hbuf, rec, prep);
+    case 12: return conz_nl<S, 12, NT>(M, ts, istot, isact, st, hbuf, rec, prep);
+    case 16: return conz_nl<S, 16, NT>(M, ts, istot, isact, st, hbuf, rec, prep);
+    case 20: return conz_nl<S, 20, NT>(M, ts, istot, isact, st, hbuf, rec, prep);
+    case 24: return conz_nl<S, 24, NT>(M, ts, istot, isact, st, hbuf, rec, prep);
+    default: return conz_nl<S, 32, NT>(M, ts, istot, isact, st, hbuf, rec, prep);
     }
 }
 
@@ -783,15 +859,16 @@ int launch_bndo(Model &M, int itr, hipStream_t st, int isact) {
     return launch_bndo(M, ts, st, isact, nullptr);
 }
 
-int launch_conz3d(Model &M, const TracerSet &ts, int istot, int isact, hipStream_t st, void *hbuf) {
+int launch_conz3d(Model &M, const TracerSet &ts, int istot, int isact, hipStream_t st, void *hbuf, void *rec,
+                  bool prep) {
     if (ts.n == 2)
-        return M.prec == SHYHIP_FP64 ? conz_t<double, 2>(M, ts, istot, isact, st, hbuf)
-                                     : conz_t<float, 2>(M, ts, istot, isact, st, hbuf);
-    return M.prec == SHYHIP_FP64 ? conz_t<double, 1>(M, ts, istot, isact, st, hbuf)
-                                 : conz_t<float, 1>(M, ts, istot, isact, st, hbuf);
+        return M.prec == SHYHIP_FP64 ? conz_t<double, 2>(M, ts, istot, isact, st, hbuf, rec, prep)
+                                     : conz_t<float, 2>(M, ts, istot, isact, st, hbuf, rec, prep);
+    return M.prec == SHYHIP_FP64 ? conz_t<double, 1>(M, ts, istot, isact, st, hbuf, rec, prep)
+                                 : conz_t<float, 1>(M, ts, istot, isact, st, hbuf, rec, prep);
 }
 
-void conz3d_key(Model &M, const TracerSet &ts, int istot, void *hbuf, void *bscal, std::string &key) {
+void conz3d_key(Model &M, const TracerSet &ts, int istot, void *hbuf, void *rec, void *bscal, std::string &key) {
     key_add(key, istot);
     key_add(key, M.nl_tmpl);
     key_add(key, M.prec);
@@ -799,22 +876,22 @@ void conz3d_key(Model &M, const TracerSet &ts, int istot, void *hbuf, void *bsca
     key_add(key, M.m.nbndo);
     key_add(key, M.m.n_ntile);
     if (M.prec == SHYHIP_FP64) {
-        key_add(key, conz_args<double>(M, ts, istot, 1, hbuf));
+        key_add(key, conz_args<double>(M, ts, istot, 1, hbuf, rec));
         key_add(key, bndo_args<double>(M, ts, 1, bscal));
     } else {
-        key_add(key, conz_args<float>(M, ts, istot, 1, hbuf));
+        key_add(key, conz_args<float>(M, ts, istot, 1, hbuf, rec));
         key_add(key, bndo_args<float>(M, ts, 1, bscal));
     }
 }
 
-int launch_conz3d(Model &M, int itr, int istot, int isact, int clip_small, hipStream_t st, void *hbuf,
+int launch_conz3d(Model &M, int itr, int istot, int isact, int clip_small, hipStream_t st, void *hbuf, void *rec,
                   const void *cbound) {
     TracerSet ts;
     ts.n = 1;
     ts.itr[0] = itr;
     ts.clip[0] = clip_small;
     ts.cbound[0] = cbound;
-    return launch_conz3d(M, ts, istot, isact, st, hbuf);
+    return launch_conz3d(M, ts, istot, isact, st, hbuf, rec, true);
 }
 
 // ------------------------------------------------------------------ stability

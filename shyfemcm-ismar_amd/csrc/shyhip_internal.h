@@ -172,7 +172,7 @@ struct Solver {               // Krylov work space (value type V = S of the mode
 
 // kernel groups timed with HIP events when timing is enabled
 enum TimerId { T_SETDRY = 0, T_COPY, T_INTERN, T_SETNOD, T_ZETA, T_SOLVE, T_WETDRY, T_FINAL, T_SETUVD, T_DEPTH,
-               T_VERT, T_TTOV, T_CONZ_ELEM, T_CONZ_NODE, T_BNDO, T_FACTOR, T_BICG_A, T_BICG_B, T_NTIMERS };
+               T_VERT, T_TTOV, T_CONZ_ELEM, T_CONZ_NODE, T_BNDO, T_FACTOR, T_BICG_A, T_BICG_B, T_CONZ_PREP, T_NTIMERS };
 
 // A record series resident on the device (time interpolation of forcing
 // files): times [nrec], values record-major [nrec][ncomp][nval]; nintp
@@ -261,6 +261,10 @@ struct Model {
     // the boundaries change)
     std::vector<void *> grp_bscal;
     void *bscal1 = nullptr;
+    // the step's conz3d element-layer records of each tracer group and of
+    // the single-tracer entry points (conz_record_bytes each)
+    std::vector<void *> grp_rec;
+    void *rec1 = nullptr;
     std::string err;
     // host-side copies needed for init / plugin / checks
     std::vector<int> h_nen3v, h_rowptr, h_cols, h_ijp, h_color, h_ilhv, h_ilhkv, h_ibnd, h_ibtyp, h_iextb;
@@ -432,11 +436,16 @@ struct TracerSet {
     int clip[2] = {0, 0};
     const void *cbound[2] = {nullptr, nullptr};
 };
-int launch_conz3d(Model &M, const TracerSet &ts, int istot, int isact, hipStream_t st, void *hbuf);
+// conz3d substep isact of istot: hbuf holds the element terms, rec the
+// step's element-layer records (conz_record_bytes), computed first when prep
+int launch_conz3d(Model &M, const TracerSet &ts, int istot, int isact, hipStream_t st, void *hbuf, void *rec,
+                  bool prep);
+size_t conz_record_bytes(const Model &M);
 // the bytes that identify the launches of a tracer group's substeps 1..istot
 // (conz3d and bndo_setbc), appended to key (a graph_run key)
-void conz3d_key(Model &M, const TracerSet &ts, int istot, void *hbuf, void *bscal, std::string &key);
-int launch_conz3d(Model &M, int itr, int istot, int isact, int clip_small, hipStream_t st, void *hbuf,
+void conz3d_key(Model &M, const TracerSet &ts, int istot, void *hbuf, void *rec, void *bscal, std::string &key);
+// one substep of one tracer, its records computed first
+int launch_conz3d(Model &M, int itr, int istot, int isact, int clip_small, hipStream_t st, void *hbuf, void *rec,
                   const void *cbound = nullptr);
 int launch_bndo(Model &M, int itr, hipStream_t st, int isact = 1);
 // bndo_setbc of substep isact; scal: the step's outflow-test buffer of the

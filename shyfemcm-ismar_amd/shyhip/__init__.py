@@ -383,7 +383,7 @@ class Model:
         name = C.c_char_p()
         ms = C.c_double()
         cnt = C.c_long()
-        for i in range(18):
+        for i in range(19):
             self._chk(self.lib.shyhip_get_timing(self.h, i, C.byref(name), C.byref(ms), C.byref(cnt)), "get_timing")
             out[name.value.decode()] = (ms.value, cnt.value)
         return out
