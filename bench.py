@@ -432,17 +432,35 @@ def run_members_alone(mems, prec, steps, warmup):
 
 
 def run_serving(args):
+    """The serving workload: a Poisson stream (4 requests/s) of storm-surge
+    forecasts on bays of 5k-400k nodes (log-uniform), 360-1440 steps of 60 s
+    (6-24 simulated hours), 8 layers, FP32 + refinement, through the
+    continuous-batching server (runs joining and leaving one union model
+    between steps) and, beside it, through one model and stream per run."""
     from shyhip import serve
-    reqs = serve.poisson_stream(args.srv_requests, rate=4.0, seed=args.seed + 505,
-                                sizes=(5000, args.srv_max_nodes), steps=(120, 360), nlv=8)
     params = dict(dt=60.0, ireib=5, block_rows=256, tri_sweeps=3)
-    srv = serve.Server("fp32", params=params, slots=args.srv_slots, hbm_budget=64 << 30, deferred=True)
-    done, wall = srv.serve(reqs)
+
+    def stream():
+        return serve.poisson_stream(args.srv_requests, rate=4.0, seed=args.seed + 505,
+                                    sizes=(5000, args.srv_max_nodes), steps=(args.srv_min_steps, args.srv_max_steps),
+                                    nlv=8)
+    reqs = stream()
+    bsrv = serve.BatchServer("fp32", params=params, slots=args.srv_batch_slots, hbm_budget=64 << 30,
+                             solo_nodes=args.srv_solo_nodes, solo_slots=args.srv_slots)
+    done, wall = bsrv.serve(reqs)
     s = serve.summary(done, wall)
-    s.update(slots=args.srv_slots, rate_per_s=4.0,
-             workload=f"{len(reqs)} requests, Poisson 4/s, lagoons of 5k-{args.srv_max_nodes // 1000}k nodes "
-                      f"(log-uniform), 8 layers, 120-360 steps of 60 s, wind ramp to 30 m/s, surge 0.3-1.2 m, "
-                      f"FP32 + FP64 refinement, hydro only")
+    s.update(server=f"continuous batching of the runs of at most {args.srv_solo_nodes} nodes "
+                    f"({args.srv_batch_slots} slots), larger runs alone ({args.srv_slots} slots)",
+             rate_per_s=4.0,
+             union_rebuilds=bsrv.rebuilds, batch_steps=bsrv.batch_steps, run_steps=bsrv.run_steps,
+             rebuild_s=round(bsrv.build_s, 3),
+             workload=f"{len(reqs)} requests, Poisson 4/s, bays of 5k-{args.srv_max_nodes // 1000}k nodes "
+                      f"(log-uniform), 8 layers, {args.srv_min_steps}-{args.srv_max_steps} steps of 60 s, "
+                      f"wind ramp to 30 m/s, surge 0.3-1.2 m, FP32 + FP64 refinement, hydro only")
+    reqs2 = stream()
+    srv = serve.Server("fp32", params=params, slots=args.srv_slots, hbm_budget=64 << 30, deferred=True)
+    done2, wall2 = srv.serve(reqs2)
+    s["one_model_per_run"] = serve.summary(done2, wall2)
     return s
 
 
@@ -469,8 +487,12 @@ def main():
     ap.add_argument("--ens-layers", type=int, default=10)
     ap.add_argument("--ens-steps", type=int, default=120)
     ap.add_argument("--srv-requests", type=int, default=12)
-    ap.add_argument("--srv-max-nodes", type=int, default=100000)
+    ap.add_argument("--srv-max-nodes", type=int, default=400000)
+    ap.add_argument("--srv-min-steps", type=int, default=360)
+    ap.add_argument("--srv-max-steps", type=int, default=1440)
     ap.add_argument("--srv-slots", type=int, default=4)
+    ap.add_argument("--srv-batch-slots", type=int, default=8)
+    ap.add_argument("--srv-solo-nodes", type=int, default=60000)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--c3-nodes", type=int, default=500000)

@@ -118,6 +118,18 @@ int  shyhip_check_params(const shyhip_params *p);
  * without it the model is one member.  No counterpart in the reference,
  * which runs one basin per process. */
 int  shyhip_set_members(shyhip_model *h, int nmem, const int *node_start, const int *elem_start);
+/* No counterpart in the reference (the serving loop's continuous batching:
+ * runs that join or leave a batch move into a new union model).  Copies
+ * the state member sm of src carries from step to step -- levels,
+ * transports, velocities, depths, wet/dry masks, friction, viscosity and
+ * diffusivity, wind stress, discharges, density, tracers and their
+ * boundary values -- into member dm of dst, device to device.  The two
+ * members must hold the same basin (node, element and layer counts),
+ * precision and tracers.  Waits for both models' work. */
+int  shyhip_copy_member_state(shyhip_model *dst, int dm, shyhip_model *src, int sm);
+/* No counterpart in the reference: a uniform wind stress on the nodes of
+ * member m only (each run of a batch has its own wind). */
+int  shyhip_set_member_wind(shyhip_model *h, int m, float taux, float tauy);
 
 /* basin and layers (init time): nen3v(3,nel) 1-based, xgv/ygv(nkn),
  * hm3v(3,nel), hev(nel), hlv/hldv(nlvdi), ilhv(nel), ilhkv(nkn).

@@ -26,10 +26,16 @@ int hip_check(Model &M, hipError_t e, const char *what) {
     return 1;
 }
 
+std::recursive_mutex &device_lock() {
+    static std::recursive_mutex m;
+    return m;
+}
+
 int graph_run(Model &M, hipStream_t st, const std::string &key, const std::function<int()> &launches) {
     if (!M.use_graphs || M.timing) return launches();
     auto it = M.graphs.find(key);
     if (it == M.graphs.end()) {
+        std::lock_guard<std::recursive_mutex> lock(device_lock());
         // a bounded cache: the argument sets of a run are few (rotating
         // buffers, substep counts); start over if a run keeps making new ones
         if (M.graphs.size() >= 256) {
@@ -111,6 +117,7 @@ template <typename T>
 int dalloc(Model &M, T **p, size_t n) {
     *p = nullptr;
     if (n == 0) n = 1;
+    std::lock_guard<std::recursive_mutex> lock(device_lock());
     HIPCHK(M, hipMalloc((void **)p, n * sizeof(T)));
     HIPCHK(M, hipMemsetAsync(*p, 0, n * sizeof(T), M.stream));
     return 0;
@@ -119,6 +126,7 @@ int dalloc(Model &M, T **p, size_t n) {
 int dalloc_s(Model &M, void **p, size_t n) {
     *p = nullptr;
     if (n == 0) n = 1;
+    std::lock_guard<std::recursive_mutex> lock(device_lock());
     HIPCHK(M, hipMalloc(p, n * M.esize()));
     HIPCHK(M, hipMemsetAsync(*p, 0, n * M.esize(), M.stream));
     return 0;
@@ -141,6 +149,7 @@ int upload_s(Model &M, void *d, const double *h, size_t n) {
 }
 
 void dfree(void *p) {
+    std::lock_guard<std::recursive_mutex> lock(device_lock());
     if (p) (void)hipFree(p);
 }
 
@@ -267,10 +276,16 @@ int shyhip_create(shyhip_model **out, int precision, int nkn, int nel, int nlvdi
     M.prec = precision;
     M.nkn = nkn; M.nel = nel; M.nlvdi = nlvdi; M.nlv = nlv; M.ntr = ntracer;
     M.nl_tmpl = tmpl;
-    if (hip_check(M, hipStreamCreate(&M.stream), "hipStreamCreate")) { delete h; return fail(nullptr, "stream"); }
-    if (hip_check(M, hipHostMalloc((void **)&M.h_pinned, 64 * sizeof(int)), "hipHostMalloc")) {
-        delete h;
-        return fail(nullptr, "pinned");
+    {
+        std::lock_guard<std::recursive_mutex> lock(device_lock());
+        if (hip_check(M, hipStreamCreateWithFlags(&M.stream, hipStreamNonBlocking), "hipStreamCreateWithFlags")) {
+            delete h;
+            return fail(nullptr, "stream");
+        }
+        if (hip_check(M, hipHostMalloc((void **)&M.h_pinned, 64 * sizeof(int)), "hipHostMalloc")) {
+            delete h;
+            return fail(nullptr, "pinned");
+        }
     }
     // default parameters (def_para.f90)
     shyhip_params &p = M.p;
@@ -319,7 +334,10 @@ int shyhip_destroy(shyhip_model *h) {
     Model &M = h->M;
     if (M.stream) (void)hipStreamSynchronize(M.stream);
     (void)pipe_drain(M);
-    for (auto &g : M.graphs) (void)hipGraphExecDestroy(g.second);
+    {
+        std::lock_guard<std::recursive_mutex> lock(device_lock());
+        for (auto &g : M.graphs) (void)hipGraphExecDestroy(g.second);
+    }
     M.graphs.clear();
     State &s = M.s;
     for (void *q : {s.zn, s.zo, s.zen, s.zeo, s.utn, s.vtn, s.uto, s.vto, s.uln, s.vln, s.un, s.vn, s.uo, s.vo,
@@ -376,10 +394,10 @@ int shyhip_destroy(shyhip_model *h) {
     dfree(M.d_err);
     dfree(M.d_cbound);
     dfree(M.d_node_mem); dfree(M.d_elem_mem); dfree(M.d_mem_istot); dfree(M.d_mem_stab);
-    if (M.h_mem_stab) (void)hipHostFree(M.h_mem_stab);
+    if (M.h_mem_stab) { std::lock_guard<std::recursive_mutex> lock(device_lock()); (void)hipHostFree(M.h_mem_stab); }
     dfree(M.d_levels);
-    if (M.h_levels) (void)hipHostFree(M.h_levels);
-    if (M.h_pinned) (void)hipHostFree(M.h_pinned);
+    if (M.h_levels) { std::lock_guard<std::recursive_mutex> lock(device_lock()); (void)hipHostFree(M.h_levels); }
+    if (M.h_pinned) { std::lock_guard<std::recursive_mutex> lock(device_lock()); (void)hipHostFree(M.h_pinned); }
     if (M.stream) (void)hipStreamDestroy(M.stream);
     delete h;
     return 0;
@@ -467,7 +485,7 @@ int shyhip_set_basin(shyhip_model *h, const int *nen3v, const float *xgv, const 
         dfree(M.d_node_mem); dfree(M.d_elem_mem); dfree(M.d_mem_istot); dfree(M.d_mem_stab);
         M.d_node_mem = M.d_elem_mem = M.d_mem_istot = nullptr;
         M.d_mem_stab = nullptr;
-        if (M.h_mem_stab) (void)hipHostFree(M.h_mem_stab);
+        if (M.h_mem_stab) { std::lock_guard<std::recursive_mutex> lock(device_lock()); (void)hipHostFree(M.h_mem_stab); }
         M.h_mem_stab = nullptr;
         M.nmem_dev = nmem;
         if (nmem > 1) {
@@ -480,6 +498,7 @@ int shyhip_set_basin(shyhip_model *h, const int *nen3v, const float *xgv, const 
                      dalloc(M, &M.d_elem_mem, nel) || upload(M, M.d_elem_mem, em.data(), nel) ||
                      dalloc(M, &M.d_mem_istot, 2 * (size_t)nmem) || dalloc(M, &M.d_mem_stab, 3 * (size_t)nmem);
             if (rc) return 1;
+            std::lock_guard<std::recursive_mutex> lock(device_lock());
             HIPCHK(M, hipHostMalloc((void **)&M.h_mem_stab, 2 * (size_t)nmem * sizeof(unsigned long long)));
             M.cur_mem_istot = M.d_mem_istot;
         }
@@ -834,11 +853,12 @@ int shyhip_set_boundaries(shyhip_model *h, int nbnd, const int *ibnd, const int 
     dfree(M.m.bnd_id);
     HIPCHK(M, hipStreamSynchronize(M.stream));     // no upload from h_levels still queued
     dfree(M.d_levels);
-    if (M.h_levels) (void)hipHostFree(M.h_levels);
+    if (M.h_levels) { std::lock_guard<std::recursive_mutex> lock(device_lock()); (void)hipHostFree(M.h_levels); }
     M.d_levels = nullptr;
     M.h_levels = nullptr;
     if (nbnd > 0) {
         if (dalloc(M, &M.d_levels, nbnd)) return 1;
+        std::lock_guard<std::recursive_mutex> lock(device_lock());
         HIPCHK(M, hipHostMalloc((void **)&M.h_levels, nbnd * sizeof(float)));
     }
     M.m.nbnd_nodes = (int)nodes.size();
@@ -1047,15 +1067,18 @@ int shyhip_pull(shyhip_model *h, int field, void *host, int dtype, size_t n) {
     HIPCHK(M, hipStreamSynchronize(M.stream));
     if (f.isint) {
         if (dtype != SHYHIP_I32) return fail(&M, "integer field needs SHYHIP_I32");
-        HIPCHK(M, hipMemcpy(host, *f.ptr, n * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(M, hipMemcpyAsync(host, *f.ptr, n * sizeof(int), hipMemcpyDeviceToHost, M.stream));
+        HIPCHK(M, hipStreamSynchronize(M.stream));
         return 0;
     }
     std::vector<double> t(n);
     if (f.isdouble_dev || M.prec == SHYHIP_FP64) {
-        HIPCHK(M, hipMemcpy(t.data(), *f.ptr, n * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(M, hipMemcpyAsync(t.data(), *f.ptr, n * sizeof(double), hipMemcpyDeviceToHost, M.stream));
+        HIPCHK(M, hipStreamSynchronize(M.stream));
     } else {
         std::vector<float> tf(n);
-        HIPCHK(M, hipMemcpy(tf.data(), *f.ptr, n * sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHK(M, hipMemcpyAsync(tf.data(), *f.ptr, n * sizeof(float), hipMemcpyDeviceToHost, M.stream));
+        HIPCHK(M, hipStreamSynchronize(M.stream));
         for (size_t i = 0; i < n; i++) t[i] = tf[i];
     }
     for (int r = 0; r < sh.rows; r++)
@@ -1102,11 +1125,70 @@ int shyhip_set_members(shyhip_model *h, int nmem, const int *node_start, const i
     return 0;
 }
 
+// The state a member carries from step to step, copied device to device
+// from member sm of src into member dm of dst (same basin: node, element
+// and layer counts, precision, tracers).  Node arrays [l*nkn + k] and
+// element arrays [l*nel + ie] are copied layer by layer with one strided
+// copy each; 2D arrays are one layer.
+int shyhip_copy_member_state(shyhip_model *dst, int dm, shyhip_model *src, int sm) {
+    if (!dst || !src) return fail(nullptr, "null handle");
+    Model &D = dst->M, &S = src->M;
+    if (pipe_drain(D) || pipe_drain(S)) return 1;
+    const int nd = (int)D.mem_node.size() - 1, ns = (int)S.mem_node.size() - 1;
+    if (dm < 0 || dm >= nd || sm < 0 || sm >= ns) return fail(&D, "copy_member_state: member out of range");
+    const int kn = D.mem_node[dm + 1] - D.mem_node[dm], ke = D.mem_elem[dm + 1] - D.mem_elem[dm];
+    if (kn != S.mem_node[sm + 1] - S.mem_node[sm] || ke != S.mem_elem[sm + 1] - S.mem_elem[sm] ||
+        D.prec != S.prec || D.nlvdi != S.nlvdi || D.ntr != S.ntr)
+        return fail(&D, "copy_member_state: the members' basins, precisions or tracers differ");
+    HIPCHK(S, hipStreamSynchronize(S.stream));
+    const size_t es = D.esize();
+    const int L = D.nlvdi;
+    // (array in dst, array in src, element size, node (0) or element (1) array, layers)
+    struct F { void *d; const void *s; size_t sz; int elem; int nl; };
+    const State &a = D.s, &b = S.s;
+    std::vector<F> f = {
+        {a.zn, b.zn, es, 0, 1}, {a.zo, b.zo, es, 0, 1}, {a.zen, b.zen, es, 1, 3}, {a.zeo, b.zeo, es, 1, 3},
+        {a.utn, b.utn, es, 1, L}, {a.vtn, b.vtn, es, 1, L}, {a.uto, b.uto, es, 1, L}, {a.vto, b.vto, es, 1, L},
+        {a.uln, b.uln, es, 1, L}, {a.vln, b.vln, es, 1, L}, {a.un, b.un, es, 1, 1}, {a.vn, b.vn, es, 1, 1},
+        {a.uo, b.uo, es, 1, 1}, {a.vo, b.vo, es, 1, 1}, {a.wln, b.wln, es, 0, L + 1}, {a.wlo, b.wlo, es, 0, L + 1},
+        {a.hdkn, b.hdkn, es, 0, L}, {a.hdko, b.hdko, es, 0, L}, {a.hden, b.hden, es, 1, L},
+        {a.hdeo, b.hdeo, es, 1, L}, {a.iweg, b.iweg, 4, 1, 1}, {a.iwet, b.iwet, 4, 1, 1}, {a.inod, b.inod, 4, 0, 1},
+        {a.rfric, b.rfric, es, 1, 1}, {a.visv, b.visv, es, 0, L + 1}, {a.difv, b.difv, es, 0, L + 1},
+        {a.taux, b.taux, es, 0, 1}, {a.tauy, b.tauy, es, 0, 1}, {a.rqv, b.rqv, es, 0, 1}, {a.rhov, b.rhov, es, 0, L},
+        {a.difhv, b.difhv, es, 1, L}, {a.mflux, b.mflux, es, 0, L},
+    };
+    for (int t = 0; t < D.ntr; t++) {
+        f.push_back({a.tracer[t], b.tracer[t], es, 0, L});
+        f.push_back({a.tracer_bc[t], b.tracer_bc[t], es, 0, L});
+    }
+    for (const F &x : f) {
+        if (!x.d || !x.s) continue;
+        const size_t w = (size_t)(x.elem ? ke : kn) * x.sz;
+        const size_t dp = (size_t)(x.elem ? D.nel : D.nkn) * x.sz, sp = (size_t)(x.elem ? S.nel : S.nkn) * x.sz;
+        const size_t d0 = (size_t)(x.elem ? D.mem_elem[dm] : D.mem_node[dm]) * x.sz;
+        const size_t s0 = (size_t)(x.elem ? S.mem_elem[sm] : S.mem_node[sm]) * x.sz;
+        // zenv/zeov are vertex-major [v*nel + ie]: three "layers" of elements
+        HIPCHK(D, hipMemcpy2DAsync((char *)x.d + d0, dp, (const char *)x.s + s0, sp, w, x.nl, hipMemcpyDeviceToDevice,
+                                   D.stream));
+    }
+    HIPCHK(D, hipStreamSynchronize(D.stream));
+    return 0;
+}
+
 int shyhip_set_wind_uniform(shyhip_model *h, float taux, float tauy) {
     if (!h) return fail(nullptr, "null handle");
     Model &M = h->M;
     if (launch_fill(M, M.s.taux, taux, M.nkn)) return 1;
     return launch_fill(M, M.s.tauy, tauy, M.nkn);
+}
+
+int shyhip_set_member_wind(shyhip_model *h, int m, float taux, float tauy) {
+    if (!h) return fail(nullptr, "null handle");
+    Model &M = h->M;
+    if (m < 0 || m + 1 >= (int)M.mem_node.size()) return fail(&M, "set_member_wind: member out of range");
+    const size_t k0 = (size_t)M.mem_node[m], n = (size_t)(M.mem_node[m + 1] - M.mem_node[m]);
+    if (launch_fill(M, (char *)M.s.taux + k0 * M.esize(), taux, n)) return 1;
+    return launch_fill(M, (char *)M.s.tauy + k0 * M.esize(), tauy, n);
 }
 
 // ------------------------------------------------------------ forcing series
@@ -1689,7 +1771,7 @@ static int tracer_streams(Model &M) {
     for (int g = 1; g < ngroups(M); g++) {
         hipStream_t st = nullptr;
         hipEvent_t ev = nullptr;
-        HIPCHK(M, hipStreamCreate(&st));
+        HIPCHK(M, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         M.tr_stream.push_back(st);
         HIPCHK(M, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
         M.tr_done.push_back(ev);
@@ -1702,7 +1784,7 @@ static int pipe_setup(Model &M) {
     if (M.tr0_stream) return 0;
     if (tracer_streams(M)) return 1;
     if (!M.ev_hydro) HIPCHK(M, hipEventCreateWithFlags(&M.ev_hydro, hipEventDisableTiming));
-    HIPCHK(M, hipStreamCreate(&M.tr0_stream));
+    HIPCHK(M, hipStreamCreateWithFlags(&M.tr0_stream, hipStreamNonBlocking));
     for (int p = 0; p < 2; p++)
         for (int g = 0; g < ngroups(M); g++) {
             hipEvent_t ev = nullptr;
@@ -1964,7 +2046,8 @@ int shyhip_system_get(shyhip_model *h, int n, float *z) {
     if (n != M.nkn) return fail(&M, "system_get: size mismatch");
     std::vector<double> x(n);
     HIPCHK(M, hipStreamSynchronize(M.stream));
-    HIPCHK(M, hipMemcpy(x.data(), M.s.x, n * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(M, hipMemcpyAsync(x.data(), M.s.x, n * sizeof(double), hipMemcpyDeviceToHost, M.stream));
+    HIPCHK(M, hipStreamSynchronize(M.stream));
     for (int k = 0; k < n; k++) z[k] = (float)x[k];
     return 0;
 }

@@ -12,6 +12,7 @@
 #include <hip/hip_ext.h>
 #include <cstdint>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -459,6 +460,13 @@ int launch_scalar_stability(Model &M, float rstol, int *istot, double *rindex);
 
 // helpers
 int hip_check(Model &M, hipError_t e, const char *what);
+// One lock for the operations that may synchronise the whole device --
+// allocating and freeing device or pinned memory -- and for stream capture,
+// shared by every model of the process: a capture on one model's stream must
+// not overlap such an operation made by another thread for another model.
+// The models' streams are non-blocking, so the copies of other threads on the
+// default stream do not touch a capturing stream either.
+std::recursive_mutex &device_lock();
 // A fixed sequence of kernel launches on stream st, replayed from a HIP
 // graph: key holds every argument of the launches (the bytes of their
 // argument structures and grids); the first call with a key captures the

@@ -27,6 +27,7 @@
 // enqueues the expected number of rounds and reads the flags once.
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
 #include <cstring>
 
 #include <algorithm>
@@ -1229,7 +1230,7 @@ static int solve_once(Model &M, int *iters, int *refinements, int wetdry_bnew, b
         if (factor<float>(M, a)) return 1;
         a.rtol = M.p.inner_rtol;
         a.atol = 0.0;
-        const int maxref = 12;
+        const int maxref = 40;
         const int itmax_inner = M.p.itermax > 0 ? (M.p.itermax + 1) / 2 : 500;
         // iterations enqueued for round r: that round's count in the previous
         // solve plus one (kernels beyond convergence return at once but
@@ -1249,10 +1250,13 @@ static int solve_once(Model &M, int *iters, int *refinements, int wetdry_bnew, b
             HIPCHK(M, hipGetLastError());
             return 0;
         };
-        int plan = std::max(1, M.sv.last_nref), enq = 0, c[CT_WORDS] = {};
+        // boost: the rounds enqueued so far did not reach the target (their
+        // chunks, sized from the previous solve, cut them short or the system
+        // changed), so each later round gets twice the iterations of the last
+        int plan = std::min((int)CT_NRITS, std::max(1, M.sv.last_nref)), enq = 0, c[CT_WORDS] = {}, boost = 1;
         for (;;) {
             for (int r = 0; r < plan && enq < maxref; r++, enq++) {
-                if (resid() || enqueue_inner<float>(M, a, chunk_of(enq))) return 1;
+                if (resid() || enqueue_inner<float>(M, a, std::min(itmax_inner, chunk_of(enq) * boost))) return 1;
                 hipLaunchKernelGGL(k_bicg_finish<float>, dim3(g1), dim3(kThreads), 0, M.stream, (const float *)x0,
                                    (const float *)a.dx, M.s.x, n, 1, (const int *)M.sv.ctrl);
             }
@@ -1265,10 +1269,19 @@ static int solve_once(Model &M, int *iters, int *refinements, int wetdry_bnew, b
             }
             if (c[CT_REFCONV]) break;
             if (enq >= maxref) {
-                M.err = "zeta solver: FP64 refinement did not reach the residual target";
+                double sc[16] = {};
+                (void)hipMemcpyAsync(sc, M.sv.scal, sizeof sc, hipMemcpyDeviceToHost, M.stream);
+                (void)hipStreamSynchronize(M.stream);
+                std::string its;
+                for (int r = 0; r < std::min((int)CT_NRITS, c[CT_NREF]); r++) its += " " + std::to_string(c[CT_RITS + r]);
+                char buf[160];
+                std::snprintf(buf, sizeof buf, " (|b - Ax| %.3e, |b| %.3e after %d rounds, %d iterations; first rounds:",
+                              sc[SC_RES64], sc[SC_BNORM], c[CT_NREF], c[CT_ITS]);
+                M.err = std::string("zeta solver: FP64 refinement did not reach the residual target") + buf + its + ")";
                 return 2;
             }
             plan = 1;
+            boost = std::min(2 * boost, 64);
         }
         nref = c[CT_NREF];
         it_total = c[CT_ITS];

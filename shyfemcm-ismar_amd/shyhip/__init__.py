@@ -246,6 +246,15 @@ class Model:
     def set_wind(self, taux, tauy):
         self._chk(self.lib.shyhip_set_wind_uniform(self.h, C.c_float(taux), C.c_float(tauy)), "set_wind_uniform")
 
+    def set_member_wind(self, m, taux, tauy):
+        """Uniform wind stress on the nodes of member m of a union basin."""
+        self._chk(self.lib.shyhip_set_member_wind(self.h, m, C.c_float(taux), C.c_float(tauy)), "set_member_wind")
+
+    def copy_member_state(self, m, src, sm):
+        """Member m of this model takes the state of member sm of model src
+        (the same basin), device to device."""
+        self._chk(self.lib.shyhip_copy_member_state(self.h, m, src.h, sm), "copy_member_state")
+
     def set_boundary_series(self, ibc, time, values, intpol=0, zfact=1.0, dtime0=None):
         """Boundary-level file of level boundary ibc held on the device:
         time (nrec,) in seconds, values (nrec,) for one level on the whole

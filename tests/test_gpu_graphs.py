@@ -83,3 +83,40 @@ def test_graph_ensemble_members_own_substeps(gpu):
     assert res[1][2][1] > 0
     for a, b_ in zip(res[0][0] + res[0][1], res[1][0] + res[1][1]):
         assert np.array_equal(a, b_)
+
+
+def test_graphs_with_models_in_threads(gpu):
+    """Models stepped from several threads at once, each capturing and
+    replaying its own graphs while the others allocate, free and copy: the
+    results equal the same runs one after another, bit for bit.  (The
+    models' streams are non-blocking and a capture holds the library's
+    device lock, so another thread's allocation or copy never lands inside a
+    capture.)"""
+    import threading
+    basins = [mesh.lagoon(n, seed=60 + i, lx=6000.0, ly=3000.0, nlv=5, tile=16) for i, n in enumerate((1500, 2500, 3500))]
+
+    def run(b, out, i):
+        tr = dict(temp=mesh.smooth_tracer(b, 70 + i, 15.0, 25.0), salt=mesh.smooth_tracer(b, 80 + i, 25.0, 38.0))
+        for rep in range(2):               # a model created and closed while others capture
+            m = gpu.Model(b, "fp32", params=PARAMS, visv=1e-3, difv=1e-4, tracers=tr)
+            m.set_wind(*mesh.wind_stress(8.0 + 3 * i, 0.5))
+            t = 0.0
+            for _ in range(12):
+                t += PARAMS["dt"]
+                m.step(_level(t), istot=0)
+            m.sync()
+            out[i] = {k: m.pull(k) for k in FIELDS}
+            m.close()
+
+    seq = [None] * 3
+    for i, b in enumerate(basins):
+        run(b, seq, i)
+    par = [None] * 3
+    ths = [threading.Thread(target=run, args=(b, par, i)) for i, b in enumerate(basins)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    for i in range(3):
+        for k in FIELDS:
+            assert np.array_equal(seq[i][k], par[i][k]), (i, k)
