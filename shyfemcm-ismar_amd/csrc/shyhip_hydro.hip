@@ -1056,21 +1056,29 @@ __global__ __launch_bounds__(kThreads) void k_hydro_vertical(HydroArgs<S> a) {
             for (int s = 0; s < kIncWidth; s++) {
                 const int ie = in.pk[s] >> 2, ii = in.pk[s] & 3;
                 sie[s] = ie;
-                slv[s] = a.ilhv[ie];
-                saj[s] = (R)(4. * a.geo[(size_t)6 * nel + ie]);
-                sb[s] = (R)a.geo[(size_t)ii * nel + ie];
-                sc[s] = (R)a.geo[(size_t)(3 + ii) * nel + ie];
+                slv[s] = 0;
+                saj[s] = sb[s] = sc[s] = (R)0;
+                if (s < in.cnt) {       // (padding slots load nothing)
+                    slv[s] = a.ilhv[ie];
+                    saj[s] = (R)(4. * a.geo[(size_t)6 * nel + ie]);
+                    sb[s] = (R)a.geo[(size_t)ii * nel + ie];
+                    sc[s] = (R)a.geo[(size_t)(3 + ii) * nel + ie];
+                }
             }
         }
         for (int l = threadIdx.x / TN + 1; l <= lim; l += lstep) {
             R vf = 0, va = 0;
-            // layer l of every slot's element loaded at once, then added in
+            // layer l of every slot's element loaded at once (only the
+            // slots that hold an element reaching layer l), then added in
             // list order
             R sun[kIncWidth], svn[kIncWidth], suo[kIncWidth], svo[kIncWidth];
 #pragma unroll
             for (int s = 0; s < kIncWidth; s++) {
-                const size_t o = (size_t)(l - 1) * nel + sie[s];
-                sun[s] = a.utn[o]; svn[s] = a.vtn[o]; suo[s] = a.uto[o]; svo[s] = a.vto[o];
+                sun[s] = svn[s] = suo[s] = svo[s] = (R)0;
+                if (s < in.cnt && slv[s] >= l) {
+                    const size_t o = (size_t)(l - 1) * nel + sie[s];
+                    sun[s] = a.utn[o]; svn[s] = a.vtn[o]; suo[s] = a.uto[o]; svo[s] = a.vto[o];
+                }
             }
             const size_t on = (size_t)(l - 1) * nkn + k;
             const R ar = a.areakv[on];
