@@ -498,6 +498,8 @@ def main():
     ap.add_argument("--c3-nodes", type=int, default=500000)
     ap.add_argument("--c3-layers", type=int, default=20)
     ap.add_argument("--c3-steps", type=int, default=30)
+    ap.add_argument("--only-c3", action="store_true",
+                    help="run only the 500k-node configuration (for its profile) and print its object")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one plain device copy of MIB MiB (calibrates HBM counters)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "lagoon50k_pmc_traffic.md"),
@@ -507,6 +509,9 @@ def main():
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design note, "
               "not a measured path (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.only_c3:
+        print(json.dumps({"c3": c3_run(args, {"fp64": dict(PARAMS), "fp32": dict(PARAMS)})}))
+        return
 
     b, tr = _lagoon(args)
     EL = int(b.ilhv.sum())

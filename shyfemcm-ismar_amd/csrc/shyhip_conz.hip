@@ -719,6 +719,7 @@ static int conz_t(Model &M, const TracerSet &ts, int istot, int isact, hipStream
 // from tables built at set-up, so a lane's loads depend on one another at
 // most two deep.
 constexpr int kBndoLanes = 64;
+constexpr int kBndoNb = 6;          // inner neighbours of a node loaded at once
 
 template <typename S>
 struct BndoArgs {
@@ -746,13 +747,14 @@ __global__ __launch_bounds__(kBndoLanes) void k_bndo(BndoArgs<S> a) {
     __shared__ R val[kBndoLanes];
     __shared__ int status[kBndoLanes];
     const int i = blockIdx.x;
-    if (i >= a.nbndo || !a.open[i]) return;
-    const int k = a.bk[i];
+    if (i >= a.nbndo) return;
+    // the node's static data in one round of loads
+    const int open = a.open[i], k = a.bk[i], lmax = a.lmax[i];
+    if (!open) return;
     if (a.mem_istot && a.isact > a.mem_istot[a.node_mem[k]]) return;
     S *const cv = a.cv[blockIdx.y];
     const S *const rbc = a.rbc[blockIdx.y];
     const int lane = threadIdx.x, l = lane + 1;
-    const int lmax = a.lmax[i];
     const int nkn = a.nkn, nel = a.nel;
     const R flag = (R)-999.;
     status[lane] = 0;
@@ -785,7 +787,29 @@ __global__ __launch_bounds__(kBndoLanes) void k_bndo(BndoArgs<S> a) {
         R bc = 0, tw = 0;
         int ntbc = 0;
         if (binside) {
-            for (int j = a.bptr[i]; j < a.bptr[i + 1]; j++) {
+            // the first kBndoNb inner neighbours' entries loaded at once, then
+            // their values, then added in list order; more one by one
+            const int j0 = a.bptr[i], nj = a.bptr[i + 1] - j0;
+            int nlev[kBndoNb], nbk[kBndoNb];
+            R w[kBndoNb], cvn[kBndoNb];
+#pragma unroll
+            for (int s = 0; s < kBndoNb; s++) {
+                nlev[s] = 0; nbk[s] = 0; w[s] = 0;
+                if (s < nj) { nlev[s] = a.nblev[j0 + s]; nbk[s] = a.bnb[j0 + s]; w[s] = a.bw[j0 + s]; }
+            }
+#pragma unroll
+            for (int s = 0; s < kBndoNb; s++) {
+                cvn[s] = 0;
+                if (s < nj && l <= nlev[s]) cvn[s] = cv[(size_t)(l - 1) * nkn + nbk[s]];
+            }
+#pragma unroll
+            for (int s = 0; s < kBndoNb; s++)
+                if (s < nj && l <= nlev[s]) {
+                    ntbc++;
+                    tw = tw + w[s];
+                    bc = bc + w[s] * cvn[s];
+                }
+            for (int j = j0 + kBndoNb; j < j0 + nj; j++) {
                 if (l <= a.nblev[j]) {
                     ntbc++;
                     tw = tw + a.bw[j];
