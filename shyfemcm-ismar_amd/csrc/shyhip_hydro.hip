@@ -137,16 +137,8 @@ __device__ inline S friction(const HydroArgs<S> &a, int ie, int lmax) {
 // tridiagonal elimination below performs exactly the operations that
 // penta_fact/penta_solve (penta.f90:104-321) perform on the decoupled bands.
 // The coupled case runs the pentadiagonal elimination itself.
-// Deep columns (16 layers and more) would hold the whole band in registers at
-// one wavefront per SIMD; they are asked for two, the rest of the band going
-// to scratch, which L1 and L2 serve.
-template <int NL>
-struct InternWaves {
-    static constexpr int value = NL >= 16 ? 2 : 1;
-};
-
 template <typename S, int NL, bool CORI>
-__global__ __launch_bounds__(kThreads, InternWaves<NL>::value) void k_hydro_intern(HydroArgs<S> a) {
+__global__ __launch_bounds__(kThreads) void k_hydro_intern(HydroArgs<S> a) {
     using R = S;
     const int ie = blockIdx.x * blockDim.x + threadIdx.x;
     if (ie >= a.nel) return;
