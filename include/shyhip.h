@@ -63,7 +63,10 @@ typedef struct {
     double inner_rtol;                          /* FP32 path: relative tolerance of the inner solve */
     /* scalars */
     float rkpar, difmol;
-    int itvd, itvdv;
+    int itvd, itvdv;                            /* horizontal TVD: 0 none, 1 gradients at the
+                                                   nodes (tvd_grad_3d, on the device each
+                                                   substep), 2 upwind points (needs
+                                                   shyhip_tvd_init); vertical TVD 0 or 1 */
     /* transport kernels */
     int node_tile;                              /* most nodes per workgroup of the conz3d node
                                                    phase (a workgroup also holds at most 256
@@ -338,11 +341,12 @@ int  shyhip_set_graphs(shyhip_model *h, int on);
 int  shyhip_graph_stats(shyhip_model *h, long *builds, long *replays);
 /* kernel-group timing with HIP events on the handle's stream (off by
  * default; enabling it adds a synchronisation at the end of every step).
- * id = 0 .. 18: set_dry, copy_uvz, hydro_intern, setnod, zeta_assemble,
+ * id = 0 .. 19: set_dry, copy_uvz, hydro_intern, setnod, zeta_assemble,
  * zeta_solve, system_get_setweg, transports_final, setuvd_baro2l, setdepth,
  * hydro_vertical, ttov_uvint, conz3d_elem, conz3d_node, bndo_setbc, inside
- * zeta_solve the single kernels bilu_factor, bicg_a, bicg_b, and
- * conz3d_prep (the step's element-layer records of the transport) */
+ * zeta_solve the single kernels bilu_factor, bicg_a, bicg_b, conz3d_prep
+ * (the step's element-layer records of the transport) and tvd_grad (the
+ * node gradients of itvd = 1) */
 int  shyhip_set_timing(shyhip_model *h, int on);
 int  shyhip_get_timing(shyhip_model *h, int id, const char **name, double *total_ms, long *count);
 

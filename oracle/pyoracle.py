@@ -291,6 +291,26 @@ class OracleModel:
                                              self.real_c(dt), self.real_c(rstol), C.byref(ri))
         return istot, ri.value
 
+    def tvd_grad(self, cv):
+        """tvd_grad_3d of cv ([nkn, nlv]): the node gradients (gx, gy)."""
+        cv = np.ascontiguousarray(cv, dtype=self.real)
+        gx, gy = np.zeros_like(cv), np.zeros_like(cv)
+        self.lib.oc_tvd_grad_3d(C.byref(self.mesh), ptr(cv), ptr(gx), ptr(gy))
+        return gx, gy
+
+    def hflux(self, cv, ie, l, dt, grads=None):
+        """The horizontal advective flux of element ie (0-based) at layer l
+        (1-based) on cv, as conz3d computes it for a substep of length dt:
+        fl[3].  grads: (gx, gy) of cv for itvd = 1 (computed if None)."""
+        cv = np.ascontiguousarray(cv, dtype=self.real)
+        gx = gy = None
+        if self.params_dict["itvd"] == 1:
+            gx, gy = grads if grads is not None else self.tvd_grad(cv)
+        fl = np.zeros(3)
+        self.lib.oc_conz_hflux(C.byref(self.mesh), C.byref(self.state), C.byref(self.params), ptr(cv), ptr(gx),
+                               ptr(gy), int(ie), int(l), C.c_double(dt), ptr(fl))
+        return fl
+
     # ---------------------------------------------------------------- step
     def set_boundary(self, level):
         self.a["rzv"][:] = FLAG

@@ -135,6 +135,7 @@ int  oc_color(int nkn, int nel, const int *nen3v, int *color, int *color_ptr, in
 void oc_set_area(const oc_mesh *m, real *areakv);
 int  oc_tvd_init(const oc_mesh *m, const real *xgv, const real *ygv, int *tvd_ie, real *tvd_x, real *tvd_y);
 void oc_tvd_upwind_table(const oc_mesh *m, int l, const real *cv, real *out);
+void oc_tvd_grad_3d(const oc_mesh *m, const real *cc, real *gx, real *gy);
 int  oc_scalar_stability(const oc_mesh *m, const oc_state *s, const oc_params *p, real dt, real rstol,
                          real *rindex);
 
@@ -177,6 +178,8 @@ void oc_ttov(const oc_mesh *m, oc_state *s);
 int  oc_hydro(const oc_mesh *m, oc_state *s, const oc_params *p, oc_stats *st);
 
 /* scalar transport */
+void oc_conz_hflux(const oc_mesh *m, const oc_state *s, const oc_params *p, const real *cv, const real *gx,
+                   const real *gy, int ie, int l, double dt, double *fl);
 int  oc_conz3d(const oc_mesh *m, const oc_state *s, const oc_params *p,
                real *cn1, const real *cbound, double ddt, int istot, int isact);
 int  oc_scal3sh(const oc_mesh *m, oc_state *s, const oc_params *p, real *cnv,

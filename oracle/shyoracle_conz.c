@@ -3,7 +3,7 @@
  * (TEST INFRASTRUCTURE; see shyoracle.h).
  *
  * Restates src/shyfem/concentration_omp.f90 (conz3d_omp, conz3d_element,
- * conz3d_nodes), tvd_admin.f90 (tvd_fluxes, tvd_get_upwind_c,
+ * conz3d_nodes), tvd_admin.f90 (tvd_grad_3d, tvd_fluxes, tvd_get_upwind_c,
  * vertical_flux_ie), evgeom.f90 (femintp_xi, xy2xi, adjust_xi),
  * bnd_routines.f90 make_scal_flux and the substep loop of
  * concentration.f90 scal3sh.  Element contributions are accumulated colour
@@ -90,10 +90,54 @@ static int smartdelta(int a, int b)
     return (int)((float)((a + b) - abs(a - b)) / (float)((a + b) + abs(a - b)));
 }
 
-/* tvd_fluxes: tvd_admin.f90:673-823 (itvd = 2 upwind-point variant) */
+/* tvd_grad_3d: tvd_admin.f90:478-543 (itvd = 1).  The gradient of cc at
+ * every node-layer: the mean of the gradients of the incident elements that
+ * have the layer, weighted by ev(10), accumulated element by element in
+ * ascending element number (the reference's ip_sort_elem order) in the
+ * reference's real.  Node-layers below the node's bottom layer are zero. */
+void oc_tvd_grad_3d(const oc_mesh *m, const real *cc, real *gx, real *gy)
+{
+    const int nlvdi = m->nlvdi;
+    const size_t nn = (size_t)nlvdi * m->nkn;
+    real *aux = (real *)calloc(nn, sizeof(real));
+    memset(gx, 0, nn * sizeof(real));
+    memset(gy, 0, nn * sizeof(real));
+    for (int ie = 0; ie < m->nel; ie++) {
+        const real area = (real)EV(10, ie);
+        const int lmax = m->ilhv[ie];
+        for (int l = 1; l <= lmax; l++) {
+            real ggx = 0, ggy = 0;
+            for (int ii = 1; ii <= 3; ii++) {
+                const int k = m->nen3v[3 * ie + ii - 1];
+                const real b = (real)EV(ii + 3, ie), c = (real)EV(ii + 6, ie);
+                ggx = ggx + A3K(cc, l, k) * b;
+                ggy = ggy + A3K(cc, l, k) * c;
+                A3K(aux, l, k) = A3K(aux, l, k) + area;
+            }
+            for (int ii = 1; ii <= 3; ii++) {
+                const int k = m->nen3v[3 * ie + ii - 1];
+                A3K(gx, l, k) = A3K(gx, l, k) + ggx * area;
+                A3K(gy, l, k) = A3K(gy, l, k) + ggy * area;
+            }
+        }
+    }
+    for (int k = 0; k < m->nkn; k++)
+        for (int l = 1; l <= m->ilhkv[k]; l++) {
+            const real area = A3K(aux, l, k);
+            if (area > (real)0) {
+                A3K(gx, l, k) = A3K(gx, l, k) / area;
+                A3K(gy, l, k) = A3K(gy, l, k) / area;
+            }
+        }
+    free(aux);
+}
+
+/* tvd_fluxes: tvd_admin.f90:673-823.  itvd = 2 (gx null): the upstream
+ * value interpolated at the upwind point; itvd = 1: the gradient of the
+ * upstream node kc projected on the edge, grad = 2 (gx dx + gy dy) */
 static void tvd_fluxes(const oc_mesh *m, const oc_state *s, int ie, int l, int itot, int isum,
                        double dt, const double *cl /* cl[ii] at layer l */, const real *cv,
-                       const double *f, double *fl)
+                       const real *gx, const real *gy, const double *f, double *fl)
 {
     const int nlvdi = m->nlvdi;
     const real eps = (real)1.e-8;
@@ -120,8 +164,15 @@ static void tvd_fluxes(const oc_mesh *m, const oc_state *s, int ie, int l, int i
         real dis = (real)EV(16 + iop, ie);
         real vel = (real)fabs((double)(u * dx + v * dy)) / dis;
         real alfa = (real)((dt * vel) / dis);
-        real conu = tvd_upwind(m, ie, l, ic, id, cond, cv);
-        real grad = cond - conu;
+        real grad;
+        if (!gx) {
+            real conu = tvd_upwind(m, ie, l, ic, id, cond, cv);
+            grad = cond - conu;
+        } else {
+            const int kc = m->nen3v[3 * ie + ic - 1];
+            const real gcx = A3K(gx, l, kc), gcy = A3K(gy, l, kc);
+            grad = (real)2. * (gcx * dx + gcy * dy);
+        }
         real rf;
         if ((real)fabs((double)(conc - cond)) < eps) rf = -1;
         else rf = grad / (cond - conc) - (real)1;
@@ -131,6 +182,52 @@ static void tvd_fluxes(const oc_mesh *m, const oc_state *s, int ie, int l, int i
         fl[ic - 1] = fl[ic - 1] - term;
         fl[id - 1] = fl[id - 1] + term;
     }
+}
+
+/* The horizontal advective flux of element ie at layer l
+ * (concentration_omp.f90:640-688): the upwind rule for the vertex
+ * projections f (itot of them negative, isum the sum of their vertex
+ * numbers), replaced by the TVD fluxes when tvd is set (TVD run, no vertex
+ * on an external boundary); cl holds the element's three values at l */
+static void hflux(const oc_mesh *m, const oc_state *s, int ie, int l, int itot, int isum, double dt,
+                  const double *cl, const real *cv, int tvd, const real *gx, const real *gy,
+                  const double *f, double *fl)
+{
+    if (itot == 1) {
+        for (int ii = 0; ii < 3; ii++) fl[ii] = f[ii] * cl[isum - 1];
+    } else if (itot == 2) {
+        int is2 = 6 - isum;
+        for (int ii = 0; ii < 3; ii++) fl[ii] = f[ii] * cl[ii];
+        fl[is2 - 1] = 0.;
+        fl[is2 - 1] = -(fl[0] + fl[1] + fl[2]);
+    } else {
+        fl[0] = fl[1] = fl[2] = 0.;
+    }
+    if (tvd) tvd_fluxes(m, s, ie, l, itot, isum, dt, cl, cv, gx, gy, f, fl);
+}
+
+/* hflux of element ie (0-based) at layer l on the tracer cv as conz3d
+ * computes it for a substep of length dt (the tests compare the TVD
+ * variants element by element); gx, gy: the node gradients of cv
+ * (oc_tvd_grad_3d) for itvd = 1, else unused */
+void oc_conz_hflux(const oc_mesh *m, const oc_state *s, const oc_params *p, const real *cv, const real *gx,
+                   const real *gy, int ie, int l, double dt, double *fl)
+{
+    const int nlvdi = m->nlvdi;
+    const double az = p->azpar, azt = 1. - az;
+    const double us = az * A3E(s->utlnv, l, ie) + azt * A3E(s->utlov, l, ie);
+    const double vs = az * A3E(s->vtlnv, l, ie) + azt * A3E(s->vtlov, l, ie);
+    int itot = 0, isum = 0, iext = 0;
+    double f[3], cl[3];
+    for (int ii = 1; ii <= 3; ii++) {
+        const int k = m->nen3v[3 * ie + ii - 1];
+        f[ii - 1] = us * EV(ii + 3, ie) + vs * EV(ii + 6, ie);
+        if (f[ii - 1] < 0.) { itot++; isum += ii; }
+        cl[ii - 1] = A3K(cv, l, k);
+        if (m->iextb && m->iextb[k]) iext++;
+    }
+    const int tvd = p->itvd > 0 && iext == 0;
+    hflux(m, s, ie, l, itot, isum, dt, cl, cv, tvd, p->itvd == 1 ? gx : NULL, p->itvd == 1 ? gy : NULL, f, fl);
 }
 
 /* conz3d_omp + conz3d_element + conz3d_nodes: concentration_omp.f90:65-886.
@@ -152,6 +249,14 @@ int oc_conz3d(const oc_mesh *m, const oc_state *s, const oc_params *p,
     double *cdiag = (double *)calloc(nn, sizeof(double));
     double *clow = (double *)calloc(nn, sizeof(double));
     double *chigh = (double *)calloc(nn, sizeof(double));
+    /* itvd = 1: the node gradients of the tracer entering the substep
+     * (tvd_grad_3d before conz3d_omp, concentration.f90:744) */
+    real *gx = NULL, *gy = NULL;
+    if (btvd && itvd == 1) {
+        gx = (real *)malloc(nn * sizeof(real));
+        gy = (real *)malloc(nn * sizeof(real));
+        oc_tvd_grad_3d(m, cn1, gx, gy);
+    }
     int L2 = nlvdi + 2;
 #define H2(a, l, ii) (a)[(l) + L2 * ((ii) - 1)]
     /* colour by colour; inside a colour no two elements share a node, so
@@ -274,19 +379,9 @@ int oc_conz3d(const oc_mesh *m, const oc_state *s, const oc_params *p,
                 else H2(clc, l, ii) -= aa * w;
                 fw[ii - 1] = aat * (H2(vflux, l - 1, ii) - H2(vflux, l, ii));
             }
-            if (itot == 1) {
-                for (int ii = 0; ii < 3; ii++) fl[ii] = f[ii] * H2(cl, l, isum);
-            } else if (itot == 2) {
-                int is2 = 6 - isum;
-                for (int ii = 0; ii < 3; ii++) fl[ii] = f[ii] * H2(cl, l, ii + 1);
-                fl[is2 - 1] = 0.;
-                fl[is2 - 1] = -(fl[0] + fl[1] + fl[2]);
-            } else {
-                fl[0] = fl[1] = fl[2] = 0.;
-            }
-            if (btvd && iext == 0) {
-                double cll[3] = {H2(cl, l, 1), H2(cl, l, 2), H2(cl, l, 3)};
-                tvd_fluxes(m, s, ie, l, itot, isum, dt, cll, cn1, f, fl);
+            {
+                const double cll[3] = {H2(cl, l, 1), H2(cl, l, 2), H2(cl, l, 3)};
+                hflux(m, s, ie, l, itot, isum, dt, cll, cn1, btvd && iext == 0, gx, gy, f, fl);
             }
             for (int ii = 1; ii <= 3; ii++) {
                 int k = kn[ii - 1];
@@ -341,6 +436,7 @@ int oc_conz3d(const oc_mesh *m, const oc_state *s, const oc_params *p,
         for (size_t i = 0; i < nn; i++) cn1[i] = (real)cn[i];
 #undef H2
     free(cn); free(cdiag); free(clow); free(chigh);
+    free(gx); free(gy);
     return ret;
 }
 

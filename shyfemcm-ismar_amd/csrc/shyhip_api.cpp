@@ -309,7 +309,7 @@ int shyhip_create(shyhip_model **out, int precision, int nkn, int nel, int nlvdi
     rc |= dalloc(M, &s.iweg, ne); rc |= dalloc(M, &s.iwet, ne); rc |= dalloc(M, &s.inod, nk);
     rc |= dalloc(M, &s.zel, 6 * ne);
     rc |= dalloc(M, &s.rhs, nk); rc |= dalloc(M, &s.x, nk);
-    rc |= dalloc_s(M, &s.hbuf, 3 * L * ne);
+    rc |= dalloc_s(M, &s.hbuf, conz_hbuf_elems(M, 1));
     s.tracer.assign(ntracer, nullptr);
     s.tracer_bc.assign(ntracer, nullptr);
     for (int t = 0; t < ntracer; t++) {
@@ -419,8 +419,7 @@ static int check_params(Model *M, const shyhip_params *p) {
     if (p->ireib < 0 || (p->ireib > 7 && p->ireib != 10))
         return fail(M, "unknown friction (ireib), as bottom_friction stops (chezy.f90:373-376)");
     if (p->rkpar != 0.f) return fail(M, "horizontal diffusion (rkpar != 0) is not supported by the device path");
-    if (p->itvd == 1) return fail(M, "itvd = 1 (gradient TVD) is not supported by the device path; use 0 or 2");
-    if (p->itvd < 0 || p->itvd > 2 || p->itvdv < 0 || p->itvdv > 1) return fail(M, "itvd must be 0 or 2, itvdv 0 or 1");
+    if (p->itvd < 0 || p->itvd > 2 || p->itvdv < 0 || p->itvdv > 1) return fail(M, "itvd must be 0, 1 or 2, itvdv 0 or 1");
     if (p->block_rows < 0 || p->block_rows > kMaxBlockRows) return fail(M, "block_rows must be in 0..1024");
     if (p->tri_sweeps < 0 || p->tri_sweeps > 64) return fail(M, "tri_sweeps must be in 0..64");
     if (p->node_tile < 0 || p->node_tile > kNodeTileMax)
@@ -1701,7 +1700,7 @@ static int group_buffers(Model &M) {
     if ((int)M.grp_hbuf.size() != ngroups(M))
         for (int g = 0; g < ngroups(M); g++) {
             void *hb = nullptr;
-            if (dalloc_s(M, &hb, (size_t)group_of(M, g).n * 3 * M.nlvdi * M.nel)) return 1;
+            if (dalloc_s(M, &hb, conz_hbuf_elems(M, group_of(M, g).n))) return 1;
             M.grp_hbuf.push_back(hb);
         }
     if ((int)M.grp_rec.size() != ngroups(M))
@@ -1911,7 +1910,8 @@ int shyhip_get_timing(shyhip_model *h, int id, const char **name, double *total_
     static const char *names[T_NTIMERS] = {"set_dry", "copy_uvz", "hydro_intern", "setnod", "zeta_assemble",
                                            "zeta_solve", "system_get_setweg", "transports_final", "setuvd_baro2l",
                                            "setdepth", "hydro_vertical", "ttov_uvint", "conz3d_elem", "conz3d_node",
-                                           "bndo_setbc", "bilu_factor", "bicg_a", "bicg_b", "conz3d_prep"};
+                                           "bndo_setbc", "bilu_factor", "bicg_a", "bicg_b", "conz3d_prep",
+                                           "tvd_grad"};
     if (!h) return fail(nullptr, "null handle");
     if (id < 0 || id >= T_NTIMERS) return fail(&h->M, "timer id out of range");
     if (timer_flush(h->M)) return 1;

@@ -3,10 +3,11 @@
 // Two kernels per substep:
 //   k_conz_elem  one work-item per element-layer of a precomputed work list
 //                (workgroups of equal work whatever the depth): horizontal
-//                advective fluxes (upwind rule or the itvd=2 TVD scheme with
-//                precomputed upwind interpolation weights) -> 3*fl per
-//                vertex, stored by the vertex's position in its node's
-//                colour-order incidence list.
+//                advective fluxes (upwind rule, the itvd=2 TVD scheme with
+//                precomputed upwind interpolation weights, or the itvd=1
+//                scheme with node gradients) -> 3*fl per vertex, stored by
+//                the vertex's position in its node's colour-order incidence
+//                list.
 //   k_conz_node  one workgroup per node tile (at most 64 nodes and 256
 //                node-layers).  Phase 1, one work-item per node-layer: the
 //                vertical diffusion/advection terms of the node column (they
@@ -68,6 +69,11 @@ struct ConzArgs {
     // the step's element-layer records (k_conz_prep): int and double columns
     int *rec_i;
     double *rec_d;
+    // itvd = 1: the node-layer list, the element-order incidence lists and
+    // the node gradients [((l-1)*NT + t)*2 + xy][nkn] (after hbuf's terms)
+    const int *nitem, *inc_cnt, *inc_el, *nk_el;
+    int n_nitem;
+    S *grad;
 };
 
 template <typename S>
@@ -179,13 +185,20 @@ __global__ __launch_bounds__(kThreads) void k_conz_prep(ConzArgs<S> a) {
             const R dis = (R)sel3(dis3, iop - 1);
             const R vel = (R)fabs((double)(ul * dx + vl * dy)) / dis;
             const R alfa = (R)((dte * (double)vel) / (double)dis);
-            const size_t po = (size_t)pidx * nel + ie;
-            const bool pup = a.tvd_el[po] >= 0 && a.tvd_lev[po] >= l;
-            code |= (ic | (id << 2) | ((pup ? 1 : 0) << 4)) << (8 + 8 * k);
+            if (a.itvd == 2) {
+                const size_t po = (size_t)pidx * nel + ie;
+                const bool pup = a.tvd_el[po] >= 0 && a.tvd_lev[po] >= l;
+                code |= (ic | (id << 2) | ((pup ? 1 : 0) << 4)) << (8 + 8 * k);
 #pragma unroll
-            for (int q = 0; q < 3; q++) {
-                ri[(size_t)(RI_TKN + 3 * k + q) * n + i] = a.tvd_kn[((size_t)q * 6 + pidx) * nel + ie];
-                rd[(size_t)(RD_TW + 3 * k + q) * n + i] = a.tvd_w[((size_t)q * 6 + pidx) * nel + ie];
+                for (int q = 0; q < 3; q++) {
+                    ri[(size_t)(RI_TKN + 3 * k + q) * n + i] = a.tvd_kn[((size_t)q * 6 + pidx) * nel + ie];
+                    rd[(size_t)(RD_TW + 3 * k + q) * n + i] = a.tvd_w[((size_t)q * 6 + pidx) * nel + ie];
+                }
+            } else {
+                // itvd = 1: the edge vector, for the gradient's projection
+                code |= (ic | (id << 2)) << (8 + 8 * k);
+                rd[(size_t)(RD_TW + 3 * k) * n + i] = (double)dx;
+                rd[(size_t)(RD_TW + 3 * k + 1) * n + i] = (double)dy;
             }
             rd[(size_t)(RD_ALFA + k) * n + i] = (double)alfa;
         }
@@ -201,7 +214,127 @@ __global__ __launch_bounds__(kThreads) void k_conz_prep(ConzArgs<S> a) {
     ri[(size_t)RI_CODE * n + i] = code;
 }
 
+// ------------------------------------------------------------------ gradient TVD
+// itvd = 1 (tvd_grad_3d, tvd_admin.f90:478-543, called before each substep's
+// conz3d_omp, concentration.f90:744): each tracer's gradient at every
+// node-layer, the mean of the gradients of the incident elements that have
+// the layer, weighted by the element area ev(10), in the precision of the
+// tracers.  k_tvd_grad_elem, one work-item per element-layer entry, writes
+// the element's gradient times its area (the reference's ggx*area,
+// ggy*area) into the element-term scratch, which k_conz_elem overwrites
+// only afterwards; k_tvd_grad_node, one work-item per node-layer entry,
+// adds the incident elements' products and areas in element order -- the
+// order of the reference's element loop -- and divides.
+
+// element-layer (ie, l) gradient product of tracer t, component xy, in hbuf
+__device__ inline size_t egrad_at(int nel, int NT, int t, int xy, int l, int ie) {
+    return ((size_t)((l - 1) * NT + t) * 2 + xy) * nel + ie;
+}
+__device__ inline size_t ngrad_at(int nkn, int NT, int t, int xy, int l, int k) {
+    return ((size_t)((l - 1) * NT + t) * 2 + xy) * nkn + k;
+}
+
 template <typename S, int NT>
+__global__ __launch_bounds__(kThreads) void k_tvd_grad_elem(ConzArgs<S> a) {
+    using R = S;
+    const int i = xcd_block() * kThreads + threadIdx.x;
+    if (i >= a.n_eitem) return;
+    const unsigned w = (unsigned)a.eitem[i];
+    const int ie = (int)(w >> 5), l = (int)(w & 31u) + 1;
+    if (a.mem_istot && a.isact > a.mem_istot[a.elem_mem[ie]]) return;
+    const int nel = a.nel, nkn = a.nkn;
+    int kn[3];
+    R b[3], c[3];
+#pragma unroll
+    for (int ii = 0; ii < 3; ii++) {
+        kn[ii] = a.nen3v[3 * ie + ii];
+        b[ii] = (R)a.geo[(size_t)ii * nel + ie];
+        c[ii] = (R)a.geo[(size_t)(3 + ii) * nel + ie];
+    }
+    const R area = (R)a.geo[(size_t)6 * nel + ie];
+    S v[NT][3];
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int ii = 0; ii < 3; ii++) {
+            const S x = a.c[t][(size_t)(l - 1) * nkn + kn[ii]];
+            v[t][ii] = (a.clip[t] && x < (S)1.e-15) ? (S)0 : x;
+        }
+    S *eg = a.hbuf;
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        R ggx = 0, ggy = 0;
+#pragma unroll
+        for (int ii = 0; ii < 3; ii++) {
+            ggx = ggx + (R)v[t][ii] * b[ii];
+            ggy = ggy + (R)v[t][ii] * c[ii];
+        }
+        eg[egrad_at(nel, NT, t, 0, l, ie)] = ggx * area;
+        eg[egrad_at(nel, NT, t, 1, l, ie)] = ggy * area;
+    }
+}
+
+template <typename S, int NT>
+__global__ __launch_bounds__(kThreads) void k_tvd_grad_node(ConzArgs<S> a) {
+    using R = S;
+    const int i = xcd_block() * kThreads + threadIdx.x;
+    if (i >= a.n_nitem) return;
+    const unsigned w = (unsigned)a.nitem[i];
+    const int k = (int)(w >> 5), l = (int)(w & 31u) + 1;
+    if (a.mem_istot && a.isact > a.mem_istot[a.node_mem[k]]) return;
+    const int nel = a.nel, nkn = a.nkn;
+    const S *eg = a.hbuf;
+    // the first kIncWidth incident elements' data loaded at once (padding
+    // slots hold element 0, a valid index), the rest one by one
+    IncSlots inc;
+    inc.load(a.inc_cnt, a.inc_el, nkn, k);
+    int lv[kIncWidth];
+    R ar[kIncWidth], gp[NT][2][kIncWidth];
+#pragma unroll
+    for (int s = 0; s < kIncWidth; s++) {
+        const int ie = inc.pk[s] >> 2;
+        lv[s] = a.ilhv[ie];
+        ar[s] = (R)a.geo[(size_t)6 * nel + ie];
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            const bool has = lv[s] >= l;      // no product below the element's bottom
+            gp[t][0][s] = has ? eg[egrad_at(nel, NT, t, 0, l, ie)] : (R)0;
+            gp[t][1][s] = has ? eg[egrad_at(nel, NT, t, 1, l, ie)] : (R)0;
+        }
+    }
+    R sx[NT], sy[NT], sa = 0;
+#pragma unroll
+    for (int t = 0; t < NT; t++) sx[t] = sy[t] = 0;
+#pragma unroll
+    for (int s = 0; s < kIncWidth; s++)
+        if (s < inc.cnt && lv[s] >= l) {
+            sa = sa + ar[s];
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                sx[t] = sx[t] + gp[t][0][s];
+                sy[t] = sy[t] + gp[t][1][s];
+            }
+        }
+    const int q0 = a.nkc_ptr[k], q1 = a.nkc_ptr[k + 1];
+    for (int q = q0 + kIncWidth; q < q1; q++) {
+        const int ie = a.nk_el[q] >> 2;
+        if (a.ilhv[ie] < l) continue;
+        sa = sa + (R)a.geo[(size_t)6 * nel + ie];
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            sx[t] = sx[t] + eg[egrad_at(nel, NT, t, 0, l, ie)];
+            sy[t] = sy[t] + eg[egrad_at(nel, NT, t, 1, l, ie)];
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        a.grad[ngrad_at(nkn, NT, t, 0, l, k)] = sa > (R)0 ? sx[t] / sa : sx[t];
+        a.grad[ngrad_at(nkn, NT, t, 1, l, k)] = sa > (R)0 ? sy[t] / sa : sy[t];
+    }
+}
+
+// GRAD: itvd = 1 (the upstream node's gradient in place of the upwind point)
+template <typename S, int NT, bool GRAD>
 __global__ __launch_bounds__(kThreads) void k_conz_elem(ConzArgs<S> a) {
     using R = S;
     const int i = xcd_block() * kThreads + threadIdx.x;
@@ -253,26 +386,41 @@ __global__ __launch_bounds__(kThreads) void k_conz_elem(ConzArgs<S> a) {
         const int itot2 = itot - 1, itot1 = 2 - itot;
         const int ip = itot2 * (6 - isum) + itot1 * isum;
         const R fact = (R)(-itot2 + itot1);
-        int tkn[2][3];
-        double tw[2][3];
+        // itvd = 2: upwind nodes and weights; itvd = 1: the edge vector
+        // (tw[k][0..1])
+        int tkn[2][3] = {};
+        double tw[2][3] = {};
         R alfa[2];
 #pragma unroll
         for (int k = 0; k < 2; k++) {
 #pragma unroll
-            for (int q = 0; q < 3; q++) {
-                tkn[k][q] = ri[(size_t)(RI_TKN + 3 * k + q) * n + i];
+            for (int q = 0; q < (GRAD ? 2 : 3); q++) {
+                if (!GRAD) tkn[k][q] = ri[(size_t)(RI_TKN + 3 * k + q) * n + i];
                 tw[k][q] = rd[(size_t)(RD_TW + 3 * k + q) * n + i];
             }
             alfa[k] = (R)rd[(size_t)(RD_ALFA + k) * n + i];
         }
         S cu[2][NT][3];
+        if (!GRAD) {
 #pragma unroll
-        for (int k = 0; k < 2; k++)
-            if ((code >> (12 + 8 * k)) & 1)
+            for (int k = 0; k < 2; k++)
+                if ((code >> (12 + 8 * k)) & 1)
 #pragma unroll
-                for (int t = 0; t < NT; t++)
+                    for (int t = 0; t < NT; t++)
 #pragma unroll
-                    for (int q = 0; q < 3; q++) cu[k][t][q] = a.c[t][(size_t)(l - 1) * nkn + tkn[k][q]];
+                        for (int q = 0; q < 3; q++) cu[k][t][q] = a.c[t][(size_t)(l - 1) * nkn + tkn[k][q]];
+        } else {
+            // the gradient (x, y) of each tracer at each pair's upstream node
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const int kc = sel3(kn, ((code >> (8 + 8 * k)) & 3) - 1);
+#pragma unroll
+                for (int t = 0; t < NT; t++) {
+                    cu[k][t][0] = a.grad[ngrad_at(nkn, NT, t, 0, l, kc)];
+                    cu[k][t][1] = a.grad[ngrad_at(nkn, NT, t, 1, l, kc)];
+                }
+            }
+        }
 #pragma unroll
         for (int k = 0; k < 2; k++) {
             const int ck = code >> (8 + 8 * k);
@@ -283,13 +431,19 @@ __global__ __launch_bounds__(kThreads) void k_conz_elem(ConzArgs<S> a) {
 #pragma unroll
             for (int t = 0; t < NT; t++) {
                 const R conc = clip(t, sel3(cr[t], ic - 1)), cond = clip(t, sel3(cr[t], id - 1));
-                R conu = cond;
-                if (pup) {
-                    double zh = 0.;
-                    for (int q = 0; q < 3; q++) zh = zh + (double)clip(t, cu[k][t][q]) * tw[k][q];
-                    conu = (R)zh;
+                R grad;
+                if (GRAD) {
+                    // itvd = 1: grad = 2 (g(kc) . edge), the edge vector in tw[k][0..1]
+                    grad = (R)2. * ((R)cu[k][t][0] * (R)tw[k][0] + (R)cu[k][t][1] * (R)tw[k][1]);
+                } else {
+                    R conu = cond;
+                    if (pup) {
+                        double zh = 0.;
+                        for (int q = 0; q < 3; q++) zh = zh + (double)clip(t, cu[k][t][q]) * tw[k][q];
+                        conu = (R)zh;
+                    }
+                    grad = cond - conu;
                 }
-                const R grad = cond - conu;
                 R rf;
                 if ((R)fabs((double)(conc - cond)) < (R)1.e-8) rf = -1;
                 else rf = grad / (cond - conc) - (R)1;
@@ -638,7 +792,17 @@ static int conz_nl(Model &M, const TracerSet &ts, int istot, int isact, hipStrea
         launch_timed(M, T_CONZ_PREP, k_conz_prep<S>, dim3(ge), dim3(kThreads), 0, st, a);
         HIPCHK(M, hipGetLastError());
     }
-    launch_timed(M, T_CONZ_ELEM, k_conz_elem<S, NT>, dim3(ge), dim3(kThreads), 0, st, a);
+    if (a.itvd == 1) {
+        if (!M.m.conz_nitem) return hip_check(M, hipErrorNotInitialized, "conz3d: no node-layer list");
+        if (!a.grad) return hip_check(M, hipErrorInvalidValue, "conz3d: no gradient buffer");
+        const int gn = (M.m.n_nitem + kThreads - 1) / kThreads;
+        launch_timed(M, T_TVD_GRAD, k_tvd_grad_elem<S, NT>, dim3(ge), dim3(kThreads), 0, st, a);
+        launch_timed(M, T_TVD_GRAD, k_tvd_grad_node<S, NT>, dim3(gn), dim3(kThreads), 0, st, a);
+        HIPCHK(M, hipGetLastError());
+        launch_timed(M, T_CONZ_ELEM, k_conz_elem<S, NT, true>, dim3(ge), dim3(kThreads), 0, st, a);
+    } else {
+        launch_timed(M, T_CONZ_ELEM, k_conz_elem<S, NT, false>, dim3(ge), dim3(kThreads), 0, st, a);
+    }
     HIPCHK(M, hipGetLastError());
     launch_timed(M, T_CONZ_NODE, k_conz_node<S, NL, NT>, dim3(M.m.n_ntile), dim3(kThreads), 0, st, a);
     HIPCHK(M, hipGetLastError());
@@ -682,6 +846,10 @@ static ConzArgs<S> conz_args(Model &M, const TracerSet &ts, int istot, int isact
     a.hbuf = (S *)hbuf;
     a.err = M.d_err;
     a.eitem = M.m.conz_eitem; a.n_eitem = M.m.n_eitem; a.ntile = M.m.conz_ntile;
+    a.nitem = M.m.conz_nitem; a.n_nitem = M.m.n_nitem;
+    a.inc_cnt = M.m.inc_cnt; a.inc_el = M.m.inc_el; a.nk_el = M.m.nk_el;
+    // the node gradients follow the element terms in the same buffer
+    a.grad = (M.p.itvd == 1 && hbuf) ? (S *)hbuf + conz_hbuf_terms(M, ts.n) : nullptr;
     a.rec_i = (int *)rec;
     a.rec_d = rec ? (double *)((char *)rec + conz_rec_bytes(M.m.n_eitem) -
                                (size_t)RD_N * M.m.n_eitem * sizeof(double))
@@ -690,6 +858,11 @@ static ConzArgs<S> conz_args(Model &M, const TracerSet &ts, int istot, int isact
 }
 
 size_t conz_record_bytes(const Model &M) { return conz_rec_bytes(M.m.n_eitem); }
+
+size_t conz_hbuf_terms(const Model &M, int ntr) { return (size_t)ntr * 3 * M.nlvdi * M.nel; }
+size_t conz_hbuf_elems(const Model &M, int ntr) {
+    return conz_hbuf_terms(M, ntr) + (size_t)ntr * 2 * M.nlvdi * M.nkn;
+}
 
 template <typename S, int NT>
 static int conz_t(Model &M, const TracerSet &ts, int istot, int isact, hipStream_t st, void *hbuf, void *rec,

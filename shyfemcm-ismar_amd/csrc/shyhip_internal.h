@@ -173,7 +173,7 @@ struct Solver {               // Krylov work space (value type V = S of the mode
 
 // kernel groups timed with HIP events when timing is enabled
 enum TimerId { T_SETDRY = 0, T_COPY, T_INTERN, T_SETNOD, T_ZETA, T_SOLVE, T_WETDRY, T_FINAL, T_SETUVD, T_DEPTH,
-               T_VERT, T_TTOV, T_CONZ_ELEM, T_CONZ_NODE, T_BNDO, T_FACTOR, T_BICG_A, T_BICG_B, T_CONZ_PREP, T_NTIMERS };
+               T_VERT, T_TTOV, T_CONZ_ELEM, T_CONZ_NODE, T_BNDO, T_FACTOR, T_BICG_A, T_BICG_B, T_CONZ_PREP, T_TVD_GRAD, T_NTIMERS };
 
 // A record series resident on the device (time interpolation of forcing
 // files): times [nrec], values record-major [nrec][ncomp][nval]; nintp
@@ -442,6 +442,11 @@ struct TracerSet {
 int launch_conz3d(Model &M, const TracerSet &ts, int istot, int isact, hipStream_t st, void *hbuf, void *rec,
                   bool prep);
 size_t conz_record_bytes(const Model &M);
+// the element-term scratch (hbuf) of a launch of ntr tracers, in values of
+// the tracer type: the element terms (conz_hbuf_terms), then room for the
+// node gradients of itvd = 1 (conz_hbuf_elems in all)
+size_t conz_hbuf_terms(const Model &M, int ntr);
+size_t conz_hbuf_elems(const Model &M, int ntr);
 // the bytes that identify the launches of a tracer group's substeps 1..istot
 // (conz3d and bndo_setbc), appended to key (a graph_run key)
 void conz3d_key(Model &M, const TracerSet &ts, int istot, void *hbuf, void *rec, void *bscal, std::string &key);

@@ -8,9 +8,11 @@
 ! (shyhip_shyfem) holds the concentration meanwhile.
 !
 ! Supported as on the device path: horizontal diffusion (rkpar), nudging
-! (robs), settling (wsink) and loads (rload) zero, itvd 0 or 2.  Any other
-! value ends the run with an error stop, as the reference stops on options
-! it cannot run.  azpar, adpar, aapar, difmol, itvd and itvdv are taken
+! (robs), settling (wsink) and loads (rload) zero.  Any other value ends the
+! run with an error stop, as the reference stops on options it cannot run.
+! With itvd = 1 the device computes the node gradients of cn1 itself (the
+! gradxv, gradyv that tvd_grad_3d computed from the same cn1 just before
+! this call, concentration.f90:744), so gradxv and gradyv are not read.  azpar, adpar, aapar, difmol, itvd and itvdv are taken
 ! over into the model's parameters; ddt must be the model's time step.
 ! The reference runs the T and S calls in concurrent OpenMP tasks
 ! (baroclinic.f90:466-500); here they take turns on the one model.
@@ -43,7 +45,6 @@ subroutine conz3d_omp(cn1, co1, ddt, rkpar, difhv, difv, difmol, cbound, itvd, i
   if (robs /= 0.) stop 'error stop conz3d_omp: nudging (robs /= 0) is not on the device'
   if (wsink /= 0.) stop 'error stop conz3d_omp: settling (wsink /= 0) is not on the device'
   if (rload /= 0.) stop 'error stop conz3d_omp: loads (rload /= 0) are not on the device'
-  if (itvd == 1) stop 'error stop conz3d_omp: itvd = 1 is not on the device'
 
 !$omp critical (shyhip_conz3d)
   call shyhip_check(shy, shyhip_get_sizes(shy, nkn, nel, nlvdi, nlv, ntr), 'conz3d_omp')

@@ -392,7 +392,7 @@ class Model:
         name = C.c_char_p()
         ms = C.c_double()
         cnt = C.c_long()
-        for i in range(19):
+        for i in range(20):
             self._chk(self.lib.shyhip_get_timing(self.h, i, C.byref(name), C.byref(ms), C.byref(cnt)), "get_timing")
             out[name.value.decode()] = (ms.value, cnt.value)
         return out
