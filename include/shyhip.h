@@ -88,6 +88,16 @@ enum {
     SHYHIP_TAUXNV, SHYHIP_TAUYNV, SHYHIP_RZV, SHYHIP_RQV, SHYHIP_MFLUXV,
     SHYHIP_AREAKV, SHYHIP_CSR_VAL, SHYHIP_RHS,
     SHYHIP_RHOV,               /* density anomaly rho - rowass (rhov), node 3D          */
+    /* optional forcing of hydro, absent until first pushed (a pull before
+     * then fails): atmospheric pressure ppv (Pa) and equilibrium tide zeqv
+     * (m) at the nodes (hydro_intern, hydrodynamic.f90:1025-1033,1235),
+     * the level nudging term andgzv (m/s, hydro_zeta :612,679); elements
+     * with fixed transports iuvfix (element, I32, 0/1: hydro_intern skips
+     * them, :1383-1392, and their momentum leaves the zeta matrix and the
+     * final transports, :627-629,1525) and the transports they take each
+     * step (utfix/vtfix, element 3D, what bclfix writes into utlnv/vtlnv,
+     * bcvel.f90:332-338; not pushed: the previous step's) */
+    SHYHIP_PPV, SHYHIP_ZEQV, SHYHIP_ANDGZV, SHYHIP_IUVFIX, SHYHIP_UTFIX, SHYHIP_VTFIX,
     SHYHIP_TRACER0 = 100,      /* tracer itr: SHYHIP_TRACER0 + itr, node 3D            */
     SHYHIP_TRACER_BC0 = 200    /* its open-boundary value (-999 = none), node 3D        */
 };

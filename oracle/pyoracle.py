@@ -320,6 +320,22 @@ class OracleModel:
         self.a["tauxnv"][:] = taux
         self.a["tauynv"][:] = tauy
 
+    def set_forcing(self, name, arr):
+        """Optional hydro forcing the default runs leave out: atmospheric
+        pressure ppv (Pa), equilibrium tide zeqv (m), level nudging andgzv
+        (m/s) at the nodes, or the fixed-transport flags iuvfix (elements)."""
+        dt = np.int32 if name == "iuvfix" else self.real
+        self.a[name] = np.ascontiguousarray(arr, dtype=dt).copy()
+        setattr(self.state, name, ptr(self.a[name]))
+
+    def set_uvfix(self, iuvfix, utfix=None, vtfix=None):
+        """Elements with fixed transports; utfix/vtfix ([nel, nlv]): the
+        transports they take each step, written into utlnv/vtlnv after
+        copy_uvz as bclfix does (bcvel.f90:332-338) -- None: they keep the
+        previous step's."""
+        self.set_forcing("iuvfix", iuvfix)
+        self.uvfix = None if utfix is None else (np.asarray(utfix), np.asarray(vtfix))
+
     def set_baroclinic(self, temp="temp", salt="salt"):
         """ibarcl = 1: barocl(0)'s start (rhov = 0, rhoset three times,
         baroclinic.f90:398-404); then every step adds the baroclinic term to
@@ -348,6 +364,11 @@ class OracleModel:
         self.state.dtime = self.t
         lib.oc_copy_uvz(m, s)
         lib.oc_copy_depth(m, s)
+        if getattr(self, "uvfix", None) is not None:
+            # the element's layers only, as bclfix writes them
+            sel = (self.a["iuvfix"] != 0)[:, None] & (np.arange(self.nlvdi)[None, :] < self.ilhv[:, None])
+            self.a["utlnv"][sel] = self.uvfix[0][sel]
+            self.a["vtlnv"][sel] = self.uvfix[1][sel]
         self.set_boundary(bnd_level)
         if getattr(self, "bcl", None):
             lib.oc_barocl(m, p, ptr(self.rhov), ptr(self.a["fxv"]), ptr(self.a["fyv"]))

@@ -167,6 +167,7 @@ struct Field {
     Kind kind;
     bool isint = false;
     bool isdouble_dev = false;  // device array always double
+    bool lazy = false;          // optional: allocated at the first push
 };
 
 bool field_of(Model &M, int id, Field &f) {
@@ -210,6 +211,12 @@ bool field_of(Model &M, int id, Field &f) {
     case SHYHIP_AREAKV: return S(&M.m.areakv, K_NODE3);
     case SHYHIP_CSR_VAL: f = Field{(void **)&M.s.aval, K_CSR, false, true}; return true;
     case SHYHIP_RHS: f = Field{(void **)&M.s.rhs, K_RHS, false, true}; return true;
+    case SHYHIP_PPV: f = Field{&M.s.ppv, K_NODE2, false, false, true}; return true;
+    case SHYHIP_ZEQV: f = Field{&M.s.zeqv, K_NODE2, false, false, true}; return true;
+    case SHYHIP_ANDGZV: f = Field{&M.s.andgz, K_NODE2, false, false, true}; return true;
+    case SHYHIP_IUVFIX: f = Field{(void **)&M.s.iuvfix, K_IELEM, true, false, true}; return true;
+    case SHYHIP_UTFIX: f = Field{&M.s.utfix, K_ELEM3, false, false, true}; return true;
+    case SHYHIP_VTFIX: f = Field{&M.s.vtfix, K_ELEM3, false, false, true}; return true;
     default:
         if (id >= SHYHIP_TRACER0 && id < SHYHIP_TRACER0 + M.ntr) return S(&M.s.tracer[id - SHYHIP_TRACER0], K_NODE3);
         if (id >= SHYHIP_TRACER_BC0 && id < SHYHIP_TRACER_BC0 + M.ntr)
@@ -345,6 +352,7 @@ int shyhip_destroy(shyhip_model *h) {
                     s.taux, s.tauy, s.rzv, s.rqv, s.mflux, s.rhov, s.hbuf})
         dfree(q);
     dfree(s.iweg); dfree(s.iwet); dfree(s.inod); dfree(s.zel); dfree(s.aval); dfree(s.zrow); dfree(s.rhs); dfree(s.x);
+    for (void *q : {s.ppv, s.zeqv, s.andgz, (void *)s.iuvfix, s.utfix, s.vtfix}) dfree(q);
     for (void *t : s.tracer) dfree(t);
     for (void *t : s.tracer_bc) dfree(t);
     dfree(M.m.bndo_k); dfree(M.m.bndo_ptr); dfree(M.m.bndo_nb); dfree(M.m.bndo_nx); dfree(M.m.bndo_ny);
@@ -1041,6 +1049,10 @@ int shyhip_push(shyhip_model *h, int field, const void *host, int dtype, size_t 
     if (field == SHYHIP_ZNV || field == SHYHIP_ZOV || field == SHYHIP_ZENV || field == SHYHIP_ZEOV)
         M.sv.have_prev = false;
     if (field == SHYHIP_ZNV || field == SHYHIP_CSR_VAL || field == SHYHIP_RHS) M.sv.fed = false;
+    if (f.lazy && !*f.ptr) {
+        const int rc = f.isint ? dalloc(M, (int **)f.ptr, sh.n) : dalloc_s(M, f.ptr, sh.n);
+        if (rc) return 1;
+    }
     if (f.isint) {
         if (dtype != SHYHIP_I32) return fail(&M, "integer field needs SHYHIP_I32");
         return upload(M, (int *)*f.ptr, (const int *)host, n);
@@ -1063,6 +1075,7 @@ int shyhip_pull(shyhip_model *h, int field, void *host, int dtype, size_t n) {
     if (!field_of(M, field, f)) return fail(&M, "unknown field id " + std::to_string(field));
     const Shape sh = shape_of(M, f.kind);
     if (n != sh.n) return fail(&M, "pull: size mismatch for field " + std::to_string(field));
+    if (f.lazy && !*f.ptr) return fail(&M, "pull: field " + std::to_string(field) + " was never pushed");
     HIPCHK(M, hipStreamSynchronize(M.stream));
     if (f.isint) {
         if (dtype != SHYHIP_I32) return fail(&M, "integer field needs SHYHIP_I32");

@@ -42,6 +42,13 @@ struct HydroArgs {
     S *rfric, *ddx, *ddy, *visv;
     S *taux, *tauy, *rzv, *rqv, *mflux;
     const S *rhov;      // density anomaly (baroclinic runs), null otherwise
+    // optional forcing (null when never pushed): atmospheric pressure ppv
+    // (Pa) and equilibrium tide zeqv (m) at the nodes, the level nudging
+    // term andgzv (m/s), elements with fixed transports (iuvfix) and the
+    // transports they take (utfix/vtfix; null: the previous step's)
+    const S *ppv, *zeqv, *andgz;
+    const int *iuvfix;
+    const S *utfix, *vtfix;
     double *zel, *aval, *rhs, *x;
     S *zrow;
     // inputs of the zeta solve written by the row assembly (null: not fed)
@@ -82,6 +89,9 @@ static HydroArgs<S> make_args(Model &M) {
     a.taux = (S *)M.s.taux; a.tauy = (S *)M.s.tauy; a.rzv = (S *)M.s.rzv; a.rqv = (S *)M.s.rqv;
     a.mflux = M.has_mflux ? (S *)M.s.mflux : nullptr;
     a.rhov = M.bcl_on ? (const S *)M.s.rhov : nullptr;
+    a.ppv = (const S *)M.s.ppv; a.zeqv = (const S *)M.s.zeqv; a.andgz = (const S *)M.s.andgz;
+    a.iuvfix = M.s.iuvfix;
+    a.utfix = (const S *)M.s.utfix; a.vtfix = (const S *)M.s.vtfix;
     a.zel = M.s.zel; a.aval = M.s.aval; a.rhs = M.s.rhs; a.x = M.s.x;
     a.err = M.d_err;
     return a;
@@ -148,15 +158,21 @@ __global__ __launch_bounds__(kThreads) void k_hydro_intern(HydroArgs<S> a) {
     const int k1 = a.nen3v[3 * ie], k2 = a.nen3v[3 * ie + 1], k3 = a.nen3v[3 * ie + 2];
     const int kn[3] = {k1, k2, k3};
     double b[3], c[3];
-    double bz = 0., cz = 0., zmm = 0., taux = 0., tauy = 0.;
+    double bz = 0., cz = 0., zmm = 0., taux = 0., tauy = 0., bpres = 0., cpres = 0.;
     for (int ii = 0; ii < 3; ii++) {
         b[ii] = a.geo[(size_t)ii * nel + ie];
         c[ii] = a.geo[(size_t)(3 + ii) * nel + ie];
         const R ze = a.zeo[(size_t)ii * nel + ie];
-        const double zz = (double)ze;
+        // the level relative to the equilibrium tide, and the atmospheric
+        // pressure gradient (hydrodynamic.f90:1025-1033)
+        const double zz = (double)(a.zeqv ? (R)(ze - a.zeqv[kn[ii]]) : ze);
         zmm += (double)ze;
         bz += zz * b[ii];
         cz += zz * c[ii];
+        if (a.ppv) {
+            bpres += (double)a.ppv[kn[ii]] * b[ii];
+            cpres += (double)a.ppv[kn[ii]] * c[ii];
+        }
         taux += (double)a.taux[kn[ii]];
         tauy += (double)a.tauy[kn[ii]];
     }
@@ -276,7 +292,7 @@ __global__ __launch_bounds__(kThreads) void k_hydro_intern(HydroArgs<S> a) {
             if (blast) { aa = aa + dt * rfric; aat = aat + rfric; }
             const R gravx = (R)(1.0 * grav * hhi * bz);
             const R gravy = (R)(1.0 * grav * hhi * cz);
-            const R presx = 0, presy = 0, xexpl = xex[l - 1], yexpl = yex[l - 1];
+            const R presx = (R)bpres, presy = (R)cpres, xexpl = xex[l - 1], yexpl = yex[l - 1];
             const R ppx_aux = aat * uui - bbt * uuip - cct * uuim - gammat * vvi + gravx + (hhi / rowass) * presx + xexpl;
             const R ppy_aux = aat * vvi - bbt * vvip - cct * vvim + gammat * uui + gravy + (hhi / rowass) * presy + yexpl;
             ppx = ppx + (double)ppx_aux;
@@ -347,7 +363,7 @@ __global__ __launch_bounds__(kThreads) void k_hydro_intern(HydroArgs<S> a) {
             if (blast) { aa = aa + dt * rfric; aat = aat + rfric; }
             const R gravx = (R)(1.0 * grav * hhi * bz);
             const R gravy = (R)(1.0 * grav * hhi * cz);
-            const R presx = 0, presy = 0, xexpl = xex[l - 1], yexpl = yex[l - 1];
+            const R presx = (R)bpres, presy = (R)cpres, xexpl = xex[l - 1], yexpl = yex[l - 1];
             const R ppx_aux = aat * uui - bbt * uuip - cct * uuim - gammat * vvi + gravx + (hhi / rowass) * presx + xexpl;
             const R ppy_aux = aat * vvi - bbt * vvip - cct * vvim + gammat * uui + gravy + (hhi / rowass) * presy + yexpl;
             ppx = ppx + (double)ppx_aux;
@@ -407,16 +423,26 @@ __global__ __launch_bounds__(kThreads) void k_hydro_intern(HydroArgs<S> a) {
         a.zel[(size_t)2 * nel + ie] = dcb;
         a.zel[(size_t)3 * nel + ie] = dcc;
     }
-    // u^hat = u_old - dt * A^-1 F ; store A^-1 H
-    const R dtafix = dt * (R)1;
+    // u^hat = u_old - dt * A^-1 F ; store A^-1 H.  Elements with fixed
+    // transports (afix = 0, hydrodynamic.f90:1383-1392) keep the ones they
+    // hold: the prescribed transports, or those of the previous step.
+    const int afix = a.iuvfix ? 1 - a.iuvfix[ie] : 1;
+    const R dtafix = dt * (R)afix;
     double uold = 0., vold = 0., uhat = 0., vhat = 0.;
     double dsum = 0.;
 #pragma unroll
     for (int l = 1; l <= NL; l++) {
         if (l > ilevel) continue;
-        const R uo = a.uto[(size_t)(l - 1) * nel + ie], vo = a.vto[(size_t)(l - 1) * nel + ie];
-        const S unew = (S)((double)uo - (double)dtafix * sxu[l - 1]);
-        const S vnew = (S)((double)vo - (double)dtafix * sxv[l - 1]);
+        const size_t ol = (size_t)(l - 1) * nel + ie;
+        const R uo = a.uto[ol], vo = a.vto[ol];
+        S unew, vnew;
+        if (afix != 0) {
+            unew = (S)((double)uo - (double)dtafix * sxu[l - 1]);
+            vnew = (S)((double)vo - (double)dtafix * sxv[l - 1]);
+        } else {
+            unew = a.utfix ? a.utfix[ol] : uo;
+            vnew = a.vtfix ? a.vtfix[ol] : vo;
+        }
         a.utn[(size_t)(l - 1) * nel + ie] = unew;
         a.vtn[(size_t)(l - 1) * nel + ie] = vnew;
         uold += (double)uo; vold += (double)vo;
@@ -450,7 +476,8 @@ __device__ inline void zeta_elem(const HydroArgs<S> &a, int ie, S hia[3][3], S h
     const int nel = a.nel;
     const double dt = (double)(R)a.dt;
     const double az = a.az, am = a.am;
-    const double delta = dt * dt * az * am * (double)a.grav * 1;
+    const int afix = a.iuvfix ? 1 - a.iuvfix[ie] : 1;
+    const double delta = dt * dt * az * am * (double)a.grav * afix;
     const double aj = a.geo[(size_t)6 * nel + ie];
     double b[3], c[3], z[3];
     int kn[3];
@@ -470,7 +497,7 @@ __device__ inline void zeta_elem(const HydroArgs<S> &a, int ie, S hia[3][3], S h
             hia[n][m] = (R)(aj * ((n == m ? 4. : 0.) + 12. * h11));
         }
         const double acu = (double)hia[n][0] * z[0] + (double)hia[n][1] * z[1] + (double)hia[n][2] * z[2];
-        const double andg = 4. * aj * dt * 0.;
+        const double andg = 4. * aj * dt * (a.andgz ? (double)a.andgz[kn[n]] : 0.);
         hik[n] = (R)(acu + andg + 12. * aj * dt * (ut * b[n] + vt * c[n]));
     }
     for (int q = 0; q < 3; q++) {
@@ -777,6 +804,7 @@ __device__ inline void transports_final_el(const HydroArgs<S> &a, int ie) {
         cz = cz + dz * a.geo[(size_t)(3 + ii) * nel + ie];
     }
     const int ilevel = a.ilhv[ie];
+    const double rfix = a.iuvfix ? 1 - a.iuvfix[ie] : 1;      // hydrodynamic.f90:1525-1527
     double um = 0., vm = 0.;
     for (int l = 1; l <= ilevel; l++) {
         const int ju = 2 * l - 2, jv = 2 * l - 1;
@@ -789,8 +817,8 @@ __device__ inline void transports_final_el(const HydroArgs<S> &a, int ie) {
             dv = beta * ((double)a.ddy[(size_t)jv * nel + ie] * cz);
         }
         const size_t o = (size_t)(l - 1) * nel + ie;
-        const S u = (S)((double)a.utn[o] - du * 1.0);
-        const S v = (S)((double)a.vtn[o] - dv * 1.0);
+        const S u = (S)((double)a.utn[o] - du * rfix);
+        const S v = (S)((double)a.vtn[o] - dv * rfix);
         a.utn[o] = u;
         a.vtn[o] = v;
         um = um + (double)u;

@@ -136,6 +136,12 @@ struct State {
     double *rhs = nullptr;    // [nkn] (as rvec2d)
     double *x = nullptr;      // [nkn] solution (FP64)
     void *hbuf = nullptr;     // conz3d horizontal terms [l][ii][ie]
+    // optional forcing, allocated when first pushed (null: absent): node
+    // atmospheric pressure (Pa), equilibrium tide (m), level nudging term
+    // (m/s); fixed-transport elements and their transports [l*nel + ie]
+    void *ppv = nullptr, *zeqv = nullptr, *andgz = nullptr;
+    int *iuvfix = nullptr;
+    void *utfix = nullptr, *vtfix = nullptr;
 };
 
 // zeta-solver control words (Solver::ctrl): iteration (+1) at which the

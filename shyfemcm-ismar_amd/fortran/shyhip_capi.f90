@@ -28,6 +28,8 @@ module shyhip_capi
   integer(c_int), parameter :: SHYHIP_RQV = 33, SHYHIP_MFLUXV = 34
   integer(c_int), parameter :: SHYHIP_AREAKV = 35, SHYHIP_CSR_VAL = 36, SHYHIP_RHS = 37
   integer(c_int), parameter :: SHYHIP_RHOV = 38
+  integer(c_int), parameter :: SHYHIP_PPV = 39, SHYHIP_ZEQV = 40, SHYHIP_ANDGZV = 41, SHYHIP_IUVFIX = 42
+  integer(c_int), parameter :: SHYHIP_UTFIX = 43, SHYHIP_VTFIX = 44
   integer(c_int), parameter :: SHYHIP_TRACER0 = 100, SHYHIP_TRACER_BC0 = 200
 
   type, bind(C) :: shyhip_params

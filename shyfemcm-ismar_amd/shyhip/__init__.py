@@ -28,10 +28,12 @@ F32, F64, I32 = 0, 1, 2
 FIELDS = dict(znv=1, zov=2, zenv=3, zeov=4, utlnv=5, vtlnv=6, utlov=7, vtlov=8, unv=9, vnv=10, uov=11, vov=12,
               ulnv=13, vlnv=14, wlnv=15, wlov=16, hdknv=17, hdkov=18, hdenv=19, hdeov=20, iwegv=21, iwetv=22,
               inodv=23, rfricv=24, ddxv=25, ddyv=26, visv=27, difv=28, difhv=29, tauxnv=30, tauynv=31, rzv=32,
-              rqv=33, mfluxv=34, areakv=35, csr_val=36, rhs=37, rhov=38)
+              rqv=33, mfluxv=34, areakv=35, csr_val=36, rhs=37, rhov=38,
+              # optional hydro forcing (absent until pushed)
+              ppv=39, zeqv=40, andgzv=41, iuvfix=42, utfix=43, vtfix=44)
 TRACER0 = 100
 TRACER_BC0 = 200
-INT_FIELDS = {"iwegv", "iwetv", "inodv"}
+INT_FIELDS = {"iwegv", "iwetv", "inodv", "iuvfix"}
 
 
 class ShyParams(C.Structure):
@@ -174,10 +176,11 @@ class Model:
                       difv=(b.nkn, L + 1), iwegv=(b.nel,), iwetv=(b.nel,), inodv=(b.nkn,), rfricv=(b.nel,),
                       ddxv=(b.nel, 2 * L), ddyv=(b.nel, 2 * L), tauxnv=(b.nkn,), tauynv=(b.nkn,), rzv=(b.nkn,),
                       rqv=(b.nkn,), hdknv=(b.nkn, L), hdkov=(b.nkn, L), mfluxv=(b.nkn, L), areakv=(b.nkn, L),
-                      rhs=(b.nkn,), rhov=(b.nkn, L))
+                      rhs=(b.nkn,), rhov=(b.nkn, L), ppv=(b.nkn,), zeqv=(b.nkn,), andgzv=(b.nkn,),
+                      iuvfix=(b.nel,))
         if name in shapes:
             return shapes[name]
-        if name in ("utlnv", "vtlnv", "utlov", "vtlov", "ulnv", "vlnv", "hdenv", "hdeov", "difhv"):
+        if name in ("utlnv", "vtlnv", "utlov", "vtlov", "ulnv", "vlnv", "hdenv", "hdeov", "difhv", "utfix", "vtfix"):
             return (b.nel, L)
         if isinstance(name, int) and name >= TRACER0:
             return (b.nkn, L)
