@@ -62,7 +62,9 @@ typedef struct {
                                                    k > 0: k Jacobi sweeps per triangular factor */
     double inner_rtol;                          /* FP32 path: relative tolerance of the inner solve */
     /* scalars */
-    float rkpar, difmol;
+    float rkpar, difmol;                        /* horizontal diffusion factor of the scalars
+                                                   (times the difhv field; 0 = none), vertical
+                                                   molecular diffusivity */
     int itvd, itvdv;                            /* horizontal TVD: 0 none, 1 gradients at the
                                                    nodes (tvd_grad_3d, on the device each
                                                    substep), 2 upwind points (needs
@@ -72,6 +74,10 @@ typedef struct {
                                                    phase (a workgroup also holds at most 256
                                                    node-layers): 0 = auto (256), else 1..256;
                                                    read by shyhip_set_basin, like block_rows */
+    int idtype;                                 /* weights of horizontal diffusion (diffweight):
+                                                   0 gradient products, 1 the same without
+                                                   positive off-diagonals, 2 (default) edge
+                                                   lengths */
 } shyhip_params;
 
 /* field ids for shyhip_push / shyhip_pull */

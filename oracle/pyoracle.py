@@ -97,7 +97,7 @@ class OracleModel:
     """The reference time step on one basin, in the oracle."""
 
     def __init__(self, basin, precision="f32", params=None, czv=2.5e-3, fcor=0.0, visv=0.0, difv=0.0,
-                 zconst=0.0, tracers=None):
+                 zconst=0.0, tracers=None, idtype=2):
         self.basin = b = basin
         self.prec = precision
         self.real = np.float32 if precision == "f32" else np.float64
@@ -144,7 +144,11 @@ class OracleModel:
         self.iextb = np.ascontiguousarray(b.iextb, dtype=np.int32)
         self.fcorv = np.full(nel, fcor, dtype=R)
         self.czv = np.full(nel, czv, dtype=R)
+        # horizontal diffusion weights (diffweight, the reference's default
+        # idtype 2); they act only where rkpar * difhv != 0
         self.wdifhv = np.zeros((nel, 3, 3), dtype=R)
+        if lib.oc_diffweight(nel, ptr(self.ev), idtype, ptr(self.wdifhv)) != 0:
+            raise RuntimeError("diffweight: error in matrix (idtype %d)" % idtype)
         self.areakv = np.zeros((nkn, nlvdi), dtype=R)
         self.tvd_ie = np.full((nel, 3, 3), -1, dtype=np.int32)
         self.tvd_x = np.zeros((nel, 3, 3), dtype=R)

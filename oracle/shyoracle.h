@@ -130,6 +130,7 @@ typedef struct {
 int  oc_set_threads(int n);
 /* geometry / topology (init) */
 void oc_set_ev(int nkn, int nel, const int *nen3v, const real *xgv, const real *ygv, double *ev);
+int  oc_diffweight(int nel, const double *ev, int idtype, real *wdifhv);
 int  oc_node_list(int nkn, int nel, const int *nen3v, int *ntg, int *cols, int maxnnz, int *ijp);
 int  oc_color(int nkn, int nel, const int *nen3v, int *color, int *color_ptr, int *color_elems);
 void oc_set_area(const oc_mesh *m, real *areakv);

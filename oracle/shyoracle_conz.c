@@ -706,7 +706,7 @@ int oc_scalar_stability(const oc_mesh *m, const oc_state *s, const oc_params *p,
         for (int l = jlevel; l <= ilevel; l++) {
             double us = az * A3E(s->utlnv, l, ie) + azt * A3E(s->utlov, l, ie);
             double vs = az * A3E(s->vtlnv, l, ie) + azt * A3E(s->vtlov, l, ie);
-            double rk3 = 3. * rkpar * (s->difhv ? A3E(s->difhv, l, ie) : (real)0);
+            double rk3 = (real)3 * rkpar * (s->difhv ? A3E(s->difhv, l, ie) : (real)0);
             double f[3], fwin[3], fwout[3], fd[3], wdiff[3];
             for (int ii = 1; ii <= 3; ii++) {
                 int k = kn[ii - 1];

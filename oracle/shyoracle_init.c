@@ -304,6 +304,74 @@ done:
     return ret;
 }
 
+/* diffweight: diffusion_admin.f90:182-340.  The horizontal diffusion
+ * weights wdifhv(iii,ii,ie) at [(iii-1) + 3(ii-1) + 9 ie]: idtype 0 the
+ * products b b + c c of the element's gradients; 1 the same with each
+ * positive off-diagonal weight moved onto the third vertex of its column;
+ * 2 -(2/3)/s^2 between the ends of each edge (s = ev(13+i), the length of
+ * the edge opposite vertex i) and the diagonal the negated row sum.  The
+ * checks of the reference (diagonal positive, off-diagonal not positive,
+ * symmetric and zero row sums within 1e-3) fail with 1 for idtype 1 and
+ * 2, as the reference stops; 0 otherwise. */
+int oc_diffweight(int nel, const double *ev, int idtype, real *wdifhv)
+{
+    const double fact = (double)(2.f / 3.f), eps = (double)1.e-3f;
+    int bad = 0;
+    for (int ie = 0; ie < nel; ie++) {
+        double b[3], c[3], bc[3][3];          /* bc[iii-1][ii-1] = bc(iii,ii) */
+        for (int ii = 0; ii < 3; ii++) {
+            b[ii] = ev[(3 + ii) + 19 * (size_t)ie];
+            c[ii] = ev[(6 + ii) + 19 * (size_t)ie];
+        }
+        for (int ii = 0; ii < 3; ii++)
+            for (int iii = 0; iii < 3; iii++) bc[iii][ii] = b[iii] * b[ii] + c[iii] * c[ii];
+        if (idtype == 1) {
+            for (int ii = 1; ii <= 3; ii++)
+                for (int iii = 1; iii <= 3; iii++) {
+                    const double w = bc[iii - 1][ii - 1];
+                    if (w > 0. && iii != ii) {
+                        const int i = 6 - ii - iii;
+                        bc[iii - 1][ii - 1] = 0.;
+                        bc[i - 1][ii - 1] = bc[i - 1][ii - 1] - w;
+                    }
+                }
+        } else if (idtype == 2) {
+            for (int i = 1; i <= 3; i++) {
+                const int ia = 1 + i % 3, ib = 1 + (i + 1) % 3;
+                const double s = ev[(12 + i) + 19 * (size_t)ie];
+                double w = 1. / (s * s);
+                w = w * fact;
+                bc[ia - 1][ib - 1] = -w;
+                bc[ib - 1][ia - 1] = -w;
+                bc[i - 1][i - 1] = 0.;
+            }
+            for (int ii = 1; ii <= 3; ii++) {
+                double w = 0.;
+                for (int iii = 1; iii <= 3; iii++) w = w + bc[ii - 1][iii - 1];
+                bc[ii - 1][ii - 1] = -w;
+            }
+        }
+        int berror = 0;
+        for (int ii = 1; ii <= 3; ii++) {
+            double wacu = 0.;
+            for (int iii = 1; iii <= 3; iii++) {
+                const double w = bc[ii - 1][iii - 1];
+                wacu = wacu + w;
+                if (ii == iii) {
+                    if (w <= 0) berror = 1;
+                } else if (w > 0) {
+                    berror = 1;
+                }
+                if (fabs(w - bc[iii - 1][ii - 1]) > eps) berror = 1;
+                wdifhv[(ii - 1) + 3 * (iii - 1) + 9 * (size_t)ie] = (real)w;
+            }
+            if (fabs(wacu) > eps) berror = 1;
+        }
+        if (berror && idtype != 0) bad = 1;
+    }
+    return bad;
+}
+
 /* set_area: elems_dealing.f90:658-700 (areafv = 12*ev(10)/3 summed per layer) */
 void oc_set_area(const oc_mesh *m, real *areakv)
 {

@@ -301,7 +301,7 @@ int shyhip_create(shyhip_model **out, int precision, int nkn, int nel, int nlvdi
     p.hzmin = 0.01f; p.hzoff = 0.05f; p.hzon = 0.10f; p.volmin = 1.f; p.iclin = 0;
     p.rtol = (double)1.0e-8f; p.atol = (double)1.0e-8f; p.itermax = 1000; p.refine_rtol = 1e-10;
     p.block_rows = 0; p.tri_sweeps = 0; p.inner_rtol = 1e-5; p.rkpar = 0.f; p.difmol = 1.0e-6f; p.itvd = 0; p.itvdv = 0;
-    p.node_tile = 0;
+    p.node_tile = 0; p.idtype = 2;
     const size_t ne = nel, nk = nkn, L = nlvdi;
     State &s = M.s;
     int rc = 0;
@@ -360,6 +360,7 @@ int shyhip_destroy(shyhip_model *h) {
     for (void *q : {(void *)M.m.bndo_open, (void *)M.m.bndo_lmax, (void *)M.m.bndo_eptr, (void *)M.m.bndo_el,
                     (void *)M.m.bndo_elev, (void *)M.m.bndo_nblev, (void *)M.m.bndo_eaj})
         dfree(q);
+    dfree(M.m.wdif);
     Mesh &m = M.m;
     for (void *q : {(void *)m.nen3v, (void *)m.geo, (void *)m.ang, (void *)m.dis, (void *)m.ilhv, (void *)m.ilhkv,
                     m.hm3v, m.hev, m.hlv, m.hldv, m.areakv, (void *)m.ibnd, (void *)m.zbound, (void *)m.iextb,
@@ -426,7 +427,8 @@ static int check_params(Model *M, const shyhip_params *p) {
         return fail(M, "ireib = 8/9 (roughness from the sediment or fluid-mud modules) is not supported by the device path");
     if (p->ireib < 0 || (p->ireib > 7 && p->ireib != 10))
         return fail(M, "unknown friction (ireib), as bottom_friction stops (chezy.f90:373-376)");
-    if (p->rkpar != 0.f) return fail(M, "horizontal diffusion (rkpar != 0) is not supported by the device path");
+    if (!(p->rkpar >= 0.f)) return fail(M, "rkpar must be zero or positive");
+    if (p->idtype < 0 || p->idtype > 2) return fail(M, "idtype must be 0, 1 or 2");
     if (p->itvd < 0 || p->itvd > 2 || p->itvdv < 0 || p->itvdv > 1) return fail(M, "itvd must be 0, 1 or 2, itvdv 0 or 1");
     if (p->block_rows < 0 || p->block_rows > kMaxBlockRows) return fail(M, "block_rows must be in 0..1024");
     if (p->tri_sweeps < 0 || p->tri_sweeps > 64) return fail(M, "tri_sweeps must be in 0..64");
@@ -471,6 +473,7 @@ int shyhip_set_basin(shyhip_model *h, const int *nen3v, const float *xgv, const 
     M.h_ilhkv.assign(ilhkv, ilhkv + nkn);
     M.h_ev.assign(19 * (size_t)nel, 0.);
     host_set_ev(nkn, nel, M.h_nen3v.data(), xgv, ygv, M.h_ev.data());
+    M.m.wdif_idtype = -1;                   // diffusion weights of the new geometry
     for (int ie = 0; ie < nel; ie++)
         if (!(M.h_ev[19 * (size_t)ie + 9] > 0.)) return fail(&M, "element " + std::to_string(ie + 1) + " is not counter-clockwise");
     M.nnz = host_node_list(nkn, nel, M.h_nen3v.data(), M.h_rowptr, M.h_cols, M.h_ijp);
@@ -1583,6 +1586,25 @@ int shyhip_hydro(shyhip_model *h, double dtime) {
     return hydro_impl(h->M, dtime, true);
 }
 
+// The horizontal diffusion weights for the current idtype, built when a
+// transport or stability check first runs with rkpar > 0 (host_diffweight,
+// diffusion_admin.f90:182-340; the reference stops on weights that break
+// its checks, and so does this)
+static int ensure_wdif(Model &M) {
+    if (!(M.p.rkpar > 0.f)) return 0;
+    if (M.m.wdif && M.m.wdif_idtype == M.p.idtype) return 0;
+    if (M.h_ev.size() != 19 * (size_t)M.nel) return fail(&M, "horizontal diffusion needs the basin (shyhip_set_basin)");
+    std::vector<double> w;
+    const int bad = host_diffweight(M.nel, M.h_ev.data(), M.p.idtype, w);
+    if (bad)
+        return fail(&M, "diffweight: error in the weights of element " + std::to_string(bad) + " (idtype " +
+                            std::to_string(M.p.idtype) + ")");
+    if (!M.m.wdif && dalloc_s(M, &M.m.wdif, w.size())) return 1;
+    if (upload_s(M, M.m.wdif, w.data(), w.size())) return 1;
+    M.m.wdif_idtype = M.p.idtype;
+    return 0;
+}
+
 // istot of scal3sh: the caller's (1..1000), or for istot <= 0 the
 // reference's choice, scalar_stability with rstol = 1 (the STR default),
 // stopping above istot_max = 1000 (concentration.f90:678-700)
@@ -1600,7 +1622,7 @@ static int resolve_istot(Model &M, int *istot) {
     if (nmem > 1) M.cur_mem_istot = M.d_mem_istot + (size_t)(M.pipe_k & 1) * nmem;
     std::vector<int> ist(nmem);
     std::vector<double> ri(nmem);
-    if (launch_scalar_stability(M, 1.0f, ist.data(), ri.data())) return 1;
+    if (ensure_wdif(M) || launch_scalar_stability(M, 1.0f, ist.data(), ri.data())) return 1;
     int mx = 1, mn = 1 << 30;
     for (int m = 0; m < nmem; m++) {
         if (ist[m] > 1000 || ist[m] < 1)
@@ -1665,7 +1687,7 @@ int shyhip_scalar_stability(shyhip_model *h, float rstol, int *istot, double *ri
     if (nmem > 1) M.cur_mem_istot = M.d_mem_istot + (size_t)(M.pipe_k & 1) * nmem;
     std::vector<int> ist(nmem);
     std::vector<double> ri(nmem);
-    if (launch_scalar_stability(M, rstol, ist.data(), ri.data())) return 1;
+    if (ensure_wdif(M) || launch_scalar_stability(M, rstol, ist.data(), ri.data())) return 1;
     *istot = *std::max_element(ist.begin(), ist.end());
     *rindex = *std::max_element(ri.begin(), ri.end());
     return 0;
@@ -1677,6 +1699,7 @@ static int scal3sh_on(Model &M, const TracerSet &ts, int istot, hipStream_t st, 
         if (ts.itr[q] < 0 || ts.itr[q] >= M.ntr) return fail(&M, "tracer index out of range");
     if (istot < 1 || istot > 1000) return fail(&M, "istot must be in 1..1000");
     if (M.p.itvd == 2 && !M.tvd_ready) return fail(&M, "itvd = 2 needs shyhip_tvd_init");
+    if (ensure_wdif(M)) return 1;
     auto substeps = [&]() -> int {
         for (int isact = 1; isact <= istot; isact++) {
             // the step's element-layer records before the first substep
@@ -1738,6 +1761,7 @@ int shyhip_conz3d(shyhip_model *h, int itr, const float *cbound, int istot, int 
     if (itr < 0 || itr >= M.ntr) return fail(&M, "tracer index out of range");
     if (istot < 1 || istot > 1000 || isact < 1 || isact > istot) return fail(&M, "conz3d: need 1 <= isact <= istot <= 1000");
     if (M.p.itvd == 2 && !M.tvd_ready) return fail(&M, "itvd = 2 needs shyhip_tvd_init");
+    if (ensure_wdif(M)) return 1;
     void *cb = nullptr;
     if (cbound) {
         if (!M.d_cbound && dalloc_s(M, &M.d_cbound, (size_t)M.nlvdi * M.nkn)) return 1;

@@ -69,6 +69,10 @@ struct ConzArgs {
     // the step's element-layer records (k_conz_prep): int and double columns
     int *rec_i;
     double *rec_d;
+    // horizontal diffusion (rkpar > 0): factor, element diffusivities
+    // [l*nel + ie] and the weights [(3*(ii-1) + (iii-1))*nel + ie]
+    float rkpar;
+    const S *difhv, *wdif;
     // itvd = 1: the node-layer list, the element-order incidence lists and
     // the node gradients [((l-1)*NT + t)*2 + xy][nkn] (after hbuf's terms)
     const int *nitem, *inc_cnt, *inc_el, *nk_el;
@@ -334,7 +338,10 @@ __global__ __launch_bounds__(kThreads) void k_tvd_grad_node(ConzArgs<S> a) {
 }
 
 // GRAD: itvd = 1 (the upstream node's gradient in place of the upwind point)
-template <typename S, int NT, bool GRAD>
+// HDIF: horizontal diffusion (rkpar > 0): the element's term
+// rk3 hmed wdiff (concentration_omp.f90:560-570,727-735) is taken off the
+// advective term before it is stored
+template <typename S, int NT, bool GRAD, bool HDIF>
 __global__ __launch_bounds__(kThreads) void k_conz_elem(ConzArgs<S> a) {
     using R = S;
     const int i = xcd_block() * kThreads + threadIdx.x;
@@ -458,11 +465,44 @@ __global__ __launch_bounds__(kThreads) void k_conz_elem(ConzArgs<S> a) {
             }
         }
     }
+    double hd[NT][3];
+#pragma unroll
+    for (int t = 0; t < NT; t++) hd[t][0] = hd[t][1] = hd[t][2] = 0.;
+    if (HDIF) {
+        // rk3 = 3 rkpar difhv in the tracers' precision, hmed the element
+        // layer's thickness at this substep, wdiff(ii) = sum over iii of
+        // wdifhv(iii, ii) c(iii)
+        const int ie = ri[(size_t)RI_IE * n + i], nel = a.nel;
+        const size_t o = (size_t)(l - 1) * nel + ie;
+        double rso = a.rso;
+        if (a.mem_istot) rso = (a.isact - 1) / (double)a.mem_istot[a.elem_mem[ie]];
+        const double rk3 = (double)((R)3 * (R)a.rkpar * a.difhv[o]);
+        const double hmed = rso * (double)a.hden[o] + (1. - rso) * (double)a.hdeo[o];
+        double wv[3][3];
+#pragma unroll
+        for (int ii = 0; ii < 3; ii++)
+#pragma unroll
+            for (int q = 0; q < 3; q++) wv[ii][q] = (double)a.wdif[(size_t)(3 * ii + q) * nel + ie];
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            double cl[3];
+#pragma unroll
+            for (int q = 0; q < 3; q++) cl[q] = (double)clip(t, cr[t][q]);
+#pragma unroll
+            for (int ii = 0; ii < 3; ii++) {
+                double waux = 0.;
+#pragma unroll
+                for (int q = 0; q < 3; q++) waux = waux + wv[ii][q] * cl[q];
+                hd[t][ii] = rk3 * hmed * waux;
+            }
+        }
+    }
 #pragma unroll
     for (int t = 0; t < NT; t++)
 #pragma unroll
         for (int ii = 0; ii < 3; ii++)
-            a.hbuf[((size_t)(l - 1) * NT + t) * a.nslot + so[ii]] = (S)(3. * fl[t][ii]);
+            a.hbuf[((size_t)(l - 1) * NT + t) * a.nslot + so[ii]] = (S)(HDIF ? 3. * fl[t][ii] - hd[t][ii]
+                                                                               : 3. * fl[t][ii]);
 }
 
 // ------------------------------------------------------------------ node phase
@@ -792,6 +832,7 @@ static int conz_nl(Model &M, const TracerSet &ts, int istot, int isact, hipStrea
         launch_timed(M, T_CONZ_PREP, k_conz_prep<S>, dim3(ge), dim3(kThreads), 0, st, a);
         HIPCHK(M, hipGetLastError());
     }
+    if (M.p.rkpar > 0.f && !a.wdif) return hip_check(M, hipErrorNotInitialized, "conz3d: no diffusion weights");
     if (a.itvd == 1) {
         if (!M.m.conz_nitem) return hip_check(M, hipErrorNotInitialized, "conz3d: no node-layer list");
         if (!a.grad) return hip_check(M, hipErrorInvalidValue, "conz3d: no gradient buffer");
@@ -799,9 +840,11 @@ static int conz_nl(Model &M, const TracerSet &ts, int istot, int isact, hipStrea
         launch_timed(M, T_TVD_GRAD, k_tvd_grad_elem<S, NT>, dim3(ge), dim3(kThreads), 0, st, a);
         launch_timed(M, T_TVD_GRAD, k_tvd_grad_node<S, NT>, dim3(gn), dim3(kThreads), 0, st, a);
         HIPCHK(M, hipGetLastError());
-        launch_timed(M, T_CONZ_ELEM, k_conz_elem<S, NT, true>, dim3(ge), dim3(kThreads), 0, st, a);
+        if (a.wdif) launch_timed(M, T_CONZ_ELEM, k_conz_elem<S, NT, true, true>, dim3(ge), dim3(kThreads), 0, st, a);
+        else launch_timed(M, T_CONZ_ELEM, k_conz_elem<S, NT, true, false>, dim3(ge), dim3(kThreads), 0, st, a);
     } else {
-        launch_timed(M, T_CONZ_ELEM, k_conz_elem<S, NT, false>, dim3(ge), dim3(kThreads), 0, st, a);
+        if (a.wdif) launch_timed(M, T_CONZ_ELEM, k_conz_elem<S, NT, false, true>, dim3(ge), dim3(kThreads), 0, st, a);
+        else launch_timed(M, T_CONZ_ELEM, k_conz_elem<S, NT, false, false>, dim3(ge), dim3(kThreads), 0, st, a);
     }
     HIPCHK(M, hipGetLastError());
     launch_timed(M, T_CONZ_NODE, k_conz_node<S, NL, NT>, dim3(M.m.n_ntile), dim3(kThreads), 0, st, a);
@@ -846,6 +889,11 @@ static ConzArgs<S> conz_args(Model &M, const TracerSet &ts, int istot, int isact
     a.hbuf = (S *)hbuf;
     a.err = M.d_err;
     a.eitem = M.m.conz_eitem; a.n_eitem = M.m.n_eitem; a.ntile = M.m.conz_ntile;
+    if (M.p.rkpar > 0.f) {
+        a.rkpar = M.p.rkpar;
+        a.difhv = (const S *)M.s.difhv;
+        a.wdif = (const S *)M.m.wdif;
+    }
     a.nitem = M.m.conz_nitem; a.n_nitem = M.m.n_nitem;
     a.inc_cnt = M.m.inc_cnt; a.inc_el = M.m.inc_el; a.nk_el = M.m.nk_el;
     // the node gradients follow the element terms in the same buffer
@@ -1140,9 +1188,9 @@ __device__ inline unsigned long long stab_ratio(int l, int k, int nkn, int nel, 
                                                 const double *geo, double az, double ad, double aa, const S *utn,
                                                 const S *uto, const S *vtn, const S *vto, const S *wln,
                                                 const S *difv, const S *hdeo, const S *hdkn, const S *hdko,
-                                                const S *mflux) {
+                                                const S *mflux, float rkpar, const S *difhv, const S *wdif) {
     const double azt = 1. - az, adt = 1. - ad, aat = 1. - aa;
-    double chigh = 0., cn = 0., cdiag = 0.;
+    double chigh = 0., cn = 0., cdiag = 0., clow = 0.;
     double co = 0.;
     if (mflux) {
         const S q = mflux[(size_t)(l - 1) * nkn + k];
@@ -1157,6 +1205,7 @@ __device__ inline unsigned long long stab_ratio(int l, int k, int nkn, int nel, 
         int ie[kStabBatch], iv[kStabBatch], ilev[kStabBatch], kj[kStabBatch][3];
         double aj4[kStabBatch], us[kStabBatch], vs[kStabBatch], hdv[kStabBatch], hdt[kStabBatch], hdb[kStabBatch];
         double gb[kStabBatch][3], gc[kStabBatch][3], wt[kStabBatch][3], wb[kStabBatch][3];
+        double rk3[kStabBatch], wd[kStabBatch];
 #pragma unroll
         for (int b = 0; b < kStabBatch; b++) {
             const int e = nk_el[min(pb + b, p1 - 1)];
@@ -1174,6 +1223,14 @@ __device__ inline unsigned long long stab_ratio(int l, int k, int nkn, int nel, 
             hdv[b] = (double)hdeo[le];
             hdt[b] = l > 1 ? (double)hdeo[le - nel] : 0.;
             hdb[b] = l < nlv ? (double)hdeo[le + nel] : 0.;
+            rk3[b] = 0.;
+            wd[b] = 0.;
+            if (wdif) {
+                // horizontal diffusion: 3 rkpar difhv in the tracers'
+                // precision and the vertex's own weight wdifhv(ii, ii)
+                rk3[b] = (double)((S)3 * (S)rkpar * difhv[le]);
+                wd[b] = (double)wdif[(size_t)(4 * iv[b]) * nel + el];
+            }
 #pragma unroll
             for (int jj = 0; jj < 3; jj++) {
                 kj[b][jj] = nen3v[3 * el + jj];
@@ -1215,11 +1272,12 @@ __device__ inline unsigned long long stab_ratio(int l, int k, int nkn, int nel, 
             if (l > 1) hmtop = 2. * dtop / (hdt[b] + hdv[b]);
             if (below) hmbot = 2. * dbot / (hdv[b] + hdb[b]);
             cn = cn + aj4[b] * (adt * (hmtop + hmbot));
+            clow = clow + aj4[b] * rk3[b] * ho * wd[b];
             cdiag = cdiag + aj4[b] * (ho < hn ? ho : hn);
         }
     }
     if (cdiag > 0.) {
-        const double r = ((chigh + 0.) + cn + co) / cdiag;
+        const double r = ((chigh + clow) + cn + co) / cdiag;
         if (r > 0.) return (unsigned long long)__double_as_longlong(r);
     }
     return 0ull;
@@ -1233,6 +1291,7 @@ __global__ void __launch_bounds__(kThreads) k_conz_stab(int nkn, int nel, int nl
                                                         double az, double ad, double aa, const S *utn, const S *uto,
                                                         const S *vtn, const S *vto, const S *wln, const S *difv,
                                                         const S *hdeo, const S *hdkn, const S *hdko, const S *mflux,
+                                                        float rkpar, const S *difhv, const S *wdif,
                                                         const int *node_mem, unsigned long long *dmax) {
     const int i = xcd_block() * blockDim.x + threadIdx.x;
     unsigned long long val = 0ull;    // this node-layer's ratio as bits (0: none)
@@ -1242,7 +1301,7 @@ __global__ void __launch_bounds__(kThreads) k_conz_stab(int nkn, int nel, int nl
         const int k = (int)(w >> 5), l = (int)(w & 31u) + 1;
         if (node_mem) key = node_mem[k];
         if (!zbound[k]) val = stab_ratio(l, k, nkn, nel, nlv, ilhv, nk_ptr, nk_el, nen3v, geo, az, ad, aa, utn, uto,
-                                         vtn, vto, wln, difv, hdeo, hdkn, hdko, mflux);
+                                         vtn, vto, wln, difv, hdeo, hdkn, hdko, mflux, rkpar, difhv, wdif);
     }
     // the wavefront's maximum when its node-layers belong to one member
     // (always, for a single model); lanes of mixed waves go one by one
@@ -1296,6 +1355,7 @@ __global__ void k_conz_stab_final(unsigned long long *w, int nmem, S dt, float r
 // per member (nmem = 1: the model): istot[m] and rindex[m] on the host
 template <typename S>
 static int stab_t(Model &M, float rstol, int *istot, double *rindex) {
+    if (M.p.rkpar > 0.f && !M.m.wdif) return hip_check(M, hipErrorNotInitialized, "scalar_stability: no diffusion weights");
     const int nmem = M.nmem_dev > 1 ? M.nmem_dev : 1;
     unsigned long long *w = nmem > 1 ? M.d_mem_stab : (unsigned long long *)(M.d_err + 8);
     const int n = M.m.n_nitem;
@@ -1304,7 +1364,8 @@ static int stab_t(Model &M, float rstol, int *istot, double *rindex) {
                        (double)M.p.azpar, (double)M.p.adpar, (double)M.p.aapar, (const S *)M.s.utn,
                        (const S *)M.s.uto, (const S *)M.s.vtn, (const S *)M.s.vto, (const S *)M.s.wln,
                        (const S *)M.s.difv, (const S *)M.s.hdeo, (const S *)M.s.hdkn, (const S *)M.s.hdko,
-                       M.has_mflux ? (const S *)M.s.mflux : (const S *)nullptr,
+                       M.has_mflux ? (const S *)M.s.mflux : (const S *)nullptr, M.p.rkpar,
+                       (const S *)M.s.difhv, M.p.rkpar > 0.f ? (const S *)M.m.wdif : (const S *)nullptr,
                        nmem > 1 ? M.d_node_mem : nullptr, w);
     HIPCHK(M, hipGetLastError());
     hipLaunchKernelGGL(k_conz_stab_final<S>, dim3((nmem + 63) / 64), dim3(64), 0, M.stream, w, nmem, (S)M.p.dt,

@@ -10,6 +10,8 @@
 //   host_tvd_tables    upwind points of the itvd=2 scheme with their
 //                      interpolation weights, src/shyfem/tvd_admin.f90:361-474,
 //                      src/utils/shyutil/evgeom.f90:531-660
+//   host_diffweight    horizontal diffusion weights of the scalars,
+//                      src/shyfem/diffusion_admin.f90:182-340
 //   host_block_schedule  block-ILU(0) pattern and level sets (no reference
 //                      counterpart: replaces ILUT's sequential factorisation)
 #include <algorithm>
@@ -119,6 +121,62 @@ int host_node_list(int nkn, int nel, const int *nen3v0, std::vector<int> &ntg, s
 // Greedy colouring in a rotating sweep; a node's colour bitmask forbids two
 // elements of one colour sharing a node; then elements move from the largest
 // to the smallest colour until no move is possible.
+// Per element the 3x3 weights w(a, b) of horizontal diffusion (wdifhv):
+// idtype 0 the products of the vertex gradients, grad(phi_a) . grad(phi_b);
+// 1 those with each positive off-diagonal weight of a column b moved onto
+// the column's third vertex; 2 the weight -(2/3)/s^2 between the two ends
+// of each edge (s its length, ev 14-16) and on the diagonal the negated sum
+// of the row.  The reference's 2/3 and tolerance are single-precision
+// constants.  Out: w[(3 (b-1) + (a-1)) nel + ie] (column b contiguous for
+// an element's three vertices).  Returns the first element (1-based) whose
+// weights break the reference's checks -- diagonal positive, off-diagonal
+// not positive, symmetric and zero row sums within the tolerance -- for
+// idtype 1 and 2, where the reference stops; 0 when all pass.
+int host_diffweight(int nel, const double *ev, int idtype, std::vector<double> &w) {
+    const double two_thirds = (double)(2.f / 3.f), tol = (double)1.e-3f;
+    w.assign(9 * (size_t)nel, 0.);
+    int first_bad = 0;
+    for (int ie = 0; ie < nel; ie++) {
+        const double *e = ev + 19 * (size_t)ie;
+        double m[3][3];                       // m[a][b] = w(a+1, b+1)
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) m[a][b] = e[3 + a] * e[3 + b] + e[6 + a] * e[6 + b];
+        if (idtype == 1) {
+            for (int b = 0; b < 3; b++)
+                for (int a = 0; a < 3; a++) {
+                    const double v = m[a][b];
+                    if (a != b && v > 0.) {
+                        m[a][b] = 0.;
+                        m[3 - a - b][b] -= v;
+                    }
+                }
+        } else if (idtype == 2) {
+            for (int v = 0; v < 3; v++) {             // the edge opposite vertex v
+                const int p = (v + 1) % 3, q = (v + 2) % 3;
+                const double len = e[13 + v];
+                const double x = (1. / (len * len)) * two_thirds;
+                m[p][q] = -x;
+                m[q][p] = -x;
+                m[v][v] = 0.;
+            }
+            for (int a = 0; a < 3; a++) m[a][a] = -(m[a][0] + m[a][1] + m[a][2]);
+        }
+        bool bad = false;
+        for (int a = 0; a < 3; a++) {
+            double row = 0.;
+            for (int b = 0; b < 3; b++) {
+                row += m[a][b];
+                if (a == b ? m[a][b] <= 0. : m[a][b] > 0.) bad = true;
+                if (std::fabs(m[a][b] - m[b][a]) > tol) bad = true;
+                w[(size_t)(3 * b + a) * nel + ie] = m[a][b];
+            }
+            if (std::fabs(row) > tol) bad = true;
+        }
+        if (bad && idtype != 0 && first_bad == 0) first_bad = ie + 1;
+    }
+    return first_bad;
+}
+
 int host_color(int nkn, int nel, const int *nen3v0, std::vector<int> &color) {
     std::vector<int> col(nel + 1, 0);
     std::vector<uint32_t> nodemask(nkn, 0u);

@@ -114,6 +114,10 @@ struct Mesh {                 // device copies of init-time data
     int *bndo_open = nullptr, *bndo_lmax = nullptr, *bndo_eptr = nullptr, *bndo_el = nullptr, *bndo_elev = nullptr;
     int *bndo_nblev = nullptr;
     double *bndo_eaj = nullptr;
+    // horizontal diffusion weights of the scalars (host_diffweight), built
+    // for wdif_idtype when a transport first runs with rkpar > 0
+    void *wdif = nullptr;
+    int wdif_idtype = -1;
 };
 
 struct State {
@@ -368,6 +372,7 @@ void host_set_ev(int nkn, int nel, const int *nen3v0, const float *x, const floa
 int host_node_list(int nkn, int nel, const int *nen3v0, std::vector<int> &ntg, std::vector<int> &cols,
                    std::vector<int> &ijp);
 int host_color(int nkn, int nel, const int *nen3v0, std::vector<int> &color);
+int host_diffweight(int nel, const double *ev, int idtype, std::vector<double> &w);
 int host_bndo_tables(int nkn, int nel, const int *nen3v0, const float *x, const float *y, const double *ev,
                      const std::vector<int> &bnodes, std::vector<double> &nx, std::vector<double> &ny,
                      std::vector<int> &ptr, std::vector<int> &nbr, std::vector<double> &w, bool f32);

@@ -7,9 +7,10 @@
 ! the device, pull cn1 back.  The model's tracer slot shy_conz_slot
 ! (shyhip_shyfem) holds the concentration meanwhile.
 !
-! Supported as on the device path: horizontal diffusion (rkpar), nudging
-! (robs), settling (wsink) and loads (rload) zero.  Any other value ends the
-! run with an error stop, as the reference stops on options it cannot run.
+! Supported as on the device path: nudging (robs), settling (wsink) and
+! loads (rload) zero.  Any other value ends the run with an error stop, as
+! the reference stops on options it cannot run.  Horizontal diffusion takes
+! rkpar and difhv (pushed with each call) and the model's idtype.
 ! With itvd = 1 the device computes the node gradients of cn1 itself (the
 ! gradxv, gradyv that tvd_grad_3d computed from the same cn1 just before
 ! this call, concentration.f90:744), so gradxv and gradyv are not read.  azpar, adpar, aapar, difmol, itvd and itvdv are taken
@@ -30,7 +31,7 @@ subroutine conz3d_omp(cn1, co1, ddt, rkpar, difhv, difv, difmol, cbound, itvd, i
   real, intent(inout), target :: cn1(nlvddi,*)
   real, intent(in) :: co1(nlvddi,*)
   real, intent(in), target :: cbound(nlvddi,*)
-  real, intent(in) :: difhv(nlvddi,*)
+  real, intent(in), target :: difhv(nlvddi,*)
   real, intent(in) :: gradxv(nlvddi,*), gradyv(nlvddi,*)
   real, intent(in) :: cobs(nlvddi,*), rtauv(nlvddi,*)
   real, intent(inout) :: load(nlvddi,*)
@@ -41,7 +42,6 @@ subroutine conz3d_omp(cn1, co1, ddt, rkpar, difhv, difv, difmol, cbound, itvd, i
   integer(c_int) :: nkn, nel, nlvdi, nlv, ntr
   logical :: change
 
-  if (rkpar /= 0.) stop 'error stop conz3d_omp: horizontal diffusion (rkpar /= 0) is not on the device'
   if (robs /= 0.) stop 'error stop conz3d_omp: nudging (robs /= 0) is not on the device'
   if (wsink /= 0.) stop 'error stop conz3d_omp: settling (wsink /= 0) is not on the device'
   if (rload /= 0.) stop 'error stop conz3d_omp: loads (rload /= 0) are not on the device'
@@ -53,7 +53,7 @@ subroutine conz3d_omp(cn1, co1, ddt, rkpar, difhv, difv, difmol, cbound, itvd, i
   call shyhip_check(shy, shyhip_get_params(shy, p), 'conz3d_omp')
   if (real(p%dt) /= ddt) stop 'error stop conz3d_omp: ddt differs from the time step of the device model'
   change = p%azpar /= azpar .or. p%adpar /= adpar .or. p%aapar /= aapar .or. p%difmol /= difmol &
-           .or. p%itvd /= itvd .or. p%itvdv /= itvdv
+           .or. p%itvd /= itvd .or. p%itvdv /= itvdv .or. p%rkpar /= rkpar
   if (change) then
     p%azpar = azpar
     p%adpar = adpar
@@ -61,12 +61,15 @@ subroutine conz3d_omp(cn1, co1, ddt, rkpar, difhv, difv, difmol, cbound, itvd, i
     p%difmol = difmol
     p%itvd = itvd
     p%itvdv = itvdv
+    p%rkpar = rkpar
     call shyhip_check(shy, shyhip_set_params(shy, p), 'conz3d_omp')
   end if
   call shyhip_check(shy, shyhip_push(shy, SHYHIP_TRACER0 + shy_conz_slot, c_loc(cn1), SHYHIP_F32, &
                                      int(nlvddi, c_size_t) * nkn), 'conz3d_omp')
   call shyhip_check(shy, shyhip_push(shy, SHYHIP_DIFV, c_loc(difv), SHYHIP_F32, &
                                      int(nlvddi + 1, c_size_t) * nkn), 'conz3d_omp')
+  if (rkpar /= 0.) call shyhip_check(shy, shyhip_push(shy, SHYHIP_DIFHV, c_loc(difhv), SHYHIP_F32, &
+                                                      int(nlvddi, c_size_t) * nel), 'conz3d_omp')
   call shyhip_check(shy, shyhip_conz3d(shy, shy_conz_slot, c_loc(cbound), istot, isact), 'conz3d_omp')
   call shyhip_check(shy, shyhip_pull(shy, SHYHIP_TRACER0 + shy_conz_slot, c_loc(cn1), SHYHIP_F32, &
                                      int(nlvddi, c_size_t) * nkn), 'conz3d_omp')

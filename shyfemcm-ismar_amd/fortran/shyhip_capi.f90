@@ -49,6 +49,7 @@ module shyhip_capi
     real(c_float)  :: rkpar, difmol
     integer(c_int) :: itvd, itvdv
     integer(c_int) :: node_tile
+    integer(c_int) :: idtype
   end type shyhip_params
 
   ! wind conversion settings of shyhip_set_wind_series

@@ -44,14 +44,15 @@ class ShyParams(C.Structure):
                 ("iclin", C.c_int), ("rtol", C.c_double), ("atol", C.c_double), ("itermax", C.c_int),
                 ("refine_rtol", C.c_double), ("block_rows", C.c_int), ("tri_sweeps", C.c_int),
                 ("inner_rtol", C.c_double), ("rkpar", C.c_float),
-                ("difmol", C.c_float), ("itvd", C.c_int), ("itvdv", C.c_int), ("node_tile", C.c_int)]
+                ("difmol", C.c_float), ("itvd", C.c_int), ("itvdv", C.c_int), ("node_tile", C.c_int),
+                ("idtype", C.c_int)]
 
 
 DEFAULTS = dict(dt=300.0, azpar=0.5, ampar=0.5, afpar=0.5, atpar=1.0, adpar=1.0, aapar=0.0, grav=9.81,
                 rowass=1025.0, vismol=1.0e-6, ireib=5, uvmin=0.2, hzmin=0.01, hzoff=0.05, hzon=0.10, volmin=1.0,
                 iclin=0, rtol=float(np.float32(1.0e-8)), atol=float(np.float32(1.0e-8)), itermax=1000,
                 refine_rtol=1.0e-10, block_rows=0, tri_sweeps=0, inner_rtol=1.0e-5, rkpar=0.0, difmol=1.0e-6, itvd=0, itvdv=0,
-                node_tile=0)
+                node_tile=0, idtype=2)
 
 # wind conversion settings (shyhip_wind_params; STR defaults, def_para.f90:782-799)
 class WindParams(C.Structure):

@@ -113,8 +113,8 @@ def test_params_rejected_without_gpu():
     """Options the device path does not implement are refused before any
     run, as the reference stops on an unknown option: friction ireib 8/9
     (roughness from the sediment and fluid-mud modules) and unknown ireib
-    (bottom_friction, chezy.f90:373-376), itvd outside 0..2, rkpar != 0,
-    aapar != 0 with itvdv > 0.  Every implemented friction law and TVD
+    (bottom_friction, chezy.f90:373-376), itvd outside 0..2, negative
+    rkpar, idtype outside 0..2, aapar != 0 with itvdv > 0.  Every implemented friction law and TVD
     scheme is accepted."""
     import shyhip
     for ireib in (0, 1, 2, 3, 4, 5, 6, 7, 10):
@@ -123,7 +123,7 @@ def test_params_rejected_without_gpu():
         shyhip.check_params(dict(itvd=itvd, itvdv=1))
     for bad, text in [(dict(ireib=8), "ireib = 8/9"), (dict(ireib=9), "ireib = 8/9"),
                       (dict(ireib=11), "unknown friction"), (dict(ireib=-1), "unknown friction"),
-                      (dict(itvd=3), "itvd must be"), (dict(rkpar=0.2), "rkpar"),
+                      (dict(itvd=3), "itvd must be"), (dict(rkpar=-0.2), "rkpar"), (dict(idtype=3), "idtype"),
                       (dict(itvdv=1, aapar=0.5), "aapar"), (dict(dt=0.0), "dt must be positive")]:
         with pytest.raises(shyhip.ShyhipError, match=text):
             shyhip.check_params(bad)
