@@ -105,7 +105,13 @@ enum {
      * bcvel.f90:332-338; not pushed: the previous step's) */
     SHYHIP_PPV, SHYHIP_ZEQV, SHYHIP_ANDGZV, SHYHIP_IUVFIX, SHYHIP_UTFIX, SHYHIP_VTFIX,
     SHYHIP_TRACER0 = 100,      /* tracer itr: SHYHIP_TRACER0 + itr, node 3D            */
-    SHYHIP_TRACER_BC0 = 200    /* its open-boundary value (-999 = none), node 3D        */
+    SHYHIP_TRACER_BC0 = 200,   /* its open-boundary value (-999 = none), node 3D        */
+    /* tracer itr's own terms (conz3d_omp's cobs, rtauv, wsinkv, load),
+     * absent until first pushed; see shyhip_set_tracer_terms */
+    SHYHIP_COBS0 = 300,        /* nudging target, node 3D                               */
+    SHYHIP_RTAUV0 = 400,       /* nudging rate (1/s), node 3D                           */
+    SHYHIP_WSINKV0 = 500,      /* sinking profile, node interfaces 0..nlvdi             */
+    SHYHIP_LOAD0 = 600         /* load, node 3D (deposition writes what it took)        */
 };
 /* dtype of host buffers */
 enum { SHYHIP_F32 = 0, SHYHIP_F64 = 1, SHYHIP_I32 = 2 };
@@ -314,6 +320,16 @@ int  shyhip_step(shyhip_model *h, double dtime, int istot);
  * by scalar_stability with rstol = 1 (concentration.f90:678-700; more than
  * 1000 is an error, status 3); shyhip_step does the same for all tracers. */
 int  shyhip_scal3sh(shyhip_model *h, int itr, int istot, int clip_small);
+/* conz3d_omp's own terms of tracer itr (concentration_omp.f90:65-151):
+ * nudging robs * rtauv towards cobs (explicit, :697-701), sinking
+ * wsink * wsinkv subtracted from the vertical velocity (:479) and loads
+ * rload * load (erosion, or deposition bounded by the node's content,
+ * :816-834); each acts once its field is pushed (SHYHIP_COBS0 ..
+ * SHYHIP_LOAD0 + itr) and its factor is non-zero.  scalar_stability
+ * includes the tracer's nudging and sinking, so shyhip_scal3sh and a step
+ * transport such a tracer alone with its own substeps (the reference's
+ * scal3sh per tracer); a step's other tracers keep theirs. */
+int  shyhip_set_tracer_terms(shyhip_model *h, int itr, float robs, float wsink, float rload);
 /* conz3d_omp (src/shyfem/concentration_omp.f90:65-340): one substep isact of
  * istot of tracer itr on the device state -- the element and node phases
  * only; the clipping of small values and the open-boundary treatment around

@@ -45,8 +45,12 @@ _STATE_FIELDS = ["znv", "zov", "zenv", "zeov", "utlnv", "vtlnv", "utlov", "vtlov
                  "tauxnv", "tauynv", "ppv", "zeqv", "andgzv", "iuvfix", "rzv", "rqv", "mfluxv"]
 
 
-class OcState(C.Structure):
-    _fields_ = [(n, _P) for n in _STATE_FIELDS] + [("dtime", C.c_double), ("dtime_old", C.c_double)]
+def _state_class(real_t):
+    class S(C.Structure):
+        _fields_ = [(n, _P) for n in _STATE_FIELDS] + [("dtime", C.c_double), ("dtime_old", C.c_double),
+                                                      ("robs", real_t), ("wsink", real_t), ("rload", real_t),
+                                                      ("cobs", _P), ("rtauv", _P), ("wsinkv", _P), ("load", _P)]
+    return S
 
 
 _PARAM_FIELDS = [("dt", C.c_double), ("azpar", None), ("ampar", None), ("afpar", None), ("atpar", None),
@@ -205,7 +209,8 @@ class OracleModel:
         a["tauxnv"] = z(nkn); a["tauynv"] = z(nkn)
         a["rzv"] = np.full(nkn, FLAG, dtype=R)
         a["rqv"] = z(nkn)
-        self.state = OcState(**{n: ptr(a[n]) if n in a else None for n in _STATE_FIELDS})
+        self.state = _state_class(self.real_c)(**{n: ptr(a[n]) if n in a else None for n in _STATE_FIELDS})
+        self.tracer_terms = {}
         self.stats = OcStats()
         self.tracers = {}
         self.t = 0.0
@@ -287,8 +292,11 @@ class OracleModel:
         the current state: utlnv, vtlnv, ddxv, ddyv, rfricv."""
         self.lib.oc_hydro_transports(C.byref(self.mesh), C.byref(self.state), C.byref(self.params))
 
-    def scalar_stability(self, dt=None, rstol=1.0):
-        """scalar_stability on the current state: (istot, rindex)."""
+    def scalar_stability(self, dt=None, rstol=1.0, tracer=None):
+        """scalar_stability on the current state: (istot, rindex), with the
+        nudging and sinking of `tracer` (set_tracer_terms) if given."""
+        if tracer is not None:
+            self._apply_terms(tracer)
         ri = self.real_c(0)
         dt = self.params_dict["dt"] if dt is None else dt
         istot = self.lib.oc_scalar_stability(C.byref(self.mesh), C.byref(self.state), C.byref(self.params),
@@ -340,6 +348,23 @@ class OracleModel:
         self.set_forcing("iuvfix", iuvfix)
         self.uvfix = None if utfix is None else (np.asarray(utfix), np.asarray(vtfix))
 
+    def set_tracer_terms(self, name, robs=0.0, cobs=None, rtauv=None, wsink=0.0, wsinkv=None, rload=0.0,
+                         load=None):
+        """conz3d_omp's terms of one scalar: nudging robs * rtauv towards
+        cobs ([nkn, nlv]), sinking wsink * wsinkv ([nkn, nlv+1]), loads
+        rload * load ([nkn, nlv], updated by deposition)."""
+        R = self.real
+        arr = lambda x: None if x is None else np.ascontiguousarray(x, dtype=R).copy()  # noqa: E731
+        self.tracer_terms[name] = dict(robs=robs, wsink=wsink, rload=rload, cobs=arr(cobs), rtauv=arr(rtauv),
+                                       wsinkv=arr(wsinkv), load=arr(load))
+
+    def _apply_terms(self, name):
+        t = self.tracer_terms.get(name, {})
+        st = self.state
+        st.robs, st.wsink, st.rload = t.get("robs", 0.0), t.get("wsink", 0.0), t.get("rload", 0.0)
+        for n in ("cobs", "rtauv", "wsinkv", "load"):
+            setattr(st, n, ptr(t.get(n)))
+
     def set_baroclinic(self, temp="temp", salt="salt"):
         """ibarcl = 1: barocl(0)'s start (rhov = 0, rhoset three times,
         baroclinic.f90:398-404); then every step adds the baroclinic term to
@@ -380,6 +405,7 @@ class OracleModel:
         if rc != 0:
             raise RuntimeError(f"oracle hydro failed with code {rc}")
         for name, arr in self.tracers.items():
+            self._apply_terms(name)
             rc = lib.oc_scal3sh(m, s, p, ptr(arr), ptr(self.tracer_bc[name]), istot, 0 if name == "temp" else 1)
             if rc != 0:
                 raise RuntimeError(f"oracle scal3sh({name}) failed with code {rc}")

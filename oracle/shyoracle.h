@@ -100,6 +100,12 @@ typedef struct {
     real *rqv;                   /* [nkn] may be NULL */
     real *mfluxv;                /* [nlvdi*nkn] may be NULL */
     double dtime, dtime_old;     /* setweg time bookkeeping */
+    /* the terms of the scalar being transported (conz3d_omp's robs, cobs,
+     * rtauv, wsink, wsinkv, rload, load; zero / NULL: none) */
+    real robs, wsink, rload;
+    real *cobs, *rtauv;          /* [nlvdi*nkn] */
+    real *wsinkv;                /* [(nlvdi+1)*nkn] */
+    real *load;                  /* [nlvdi*nkn], updated by deposition */
 } oc_state;
 
 typedef struct {

@@ -8,8 +8,8 @@
  * bnd_routines.f90 make_scal_flux and the substep loop of
  * concentration.f90 scal3sh.  Element contributions are accumulated colour
  * by colour, in the colouring's element order, exactly as conz3d_omp does.
- * Nudging (robs = 0), sinking (wsink = 0) and loads (rload = 0) are zero for
- * the supported configurations and omitted.
+ * The scalar's own terms -- nudging (robs, rtauv, cobs), sinking (wsink,
+ * wsinkv) and loads (rload, load) -- come with the state (oc_state).
  */
 #include <math.h>
 #include <stdlib.h>
@@ -306,7 +306,8 @@ int oc_conz3d(const oc_mesh *m, const oc_state *s, const oc_params *p,
                 H2(hold, l, ii) = rso * hn + rsot * ho;
                 H2(hnew, l, ii) = rsn * hn + rsnt * ho;
                 H2(cl, l, ii) = A3K(cn1, l, k);
-                H2(wl, l, ii) = AW(s->wlnv, l, k);
+                /* sinking enters the vertical velocity (concentration_omp.f90:479) */
+                H2(wl, l, ii) = AW(s->wlnv, l, k) - (s->wsinkv ? s->wsink * AW(s->wsinkv, l, k) : (real)0);
             }
         }
         for (int ii = 1; ii <= 3; ii++) H2(wl, ilevel, ii) = 0.;
@@ -386,8 +387,13 @@ int oc_conz3d(const oc_mesh *m, const oc_state *s, const oc_params *p,
             for (int ii = 1; ii <= 3; ii++) {
                 int k = kn[ii - 1];
                 double hmed = haver[l];
+                /* nudging towards cobs, explicit (an = 0: ant = 1, finu = 0;
+                 * concentration_omp.f90:203-208,697-701) */
+                double fnudge = 0.;
+                if (s->cobs && s->rtauv)
+                    fnudge = s->robs * (double)A3K(s->rtauv, l, k) * ((double)A3K(s->cobs, l, k) - 1. * H2(cl, l, ii));
                 double cexpl = aj4 * (H2(hold, l, ii) * H2(cl, l, ii) +
-                                      dt * (0. + 3. * fl[ii - 1] - fw[ii - 1] -
+                                      dt * (H2(hold, l, ii) * fnudge + 3. * fl[ii - 1] - fw[ii - 1] -
                                             rk3 * hmed * wdiff[ii - 1] - fd[ii - 1]));
                 double alow = aj4 * dt * H2(clm, l, ii);
                 double ahigh = aj4 * dt * H2(clp, l, ii);
@@ -414,6 +420,27 @@ int oc_conz3d(const oc_mesh *m, const oc_state *s, const oc_params *p,
             if (qflux < 0. && m->ibnd && m->ibnd[k] > 0) cconz = A3K(cn1, l, k);
             double mflux = qflux * cconz;
             A3K(cn, l, k) += dt * mflux;
+            /* loads: erosion or, negative, deposition bounded by what the
+             * node holds (concentration_omp.f90:816-834) */
+            if (s->load) {
+                double loading = (double)(s->rload * A3K(s->load, l, k));
+                if (loading == 0.) {
+                } else if (loading < 0.) {
+                    double cload = 0.;
+                    if (A3K(cn, l, k) > 0.) {
+                        cload = -dt * loading;
+                        cload = A3K(cn, l, k) * (1. - exp(-cload / A3K(cn, l, k)));
+                    } else if (A3K(cn, l, k) < 0.) {
+                        A3K(cn, l, k) = 0.;
+                    }
+                    if (cload > A3K(cn, l, k)) ret |= 4;
+                    loading = -cload / dt;
+                    if (s->rload > 0) A3K(s->load, l, k) = (real)(loading / s->rload);
+                    A3K(cn, l, k) += dt * loading;
+                } else {
+                    A3K(cn, l, k) += dt * loading;
+                }
+            }
         }
         if ((aa == 0. && ad == 0.) || m->nlv == 1) {
             for (int l = jlevel; l <= ilevel; l++)
@@ -698,7 +725,7 @@ int oc_scalar_stability(const oc_mesh *m, const oc_state *s, const oc_params *p,
                 double hn = A3K(s->hdknv, l, k), ho = A3K(s->hdkov, l, k);
                 H2(hold, l, ii) = rso * hn + rsot * ho;
                 H2(hnew, l, ii) = rsn * hn + rsnt * ho;
-                H2(wl, l, ii) = AW(s->wlnv, l, k);
+                H2(wl, l, ii) = AW(s->wlnv, l, k) - (s->wsinkv ? s->wsink * AW(s->wsinkv, l, k) : (real)0);
             }
         }
         for (int l = ilevel + 1; l <= m->nlv; l++) presentl[l] = 0.;
@@ -741,6 +768,7 @@ int oc_scalar_stability(const oc_mesh *m, const oc_state *s, const oc_params *p,
                 A3K(chigh, l, k) += cadv;
                 A3K(clow, l, k) += chdiff;
                 A3K(cn, l, k) += cvdiff;
+                if (s->rtauv) A3K(co, l, k) += dtt * aj4 * hmed * s->robs * (double)A3K(s->rtauv, l, k);
             }
         }
         for (int l = jlevel; l <= ilevel; l++)

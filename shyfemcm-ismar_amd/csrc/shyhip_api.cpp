@@ -96,6 +96,14 @@ int timer_flush(Model &M) {
     return 0;
 }
 
+// true when tracer itr has nudging, sinking or loads switched on with their
+// fields pushed
+bool tracer_has_terms(const Model &M, int itr) {
+    if (itr < 0 || itr >= M.ntr) return false;
+    return (M.tr_robs[itr] != 0.f && M.s.cobs[itr] && M.s.rtauv[itr]) ||
+           (M.tr_wsink[itr] != 0.f && M.s.wsinkv[itr]) || (M.tr_rload[itr] != 0.f && M.s.load[itr]);
+}
+
 }  // namespace shy
 
 namespace {
@@ -218,6 +226,22 @@ bool field_of(Model &M, int id, Field &f) {
     case SHYHIP_UTFIX: f = Field{&M.s.utfix, K_ELEM3, false, false, true}; return true;
     case SHYHIP_VTFIX: f = Field{&M.s.vtfix, K_ELEM3, false, false, true}; return true;
     default:
+        if (id >= SHYHIP_COBS0 && id < SHYHIP_COBS0 + M.ntr) {
+            f = Field{&M.s.cobs[id - SHYHIP_COBS0], K_NODE3, false, false, true};
+            return true;
+        }
+        if (id >= SHYHIP_RTAUV0 && id < SHYHIP_RTAUV0 + M.ntr) {
+            f = Field{&M.s.rtauv[id - SHYHIP_RTAUV0], K_NODE3, false, false, true};
+            return true;
+        }
+        if (id >= SHYHIP_WSINKV0 && id < SHYHIP_WSINKV0 + M.ntr) {
+            f = Field{&M.s.wsinkv[id - SHYHIP_WSINKV0], K_NODEW, false, false, true};
+            return true;
+        }
+        if (id >= SHYHIP_LOAD0 && id < SHYHIP_LOAD0 + M.ntr) {
+            f = Field{&M.s.load[id - SHYHIP_LOAD0], K_NODE3, false, false, true};
+            return true;
+        }
         if (id >= SHYHIP_TRACER0 && id < SHYHIP_TRACER0 + M.ntr) return S(&M.s.tracer[id - SHYHIP_TRACER0], K_NODE3);
         if (id >= SHYHIP_TRACER_BC0 && id < SHYHIP_TRACER_BC0 + M.ntr)
             return S(&M.s.tracer_bc[id - SHYHIP_TRACER_BC0], K_NODE3);
@@ -319,6 +343,9 @@ int shyhip_create(shyhip_model **out, int precision, int nkn, int nel, int nlvdi
     rc |= dalloc_s(M, &s.hbuf, conz_hbuf_elems(M, 1));
     s.tracer.assign(ntracer, nullptr);
     s.tracer_bc.assign(ntracer, nullptr);
+    s.cobs.assign(ntracer, nullptr); s.rtauv.assign(ntracer, nullptr);
+    s.wsinkv.assign(ntracer, nullptr); s.load.assign(ntracer, nullptr);
+    M.tr_robs.assign(ntracer, 0.f); M.tr_wsink.assign(ntracer, 0.f); M.tr_rload.assign(ntracer, 0.f);
     for (int t = 0; t < ntracer; t++) {
         rc |= dalloc_s(M, &s.tracer[t], L * nk);
         rc |= dalloc_s(M, &s.tracer_bc[t], L * nk);
@@ -355,6 +382,8 @@ int shyhip_destroy(shyhip_model *h) {
     for (void *q : {s.ppv, s.zeqv, s.andgz, (void *)s.iuvfix, s.utfix, s.vtfix}) dfree(q);
     for (void *t : s.tracer) dfree(t);
     for (void *t : s.tracer_bc) dfree(t);
+    for (auto *v : {&s.cobs, &s.rtauv, &s.wsinkv, &s.load})
+        for (void *t : *v) dfree(t);
     dfree(M.m.bndo_k); dfree(M.m.bndo_ptr); dfree(M.m.bndo_nb); dfree(M.m.bndo_nx); dfree(M.m.bndo_ny);
     dfree(M.m.bndo_w);
     for (void *q : {(void *)M.m.bndo_open, (void *)M.m.bndo_lmax, (void *)M.m.bndo_eptr, (void *)M.m.bndo_el,
@@ -1787,10 +1816,27 @@ int shyhip_bndo_setbc(shyhip_model *h, int itr) {
     return check_dev_err(M, "bndo_setbc");
 }
 
+int shyhip_set_tracer_terms(shyhip_model *h, int itr, float robs, float wsink, float rload) {
+    if (!h) return fail(nullptr, "null handle");
+    Model &M = h->M;
+    if (pipe_drain(M)) return 1;
+    if (itr < 0 || itr >= M.ntr) return fail(&M, "tracer index out of range");
+    if (!std::isfinite(robs) || !std::isfinite(wsink) || !std::isfinite(rload))
+        return fail(&M, "tracer terms must be finite");
+    M.tr_robs[itr] = robs;
+    M.tr_wsink[itr] = wsink;
+    M.tr_rload[itr] = rload;
+    return 0;
+}
+
 int shyhip_scal3sh(shyhip_model *h, int itr, int istot, int clip_small) {
     if (!h) return fail(nullptr, "null handle");
     if (pipe_drain(h->M)) return 1;
-    if (resolve_istot(h->M, &istot)) return 1;
+    if (itr < 0 || itr >= h->M.ntr) return fail(&h->M, "tracer index out of range");
+    h->M.terms_tracer = tracer_has_terms(h->M, itr) ? itr : -1;
+    const int rc = resolve_istot(h->M, &istot);
+    h->M.terms_tracer = -1;
+    if (rc) return 1;
     TracerSet ts;
     ts.itr[0] = itr;
     ts.clip[0] = clip_small;
@@ -1842,9 +1888,13 @@ int shyhip_step(shyhip_model *h, double dtime, int istot) {
     Model &M = h->M;
     // the forcing series must cover dtime; checked before the step starts
     if ((M.f.nseries > 0 || M.f.wind_on) && forcing_check(M, dtime)) return 2;
+    // tracers with their own terms (nudging, sinking, loads) are each
+    // transported alone with their own substeps, as scal3sh does per tracer
+    bool terms = false;
+    for (int t = 0; t < M.ntr; t++) terms = terms || tracer_has_terms(M, t);
     // the transport overlaps the next step's hydro unless that hydro needs
     // this step's density (baroclinic runs)
-    const bool pipe = M.deferred && !M.timing && !M.has_mflux && M.ntr >= 1 && !M.bcl_on;
+    const bool pipe = M.deferred && !M.timing && !M.has_mflux && M.ntr >= 1 && !M.bcl_on && !terms;
     int par = 0;
     if (!pipe) {
         if (pipe_drain(M)) return 1;
@@ -1866,6 +1916,24 @@ int shyhip_step(shyhip_model *h, double dtime, int istot) {
     if ((M.f.nseries > 0 || M.f.wind_on) && launch_forcing(M, dtime)) return 1;
     if (hydro_impl(M, dtime, false)) return M.err.empty() ? 1 : 3;
     M.iweg_snap = nullptr;
+    if (terms) {
+        if (tracer_streams(M)) return 1;
+        for (int t = 0; t < M.ntr; t++) {
+            int ist = istot;
+            M.terms_tracer = tracer_has_terms(M, t) ? t : -1;
+            const int rc = resolve_istot(M, &ist);
+            M.terms_tracer = -1;
+            if (rc) return 1;
+            TracerSet ts;
+            ts.itr[0] = t;
+            ts.clip[0] = t == 0 ? 0 : 1;
+            if (scal3sh_on(M, ts, ist, M.stream, M.grp_hbuf[t / 2], M.grp_rec[t / 2], M.grp_bscal[t / 2])) return 1;
+        }
+        if (launch_rhoset(M, 1, M.stream)) return 1;
+        if (timer_flush(M)) return 1;
+        if (M.deferred && !M.timing) return 0;
+        return check_dev_err(M, "step");
+    }
     // one istot for all tracers: conzstab does not depend on the tracer
     if (M.ntr > 0 && resolve_istot(M, &istot)) return 1;
     if (pipe) {

@@ -69,6 +69,12 @@ struct ConzArgs {
     // the step's element-layer records (k_conz_prep): int and double columns
     int *rec_i;
     double *rec_d;
+    // the launch's single tracer's own terms (null: none): nudging factor
+    // robs with target cobs and rate rtauv, sinking wsink * wsinkv, loads
+    // rload * load (node arrays like the tracer; wsinkv on interfaces)
+    float robs, wsink, rload;
+    const S *cobs, *rtauv, *wsinkv;
+    S *load;
     // horizontal diffusion (rkpar > 0): factor, element diffusivities
     // [l*nel + ie] and the weights [(3*(ii-1) + (iii-1))*nel + ie]
     float rkpar;
@@ -521,7 +527,10 @@ __device__ inline double tvd_face(double conc, double cond, double conu, double 
 // the node_tile setting (<= kNodeTileMax = the workgroup size) and at most
 // kNodeItems node-layers, so that phase 1 is one pass of the workgroup
 // whatever the depth of the tile's columns.
-template <typename S, int NL, int NT>
+// EXT: the tracer's own terms (one tracer per launch): sinking in the
+// vertical velocity, nudging in the explicit term (concentration_omp.f90:
+// 479,697-701,727-735) and loads after the boundary flux (:816-834)
+template <typename S, int NL, int NT, bool EXT>
 __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
     using R = S;
     // phase-1 results by list position: right-hand sides of NT tracers and
@@ -613,8 +622,23 @@ __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
         }
         const double rvptop = (double)(R)(a.difv[(size_t)(l - 1) * nkn + k] + (R)a.difmol);
         const double rvpbot = (double)(R)(a.difv[(size_t)l * nkn + k] + (R)a.difmol);
-        const double wtop = (l == 1) ? 0. : (double)a.wln[(size_t)(l - 1) * nkn + k];
-        const double wbot = (double)a.wln[(size_t)l * nkn + k];
+        double wtop = (l == 1) ? 0. : (double)a.wln[(size_t)(l - 1) * nkn + k];
+        double wbot = (double)a.wln[(size_t)l * nkn + k];
+        double hfn = 0.;            // hold * fnudge of the tracer (EXT, NT = 1)
+        if (EXT) {
+            // the vertical velocity less the sinking, in the tracers' precision
+            if (a.wsinkv) {
+                if (l > 1)
+                    wtop = (double)(R)(a.wln[(size_t)(l - 1) * nkn + k] -
+                                       (R)a.wsink * a.wsinkv[(size_t)(l - 1) * nkn + k]);
+                wbot = (double)(R)(a.wln[(size_t)l * nkn + k] - (R)a.wsink * a.wsinkv[(size_t)l * nkn + k]);
+            }
+            if (a.cobs) {
+                const size_t o = (size_t)(l - 1) * nkn + k;
+                const double fn = (double)a.robs * (double)a.rtauv[o] * ((double)a.cobs[o] - 1. * ckw[0][2]);
+                hfn = hokw[2] * fn;
+            }
+        }
         // element terms of the three column truncations: bottom layer l
         // (v = 0), l+1 (v = 1) or deeper (v = 2; nothing below l+2 is read).
         // The truncation enters only through the layers below l: the
@@ -712,7 +736,7 @@ __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
             _Pragma("unroll") for (int t = 0; t < NT; t++) {                                       \
                 const double fw = deeper ? fw2[t] : (deep ? fw1[t] : fw0[t]);                      \
                 const double fd = deep ? fd1[t] : fd0[t];                                          \
-                const double cexpl = aj4 * (hoc * ckw[t][2] + dt * (0. + hfv[t] - fw - fd));       \
+                const double cexpl = aj4 * (hoc * ckw[t][2] + dt * (hfn + hfv[t] - fw - fd));      \
                 cn[t] += cexpl;                                                                    \
             }                                                                                      \
             clo += aj4 * dt * clm;                                                                 \
@@ -749,6 +773,26 @@ __global__ __launch_bounds__(kThreads) void k_conz_node(ConzArgs<S> a) {
                 double cconz = a.cbound[t] ? (double)a.cbound[t][o] : 0.;
                 if (qflux < 0. && a.ibnd[k] > 0) cconz = ckw[t][2];
                 cn[t] += dt * (qflux * cconz);
+            }
+        }
+        if (EXT && a.load) {
+            // erosion, or deposition of at most what the node-layer holds
+            const size_t o = (size_t)(l - 1) * nkn + k;
+            double loading = (double)((R)a.rload * a.load[o]);
+            if (loading < 0.) {
+                double cload = 0.;
+                if (cn[0] > 0.) {
+                    cload = -dt * loading;
+                    cload = cn[0] * (1. - exp(-cload / cn[0]));
+                } else if (cn[0] < 0.) {
+                    cn[0] = 0.;
+                }
+                if (cload > cn[0]) atomicCAS(a.err, 0, ERR_CONZ);
+                loading = -cload / dt;
+                if (a.rload > 0.f) a.load[o] = (S)(loading / (double)a.rload);
+                cn[0] += dt * loading;
+            } else if (loading > 0.) {
+                cn[0] += dt * loading;
             }
         }
 #pragma unroll
@@ -847,7 +891,12 @@ static int conz_nl(Model &M, const TracerSet &ts, int istot, int isact, hipStrea
         else launch_timed(M, T_CONZ_ELEM, k_conz_elem<S, NT, false, false>, dim3(ge), dim3(kThreads), 0, st, a);
     }
     HIPCHK(M, hipGetLastError());
-    launch_timed(M, T_CONZ_NODE, k_conz_node<S, NL, NT>, dim3(M.m.n_ntile), dim3(kThreads), 0, st, a);
+    if (a.cobs || a.wsinkv || a.load) {
+        if (NT != 1) return hip_check(M, hipErrorInvalidValue, "conz3d: tracer terms need one tracer per launch");
+        launch_timed(M, T_CONZ_NODE, k_conz_node<S, NL, 1, true>, dim3(M.m.n_ntile), dim3(kThreads), 0, st, a);
+    } else {
+        launch_timed(M, T_CONZ_NODE, k_conz_node<S, NL, NT, false>, dim3(M.m.n_ntile), dim3(kThreads), 0, st, a);
+    }
     HIPCHK(M, hipGetLastError());
     return 0;
 }
@@ -889,6 +938,22 @@ static ConzArgs<S> conz_args(Model &M, const TracerSet &ts, int istot, int isact
     a.hbuf = (S *)hbuf;
     a.err = M.d_err;
     a.eitem = M.m.conz_eitem; a.n_eitem = M.m.n_eitem; a.ntile = M.m.conz_ntile;
+    if (ts.n == 1 && tracer_has_terms(M, ts.itr[0])) {
+        const int t = ts.itr[0];
+        if (M.tr_robs[t] != 0.f && M.s.cobs[t] && M.s.rtauv[t]) {
+            a.robs = M.tr_robs[t];
+            a.cobs = (const S *)M.s.cobs[t];
+            a.rtauv = (const S *)M.s.rtauv[t];
+        }
+        if (M.tr_wsink[t] != 0.f && M.s.wsinkv[t]) {
+            a.wsink = M.tr_wsink[t];
+            a.wsinkv = (const S *)M.s.wsinkv[t];
+        }
+        if (M.tr_rload[t] != 0.f && M.s.load[t]) {
+            a.rload = M.tr_rload[t];
+            a.load = (S *)M.s.load[t];
+        }
+    }
     if (M.p.rkpar > 0.f) {
         a.rkpar = M.p.rkpar;
         a.difhv = (const S *)M.s.difhv;
@@ -1188,7 +1253,8 @@ __device__ inline unsigned long long stab_ratio(int l, int k, int nkn, int nel, 
                                                 const double *geo, double az, double ad, double aa, const S *utn,
                                                 const S *uto, const S *vtn, const S *vto, const S *wln,
                                                 const S *difv, const S *hdeo, const S *hdkn, const S *hdko,
-                                                const S *mflux, float rkpar, const S *difhv, const S *wdif) {
+                                                const S *mflux, float rkpar, const S *difhv, const S *wdif,
+                                                float robs, const S *rtauv, float wsink, const S *wsinkv) {
     const double azt = 1. - az, adt = 1. - ad, aat = 1. - aa;
     double chigh = 0., cn = 0., cdiag = 0., clow = 0.;
     double co = 0.;
@@ -1200,6 +1266,7 @@ __device__ inline unsigned long long stab_ratio(int l, int k, int nkn, int nel, 
     const double dtop = l > 1 ? (double)difv[(size_t)(l - 1) * nkn + k] : 0.;
     const double dbot = l < nlv ? (double)difv[(size_t)l * nkn + k] : 0.;
     const double hn = (double)hdkn[(size_t)(l - 1) * nkn + k], ho = (double)hdko[(size_t)(l - 1) * nkn + k];
+    const double rtau = rtauv ? (double)rtauv[(size_t)(l - 1) * nkn + k] : 0.;
     const int p0 = nk_ptr[k], p1 = nk_ptr[k + 1];
     for (int pb = p0; pb < p1; pb += kStabBatch) {
         int ie[kStabBatch], iv[kStabBatch], ilev[kStabBatch], kj[kStabBatch][3];
@@ -1244,6 +1311,15 @@ __device__ inline unsigned long long stab_ratio(int l, int k, int nkn, int nel, 
             for (int jj = 0; jj < 3; jj++) {
                 wt[b][jj] = l > 1 ? (double)wln[(size_t)(l - 1) * nkn + kj[b][jj]] : 0.;
                 wb[b][jj] = l < nlv ? (double)wln[(size_t)l * nkn + kj[b][jj]] : 0.;
+                if (wsinkv) {
+                    // less the tracer's sinking, in the tracers' precision
+                    if (l > 1)
+                        wt[b][jj] = (double)(S)(wln[(size_t)(l - 1) * nkn + kj[b][jj]] -
+                                                (S)wsink * wsinkv[(size_t)(l - 1) * nkn + kj[b][jj]]);
+                    if (l < nlv)
+                        wb[b][jj] = (double)(S)(wln[(size_t)l * nkn + kj[b][jj]] -
+                                                (S)wsink * wsinkv[(size_t)l * nkn + kj[b][jj]]);
+                }
             }
 #pragma unroll
         for (int b = 0; b < kStabBatch; b++) {
@@ -1273,6 +1349,7 @@ __device__ inline unsigned long long stab_ratio(int l, int k, int nkn, int nel, 
             if (below) hmbot = 2. * dbot / (hdv[b] + hdb[b]);
             cn = cn + aj4[b] * (adt * (hmtop + hmbot));
             clow = clow + aj4[b] * rk3[b] * ho * wd[b];
+            if (rtauv) co = co + aj4[b] * ho * (double)robs * rtau;      // nudging
             cdiag = cdiag + aj4[b] * (ho < hn ? ho : hn);
         }
     }
@@ -1291,7 +1368,8 @@ __global__ void __launch_bounds__(kThreads) k_conz_stab(int nkn, int nel, int nl
                                                         double az, double ad, double aa, const S *utn, const S *uto,
                                                         const S *vtn, const S *vto, const S *wln, const S *difv,
                                                         const S *hdeo, const S *hdkn, const S *hdko, const S *mflux,
-                                                        float rkpar, const S *difhv, const S *wdif,
+                                                        float rkpar, const S *difhv, const S *wdif, float robs,
+                                                        const S *rtauv, float wsink, const S *wsinkv,
                                                         const int *node_mem, unsigned long long *dmax) {
     const int i = xcd_block() * blockDim.x + threadIdx.x;
     unsigned long long val = 0ull;    // this node-layer's ratio as bits (0: none)
@@ -1301,7 +1379,8 @@ __global__ void __launch_bounds__(kThreads) k_conz_stab(int nkn, int nel, int nl
         const int k = (int)(w >> 5), l = (int)(w & 31u) + 1;
         if (node_mem) key = node_mem[k];
         if (!zbound[k]) val = stab_ratio(l, k, nkn, nel, nlv, ilhv, nk_ptr, nk_el, nen3v, geo, az, ad, aa, utn, uto,
-                                         vtn, vto, wln, difv, hdeo, hdkn, hdko, mflux, rkpar, difhv, wdif);
+                                         vtn, vto, wln, difv, hdeo, hdkn, hdko, mflux, rkpar, difhv, wdif, robs,
+                                         rtauv, wsink, wsinkv);
     }
     // the wavefront's maximum when its node-layers belong to one member
     // (always, for a single model); lanes of mixed waves go one by one
@@ -1356,6 +1435,14 @@ __global__ void k_conz_stab_final(unsigned long long *w, int nmem, S dt, float r
 template <typename S>
 static int stab_t(Model &M, float rstol, int *istot, double *rindex) {
     if (M.p.rkpar > 0.f && !M.m.wdif) return hip_check(M, hipErrorNotInitialized, "scalar_stability: no diffusion weights");
+    // the nudging and sinking of the tracer being transported, if any
+    const int t = M.terms_tracer;
+    float robs = 0.f, wsink = 0.f;
+    const void *rtauv = nullptr, *wsinkv = nullptr;
+    if (t >= 0 && t < M.ntr) {
+        if (M.tr_robs[t] != 0.f && M.s.cobs[t] && M.s.rtauv[t]) { robs = M.tr_robs[t]; rtauv = M.s.rtauv[t]; }
+        if (M.tr_wsink[t] != 0.f && M.s.wsinkv[t]) { wsink = M.tr_wsink[t]; wsinkv = M.s.wsinkv[t]; }
+    }
     const int nmem = M.nmem_dev > 1 ? M.nmem_dev : 1;
     unsigned long long *w = nmem > 1 ? M.d_mem_stab : (unsigned long long *)(M.d_err + 8);
     const int n = M.m.n_nitem;
@@ -1366,7 +1453,7 @@ static int stab_t(Model &M, float rstol, int *istot, double *rindex) {
                        (const S *)M.s.difv, (const S *)M.s.hdeo, (const S *)M.s.hdkn, (const S *)M.s.hdko,
                        M.has_mflux ? (const S *)M.s.mflux : (const S *)nullptr, M.p.rkpar,
                        (const S *)M.s.difhv, M.p.rkpar > 0.f ? (const S *)M.m.wdif : (const S *)nullptr,
-                       nmem > 1 ? M.d_node_mem : nullptr, w);
+                       robs, (const S *)rtauv, wsink, (const S *)wsinkv, nmem > 1 ? M.d_node_mem : nullptr, w);
     HIPCHK(M, hipGetLastError());
     hipLaunchKernelGGL(k_conz_stab_final<S>, dim3((nmem + 63) / 64), dim3(64), 0, M.stream, w, nmem, (S)M.p.dt,
                        rstol, nmem > 1 ? M.cur_mem_istot : nullptr);

@@ -134,6 +134,10 @@ struct State {
     void *rhov = nullptr;     // density anomaly rho - rowass [l*nkn + k] (baroclinic runs)
     std::vector<void *> tracer;
     std::vector<void *> tracer_bc;   // boundary values per tracer (-999 = none)
+    // each tracer's own terms (allocated when first pushed): nudging target
+    // cobs and rate rtauv [l*nkn + k], sinking profile wsinkv [l*nkn + k],
+    // l = 0..nlvdi, and loads [l*nkn + k] (updated by deposition)
+    std::vector<void *> cobs, rtauv, wsinkv, load;
     double *zel = nullptr;    // [6*nel] dbb dbc dcb dcc ut vt (hydro_zeta inputs)
     double *aval = nullptr;   // [nnz] assembled zeta matrix (FP64, as c2coo)
     void *zrow = nullptr;     // S [12*nel] element matrices hia (9) and vectors hik (3)
@@ -241,6 +245,11 @@ struct Model {
     int *h_pinned = nullptr;  // pinned host mirror for small readbacks
     double dtime = 0., dtime_old = -1.;
     bool has_coriolis = false, has_mflux = false, tvd_ready = false;
+    // each tracer's nudging factor, sinking velocity and load factor
+    // (conz3d_omp's robs, wsink, rload); the tracer whose terms the
+    // stability index includes (-1: none)
+    std::vector<float> tr_robs, tr_wsink, tr_rload;
+    int terms_tracer = -1;
     // baroclinic coupling (ibarcl = 1): tracers holding T and S
     bool bcl_on = false;
     int bcl_temp = -1, bcl_salt = -1;
@@ -442,6 +451,9 @@ int solver_solve(Model &M, int *iters, int *refinements, int wetdry_bnew = -1);
 // the tracers one conz3d launch transports together (1 or 2): slots, whether
 // small values are clipped (not temperature), source concentrations (device,
 // may be null)
+// true when tracer itr has nudging, sinking or loads switched on with
+// their fields pushed
+bool tracer_has_terms(const Model &M, int itr);
 struct TracerSet {
     int n = 1;
     int itr[2] = {0, 0};

@@ -7,10 +7,10 @@
 ! the device, pull cn1 back.  The model's tracer slot shy_conz_slot
 ! (shyhip_shyfem) holds the concentration meanwhile.
 !
-! Supported as on the device path: nudging (robs), settling (wsink) and
-! loads (rload) zero.  Any other value ends the run with an error stop, as
-! the reference stops on options it cannot run.  Horizontal diffusion takes
-! rkpar and difhv (pushed with each call) and the model's idtype.
+! Horizontal diffusion takes rkpar and difhv (pushed with each call) and the
+! model's idtype; nudging (robs, cobs, rtauv), settling (wsink, wsinkv) and
+! loads (rload, load) go to the slot's tracer terms, and the loads that
+! deposition leaves come back in load.
 ! With itvd = 1 the device computes the node gradients of cn1 itself (the
 ! gradxv, gradyv that tvd_grad_3d computed from the same cn1 just before
 ! this call, concentration.f90:744), so gradxv and gradyv are not read.  azpar, adpar, aapar, difmol, itvd and itvdv are taken
@@ -33,18 +33,15 @@ subroutine conz3d_omp(cn1, co1, ddt, rkpar, difhv, difv, difmol, cbound, itvd, i
   real, intent(in), target :: cbound(nlvddi,*)
   real, intent(in), target :: difhv(nlvddi,*)
   real, intent(in) :: gradxv(nlvddi,*), gradyv(nlvddi,*)
-  real, intent(in) :: cobs(nlvddi,*), rtauv(nlvddi,*)
-  real, intent(inout) :: load(nlvddi,*)
+  real, intent(in), target :: cobs(nlvddi,*), rtauv(nlvddi,*)
+  real, intent(inout), target :: load(nlvddi,*)
   real, intent(in), target :: difv(0:nlvddi,*)
-  real, intent(in) :: wsinkv(0:nlvddi,*)
+  real, intent(in), target :: wsinkv(0:nlvddi,*)
 
   type(shyhip_params) :: p
   integer(c_int) :: nkn, nel, nlvdi, nlv, ntr
   logical :: change
 
-  if (robs /= 0.) stop 'error stop conz3d_omp: nudging (robs /= 0) is not on the device'
-  if (wsink /= 0.) stop 'error stop conz3d_omp: settling (wsink /= 0) is not on the device'
-  if (rload /= 0.) stop 'error stop conz3d_omp: loads (rload /= 0) are not on the device'
 
 !$omp critical (shyhip_conz3d)
   call shyhip_check(shy, shyhip_get_sizes(shy, nkn, nel, nlvdi, nlv, ntr), 'conz3d_omp')
@@ -70,7 +67,20 @@ subroutine conz3d_omp(cn1, co1, ddt, rkpar, difhv, difv, difmol, cbound, itvd, i
                                      int(nlvddi + 1, c_size_t) * nkn), 'conz3d_omp')
   if (rkpar /= 0.) call shyhip_check(shy, shyhip_push(shy, SHYHIP_DIFHV, c_loc(difhv), SHYHIP_F32, &
                                                       int(nlvddi, c_size_t) * nel), 'conz3d_omp')
+  if (robs /= 0.) then
+    call shyhip_check(shy, shyhip_push(shy, SHYHIP_COBS0 + shy_conz_slot, c_loc(cobs), SHYHIP_F32, &
+                                       int(nlvddi, c_size_t) * nkn), 'conz3d_omp')
+    call shyhip_check(shy, shyhip_push(shy, SHYHIP_RTAUV0 + shy_conz_slot, c_loc(rtauv), SHYHIP_F32, &
+                                       int(nlvddi, c_size_t) * nkn), 'conz3d_omp')
+  end if
+  if (wsink /= 0.) call shyhip_check(shy, shyhip_push(shy, SHYHIP_WSINKV0 + shy_conz_slot, c_loc(wsinkv), &
+                                                      SHYHIP_F32, int(nlvddi + 1, c_size_t) * nkn), 'conz3d_omp')
+  if (rload /= 0.) call shyhip_check(shy, shyhip_push(shy, SHYHIP_LOAD0 + shy_conz_slot, c_loc(load), SHYHIP_F32, &
+                                                      int(nlvddi, c_size_t) * nkn), 'conz3d_omp')
+  call shyhip_check(shy, shyhip_set_tracer_terms(shy, shy_conz_slot, robs, wsink, rload), 'conz3d_omp')
   call shyhip_check(shy, shyhip_conz3d(shy, shy_conz_slot, c_loc(cbound), istot, isact), 'conz3d_omp')
+  if (rload /= 0.) call shyhip_check(shy, shyhip_pull(shy, SHYHIP_LOAD0 + shy_conz_slot, c_loc(load), SHYHIP_F32, &
+                                                      int(nlvddi, c_size_t) * nkn), 'conz3d_omp')
   call shyhip_check(shy, shyhip_pull(shy, SHYHIP_TRACER0 + shy_conz_slot, c_loc(cn1), SHYHIP_F32, &
                                      int(nlvddi, c_size_t) * nkn), 'conz3d_omp')
 !$omp end critical (shyhip_conz3d)

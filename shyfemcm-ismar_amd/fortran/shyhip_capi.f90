@@ -31,6 +31,7 @@ module shyhip_capi
   integer(c_int), parameter :: SHYHIP_PPV = 39, SHYHIP_ZEQV = 40, SHYHIP_ANDGZV = 41, SHYHIP_IUVFIX = 42
   integer(c_int), parameter :: SHYHIP_UTFIX = 43, SHYHIP_VTFIX = 44
   integer(c_int), parameter :: SHYHIP_TRACER0 = 100, SHYHIP_TRACER_BC0 = 200
+  integer(c_int), parameter :: SHYHIP_COBS0 = 300, SHYHIP_RTAUV0 = 400, SHYHIP_WSINKV0 = 500, SHYHIP_LOAD0 = 600
 
   type, bind(C) :: shyhip_params
     real(c_double) :: dt
@@ -249,6 +250,14 @@ module shyhip_capi
       import :: c_ptr, c_int
       type(c_ptr), value :: h
       integer(c_int), value :: itr, istot, clip_small
+      integer(c_int) :: rc
+    end function
+
+    function shyhip_set_tracer_terms(h, itr, robs, wsink, rload) bind(C, name="shyhip_set_tracer_terms") result(rc)
+      import :: c_ptr, c_int, c_float
+      type(c_ptr), value :: h
+      integer(c_int), value :: itr
+      real(c_float), value :: robs, wsink, rload
       integer(c_int) :: rc
     end function
 

@@ -304,6 +304,25 @@ class Model:
     def rhoset(self, niter=1):
         self._chk(self.lib.shyhip_rhoset(self.h, niter), "rhoset")
 
+    def set_tracer_terms(self, name, robs=0.0, cobs=None, rtauv=None, wsink=0.0, wsinkv=None, rload=0.0,
+                         load=None):
+        """conz3d_omp's own terms of one tracer: nudging robs * rtauv towards
+        cobs ([nkn, nlv]), sinking wsink * wsinkv ([nkn, nlv+1]), loads
+        rload * load ([nkn, nlv]; tracer_load() reads what deposition left)."""
+        i = self.tracer_names.index(name)
+        for base, arr in ((300, cobs), (400, rtauv), (500, wsinkv), (600, load)):
+            if arr is not None:
+                a = np.ascontiguousarray(arr, dtype=np.float64)
+                self._chk(self.lib.shyhip_push(self.h, base + i, _p(a), F64, C.c_size_t(a.size)), "push(terms)")
+        self._chk(self.lib.shyhip_set_tracer_terms(self.h, i, C.c_float(robs), C.c_float(wsink), C.c_float(rload)),
+                  "set_tracer_terms")
+
+    def tracer_load(self, name):
+        i = self.tracer_names.index(name)
+        out = np.zeros((self.basin.nkn, self.nlvdi))
+        self._chk(self.lib.shyhip_pull(self.h, 600 + i, _p(out), F64, C.c_size_t(out.size)), "pull(load)")
+        return out
+
     def set_sources(self, nodes, levmin=None, levmax=None):
         """Point sources of set_mass_flux: 0-based nodes in the reference's
         boundary loop order, with their boundary's levmin/levmax (0: the
