@@ -370,9 +370,9 @@ def ensemble_members(args):
     return out
 
 
-def run_ensemble(mems, prec, steps, warmup):
+def run_ensemble(mems, prec, steps, warmup, params=None):
     from shyhip import ensemble
-    ens = ensemble.Ensemble(mems, prec, params=PARAMS, visv=1e-3, difv=1e-4, deferred=True)
+    ens = ensemble.Ensemble(mems, prec, params=params or PARAMS, visv=1e-3, difv=1e-4, deferred=True)
     ens.set_level_series(PARAMS["dt"] * (warmup + steps + 2))     # each member's boundary file on the device
     for _ in range(warmup):
         ens.step(istot=ISTOT)
@@ -408,12 +408,12 @@ def ensemble_zeta_err(mems, r32, r64):
             "max_same_wetdry": float(s.max()) if s.size else None, "elements_wetdry_differ": differ}
 
 
-def run_members_alone(mems, prec, steps, warmup):
+def run_members_alone(mems, prec, steps, warmup, params=None):
     """The same members one after the other, each its own model (no batching)."""
     import shyhip
     tot_el, tot_t = 0, 0.0
     for mem in mems:
-        m = shyhip.Model(mem["basin"], prec, params=PARAMS, visv=1e-3, difv=1e-4, tracers=mem["tracers"])
+        m = shyhip.Model(mem["basin"], prec, params=params or PARAMS, visv=1e-3, difv=1e-4, tracers=mem["tracers"])
         m.set_deferred_checks(True)
         m.set_wind(*mem["wind"])
         rec_t = PARAMS["dt"] * np.arange(-2, warmup + steps + 4)
@@ -615,9 +615,10 @@ def main():
             out["baroclinic"] = {"error": str(e)}
     if not args.no_ensemble:
         mems = ensemble_members(args)
-        e32, t32, z32 = run_ensemble(mems, "fp32", args.ens_steps, 5)
-        e64, t64, z64 = run_ensemble(mems, "fp64", args.ens_steps, 5)
-        a64, ta64 = run_members_alone(mems, "fp64", max(10, args.ens_steps // 4), 3)
+        # every leg with the settings the autotuner picked for its precision
+        e32, t32, z32 = run_ensemble(mems, "fp32", args.ens_steps, 5, params["fp32"])
+        e64, t64, z64 = run_ensemble(mems, "fp64", args.ens_steps, 5, params["fp64"])
+        a64, ta64 = run_members_alone(mems, "fp64", max(10, args.ens_steps // 4), 3, params["fp64"])
         out["ensemble"] = {
             "workload": f"{len(mems)} lagoons of {args.ens_min // 1000}k-{args.ens_max // 1000}k nodes (log-uniform), "
                         f"{args.ens_layers} layers, own wind 0-10 m/s and tidal phase, T+S, dt=60 s",
@@ -628,6 +629,7 @@ def main():
             "batched_fp32_over_sequential_fp64": round(e32 / a64, 3),
             "batched_fp32_over_batched_fp64": round(e32 / e64, 3),
             "steps": args.ens_steps,
+            "settings": {p: {k: params[p].get(k, 0) for k in TUNE_GRID} for p in params},
             "zeta_err_batched_fp32_vs_fp64_m": ensemble_zeta_err(mems, z32, z64),
         }
     if not args.no_serving:
