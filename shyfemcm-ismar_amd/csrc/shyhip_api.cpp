@@ -543,19 +543,26 @@ int shyhip_set_basin(shyhip_model *h, const int *nen3v, const float *xgv, const 
         }
     }
     // element colouring of every member on its own (domain_clusterization_ggu)
+    // (members on the host's threads: each writes its own elements)
     M.h_color.assign(nel, 0);
+    const int nmemb = (int)M.mem_node.size() - 1;
+    std::vector<int> mem_nc(nmemb, 0);
+    parallel_for(nmemb, 1, [&](int m_begin, int m_end) {
+        for (int m = m_begin; m < m_end; m++) {
+            const int k0 = M.mem_node[m], e0 = M.mem_elem[m], ne = M.mem_elem[m + 1] - e0;
+            std::vector<int> loc(3 * (size_t)ne), col;
+            for (size_t i = 0; i < loc.size(); i++) loc[i] = M.h_nen3v[3 * (size_t)e0 + i] - k0;
+            mem_nc[m] = host_color(M.mem_node[m + 1] - k0, ne, loc.data(), col);
+            if (mem_nc[m] > 0) std::copy(col.begin(), col.end(), M.h_color.begin() + e0);
+        }
+    });
     M.ncolor = 0;
-    for (size_t m = 0; m + 1 < M.mem_node.size(); m++) {
-        const int k0 = M.mem_node[m], e0 = M.mem_elem[m], ne = M.mem_elem[m + 1] - e0;
-        std::vector<int> loc(3 * (size_t)ne), col;
-        for (size_t i = 0; i < loc.size(); i++) loc[i] = M.h_nen3v[3 * (size_t)e0 + i] - k0;
-        const int nc = host_color(M.mem_node[m + 1] - k0, ne, loc.data(), col);
-        if (nc <= 0) {
-            M.ncolor = nc;
+    for (int m = 0; m < nmemb; m++) {
+        if (mem_nc[m] <= 0) {
+            M.ncolor = mem_nc[m];
             break;
         }
-        std::copy(col.begin(), col.end(), M.h_color.begin() + e0);
-        M.ncolor = std::max(M.ncolor, nc);
+        M.ncolor = std::max(M.ncolor, mem_nc[m]);
     }
     if (M.ncolor <= 0) return fail(&M, "element colouring failed (more than 31 colours)");
     Mesh &m = M.m;

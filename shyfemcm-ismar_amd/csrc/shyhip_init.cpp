@@ -17,11 +17,29 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "shyhip_internal.h"
 
 namespace shy {
+
+// fn(begin, end) over [0, n) split into contiguous ranges on up to 16 host
+// threads (one when n is below `grain`); the ranges must write disjoint data
+void parallel_for(int n, int grain, const std::function<void(int, int)> &fn) {
+    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+    const int nt = std::max(1, std::min({16, hw, n / std::max(1, grain)}));
+    if (nt <= 1) {
+        fn(0, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; t++) {
+        const int b = (int)((long long)n * t / nt), e = (int)((long long)n * (t + 1) / nt);
+        pool.emplace_back([&fn, b, e] { fn(b, e); });
+    }
+    for (auto &th : pool) th.join();
+}
 
 // b and c coefficients are corrected so that each triple sums to zero
 // (evgeom.f90:1007-1042)
@@ -339,7 +357,9 @@ void host_tvd_tables(int nkn, int nel, const int *nen3v0, const float *xg, const
     tel.assign(6 * (size_t)nel, -1);
     tkn.assign(18 * (size_t)nel, 0);
     tw.assign(18 * (size_t)nel, 0.);
-    for (int ie = 0; ie < nel; ie++) {
+    // the elements are independent: ranges of them on the host's threads
+    parallel_for(nel, 4096, [&](int e_begin, int e_end) {
+    for (int ie = e_begin; ie < e_end; ie++) {
         for (int ic = 1; ic <= 3; ic++) {
             const int kc = nen3v0[3 * ie + ic - 1];
             for (int t = 0; t < 2; t++) {
@@ -360,6 +380,7 @@ void host_tvd_tables(int nkn, int nel, const int *nen3v0, const float *xg, const
             }
         }
     }
+    });
 }
 
 // Scalar open-boundary tables (bndo.f90:210-470): inward normal from the

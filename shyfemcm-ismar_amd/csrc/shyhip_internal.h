@@ -382,6 +382,7 @@ int host_node_list(int nkn, int nel, const int *nen3v0, std::vector<int> &ntg, s
                    std::vector<int> &ijp);
 int host_color(int nkn, int nel, const int *nen3v0, std::vector<int> &color);
 int host_diffweight(int nel, const double *ev, int idtype, std::vector<double> &w);
+void parallel_for(int n, int grain, const std::function<void(int, int)> &fn);
 int host_bndo_tables(int nkn, int nel, const int *nen3v0, const float *x, const float *y, const double *ev,
                      const std::vector<int> &bnodes, std::vector<double> &nx, std::vector<double> &ny,
                      std::vector<int> &ptr, std::vector<int> &nbr, std::vector<double> &w, bool f32);
