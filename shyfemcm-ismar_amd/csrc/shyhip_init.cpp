@@ -103,42 +103,56 @@ void host_set_ev(int nkn, int nel, const int *nen3v0, const float *xg, const flo
 
 int host_node_list(int nkn, int nel, const int *nen3v0, std::vector<int> &ntg, std::vector<int> &cols,
                    std::vector<int> &ijp) {
-    // adjacency through element edges, plus the node itself, sorted and unique
-    std::vector<std::vector<int>> adj(nkn);
+    // adjacency through element edges, plus the node itself, sorted and
+    // unique: every edge end and the node in one array by node (counted
+    // first), then each node's run sorted and compacted
+    std::vector<int> cnt(nkn + 1, 1);                      // the node itself
     for (int ie = 0; ie < nel; ie++)
         for (int ii = 0; ii < 3; ii++) {
-            int k1 = nen3v0[3 * ie + ii], k2 = nen3v0[3 * ie + (ii + 1) % 3];
-            adj[k1].push_back(k2);
-            adj[k2].push_back(k1);
+            cnt[nen3v0[3 * ie + ii]] += 1;
+            cnt[nen3v0[3 * ie + (ii + 1) % 3]] += 1;
         }
-    ntg.assign(nkn + 1, 0);
-    cols.clear();
-    for (int k = 0; k < nkn; k++) {
-        auto &a = adj[k];
-        a.push_back(k);
-        std::sort(a.begin(), a.end());
-        a.erase(std::unique(a.begin(), a.end()), a.end());
-        cols.insert(cols.end(), a.begin(), a.end());
-        ntg[k + 1] = (int)cols.size();
-    }
-    ijp.assign(9 * (size_t)nel, -1);
+    std::vector<int> start(nkn + 1, 0);
+    for (int k = 0; k < nkn; k++) start[k + 1] = start[k] + cnt[k];
+    std::vector<int> raw(start[nkn]), fill(start.begin(), start.end() - 1);
+    for (int k = 0; k < nkn; k++) raw[fill[k]++] = k;
     for (int ie = 0; ie < nel; ie++)
-        for (int i = 0; i < 3; i++) {
-            const int ki = nen3v0[3 * ie + i];
-            for (int j = 0; j < 3; j++) {
-                const int kj = nen3v0[3 * ie + j];
-                auto first = cols.begin() + ntg[ki], last = cols.begin() + ntg[ki + 1];
-                auto it = std::lower_bound(first, last, kj);
-                if (it == last || *it != kj) return -1;
-                ijp[9 * (size_t)ie + 3 * i + j] = (int)(it - cols.begin());
-            }
+        for (int ii = 0; ii < 3; ii++) {
+            const int k1 = nen3v0[3 * ie + ii], k2 = nen3v0[3 * ie + (ii + 1) % 3];
+            raw[fill[k1]++] = k2;
+            raw[fill[k2]++] = k1;
         }
+    std::vector<int> len(nkn);
+    parallel_for(nkn, 8192, [&](int k_begin, int k_end) {
+        for (int k = k_begin; k < k_end; k++) {
+            int *a = raw.data() + start[k];
+            std::sort(a, a + cnt[k]);
+            len[k] = (int)(std::unique(a, a + cnt[k]) - a);
+        }
+    });
+    ntg.assign(nkn + 1, 0);
+    for (int k = 0; k < nkn; k++) ntg[k + 1] = ntg[k] + len[k];
+    cols.resize(ntg[nkn]);
+    for (int k = 0; k < nkn; k++) std::copy(raw.begin() + start[k], raw.begin() + start[k] + len[k], cols.begin() + ntg[k]);
+    ijp.assign(9 * (size_t)nel, -1);
+    parallel_for(nel, 8192, [&](int e_begin, int e_end) {
+        for (int ie = e_begin; ie < e_end; ie++)
+            for (int i = 0; i < 3; i++) {
+                const int ki = nen3v0[3 * ie + i];
+                for (int j = 0; j < 3; j++) {
+                    const int kj = nen3v0[3 * ie + j];
+                    auto first = cols.begin() + ntg[ki], last = cols.begin() + ntg[ki + 1];
+                    auto it = std::lower_bound(first, last, kj);
+                    if (it == last || *it != kj) continue;       // cannot happen: kj is a neighbour
+                    ijp[9 * (size_t)ie + 3 * i + j] = (int)(it - cols.begin());
+                }
+            }
+    });
+    for (size_t q = 0; q < ijp.size(); q++)
+        if (ijp[q] < 0) return -1;
     return (int)cols.size();
 }
 
-// Greedy colouring in a rotating sweep; a node's colour bitmask forbids two
-// elements of one colour sharing a node; then elements move from the largest
-// to the smallest colour until no move is possible.
 // Per element the 3x3 weights w(a, b) of horizontal diffusion (wdifhv):
 // idtype 0 the products of the vertex gradients, grad(phi_a) . grad(phi_b);
 // 1 those with each positive off-diagonal weight of a column b moved onto
@@ -195,6 +209,9 @@ int host_diffweight(int nel, const double *ev, int idtype, std::vector<double> &
     return first_bad;
 }
 
+// Greedy colouring in a rotating sweep; a node's colour bitmask forbids two
+// elements of one colour sharing a node; then elements move from the largest
+// to the smallest colour until no move is possible.
 int host_color(int nkn, int nel, const int *nen3v0, std::vector<int> &color) {
     std::vector<int> col(nel + 1, 0);
     std::vector<uint32_t> nodemask(nkn, 0u);
@@ -310,7 +327,10 @@ void host_tvd_tables(int nkn, int nel, const int *nen3v0, const float *xg, const
     }
     const int nc = (int)std::sqrt((double)nel) + 1;
     const double dxc = (xmax - xmin) / nc * (1 + 1e-9) + 1e-9, dyc = (ymax - ymin) / nc * (1 + 1e-9) + 1e-9;
-    std::vector<std::vector<int>> cell((size_t)nc * nc);
+    // the elements of each cell (every cell its bounding box touches), in
+    // ascending element number, as one array by cell: counted, then filled
+    const size_t ncell = (size_t)nc * nc;
+    std::vector<int> cbox(4 * (size_t)nel);
     for (int ie = 0; ie < nel; ie++) {
         const int *kn = nen3v0 + 3 * ie;
         double bx0 = 1e300, bx1 = -1e300, by0 = 1e300, by1 = -1e300;
@@ -320,11 +340,22 @@ void host_tvd_tables(int nkn, int nel, const int *nen3v0, const float *xg, const
             by0 = std::min(by0, (double)yg[kn[ii]]);
             by1 = std::max(by1, (double)yg[kn[ii]]);
         }
-        const int cx0 = (int)std::floor((bx0 - xmin) / dxc), cx1 = (int)std::floor((bx1 - xmin) / dxc);
-        const int cy0 = (int)std::floor((by0 - ymin) / dyc), cy1 = (int)std::floor((by1 - ymin) / dyc);
-        for (int cy = cy0; cy <= cy1; cy++)
-            for (int cx = cx0; cx <= cx1; cx++) cell[(size_t)cx + (size_t)nc * cy].push_back(ie);
+        cbox[4 * (size_t)ie] = (int)std::floor((bx0 - xmin) / dxc);
+        cbox[4 * (size_t)ie + 1] = (int)std::floor((bx1 - xmin) / dxc);
+        cbox[4 * (size_t)ie + 2] = (int)std::floor((by0 - ymin) / dyc);
+        cbox[4 * (size_t)ie + 3] = (int)std::floor((by1 - ymin) / dyc);
     }
+    std::vector<int> cell_ptr(ncell + 1, 0);
+    for (int ie = 0; ie < nel; ie++)
+        for (int cy = cbox[4 * (size_t)ie + 2]; cy <= cbox[4 * (size_t)ie + 3]; cy++)
+            for (int cx = cbox[4 * (size_t)ie]; cx <= cbox[4 * (size_t)ie + 1]; cx++)
+                cell_ptr[(size_t)cx + (size_t)nc * cy + 1]++;
+    for (size_t c = 0; c < ncell; c++) cell_ptr[c + 1] += cell_ptr[c];
+    std::vector<int> cell_el(cell_ptr[ncell]), cell_fill(cell_ptr.begin(), cell_ptr.end() - 1);
+    for (int ie = 0; ie < nel; ie++)
+        for (int cy = cbox[4 * (size_t)ie + 2]; cy <= cbox[4 * (size_t)ie + 3]; cy++)
+            for (int cx = cbox[4 * (size_t)ie]; cx <= cbox[4 * (size_t)ie + 1]; cx++)
+                cell_el[cell_fill[(size_t)cx + (size_t)nc * cy]++] = ie;
     // quad_tree_search (src/utils/shyutil/quad_tree.f90:313-397): of the
     // elements containing the point by in_element (find.f90:365-400: bounding
     // box in default reals, then intri's edge tests, regular.f90:1252-1282, in
@@ -350,8 +381,11 @@ void host_tvd_tables(int nkn, int nel, const int *nen3v0, const float *xg, const
         const int cx = (int)std::floor(((double)px - xmin) / dxc), cy = (int)std::floor(((double)py - ymin) / dyc);
         if (cx < 0 || cy < 0 || cx >= nc || cy >= nc) return -1;
         int best = -1;
-        for (int ie : cell[(size_t)cx + (size_t)nc * cy])
+        const size_t c = (size_t)cx + (size_t)nc * cy;
+        for (int q = cell_ptr[c]; q < cell_ptr[c + 1]; q++) {
+            const int ie = cell_el[q];
             if (ie > best && inside(ie, px, py)) best = ie;
+        }
         return best;
     };
     tel.assign(6 * (size_t)nel, -1);
