@@ -434,14 +434,26 @@ int host_bndo_tables(int nkn, int nel, const int *nen3v0, const float *x, const 
             const long long a = nen3v0[3 * ie + ii], c = nen3v0[3 * ie + (ii + 1) % 3];
             key[3 * (size_t)ie + ii] = std::min(a, c) * nkn + std::max(a, c);
         }
-    std::vector<long long> sorted(key);
-    std::sort(sorted.begin(), sorted.end());
+    // how many elements share each edge: the edges by their lower node
+    // (counted, then filled), each node's far ends sorted
+    std::vector<int> eptr(nkn + 1, 0);
+    for (size_t q = 0; q < key.size(); q++) eptr[key[q] / nkn + 1]++;
+    for (int k = 0; k < nkn; k++) eptr[k + 1] += eptr[k];
+    std::vector<int> efar(key.size()), efill(eptr.begin(), eptr.end() - 1);
+    for (size_t q = 0; q < key.size(); q++) efar[efill[key[q] / nkn]++] = (int)(key[q] % nkn);
+    parallel_for(nkn, 8192, [&](int k_begin, int k_end) {
+        for (int k = k_begin; k < k_end; k++) std::sort(efar.begin() + eptr[k], efar.begin() + eptr[k + 1]);
+    });
+    auto shared_by = [&](long long kk) {
+        const int lo = (int)(kk / nkn), far = (int)(kk % nkn);
+        auto r = std::equal_range(efar.begin() + eptr[lo], efar.begin() + eptr[lo + 1], far);
+        return r.second - r.first;
+    };
     std::vector<int> knext(nkn, -1), klast(nkn, -1);
     for (int ie = 0; ie < nel; ie++)
         for (int ii = 0; ii < 3; ii++) {
             const long long kk = key[3 * (size_t)ie + ii];
-            auto r = std::equal_range(sorted.begin(), sorted.end(), kk);
-            if (r.second - r.first == 1) {
+            if (shared_by(kk) == 1) {
                 knext[nen3v0[3 * ie + ii]] = nen3v0[3 * ie + (ii + 1) % 3];
                 klast[nen3v0[3 * ie + (ii + 1) % 3]] = nen3v0[3 * ie + ii];
             }
