@@ -68,7 +68,9 @@ void host_set_ev(int nkn, int nel, const int *nen3v0, const float *xg, const flo
     (void)nkn;
     const double pi = 4. * std::atan(1.);
     const double rad = 180. / pi;
-    for (int ie = 0; ie < nel; ie++) {
+    // every element on its own: ranges of them on the host's threads
+    parallel_for(nel, 8192, [&](int e_begin, int e_end) {
+    for (int ie = e_begin; ie < e_end; ie++) {
         const int *kn = nen3v0 + 3 * ie;
         const double x1 = xg[kn[0]], y1 = yg[kn[0]];
         const double x2 = xg[kn[1]], y2 = yg[kn[1]];
@@ -99,6 +101,7 @@ void host_set_ev(int nkn, int nel, const int *nen3v0, const float *xg, const flo
         e[17] = s2;
         e[18] = s3;
     }
+    });
 }
 
 int host_node_list(int nkn, int nel, const int *nen3v0, std::vector<int> &ntg, std::vector<int> &cols,
