@@ -542,8 +542,10 @@ void host_block_schedule(int n, const std::vector<int> &rowptr, const std::vecto
     bs.bwd_lptr.assign((size_t)bs.nblk * stride, 0);
     bs.fwd_nlev.assign(bs.nblk, 0);
     bs.bwd_nlev.assign(bs.nblk, 0);
+    // blocks are independent: ranges of them on the host's threads
+    parallel_for(bs.nblk, 16, [&](int blk_begin, int blk_end) {
     std::vector<int> lev(brows);
-    for (int b = 0; b < bs.nblk; b++) {
+    for (int b = blk_begin; b < blk_end; b++) {
         const int b0 = b * brows, nb = std::min(n, b0 + brows) - b0;
         for (int dir = 0; dir < 2; dir++) {
             int nlev = 0;
@@ -569,47 +571,61 @@ void host_block_schedule(int n, const std::vector<int> &rowptr, const std::vecto
             (dir == 0 ? bs.fwd_nlev : bs.bwd_nlev)[b] = nlev;
         }
     }
-    // update pairs of the fine-grained factorisation, in ascending k
+    });
+    // update pairs of the fine-grained factorisation, in ascending k: each
+    // entry's pairs counted, then written at their offsets (rows in parallel)
     const size_t ne = bs.cols.size();
     bs.cp_ptr.assign(ne + 1, 0);
     bs.ent_jd.assign(ne, -1);
-    bs.cp_a.clear();
-    bs.cp_b.clear();
-    for (int i = 0; i < n; i++) {
+    auto row_pairs = [&](int i, bool write) {
         const int b0 = (i / brows) * brows, li = i - b0;
         for (int q = bs.rowptr[i]; q < bs.rowptr[i + 1]; q++) {
             const int lj = bs.cols[q];
-            bs.ent_jd[q] = lj < li ? bs.diag[b0 + lj] : (lj == li ? -2 - li : -1);
+            if (!write) bs.ent_jd[q] = lj < li ? bs.diag[b0 + lj] : (lj == li ? -2 - li : -1);
             const int m = std::min(li, lj);
+            int c = write ? bs.cp_ptr[q] : 0;
             for (int qk = bs.rowptr[i]; qk < bs.rowptr[i + 1]; qk++) {
                 const int lk = bs.cols[qk];
                 if (lk >= m) break;
                 const int r0 = bs.rowptr[b0 + lk], r1 = bs.rowptr[b0 + lk + 1];
                 const int *hit = std::lower_bound(bs.cols.data() + r0, bs.cols.data() + r1, lj);
                 if (hit != bs.cols.data() + r1 && *hit == lj) {
-                    bs.cp_a.push_back(qk);
-                    bs.cp_b.push_back((int)(hit - bs.cols.data()));
+                    if (write) {
+                        bs.cp_a[c] = qk;
+                        bs.cp_b[c] = (int)(hit - bs.cols.data());
+                    }
+                    c++;
                 }
             }
-            bs.cp_ptr[q + 1] = (int)bs.cp_a.size();
+            if (!write) bs.cp_ptr[q + 1] = c;
         }
-    }
+    };
+    parallel_for(n, 8192, [&](int r_begin, int r_end) {
+        for (int i = r_begin; i < r_end; i++) row_pairs(i, false);
+    });
+    for (size_t q = 0; q < ne; q++) bs.cp_ptr[q + 1] += bs.cp_ptr[q];
+    bs.cp_a.assign(bs.cp_ptr[ne], 0);
+    bs.cp_b.assign(bs.cp_ptr[ne], 0);
+    parallel_for(n, 8192, [&](int r_begin, int r_end) {
+        for (int i = r_begin; i < r_end; i++) row_pairs(i, true);
+    });
     // entry levels: every entry an update pair or a pivot of q refers to
-    // lies in an earlier row or earlier in the same row, so one pass in
-    // entry order sees them computed
+    // lies in an earlier row of its block or earlier in the same row, so one
+    // pass in entry order sees them computed (blocks in parallel)
     bs.ent_lev.assign(ne, 0);
     bs.ent_nlev.assign(bs.nblk, 0);
-    for (int i = 0; i < n; i++) {
-        const int blk = i / brows;
-        for (int q = bs.rowptr[i]; q < bs.rowptr[i + 1]; q++) {
-            int lv = 0;
-            for (int p = bs.cp_ptr[q]; p < bs.cp_ptr[q + 1]; p++)
-                lv = std::max(lv, std::max(bs.ent_lev[bs.cp_a[p]], bs.ent_lev[bs.cp_b[p]]) + 1);
-            if (bs.ent_jd[q] >= 0) lv = std::max(lv, bs.ent_lev[bs.ent_jd[q]] + 1);
-            bs.ent_lev[q] = lv;
-            bs.ent_nlev[blk] = std::max(bs.ent_nlev[blk], lv + 1);
-        }
-    }
+    parallel_for(bs.nblk, 16, [&](int blk_begin, int blk_end) {
+        for (int blk = blk_begin; blk < blk_end; blk++)
+            for (int i = blk * brows; i < std::min(n, (blk + 1) * brows); i++)
+                for (int q = bs.rowptr[i]; q < bs.rowptr[i + 1]; q++) {
+                    int lv = 0;
+                    for (int p = bs.cp_ptr[q]; p < bs.cp_ptr[q + 1]; p++)
+                        lv = std::max(lv, std::max(bs.ent_lev[bs.cp_a[p]], bs.ent_lev[bs.cp_b[p]]) + 1);
+                    if (bs.ent_jd[q] >= 0) lv = std::max(lv, bs.ent_lev[bs.ent_jd[q]] + 1);
+                    bs.ent_lev[q] = lv;
+                    bs.ent_nlev[blk] = std::max(bs.ent_nlev[blk], lv + 1);
+                }
+    });
 }
 
 }  // namespace shy
