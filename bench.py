@@ -110,9 +110,9 @@ def _kernel_bytes(b, p, brows, ntr=2):
 # kernel-name prefixes (template arguments after the value type vary with
 # the iteration-kernel variant) of each timed group's kernel
 KERNEL_NAMES = {"bicg_a": "k_bicg_a<{v},", "bicg_b": "k_bicg_b<{v},",
-                "bilu_factor": "k_bilu_factor<{v}>", "conz3d_elem": "k_conz_elem<{v}, 2>",
+                "bilu_factor": "k_bilu_factor<{v}>", "conz3d_elem": "k_conz_elem<{v}, 2, false, false>",
                 "conz3d_prep": "k_conz_prep<{v}>",
-                "conz3d_node": "k_conz_node<{v}, 12, 2>", "hydro_vertical": "k_hydro_vertical<{v}, 12>",
+                "conz3d_node": "k_conz_node<{v}, 12, 2, false>", "hydro_vertical": "k_hydro_vertical<{v}, 12>",
                 "hydro_intern": "k_hydro_intern<{v}, 12, false>"}
 
 
